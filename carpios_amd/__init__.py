@@ -1,0 +1,257 @@
+"""carpios_amd -- MI355X-native structured-grid projection hot path of Carpios.
+
+The compute path is libcpgpu.so (hand-written HIP for gfx950 behind the C-ABI
+of include/cpgpu.h).  This module is a thin ctypes view of that ABI for
+Python callers (tests, bench); it has no compute of its own and raises if
+the library is missing -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcpgpu.so")
+
+# status codes / constants mirrored from include/cpgpu.h
+OK, MAXITER, BREAKDOWN_RHO, BREAKDOWN_OMEGA = 0, 1, 2, 3
+EINVAL, EHIP, EUNSUPPORTED, ENOMEM, EZERODIAG = -1, -2, -3, -4, -10
+BC_DIRICHLET, BC_NEUMANN = 1, 2
+CG, JPCG, BICGSTAB, JACOBI, MIXED = 0, 1, 2, 3, 4
+NORM_REF, NORM_TRUE = 0, 1
+OPT_PROJECT_MEAN = 1
+
+METHODS = {"CG": CG, "JPCG": JPCG, "Dia_BiCGSTAB": BICGSTAB,
+           "Jacobi": JACOBI, "MIXED": MIXED}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Grid2D(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("gl", C.c_int),
+                ("cx", _dp), ("sx", _dp), ("cy", _dp), ("sy", _dp),
+                ("beta", C.c_double), ("bc_type", C.c_int * 4),
+                ("bc_val", _dp)]
+
+
+class CpgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cpgpu error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# every symbol include/cpgpu.h declares
+EXPORTS = [
+    "cpgpu_grid_uniform_axis", "cpgpu_create", "cpgpu_destroy", "cpgpu_last_error", "cpgpu_synchronize",
+    "cpgpu_stream", "cpgpu_poisson_create", "cpgpu_poisson_destroy",
+    "cpgpu_poisson_set_source", "cpgpu_poisson_set_source_device",
+    "cpgpu_poisson_set_phi", "cpgpu_poisson_get_phi", "cpgpu_poisson_solve",
+    "cpgpu_poisson_set_mixed", "cpgpu_poisson_apply",
+    "cpgpu_poisson_residual", "cpgpu_poisson_device_vec",
+    "cpgpu_poisson_get_vec", "cpgpu_poisson_set_source_blobs",
+    "cpgpu_blob_params", "cpgpu_poisson_fill_phi",
+    "cpgpu_poisson2d_solve", "cpgpu_timer_enable", "cpgpu_timer_read",
+]
+
+
+def lib():
+    """Load libcpgpu.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        L.cpgpu_last_error.restype = C.c_char_p
+        L.cpgpu_stream.restype = C.c_void_p
+        L.cpgpu_poisson_device_vec.restype = C.c_void_p
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Context:
+    """One device, one stream (cpgpu_ctx)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        rc = lib().cpgpu_create(C.c_int(device), C.byref(self.h))
+        if rc != OK:
+            raise CpgpuError(rc, "cpgpu_create failed (no GPU?)")
+
+    def check(self, rc, allow=(OK,)):
+        if rc not in allow:
+            msg = lib().cpgpu_last_error(self.h)
+            raise CpgpuError(rc, msg.decode() if msg else "")
+        return rc
+
+    def sync(self):
+        self.check(lib().cpgpu_synchronize(self.h))
+
+    def timer_enable(self, family, on=True):
+        self.check(lib().cpgpu_timer_enable(self.h, family, int(on)))
+
+    def timer_read(self, family):
+        n, ms = C.c_int(0), C.c_double(0)
+        self.check(lib().cpgpu_timer_read(self.h, family, C.byref(n),
+                                          C.byref(ms)))
+        return n.value, ms.value
+
+    def close(self):
+        if self.h:
+            lib().cpgpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_grid_desc(cx, sx, cy, sy, nx, ny, gl, bc_type, bc_val=None,
+                   beta=1.0):
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (cx, sx, cy, sy)]
+    g = Grid2D()
+    g.nx, g.ny, g.gl = nx, ny, gl
+    g.cx, g.sx, g.cy, g.sy = [_p(a) for a in keep]
+    g.beta = beta
+    for k in range(4):
+        g.bc_type[k] = int(bc_type[k])
+    if bc_val is not None:
+        bv = np.ascontiguousarray(bc_val, dtype=np.float64)
+        keep.append(bv)
+        g.bc_val = _p(bv)
+    g._keep = keep
+    return g
+
+
+def uniform_axis(n, gl, vmin, vmax):
+    """(centres, sizes) of a uniform structure::Grid_ axis, ghosts included."""
+    c = np.zeros(n + 2 * gl)
+    s = np.zeros(n + 2 * gl)
+    rc = lib().cpgpu_grid_uniform_axis(C.c_int(n), C.c_int(gl),
+                                       C.c_double(vmin), C.c_double(vmax),
+                                       _p(c), _p(s))
+    if rc != OK:
+        raise CpgpuError(rc, "bad axis")
+    return c, s
+
+
+def uniform_grid_desc(nx, ny, gl, bc_type, bc_val=None, beta=1.0,
+                      xlim=(0.0, 1.0), ylim=(0.0, 1.0)):
+    """Descriptor of a uniform grid on [xlim] x [ylim]."""
+    cx, sx = uniform_axis(nx, gl, *xlim)
+    cy, sy = uniform_axis(ny, gl, *ylim)
+    return make_grid_desc(cx, sx, cy, sy, nx, ny, gl, bc_type, bc_val, beta)
+
+
+class Poisson:
+    """Device-resident Poisson problem (cpgpu_poisson)."""
+
+    def __init__(self, ctx, desc):
+        self.ctx, self.desc = ctx, desc
+        self.nx, self.ny, self.gl = desc.nx, desc.ny, desc.gl
+        self.h = C.c_void_p()
+        ctx.check(lib().cpgpu_poisson_create(ctx.h, C.byref(desc),
+                                             C.byref(self.h)))
+
+    @property
+    def shape(self):
+        return (self.nx + 2 * self.gl, self.ny + 2 * self.gl)
+
+    def set_source(self, src_field=None, uniform=0.0):
+        f = None if src_field is None else np.ascontiguousarray(
+            src_field, dtype=np.float64)
+        self.ctx.check(lib().cpgpu_poisson_set_source(self.h, _p(f),
+                                                      C.c_double(uniform)))
+
+    def set_phi(self, phi_field):
+        f = np.ascontiguousarray(phi_field, dtype=np.float64)
+        self.ctx.check(lib().cpgpu_poisson_set_phi(self.h, _p(f)))
+
+    def get_phi(self, out=None):
+        out = np.zeros(self.shape) if out is None else out
+        self.ctx.check(lib().cpgpu_poisson_get_phi(self.h, _p(out)))
+        return out
+
+    def apply(self, x_field):
+        x = np.ascontiguousarray(x_field, dtype=np.float64)
+        y = np.zeros(self.shape)
+        self.ctx.check(lib().cpgpu_poisson_apply(self.h, _p(x), _p(y)))
+        return y
+
+    def residual(self):
+        r = C.c_double(0)
+        self.ctx.check(lib().cpgpu_poisson_residual(self.h, C.byref(r)))
+        return r.value
+
+    def set_mixed(self, inner_tol=1e-4, max_outer=40):
+        self.ctx.check(lib().cpgpu_poisson_set_mixed(
+            self.h, C.c_double(inner_tol), C.c_int(max_outer)))
+
+    def solve(self, method="JPCG", max_iter=100000, tol=1e-10, norm=NORM_REF,
+              opts=0, hist_cap=0):
+        """Returns (code, iters, tol_out, hist)."""
+        mi, t = C.c_int(max_iter), C.c_double(tol)
+        hist = np.zeros(max(hist_cap, 1))
+        hl = C.c_int(0)
+        code = lib().cpgpu_poisson_solve(
+            self.h, C.c_int(METHODS[method]), C.byref(mi), C.byref(t),
+            C.c_int(norm), C.c_int(opts), _p(hist) if hist_cap else None,
+            C.c_int(hist_cap), C.byref(hl))
+        self.ctx.check(code, allow=(OK, MAXITER, BREAKDOWN_RHO,
+                                    BREAKDOWN_OMEGA))
+        return code, mi.value, t.value, hist[:min(hl.value, hist_cap)].copy()
+
+    def close(self):
+        if self.h:
+            lib().cpgpu_poisson_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _get_vec(self, which):
+    out = np.zeros(self.shape)
+    self.ctx.check(lib().cpgpu_poisson_get_vec(self.h, C.c_int(which), _p(out)))
+    return out
+
+
+Poisson.get_vec = _get_vec
+
+
+def _set_source_blobs(self, seed, nblobs=64, wmin=0.01, wmax=0.1):
+    self.ctx.check(lib().cpgpu_poisson_set_source_blobs(
+        self.h, C.c_ulonglong(seed), C.c_int(nblobs), C.c_double(wmin),
+        C.c_double(wmax)))
+
+
+Poisson.set_source_blobs = _set_source_blobs
+
+
+def _fill_phi(self, value=0.0):
+    self.ctx.check(lib().cpgpu_poisson_fill_phi(self.h, C.c_double(value)))
+
+
+Poisson.fill_phi = _fill_phi
+
+
+def blob_params(seed, nblobs=64, wmin=0.01, wmax=0.1):
+    out = np.zeros(4 * nblobs)
+    rc = lib().cpgpu_blob_params(C.c_ulonglong(seed), C.c_int(nblobs),
+                                 C.c_double(wmin), C.c_double(wmax), _p(out))
+    if rc != OK:
+        raise CpgpuError(rc, "bad blob parameters")
+    return out.reshape(nblobs, 4)

@@ -1,0 +1,167 @@
+/*
+ * cpgpu.h -- C-ABI of the MI355X (gfx950) structured-grid projection path of
+ * Carpios: matrix-free 5-point Poisson solves and the NS_explicit_ projection
+ * step, with fields resident in HBM.
+ *
+ * Plain C types only.  Every entry point returns an int status: the
+ * reference's solver codes where a solver runs (0 converged, 1 max_iter,
+ * 2 rho breakdown, 3 omega breakdown; lib/algebra/solver_matrix.hpp:
+ * 599-672) and negative CPGPU_E* codes for invalid arguments and HIP
+ * failures.  Nothing throws across the ABI and nothing calls exit().  The
+ * library never frees or retains caller pointers.  A context owns one HIP
+ * stream; all work is stream-ordered on it and every reduction is
+ * deterministic (run-to-run bitwise identical).
+ *
+ * Field layout (lib/structure/s_data.hpp:134-137): a cell field is a
+ * row-major (nx+2gl) x (ny+2gl) double buffer, j contiguous, cell (i,j) at
+ * ((i+gl)*(ny+2gl) + (j+gl)) -- the buffer of a structure::Scalar_<2>.
+ */
+#ifndef CPGPU_H
+#define CPGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define CPGPU_OK 0
+#define CPGPU_MAXITER 1
+#define CPGPU_BREAKDOWN_RHO 2
+#define CPGPU_BREAKDOWN_OMEGA 3
+#define CPGPU_EINVAL (-1)
+#define CPGPU_EHIP (-2)
+#define CPGPU_EUNSUPPORTED (-3)
+#define CPGPU_ENOMEM (-4)
+#define CPGPU_EZERODIAG (-10) /* DiaPre zero diagonal: the reference exit()s */
+
+/* ---- boundary types, lib/structure/s_boundary.hpp:16-18 ---- */
+#define CPGPU_BC_DIRICHLET 1 /* BC1 */
+#define CPGPU_BC_NEUMANN 2   /* BC2 (also the reference default, s_boundary.hpp:29-36) */
+
+/* ---- solver methods ---- */
+#define CPGPU_CG 0        /* carpio::CG, solver_matrix.hpp:264-316 */
+#define CPGPU_JPCG 1      /* IC_CG loop (:318-374) with DiaPre(A,1) (preconditioner.hpp:182-262) */
+#define CPGPU_BICGSTAB 2  /* carpio::Dia_BiCGSTAB, solver_matrix.hpp:599-672 */
+#define CPGPU_JACOBI 3    /* carpio::Solver_Jacobi_, solver_matrix.hpp:820-910 */
+#define CPGPU_MIXED 4     /* FP32 J-PCG inside FP64 iterative refinement (new) */
+
+/* ---- stopping-norm modes ---- */
+#define CPGPU_NORM_REF 0  /* carpio::nrm2: drops the last unknown (BLAS_Level1.hpp:569) */
+#define CPGPU_NORM_TRUE 1 /* plain Euclidean norm */
+
+/* ---- option flags for solves ---- */
+#define CPGPU_OPT_PROJECT_MEAN 1 /* all-Neumann: zero-mean b and the update (new) */
+
+typedef struct cpgpu_ctx cpgpu_ctx;
+typedef struct cpgpu_poisson cpgpu_poisson;
+
+/* Matrix-free description of structure::StencilPoisson_<2> +
+ * GhostRegular_<2> (s_poisson.hpp:60-181, s_ghost.hpp:140-357) on a
+ * structure::Grid_<2> (s_grid.hpp:10-440).  cx/sx (cy/sy) are the grid's
+ * per-axis centre and size arrays including ghosts (nx+2gl, ny+2gl
+ * entries, Grid_::_c and Grid_::_cs).  bc_type[4] for segments x-, x+, y-,
+ * y+ (s_ghost.hpp:218-232); bc_val is NULL (all zero) or the packed
+ * per-line boundary values [x- (ny+2gl) | x+ (ny+2gl) | y- (nx+2gl) |
+ * y+ (nx+2gl)] indexed by the ghost-inclusive line index, sampled where the
+ * reference's BoundaryCondition::get_val is evaluated (s_ghost.hpp:161-165). */
+typedef struct {
+  int nx, ny, gl;
+  const double *cx, *sx;
+  const double *cy, *sy;
+  double beta; /* uniform beta (Poisson_::set_uniform_beta, s_poisson.hpp:454-458) */
+  int bc_type[4];
+  const double *bc_val;
+} cpgpu_grid2d;
+
+/* ---- grid helper ---- */
+/* Centre and size arrays (n+2gl entries, ghosts included) of one uniform
+ * axis of a structure::Grid_ (Grid_::_set_uniform_grid, s_grid.hpp:
+ * 372-395): sizes (vmax-vmin)/n, centres accumulated cell by cell. */
+int cpgpu_grid_uniform_axis(int n, int gl, double vmin, double vmax, double *c,
+                            double *s);
+
+/* ---- context ---- */
+/* Replaces: nothing in the reference (it is single-threaded host code); owns
+ * the device, one stream and the scratch of the solvers. */
+int cpgpu_create(int device, cpgpu_ctx **out);
+int cpgpu_destroy(cpgpu_ctx *ctx);
+const char *cpgpu_last_error(const cpgpu_ctx *ctx);
+int cpgpu_synchronize(cpgpu_ctx *ctx);
+/* The context's HIP stream as an opaque pointer (hipStream_t). */
+void *cpgpu_stream(cpgpu_ctx *ctx);
+
+/* ---- Poisson problem (device resident) ----
+ * Replaces the assembly + solve of structure::Solver_<2>::solve()
+ * (s_solver.hpp:94-171, :261-291): the operator is built once from the
+ * descriptor and applied matrix-free. */
+int cpgpu_poisson_create(cpgpu_ctx *ctx, const cpgpu_grid2d *g,
+                         cpgpu_poisson **out);
+int cpgpu_poisson_destroy(cpgpu_poisson *P);
+
+/* b = -(vol*src + ghost constants), as Solver_::_build_matrix
+ * (s_solver.hpp:122) builds it; src is a host cell field or NULL for the
+ * uniform value (Poisson_::set_uniform_source, s_poisson.hpp:487-491). */
+int cpgpu_poisson_set_source(cpgpu_poisson *P, const double *src_field,
+                             double src_uniform);
+/* Same with a device cell field (interior-only nx*ny, j contiguous). */
+int cpgpu_poisson_set_source_device(cpgpu_poisson *P, const double *d_src);
+/* Synthetic source (no reference counterpart): a sum of nblobs Gaussian
+ * blobs A_k exp(-|x-c_k|^2/(2 w_k^2)) at the cell centres with c_k uniform
+ * in [0,1)^2, w_k uniform in [wmin,wmax), A_k standard normal, drawn from
+ * xoshiro256** seeded by SplitMix64(seed); the cell mean is subtracted. */
+int cpgpu_poisson_set_source_blobs(cpgpu_poisson *P, unsigned long long seed,
+                                   int nblobs, double wmin, double wmax);
+/* The blob parameters of that generator: out[4k..4k+3] = cx, cy, w, A. */
+int cpgpu_blob_params(unsigned long long seed, int nblobs, double wmin,
+                      double wmax, double *out);
+/* Warm start / result: host cell fields (interior read or written). */
+int cpgpu_poisson_set_phi(cpgpu_poisson *P, const double *phi_field);
+int cpgpu_poisson_get_phi(cpgpu_poisson *P, double *phi_field);
+/* Device-side fill of phi (a zero initial guess without a host transfer). */
+int cpgpu_poisson_fill_phi(cpgpu_poisson *P, double value);
+
+/* Solve A phi = b from the current phi.  max_iter and tol are in/out with the
+ * reference's meaning (on return: iterations used and the final relative
+ * residual).  hist (capacity hist_cap, may be NULL) receives the residual
+ * history; *hist_len its full length.  opts: CPGPU_OPT_* flags. */
+int cpgpu_poisson_solve(cpgpu_poisson *P, int method, int *max_iter,
+                        double *tol, int norm_mode, int opts, double *hist,
+                        int hist_cap, int *hist_len);
+
+/* Mixed solve controls (CPGPU_MIXED): inner relative tolerance and the
+ * maximum number of refinement sweeps. */
+int cpgpu_poisson_set_mixed(cpgpu_poisson *P, double inner_tol, int max_outer);
+
+/* y = A x on interior host fields (tests of the operator). */
+int cpgpu_poisson_apply(cpgpu_poisson *P, const double *x_field,
+                        double *y_field);
+/* True FP64 relative residual ||b - A phi|| / ||b|| of the current phi. */
+int cpgpu_poisson_residual(cpgpu_poisson *P, double *rel);
+
+/* Device pointers of the problem's interior-only vectors (nx*ny doubles,
+ * j contiguous): which = 0 phi, 1 b, 2 last residual.  For device-resident
+ * callers. */
+double *cpgpu_poisson_device_vec(cpgpu_poisson *P, int which);
+/* Host copy of one of those vectors into the interior of a cell field. */
+int cpgpu_poisson_get_vec(cpgpu_poisson *P, int which, double *field);
+
+/* One-shot host-in/host-out solve, the drop-in body of a
+ * structure::Solver_<2>::solve() (s_solver.hpp:261-291). */
+int cpgpu_poisson2d_solve(cpgpu_ctx *ctx, const cpgpu_grid2d *g,
+                          const double *src_field, double src_uniform,
+                          double *phi_field, int method, int *max_iter,
+                          double *tol, int norm_mode, double *hist,
+                          int hist_cap, int *hist_len);
+
+/* ---- kernel timing (HIP events on the context stream) ----
+ * When enabled, every launch of the named kernel family is bracketed by
+ * events; cpgpu_timer_read returns launches and summed milliseconds.
+ * family: 0 = stencil apply (p update + A p + p.q), 1 = vector update
+ * (x, r update + dots). */
+int cpgpu_timer_enable(cpgpu_ctx *ctx, int family, int on);
+int cpgpu_timer_read(cpgpu_ctx *ctx, int family, int *launches, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

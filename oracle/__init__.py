@@ -1,0 +1,339 @@
+"""ctypes view of the CPU oracle (oracle/liboracle.so) and, where built, of the
+reference's own lib/algebra solvers (oracle/_ref/libcarpios_ref.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module as the checker.  The product package
+``carpios_amd`` never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libcarpios_ref.so")
+
+BC1, BC2 = 1, 2
+NORM_REF, NORM_TRUE = 0, 1
+LIM = {"VanLeer": 0, "superbee": 1, "WAHYD": 2, "QUICK": 3, "CUI": 4,
+       "upwind2": 5}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("gl", C.c_int),
+                ("cx", _dp), ("sx", _dp), ("cy", _dp), ("sy", _dp)]
+
+
+class CSR(C.Structure):
+    _fields_ = [("n", C.c_int), ("row_ptr", _ip), ("col", _ip), ("val", _dp)]
+
+
+class BC(C.Structure):
+    _fields_ = [("type", C.c_int * 4), ("val", _dp)]
+
+
+class NSParams(C.Structure):
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double), ("cfl", C.c_double),
+                ("dnum", C.c_double), ("dcoef", C.c_double),
+                ("limiter", C.c_int), ("max_iter", C.c_int),
+                ("tol", C.c_double), ("norm_mode", C.c_int)]
+
+
+class NSState(C.Structure):
+    _fields_ = [(n, _dp) for n in ("u", "v", "uf", "vf", "us", "vs", "usf",
+                                   "vsf", "p", "phi", "div")] + \
+               [("dt", C.c_double), ("t", C.c_double)]
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        L.orc_dot.restype = C.c_double
+        L.orc_nrm2_ref.restype = C.c_double
+        L.orc_nrm2_true.restype = C.c_double
+        L.orc_tvd_cell.restype = C.c_double
+        L.orc_diffusion_cell.restype = C.c_double
+        L.orc_max_magnitude.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("oracle/_ref not built (needs the reference tree)")
+        L = C.CDLL(REF_PATH)
+        L.ref_dot.restype = C.c_double
+        L.ref_nrm2.restype = C.c_double
+        _ref = L
+    return _ref
+
+
+class Axis:
+    """Per-axis centre and size arrays (ghosts included)."""
+
+    def __init__(self, n, gl, vmin, vmax):
+        self.n, self.gl = n, gl
+        self.c = np.zeros(n + 2 * gl)
+        self.s = np.zeros(n + 2 * gl)
+        lib().orc_axis_uniform(C.c_int(n), C.c_int(gl), C.c_double(vmin),
+                               C.c_double(vmax), _ptr(self.c), _ptr(self.s))
+
+
+class StructGrid:
+    """Uniform structured grid as lib/structure/s_grid.hpp builds it."""
+
+    def __init__(self, nx, ny, gl=1, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0):
+        self.nx, self.ny, self.gl = nx, ny, gl
+        self.ax = Axis(nx, gl, xmin, xmax)
+        self.ay = Axis(ny, gl, ymin, ymax)
+        self.c = Grid(nx, ny, gl, _ptr(self.ax.c), _ptr(self.ax.s),
+                      _ptr(self.ay.c), _ptr(self.ay.s))
+
+    @property
+    def shape(self):
+        return (self.nx + 2 * self.gl, self.ny + 2 * self.gl)
+
+    def field(self):
+        return np.zeros(self.shape)
+
+    def interior(self, f):
+        g = self.gl
+        return f[g:g + self.nx, g:g + self.ny]
+
+    def centres(self):
+        """Interior cell-centre coordinates as (X, Y) arrays of shape (nx, ny)."""
+        g = self.gl
+        return np.meshgrid(self.ax.c[g:g + self.nx], self.ay.c[g:g + self.ny],
+                           indexing="ij")
+
+
+def make_bc(grid, types, values=None):
+    """types: 4 ints (x-, x+, y-, y+); values: None, 4 scalars, or the packed
+    [seg0 | seg1 | seg2 | seg3] array of per-line values."""
+    NX, NY = grid.shape
+    if values is None:
+        arr = np.zeros(2 * NY + 2 * NX)
+    elif np.ndim(values) == 1 and len(values) == 4:
+        arr = np.concatenate([np.full(NY, values[0]), np.full(NY, values[1]),
+                              np.full(NX, values[2]), np.full(NX, values[3])])
+    else:
+        arr = np.ascontiguousarray(values, dtype=np.float64)
+    bc = BC()
+    for k in range(4):
+        bc.type[k] = int(types[k])
+    bc.val = _ptr(arr)
+    bc._keep = arr
+    return bc
+
+
+class Matrix:
+    """Assembled CSR (i-fastest unknowns) plus its right-hand side."""
+
+    def __init__(self, grid, bc, beta=1.0, src_field=None, src_uniform=0.0):
+        n = grid.nx * grid.ny
+        nnz_max = 5 * n
+        self.n = n
+        self.row_ptr = np.zeros(n + 1, dtype=np.int32)
+        self.col = np.zeros(nnz_max, dtype=np.int32)
+        self.val = np.zeros(nnz_max)
+        self.b = np.zeros(n)
+        sp = None
+        if src_field is not None:
+            self._src = np.ascontiguousarray(src_field, dtype=np.float64)
+            sp = _ptr(self._src)
+        nnz = lib().orc_poisson_assemble(
+            C.byref(grid.c), C.c_double(beta), sp, C.c_double(src_uniform),
+            C.byref(bc), _ptr(self.row_ptr, _ip), _ptr(self.col, _ip),
+            _ptr(self.val), _ptr(self.b))
+        if nnz < 0:
+            raise ValueError("assembly failed")
+        self.nnz = nnz
+        self.col = self.col[:nnz].copy()
+        self.val = self.val[:nnz].copy()
+        self.c = CSR(n, _ptr(self.row_ptr, _ip), _ptr(self.col, _ip),
+                     _ptr(self.val))
+
+    def dense(self):
+        A = np.zeros((self.n, self.n))
+        for i in range(self.n):
+            for k in range(self.row_ptr[i], self.row_ptr[i + 1]):
+                A[i, self.col[k]] = self.val[k]
+        return A
+
+    def mult(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n)
+        lib().orc_csr_mult(C.byref(self.c), _ptr(x), _ptr(y))
+        return y
+
+
+_SOLVERS = {"CG": "orc_cg", "JPCG": "orc_jpcg",
+            "Dia_BiCGSTAB": "orc_dia_bicgstab", "Jacobi": "orc_jacobi"}
+
+
+def solve(method, A, b, x0=None, max_iter=1000, tol=1e-10, norm=NORM_REF):
+    """Returns (code, x, iters, tol_out, hist)."""
+    x = np.zeros(A.n) if x0 is None else np.array(x0, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    mi = C.c_int(max_iter)
+    t = C.c_double(tol)
+    hist = np.zeros(max_iter + 2)
+    hl = C.c_int(0)
+    fn = getattr(lib(), _SOLVERS[method])
+    code = fn(C.byref(A.c), _ptr(x), _ptr(b), C.byref(mi), C.byref(t),
+              C.c_int(norm), _ptr(hist), C.c_int(len(hist)), C.byref(hl))
+    return code, x, mi.value, t.value, hist[:min(hl.value, len(hist))].copy()
+
+
+REF_METHODS = {"CG": 0, "JPCG": 1, "Dia_BiCGSTAB": 2, "Jacobi": 3,
+               "Solver_Jacobi_": 4, "IC_CG": 5, "IC_CGS": 6, "IC_BiCGSTAB": 7}
+
+
+def ref_solve(method, A, b, x0=None, max_iter=1000, tol=1e-10):
+    """The reference's own solver on the same CSR (oracle/_ref)."""
+    x = np.zeros(A.n) if x0 is None else np.array(x0, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    mi = C.c_int(max_iter)
+    t = C.c_double(tol)
+    hist = np.zeros(max_iter + 2)
+    hl = C.c_int(0)
+    code = ref().ref_solve(C.c_int(REF_METHODS[method]), C.c_int(A.n),
+                           _ptr(A.row_ptr, _ip), _ptr(A.col, _ip),
+                           _ptr(A.val), _ptr(x), _ptr(b), C.byref(mi),
+                           C.byref(t), _ptr(hist), C.c_int(len(hist)),
+                           C.byref(hl))
+    return code, x, mi.value, t.value, hist[:min(hl.value, len(hist))].copy()
+
+
+def jpcg_mixed(A, b, x0=None, tol=1e-10, inner_tol=1e-4, max_outer=50,
+               max_inner=100000, project_mean=False):
+    x = np.zeros(A.n) if x0 is None else np.array(x0, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    outer, inner, res = C.c_int(0), C.c_int(0), C.c_double(0)
+    code = lib().orc_jpcg_mixed(C.byref(A.c), _ptr(x), _ptr(b),
+                                C.c_double(tol), C.c_double(inner_tol),
+                                C.c_int(max_outer), C.c_int(max_inner),
+                                C.c_int(int(project_mean)), C.byref(outer),
+                                C.byref(inner), C.byref(res))
+    return code, x, outer.value, inner.value, res.value
+
+
+def field_to_x(grid, f):
+    x = np.zeros(grid.nx * grid.ny)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    lib().orc_field_to_x(C.byref(grid.c), _ptr(f), _ptr(x))
+    return x
+
+
+def x_to_field(grid, x, f=None):
+    f = grid.field() if f is None else f
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lib().orc_x_to_field(C.byref(grid.c), _ptr(x), _ptr(f))
+    return f
+
+
+def apply_bc(grid, f, bc):
+    assert f.flags.c_contiguous and f.dtype == np.float64
+    lib().orc_apply_bc(C.byref(grid.c), _ptr(f), C.byref(bc))
+    return f
+
+
+class NS:
+    """NS_explicit_ state driven by the oracle (s_ns_explicit.hpp:118-155)."""
+
+    def __init__(self, grid, bcu, bcv, rho=1.0, mu=1.0, cfl=0.5, dnum=1e10,
+                 dcoef=1.0, limiter="VanLeer", max_iter=100000, tol=1e-10,
+                 norm=NORM_REF, dt=1e-3):
+        self.g = grid
+        self.bcu, self.bcv = bcu, bcv
+        self.prm = NSParams(rho, mu, cfl, dnum, dcoef, LIM[limiter], max_iter,
+                            tol, norm)
+        self.f = {n: grid.field() for n in ("u", "v", "uf", "vf", "us", "vs",
+                                            "usf", "vsf", "p", "phi", "div")}
+        self.st = NSState(*[_ptr(self.f[n]) for n in
+                            ("u", "v", "uf", "vf", "us", "vs", "usf", "vsf",
+                             "p", "phi", "div")], dt, 0.0)
+        bcphi = make_bc(grid, [BC2] * 4)
+        self.A = Matrix(grid, bcphi, beta=1.0)
+        self.iters = []
+
+    @property
+    def dt(self):
+        return self.st.dt
+
+    def step(self):
+        it, res = C.c_int(0), C.c_double(0)
+        code = lib().orc_ns_step(C.byref(self.g.c), C.byref(self.A.c),
+                                 C.byref(self.bcu), C.byref(self.bcv),
+                                 C.byref(self.prm), C.byref(self.st),
+                                 C.byref(it), C.byref(res))
+        self.iters.append(it.value)
+        return code, it.value, res.value
+
+
+def tvd_field(grid, uf, vf, phi, limiter="VanLeer"):
+    out = np.zeros((grid.nx, grid.ny))
+    L = lib()
+    for i in range(grid.nx):
+        for j in range(grid.ny):
+            out[i, j] = L.orc_tvd_cell(C.byref(grid.c), _ptr(uf), _ptr(vf),
+                                       _ptr(phi), i, j, LIM[limiter])
+    return out
+
+
+def diffusion_field(grid, mu, phi):
+    out = np.zeros((grid.nx, grid.ny))
+    L = lib()
+    for i in range(grid.nx):
+        for j in range(grid.ny):
+            out[i, j] = L.orc_diffusion_cell(C.byref(grid.c), C.c_double(mu),
+                                             _ptr(phi), i, j)
+    return out
+
+
+def c2f(grid, uc, vc, uf=None, vf=None):
+    uf = grid.field() if uf is None else uf
+    vf = grid.field() if vf is None else vf
+    lib().orc_interp_c2f(C.byref(grid.c), _ptr(uc), _ptr(vc), _ptr(uf),
+                         _ptr(vf))
+    return uf, vf
+
+
+def div(grid, uf, vf):
+    d = grid.field()
+    lib().orc_div(C.byref(grid.c), _ptr(uf), _ptr(vf), _ptr(d))
+    return d
+
+
+def grad(grid, phi):
+    gx, gy = grid.field(), grid.field()
+    lib().orc_grad(C.byref(grid.c), _ptr(phi), _ptr(gx), _ptr(gy))
+    return gx, gy
+
+
+def f2c(grid, gx, gy):
+    cx, cy = grid.field(), grid.field()
+    lib().orc_interp_f2c(C.byref(grid.c), _ptr(gx), _ptr(gy), _ptr(cx),
+                         _ptr(cy))
+    return cx, cy

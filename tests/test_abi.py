@@ -1,0 +1,43 @@
+"""The C-ABI library loads and exports every symbol include/cpgpu.h declares
+(no compute: runs without a GPU)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+import carpios_amd as ca
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "cpgpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(cpgpu_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ca.lib()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(ca.EXPORTS) == syms
+
+
+def test_invalid_arguments_are_rejected_without_a_device():
+    L = ca.lib()
+    assert L.cpgpu_create(C.c_int(0), None) == ca.EINVAL
+    assert L.cpgpu_poisson_destroy(None) == ca.EINVAL
+    c = np.zeros(4)
+    assert L.cpgpu_grid_uniform_axis(0, 1, C.c_double(0), C.c_double(1),
+                                     ca._p(c), ca._p(c)) == ca.EINVAL
+
+
+def test_uniform_axis_matches_grid_construction():
+    import oracle as O
+    for n, gl, lo, hi in ((128, 1, 0.0, 1.0), (512, 2, -0.5, 0.5), (7, 3, 2.0, 5.5)):
+        c, s = ca.uniform_axis(n, gl, lo, hi)
+        a = O.Axis(n, gl, lo, hi)
+        assert np.array_equal(c, a.c) and np.array_equal(s, a.s)
