@@ -7,6 +7,7 @@ the library is missing -- there is no CPU fallback.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -82,9 +83,14 @@ class Context:
 
     def __init__(self, device=0):
         self.h = C.c_void_p()
+        self._children = weakref.WeakSet()
         rc = lib().cpgpu_create(C.c_int(device), C.byref(self.h))
         if rc != OK:
             raise CpgpuError(rc, "cpgpu_create failed (no GPU?)")
+
+    def adopt(self, obj):
+        """Objects created on this context are closed before it."""
+        self._children.add(obj)
 
     def check(self, rc, allow=(OK,)):
         if rc not in allow:
@@ -106,6 +112,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for ch in list(self._children):
+                ch.close()
             lib().cpgpu_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -162,6 +170,7 @@ class Poisson:
         self.h = C.c_void_p()
         ctx.check(lib().cpgpu_poisson_create(ctx.h, C.byref(desc),
                                              C.byref(self.h)))
+        ctx.adopt(self)
 
     @property
     def shape(self):
@@ -212,7 +221,7 @@ class Poisson:
         return code, mi.value, t.value, hist[:min(hl.value, hist_cap)].copy()
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().cpgpu_poisson_destroy(self.h)
             self.h = C.c_void_p()
 

@@ -1,5 +1,13 @@
 // Shared internals of libcpgpu: context, error plumbing, kernel timers and
-// the deterministic block/grid reduction used by every dot product.
+// the deterministic reductions used by every dot product.
+//
+// Reductions never use device-scope fences or atomics: on gfx950 each of the
+// eight XCDs has its own L2, and an agent-scope release in every block writes
+// that L2 back.  Instead every block of a kernel stores its partial sums with
+// plain stores; the kernel boundary makes them visible, and the consumer (the
+// next kernel) reduces all partials in a fixed order in its prologue.  Every
+// block of the consumer computes the identical total, so the result is
+// deterministic and needs no broadcast.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,10 +49,10 @@ struct TimerScope {
   ~TimerScope();
 };
 
-constexpr int kBlock = 256;  // threads per block for 1-D kernels
+constexpr int kThreads = 256;  // threads per block of every reduction kernel
+constexpr int kMaxBlocks = 1024;  // upper bound of partials per reduction
 
-// Wavefront (64 lanes) sum through cross-lane shuffles; the order of the
-// additions is fixed, so the result is deterministic.
+// Wavefront (64 lanes) sum through cross-lane shuffles in a fixed order.
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -52,71 +60,49 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// Block sum of NV values per thread (blockDim.x*blockDim.y threads, a
-// multiple of 64).  Result valid in thread 0.
+// Block sum of NV values per thread; every thread receives the totals.
+// Needs kThreads threads; lds holds NV*4 values.
 template <typename T, int NV>
-__device__ __forceinline__ void block_sum(T (&v)[NV], T* lds /* >= NV*16 */) {
+__device__ __forceinline__ void block_allsum(T (&v)[NV], T* lds) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int nthreads = blockDim.x * blockDim.y;
-  const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
+  const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();  // lds may still be read from a previous use
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) lds[k * 16 + wid] = v[k];
+    for (int k = 0; k < NV; ++k) lds[k * 4 + wid] = v[k];
   }
   __syncthreads();
-  if (tid == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      T s = lds[k * 16];
-      for (int w = 1; w < nw; ++w) s += lds[k * 16 + w];
-      v[k] = s;
-    }
+  for (int k = 0; k < NV; ++k)
+    v[k] = ((lds[k * 4] + lds[k * 4 + 1]) + lds[k * 4 + 2]) + lds[k * 4 + 3];
+}
+
+// Block partials: partials[k*G + bid] = block sum of v[k].
+template <typename T, int NV>
+__device__ __forceinline__ void store_partials(T (&v)[NV], T* partials, int G,
+                                               int bid, T* lds) {
+  block_allsum<T, NV>(v, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[k * G + bid] = v[k];
   }
 }
 
-// Two-pass grid reduction.  Every block writes its NV partials; the last
-// block to arrive (ticket counter) sums all partials in index order, so the
-// result does not depend on block scheduling.  Returns true in the block
-// that holds the totals (valid in thread 0 of that block, in tot[]).
+// Totals of the partials written by the previous kernel (G blocks), same
+// value in every thread of every block.
 template <typename T, int NV>
-__device__ __forceinline__ bool grid_sum(T (&v)[NV], T* partials,
-                                         unsigned int* ticket, int nblocks,
-                                         T (&tot)[NV]) {
-  __shared__ T lds[NV * 16 > 64 ? NV * 16 : 64];
-  __shared__ bool is_last;
+__device__ __forceinline__ void reduce_partials(const T* __restrict__ partials,
+                                                int G, T (&tot)[NV], T* lds) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int nthreads = blockDim.x * blockDim.y;
-  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
-  block_sum<T, NV>(v, lds);
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) partials[k * nblocks + bid] = v[k];
-    __threadfence();
-    unsigned int t = atomicAdd(ticket, 1u);
-    is_last = (t == (unsigned int)(nblocks - 1));
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  T acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     T s = T(0);
-    for (int b = tid; b < nblocks; b += nthreads)
-      s += __hip_atomic_load(&partials[k * nblocks + b], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    acc[k] = s;
+    for (int b = tid; b < G; b += kThreads) s += partials[k * G + b];
+    tot[k] = s;
   }
-  __syncthreads();
-  block_sum<T, NV>(acc, lds);
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) tot[k] = acc[k];
-    *ticket = 0u;
-  }
-  return true;
+  block_allsum<T, NV>(tot, lds);
 }
 
 }  // namespace cpgpu

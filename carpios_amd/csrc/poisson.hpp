@@ -1,0 +1,114 @@
+// Host side of a device-resident Poisson problem, shared by the Poisson API
+// and the NS projection step.
+#pragma once
+
+#include <vector>
+
+#include "stencil.hpp"
+
+namespace cpgpu {
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  int alloc(size_t count) {
+    if (n >= count && p) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (hipMalloc(&p, sizeof(T) * (count ? count : 1)) != hipSuccess) return -1;
+    n = count;
+    return 0;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// b = -(vol*src + ghost constants) (s_poisson.hpp:79-97,
+// s_ghost.hpp:161-175, s_solver.hpp:122)
+struct RhsSeg {
+  int type;       // 1 Dirichlet, 2 Neumann
+  double coe;     // coefficient of the ghost term (negative)
+  double dl, d1;  // distances of the substitution
+};
+struct RhsDesc {
+  int nx, ny;
+  const double* vol;    // nx: (1.0*sx)*sx
+  const double* bcval;  // packed interior lines: x- ny | x+ ny | y- nx | y+ nx
+  RhsSeg seg[4];
+};
+
+// Sums gathered for the refinement loop and the mean projections.
+struct RState {
+  double bb, rr, rsum, xsum;
+};
+
+}  // namespace cpgpu
+
+struct cpgpu_poisson {
+  cpgpu_ctx* ctx = nullptr;
+  int nx = 0, ny = 0, gl = 0;
+  size_t N = 0;
+  double beta = 1.0;
+  cpgpu::Tiling tl{};
+  // host copies of the descriptor
+  std::vector<double> KX, KY, vol, bcv;
+  std::vector<double> cxi, cyi;  // interior cell centres
+  double adj[4] = {0, 0, 0, 0};
+  cpgpu::RhsSeg seg[4];
+  // device operator
+  cpgpu::DevBuf<double> dKX, dKY, dvol, dbcv;
+  cpgpu::DevBuf<float> fKX, fKY;
+  // FP64 vectors
+  cpgpu::DevBuf<double> x, b, r, p0, p1, q, src;
+  // FP32 vectors of the inner solve
+  cpgpu::DevBuf<float> b32, x32, r32, pa32, pb32, q32;
+  // reductions and states
+  cpgpu::DevBuf<double> partA64, partB64, part64;
+  cpgpu::DevBuf<float> partA32, partB32;
+  cpgpu::DevBuf<cpgpu::KState> kst;
+  cpgpu::DevBuf<cpgpu::RState> rst;
+  cpgpu::DevBuf<double> dhist;
+  cpgpu::KState* h_kst = nullptr;  // pinned mirrors
+  cpgpu::RState* h_rst = nullptr;
+  double inner_tol = 1e-4;
+  int max_outer = 40;
+
+  cpgpu::Sten<double> sten64() const {
+    cpgpu::Sten<double> s;
+    s.nx = nx;
+    s.ny = ny;
+    s.KX = dKX.p;
+    s.KY = dKY.p;
+    for (int k = 0; k < 4; ++k) s.adj[k] = adj[k];
+    return s;
+  }
+  cpgpu::Sten<float> sten32() const {
+    cpgpu::Sten<float> s;
+    s.nx = nx;
+    s.ny = ny;
+    s.KX = fKX.p;
+    s.KY = fKY.p;
+    for (int k = 0; k < 4; ++k) s.adj[k] = (float)adj[k];
+    return s;
+  }
+  ~cpgpu_poisson() {
+    if (h_kst) (void)hipHostFree(h_kst);
+    if (h_rst) (void)hipHostFree(h_rst);
+  }
+};
+
+namespace cpgpu {
+// Solves with the problem's own vectors (used by the NS step as well).
+int poisson_solve_internal(cpgpu_poisson* P, int method, int* max_iter,
+                           double* tol, int norm_mode, int opts, double* hist,
+                           int hist_cap, int* hist_len);
+// b from a device source field (interior-only, j contiguous).
+int poisson_rhs_from_device(cpgpu_poisson* P, const double* dsrc,
+                            double src_uniform);
+}  // namespace cpgpu

@@ -1,0 +1,268 @@
+// Device side of the matrix-free 5-point Poisson operator and the Krylov
+// kernels built on it.
+//
+// The operator is structure::StencilPoisson_<2> with GhostRegular_<2>
+// substitution (reference lib/structure/s_poisson.hpp:60-181,
+// s_ghost.hpp:140-357) applied without assembling CSR: per-face
+// coefficients K = beta*a/d are 1-D arrays (the grid is uniform along the
+// other axis), the diagonal is rebuilt per cell in the reference's order of
+// accumulation, and boundary rows fold the ghost terms exactly as the
+// reference's substitution does.  Vectors are interior-only nx*ny arrays,
+// j contiguous (the interior of a structure::Scalar_<2> buffer).
+//
+// Files including this header compile with fp contract off: the SpMV adds
+// the five terms in the reference's CSR order (matrix_SparCompRow.hpp:
+// 236-249) without FMAs, so A*x is bitwise the reference's.
+//
+// Work decomposition: TI x TJ tiles (TJ = one wavefront along the
+// contiguous j), a 1-D grid of G <= kMaxBlocks blocks of 64x4 threads walks
+// the tiles (tile t = bid + k*G).  Reductions follow core.hpp: partials per
+// block, totals formed in the consumer's prologue.
+#pragma once
+
+#include <cmath>
+
+#include "core.hpp"
+
+namespace cpgpu {
+
+template <typename T>
+struct Sten {
+  int nx, ny;
+  const T* KX;  // nx+1 face coefficients along x: face f between cells f-1, f
+  const T* KY;  // ny+1
+  T adj[4];     // diagonal adjustment per boundary segment x-, x+, y-, y+
+};
+
+constexpr int TJ = 64;  // tile width (j, contiguous) = one wavefront
+constexpr int TI = 16;  // tile height (i)
+constexpr int BY = 4;   // thread rows per block; each thread does TI/BY rows
+
+struct Tiling {
+  int ntj, ntiles, G;
+};
+
+inline Tiling make_tiling(int nx, int ny) {
+  Tiling t;
+  t.ntj = (ny + TJ - 1) / TJ;
+  t.ntiles = t.ntj * ((nx + TI - 1) / TI);
+  t.G = t.ntiles < kMaxBlocks ? t.ntiles : kMaxBlocks;
+  return t;
+}
+
+__device__ __forceinline__ void tile_origin(int t, int ntj, int& i0, int& j0) {
+  const int ti = t / ntj;
+  i0 = ti * TI;
+  j0 = (t - ti * ntj) * TJ;
+}
+
+// Centre coefficient: coe(index,0,_C_) (s_poisson.hpp:64-77) then the ghost
+// substitutions in Index_compare_ order x-, y-, y+, x+ (s_ghost.hpp:334-357).
+template <typename T>
+__device__ __forceinline__ T diag_at(const Sten<T>& s, int i, int j) {
+  T d = s.KX[i] + s.KX[i + 1];
+  d = d + s.KY[j];
+  d = d + s.KY[j + 1];
+  if (i == 0) d = d + s.adj[0];
+  if (j == 0) d = d + s.adj[2];
+  if (j == s.ny - 1) d = d + s.adj[3];
+  if (i == s.nx - 1) d = d + s.adj[1];
+  return d;
+}
+
+template <typename T>
+__device__ __forceinline__ T apply_global(const Sten<T>& s,
+                                          const T* __restrict__ x, int i,
+                                          int j, T dg) {
+  const int ny = s.ny;
+  const size_t k = (size_t)i * ny + j;
+  T acc = T(0);
+  if (i > 0) acc = acc + x[k - ny] * (-s.KX[i]);
+  if (j > 0) acc = acc + x[k - 1] * (-s.KY[j]);
+  acc = acc + x[k] * dg;
+  if (j < ny - 1) acc = acc + x[k + 1] * (-s.KY[j + 1]);
+  if (i < s.nx - 1) acc = acc + x[k + ny] * (-s.KX[i + 1]);
+  return acc;
+}
+
+// Device state of one Krylov solve.  FP32 solves store FP32 values exactly.
+// rho is double-buffered by iteration parity: block 0 of iteration it writes
+// rho[it & 1] while every block reads rho[(it - 1) & 1].
+struct KState {
+  double normb, tol, resid;
+  double rho[2];
+  int iter, max_iter;
+  int done;  // 0 running, 1 converged, 2 stopped at max_iter
+  int pad;
+};
+
+// Partial slots of the Krylov reductions (NV = 3 for every producer).
+enum { SLOT_RZ = 0, SLOT_RR = 1, SLOT_BB = 2 };
+
+// y = A x
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply(Sten<T> s, const T* __restrict__ x,
+                                               T* __restrict__ y, Tiling tl) {
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny)
+        y[(size_t)i * s.ny + j] = apply_global(s, x, i, j, diag_at(s, i, j));
+    }
+  }
+}
+
+// r = b - A x and the partials of r.z, r.r, b.b (nrm2 modes: the reference
+// drops the last unknown, BLAS_Level1.hpp:569).
+template <typename T, bool PC>
+__global__ __launch_bounds__(256) void k_cg_init(Sten<T> s, const T* __restrict__ x,
+                                                 const T* __restrict__ b,
+                                                 T* __restrict__ r,
+                                                 T* __restrict__ partB,
+                                                 Tiling tl, int norm_ref) {
+  __shared__ T lds[16];
+  const size_t nlast = (size_t)s.nx * s.ny - 1;
+  T v[3] = {T(0), T(0), T(0)};
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i >= s.nx || j >= s.ny) continue;
+      const size_t idx = (size_t)i * s.ny + j;
+      const T dg = diag_at(s, i, j);
+      const T bv = b[idx];
+      const T rv = bv - apply_global(s, x, i, j, dg);
+      r[idx] = rv;
+      const T z = PC ? rv * (T(1) / dg) : rv;
+      v[SLOT_RZ] += rv * z;
+      if (!(norm_ref && idx == nlast)) {
+        v[SLOT_RR] += rv * rv;
+        v[SLOT_BB] += bv * bv;
+      }
+    }
+  }
+  store_partials<T, 3>(v, partB, gridDim.x, blockIdx.x, lds);
+}
+
+// Iteration `it` (1-based), first kernel.  Prologue: totals of the previous
+// kernel's partials -> residual of iteration it-1, stopping test
+// (solver_matrix.hpp:356-372: resid <= tol), rho_it = r.z.  Body: p = z +
+// beta p_old over a (TI+2)x(TJ+2) LDS tile (halo recomputed from r and
+// p_old), q = A p from LDS, partial p.q.  p is double-buffered because
+// neighbouring tiles read p_old in their halos.
+template <typename T, bool PC>
+__global__ __launch_bounds__(256) void k_cg_dir(
+    Sten<T> s, const T* __restrict__ r, const T* __restrict__ pold,
+    T* __restrict__ pnew, T* __restrict__ q, KState* st,
+    const T* __restrict__ partB, T* __restrict__ partA, double* hist,
+    int hist_cap, int it, Tiling tl) {
+  if (st->done) return;
+  __shared__ T lds[16];
+  __shared__ T tile[TI + 2][TJ + 2];
+  T tot[3];
+  reduce_partials<T, 3>(partB, gridDim.x, tot, lds);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+  double normb;
+  if (it == 1) {
+    normb = std::sqrt((double)tot[SLOT_BB]);
+    if (normb == 0.0) normb = 1;
+    if (lead) st->normb = normb;
+  } else {
+    normb = st->normb;
+  }
+  const double resid = std::sqrt((double)tot[SLOT_RR]) / normb;
+  if (lead && it >= 2 && it - 2 < hist_cap && hist) hist[it - 2] = resid;
+  if (resid <= st->tol || it > st->max_iter) {
+    if (lead) {
+      st->iter = it - 1;
+      st->resid = resid;
+      st->done = resid <= st->tol ? 1 : 2;
+    }
+    return;
+  }
+  const T rho = tot[SLOT_RZ];
+  const bool first = it == 1;
+  const T beta = first ? T(0) : rho / (T)st->rho[(it - 1) & 1];
+  if (lead) st->rho[it & 1] = (double)rho;
+
+  const int nx = s.nx, ny = s.ny;
+  const int tid = threadIdx.x + TJ * threadIdx.y;
+  T v[1] = {T(0)};
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    __syncthreads();  // previous tile fully consumed
+    for (int e = tid; e < (TI + 2) * (TJ + 2); e += TJ * BY) {
+      const int ii = e / (TJ + 2), jj = e - ii * (TJ + 2);
+      const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
+      T pv = T(0);
+      if (gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
+        const size_t k = (size_t)gi * ny + gj;
+        const T rv = r[k];
+        const T z = PC ? rv * (T(1) / diag_at(s, gi, gj)) : rv;
+        pv = first ? z : z + beta * pold[k];
+        if (ii >= 1 && ii <= TI && jj >= 1 && jj <= TJ) pnew[k] = pv;
+      }
+      tile[ii][jj] = pv;
+    }
+    __syncthreads();
+    const int lj = threadIdx.x + 1, j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int li = threadIdx.y + BY * k + 1, i = i0 + li - 1;
+      if (i >= nx || j >= ny) continue;
+      const T dg = diag_at(s, i, j);
+      T acc = T(0);
+      if (i > 0) acc = acc + tile[li - 1][lj] * (-s.KX[i]);
+      if (j > 0) acc = acc + tile[li][lj - 1] * (-s.KY[j]);
+      acc = acc + tile[li][lj] * dg;
+      if (j < ny - 1) acc = acc + tile[li][lj + 1] * (-s.KY[j + 1]);
+      if (i < nx - 1) acc = acc + tile[li + 1][lj] * (-s.KX[i + 1]);
+      q[(size_t)i * ny + j] = acc;
+      v[0] += tile[li][lj] * acc;
+    }
+  }
+  store_partials<T, 1>(v, partA, gridDim.x, blockIdx.x, lds);
+}
+
+// Iteration `it`, second kernel: alpha = rho_it / p.q; x += alpha p;
+// r -= alpha q; partials of r.z and r.r for the next prologue.
+template <typename T, bool PC>
+__global__ __launch_bounds__(256) void k_cg_upd(
+    Sten<T> s, T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r,
+    const T* __restrict__ q, const KState* st, const T* __restrict__ partA,
+    T* __restrict__ partB, int it, Tiling tl, int norm_ref) {
+  if (st->done) return;
+  __shared__ T lds[16];
+  T pq[1];
+  reduce_partials<T, 1>(partA, gridDim.x, pq, lds);
+  const T alpha = (T)st->rho[it & 1] / pq[0];
+  const size_t nlast = (size_t)s.nx * s.ny - 1;
+  T v[3] = {T(0), T(0), T(0)};
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i >= s.nx || j >= s.ny) continue;
+      const size_t idx = (size_t)i * s.ny + j;
+      x[idx] = x[idx] + alpha * p[idx];
+      const T rv = r[idx] - alpha * q[idx];
+      r[idx] = rv;
+      const T z = PC ? rv * (T(1) / diag_at(s, i, j)) : rv;
+      v[SLOT_RZ] += rv * z;
+      if (!(norm_ref && idx == nlast)) v[SLOT_RR] += rv * rv;
+    }
+  }
+  store_partials<T, 3>(v, partB, gridDim.x, blockIdx.x, lds);
+}
+
+}  // namespace cpgpu
