@@ -55,6 +55,9 @@ EXPORTS = [
     "cpgpu_poisson_get_vec", "cpgpu_poisson_set_source_blobs",
     "cpgpu_blob_params", "cpgpu_poisson_fill_phi",
     "cpgpu_poisson2d_solve", "cpgpu_timer_enable", "cpgpu_timer_read",
+    "cpgpu_ns_create", "cpgpu_ns_destroy", "cpgpu_ns_set_field",
+    "cpgpu_ns_get_field", "cpgpu_ns_step", "cpgpu_ns_time",
+    "cpgpu_ns_projection",
 ]
 
 
@@ -264,3 +267,98 @@ def blob_params(seed, nblobs=64, wmin=0.01, wmax=0.1):
     if rc != OK:
         raise CpgpuError(rc, "bad blob parameters")
     return out.reshape(nblobs, 4)
+
+
+# ---------------------------------------------------------------- NS step
+LIMITERS = {"VanLeer": 0, "superbee": 1, "WAHYD": 2, "QUICK": 3, "CUI": 4,
+            "upwind2": 5}
+FIELDS = {"u": 0, "v": 1, "uf": 2, "vf": 3, "us": 4, "vs": 5, "usf": 6,
+          "vsf": 7, "p": 8, "phi": 9, "div": 10}
+
+
+class NSDesc(C.Structure):
+    _fields_ = [("grid", Grid2D), ("bcu_type", C.c_int * 4), ("bcu_val", _dp),
+                ("bcv_type", C.c_int * 4), ("bcv_val", _dp),
+                ("rho", C.c_double), ("mu", C.c_double), ("cfl", C.c_double),
+                ("dnum", C.c_double), ("dcoef", C.c_double),
+                ("limiter", C.c_int), ("solver", C.c_int),
+                ("max_iter", C.c_int), ("tol", C.c_double),
+                ("norm_mode", C.c_int), ("dt", C.c_double)]
+
+
+def pack_bc_values(nx, ny, gl, values):
+    """None, 4 per-segment scalars, or an already packed array."""
+    NX, NY = nx + 2 * gl, ny + 2 * gl
+    if values is None:
+        return np.zeros(2 * NY + 2 * NX)
+    if np.ndim(values) == 1 and len(values) == 4:
+        return np.concatenate([np.full(NY, values[0]), np.full(NY, values[1]),
+                               np.full(NX, values[2]), np.full(NX, values[3])])
+    return np.ascontiguousarray(values, dtype=np.float64)
+
+
+class NS:
+    """Device-resident NS_explicit_ projection stepper (cpgpu_ns)."""
+
+    def __init__(self, ctx, nx, ny, gl=2, xlim=(0.0, 1.0), ylim=(0.0, 1.0),
+                 bcu=((1, 1, 1, 1), None), bcv=((1, 1, 1, 1), None), rho=1.0,
+                 mu=1.0, cfl=0.5, dnum=1e10, dcoef=1.0, limiter="VanLeer",
+                 solver="JPCG", max_iter=100000, tol=1e-10, norm=NORM_REF,
+                 dt=1e-3):
+        self.ctx, self.nx, self.ny, self.gl = ctx, nx, ny, gl
+        g = uniform_grid_desc(nx, ny, gl, [2, 2, 2, 2], xlim=xlim, ylim=ylim)
+        d = NSDesc()
+        d.grid = g
+        self._keep = [g, g._keep]
+        for name, (types, vals) in (("bcu", bcu), ("bcv", bcv)):
+            arr = pack_bc_values(nx, ny, gl, vals)
+            self._keep.append(arr)
+            t = getattr(d, name + "_type")
+            for k in range(4):
+                t[k] = int(types[k])
+            setattr(d, name + "_val", _p(arr))
+        d.rho, d.mu, d.cfl, d.dnum, d.dcoef = rho, mu, cfl, dnum, dcoef
+        d.limiter = LIMITERS[limiter]
+        d.solver = METHODS[solver]
+        d.max_iter, d.tol, d.norm_mode, d.dt = max_iter, tol, norm, dt
+        self.desc = d
+        self.h = C.c_void_p()
+        ctx.check(lib().cpgpu_ns_create(ctx.h, C.byref(d), C.byref(self.h)))
+        ctx.adopt(self)
+
+    @property
+    def shape(self):
+        return (self.nx + 2 * self.gl, self.ny + 2 * self.gl)
+
+    def set_field(self, name, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.shape == self.shape
+        self.ctx.check(lib().cpgpu_ns_set_field(self.h, FIELDS[name], _p(f)))
+
+    def get_field(self, name):
+        out = np.zeros(self.shape)
+        self.ctx.check(lib().cpgpu_ns_get_field(self.h, FIELDS[name], _p(out)))
+        return out
+
+    def step(self, n=1):
+        it = np.zeros(max(n, 1), dtype=np.int32)
+        res = np.zeros(max(n, 1))
+        code = lib().cpgpu_ns_step(self.h, C.c_int(n), _p(it, _ip), _p(res))
+        self.ctx.check(code, allow=(OK, MAXITER))
+        return it[:n], res[:n]
+
+    def time(self):
+        t, dt = C.c_double(0), C.c_double(0)
+        self.ctx.check(lib().cpgpu_ns_time(self.h, C.byref(t), C.byref(dt)))
+        return t.value, dt.value
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().cpgpu_ns_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,647 @@
+// Device-resident NS_explicit_ projection step on gfx950.
+//
+// One step of structure::NS_explicit_<2>::run_one_step (reference
+// lib/structure/s_ns_explicit.hpp:118-155) followed by Time_::advance
+// (s_time.hpp:61-64), with every field resident in HBM:
+//   k_maxmag + k_dt ........ NS_::restrict_time (s_ns.hpp:146-169)
+//   k_ghost (u, v) ......... Equation_::apply_bc (s_equation.hpp:340-378)
+//   k_veo_star ............. TVD advection + explicit diffusion for both
+//                            components (s_ns_explicit.hpp:173-229,
+//                            s_operation.hpp:549-620, :664-688)
+//   k_ghost (u*, v*)
+//   k_c2f_div .............. InterpolateC2F + Div (s_operation.hpp:100-120,
+//                            :166-187) in one pass
+//   Poisson solve .......... the projection, all-Neumann, warm-started from
+//                            the previous phi (s_poisson.hpp:320-420)
+//   k_grad_correct ......... Grad, p = phi/dt, u*f -= grad phi,
+//                            u* -= F2C(grad phi), u <- u*, uf <- u*f
+//                            (s_operation.hpp:46-72, :141-164, :732-845)
+// Fields are ghost-padded (nx+2gl) x (ny+2gl) buffers, j contiguous, exactly
+// the reference's Scalar_ layout.  Arithmetic follows the reference's
+// evaluation order without FMA contraction; the face values that a
+// neighbouring cell would write are recomputed with the same operands, which
+// gives the same bits because IEEE addition and multiplication commute.
+//
+// Corner ghost cells (outside the domain on both axes) are filled from the
+// freshly filled edge ghosts; the reference's sweep leaves them depending on
+// the visiting order, and no stencil of the step reads them.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "poisson.hpp"
+
+namespace cpgpu {
+
+constexpr double kSmall = 1e-10;  // lib/type_define.hpp:16
+
+struct Axes {
+  int nx, ny, gl, NX, NY;
+  const double *cx, *sx, *cy, *sy;  // ghost-inclusive, indexed by I = i+gl
+};
+
+__device__ __forceinline__ size_t fidx(const Axes& a, int i, int j) {
+  return (size_t)(i + a.gl) * a.NY + (j + a.gl);
+}
+__device__ __forceinline__ double cax(const Axes& a, int d, int k) {
+  return d == 0 ? a.cx[k + a.gl] : a.cy[k + a.gl];
+}
+__device__ __forceinline__ double sax(const Axes& a, int d, int k) {
+  return d == 0 ? a.sx[k + a.gl] : a.sy[k + a.gl];
+}
+
+struct BCDev {
+  int type[4];
+  const double* val;  // packed per ghost line: x- NY | x+ NY | y- NX | y+ NX
+};
+
+struct NSDev {
+  double dt, t;
+  double maxmag;
+};
+
+// Limiters, s_operation.hpp:501-546 (std::min/std::max semantics).
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+
+template <int LIM>
+__device__ __forceinline__ double limiter(double r, double R) {
+  if (LIM == 0) return 0.5 * R * (r + fabs(r)) / (R - 1 + r);  // VanLeer
+  if (LIM == 1) return dmax(dmax(0.0, dmin(R * r, 1.0)), dmin(r, R));  // superbee
+  if (LIM == 2) {  // WAHYD
+    if (r <= 1.0) return 0.5 * R * (r + fabs(r)) / (R - 1 + r);
+    return dmin((r + R * r * fabs(r)) / (R + r * r), R);
+  }
+  if (LIM == 3) return 0.75 * r + 0.25;                 // QUICK
+  if (LIM == 4) return 2.0 / 3.0 * r + 1.0 / 3.0;       // CUI
+  return r;                                             // upwind2
+}
+
+// ---------------------------------------------------------------------
+// kernels
+
+// partial max of sqrt(u^2 + v^2) over the interior (s_vector.hpp:89-103)
+__global__ __launch_bounds__(256) void k_maxmag(Axes a, const double* __restrict__ u,
+                                                const double* __restrict__ v,
+                                                double* partials) {
+  __shared__ double lds[4];
+  double m = 0.0;
+  const size_t n = (size_t)a.nx * a.ny;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(k / a.ny), j = (int)(k - (size_t)i * a.ny);
+    const size_t f = fidx(a, i, j);
+    double sum = 0.0;
+    sum += u[f] * u[f];
+    sum += v[f] * v[f];
+    m = dmax(m, sqrt(sum));
+  }
+  // max is order independent: wave then block then store
+  for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_xor(m, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    partials[blockIdx.x] = dmax(dmax(dmax(lds[0], lds[1]), lds[2]), lds[3]);
+}
+
+// dt <- min(dt, CFL*min_size/max|u|, DN/kappa/sum(1/h^2)) when smaller
+// (s_ns.hpp:146-169, s_operation.hpp:847-867)
+__global__ __launch_bounds__(256) void k_dt(const double* partials, int G,
+                                            NSDev* st, double cfl,
+                                            double min_size, double dnum,
+                                            double ddenom, double dcoef) {
+  __shared__ double lds[4];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < G; b += blockDim.x) m = dmax(m, partials[b]);
+  for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_xor(m, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double mx = dmax(dmax(dmax(lds[0], lds[1]), lds[2]), lds[3]);
+    st->maxmag = mx;
+    const double rdt = cfl * min_size / mx;
+    double rdt2 = st->dt;
+    if (dnum < 100) rdt2 = dnum / dcoef / ddenom;
+    const double mn = dmin(rdt, rdt2);
+    if (st->dt > mn) st->dt = mn;
+  }
+}
+
+// Ghost fill of one field (s_equation.hpp:340-378).  One thread per ghost
+// cell; edge ghosts first (z = 0), corners second (z = 1).
+__device__ __forceinline__ void ghost_one(const Axes& a, double* f,
+                                          const BCDev& bc, int i, int j) {
+  int ax, seg, ori, oi, oj;
+  if (i < 0) { ax = 0; seg = 0; ori = -1; oi = 0; oj = j; }
+  else if (i >= a.nx) { ax = 0; seg = 1; ori = 1; oi = a.nx - 1; oj = j; }
+  else if (j < 0) { ax = 1; seg = 2; ori = -1; oi = i; oj = 0; }
+  else { ax = 1; seg = 3; ori = 1; oi = i; oj = a.ny - 1; }
+  const int line = (ax == 0 ? j : i) + a.gl;
+  const int off = seg == 0 ? 0 : seg == 1 ? a.NY : seg == 2 ? 2 * a.NY : 2 * a.NY + a.NX;
+  const double val = bc.val ? bc.val[off + line] : 0.0;
+  const double xg = cax(a, ax, ax == 0 ? i : j);
+  const double xo = cax(a, ax, ax == 0 ? oi : oj);
+  const double so = sax(a, ax, ax == 0 ? oi : oj);
+  const double xf = ori > 0 ? xo + so * 0.5 : xo - so * 0.5;
+  const double fo = f[fidx(a, oi, oj)];
+  if (bc.type[seg] == CPGPU_BC_DIRICHLET) {
+    const double u = (xg - xf) / (xo - xf);
+    f[fidx(a, i, j)] = (val * (1.0 - u)) + (fo * u);
+  } else {
+    const double dl = fabs(xg - xo);
+    f[fidx(a, i, j)] = fo - val * dl;
+  }
+}
+
+// edge ghosts of two fields (blockIdx.y selects the field)
+__global__ __launch_bounds__(256) void k_ghost_edges(Axes a, double* f0, BCDev b0,
+                                                     double* f1, BCDev b1) {
+  double* f = blockIdx.y == 0 ? f0 : f1;
+  const BCDev& bc = blockIdx.y == 0 ? b0 : b1;
+  // x edges: gl*ny each side, y edges: gl*nx each side
+  const int nxe = 2 * a.gl * a.ny, nye = 2 * a.gl * a.nx;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nxe + nye;
+       k += gridDim.x * blockDim.x) {
+    int i, j;
+    if (k < nxe) {
+      const int side = k / (a.gl * a.ny), r = k - side * a.gl * a.ny;
+      const int g = r / a.ny;
+      j = r - g * a.ny;
+      i = side == 0 ? -1 - g : a.nx + g;
+    } else {
+      const int kk = k - nxe;
+      const int side = kk / (a.gl * a.nx), r = kk - side * a.gl * a.nx;
+      const int g = r / a.nx;
+      i = r - g * a.nx;
+      j = side == 0 ? -1 - g : a.ny + g;
+    }
+    ghost_one(a, f, bc, i, j);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ghost_corners(Axes a, double* f0, BCDev b0,
+                                                       double* f1, BCDev b1) {
+  double* f = blockIdx.y == 0 ? f0 : f1;
+  const BCDev& bc = blockIdx.y == 0 ? b0 : b1;
+  const int per = a.gl * a.gl;
+  for (int k = threadIdx.x; k < 4 * per; k += blockDim.x) {
+    const int c = k / per, r = k - c * per;
+    const int gi = r / a.gl, gj = r - gi * a.gl;
+    const int i = (c & 1) ? a.nx + gi : -1 - gi;
+    const int j = (c & 2) ? a.ny + gj : -1 - gj;
+    ghost_one(a, f, bc, i, j);
+  }
+}
+
+// VdotNabla_TVD for one cell and component (s_operation.hpp:549-596)
+template <int LIM>
+__device__ __forceinline__ double tvd_cell(const Axes& a,
+                                           const double* __restrict__ uf,
+                                           const double* __restrict__ vf,
+                                           const double* __restrict__ phi,
+                                           int i, int j) {
+  double vdd[2];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const double* vface = d == 0 ? uf : vf;
+    const int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1;
+    const int c = d == 0 ? i : j;
+    const double vfp = vface[fidx(a, i, j)];
+    const double vfm = vface[fidx(a, i - ei, j - ej)];
+    const double veo = (vfp + vfm) * 0.5;
+    int Cm, Um, Dm, Cp, Up, Dp;
+    if (veo >= 0.0) {
+      Dm = 0; Cm = -1; Um = -2;
+      Dp = 1; Cp = 0; Up = -1;
+    } else {
+      Dm = -1; Cm = 0; Um = 1;
+      Dp = 0; Cp = 1; Up = 2;
+    }
+#define VAL(o) phi[fidx(a, i + ei * (o), j + ej * (o))]
+#define SZ(o) sax(a, d, c + (o))
+    double sC = SZ(Cp), sD = SZ(Dp), sU = SZ(Up);
+    double vC = VAL(Cp), vD = VAL(Dp), vU = VAL(Up);
+    double R = (sD + sC) / sC;
+    double r = (vC - vU) * (sD + sC) / (vD - vC + kSmall) / (sC + sU);
+    const double fp = vC + limiter<LIM>(r, R) / R * (vD - vC);
+    sC = SZ(Cm); sD = SZ(Dm); sU = SZ(Um);
+    vC = VAL(Cm); vD = VAL(Dm); vU = VAL(Um);
+    R = (sD + sC) / sC;
+    r = (vC - vU) * (sD + sC) / (vD - vC + kSmall) / (sC + sU);
+    const double fm = vC + limiter<LIM>(r, R) / R * (vD - vC);
+#undef VAL
+#undef SZ
+    vdd[d] = (vfp * fp - vfm * fm) / sax(a, d, c);
+  }
+  double res = 0.0;
+  res += vdd[0];
+  res += vdd[1];
+  return res;
+}
+
+// NablaMuNabla with uniform mu (s_operation.hpp:664-688)
+__device__ __forceinline__ double diffusion_cell(const Axes& a, double mu,
+                                                 const double* __restrict__ phi,
+                                                 int i, int j) {
+  double td[2];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1, c = d == 0 ? i : j;
+    const double f = phi[fidx(a, i, j)];
+    const double fp = phi[fidx(a, i + ei, j + ej)];
+    const double fm = phi[fidx(a, i - ei, j - ej)];
+    const double dm = mu * (f - fm) / (cax(a, d, c) - cax(a, d, c - 1));
+    const double dp = mu * (fp - f) / (cax(a, d, c + 1) - cax(a, d, c));
+    td[d] = (dp - dm) / sax(a, d, c);
+  }
+  double res = 0.0;
+  res += td[0];
+  res += td[1];
+  return res;
+}
+
+// u* for both components (s_ns_explicit.hpp:196-211)
+template <int LIM>
+__global__ __launch_bounds__(256) void k_veo_star(Axes a, const double* __restrict__ u,
+                                                  const double* __restrict__ v,
+                                                  const double* __restrict__ uf,
+                                                  const double* __restrict__ vf,
+                                                  double* __restrict__ us,
+                                                  double* __restrict__ vs,
+                                                  const NSDev* st, double rho,
+                                                  double mu) {
+  const double dt = st->dt;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= a.nx || j >= a.ny) return;
+  const size_t f = fidx(a, i, j);
+  {
+    const double adv = tvd_cell<LIM>(a, uf, vf, u, i, j);
+    const double dif = diffusion_cell(a, mu, u, i, j);
+    const double src = 0.0;
+    us[f] = (-adv + (dif + src) / rho) * dt + u[f];
+  }
+  {
+    const double adv = tvd_cell<LIM>(a, uf, vf, v, i, j);
+    const double dif = diffusion_cell(a, mu, v, i, j);
+    const double src = 0.0;
+    vs[f] = (-adv + (dif + src) / rho) * dt + v[f];
+  }
+}
+
+// face value of InterpolateC2F between cells k and k+1 along d
+__device__ __forceinline__ double c2f(const Axes& a, int d, int k, double ck,
+                                      double ckp) {
+  const double hs = sax(a, d, k) * 0.5;
+  const double hsp = sax(a, d, k + 1) * 0.5;
+  return (ckp * hs + ck * hsp) / (hs + hsp);
+}
+
+// u*f, v*f and div (s_operation.hpp:100-120, :166-187); div also into the
+// interior-only source of the projection.
+__global__ __launch_bounds__(256) void k_c2f_div(Axes a, const double* __restrict__ us,
+                                                 const double* __restrict__ vs,
+                                                 double* __restrict__ usf,
+                                                 double* __restrict__ vsf,
+                                                 double* __restrict__ div,
+                                                 double* __restrict__ div_int) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= a.nx || j >= a.ny) return;
+  const double uc = us[fidx(a, i, j)], vc = vs[fidx(a, i, j)];
+  const double fxp = c2f(a, 0, i, uc, us[fidx(a, i + 1, j)]);
+  const double fxm = c2f(a, 0, i - 1, us[fidx(a, i - 1, j)], uc);
+  const double fyp = c2f(a, 1, j, vc, vs[fidx(a, i, j + 1)]);
+  const double fym = c2f(a, 1, j - 1, vs[fidx(a, i, j - 1)], vc);
+  usf[fidx(a, i, j)] = fxp;
+  vsf[fidx(a, i, j)] = fyp;
+  if (i == 0) usf[fidx(a, -1, j)] = fxm;
+  if (j == 0) vsf[fidx(a, i, -1)] = fym;
+  double dv = 0.0;
+  dv += (fxp - fxm) / sax(a, 0, i);
+  dv += (fyp - fym) / sax(a, 1, j);
+  div[fidx(a, i, j)] = dv;
+  div_int[(size_t)i * a.ny + j] = dv;
+}
+
+// Grad of phi (Neumann value 0 ghosts equal the boundary cell), p = phi/dt,
+// face and centre corrections (s_ns_explicit.hpp:140-153).  phi is the
+// projection's interior-only vector.
+__device__ __forceinline__ double phi_at(const Axes& a, const double* __restrict__ x,
+                                         int i, int j) {
+  i = i < 0 ? 0 : (i >= a.nx ? a.nx - 1 : i);
+  j = j < 0 ? 0 : (j >= a.ny ? a.ny - 1 : j);
+  return x[(size_t)i * a.ny + j];
+}
+
+__global__ __launch_bounds__(256) void k_grad_correct(Axes a, const double* __restrict__ x,
+                                                      double* __restrict__ usf,
+                                                      double* __restrict__ vsf,
+                                                      double* __restrict__ us,
+                                                      double* __restrict__ vs,
+                                                      double* __restrict__ p,
+                                                      double* __restrict__ phi,
+                                                      const NSDev* st) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x;
+  const int I = blockIdx.y * blockDim.y + threadIdx.y;
+  if (I >= a.NX || J >= a.NY) return;
+  const int i = I - a.gl, j = J - a.gl;
+  const bool inx = i >= 0 && i < a.nx, iny = j >= 0 && j < a.ny;
+  const size_t f = (size_t)I * a.NY + J;
+  if (!inx && !iny) {  // corner: p and phi carry the ghost value
+    const double ph = phi_at(a, x, i, j);
+    phi[f] = ph;
+    p[f] = ph;
+    return;
+  }
+  if (!inx || !iny) {  // edge ghost
+    const double ph = phi_at(a, x, i, j);
+    phi[f] = ph;
+    p[f] = ph;
+    // the m-face slot of the boundary row/column carries grad phi
+    if (i == -1 && iny) {
+      const double g = (ph - phi_at(a, x, 0, j)) / (cax(a, 0, -1) - cax(a, 0, 0));
+      usf[f] = usf[f] - g;
+    }
+    if (j == -1 && inx) {
+      const double g = (ph - phi_at(a, x, i, 0)) / (cax(a, 1, -1) - cax(a, 1, 0));
+      vsf[f] = vsf[f] - g;
+    }
+    return;
+  }
+  const double ph = x[(size_t)i * a.ny + j];
+  const double pxp = phi_at(a, x, i + 1, j), pxm = phi_at(a, x, i - 1, j);
+  const double pyp = phi_at(a, x, i, j + 1), pym = phi_at(a, x, i, j - 1);
+  const double gxp = (pxp - ph) / (cax(a, 0, i + 1) - cax(a, 0, i));
+  const double gyp = (pyp - ph) / (cax(a, 1, j + 1) - cax(a, 1, j));
+  const double gxm = (ph - pxm) / (cax(a, 0, i) - cax(a, 0, i - 1));
+  const double gym = (ph - pym) / (cax(a, 1, j) - cax(a, 1, j - 1));
+  phi[f] = ph;
+  p[f] = ph * (1.0 / st->dt);
+  usf[f] = usf[f] - gxp;
+  vsf[f] = vsf[f] - gyp;
+  us[f] = us[f] - (gxp + gxm) * 0.5;
+  vs[f] = vs[f] - (gyp + gym) * 0.5;
+}
+
+__global__ void k_advance(NSDev* st) {
+  if (threadIdx.x == 0) st->t += st->dt;
+}
+
+}  // namespace cpgpu
+
+using namespace cpgpu;
+
+struct cpgpu_ns {
+  cpgpu_ctx* ctx = nullptr;
+  int nx = 0, ny = 0, gl = 0, NX = 0, NY = 0;
+  size_t NT = 0;
+  cpgpu_poisson* P = nullptr;  // all-Neumann projection problem
+  DevBuf<double> cx, sx, cy, sy;
+  DevBuf<double> bcu_val, bcv_val;
+  int bcu_type[4], bcv_type[4];
+  // fields: u, v, uf, vf, us, vs, usf, vsf, p, phi, div
+  DevBuf<double> F[11];
+  int slot[11];  // field id -> buffer (u/us and uf/usf swap every step)
+  DevBuf<double> div_int, part;
+  DevBuf<NSDev> st;
+  NSDev* h_st = nullptr;
+  double rho, mu, cfl, dnum, dcoef, min_size, ddenom;
+  int limiter, solver, max_iter, norm_mode, opts;
+  double tol;
+  int last_iters = 0;
+  double last_resid = 0.0;
+  long long total_iters = 0;
+
+  double* fld(int id) { return F[slot[id]].p; }
+  Axes axes() const {
+    Axes a;
+    a.nx = nx; a.ny = ny; a.gl = gl; a.NX = NX; a.NY = NY;
+    a.cx = cx.p; a.sx = sx.p; a.cy = cy.p; a.sy = sy.p;
+    return a;
+  }
+  ~cpgpu_ns() {
+    if (P) cpgpu_poisson_destroy(P);
+    if (h_st) (void)hipHostFree(h_st);
+  }
+};
+
+namespace {
+
+enum { F_U, F_V, F_UF, F_VF, F_US, F_VS, F_USF, F_VSF, F_P, F_PHI, F_DIV };
+
+int ns_upload(cpgpu_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  CPGPU_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                                ctx->stream));
+  return CPGPU_OK;
+}
+
+template <int LIM>
+void launch_veo(cpgpu_ns* S, dim3 g, dim3 b) {
+  hipLaunchKernelGGL(k_veo_star<LIM>, g, b, 0, S->ctx->stream, S->axes(),
+                     S->fld(F_U), S->fld(F_V), S->fld(F_UF), S->fld(F_VF),
+                     S->fld(F_US), S->fld(F_VS), S->st.p, S->rho, S->mu);
+}
+
+int ns_one_step(cpgpu_ns* S) {
+  cpgpu_ctx* ctx = S->ctx;
+  hipStream_t st = ctx->stream;
+  const Axes a = S->axes();
+  BCDev bu{{S->bcu_type[0], S->bcu_type[1], S->bcu_type[2], S->bcu_type[3]},
+           S->bcu_val.p};
+  BCDev bv{{S->bcv_type[0], S->bcv_type[1], S->bcv_type[2], S->bcv_type[3]},
+           S->bcv_val.p};
+  const int Gm = 512;
+  // restrict_time
+  hipLaunchKernelGGL(k_maxmag, dim3(Gm), dim3(256), 0, st, a, S->fld(F_U),
+                     S->fld(F_V), S->part.p);
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, st, S->part.p, Gm, S->st.p,
+                     S->cfl, S->min_size, S->dnum, S->ddenom, S->dcoef);
+  // u*, v*
+  const dim3 gedge(64, 2), gcorner(1, 2);
+  hipLaunchKernelGGL(k_ghost_edges, gedge, dim3(256), 0, st, a, S->fld(F_U), bu,
+                     S->fld(F_V), bv);
+  hipLaunchKernelGGL(k_ghost_corners, gcorner, dim3(256), 0, st, a, S->fld(F_U),
+                     bu, S->fld(F_V), bv);
+  const dim3 b2(64, 4), g2((S->ny + 63) / 64, (S->nx + 3) / 4);
+  switch (S->limiter) {
+    case 0: launch_veo<0>(S, g2, b2); break;
+    case 1: launch_veo<1>(S, g2, b2); break;
+    case 2: launch_veo<2>(S, g2, b2); break;
+    case 3: launch_veo<3>(S, g2, b2); break;
+    case 4: launch_veo<4>(S, g2, b2); break;
+    default: launch_veo<5>(S, g2, b2); break;
+  }
+  hipLaunchKernelGGL(k_ghost_edges, gedge, dim3(256), 0, st, a, S->fld(F_US),
+                     bu, S->fld(F_VS), bv);
+  hipLaunchKernelGGL(k_ghost_corners, gcorner, dim3(256), 0, st, a,
+                     S->fld(F_US), bu, S->fld(F_VS), bv);
+  // C2F + Div
+  hipLaunchKernelGGL(k_c2f_div, g2, b2, 0, st, a, S->fld(F_US), S->fld(F_VS),
+                     S->fld(F_USF), S->fld(F_VSF), S->fld(F_DIV), S->div_int.p);
+  CPGPU_HIP(ctx, hipGetLastError());
+  // projection
+  int rc = poisson_rhs_from_device(S->P, S->div_int.p, 0.0);
+  if (rc) return rc;
+  int mi = S->max_iter;
+  double tol = S->tol;
+  int hl = 0;
+  int code = poisson_solve_internal(S->P, S->solver, &mi, &tol, S->norm_mode,
+                                    S->opts, nullptr, 0, &hl);
+  if (code < 0) return code;
+  S->last_iters = mi;
+  S->last_resid = tol;
+  S->total_iters += mi;
+  // grad, corrections, p
+  const dim3 g3((S->NY + 63) / 64, (S->NX + 3) / 4);
+  hipLaunchKernelGGL(k_grad_correct, g3, b2, 0, st, a, S->P->x.p,
+                     S->fld(F_USF), S->fld(F_VSF), S->fld(F_US), S->fld(F_VS),
+                     S->fld(F_P), S->fld(F_PHI), S->st.p);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, S->st.p);
+  CPGPU_HIP(ctx, hipGetLastError());
+  // u <- u*, uf <- u*f by exchanging the buffers
+  std::swap(S->slot[F_U], S->slot[F_US]);
+  std::swap(S->slot[F_V], S->slot[F_VS]);
+  std::swap(S->slot[F_UF], S->slot[F_USF]);
+  std::swap(S->slot[F_VF], S->slot[F_VSF]);
+  return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpgpu_ns_create(cpgpu_ctx* ctx, const cpgpu_ns_desc* d, cpgpu_ns** out) {
+  if (!ctx || !d || !out) return CPGPU_EINVAL;
+  *out = nullptr;
+  const cpgpu_grid2d& g = d->grid;
+  if (g.nx < 1 || g.ny < 1 || g.gl < 2 || !g.cx || !g.sx || !g.cy || !g.sy)
+    return set_error(ctx, CPGPU_EINVAL, "NS needs gl >= 2 and axis arrays");
+  if (d->limiter < 0 || d->limiter > 5 || !(d->rho > 0.0) ||
+      !(d->dt > 0.0) || (d->solver != CPGPU_JPCG && d->solver != CPGPU_MIXED &&
+                         d->solver != CPGPU_CG))
+    return set_error(ctx, CPGPU_EINVAL, "bad NS parameters");
+  for (int s = 0; s < 4; ++s)
+    if ((d->bcu_type[s] != 1 && d->bcu_type[s] != 2) ||
+        (d->bcv_type[s] != 1 && d->bcv_type[s] != 2))
+      return set_error(ctx, CPGPU_EINVAL, "bad velocity bc type");
+  CPGPU_HIP(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<cpgpu_ns> S(new cpgpu_ns());
+  S->ctx = ctx;
+  S->nx = g.nx; S->ny = g.ny; S->gl = g.gl;
+  S->NX = g.nx + 2 * g.gl; S->NY = g.ny + 2 * g.gl;
+  S->NT = (size_t)S->NX * S->NY;
+  // projection: all-Neumann, value 0 (an empty BoundaryIndex, s_ns.hpp:222)
+  cpgpu_grid2d pg = g;
+  pg.beta = 1.0;
+  for (int s = 0; s < 4; ++s) pg.bc_type[s] = CPGPU_BC_NEUMANN;
+  pg.bc_val = nullptr;
+  int rc = cpgpu_poisson_create(ctx, &pg, &S->P);
+  if (rc) return rc;
+  for (int k = 0; k < 11; ++k) {
+    if (S->F[k].alloc(S->NT)) return set_error(ctx, CPGPU_ENOMEM, "ns fields");
+    CPGPU_HIP(ctx, hipMemsetAsync(S->F[k].p, 0, sizeof(double) * S->NT,
+                                  ctx->stream));
+    S->slot[k] = k;
+  }
+  const size_t nbc = 2 * (size_t)S->NY + 2 * (size_t)S->NX;
+  if (S->cx.alloc(S->NX) || S->sx.alloc(S->NX) || S->cy.alloc(S->NY) ||
+      S->sy.alloc(S->NY) || S->bcu_val.alloc(nbc) || S->bcv_val.alloc(nbc) ||
+      S->div_int.alloc((size_t)g.nx * g.ny) || S->part.alloc(1024) ||
+      S->st.alloc(1))
+    return set_error(ctx, CPGPU_ENOMEM, "ns buffers");
+  std::vector<double> zero(nbc, 0.0);
+  if ((rc = ns_upload(ctx, S->cx.p, g.cx, sizeof(double) * S->NX)) ||
+      (rc = ns_upload(ctx, S->sx.p, g.sx, sizeof(double) * S->NX)) ||
+      (rc = ns_upload(ctx, S->cy.p, g.cy, sizeof(double) * S->NY)) ||
+      (rc = ns_upload(ctx, S->sy.p, g.sy, sizeof(double) * S->NY)) ||
+      (rc = ns_upload(ctx, S->bcu_val.p, d->bcu_val ? d->bcu_val : zero.data(),
+                      sizeof(double) * nbc)) ||
+      (rc = ns_upload(ctx, S->bcv_val.p, d->bcv_val ? d->bcv_val : zero.data(),
+                      sizeof(double) * nbc)))
+    return rc;
+  for (int s = 0; s < 4; ++s) {
+    S->bcu_type[s] = d->bcu_type[s];
+    S->bcv_type[s] = d->bcv_type[s];
+  }
+  S->rho = d->rho; S->mu = d->mu; S->cfl = d->cfl; S->dnum = d->dnum;
+  S->dcoef = d->dcoef; S->limiter = d->limiter; S->solver = d->solver;
+  S->max_iter = d->max_iter; S->tol = d->tol; S->norm_mode = d->norm_mode;
+  S->opts = d->solver == CPGPU_MIXED ? CPGPU_OPT_PROJECT_MEAN : 0;
+  // Grid_::min_size over the ghost-inclusive size arrays; the diffusion
+  // denominator sum over d of 1/(min_size(d)^2) (s_operation.hpp:856-867)
+  double msx = g.sx[0], msy = g.sy[0];
+  for (int k = 1; k < S->NX; ++k) msx = g.sx[k] < msx ? g.sx[k] : msx;
+  for (int k = 1; k < S->NY; ++k) msy = g.sy[k] < msy ? g.sy[k] : msy;
+  S->min_size = msy < msx ? msy : msx;
+  double den = 0.0;
+  den += 1.0 / (msx * msx);
+  den += 1.0 / (msy * msy);
+  S->ddenom = den;
+  CPGPU_HIP(ctx, hipHostMalloc((void**)&S->h_st, sizeof(NSDev), 0));
+  S->h_st->dt = d->dt;
+  S->h_st->t = 0.0;
+  S->h_st->maxmag = 0.0;
+  CPGPU_HIP(ctx, hipMemcpyAsync(S->st.p, S->h_st, sizeof(NSDev),
+                                hipMemcpyHostToDevice, ctx->stream));
+  CPGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *out = S.release();
+  return CPGPU_OK;
+}
+
+int cpgpu_ns_destroy(cpgpu_ns* S) {
+  if (!S) return CPGPU_EINVAL;
+  (void)hipStreamSynchronize(S->ctx->stream);
+  delete S;
+  return CPGPU_OK;
+}
+
+int cpgpu_ns_set_field(cpgpu_ns* S, int which, const double* field) {
+  if (!S || !field || which < 0 || which > 10) return CPGPU_EINVAL;
+  if (which == F_PHI) {  // the warm start lives in the projection's vector
+    int rc = cpgpu_poisson_set_phi(S->P, field);
+    if (rc) return rc;
+  }
+  CPGPU_HIP(S->ctx, hipMemcpyAsync(S->fld(which), field, sizeof(double) * S->NT,
+                                   hipMemcpyHostToDevice, S->ctx->stream));
+  CPGPU_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
+  return CPGPU_OK;
+}
+
+int cpgpu_ns_get_field(cpgpu_ns* S, int which, double* field) {
+  if (!S || !field || which < 0 || which > 10) return CPGPU_EINVAL;
+  CPGPU_HIP(S->ctx, hipMemcpyAsync(field, S->fld(which), sizeof(double) * S->NT,
+                                   hipMemcpyDeviceToHost, S->ctx->stream));
+  CPGPU_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
+  return CPGPU_OK;
+}
+
+int cpgpu_ns_step(cpgpu_ns* S, int nsteps, int* iters, double* resid) {
+  if (!S || nsteps < 0) return CPGPU_EINVAL;
+  CPGPU_HIP(S->ctx, hipSetDevice(S->ctx->device));
+  int code = CPGPU_OK;
+  for (int k = 0; k < nsteps; ++k) {
+    code = ns_one_step(S);
+    if (code < 0) return code;
+    if (iters) iters[k] = S->last_iters;
+    if (resid) resid[k] = S->last_resid;
+  }
+  return code;
+}
+
+int cpgpu_ns_time(cpgpu_ns* S, double* t, double* dt) {
+  if (!S) return CPGPU_EINVAL;
+  CPGPU_HIP(S->ctx, hipMemcpyAsync(S->h_st, S->st.p, sizeof(NSDev),
+                                   hipMemcpyDeviceToHost, S->ctx->stream));
+  CPGPU_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
+  if (t) *t = S->h_st->t;
+  if (dt) *dt = S->h_st->dt;
+  return CPGPU_OK;
+}
+
+cpgpu_poisson* cpgpu_ns_projection(cpgpu_ns* S) { return S ? S->P : nullptr; }
+
+}  // extern "C"

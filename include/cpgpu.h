@@ -153,6 +153,65 @@ int cpgpu_poisson2d_solve(cpgpu_ctx *ctx, const cpgpu_grid2d *g,
                           double *tol, int norm_mode, double *hist,
                           int hist_cap, int *hist_len);
 
+/* ---- NS_explicit_ projection step (device resident) ----
+ * Replaces structure::NS_explicit_<2>::run_one_step + Time_::advance
+ * (s_ns_explicit.hpp:118-155, s_time.hpp:61-64) for a uniform grid, uniform
+ * rho and mu, zero source, the TVD advection schemes of
+ * NS_::set_advection_scheme (s_ns.hpp:171-211) and a Krylov projection. */
+typedef struct cpgpu_ns cpgpu_ns;
+
+/* limiter ids (s_operation.hpp:501-546) */
+#define CPGPU_LIM_VANLEER 0
+#define CPGPU_LIM_SUPERBEE 1
+#define CPGPU_LIM_WAHYD 2
+#define CPGPU_LIM_QUICK 3
+#define CPGPU_LIM_CUI 4
+#define CPGPU_LIM_UPWIND2 5
+
+/* field ids of cpgpu_ns_set_field / cpgpu_ns_get_field: the reference's
+ * "u","v","uf","vf","us","vs","usf","vsf","p","phi" scalars and the
+ * divergence of u*f */
+#define CPGPU_F_U 0
+#define CPGPU_F_V 1
+#define CPGPU_F_UF 2
+#define CPGPU_F_VF 3
+#define CPGPU_F_US 4
+#define CPGPU_F_VS 5
+#define CPGPU_F_USF 6
+#define CPGPU_F_VSF 7
+#define CPGPU_F_P 8
+#define CPGPU_F_PHI 9
+#define CPGPU_F_DIV 10
+
+typedef struct {
+  cpgpu_grid2d grid;   /* axes (gl >= 2 for TVD); bc/beta fields unused */
+  int bcu_type[4];     /* velocity boundary types per segment */
+  const double *bcu_val; /* packed per ghost line, as cpgpu_grid2d.bc_val */
+  int bcv_type[4];
+  const double *bcv_val;
+  double rho, mu;      /* NS_::set_uniform_rho / set_uniform_mu */
+  double cfl;          /* NS_::set_CFL_number */
+  double dnum, dcoef;  /* NS_::set_diffusion_number (no limit when >= 100) */
+  int limiter;         /* CPGPU_LIM_* */
+  int solver;          /* CPGPU_JPCG, CPGPU_CG or CPGPU_MIXED */
+  int max_iter;        /* projection solver limits */
+  double tol;
+  int norm_mode;
+  double dt;           /* initial time step (Time_ dt) */
+} cpgpu_ns_desc;
+
+int cpgpu_ns_create(cpgpu_ctx *ctx, const cpgpu_ns_desc *d, cpgpu_ns **out);
+int cpgpu_ns_destroy(cpgpu_ns *S);
+/* Host ghost-padded fields in/out (CPGPU_F_*). */
+int cpgpu_ns_set_field(cpgpu_ns *S, int which, const double *field);
+int cpgpu_ns_get_field(cpgpu_ns *S, int which, double *field);
+/* nsteps projection steps; per-step projection iterations and final
+ * residuals into iters/resid (may be NULL).  Returns the last solver code. */
+int cpgpu_ns_step(cpgpu_ns *S, int nsteps, int *iters, double *resid);
+int cpgpu_ns_time(cpgpu_ns *S, double *t, double *dt);
+/* The projection problem (all-Neumann Poisson) of the step. */
+cpgpu_poisson *cpgpu_ns_projection(cpgpu_ns *S);
+
 /* ---- kernel timing (HIP events on the context stream) ----
  * When enabled, every launch of the named kernel family is bracketed by
  * events; cpgpu_timer_read returns launches and summed milliseconds.

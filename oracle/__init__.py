@@ -5,7 +5,7 @@ TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg import this module as the checker.  The product package
 ``carpios_amd`` never imports it.
 """
-import ctypes as C
+import ctypes as C  # noqa: N812
 import os
 
 import numpy as np

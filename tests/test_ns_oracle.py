@@ -1,0 +1,183 @@
+"""Pins the oracle's structured operations and the NS_explicit_ step:
+exactness on constant/linear/quadratic fields, limiter special values,
+Div(Grad) consistency with the assembled Poisson operator, and the
+projection property of one step (s_operation.hpp, s_ns_explicit.hpp)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def grid(n=24, m=20, gl=2, lo=-0.5, hi=0.5):
+    return O.StructGrid(n, m, gl, lo, hi, lo, hi)
+
+
+def full_centres(g):
+    return np.meshgrid(g.ax.c, g.ay.c, indexing="ij")
+
+
+@pytest.mark.parametrize("lim", list(O.LIM))
+def test_tvd_of_constant_field_is_constant_times_face_divergence(lim):
+    g = grid()
+    phi = np.full(g.shape, 3.25)
+    rng = np.random.default_rng(0)
+    uf = rng.standard_normal(g.shape)
+    vf = rng.standard_normal(g.shape)
+    adv = O.tvd_field(g, uf, vf, phi, lim)
+    # (vf(i)*c - vf(i-1)*c)/h: a divergence term times the constant
+    hx, hy = g.ax.s[0], g.ay.s[0]
+    expect = np.zeros((g.nx, g.ny))
+    G = g.gl
+    for i in range(g.nx):
+        for j in range(g.ny):
+            I, J = i + G, j + G
+            expect[i, j] = ((uf[I, J] * 3.25 - uf[I - 1, J] * 3.25) / hx +
+                            (vf[I, J] * 3.25 - vf[I, J - 1] * 3.25) / hy)
+    assert np.allclose(adv, expect, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("lim", ["VanLeer", "superbee", "WAHYD", "QUICK", "CUI"])
+def test_tvd_exact_for_linear_field_constant_velocity(lim):
+    # r = 1 everywhere and psi(1) = 1 for these limiters: central faces,
+    # exact for a linear phi
+    g = grid(16, 16)
+    X, Y = full_centres(g)
+    phi = 0.7 * X - 1.3 * Y + 0.2
+    for (a, b) in ((0.9, -0.4), (-0.3, 0.6)):
+        uf = np.full(g.shape, a)
+        vf = np.full(g.shape, b)
+        adv = O.tvd_field(g, uf, vf, phi, lim)
+        assert np.allclose(adv, a * 0.7 + b * (-1.3), atol=1e-9)
+
+
+def test_upwind2_is_second_order_upwind():
+    # limiter r: face = C + r/R (D - C) = C + (C - U)/2 on a uniform grid
+    g = grid(12, 12)
+    X, Y = full_centres(g)
+    phi = X ** 2
+    uf = np.full(g.shape, 1.0)
+    vf = np.zeros(g.shape)
+    adv = O.tvd_field(g, uf, vf, phi, "upwind2")
+    h = g.ax.s[0]
+    Xi = X[g.gl:-g.gl, g.gl:-g.gl]
+    # second-order upwind face C + (C-U)/2, with the reference's SMALL in
+    # r = (C-U)/((D-C)+SMALL): where D == C (x^2 at +-h/2) the face is C
+    def face(U, Cc, D):
+        return Cc + (Cc - U) * (D - Cc) / ((D - Cc) + 1e-10) / 2
+    G = g.gl
+    P = phi[:, G:G + g.ny]
+    fp = face(P[G - 1:G - 1 + g.nx], P[G:G + g.nx], P[G + 1:G + 1 + g.nx])
+    fm = face(P[G - 2:G - 2 + g.nx], P[G - 1:G - 1 + g.nx], P[G:G + g.nx])
+    assert np.allclose(adv, (fp - fm) / h, atol=1e-9)
+    # away from the symmetric centre it is the unregularised upwind formula
+    far = np.abs(Xi[:, 0]) > 2 * h
+    exact = ((Xi ** 2 + (Xi ** 2 - (Xi - h) ** 2) / 2) -
+             ((Xi - h) ** 2 + ((Xi - h) ** 2 - (Xi - 2 * h) ** 2) / 2)) / h
+    assert np.allclose(adv[far], exact[far], atol=1e-6)
+
+
+def test_vanleer_falls_back_to_upwind_at_extremum():
+    # at a local maximum r < 0 -> psi = 0 -> face value = upwind cell value
+    g = grid(9, 9)
+    phi = np.zeros(g.shape)
+    G = g.gl
+    phi[G + 4, :] = 1.0
+    uf = np.full(g.shape, 1.0)
+    vf = np.zeros(g.shape)
+    adv = O.tvd_field(g, uf, vf, phi, "VanLeer")
+    h = g.ax.s[0]
+    # cell 4 (max, r<0 on its p face): fp = 1, fm from cell 3 (r = 0 -> 0)
+    assert adv[4, 3] == pytest.approx((1.0 - 0.0) / h, rel=1e-12)
+    # cell 5: fp = phi5 (0) + ..., fm = phi4 = 1 (upwind, limiter 0)
+    assert adv[5, 3] == pytest.approx((0.0 - 1.0) / h, rel=1e-12)
+
+
+def test_diffusion_exact_for_quadratic():
+    g = grid(14, 11)
+    X, Y = full_centres(g)
+    phi = X ** 2 + 2 * Y ** 2
+    d = O.diffusion_field(g, 0.5, phi)
+    assert np.allclose(d, 0.5 * (2 + 4), atol=1e-8)
+
+
+def test_div_grad_equals_neumann_poisson_operator():
+    g = O.StructGrid(15, 12, 2)
+    rng = np.random.default_rng(4)
+    phi = g.field()
+    g.interior(phi)[:] = rng.standard_normal((15, 12))
+    O.apply_bc(g, phi, O.make_bc(g, [2, 2, 2, 2]))
+    gx, gy = O.grad(g, phi)
+    dv = g.interior(O.div(g, gx, gy))
+    A = O.Matrix(g, O.make_bc(g, [2, 2, 2, 2]), 1.0)
+    vol = g.ax.s[0] * g.ay.s[0]
+    lhs = O.field_to_x(g, np.pad(dv, 2)) * vol
+    rhs = -A.mult(O.field_to_x(g, phi))
+    assert np.allclose(lhs, rhs, atol=1e-10 * np.abs(rhs).max())
+
+
+def test_c2f_f2c_linear_and_ghost_slot():
+    g = grid(10, 10)
+    X, Y = full_centres(g)
+    uc, vc = 2 * X + 1, -Y
+    uf, vf = O.c2f(g, uc, vc)
+    G = g.gl
+    h = g.ax.s[0]
+    # x faces at i = -1..nx-1 hold the linear value at the face
+    fx = 2 * (X[G - 1:G + g.nx, G:G + g.ny] + h / 2) + 1
+    assert np.allclose(uf[G - 1:G + g.nx, G:G + g.ny], fx, atol=1e-12)
+    cx, cy = O.f2c(g, uf, vf)
+    assert np.allclose(g.interior(cx), g.interior(uc), atol=1e-12)
+
+
+def test_ghost_fill_closed_forms():
+    g = O.StructGrid(8, 8, 2)
+    f = g.field()
+    g.interior(f)[:] = 5.0
+    O.apply_bc(g, f, O.make_bc(g, [1, 2, 1, 1], [2.0, 0.5, 0.0, 1.0]))
+    G = g.gl
+    h = g.ax.s[0]
+    # BC1: g1 = 2v - phi0, g2 = 4v - 3 phi0
+    assert np.allclose(f[G - 1, G:-G], 2 * 2.0 - 5.0)
+    assert np.allclose(f[G - 2, G:-G], 4 * 2.0 - 3 * 5.0)
+    # BC2: g = phi0 - v*dl with dl = h and 2h
+    assert np.allclose(f[G + 8, G:-G], 5.0 - 0.5 * h)
+    assert np.allclose(f[G + 9, G:-G], 5.0 - 0.5 * 2 * h)
+
+
+def cavity(n, Re=1000.0, steps=0, lim="VanLeer", tol=1e-10):
+    g = O.StructGrid(n, n, 2, -0.5, 0.5, -0.5, 0.5)
+    bcu = O.make_bc(g, [1, 1, 1, 1], [0, 0, 0, 1.0])
+    bcv = O.make_bc(g, [1, 1, 1, 1])
+    h = 1.0 / n
+    ns = O.NS(g, bcu, bcv, rho=Re, mu=1.0, cfl=0.5, dnum=0.25, dcoef=1 / Re,
+              limiter=lim, tol=tol, dt=0.5 * h)
+    for _ in range(steps):
+        ns.step()
+    return g, ns
+
+
+def test_projection_makes_faces_divergence_free():
+    g, ns = cavity(24, steps=3)
+    dv = g.interior(O.div(g, ns.f["uf"], ns.f["vf"]))
+    dstar = g.interior(ns.f["div"])
+    assert np.abs(dv).max() < 1e-7 * max(np.abs(dstar).max(), 1e-300)
+    # lid drives the top row forward
+    assert g.interior(ns.f["u"])[:, -1].mean() > 0
+
+
+def test_cavity_dt_follows_restrict_time():
+    g, ns = cavity(32, steps=2)
+    h = 1.0 / 32
+    # dt only shrinks: min(dt0 = 0.5 h, DN/kappa/sum(1/h^2)), CFL not binding
+    # while max|u| < 1 (s_ns.hpp:146-169)
+    assert ns.dt == min(0.5 * h, 0.25 / (1 / 1000.0) / (2.0 / h ** 2))
+    g2 = O.StructGrid(512, 512, 2, -0.5, 0.5, -0.5, 0.5)
+    A = O.make_bc(g2, [1, 1, 1, 1])
+    prm = O.NSParams(1000.0, 1.0, 0.5, 0.25, 1 / 1000.0, 0, 1, 1e-10, 0)
+    st = O.NSState(*([None] * 11), 0.5 / 512, 0.0)
+    u = g2.field()
+    st.u, st.v = O._ptr(u), O._ptr(u)
+    O.lib().orc_ns_restrict_time(O.C.byref(g2.c), O.C.byref(prm), O.C.byref(st))
+    h2 = 1.0 / 512
+    assert st.dt == 0.25 / (1 / 1000.0) / (1.0 / (h2 * h2) + 1.0 / (h2 * h2))
+    assert st.dt < 0.5 * h2
