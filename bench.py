@@ -7,8 +7,12 @@ timed method is the fastest path that reaches the FP64 tolerance (FP32
 Jacobi-PCG inside FP64 iterative refinement); plain FP64 Jacobi-PCG on the
 same GPU and the CPU oracle are reported beside it.
 
-Prints one JSON line (contract in the repository docs).  --gpus N>1 exits:
-this build runs on one GPU.
+The roofline figure is the GPU time per launch of the dominant kernel of
+the timed method, measured in this process with HIP events around a
+replayed hipGraph of back-to-back launches of that kernel
+(cpgpu_poisson_kernel_times), against the kernel's algorithmic bytes.
+
+Prints one JSON line.  --gpus N>1 exits: this build runs on one GPU.
 """
 import argparse
 import json
@@ -36,14 +40,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hbm", action="store_true",
                     help="skip the 4096^2 HBM roofline probe")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the plain FP64 J-PCG comparison")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
         log("bench.py: only --gpus 1 is supported (single-GPU build)")
         sys.exit(2)
-
-    import numpy as np
 
     import carpios_amd as ca
 
@@ -54,7 +58,8 @@ def main():
     P = ca.Poisson(ctx, desc)
     P.set_source_blobs(seed=0, nblobs=64, wmin=0.01, wmax=0.1)
     P.set_mixed(inner_tol=1e-4, max_outer=40)
-    log(f"bench: {n}^2 all-Neumann blob problem ready")
+    tune = P.autotune(reps=20)
+    log(f"bench: {n}^2 all-Neumann blob problem ready; autotune {tune}")
 
     def solve(method):
         P.fill_phi(0.0)
@@ -69,81 +74,72 @@ def main():
     for _ in range(a.warmup):
         solve("MIXED")
     ctx.sync()
-    ctx.timer_enable(0, True)
-    ctx.timer_enable(1, True)
     t0 = time.perf_counter()
     iters = []
-    for s in range(a.steps):
+    for _ in range(a.steps):
         it, res = solve("MIXED")
         iters.append(it)
     ctx.sync()
-    t1 = time.perf_counter()
-    n0, ms0 = ctx.timer_read(0)
-    n1, ms1 = ctx.timer_read(1)
-    ctx.timer_enable(0, False)
-    ctx.timer_enable(1, False)
-    wall = t1 - t0
+    wall = time.perf_counter() - t0
     value = a.steps / wall
     true_res = P.residual()
     log(f"bench: mixed {value:.2f} solves/s, {iters[-1]} inner it, true res {true_res:.2e}")
 
     # ---- GPU baseline: plain FP64 Jacobi-PCG ----
-    nb = max(1, min(a.steps, 3))
-    solve("JPCG")
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(nb):
-        it64, res64 = solve("JPCG")
-    ctx.sync()
-    fp64_rate = nb / (time.perf_counter() - t0)
-    log(f"bench: fp64 J-PCG {fp64_rate:.2f} solves/s, {it64} it")
+    fp64 = None
+    it64 = None
+    if not a.no_fp64:
+        nb = max(1, min(a.steps, 3))
+        solve("JPCG")
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(nb):
+            it64, res64 = solve("JPCG")
+        ctx.sync()
+        rate = nb / (time.perf_counter() - t0)
+        fp64 = {"solves_per_s": round(rate, 4), "iterations": it64,
+                "speedup_of_value": round(value / rate, 3)}
+        log(f"bench: fp64 J-PCG {rate:.2f} solves/s, {it64} it")
 
-    # dominant kernel family of the headline solve
-    fam = [(ms0, n0, 4, "k_cg_dir<float> (p update + A p + p.q)"),
-           (ms1, n1, 6, "k_cg_upd<float> (x,r update + r.z, r.r)")]
-    ms, launches, nvec, kname = max(fam)
-    avg_s = ms / 1e3 / max(launches, 1)
+    # ---- dominant kernel of the headline solve: GPU time per launch ----
+    d32, u32 = P.kernel_times(32, reps=100)
+    fam = [(d32, 4, f"k_cg_dir (variant {tune['fp32'][0]}) <float>: p update + A p + p.q"),
+           (u32, 6, "k_cg_upd<float>: x, r update + r.z, r.r")]
+    us, nvec, kname = max(fam)
     alg_bytes = nvec * N * 4
-    achieved = alg_bytes / avg_s / 1e9
+    achieved = alg_bytes / (us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": kname, "bytes_per_launch": alg_bytes,
-                "avg_launch_us": round(avg_s * 1e6, 3), "launches": launches,
-                "note": "1024^2 FP32 vectors (4 MiB each) stay in the 256 MiB "
-                        "Infinity Cache, so this rate can exceed HBM"}
+                "avg_launch_us": round(us, 3),
+                "other_kernel_us": round(min(d32, u32), 3),
+                "iteration_us": round(d32 + u32, 3),
+                "note": f"{n}^2 FP32 vectors ({N * 4 / 2**20:.0f} MiB each) are "
+                        "Infinity-Cache resident; see roofline_hbm_probe for HBM"}
 
-    # ---- HBM-resident probe: FP64 stencil kernel at 4096^2 ----
+    # ---- HBM-resident probe: the same kernels at 4096^2 in FP64 ----
     hbm = None
     if not a.no_hbm:
         m = 4096
         Pm = ca.Poisson(ctx, ca.uniform_grid_desc(m, m, 1, [ca.BC_NEUMANN] * 4))
         Pm.set_source_blobs(seed=0)
-        Pm.fill_phi(0.0)
-        Pm.solve("JPCG", max_iter=4, tol=1e-30, norm=ca.NORM_TRUE)
-        ctx.sync()
-        ctx.timer_enable(0, True)
-        ctx.timer_enable(1, True)
-        Pm.fill_phi(0.0)
-        Pm.solve("JPCG", max_iter=60, tol=1e-30, norm=ca.NORM_TRUE)
-        k0, t0ms = ctx.timer_read(0)
-        k1, t1ms = ctx.timer_read(1)
-        ctx.timer_enable(0, False)
-        ctx.timer_enable(1, False)
+        tune_m = Pm.autotune(reps=5)
+        d64, u64 = Pm.kernel_times(64, reps=20)
         Nm = m * m
-        d_gbs = 4 * Nm * 8 / (t0ms / 1e3 / k0) / 1e9
-        u_gbs = 6 * Nm * 8 / (t1ms / 1e3 / k1) / 1e9
-        hbm = {"grid": "4096^2 FP64", "k_cg_dir_GBs": round(d_gbs, 1),
+        d_gbs = 4 * Nm * 8 / (d64 * 1e-6) / 1e9
+        u_gbs = 6 * Nm * 8 / (u64 * 1e-6) / 1e9
+        hbm = {"grid": "4096^2 FP64", "dir_variant": tune_m["fp64"][0],
+               "k_cg_dir_us": round(d64, 2), "k_cg_dir_GBs": round(d_gbs, 1),
                "k_cg_dir_frac": round(d_gbs / HBM_PEAK_GBS, 4),
-               "k_cg_upd_GBs": round(u_gbs, 1),
-               "k_cg_upd_frac": round(u_gbs / HBM_PEAK_GBS, 4),
-               "iteration_us": round((t0ms / k0 + t1ms / k1) * 1e3, 2)}
+               "k_cg_upd_us": round(u64, 2), "k_cg_upd_GBs": round(u_gbs, 1),
+               "k_cg_upd_frac": round(u_gbs / HBM_PEAK_GBS, 4)}
         Pm.close()
         log(f"bench: 4096^2 fp64 dir {d_gbs:.0f} GB/s, upd {u_gbs:.0f} GB/s")
 
     # ---- CPU baseline: the oracle's FP64 J-PCG, bounded sample ----
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and it64:
         import oracle as O
         og = O.StructGrid(n, n, 1)
         A = O.Matrix(og, O.make_bc(og, [2, 2, 2, 2]), 1.0)
@@ -172,9 +168,8 @@ def main():
                                f"true rel-res {a.tol:g}; FP32 Jacobi-PCG inner "
                                f"(rel 1e-4) in FP64 iterative refinement",
                    "method": "MIXED", "inner_iterations": iters[-1],
-                   "true_rel_residual": true_res},
-        "gpu_fp64_jpcg": {"solves_per_s": round(fp64_rate, 4), "iterations": it64,
-                          "speedup_of_value": round(value / fp64_rate, 3)},
+                   "autotune": tune, "true_rel_residual": true_res},
+        "gpu_fp64_jpcg": fp64,
         "roofline": roofline,
         "roofline_hbm_probe": hbm,
         "cpu_baseline": cpu,

@@ -57,7 +57,9 @@ EXPORTS = [
     "cpgpu_poisson2d_solve", "cpgpu_timer_enable", "cpgpu_timer_read",
     "cpgpu_ns_create", "cpgpu_ns_destroy", "cpgpu_ns_set_field",
     "cpgpu_ns_get_field", "cpgpu_ns_step", "cpgpu_ns_time",
-    "cpgpu_ns_projection",
+    "cpgpu_ns_projection", "cpgpu_poisson_set_variant",
+    "cpgpu_poisson_autotune", "cpgpu_poisson_kernel_times",
+    "cpgpu_poisson_set_batch",
 ]
 
 
@@ -362,3 +364,36 @@ class NS:
             self.close()
         except Exception:
             pass
+
+
+def _set_variant(self, v64, v32):
+    self.ctx.check(lib().cpgpu_poisson_set_variant(self.h, int(v64), int(v32)))
+
+
+def _autotune(self, reps=10):
+    v64, v32 = C.c_int(0), C.c_int(0)
+    t64, t32 = C.c_float(0), C.c_float(0)
+    self.ctx.check(lib().cpgpu_poisson_autotune(
+        self.h, int(reps), C.byref(v64), C.byref(v32), C.byref(t64),
+        C.byref(t32)))
+    return {"fp64": (v64.value, t64.value), "fp32": (v32.value, t32.value)}
+
+
+Poisson.set_variant = _set_variant
+Poisson.autotune = _autotune
+
+
+def _kernel_times(self, precision=32, reps=50):
+    d, u = C.c_double(0), C.c_double(0)
+    self.ctx.check(lib().cpgpu_poisson_kernel_times(
+        self.h, int(precision), int(reps), C.byref(d), C.byref(u)))
+    return d.value, u.value
+
+
+def _set_batch(self, iters=16, use_graphs=True):
+    self.ctx.check(lib().cpgpu_poisson_set_batch(self.h, int(iters),
+                                                 int(bool(use_graphs))))
+
+
+Poisson.kernel_times = _kernel_times
+Poisson.set_batch = _set_batch

@@ -331,9 +331,98 @@ struct CGVecs {
   T *x, *b, *r, *pa, *pb, *q, *partA, *partB;
 };
 
-// Jacobi-PCG (PC) or CG on device vectors.  Iterations are launched in
-// batches; the device flag stops them exactly where the reference stops,
-// and launches after the stop return at once.
+// Launch of the direction kernel of iteration `it`: variant 0 is the LDS
+// tile kernel, variants 1..4 the column march with H = 4, 8, 16, 32.
+template <typename T, bool PC>
+int dir_blocks(int variant, int nx, int ny, const Tiling& tl) {
+  if (variant == 0) return tl.G;
+  static const int Hs[5] = {0, 4, 8, 16, 32};
+  const March m = make_march(nx, ny, Hs[variant]);
+  return m.nunits < kMaxBlocks ? m.nunits : kMaxBlocks;
+}
+
+template <typename T, bool PC>
+void launch_dir(int variant, hipStream_t st, const Sten<T>& s, const T* r,
+                const T* pold, T* pnew, T* q, KState* kst, const T* partB,
+                T* partA, double* hist, int hist_cap, const Tiling& tl,
+                int Gprev, int G) {
+  const dim3 block(TJ, BY);
+  if (variant == 0) {
+    hipLaunchKernelGGL((k_cg_dir<T, PC>), dim3(G), block, 0, st, s, r, pold,
+                       pnew, q, kst, partB, partA, hist, hist_cap, tl, Gprev);
+    return;
+  }
+#define CPGPU_DIR_MARCH(HV)                                                   \
+  hipLaunchKernelGGL((k_cg_dir_march<T, PC, HV>), dim3(G), block, 0, st, s, r, \
+                     pold, pnew, q, kst, partB, partA, hist, hist_cap,        \
+                     make_march(s.nx, s.ny, HV), Gprev)
+  switch (variant) {
+    case 1: CPGPU_DIR_MARCH(4); break;
+    case 2: CPGPU_DIR_MARCH(8); break;
+    case 3: CPGPU_DIR_MARCH(16); break;
+    default: CPGPU_DIR_MARCH(32); break;
+  }
+#undef CPGPU_DIR_MARCH
+}
+
+// One batch of `iters` CG iterations (iters even: the p buffers alternate
+// and every batch starts at an odd iteration).
+template <typename T, bool PC>
+void launch_batch(cpgpu_poisson* P, hipStream_t st, const Sten<T>& s,
+                  const CGVecs<T>& v, int variant, int Gd, int iters,
+                  int norm_ref, double* dhist, int hist_cap, bool timed) {
+  const Tiling& tl = P->tl;
+  for (int k = 0; k < iters; ++k) {
+    T* pold = (k & 1) ? v.pb : v.pa;
+    T* pnew = (k & 1) ? v.pa : v.pb;
+    {
+      TimerScope ts(timed ? P->ctx : nullptr, 0);
+      launch_dir<T, PC>(variant, st, s, v.r, pold, pnew, v.q, P->kst.p,
+                        v.partB, v.partA, dhist, hist_cap, tl, tl.G, Gd);
+    }
+    {
+      TimerScope ts(timed ? P->ctx : nullptr, 1);
+      hipLaunchKernelGGL((k_cg_upd<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st,
+                         s, v.x, pnew, v.r, v.q, P->kst.p, v.partA, v.partB,
+                         tl, norm_ref, Gd);
+    }
+  }
+}
+
+// The instantiated graph of one batch, cached per solve configuration.
+template <typename T, bool PC>
+hipGraphExec_t batch_graph(cpgpu_poisson* P, const Sten<T>& s,
+                           const CGVecs<T>& v, int variant, int Gd, int iters,
+                           int norm_ref, double* dhist, int hist_cap) {
+  GraphKey key{(const void*)v.x, (const void*)dhist, sizeof(T), PC ? 1 : 0,
+               variant, iters, norm_ref, hist_cap};
+  for (auto& g : P->graphs)
+    if (g.first == key) return g.second;
+  hipStream_t cs;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+    return nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) ==
+      hipSuccess) {
+    launch_batch<T, PC>(P, cs, s, v, variant, Gd, iters, norm_ref, dhist,
+                        hist_cap, false);
+    if (hipStreamEndCapture(cs, &graph) == hipSuccess && graph) {
+      if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+        exec = nullptr;
+      (void)hipGraphDestroy(graph);
+    }
+  }
+  (void)hipStreamDestroy(cs);
+  if (exec) P->graphs.push_back({key, exec});
+  return exec;
+}
+
+// Jacobi-PCG (PC) or CG on device vectors.  Iterations run in batches (a
+// replayed hipGraph, or direct launches while kernel timers are on); the
+// device state stops them exactly where the reference stops and launches
+// after the stop return at once.  The host polls the state of the previous
+// batch while the next one runs.
 template <typename T, bool PC>
 int run_cg(cpgpu_poisson* P, const Sten<T>& s, CGVecs<T> v, int max_iter,
            double tol, int norm_ref, double* dhist, int hist_cap,
@@ -343,48 +432,53 @@ int run_cg(cpgpu_poisson* P, const Sten<T>& s, CGVecs<T> v, int max_iter,
   KState init{};
   init.tol = tol;
   init.max_iter = max_iter;
-  std::memcpy(P->h_kst, &init, sizeof(KState));
-  CPGPU_HIP(ctx, hipMemcpyAsync(P->kst.p, P->h_kst, sizeof(KState),
+  init.it_dir = 1;
+  std::memcpy(&P->h_kst[0], &init, sizeof(KState));
+  CPGPU_HIP(ctx, hipMemcpyAsync(P->kst.p, &P->h_kst[0], sizeof(KState),
                                 hipMemcpyHostToDevice, st));
   const Tiling tl = P->tl;
-  const dim3 grid(tl.G), block(TJ, BY);
-  hipLaunchKernelGGL((k_cg_init<T, PC>), grid, block, 0, st, s, v.x, v.b, v.r,
-                     v.partB, tl, norm_ref);
+  const int variant = sizeof(T) == 8 ? P->dir_variant64 : P->dir_variant32;
+  const int Gd = dir_blocks<T, PC>(variant, P->nx, P->ny, tl);
+  hipLaunchKernelGGL((k_cg_init<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st, s,
+                     v.x, v.b, v.r, v.partB, tl, norm_ref);
   CPGPU_HIP(ctx, hipGetLastError());
-  int it = 1;  // next iteration to launch
-  int batch = 8;
-  bool final_test = false;
-  while (!final_test) {
-    // iterations it .. last; iteration max_iter+1 only runs the final test
-    const int last = std::min(it + batch - 1, max_iter + 1);
-    for (; it <= last; ++it) {
-      T* pold = (it & 1) ? v.pa : v.pb;
-      T* pnew = (it & 1) ? v.pb : v.pa;
-      {
-        TimerScope ts(ctx, 0);
-        hipLaunchKernelGGL((k_cg_dir<T, PC>), grid, block, 0, st, s, v.r, pold,
-                           pnew, v.q, P->kst.p, v.partB, v.partA, dhist,
-                           hist_cap, it, tl);
-      }
-      if (it > max_iter) {
-        final_test = true;
-        break;
-      }
-      {
-        TimerScope ts(ctx, 1);
-        hipLaunchKernelGGL((k_cg_upd<T, PC>), grid, block, 0, st, s, v.x, pnew,
-                           v.r, v.q, P->kst.p, v.partA, v.partB, it, tl,
-                           norm_ref);
-      }
+  const bool timed = ctx->timer_on[0] || ctx->timer_on[1];
+  // batch size: a handful of iterations for small iteration budgets, then
+  // P->batch (even) iterations per replay
+  const int B = P->batch;
+  hipGraphExec_t ge = nullptr;
+  if (P->use_graphs && !timed)
+    ge = batch_graph<T, PC>(P, s, v, variant, Gd, B, norm_ref, dhist, hist_cap);
+  // the kernels stop by themselves at max_iter + 1; one batch past it
+  // guarantees the final test ran
+  const long long total = (long long)max_iter + 2;
+  long long launched = 0;
+  int nb = 0;
+  for (;;) {
+    if (ge) {
+      CPGPU_HIP(ctx, hipGraphLaunch(ge, st));
+    } else {
+      launch_batch<T, PC>(P, st, s, v, variant, Gd, B, norm_ref, dhist,
+                          hist_cap, timed);
+      CPGPU_HIP(ctx, hipGetLastError());
     }
-    CPGPU_HIP(ctx, hipGetLastError());
-    CPGPU_HIP(ctx, hipMemcpyAsync(P->h_kst, P->kst.p, sizeof(KState),
+    launched += B;
+    KState* slot = &P->h_kst[1 + (nb & 1)];
+    CPGPU_HIP(ctx, hipMemcpyAsync(slot, P->kst.p, sizeof(KState),
                                   hipMemcpyDeviceToHost, st));
-    CPGPU_HIP(ctx, hipStreamSynchronize(st));
-    if (P->h_kst->done) break;
-    batch = std::min(batch * 2, 64);
+    CPGPU_HIP(ctx, hipEventRecord(P->ev_batch[nb & 1], st));
+    // check the batch before this one (its copy completed long ago)
+    if (nb >= 1) {
+      CPGPU_HIP(ctx, hipEventSynchronize(P->ev_batch[(nb - 1) & 1]));
+      if (P->h_kst[1 + ((nb - 1) & 1)].done) break;
+    }
+    ++nb;
+    if (launched >= total + B) break;
   }
-  *out = *P->h_kst;
+  CPGPU_HIP(ctx, hipStreamSynchronize(st));
+  CPGPU_HIP(ctx, hipMemcpy(&P->h_kst[0], P->kst.p, sizeof(KState),
+                           hipMemcpyDeviceToHost));
+  *out = P->h_kst[0];
   return CPGPU_OK;
 }
 
@@ -473,6 +567,101 @@ int run_mixed(cpgpu_poisson* P, int max_inner_total, double tol, int project,
   *iters = total_inner;
   *resid_out = resid;
   return code;
+}
+
+// Average duration (us) of the direction and update kernels of one variant:
+// one real iteration sets up a valid state, then `reps` launches of each
+// kernel are captured in a graph and replayed between two events, so the
+// figure is GPU time per launch without host launch gaps.  The kernels
+// overwrite the Krylov vectors (x included); callers save what they keep.
+template <typename T, bool PC>
+int time_kernels(cpgpu_poisson* P, const Sten<T>& s, const CGVecs<T>& v,
+                 int variant, int reps, float* us_dir, float* us_upd) {
+  cpgpu_ctx* ctx = P->ctx;
+  hipStream_t st = ctx->stream;
+  const Tiling tl = P->tl;
+  KState init{};
+  init.tol = 0.0;
+  init.max_iter = 1 << 30;
+  init.it_dir = 1;
+  std::memcpy(&P->h_kst[0], &init, sizeof(KState));
+  CPGPU_HIP(ctx, hipMemcpyAsync(P->kst.p, &P->h_kst[0], sizeof(KState),
+                                hipMemcpyHostToDevice, st));
+  const int Gd = dir_blocks<T, PC>(variant, P->nx, P->ny, tl);
+  hipLaunchKernelGGL((k_cg_init<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st, s,
+                     v.x, v.b, v.r, v.partB, tl, 0);
+  launch_dir<T, PC>(variant, st, s, v.r, v.pa, v.pb, v.q, P->kst.p, v.partB,
+                    v.partA, nullptr, 0, tl, tl.G, Gd);
+  hipLaunchKernelGGL((k_cg_upd<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st, s,
+                     v.x, v.pb, v.r, v.q, P->kst.p, v.partA, v.partB, tl, 0,
+                     Gd);
+  CPGPU_HIP(ctx, hipGetLastError());
+  CPGPU_HIP(ctx, hipStreamSynchronize(st));
+  float out[2] = {0.f, 0.f};
+  for (int which = 0; which < 2; ++which) {
+    hipStream_t cs;
+    CPGPU_HIP(ctx, hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    CPGPU_HIP(ctx, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < reps; ++k) {
+      if (which == 0)
+        launch_dir<T, PC>(variant, cs, s, v.r, v.pb, v.pa, v.q, P->kst.p,
+                          v.partB, v.partA, nullptr, 0, tl, tl.G, Gd);
+      else
+        hipLaunchKernelGGL((k_cg_upd<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, cs,
+                           s, v.x, v.pa, v.r, v.q, P->kst.p, v.partA, v.partB,
+                           tl, 0, Gd);
+    }
+    CPGPU_HIP(ctx, hipStreamEndCapture(cs, &graph));
+    CPGPU_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    CPGPU_HIP(ctx, hipGraphLaunch(exec, st));  // warm
+    hipEvent_t e0, e1;
+    CPGPU_HIP(ctx, hipEventCreate(&e0));
+    CPGPU_HIP(ctx, hipEventCreate(&e1));
+    CPGPU_HIP(ctx, hipEventRecord(e0, st));
+    CPGPU_HIP(ctx, hipGraphLaunch(exec, st));
+    CPGPU_HIP(ctx, hipEventRecord(e1, st));
+    CPGPU_HIP(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    CPGPU_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    out[which] = ms * 1e3f / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipStreamDestroy(cs);
+  }
+  *us_dir = out[0];
+  *us_upd = out[1];
+  return CPGPU_OK;
+}
+
+// Times every direction-kernel variant and returns the fastest.
+template <typename T, bool PC>
+int tune_dir(cpgpu_poisson* P, const Sten<T>& s, CGVecs<T> v, int reps,
+             int* best, float* best_ms) {
+  cpgpu_ctx* ctx = P->ctx;
+  hipStream_t st = ctx->stream;
+  const Tiling tl = P->tl;
+  float bt = 1e30f, up = 0.f;
+  int bv = 3;
+  for (int variant = 0; variant <= 4; ++variant) {
+    float ud = 0.f, uu = 0.f;
+    int rc = time_kernels<T, PC>(P, s, v, variant, reps, &ud, &uu);
+    if (rc) return rc;
+    if (ud < bt) {
+      bt = ud;
+      bv = variant;
+      up = uu;
+    }
+  }
+  (void)up;
+  (void)st;
+  (void)tl;
+  *best = bv;
+  *best_ms = bt / 1e3f;
+  return CPGPU_OK;
 }
 
 }  // namespace
@@ -597,7 +786,9 @@ int cpgpu_poisson_create(cpgpu_ctx* ctx, const cpgpu_grid2d* g,
   hipLaunchKernelGGL(k_fill<double>, dim3(1024), dim3(256), 0, ctx->stream,
                      P->b.p, 0.0, N);
   CPGPU_HIP(ctx, hipGetLastError());
-  CPGPU_HIP(ctx, hipHostMalloc((void**)&P->h_kst, sizeof(KState), 0));
+  CPGPU_HIP(ctx, hipHostMalloc((void**)&P->h_kst, 3 * sizeof(KState), 0));
+  for (auto& e : P->ev_batch)
+    CPGPU_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   CPGPU_HIP(ctx, hipHostMalloc((void**)&P->h_rst, sizeof(RState), 0));
   CPGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *out = P.release();
@@ -702,6 +893,91 @@ int cpgpu_poisson_set_mixed(cpgpu_poisson* P, double inner_tol, int max_outer) {
     return CPGPU_EINVAL;
   P->inner_tol = inner_tol;
   P->max_outer = max_outer;
+  return CPGPU_OK;
+}
+
+int cpgpu_poisson_set_variant(cpgpu_poisson* P, int v64, int v32) {
+  if (!P || v64 < 0 || v64 > 4 || v32 < 0 || v32 > 4) return CPGPU_EINVAL;
+  P->dir_variant64 = v64;
+  P->dir_variant32 = v32;
+  return CPGPU_OK;
+}
+
+int cpgpu_poisson_autotune(cpgpu_poisson* P, int reps, int* v64, int* v32,
+                           float* us64, float* us32) {
+  if (!P || reps < 1) return CPGPU_EINVAL;
+  cpgpu_ctx* ctx = P->ctx;
+  CPGPU_HIP(ctx, hipSetDevice(ctx->device));
+  // keep phi: the tuning runs Krylov kernels on the problem's vectors
+  DevBuf<double> save;
+  if (save.alloc(P->N)) return set_error(ctx, CPGPU_ENOMEM, "tune buffer");
+  CPGPU_HIP(ctx, hipMemcpyAsync(save.p, P->x.p, sizeof(double) * P->N,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  int b64 = 3, b32 = 3;
+  float t64 = 0.f, t32 = 0.f;
+  CGVecs<double> v{P->x.p, P->b.p, P->r.p, P->p0.p, P->p1.p, P->q.p,
+                   P->partA64.p, P->partB64.p};
+  int rc = tune_dir<double, true>(P, P->sten64(), v, reps, &b64, &t64);
+  if (rc) return rc;
+  if ((rc = alloc_f32(P))) return rc;
+  hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, ctx->stream, P->b.p,
+                     P->b32.p, P->x32.p, P->rst.p, P->N, 0);
+  CGVecs<float> w{P->x32.p, P->b32.p, P->r32.p, P->pa32.p, P->pb32.p,
+                  P->q32.p, P->partA32.p, P->partB32.p};
+  rc = tune_dir<float, true>(P, P->sten32(), w, reps, &b32, &t32);
+  if (rc) return rc;
+  CPGPU_HIP(ctx, hipMemcpyAsync(P->x.p, save.p, sizeof(double) * P->N,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  CPGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  P->dir_variant64 = b64;
+  P->dir_variant32 = b32;
+  if (v64) *v64 = b64;
+  if (v32) *v32 = b32;
+  if (us64) *us64 = t64 * 1e3f;
+  if (us32) *us32 = t32 * 1e3f;
+  return CPGPU_OK;
+}
+
+int cpgpu_poisson_kernel_times(cpgpu_poisson* P, int precision, int reps,
+                               double* us_dir, double* us_upd) {
+  if (!P || reps < 1 || !us_dir || !us_upd ||
+      (precision != 32 && precision != 64))
+    return CPGPU_EINVAL;
+  cpgpu_ctx* ctx = P->ctx;
+  CPGPU_HIP(ctx, hipSetDevice(ctx->device));
+  DevBuf<double> save;
+  if (save.alloc(P->N)) return set_error(ctx, CPGPU_ENOMEM, "timing buffer");
+  CPGPU_HIP(ctx, hipMemcpyAsync(save.p, P->x.p, sizeof(double) * P->N,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  float d = 0.f, u = 0.f;
+  int rc;
+  if (precision == 64) {
+    CGVecs<double> v{P->x.p, P->b.p, P->r.p, P->p0.p, P->p1.p, P->q.p,
+                     P->partA64.p, P->partB64.p};
+    rc = time_kernels<double, true>(P, P->sten64(), v, P->dir_variant64, reps,
+                                    &d, &u);
+  } else {
+    if ((rc = alloc_f32(P))) return rc;
+    hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, ctx->stream, P->b.p,
+                       P->b32.p, P->x32.p, P->rst.p, P->N, 0);
+    CGVecs<float> w{P->x32.p, P->b32.p, P->r32.p, P->pa32.p, P->pb32.p,
+                    P->q32.p, P->partA32.p, P->partB32.p};
+    rc = time_kernels<float, true>(P, P->sten32(), w, P->dir_variant32, reps,
+                                   &d, &u);
+  }
+  if (rc) return rc;
+  CPGPU_HIP(ctx, hipMemcpyAsync(P->x.p, save.p, sizeof(double) * P->N,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  CPGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *us_dir = d;
+  *us_upd = u;
+  return CPGPU_OK;
+}
+
+int cpgpu_poisson_set_batch(cpgpu_poisson* P, int iters, int use_graphs) {
+  if (!P || iters < 2 || (iters & 1)) return CPGPU_EINVAL;
+  P->batch = iters;
+  P->use_graphs = use_graphs != 0;
   return CPGPU_OK;
 }
 

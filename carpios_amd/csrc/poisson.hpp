@@ -48,6 +48,19 @@ struct RState {
   double bb, rr, rsum, xsum;
 };
 
+// Identity of a captured batch of Krylov iterations.
+struct GraphKey {
+  const void* x;
+  const void* hist;
+  size_t tsize;
+  int pc, variant, iters, norm_ref, hist_cap;
+  bool operator==(const GraphKey& o) const {
+    return x == o.x && hist == o.hist && tsize == o.tsize && pc == o.pc &&
+           variant == o.variant && iters == o.iters &&
+           norm_ref == o.norm_ref && hist_cap == o.hist_cap;
+  }
+};
+
 }  // namespace cpgpu
 
 struct cpgpu_poisson {
@@ -74,8 +87,17 @@ struct cpgpu_poisson {
   cpgpu::DevBuf<cpgpu::KState> kst;
   cpgpu::DevBuf<cpgpu::RState> rst;
   cpgpu::DevBuf<double> dhist;
-  cpgpu::KState* h_kst = nullptr;  // pinned mirrors
+  cpgpu::KState* h_kst = nullptr;  // pinned: [0] staging, [1..2] batch ring
   cpgpu::RState* h_rst = nullptr;
+  // batched Krylov iterations: replayed graphs and the polling events
+  bool use_graphs = true;
+  int batch = 16;  // iterations per batch (even)
+  std::vector<std::pair<cpgpu::GraphKey, hipGraphExec_t>> graphs;
+  hipEvent_t ev_batch[2] = {nullptr, nullptr};
+  // direction-kernel variant per precision (0 LDS tile, 1..4 column march
+  // with H = 4, 8, 16, 32); set by heuristic, cpgpu_poisson_set_variant or
+  // the autotuner
+  int dir_variant64 = 3, dir_variant32 = 3;
   double inner_tol = 1e-4;
   int max_outer = 40;
 
@@ -98,6 +120,9 @@ struct cpgpu_poisson {
     return s;
   }
   ~cpgpu_poisson() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto e : ev_batch)
+      if (e) (void)hipEventDestroy(e);
     if (h_kst) (void)hipHostFree(h_kst);
     if (h_rst) (void)hipHostFree(h_rst);
   }

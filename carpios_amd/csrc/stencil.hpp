@@ -93,6 +93,12 @@ struct KState {
   double rho[2];
   int iter, max_iter;
   int done;  // 0 running, 1 converged, 2 stopped at max_iter
+  // iteration index handed from kernel to kernel: the direction kernel of
+  // iteration it reads it_dir and publishes it_upd = it; the update kernel
+  // reads it_upd and publishes it_dir = it + 1.  Each field is written only
+  // by the kernel that does not read it, so a captured graph of iterations
+  // replays for any iteration number.
+  int it_dir, it_upd;
   int pad;
 };
 
@@ -161,12 +167,13 @@ __global__ __launch_bounds__(256) void k_cg_dir(
     Sten<T> s, const T* __restrict__ r, const T* __restrict__ pold,
     T* __restrict__ pnew, T* __restrict__ q, KState* st,
     const T* __restrict__ partB, T* __restrict__ partA, double* hist,
-    int hist_cap, int it, Tiling tl) {
+    int hist_cap, Tiling tl, int Gprev) {
   if (st->done) return;
+  const int it = st->it_dir;
   __shared__ T lds[16];
   __shared__ T tile[TI + 2][TJ + 2];
   T tot[3];
-  reduce_partials<T, 3>(partB, gridDim.x, tot, lds);
+  reduce_partials<T, 3>(partB, Gprev, tot, lds);
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0;
   double normb;
   if (it == 1) {
@@ -189,7 +196,10 @@ __global__ __launch_bounds__(256) void k_cg_dir(
   const T rho = tot[SLOT_RZ];
   const bool first = it == 1;
   const T beta = first ? T(0) : rho / (T)st->rho[(it - 1) & 1];
-  if (lead) st->rho[it & 1] = (double)rho;
+  if (lead) {
+    st->rho[it & 1] = (double)rho;
+    st->it_upd = it;
+  }
 
   const int nx = s.nx, ny = s.ny;
   const int tid = threadIdx.x + TJ * threadIdx.y;
@@ -231,17 +241,149 @@ __global__ __launch_bounds__(256) void k_cg_dir(
   store_partials<T, 1>(v, partA, gridDim.x, blockIdx.x, lds);
 }
 
+// ---------------------------------------------------------------------
+// Column-march variant of k_cg_dir.  A wavefront owns a strip of 62 output
+// columns [c0, c0+62) -- lanes 0 and 63 load the two halo columns -- and a
+// chunk of H rows [r0, r0+H).  It issues every load of rows r0-1 .. r0+H
+// up front (registers), forms p_new for all of them, then walks the rows
+// taking the j neighbours from adjacent lanes.  No LDS and no barriers on
+// the streaming path; loads per output are (H+2)/H x 64/62 of the
+// algorithmic r and p_old, the excess mostly served by L2.
+// Work items (strip, chunk) are numbered chunk-fastest so the four waves of
+// a block take four consecutive chunks of one strip and share halo rows in
+// the CU's caches.
+struct March {
+  int nstrip, nchunk, nunits;  // units = groups of 4 work items
+};
+
+inline March make_march(int nx, int ny, int H) {
+  March m;
+  m.nstrip = (ny + 61) / 62;  // 62 output columns per wavefront
+  m.nchunk = (nx + H - 1) / H;
+  const int items = m.nstrip * m.nchunk;
+  m.nunits = (items + 3) / 4;
+  return m;
+}
+
+template <typename T, bool PC>
+__device__ __forceinline__ T pnew_at(const Sten<T>& s, const T* __restrict__ r,
+                                     const T* __restrict__ pold, int i, int j,
+                                     T beta, bool first) {
+  if (i < 0 || i >= s.nx || j < 0 || j >= s.ny) return T(0);
+  const size_t k = (size_t)i * s.ny + j;
+  const T rv = r[k];
+  const T z = PC ? rv * (T(1) / diag_at(s, i, j)) : rv;
+  return first ? z : z + beta * pold[k];
+}
+
+template <typename T, bool PC, int H>
+__global__ __launch_bounds__(256) void k_cg_dir_march(
+    Sten<T> s, const T* __restrict__ r, const T* __restrict__ pold,
+    T* __restrict__ pnew, T* __restrict__ q, KState* st,
+    const T* __restrict__ partB, T* __restrict__ partA, double* hist,
+    int hist_cap, March mr, int Gprev) {
+  if (st->done) return;
+  const int it = st->it_dir;
+  __shared__ T lds[16];
+  T tot[3];
+  reduce_partials<T, 3>(partB, Gprev, tot, lds);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+  double normb;
+  if (it == 1) {
+    normb = std::sqrt((double)tot[SLOT_BB]);
+    if (normb == 0.0) normb = 1;
+    if (lead) st->normb = normb;
+  } else {
+    normb = st->normb;
+  }
+  const double resid = std::sqrt((double)tot[SLOT_RR]) / normb;
+  if (lead && it >= 2 && it - 2 < hist_cap && hist) hist[it - 2] = resid;
+  if (resid <= st->tol || it > st->max_iter) {
+    if (lead) {
+      st->iter = it - 1;
+      st->resid = resid;
+      st->done = resid <= st->tol ? 1 : 2;
+    }
+    return;
+  }
+  const T rho = tot[SLOT_RZ];
+  const bool first = it == 1;
+  const T beta = first ? T(0) : rho / (T)st->rho[(it - 1) & 1];
+  if (lead) {
+    st->rho[it & 1] = (double)rho;
+    st->it_upd = it;
+  }
+
+  const int nx = s.nx, ny = s.ny;
+  const int lane = threadIdx.x;  // blockDim = (64, 4)
+  T acc_pq = T(0);
+  for (int u = blockIdx.x; u < mr.nunits; u += gridDim.x) {
+    const int item = u * 4 + threadIdx.y;
+    const int strip = item / mr.nchunk;
+    if (strip >= mr.nstrip) continue;  // tail of the last unit
+    const int chunk = item - strip * mr.nchunk;
+    const int r0 = chunk * H, c0 = strip * 62;
+    // lanes 0 and 63 carry the halo columns c0-1 and c0+62
+    const int j = c0 - 1 + lane;
+    const bool jin = j >= 0 && j < ny;
+    const bool out = lane >= 1 && lane <= 62 && j < ny;
+    // every load of the chunk first (rows r0-1 .. r0+H), then the math
+    T pn[H + 2], pv[H + 2];
+#pragma unroll
+    for (int h = 0; h < H + 2; ++h) {
+      const int i = r0 - 1 + h;
+      const bool ok = jin && i >= 0 && i < nx;
+      const size_t k = ok ? (size_t)i * ny + j : 0;
+      pn[h] = ok ? r[k] : T(0);
+      pv[h] = (ok && !first) ? pold[k] : T(0);
+    }
+#pragma unroll
+    for (int h = 0; h < H + 2; ++h) {
+      const int i = r0 - 1 + h;
+      if (jin && i >= 0 && i < nx) {
+        const T z = PC ? pn[h] * (T(1) / diag_at(s, i, j)) : pn[h];
+        pn[h] = first ? z : z + beta * pv[h];
+      }
+    }
+    const T kyl = out && j > 0 ? -s.KY[j] : T(0);
+    const T kyr = out && j < ny - 1 ? -s.KY[j + 1] : T(0);
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const int i = r0 + h;
+      const T pc = pn[h + 1];
+      const T left = __shfl_up(pc, 1, 64);
+      const T right = __shfl_down(pc, 1, 64);
+      if (out && i < nx) {
+        const T dg = diag_at(s, i, j);
+        T acc = T(0);
+        if (i > 0) acc = acc + pn[h] * (-s.KX[i]);
+        if (j > 0) acc = acc + left * kyl;
+        acc = acc + pc * dg;
+        if (j < ny - 1) acc = acc + right * kyr;
+        if (i < nx - 1) acc = acc + pn[h + 2] * (-s.KX[i + 1]);
+        const size_t k = (size_t)i * ny + j;
+        pnew[k] = pc;
+        q[k] = acc;
+        acc_pq += pc * acc;
+      }
+    }
+  }
+  T v[1] = {acc_pq};
+  store_partials<T, 1>(v, partA, gridDim.x, blockIdx.x, lds);
+}
+
 // Iteration `it`, second kernel: alpha = rho_it / p.q; x += alpha p;
 // r -= alpha q; partials of r.z and r.r for the next prologue.
 template <typename T, bool PC>
 __global__ __launch_bounds__(256) void k_cg_upd(
     Sten<T> s, T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r,
-    const T* __restrict__ q, const KState* st, const T* __restrict__ partA,
-    T* __restrict__ partB, int it, Tiling tl, int norm_ref) {
+    const T* __restrict__ q, KState* st, const T* __restrict__ partA,
+    T* __restrict__ partB, Tiling tl, int norm_ref, int Gprev) {
   if (st->done) return;
+  const int it = st->it_upd;
   __shared__ T lds[16];
   T pq[1];
-  reduce_partials<T, 1>(partA, gridDim.x, pq, lds);
+  reduce_partials<T, 1>(partA, Gprev, pq, lds);
   const T alpha = (T)st->rho[it & 1] / pq[0];
   const size_t nlast = (size_t)s.nx * s.ny - 1;
   T v[3] = {T(0), T(0), T(0)};
@@ -263,6 +405,8 @@ __global__ __launch_bounds__(256) void k_cg_upd(
     }
   }
   store_partials<T, 3>(v, partB, gridDim.x, blockIdx.x, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0)
+    st->it_dir = it + 1;
 }
 
 }  // namespace cpgpu

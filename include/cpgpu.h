@@ -132,6 +132,24 @@ int cpgpu_poisson_solve(cpgpu_poisson *P, int method, int *max_iter,
  * maximum number of refinement sweeps. */
 int cpgpu_poisson_set_mixed(cpgpu_poisson *P, double inner_tol, int max_outer);
 
+/* Direction-kernel variant of the Krylov iteration per precision: 0 LDS
+ * tile, 1..4 column march with 4, 8, 16, 32 rows per wavefront. */
+int cpgpu_poisson_set_variant(cpgpu_poisson *P, int v64, int v32);
+/* Autotuner: times reps launches of every variant on this problem (FP64 and
+ * FP32), keeps the fastest and reports it with its time per launch (us).
+ * phi is preserved. */
+int cpgpu_poisson_autotune(cpgpu_poisson *P, int reps, int *v64, int *v32,
+                           float *us64, float *us32);
+
+/* GPU time per launch (us) of the current direction and update kernels of
+ * the Krylov iteration (precision 32 or 64): reps launches of each are
+ * captured in a hipGraph and replayed between two events.  phi is kept. */
+int cpgpu_poisson_kernel_times(cpgpu_poisson *P, int precision, int reps,
+                               double *us_dir, double *us_upd);
+/* Krylov iterations per launched batch (even, >= 2) and whether batches
+ * are replayed hipGraphs (default 16, graphs on). */
+int cpgpu_poisson_set_batch(cpgpu_poisson *P, int iters, int use_graphs);
+
 /* y = A x on interior host fields (tests of the operator). */
 int cpgpu_poisson_apply(cpgpu_poisson *P, const double *x_field,
                         double *y_field);

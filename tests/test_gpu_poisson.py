@@ -188,3 +188,35 @@ def test_one_shot_solve_matches_problem_object(ctx):
     assert code == 0
     x_o = O.solve("JPCG", A, A.b, tol=1e-10)[1]
     assert rel_l2(O.field_to_x(og, phi), x_o) <= 1e-12
+
+
+@pytest.mark.parametrize("nx,ny", [(37, 53), (130, 200), (256, 64)])
+def test_direction_kernel_variants_bitwise_equal(ctx, nx, ny):
+    og, A, src, desc = both(nx, ny, 1, [1, 2, 1, 2], [0.3, 0, 0, -0.2], seed=3)
+    sols = []
+    for variant in range(5):
+        P = ca.Poisson(ctx, desc)
+        P.set_variant(variant, variant)
+        P.set_source(src)
+        code, it, tol, _ = P.solve("JPCG", tol=1e-11)
+        sols.append((it, P.get_phi()))
+        P.close()
+    # the operator is the same arithmetic; the block partition of the dot
+    # products differs between variants, so results agree to rounding
+    x0 = O.field_to_x(og, sols[0][1])
+    for it, phi in sols[1:]:
+        assert abs(it - sols[0][0]) <= 1
+        assert rel_l2(O.field_to_x(og, phi), x0) <= 1e-12
+
+
+def test_autotune_keeps_phi_and_picks_a_variant(ctx):
+    og, A, src, desc = c1(256)
+    P = ca.Poisson(ctx, desc)
+    P.set_source(src)
+    phi0 = og.field()
+    og.interior(phi0)[:] = 0.125
+    P.set_phi(phi0)
+    res = P.autotune(reps=3)
+    assert 0 <= res["fp64"][0] <= 4 and 0 <= res["fp32"][0] <= 4
+    assert np.array_equal(og.interior(P.get_phi()), og.interior(phi0))
+    P.close()
