@@ -22,8 +22,9 @@ CG, JPCG, BICGSTAB, JACOBI, MIXED = 0, 1, 2, 3, 4
 NORM_REF, NORM_TRUE = 0, 1
 OPT_PROJECT_MEAN = 1
 
+MIXED_RESTART = 5
 METHODS = {"CG": CG, "JPCG": JPCG, "Dia_BiCGSTAB": BICGSTAB,
-           "Jacobi": JACOBI, "MIXED": MIXED}
+           "Jacobi": JACOBI, "MIXED": MIXED, "MIXED_RESTART": MIXED_RESTART}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

@@ -345,17 +345,19 @@ template <typename T, bool PC>
 void launch_dir(int variant, hipStream_t st, const Sten<T>& s, const T* r,
                 const T* pold, T* pnew, T* q, KState* kst, const T* partB,
                 T* partA, double* hist, int hist_cap, const Tiling& tl,
-                int Gprev, int G) {
+                int Gprev, int G, const double* partR64 = nullptr,
+                int Gref = 0) {
   const dim3 block(TJ, BY);
   if (variant == 0) {
     hipLaunchKernelGGL((k_cg_dir<T, PC>), dim3(G), block, 0, st, s, r, pold,
-                       pnew, q, kst, partB, partA, hist, hist_cap, tl, Gprev);
+                       pnew, q, kst, partB, partA, hist, hist_cap, tl, Gprev,
+                       partR64, Gref);
     return;
   }
 #define CPGPU_DIR_MARCH(HV)                                                   \
   hipLaunchKernelGGL((k_cg_dir_march<T, PC, HV>), dim3(G), block, 0, st, s, r, \
                      pold, pnew, q, kst, partB, partA, hist, hist_cap,        \
-                     make_march(s.nx, s.ny, HV), Gprev)
+                     make_march(s.nx, s.ny, HV), Gprev, partR64, Gref)
   switch (variant) {
     case 1: CPGPU_DIR_MARCH(4); break;
     case 2: CPGPU_DIR_MARCH(8); break;
@@ -366,19 +368,23 @@ void launch_dir(int variant, hipStream_t st, const Sten<T>& s, const T* r,
 }
 
 // One batch of `iters` CG iterations (iters even: the p buffers alternate
-// and every batch starts at an odd iteration).
+// and every batch starts at an odd iteration).  reliable: FP32 iterations
+// closed by the FP64 residual replacement (x32 folded into P->x).
 template <typename T, bool PC>
 void launch_batch(cpgpu_poisson* P, hipStream_t st, const Sten<T>& s,
                   const CGVecs<T>& v, int variant, int Gd, int iters,
-                  int norm_ref, double* dhist, int hist_cap, bool timed) {
+                  int norm_ref, double* dhist, int hist_cap, bool timed,
+                  bool reliable = false) {
   const Tiling& tl = P->tl;
+  const double* pr = reliable ? P->partR64.p : nullptr;
   for (int k = 0; k < iters; ++k) {
     T* pold = (k & 1) ? v.pb : v.pa;
     T* pnew = (k & 1) ? v.pa : v.pb;
     {
       TimerScope ts(timed ? P->ctx : nullptr, 0);
       launch_dir<T, PC>(variant, st, s, v.r, pold, pnew, v.q, P->kst.p,
-                        v.partB, v.partA, dhist, hist_cap, tl, tl.G, Gd);
+                        v.partB, v.partA, dhist, hist_cap, tl, tl.G, Gd, pr,
+                        tl.G);
     }
     {
       TimerScope ts(timed ? P->ctx : nullptr, 1);
@@ -387,15 +393,25 @@ void launch_batch(cpgpu_poisson* P, hipStream_t st, const Sten<T>& s,
                          tl, norm_ref, Gd);
     }
   }
+  if constexpr (sizeof(T) == 4) {
+    if (reliable) {
+      hipLaunchKernelGGL(k_merge_x, dim3(1024), dim3(256), 0, st, P->x.p, v.x,
+                         P->N);
+      hipLaunchKernelGGL((k_refresh<PC>), dim3(tl.G), dim3(TJ, BY), 0, st,
+                         P->sten64(), P->sten32(), P->x.p, P->b.p, v.r,
+                         v.partB, P->partR64.p, tl);
+    }
+  }
 }
 
 // The instantiated graph of one batch, cached per solve configuration.
 template <typename T, bool PC>
 hipGraphExec_t batch_graph(cpgpu_poisson* P, const Sten<T>& s,
                            const CGVecs<T>& v, int variant, int Gd, int iters,
-                           int norm_ref, double* dhist, int hist_cap) {
+                           int norm_ref, double* dhist, int hist_cap,
+                           bool reliable) {
   GraphKey key{(const void*)v.x, (const void*)dhist, sizeof(T), PC ? 1 : 0,
-               variant, iters, norm_ref, hist_cap};
+               variant, iters, norm_ref, hist_cap, reliable ? 1 : 0};
   for (auto& g : P->graphs)
     if (g.first == key) return g.second;
   hipStream_t cs;
@@ -406,7 +422,7 @@ hipGraphExec_t batch_graph(cpgpu_poisson* P, const Sten<T>& s,
   if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) ==
       hipSuccess) {
     launch_batch<T, PC>(P, cs, s, v, variant, Gd, iters, norm_ref, dhist,
-                        hist_cap, false);
+                        hist_cap, false, reliable);
     if (hipStreamEndCapture(cs, &graph) == hipSuccess && graph) {
       if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
         exec = nullptr;
@@ -426,13 +442,16 @@ hipGraphExec_t batch_graph(cpgpu_poisson* P, const Sten<T>& s,
 template <typename T, bool PC>
 int run_cg(cpgpu_poisson* P, const Sten<T>& s, CGVecs<T> v, int max_iter,
            double tol, int norm_ref, double* dhist, int hist_cap,
-           KState* out) {
+           KState* out, bool reliable = false, double normb64 = 1.0) {
   cpgpu_ctx* ctx = P->ctx;
   hipStream_t st = ctx->stream;
   KState init{};
   init.tol = tol;
   init.max_iter = max_iter;
   init.it_dir = 1;
+  init.mode = reliable ? 1 : 0;
+  init.refresh_every = P->batch;
+  init.normb = normb64;
   std::memcpy(&P->h_kst[0], &init, sizeof(KState));
   CPGPU_HIP(ctx, hipMemcpyAsync(P->kst.p, &P->h_kst[0], sizeof(KState),
                                 hipMemcpyHostToDevice, st));
@@ -448,7 +467,8 @@ int run_cg(cpgpu_poisson* P, const Sten<T>& s, CGVecs<T> v, int max_iter,
   const int B = P->batch;
   hipGraphExec_t ge = nullptr;
   if (P->use_graphs && !timed)
-    ge = batch_graph<T, PC>(P, s, v, variant, Gd, B, norm_ref, dhist, hist_cap);
+    ge = batch_graph<T, PC>(P, s, v, variant, Gd, B, norm_ref, dhist, hist_cap,
+                            reliable);
   // the kernels stop by themselves at max_iter + 1; one batch past it
   // guarantees the final test ran
   const long long total = (long long)max_iter + 2;
@@ -459,7 +479,7 @@ int run_cg(cpgpu_poisson* P, const Sten<T>& s, CGVecs<T> v, int max_iter,
       CPGPU_HIP(ctx, hipGraphLaunch(ge, st));
     } else {
       launch_batch<T, PC>(P, st, s, v, variant, Gd, B, norm_ref, dhist,
-                          hist_cap, timed);
+                          hist_cap, timed, reliable);
       CPGPU_HIP(ctx, hipGetLastError());
     }
     launched += B;
@@ -487,6 +507,7 @@ int alloc_f32(cpgpu_poisson* P) {
   if (P->b32.alloc(N) || P->x32.alloc(N) || P->r32.alloc(N) ||
       P->pa32.alloc(N) || P->pb32.alloc(N) || P->q32.alloc(N) ||
       P->partA32.alloc(kMaxBlocks) || P->partB32.alloc(3 * kMaxBlocks) ||
+      P->partR64.alloc(kMaxBlocks) ||
       P->fKX.alloc(P->nx + 1) || P->fKY.alloc(P->ny + 1))
     return set_error(P->ctx, CPGPU_ENOMEM, "device allocation failed");
   std::vector<float> kx(P->KX.begin(), P->KX.end()), ky(P->KY.begin(), P->KY.end());
@@ -567,6 +588,61 @@ int run_mixed(cpgpu_poisson* P, int max_inner_total, double tol, int project,
   *iters = total_inner;
   *resid_out = resid;
   return code;
+}
+
+// FP32 Jacobi-PCG with FP64 residual replacement (no reference
+// counterpart): one uninterrupted FP32 Krylov iteration whose solution
+// increments are folded into the FP64 solution and whose residual is
+// replaced by the true FP64 residual every P->batch iterations; the stopping
+// test is on the true FP64 relative residual.  hist receives the FP64
+// residual at every replacement.
+int run_reliable(cpgpu_poisson* P, int max_iter, double tol, int project,
+                 double* hist, int hist_cap, int* hist_len, int* iters,
+                 double* resid_out) {
+  cpgpu_ctx* ctx = P->ctx;
+  hipStream_t st = ctx->stream;
+  int rc = alloc_f32(P);
+  if (rc) return rc;
+  if ((rc = residual64(P, true))) return rc;
+  double normb = std::sqrt(P->h_rst->bb);
+  if (normb == 0.0) normb = 1.0;
+  const double resid0 = std::sqrt(P->h_rst->rr) / normb;
+  *iters = 0;
+  *hist_len = 0;
+  *resid_out = resid0;
+  if (resid0 <= tol) return CPGPU_OK;
+  // r32 = (float)r64 as the right-hand side of a zero-start FP32 solve
+  hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, st, P->r.p, P->b32.p,
+                     P->x32.p, P->rst.p, P->N, project);
+  if (hist && hist_cap > 0 && P->dhist.alloc(hist_cap))
+    return set_error(ctx, CPGPU_ENOMEM, "hist alloc");
+  double* dh = hist && hist_cap > 0 ? P->dhist.p : nullptr;
+  CGVecs<float> v{P->x32.p, P->b32.p, P->r32.p, P->pa32.p, P->pb32.p,
+                  P->q32.p, P->partA32.p, P->partB32.p};
+  KState ks;
+  rc = run_cg<float, true>(P, P->sten32(), v, max_iter, tol, 0, dh, hist_cap,
+                           &ks, true, normb);
+  if (rc) return rc;
+  // increments since the last replacement (none when it converged there)
+  hipLaunchKernelGGL(k_merge_x, dim3(1024), dim3(256), 0, st, P->x.p,
+                     P->x32.p, P->N);
+  if (project) {
+    hipLaunchKernelGGL(k_add_f32, dim3(1024), dim3(256), 0, st, P->x.p,
+                       P->x32.p, P->part64.p, P->N);  // x32 is zero: sums x
+    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kThreads), 0, st,
+                       P->part64.p, 1024, &P->rst.p->xsum);
+    hipLaunchKernelGGL(k_sub_mean, dim3(1024), dim3(256), 0, st, P->x.p,
+                       &P->rst.p->xsum, P->N);
+  }
+  CPGPU_HIP(ctx, hipGetLastError());
+  const int nh = (ks.iter + P->batch - 1) / P->batch;
+  if (dh && nh > 0)
+    CPGPU_HIP(ctx, hipMemcpy(hist, dh, sizeof(double) * std::min(nh, hist_cap),
+                             hipMemcpyDeviceToHost));
+  *hist_len = nh;
+  *iters = ks.iter;
+  *resid_out = ks.resid;
+  return ks.done == 1 ? CPGPU_OK : CPGPU_MAXITER;
 }
 
 // Average duration (us) of the direction and update kernels of one variant:
@@ -713,11 +789,15 @@ int poisson_solve_internal(cpgpu_poisson* P, int method, int* max_iter,
     }
     return CPGPU_MAXITER;
   }
-  if (method == CPGPU_MIXED) {
+  if (method == CPGPU_MIXED || method == CPGPU_MIXED_RESTART) {
     int iters = 0;
     double resid = 0;
-    int code = run_mixed(P, *max_iter, *tol, opts & CPGPU_OPT_PROJECT_MEAN,
-                         hist, hist_cap, &hl, &iters, &resid);
+    const int proj = opts & CPGPU_OPT_PROJECT_MEAN;
+    int code = method == CPGPU_MIXED
+                   ? run_reliable(P, *max_iter, *tol, proj, hist, hist_cap,
+                                  &hl, &iters, &resid)
+                   : run_mixed(P, *max_iter, *tol, proj, hist, hist_cap, &hl,
+                               &iters, &resid);
     if (code < 0) return code;
     if (hist_len) *hist_len = hl;
     *max_iter = iters;

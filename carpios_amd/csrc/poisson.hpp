@@ -53,11 +53,12 @@ struct GraphKey {
   const void* x;
   const void* hist;
   size_t tsize;
-  int pc, variant, iters, norm_ref, hist_cap;
+  int pc, variant, iters, norm_ref, hist_cap, reliable;
   bool operator==(const GraphKey& o) const {
     return x == o.x && hist == o.hist && tsize == o.tsize && pc == o.pc &&
            variant == o.variant && iters == o.iters &&
-           norm_ref == o.norm_ref && hist_cap == o.hist_cap;
+           norm_ref == o.norm_ref && hist_cap == o.hist_cap &&
+           reliable == o.reliable;
   }
 };
 
@@ -82,7 +83,7 @@ struct cpgpu_poisson {
   // FP32 vectors of the inner solve
   cpgpu::DevBuf<float> b32, x32, r32, pa32, pb32, q32;
   // reductions and states
-  cpgpu::DevBuf<double> partA64, partB64, part64;
+  cpgpu::DevBuf<double> partA64, partB64, part64, partR64;
   cpgpu::DevBuf<float> partA32, partB32;
   cpgpu::DevBuf<cpgpu::KState> kst;
   cpgpu::DevBuf<cpgpu::RState> rst;

@@ -99,7 +99,8 @@ struct KState {
   // by the kernel that does not read it, so a captured graph of iterations
   // replays for any iteration number.
   int it_dir, it_upd;
-  int pad;
+  int mode;           // 0 reference test each iteration, 1 FP64 replacement
+  int refresh_every;  // mode 1: iterations between residual replacements
 };
 
 // Partial slots of the Krylov reductions (NV = 3 for every producer).
@@ -156,6 +157,70 @@ __global__ __launch_bounds__(256) void k_cg_init(Sten<T> s, const T* __restrict_
   store_partials<T, 3>(v, partB, gridDim.x, blockIdx.x, lds);
 }
 
+// Prologue of the direction kernel of iteration it = st->it_dir: totals of
+// the previous kernel's partials, the stopping test of iteration it-1, and
+// beta.  Returns false when the iteration must not run (every block reaches
+// the same decision from the same totals).
+//   mode 0: the reference's test on every iteration (solver_matrix.hpp:
+//           356-372, resid <= tol), normb from b.b at it == 1.
+//   mode 1: FP32 iterations with FP64 residual replacement every
+//           refresh_every iterations; the test uses the FP64 residual norm
+//           (partR64, Gref partials) right after a replacement, normb is the
+//           FP64 ||b|| set by the host.
+template <typename T>
+__device__ __forceinline__ bool dir_prologue(KState* st,
+                                             const T* __restrict__ partB,
+                                             int Gprev,
+                                             const double* __restrict__ partR64,
+                                             int Gref, double* hist,
+                                             int hist_cap, T* lds, T* beta,
+                                             bool* first) {
+  const int it = st->it_dir;
+  T tot[3];
+  reduce_partials<T, 3>(partB, Gprev, tot, lds);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+  bool stop = false, conv = false;
+  double resid = 0.0;
+  if (st->mode == 0) {
+    double normb;
+    if (it == 1) {
+      normb = std::sqrt((double)tot[SLOT_BB]);
+      if (normb == 0.0) normb = 1;
+      if (lead) st->normb = normb;
+    } else {
+      normb = st->normb;
+    }
+    resid = std::sqrt((double)tot[SLOT_RR]) / normb;
+    if (lead && it >= 2 && it - 2 < hist_cap && hist) hist[it - 2] = resid;
+    conv = resid <= st->tol;
+  } else if (it > 1 && (it - 1) % st->refresh_every == 0) {
+    __shared__ double lds64[4];
+    double rr[1];
+    reduce_partials<double, 1>(partR64, Gref, rr, lds64);
+    resid = std::sqrt(rr[0]) / st->normb;
+    const int k = (it - 1) / st->refresh_every - 1;
+    if (lead && k < hist_cap && hist) hist[k] = resid;
+    conv = resid <= st->tol;
+  }
+  stop = conv || it > st->max_iter;
+  if (stop) {
+    if (lead) {
+      st->iter = it - 1;
+      st->resid = resid;
+      st->done = conv ? 1 : 2;
+    }
+    return false;
+  }
+  const T rho = tot[SLOT_RZ];
+  *first = it == 1;
+  *beta = *first ? T(0) : rho / (T)st->rho[(it - 1) & 1];
+  if (lead) {
+    st->rho[it & 1] = (double)rho;
+    st->it_upd = it;
+  }
+  return true;
+}
+
 // Iteration `it` (1-based), first kernel.  Prologue: totals of the previous
 // kernel's partials -> residual of iteration it-1, stopping test
 // (solver_matrix.hpp:356-372: resid <= tol), rho_it = r.z.  Body: p = z +
@@ -167,39 +232,16 @@ __global__ __launch_bounds__(256) void k_cg_dir(
     Sten<T> s, const T* __restrict__ r, const T* __restrict__ pold,
     T* __restrict__ pnew, T* __restrict__ q, KState* st,
     const T* __restrict__ partB, T* __restrict__ partA, double* hist,
-    int hist_cap, Tiling tl, int Gprev) {
+    int hist_cap, Tiling tl, int Gprev, const double* __restrict__ partR64,
+    int Gref) {
   if (st->done) return;
-  const int it = st->it_dir;
   __shared__ T lds[16];
   __shared__ T tile[TI + 2][TJ + 2];
-  T tot[3];
-  reduce_partials<T, 3>(partB, Gprev, tot, lds);
-  const bool lead = blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0;
-  double normb;
-  if (it == 1) {
-    normb = std::sqrt((double)tot[SLOT_BB]);
-    if (normb == 0.0) normb = 1;
-    if (lead) st->normb = normb;
-  } else {
-    normb = st->normb;
-  }
-  const double resid = std::sqrt((double)tot[SLOT_RR]) / normb;
-  if (lead && it >= 2 && it - 2 < hist_cap && hist) hist[it - 2] = resid;
-  if (resid <= st->tol || it > st->max_iter) {
-    if (lead) {
-      st->iter = it - 1;
-      st->resid = resid;
-      st->done = resid <= st->tol ? 1 : 2;
-    }
+  T beta;
+  bool first;
+  if (!dir_prologue<T>(st, partB, Gprev, partR64, Gref, hist, hist_cap, lds,
+                       &beta, &first))
     return;
-  }
-  const T rho = tot[SLOT_RZ];
-  const bool first = it == 1;
-  const T beta = first ? T(0) : rho / (T)st->rho[(it - 1) & 1];
-  if (lead) {
-    st->rho[it & 1] = (double)rho;
-    st->it_upd = it;
-  }
 
   const int nx = s.nx, ny = s.ny;
   const int tid = threadIdx.x + TJ * threadIdx.y;
@@ -281,43 +323,42 @@ __global__ __launch_bounds__(256) void k_cg_dir_march(
     Sten<T> s, const T* __restrict__ r, const T* __restrict__ pold,
     T* __restrict__ pnew, T* __restrict__ q, KState* st,
     const T* __restrict__ partB, T* __restrict__ partA, double* hist,
-    int hist_cap, March mr, int Gprev) {
+    int hist_cap, March mr, int Gprev, const double* __restrict__ partR64,
+    int Gref) {
   if (st->done) return;
-  const int it = st->it_dir;
   __shared__ T lds[16];
-  T tot[3];
-  reduce_partials<T, 3>(partB, Gprev, tot, lds);
-  const bool lead = blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0;
-  double normb;
-  if (it == 1) {
-    normb = std::sqrt((double)tot[SLOT_BB]);
-    if (normb == 0.0) normb = 1;
-    if (lead) st->normb = normb;
-  } else {
-    normb = st->normb;
-  }
-  const double resid = std::sqrt((double)tot[SLOT_RR]) / normb;
-  if (lead && it >= 2 && it - 2 < hist_cap && hist) hist[it - 2] = resid;
-  if (resid <= st->tol || it > st->max_iter) {
-    if (lead) {
-      st->iter = it - 1;
-      st->resid = resid;
-      st->done = resid <= st->tol ? 1 : 2;
-    }
-    return;
-  }
-  const T rho = tot[SLOT_RZ];
-  const bool first = it == 1;
-  const T beta = first ? T(0) : rho / (T)st->rho[(it - 1) & 1];
-  if (lead) {
-    st->rho[it & 1] = (double)rho;
-    st->it_upd = it;
-  }
-
   const int nx = s.nx, ny = s.ny;
   const int lane = threadIdx.x;  // blockDim = (64, 4)
+  const bool first0 = st->it_dir == 1;
+  // Issue the loads of the block's first unit before the prologue so the
+  // memory latency overlaps the reduction of the partials.
+  T pn[H + 2], pv[H + 2];
+  auto load_unit = [&](int u) {
+    const int item = u * 4 + threadIdx.y;
+    const int strip = item / mr.nchunk;
+    const int chunk = item - strip * mr.nchunk;
+    const int r0 = chunk * H, c0 = strip * 62;
+    const int j = c0 - 1 + lane;
+    const bool jin = strip < mr.nstrip && j >= 0 && j < ny;
+#pragma unroll
+    for (int h = 0; h < H + 2; ++h) {
+      const int i = r0 - 1 + h;
+      const bool ok = jin && i >= 0 && i < nx;
+      const size_t k = ok ? (size_t)i * ny + j : 0;
+      pn[h] = ok ? r[k] : T(0);
+      pv[h] = (ok && !first0) ? pold[k] : T(0);
+    }
+  };
+  if (blockIdx.x < mr.nunits) load_unit(blockIdx.x);
+  T beta;
+  bool first;
+  if (!dir_prologue<T>(st, partB, Gprev, partR64, Gref, hist, hist_cap, lds,
+                       &beta, &first))
+    return;
+
   T acc_pq = T(0);
   for (int u = blockIdx.x; u < mr.nunits; u += gridDim.x) {
+    if (u != (int)blockIdx.x) load_unit(u);
     const int item = u * 4 + threadIdx.y;
     const int strip = item / mr.nchunk;
     if (strip >= mr.nstrip) continue;  // tail of the last unit
@@ -327,16 +368,6 @@ __global__ __launch_bounds__(256) void k_cg_dir_march(
     const int j = c0 - 1 + lane;
     const bool jin = j >= 0 && j < ny;
     const bool out = lane >= 1 && lane <= 62 && j < ny;
-    // every load of the chunk first (rows r0-1 .. r0+H), then the math
-    T pn[H + 2], pv[H + 2];
-#pragma unroll
-    for (int h = 0; h < H + 2; ++h) {
-      const int i = r0 - 1 + h;
-      const bool ok = jin && i >= 0 && i < nx;
-      const size_t k = ok ? (size_t)i * ny + j : 0;
-      pn[h] = ok ? r[k] : T(0);
-      pv[h] = (ok && !first) ? pold[k] : T(0);
-    }
 #pragma unroll
     for (int h = 0; h < H + 2; ++h) {
       const int i = r0 - 1 + h;
@@ -370,6 +401,60 @@ __global__ __launch_bounds__(256) void k_cg_dir_march(
   }
   T v[1] = {acc_pq};
   store_partials<T, 1>(v, partA, gridDim.x, blockIdx.x, lds);
+}
+
+// ---------------------------------------------------------------------
+// FP64 residual replacement of the mixed-precision iteration (no reference
+// counterpart).  Every refresh_every FP32 iterations the FP32 increment
+// x32 is folded into the FP64 solution and the recurrence residual is
+// replaced by the true FP64 residual b - A x rounded to FP32; the search
+// direction is kept, so the Krylov information is not thrown away.
+
+// x64 += (double)x32; x32 = 0
+static __global__ __launch_bounds__(256) void k_merge_x(double* __restrict__ x64,
+                                                 float* __restrict__ x32,
+                                                 size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    x64[k] = x64[k] + (double)x32[k];
+    x32[k] = 0.f;
+  }
+}
+
+// r32 = (float)(b - A x64) with the FP64 operator; partials of the FP32
+// r.z, r.r (partB, same slots as the update kernel) and of the FP64 r.r.
+template <bool PC>
+__global__ __launch_bounds__(256) void k_refresh(Sten<double> s64, Sten<float> s32,
+                                                 const double* __restrict__ x64,
+                                                 const double* __restrict__ b64,
+                                                 float* __restrict__ r32,
+                                                 float* __restrict__ partB,
+                                                 double* __restrict__ partR64,
+                                                 Tiling tl) {
+  __shared__ float lds[16];
+  __shared__ double lds64[4];
+  float v[3] = {0.f, 0.f, 0.f};
+  double w[1] = {0.0};
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i >= s64.nx || j >= s64.ny) continue;
+      const size_t idx = (size_t)i * s64.ny + j;
+      const double rv = b64[idx] - apply_global(s64, x64, i, j, diag_at(s64, i, j));
+      w[0] += rv * rv;
+      const float rf = (float)rv;
+      r32[idx] = rf;
+      const float z = PC ? rf * (1.f / diag_at(s32, i, j)) : rf;
+      v[SLOT_RZ] += rf * z;
+      v[SLOT_RR] += rf * rf;
+    }
+  }
+  store_partials<float, 3>(v, partB, gridDim.x, blockIdx.x, lds);
+  store_partials<double, 1>(w, partR64, gridDim.x, blockIdx.x, lds64);
 }
 
 // Iteration `it`, second kernel: alpha = rho_it / p.q; x += alpha p;

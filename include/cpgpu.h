@@ -43,7 +43,8 @@ extern "C" {
 #define CPGPU_JPCG 1      /* IC_CG loop (:318-374) with DiaPre(A,1) (preconditioner.hpp:182-262) */
 #define CPGPU_BICGSTAB 2  /* carpio::Dia_BiCGSTAB, solver_matrix.hpp:599-672 */
 #define CPGPU_JACOBI 3    /* carpio::Solver_Jacobi_, solver_matrix.hpp:820-910 */
-#define CPGPU_MIXED 4     /* FP32 J-PCG inside FP64 iterative refinement (new) */
+#define CPGPU_MIXED 4     /* FP32 J-PCG with FP64 residual replacement (new) */
+#define CPGPU_MIXED_RESTART 5 /* FP32 J-PCG restarted inside FP64 refinement (new) */
 
 /* ---- stopping-norm modes ---- */
 #define CPGPU_NORM_REF 0  /* carpio::nrm2: drops the last unknown (BLAS_Level1.hpp:569) */

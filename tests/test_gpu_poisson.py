@@ -220,3 +220,25 @@ def test_autotune_keeps_phi_and_picks_a_variant(ctx):
     assert 0 <= res["fp64"][0] <= 4 and 0 <= res["fp32"][0] <= 4
     assert np.array_equal(og.interior(P.get_phi()), og.interior(phi0))
     P.close()
+
+
+@pytest.mark.parametrize("method", ["MIXED", "MIXED_RESTART"])
+@pytest.mark.parametrize("types", [[1, 1, 1, 1], [2, 1, 2, 1]])
+def test_mixed_methods_reach_fp64_tolerance(ctx, method, types):
+    og, A, src, desc = both(300, 200, 1, types, [0.0, 0.5, 0.0, -0.25], seed=8)
+    P = ca.Poisson(ctx, desc)
+    P.set_source(src)
+    code, it, res, hist = P.solve(method, max_iter=400000, tol=1e-10,
+                                  norm=ca.NORM_TRUE, hist_cap=256)
+    assert code == 0 and res <= 1e-10
+    assert P.residual() <= 1e-10
+    x_ref = O.solve("JPCG", A, A.b, tol=1e-13, max_iter=100000, norm=O.NORM_TRUE)[1]
+    err = rel_l2(O.field_to_x(og, P.get_phi()), x_ref)
+    it64 = O.solve("JPCG", A, A.b, tol=1e-10, max_iter=100000, norm=O.NORM_TRUE)[2]
+    print(f"{method} {types}: {it} FP32 iterations (FP64 J-PCG {it64}), "
+          f"err {err:.2e}, residual history {len(hist)}")
+    assert err < 1e-7
+    if method == "MIXED":
+        # no restarts: the iteration count stays near the FP64 one
+        assert it <= 1.5 * it64 + 32
+    P.close()
