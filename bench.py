@@ -42,6 +42,9 @@ def main():
                     help="skip the 4096^2 HBM roofline probe")
     ap.add_argument("--no-fp64", action="store_true",
                     help="skip the plain FP64 J-PCG comparison")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the Krylov batches directly instead of as "
+                         "replayed hipGraphs (same kernels; for profilers)")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -58,6 +61,8 @@ def main():
     P = ca.Poisson(ctx, desc)
     P.set_source_blobs(seed=0, nblobs=64, wmin=0.01, wmax=0.1)
     P.set_mixed(inner_tol=1e-4, max_outer=40)
+    if a.no_graphs:
+        P.set_batch(16, use_graphs=False)
     tune = P.autotune(reps=20)
     log(f"bench: {n}^2 all-Neumann blob problem ready; autotune {tune}")
 
@@ -102,17 +107,28 @@ def main():
         log(f"bench: fp64 J-PCG {rate:.2f} solves/s, {it64} it")
 
     # ---- dominant kernel of the headline solve: GPU time per launch ----
-    d32, u32 = P.kernel_times(32, reps=100)
-    fam = [(d32, 4, f"k_cg_dir (variant {tune['fp32'][0]}) <float>: p update + A p + p.q"),
-           (u32, 6, "k_cg_upd<float>: x, r update + r.z, r.r")]
-    us, nvec, kname = max(fam)
+    # One more solve of the same problem with HIP events around every
+    # launch of the two Krylov kernel families (direct launches on the
+    # context stream; the timed solves replay the same kernels as graphs).
+    ctx.timer_enable(0)
+    ctx.timer_enable(1)
+    solve("MIXED")
+    n_d, ms_d = ctx.timer_read(0)
+    n_u, ms_u = ctx.timer_read(1)
+    ctx.timer_enable(0, False)
+    ctx.timer_enable(1, False)
+    d32, u32 = ms_d * 1e3 / max(n_d, 1), ms_u * 1e3 / max(n_u, 1)
+    fam = [(ms_d, d32, n_d, 4, f"k_cg_dir (variant {tune['fp32'][0]}) <float>: "
+                                "p update + A p + p.q"),
+           (ms_u, u32, n_u, 6, "k_cg_upd<float>: x, r update + r.z, r.r")]
+    _, us, launches, nvec, kname = max(fam)
     alg_bytes = nvec * N * 4
     achieved = alg_bytes / (us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": kname, "bytes_per_launch": alg_bytes,
-                "avg_launch_us": round(us, 3),
+                "avg_launch_us": round(us, 3), "launches": launches,
                 "other_kernel_us": round(min(d32, u32), 3),
                 "iteration_us": round(d32 + u32, 3),
                 "note": f"{n}^2 FP32 vectors ({N * 4 / 2**20:.0f} MiB each) are "

@@ -23,8 +23,11 @@ NORM_REF, NORM_TRUE = 0, 1
 OPT_PROJECT_MEAN = 1
 
 MIXED_RESTART = 5
+FCG_UNET = 6
 METHODS = {"CG": CG, "JPCG": JPCG, "Dia_BiCGSTAB": BICGSTAB,
-           "Jacobi": JACOBI, "MIXED": MIXED, "MIXED_RESTART": MIXED_RESTART}
+           "Jacobi": JACOBI, "MIXED": MIXED, "MIXED_RESTART": MIXED_RESTART,
+           "FCG_UNET": FCG_UNET}
+UNET_WARM, UNET_PRECOND = 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -60,7 +63,11 @@ EXPORTS = [
     "cpgpu_ns_get_field", "cpgpu_ns_step", "cpgpu_ns_time",
     "cpgpu_ns_projection", "cpgpu_poisson_set_variant",
     "cpgpu_poisson_autotune", "cpgpu_poisson_kernel_times",
-    "cpgpu_poisson_set_batch",
+    "cpgpu_poisson_set_batch", "cpgpu_unet_create", "cpgpu_unet_destroy",
+    "cpgpu_unet_num_params", "cpgpu_unet_get_params", "cpgpu_unet_set_params",
+    "cpgpu_unet_forward", "cpgpu_unet_backward", "cpgpu_unet_adam",
+    "cpgpu_unet_set_sample", "cpgpu_unet_num_samples", "cpgpu_unet_train",
+    "cpgpu_unet_eval", "cpgpu_poisson_set_unet", "cpgpu_unet_tensor",
 ]
 
 
@@ -76,6 +83,7 @@ def lib():
         L.cpgpu_last_error.restype = C.c_char_p
         L.cpgpu_stream.restype = C.c_void_p
         L.cpgpu_poisson_device_vec.restype = C.c_void_p
+        L.cpgpu_unet_num_params.restype = C.c_longlong
         _lib = L
     return _lib
 
@@ -398,3 +406,108 @@ def _set_batch(self, iters=16, use_graphs=True):
 
 Poisson.kernel_times = _kernel_times
 Poisson.set_batch = _set_batch
+
+
+# ------------------------------------------------------- learned preconditioner
+_fp = C.POINTER(C.c_float)
+
+
+class UNet:
+    """BF16 U-Net preconditioner (cpgpu_unet): 4 levels, 16-32-64-128
+    channels, stride-4 pooling, FP32 master parameters, Adam."""
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.check(lib().cpgpu_unet_create(ctx.h, C.c_ulonglong(seed),
+                                          C.byref(self.h)))
+        ctx.adopt(self)
+        self.num_params = int(lib().cpgpu_unet_num_params(self.h))
+
+    def get_params(self, grads=False):
+        out = np.zeros(self.num_params, dtype=np.float32)
+        self.ctx.check(lib().cpgpu_unet_get_params(self.h, _p(out, _fp),
+                                                   int(bool(grads))))
+        return out
+
+    def set_params(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        assert a.size == self.num_params
+        self.ctx.check(lib().cpgpu_unet_set_params(self.h, _p(a, _fp)))
+
+    def forward(self, x):
+        """Raw network output for an (H, W) array (no input scaling)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros_like(x)
+        H, W = x.shape
+        self.ctx.check(lib().cpgpu_unet_forward(self.h, H, W, _p(x, _fp),
+                                                _p(y, _fp)))
+        return y
+
+    def backward(self, dy):
+        """Gradients of sum(dy * y) for the last forward()."""
+        dy = np.ascontiguousarray(dy, dtype=np.float32)
+        g = np.zeros(self.num_params, dtype=np.float32)
+        self.ctx.check(lib().cpgpu_unet_backward(self.h, _p(dy, _fp),
+                                                 _p(g, _fp)))
+        return g
+
+    def adam(self, lr):
+        self.ctx.check(lib().cpgpu_unet_adam(self.h, C.c_float(lr)))
+
+    TENSORS = ["x", "a0", "C0", "d0a", "d0", "p0", "a1", "C1", "d1a", "d1",
+               "p1", "a2", "C2", "d2a", "d2", "p2", "a3", "b3", "y"]
+
+    def tensor(self, name, grad=False):
+        """Internal activation (or its gradient) of the last forward
+        (backward) as an (H, W, C) float32 array."""
+        tid = self.TENSORS.index(name)
+        dims = (C.c_int * 3)()
+        self.ctx.check(lib().cpgpu_unet_tensor(self.h, tid, int(grad), None, dims))
+        out = np.zeros((dims[0], dims[1], dims[2]), dtype=np.float32)
+        self.ctx.check(lib().cpgpu_unet_tensor(self.h, tid, int(grad),
+                                               _p(out, _fp), dims))
+        return out
+
+    def set_sample(self, P, slot, e):
+        """Training sample: error e (nx, ny) of problem P; r = A e on device."""
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        assert e.shape == (P.nx, P.ny)
+        self.ctx.check(lib().cpgpu_unet_set_sample(self.h, P.h, int(slot),
+                                                   _p(e)))
+
+    @property
+    def num_samples(self):
+        return int(lib().cpgpu_unet_num_samples(self.h))
+
+    def train(self, P, steps, batch=4, lr=1e-3, seed=0):
+        losses = np.zeros(max(steps, 1), dtype=np.float32)
+        self.ctx.check(lib().cpgpu_unet_train(
+            self.h, P.h, int(steps), int(batch), C.c_float(lr),
+            C.c_ulonglong(seed), _p(losses, _fp)))
+        return losses[:steps]
+
+    def eval(self, P):
+        out = np.zeros(self.num_samples, dtype=np.float32)
+        self.ctx.check(lib().cpgpu_unet_eval(self.h, P.h, _p(out, _fp)))
+        return out
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().cpgpu_unet_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _set_unet(self, unet, flags=UNET_PRECOND):
+    self.ctx.check(lib().cpgpu_poisson_set_unet(
+        self.h, unet.h if unet is not None else None, int(flags)))
+    self._unet = unet  # keep it alive while attached
+
+
+Poisson.set_unet = _set_unet

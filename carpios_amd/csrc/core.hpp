@@ -11,6 +11,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdio>
 #include <string>
@@ -40,14 +41,24 @@ int set_error(cpgpu_ctx* ctx, int code, const char* fmt, ...);
                                 __LINE__, #call, hipGetErrorString(e_));      \
   } while (0)
 
-// Brackets kernel launches of one family with events when timing is on.
-struct TimerScope {
-  cpgpu_ctx* ctx;
-  int fam;
-  hipEvent_t stop = nullptr;
-  TimerScope(cpgpu_ctx* c, int f);
-  ~TimerScope();
+// Start/stop events of one timed launch of a kernel family (both null when
+// timing is off for the family or ctx is null).
+struct EvPair {
+  hipEvent_t a = nullptr, b = nullptr;
 };
+EvPair timer_events(cpgpu_ctx* ctx, int fam);
+
+// Launch on a stream; with timer events through hipExtLaunchKernelGGL,
+// whose events take the dispatch's own start and end timestamps (no extra
+// packets between kernels, so the figure is the kernel's execution time).
+#define CPGPU_LAUNCH(ev, kernel, grid, block, shmem, st, ...)                  \
+  do {                                                                         \
+    if ((ev).a)                                                                \
+      hipExtLaunchKernelGGL(kernel, grid, block, shmem, st, (ev).a, (ev).b, 0, \
+                            __VA_ARGS__);                                      \
+    else                                                                       \
+      hipLaunchKernelGGL(kernel, grid, block, shmem, st, __VA_ARGS__);         \
+  } while (0)
 
 constexpr int kThreads = 256;  // threads per block of every reduction kernel
 constexpr int kMaxBlocks = 1024;  // upper bound of partials per reduction

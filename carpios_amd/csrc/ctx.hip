@@ -17,22 +17,20 @@ int set_error(cpgpu_ctx* ctx, int code, const char* fmt, ...) {
   return code;
 }
 
-TimerScope::TimerScope(cpgpu_ctx* c, int f) : ctx(c), fam(f) {
-  if (!ctx || !ctx->timer_on[fam]) return;
+EvPair timer_events(cpgpu_ctx* ctx, int fam) {
+  EvPair p;
+  if (!ctx || !ctx->timer_on[fam]) return p;
   auto& v = ctx->ev[fam];
   size_t& u = ctx->ev_used[fam];
   while (v.size() < u + 2) {
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
+    if (hipEventCreate(&e) != hipSuccess) return p;
     v.push_back(e);
   }
-  hipEventRecord(v[u], ctx->stream);
-  stop = v[u + 1];
+  p.a = v[u];
+  p.b = v[u + 1];
   u += 2;
-}
-
-TimerScope::~TimerScope() {
-  if (stop) hipEventRecord(stop, ctx->stream);
+  return p;
 }
 
 }  // namespace cpgpu

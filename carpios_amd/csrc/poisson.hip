@@ -346,18 +346,18 @@ void launch_dir(int variant, hipStream_t st, const Sten<T>& s, const T* r,
                 const T* pold, T* pnew, T* q, KState* kst, const T* partB,
                 T* partA, double* hist, int hist_cap, const Tiling& tl,
                 int Gprev, int G, const double* partR64 = nullptr,
-                int Gref = 0) {
+                int Gref = 0, EvPair ev = EvPair()) {
   const dim3 block(TJ, BY);
   if (variant == 0) {
-    hipLaunchKernelGGL((k_cg_dir<T, PC>), dim3(G), block, 0, st, s, r, pold,
-                       pnew, q, kst, partB, partA, hist, hist_cap, tl, Gprev,
-                       partR64, Gref);
+    CPGPU_LAUNCH(ev, (k_cg_dir<T, PC>), dim3(G), block, 0, st, s, r, pold,
+                 pnew, q, kst, partB, partA, hist, hist_cap, tl, Gprev,
+                 partR64, Gref);
     return;
   }
 #define CPGPU_DIR_MARCH(HV)                                                   \
-  hipLaunchKernelGGL((k_cg_dir_march<T, PC, HV>), dim3(G), block, 0, st, s, r, \
-                     pold, pnew, q, kst, partB, partA, hist, hist_cap,        \
-                     make_march(s.nx, s.ny, HV), Gprev, partR64, Gref)
+  CPGPU_LAUNCH(ev, (k_cg_dir_march<T, PC, HV>), dim3(G), block, 0, st, s, r,  \
+               pold, pnew, q, kst, partB, partA, hist, hist_cap,              \
+               make_march(s.nx, s.ny, HV), Gprev, partR64, Gref)
   switch (variant) {
     case 1: CPGPU_DIR_MARCH(4); break;
     case 2: CPGPU_DIR_MARCH(8); break;
@@ -380,18 +380,13 @@ void launch_batch(cpgpu_poisson* P, hipStream_t st, const Sten<T>& s,
   for (int k = 0; k < iters; ++k) {
     T* pold = (k & 1) ? v.pb : v.pa;
     T* pnew = (k & 1) ? v.pa : v.pb;
-    {
-      TimerScope ts(timed ? P->ctx : nullptr, 0);
-      launch_dir<T, PC>(variant, st, s, v.r, pold, pnew, v.q, P->kst.p,
-                        v.partB, v.partA, dhist, hist_cap, tl, tl.G, Gd, pr,
-                        tl.G);
-    }
-    {
-      TimerScope ts(timed ? P->ctx : nullptr, 1);
-      hipLaunchKernelGGL((k_cg_upd<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st,
-                         s, v.x, pnew, v.r, v.q, P->kst.p, v.partA, v.partB,
-                         tl, norm_ref, Gd);
-    }
+    launch_dir<T, PC>(variant, st, s, v.r, pold, pnew, v.q, P->kst.p,
+                      v.partB, v.partA, dhist, hist_cap, tl, tl.G, Gd, pr,
+                      tl.G, timer_events(timed ? P->ctx : nullptr, 0));
+    const EvPair eu = timer_events(timed ? P->ctx : nullptr, 1);
+    CPGPU_LAUNCH(eu, (k_cg_upd<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st, s,
+                 v.x, pnew, v.r, v.q, P->kst.p, v.partA, v.partB, tl,
+                 norm_ref, Gd);
   }
   if constexpr (sizeof(T) == 4) {
     if (reliable) {
@@ -756,11 +751,44 @@ int poisson_rhs_from_device(cpgpu_poisson* P, const double* dsrc,
   return CPGPU_OK;
 }
 
+int poisson_prepare_f32(cpgpu_poisson* P) { return alloc_f32(P); }
+
+int poisson_jpcg64(cpgpu_poisson* P, int* iters, double tol, int norm_ref,
+                   double* resid, int* done) {
+  CGVecs<double> v{P->x.p, P->b.p, P->r.p, P->p0.p, P->p1.p, P->q.p,
+                   P->partA64.p, P->partB64.p};
+  KState ks;
+  int rc = run_cg<double, true>(P, P->sten64(), v, *iters, tol, norm_ref,
+                                nullptr, 0, &ks);
+  if (rc) return rc;
+  *iters = ks.iter;
+  *resid = ks.resid;
+  *done = ks.done == 1;
+  return CPGPU_OK;
+}
+
 int poisson_solve_internal(cpgpu_poisson* P, int method, int* max_iter,
                            double* tol, int norm_mode, int opts, double* hist,
                            int hist_cap, int* hist_len) {
   cpgpu_ctx* ctx = P->ctx;
   int hl = 0;
+  if (method == CPGPU_FCG_UNET) {
+    const int nref = norm_mode == CPGPU_NORM_REF && P->N > 1;
+    int iters = 0;
+    double resid = 0;
+    int code = run_fcg_unet(P, *max_iter, *tol, nref,
+                            opts & CPGPU_OPT_PROJECT_MEAN, hist, hist_cap, &hl,
+                            &iters, &resid);
+    if (code < 0) return code;
+    if (hist_len) *hist_len = hl;
+    *max_iter = iters;
+    *tol = resid;
+    return code;
+  }
+  if (P->unet && (P->unet_flags & CPGPU_UNET_WARM)) {
+    int rc = unet_warm_start(P);
+    if (rc) return rc;
+  }
   if (method == CPGPU_CG || method == CPGPU_JPCG) {
     if (hist && hist_cap > 0 && P->dhist.alloc(hist_cap))
       return set_error(ctx, CPGPU_ENOMEM, "hist alloc");
@@ -1058,6 +1086,19 @@ int cpgpu_poisson_set_batch(cpgpu_poisson* P, int iters, int use_graphs) {
   if (!P || iters < 2 || (iters & 1)) return CPGPU_EINVAL;
   P->batch = iters;
   P->use_graphs = use_graphs != 0;
+  return CPGPU_OK;
+}
+
+int cpgpu_poisson_set_unet(cpgpu_poisson* P, cpgpu_unet* U, int flags) {
+  if (!P || (flags & ~(CPGPU_UNET_WARM | CPGPU_UNET_PRECOND))) return CPGPU_EINVAL;
+  if (U && unet_context(U) != P->ctx)
+    return set_error(P->ctx, CPGPU_EINVAL, "U-Net and problem use different contexts");
+  if (U && (P->nx % 64 || P->ny % 64))
+    return set_error(P->ctx, CPGPU_EUNSUPPORTED,
+                     "the U-Net needs grid sides that are multiples of 64 (got %dx%d)",
+                     P->nx, P->ny);
+  P->unet = U;
+  P->unet_flags = U ? flags : 0;
   return CPGPU_OK;
 }
 

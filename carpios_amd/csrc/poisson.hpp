@@ -48,6 +48,15 @@ struct RState {
   double bb, rr, rsum, xsum;
 };
 
+// Flexible-CG state (device, mirrored to pinned host memory).
+struct FState {
+  double normb;   // ||b|| of the stopping test
+  double pq;      // p.q of the last direction (for the next beta)
+  double resid;   // ||r|| / ||b|| after the last update
+  int iter;       // updates done
+  int breakdown;  // p.q <= 0 or not finite
+};
+
 // Identity of a captured batch of Krylov iterations.
 struct GraphKey {
   const void* x;
@@ -101,6 +110,14 @@ struct cpgpu_poisson {
   int dir_variant64 = 3, dir_variant32 = 3;
   double inner_tol = 1e-4;
   int max_outer = 40;
+  // attached U-Net (warm start / flexible-CG preconditioner) and the
+  // flexible-CG state
+  cpgpu_unet* unet = nullptr;
+  int unet_flags = 0;
+  int fcg_iters = 0;  // flexible-CG iterations of the last CPGPU_FCG_UNET solve
+  cpgpu::DevBuf<double> pf[2];
+  cpgpu::DevBuf<cpgpu::FState> fst;
+  cpgpu::FState* h_fst = nullptr;  // pinned
 
   cpgpu::Sten<double> sten64() const {
     cpgpu::Sten<double> s;
@@ -126,6 +143,7 @@ struct cpgpu_poisson {
       if (e) (void)hipEventDestroy(e);
     if (h_kst) (void)hipHostFree(h_kst);
     if (h_rst) (void)hipHostFree(h_rst);
+    if (h_fst) (void)hipHostFree(h_fst);
   }
 };
 
@@ -137,4 +155,20 @@ int poisson_solve_internal(cpgpu_poisson* P, int method, int* max_iter,
 // b from a device source field (interior-only, j contiguous).
 int poisson_rhs_from_device(cpgpu_poisson* P, const double* dsrc,
                             double src_uniform);
+// FP32 operator coefficients and FP32 work vectors of the problem.
+int poisson_prepare_f32(cpgpu_poisson* P);
+// Plain FP64 Jacobi-PCG from the current phi (the reference's test norm
+// when norm_ref, else the Euclidean one); *iters in: limit, out: used.
+int poisson_jpcg64(cpgpu_poisson* P, int* iters, double tol, int norm_ref,
+                   double* resid, int* done);
+// Flexible CG with the attached U-Net as preconditioner (fcg.hip).
+int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
+                 int project, double* hist, int hist_cap, int* hist_len,
+                 int* iters, double* resid);
+// phi = s net(b / s), s = rms(b), with the attached U-Net (fcg.hip).
+int unet_warm_start(cpgpu_poisson* P);
+// out = s net(in / s), s = rms(in), on H x W device FP32 arrays (unet.hip).
+int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out);
+// The context a U-Net was created on (unet.hip).
+cpgpu_ctx* unet_context(const cpgpu_unet* U);
 }  // namespace cpgpu

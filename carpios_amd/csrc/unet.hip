@@ -1,0 +1,1010 @@
+// U-Net preconditioner on gfx950 (no reference counterpart): parameters,
+// forward, backward, Adam, energy-norm training loss, and the C-ABI.
+//
+// Architecture (4 levels, 16-32-64-128 channels, 3x3 convs + ReLU, 4x4
+// average pooling with stride 4 and 4x4 stride-4 transposed convs, skip
+// connections by channel concatenation, 1x1 linear output):
+//   L0 16 : c0a 1->16, c0b 16->16 ---------------------------- skip --+
+//   L1 32 :   pool, c1a 16->32, c1b 32->32 ------------------ skip -+ |
+//   L2 64 :     pool, c2a 32->64, c2b 64->64 ------------- skip -+ | |
+//   L3 128:       pool, c3a 64->128, c3b 128->128              | | |
+//   L2    :     up 128->64, concat, d2a 128->64, d2b 64->64 <--+ | |
+//   L1    :   up 64->32, concat, d1a 64->32, d1b 32->32 <--------+ |
+//   L0    : up 32->16, concat, d0a 32->16, d0b 16->16 <------------+
+//   out   : 1x1 16->1
+// Stride-4 pooling gives the 1024^2 grid a 16^2 bottleneck, so one
+// application couples the whole domain.  Grid sides must be multiples of 64.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "poisson.hpp"
+#include "unet_kernels.hpp"
+
+namespace cpgpu {
+namespace unet {
+
+enum Kind { K_FIRST, K_CONV, K_UP, K_OUT };
+
+struct LayerDef {
+  Kind kind;
+  int cin, cout;
+};
+
+// fixed layer list (order = parameter order)
+static const LayerDef kLayers[] = {
+    {K_FIRST, 1, 16},  {K_CONV, 16, 16},  {K_CONV, 16, 32},  {K_CONV, 32, 32},
+    {K_CONV, 32, 64},  {K_CONV, 64, 64},  {K_CONV, 64, 128}, {K_CONV, 128, 128},
+    {K_UP, 128, 64},   {K_CONV, 128, 64}, {K_CONV, 64, 64},  {K_UP, 64, 32},
+    {K_CONV, 64, 32},  {K_CONV, 32, 32},  {K_UP, 32, 16},    {K_CONV, 32, 16},
+    {K_CONV, 16, 16},  {K_OUT, 16, 1}};
+constexpr int kNL = sizeof(kLayers) / sizeof(kLayers[0]);
+
+inline int ntaps(Kind k) { return k == K_UP ? 16 : (k == K_OUT ? 1 : 9); }
+
+// resolution level (pixels / 16^level) of each layer's output
+static const int kLevel[] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 2, 1, 1, 1, 0, 0, 0, 0};
+static_assert(sizeof(kLevel) / sizeof(kLevel[0]) == kNL, "one level per layer");
+inline int level_of(int l) { return kLevel[l]; }
+
+// chunks of the weight-gradient GEMM over pixels (K dimension)
+inline int wg_chunks(int M) {
+  int c = (M + 2047) / 2048;
+  return c < 256 ? c : 256;
+}
+
+// ---------------------------------------------------------------------
+// small kernels
+
+// first conv 1 -> 16 (FP32 input, direct): a[m][co] = relu(b + sum W x)
+__global__ __launch_bounds__(256) void k_first_fwd(const float* __restrict__ x,
+                                                   int H, int W, View a,
+                                                   const float* __restrict__ w,
+                                                   const float* __restrict__ b) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= H * W) return;
+  const int i = m / W, j = m - i * W;
+  float xv[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int si = i + t / 3 - 1, sj = j + t % 3 - 1;
+    xv[t] = (si >= 0 && si < H && sj >= 0 && sj < W) ? x[si * W + sj] : 0.f;
+  }
+  bf16 out[16];
+#pragma unroll
+  for (int co = 0; co < 16; ++co) {
+    float s = b[co];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) s += w[t * 16 + co] * xv[t];
+    out[co] = f2bf(s > 0.f ? s : 0.f);
+  }
+  __builtin_memcpy(a.p + (size_t)m * a.cs + a.co, out, 32);
+}
+
+// first conv weight/bias gradient partials: per block 9*16 + 16 values
+__global__ __launch_bounds__(256) void k_first_wgrad(const float* __restrict__ x,
+                                                     int H, int W, GView g,
+                                                     View a,
+                                                     float* __restrict__ part) {
+  // thread t owns output channel co = t % 16 and walks pixels 16 apart
+  __shared__ float red[10][256];
+  const int co = threadIdx.x & 15;
+  float acc[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) acc[k] = 0.f;
+  const int M = H * W;
+  for (int m = blockIdx.x * 16 + (threadIdx.x >> 4); m < M; m += gridDim.x * 16) {
+    const int i = m / W, j = m - i * W;
+    float gv = g.p[(size_t)m * g.cs + g.co + co];
+    if (!(bf2f(a.p[(size_t)m * a.cs + a.co + co]) > 0.f)) gv = 0.f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int si = i + t / 3 - 1, sj = j + t % 3 - 1;
+      const float xv = (si >= 0 && si < H && sj >= 0 && sj < W) ? x[si * W + sj] : 0.f;
+      acc[t] += gv * xv;
+    }
+    acc[9] += gv;
+  }
+#pragma unroll
+  for (int k = 0; k < 10; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int s = 128; s >= 16; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int k = 0; k < 10; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  // part[blk*160 + tap*16 + co] (weights [tap][co]), then 16 biases
+  if (threadIdx.x < 16) {
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
+      part[(size_t)blockIdx.x * 160 + k * 16 + threadIdx.x] = red[k][threadIdx.x];
+  }
+}
+
+// output 1x1 conv 16 -> 1 (FP32 out)
+__global__ __launch_bounds__(256) void k_out_fwd(View d, const float* __restrict__ w,
+                                                 const float* __restrict__ b,
+                                                 float* __restrict__ y, int M) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  bf16 v[16];
+  __builtin_memcpy(v, d.p + (size_t)m * d.cs + d.co, 32);
+  float s = b[0];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) s += w[c] * bf2f(v[c]);
+  y[m] = s;
+}
+
+// output layer backward: g_d[m][c] = dy[m] w[c]; partials of dW (16), db
+__global__ __launch_bounds__(256) void k_out_bwd(View d, const float* __restrict__ w,
+                                                 const float* __restrict__ dy,
+                                                 GView gd, int M,
+                                                 float* __restrict__ part) {
+  __shared__ float red[256];
+  float acc[17];
+#pragma unroll
+  for (int k = 0; k < 17; ++k) acc[k] = 0.f;
+  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M;
+       m += gridDim.x * blockDim.x) {
+    const float g = dy[m];
+    bf16 v[16];
+    __builtin_memcpy(v, d.p + (size_t)m * d.cs + d.co, 32);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      gd.p[(size_t)m * gd.cs + gd.co + c] = g * w[c];
+      acc[c] += g * bf2f(v[c]);
+    }
+    acc[16] += g;
+  }
+  for (int k = 0; k < 17; ++k) {
+    red[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * 17 + k] = red[0];
+    __syncthreads();
+  }
+}
+
+// 4x4 average pooling, C channels: out[I][J][c] = mean of the 4x4 block
+__global__ __launch_bounds__(256) void k_pool_fwd(View in, View out, int C) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Mc = out.H * out.W;
+  if (e >= Mc * C) return;
+  const int m = e / C, c = e - m * C;
+  const int I = m / out.W, J = m - I * out.W;
+  float s = 0.f;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      s += bf2f(in.p[((size_t)(4 * I + a) * in.W + 4 * J + b) * in.cs + in.co + c]);
+  out.p[(size_t)m * out.cs + out.co + c] = f2bf(s * (1.f / 16.f));
+}
+
+// pooling backward accumulated into the fine gradient: gin += gout/16
+__global__ __launch_bounds__(256) void k_pool_bwd(GView gout, GView gin, int C) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Mf = gin.H * gin.W;
+  if (e >= Mf * C) return;
+  const int m = e / C, c = e - m * C;
+  const int i = m / gin.W, j = m - i * gin.W;
+  const size_t mc = (size_t)(i / 4) * gout.W + (j / 4);
+  gin.p[(size_t)m * gin.cs + gin.co + c] +=
+      gout.p[mc * gout.cs + gout.co + c] * (1.f / 16.f);
+}
+
+// bias gradient partials: sum over pixels of g (masked by y when relu);
+// for up layers the fine pixels of the output.  C divides 256, so thread t
+// always accumulates channel t % C (coalesced over (pixel, channel)).
+__global__ __launch_bounds__(256) void k_bias_grad(GView g, View y, int relu,
+                                                   int C, float* __restrict__ part) {
+  __shared__ float red[256];
+  const size_t n = (size_t)g.H * g.W * C;
+  float s = 0.f;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t m = e / C;
+    const int c = (int)(e - m * C);
+    float v = g.p[m * g.cs + g.co + c];
+    if (relu && !(bf2f(y.p[m * y.cs + y.co + c]) > 0.f)) v = 0.f;
+    s += v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k >= C; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < C) part[(size_t)blockIdx.x * C + threadIdx.x] = red[threadIdx.x];
+}
+
+// out[e] = sum_k part[k*count + e], fixed order
+__global__ __launch_bounds__(256) void k_sum_chunks(const float* __restrict__ part,
+                                                    int nchunks, int count,
+                                                    float* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  float s = 0.f;
+  for (int k = 0; k < nchunks; ++k) s += part[(size_t)k * count + e];
+  out[e] = s;
+}
+
+// Adam with bias correction on the FP32 master parameters
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p,
+                                              const float* __restrict__ g,
+                                              float* __restrict__ m1,
+                                              float* __restrict__ m2, size_t n,
+                                              float lr, float b1, float b2,
+                                              float eps, float c1, float c2) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const float gv = g[k];
+    const float a = b1 * m1[k] + (1.f - b1) * gv;
+    const float v = b2 * m2[k] + (1.f - b2) * gv * gv;
+    m1[k] = a;
+    m2[k] = v;
+    p[k] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
+  }
+}
+
+// BF16 weight copies: w[tap][co][ci] and wt[tap][ci][co]
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ p,
+                                              bf16* __restrict__ w,
+                                              bf16* __restrict__ wt, int ntap,
+                                              int cout, int cin) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ntap * cout * cin) return;
+  const int tap = e / (cout * cin), r = e - tap * cout * cin;
+  const int co = r / cin, ci = r - co * cin;
+  const bf16 v = f2bf(p[e]);
+  w[e] = v;
+  wt[((size_t)tap * cin + ci) * cout + co] = v;
+}
+
+// FP32 <-> scaled copies for the network input/output
+__global__ __launch_bounds__(256) void k_scale_copy(const float* __restrict__ a,
+                                                    float* __restrict__ b,
+                                                    const float* scale, int inv,
+                                                    float extra, size_t n) {
+  const float s = inv ? 1.f / *scale : *scale * extra;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    b[k] = a[k] * s;
+}
+
+// sum of squares partials (FP32 data, FP64 partial sums)
+__global__ __launch_bounds__(256) void k_sumsq(const float* __restrict__ a,
+                                               size_t n, double* part) {
+  __shared__ double lds[4];
+  double v[1] = {0.0};
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    v[0] += (double)a[k] * a[k];
+  store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
+}
+
+// rms = sqrt(sum/n) (0 -> 1) into *out
+__global__ __launch_bounds__(256) void k_rms(const double* part, int G, size_t n,
+                                             float* out) {
+  __shared__ double lds[4];
+  double t[1];
+  reduce_partials<double, 1>(part, G, t, lds);
+  if (threadIdx.x == 0) {
+    const double r = std::sqrt(t[0] / (double)n);
+    out[0] = r > 0.0 ? (float)r : 1.f;
+  }
+}
+
+// training sample: r = A e in FP64, FP32 copies of r and e, partials of e.r
+__global__ __launch_bounds__(256) void k_sample(Sten<double> s, const double* __restrict__ e,
+                                                float* __restrict__ r32,
+                                                float* __restrict__ e32,
+                                                double* part, Tiling tl) {
+  __shared__ double lds[4];
+  double v[1] = {0.0};
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny) {
+        const size_t idx = (size_t)i * s.ny + j;
+        const double rv = apply_global(s, e, i, j, diag_at(s, i, j));
+        r32[idx] = (float)rv;
+        e32[idx] = (float)e[idx];
+        v[0] += e[idx] * rv;
+      }
+    }
+  }
+  store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
+}
+
+// total of G partials into out[0]
+__global__ __launch_bounds__(256) void k_total(const double* part, int G,
+                                               double* out) {
+  __shared__ double lds[4];
+  double t[1];
+  reduce_partials<double, 1>(part, G, t, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0) out[0] = t[0];
+}
+
+// d = s y - e (the error of the scaled network output)
+__global__ __launch_bounds__(256) void k_loss_d(const float* __restrict__ y,
+                                                const float* __restrict__ e,
+                                                const float* scale,
+                                                float* __restrict__ d, size_t n) {
+  const float s = *scale;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    d[k] = s * y[k] - e[k];
+}
+
+// partials of d.q and dL/dy = c s q with c = 2 / (e'Ae * batch)
+__global__ __launch_bounds__(256) void k_loss_q(const float* __restrict__ d,
+                                                const float* __restrict__ q,
+                                                const float* scale, float c,
+                                                float* __restrict__ dy,
+                                                double* part, size_t n) {
+  __shared__ double lds[4];
+  const float cs = c * *scale;
+  double v[1] = {0.0};
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const float qv = q[k];
+    v[0] += (double)d[k] * qv;
+    if (dy) dy[k] = cs * qv;
+  }
+  store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
+}
+
+// out[idx] = total / denom
+__global__ __launch_bounds__(256) void k_loss_fin(const double* part, int G,
+                                                  double denom, float* out) {
+  __shared__ double lds[4];
+  double t[1];
+  reduce_partials<double, 1>(part, G, t, lds);
+  if (threadIdx.x == 0) out[0] = (float)(t[0] / denom);
+}
+
+// a = b (first) or a += b
+__global__ __launch_bounds__(256) void k_accum(float* __restrict__ a,
+                                               const float* __restrict__ b,
+                                               size_t n, int first) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    a[k] = first ? b[k] : a[k] + b[k];
+}
+
+}  // namespace unet
+}  // namespace cpgpu
+
+using namespace cpgpu;
+using namespace cpgpu::unet;
+
+struct cpgpu_unet {
+  cpgpu_ctx* ctx = nullptr;
+  size_t woff[kNL], boff[kNL], wbf_off[kNL];
+  size_t nparams = 0, nwbf = 0;
+  DevBuf<float> params, grads, m1, m2;
+  DevBuf<bf16> wbf, wtbf;
+  long long adam_t = 0;
+  // activation allocation for one grid size
+  int H = 0, W = 0;
+  DevBuf<bf16> act;
+  DevBuf<float> gact;
+  // per-level tensors: offsets into act/gact
+  struct T {
+    size_t off;
+    int H, W, C;
+  };
+  T a0, C0, d0a, d0, p0, a1, C1, d1a, d1, p1, a2, C2, d2a, d2, p2, a3, b3;
+  DevBuf<float> x0, y, dy, part, scale;
+  DevBuf<double> dpart;
+  // training: samples (r = A e, e) in FP32, e' A e per sample, loss scratch
+  struct Sample {
+    DevBuf<float> r, e;
+    double eAe = 0.0;
+  };
+  std::vector<std::unique_ptr<Sample>> samples;
+  int sH = 0, sW = 0;  // grid of the samples
+  DevBuf<float> gacc, dbuf, qbuf, lossdev;
+  DevBuf<double> tmp64, lpart;
+
+  View v(const T& t, int co = 0) { return View{act.p + t.off, t.H, t.W, t.C, co}; }
+  GView g(const T& t, int co = 0) {
+    return GView{gact.p + t.off, t.H, t.W, t.C, co};
+  }
+};
+
+namespace {
+
+int unet_alloc(cpgpu_unet* U, int H, int W) {
+  if (U->H == H && U->W == W) return CPGPU_OK;
+  if (H < 64 || W < 64 || (H % 64) || (W % 64))
+    return set_error(U->ctx, CPGPU_EUNSUPPORTED,
+                     "U-Net grid sides must be multiples of 64 (got %dx%d)", H, W);
+  size_t off = 0;
+  auto mk = [&](int lvl, int C) {
+    cpgpu_unet::T t;
+    int s = 1 << (2 * lvl);
+    t.H = H / s;
+    t.W = W / s;
+    t.C = C;
+    t.off = off;
+    off += (size_t)t.H * t.W * C;
+    off = (off + 63) & ~(size_t)63;
+    return t;
+  };
+  U->a0 = mk(0, 16); U->C0 = mk(0, 32); U->d0a = mk(0, 16); U->d0 = mk(0, 16);
+  U->p0 = mk(1, 16); U->a1 = mk(1, 32); U->C1 = mk(1, 64); U->d1a = mk(1, 32);
+  U->d1 = mk(1, 32);
+  U->p1 = mk(2, 32); U->a2 = mk(2, 64); U->C2 = mk(2, 128); U->d2a = mk(2, 64);
+  U->d2 = mk(2, 64);
+  U->p2 = mk(3, 64); U->a3 = mk(3, 128); U->b3 = mk(3, 128);
+  const size_t N = (size_t)H * W;
+  // partial sums of the largest weight-gradient launch (chunks x taps x
+  // cout x cin), of the bias/first/output reductions otherwise
+  size_t npart = 64 * 160;
+  for (int l = 0; l < kNL; ++l) {
+    const LayerDef& d = kLayers[l];
+    if (d.kind != K_CONV && d.kind != K_UP) continue;
+    const int lvl = level_of(l);
+    const size_t M = (N >> (4 * lvl)) / (d.kind == K_UP ? 16 : 1);
+    npart = std::max(npart, (size_t)wg_chunks((int)M) * ntaps(d.kind) * d.cout * d.cin);
+  }
+  if (U->act.alloc(off) || U->gact.alloc(off) || U->x0.alloc(N) ||
+      U->y.alloc(N) || U->dy.alloc(N) || U->part.alloc(npart))
+    return set_error(U->ctx, CPGPU_ENOMEM, "U-Net activations");
+  U->H = H;
+  U->W = W;
+  return CPGPU_OK;
+}
+
+template <int CIN, int COUT>
+void conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu) {
+  constexpr int NB = COUT / 16 < 4 ? COUT / 16 : 4;
+  const int M = x.H * x.W;
+  dim3 grid((M + 64 * kMP - 1) / (64 * kMP), COUT / (16 * NB));
+  hipLaunchKernelGGL((k_conv3_fwd<CIN, COUT, NB>), grid, dim3(256), 0,
+                     U->ctx->stream, x, y, U->wbf.p + U->wbf_off[l],
+                     U->params.p + U->boff[l], relu);
+}
+
+template <int CIN, int COUT>
+void up_fwd(cpgpu_unet* U, View x, View y, int l) {
+  constexpr int NB = 4;
+  const int M = x.H * x.W;
+  dim3 grid((M + 64 * kMP - 1) / (64 * kMP), 16 * COUT / (16 * NB));
+  hipLaunchKernelGGL((k_up_fwd<CIN, COUT, NB>), grid, dim3(256), 0,
+                     U->ctx->stream, x, y, U->wbf.p + U->wbf_off[l],
+                     U->params.p + U->boff[l]);
+}
+
+int forward(cpgpu_unet* U) {
+  hipStream_t st = U->ctx->stream;
+  const int M = U->H * U->W;
+  hipLaunchKernelGGL(k_first_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
+                     U->x0.p, U->H, U->W, U->v(U->a0), U->params.p + U->woff[0],
+                     U->params.p + U->boff[0]);
+  conv_fwd<16, 16>(U, U->v(U->a0), U->v(U->C0, 16), 1, 1);
+  auto pool = [&](View in, View out, int C) {
+    const int n = out.H * out.W * C;
+    hipLaunchKernelGGL(k_pool_fwd, dim3((n + 255) / 256), dim3(256), 0, st, in,
+                       out, C);
+  };
+  pool(U->v(U->C0, 16), U->v(U->p0), 16);
+  conv_fwd<16, 32>(U, U->v(U->p0), U->v(U->a1), 2, 1);
+  conv_fwd<32, 32>(U, U->v(U->a1), U->v(U->C1, 32), 3, 1);
+  pool(U->v(U->C1, 32), U->v(U->p1), 32);
+  conv_fwd<32, 64>(U, U->v(U->p1), U->v(U->a2), 4, 1);
+  conv_fwd<64, 64>(U, U->v(U->a2), U->v(U->C2, 64), 5, 1);
+  pool(U->v(U->C2, 64), U->v(U->p2), 64);
+  conv_fwd<64, 128>(U, U->v(U->p2), U->v(U->a3), 6, 1);
+  conv_fwd<128, 128>(U, U->v(U->a3), U->v(U->b3), 7, 1);
+  up_fwd<128, 64>(U, U->v(U->b3), U->v(U->C2, 0), 8);
+  conv_fwd<128, 64>(U, U->v(U->C2), U->v(U->d2a), 9, 1);
+  conv_fwd<64, 64>(U, U->v(U->d2a), U->v(U->d2), 10, 1);
+  up_fwd<64, 32>(U, U->v(U->d2), U->v(U->C1, 0), 11);
+  conv_fwd<64, 32>(U, U->v(U->C1), U->v(U->d1a), 12, 1);
+  conv_fwd<32, 32>(U, U->v(U->d1a), U->v(U->d1), 13, 1);
+  up_fwd<32, 16>(U, U->v(U->d1), U->v(U->C0, 0), 14);
+  conv_fwd<32, 16>(U, U->v(U->C0), U->v(U->d0a), 15, 1);
+  conv_fwd<16, 16>(U, U->v(U->d0a), U->v(U->d0), 16, 1);
+  hipLaunchKernelGGL(k_out_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
+                     U->v(U->d0), U->params.p + U->woff[17],
+                     U->params.p + U->boff[17], U->y.p, M);
+  CPGPU_HIP(U->ctx, hipGetLastError());
+  return CPGPU_OK;
+}
+
+template <int CIN, int COUT>
+int conv_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, int in_co,
+             const cpgpu_unet::T& out, int out_co, int l, bool need_dx) {
+  hipStream_t st = U->ctx->stream;
+  GView gout = U->g(out, out_co);
+  View yout = U->v(out, out_co);
+  const int M = in.H * in.W;
+  if (need_dx) {
+    constexpr int NB = CIN / 16 < 4 ? CIN / 16 : 4;
+    dim3 grid((M + 64 * kMP - 1) / (64 * kMP), CIN / (16 * NB));
+    hipLaunchKernelGGL((k_conv3_dgrad<CIN, COUT, NB>), grid, dim3(256), 0, st,
+                       gout, yout, U->g(in, in_co), U->wtbf.p + U->wbf_off[l], 1);
+  }
+  const int nch = wg_chunks(M);
+  const int ppc = (M + nch - 1) / nch;
+  hipLaunchKernelGGL((k_wgrad<CIN, COUT, false>), dim3(nch, 9), dim3(256), 0, st,
+                     gout, yout, U->v(in, in_co), 1, ppc, U->part.p);
+  const int count = 9 * COUT * CIN;
+  hipLaunchKernelGGL(k_sum_chunks, dim3((count + 255) / 256), dim3(256), 0, st,
+                     U->part.p, nch, count, U->grads.p + U->woff[l]);
+  const int nb = 64;
+  hipLaunchKernelGGL(k_bias_grad, dim3(nb), dim3(256), 0, st, gout, yout, 1,
+                     COUT, U->part.p);
+  hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, st, U->part.p, nb,
+                     COUT, U->grads.p + U->boff[l]);
+  return CPGPU_OK;
+}
+
+template <int CIN, int COUT>
+int up_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, const cpgpu_unet::T& out,
+           int l) {
+  hipStream_t st = U->ctx->stream;
+  GView gout = U->g(out, 0);  // channels [0, COUT) of the concat gradient
+  const int M = in.H * in.W;
+  {
+    constexpr int NB = CIN / 16 < 4 ? CIN / 16 : 4;
+    dim3 grid((M + 64 * kMP - 1) / (64 * kMP), CIN / (16 * NB));
+    hipLaunchKernelGGL((k_up_dgrad<CIN, COUT, NB>), grid, dim3(256), 0, st,
+                       gout, U->g(in), U->wtbf.p + U->wbf_off[l]);
+  }
+  const int nch = wg_chunks(M);
+  const int ppc = (M + nch - 1) / nch;
+  hipLaunchKernelGGL((k_wgrad<CIN, COUT, true>), dim3(nch, 16), dim3(256), 0, st,
+                     gout, U->v(out), U->v(in), 0, ppc, U->part.p);
+  const int count = 16 * COUT * CIN;
+  hipLaunchKernelGGL(k_sum_chunks, dim3((count + 255) / 256), dim3(256), 0, st,
+                     U->part.p, nch, count, U->grads.p + U->woff[l]);
+  const int nb = 64;
+  hipLaunchKernelGGL(k_bias_grad, dim3(nb), dim3(256), 0, st, gout, U->v(out),
+                     0, COUT, U->part.p);
+  hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, st, U->part.p, nb,
+                     COUT, U->grads.p + U->boff[l]);
+  return CPGPU_OK;
+}
+
+// gradients of every parameter for dL/dy in U->dy (after forward)
+int backward(cpgpu_unet* U) {
+  hipStream_t st = U->ctx->stream;
+  const int M = U->H * U->W;
+  const int nb = 64;
+  hipLaunchKernelGGL(k_out_bwd, dim3(nb), dim3(256), 0, st, U->v(U->d0),
+                     U->params.p + U->woff[17], U->dy.p, U->g(U->d0), M,
+                     U->part.p);
+  hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, st, U->part.p, nb, 17,
+                     U->grads.p + U->woff[17]);  // 16 weights then the bias
+  conv_bwd<16, 16>(U, U->d0a, 0, U->d0, 0, 16, true);
+  conv_bwd<32, 16>(U, U->C0, 0, U->d0a, 0, 15, true);
+  up_bwd<32, 16>(U, U->d1, U->C0, 14);
+  conv_bwd<32, 32>(U, U->d1a, 0, U->d1, 0, 13, true);
+  conv_bwd<64, 32>(U, U->C1, 0, U->d1a, 0, 12, true);
+  up_bwd<64, 32>(U, U->d2, U->C1, 11);
+  conv_bwd<64, 64>(U, U->d2a, 0, U->d2, 0, 10, true);
+  conv_bwd<128, 64>(U, U->C2, 0, U->d2a, 0, 9, true);
+  up_bwd<128, 64>(U, U->b3, U->C2, 8);
+  conv_bwd<128, 128>(U, U->a3, 0, U->b3, 0, 7, true);
+  conv_bwd<64, 128>(U, U->p2, 0, U->a3, 0, 6, true);
+  auto unpool = [&](const cpgpu_unet::T& coarse, const cpgpu_unet::T& fine,
+                    int fco, int C) {
+    const int n = fine.H * fine.W * C;
+    hipLaunchKernelGGL(k_pool_bwd, dim3((n + 255) / 256), dim3(256), 0, st,
+                       U->g(coarse), U->g(fine, fco), C);
+  };
+  unpool(U->p2, U->C2, 64, 64);
+  conv_bwd<64, 64>(U, U->a2, 0, U->C2, 64, 5, true);
+  conv_bwd<32, 64>(U, U->p1, 0, U->a2, 0, 4, true);
+  unpool(U->p1, U->C1, 32, 32);
+  conv_bwd<32, 32>(U, U->a1, 0, U->C1, 32, 3, true);
+  conv_bwd<16, 32>(U, U->p0, 0, U->a1, 0, 2, true);
+  unpool(U->p0, U->C0, 16, 16);
+  conv_bwd<16, 16>(U, U->a0, 0, U->C0, 16, 1, true);
+  const int nbf = 64;
+  hipLaunchKernelGGL(k_first_wgrad, dim3(nbf), dim3(256), 0, st, U->x0.p, U->H,
+                     U->W, U->g(U->a0), U->v(U->a0), U->part.p);
+  hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, st, U->part.p, nbf,
+                     160, U->grads.p + U->woff[0]);  // 144 weights then 16 biases
+  CPGPU_HIP(U->ctx, hipGetLastError());
+  return CPGPU_OK;
+}
+
+int pack(cpgpu_unet* U) {
+  for (int l = 0; l < kNL; ++l) {
+    const LayerDef& d = kLayers[l];
+    if (d.kind != K_CONV && d.kind != K_UP) continue;
+    const int n = ntaps(d.kind) * d.cout * d.cin;
+    hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0,
+                       U->ctx->stream, U->params.p + U->woff[l],
+                       U->wbf.p + U->wbf_off[l], U->wtbf.p + U->wbf_off[l],
+                       ntaps(d.kind), d.cout, d.cin);
+  }
+  CPGPU_HIP(U->ctx, hipGetLastError());
+  return CPGPU_OK;
+}
+
+// x0 = in / rms(in); forward; out = rms(in) * y.  Device FP32 arrays.
+int apply_scaled(cpgpu_unet* U, const float* in, float* out, size_t N) {
+  hipStream_t st = U->ctx->stream;
+  hipLaunchKernelGGL(k_sumsq, dim3(1024), dim3(256), 0, st, in, N, U->dpart.p);
+  hipLaunchKernelGGL(k_rms, dim3(1), dim3(256), 0, st, U->dpart.p, 1024, N,
+                     U->scale.p);
+  hipLaunchKernelGGL(k_scale_copy, dim3(1024), dim3(256), 0, st, in, U->x0.p,
+                     U->scale.p, 1, 1.f, N);
+  int rc = forward(U);
+  if (rc) return rc;
+  if (out)
+    hipLaunchKernelGGL(k_scale_copy, dim3(1024), dim3(256), 0, st, U->y.p, out,
+                       U->scale.p, 0, 1.f, N);
+  CPGPU_HIP(U->ctx, hipGetLastError());
+  return CPGPU_OK;
+}
+
+int check_problem(cpgpu_unet* U, cpgpu_poisson* P) {
+  if (P->ctx != U->ctx)
+    return set_error(U->ctx, CPGPU_EINVAL, "U-Net and problem use different contexts");
+  if (!U->samples.empty() && (P->nx != U->sH || P->ny != U->sW))
+    return set_error(U->ctx, CPGPU_EINVAL,
+                     "problem grid %dx%d differs from the samples' %dx%d", P->nx,
+                     P->ny, U->sH, U->sW);
+  int rc = unet_alloc(U, P->nx, P->ny);
+  if (rc) return rc;
+  return poisson_prepare_f32(P);
+}
+
+// forward on sample slot and its energy-norm loss into *lossout (device);
+// with dy_scale > 0 also dL/dy * dy_scale into U->dy
+int sample_loss(cpgpu_unet* U, cpgpu_poisson* P, int slot, float dy_scale,
+                float* lossout) {
+  hipStream_t st = U->ctx->stream;
+  cpgpu_unet::Sample& S = *U->samples[slot];
+  const size_t N = (size_t)U->H * U->W;
+  int rc = apply_scaled(U, S.r.p, nullptr, N);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_loss_d, dim3(1024), dim3(256), 0, st, U->y.p, S.e.p,
+                     U->scale.p, U->dbuf.p, N);
+  hipLaunchKernelGGL(k_apply<float>, dim3(P->tl.G), dim3(TJ, BY), 0, st,
+                     P->sten32(), U->dbuf.p, U->qbuf.p, P->tl);
+  const float c = (float)(2.0 / S.eAe) * dy_scale;
+  hipLaunchKernelGGL(k_loss_q, dim3(1024), dim3(256), 0, st, U->dbuf.p,
+                     U->qbuf.p, U->scale.p, c, dy_scale > 0.f ? U->dy.p : nullptr,
+                     U->lpart.p, N);
+  hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(256), 0, st, U->lpart.p, 1024,
+                     S.eAe, lossout);
+  CPGPU_HIP(U->ctx, hipGetLastError());
+  return CPGPU_OK;
+}
+
+int adam_step(cpgpu_unet* U, float lr, const float* g) {
+  U->adam_t++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)U->adam_t);
+  const float c2 = 1.f - std::pow(b2, (float)U->adam_t);
+  hipLaunchKernelGGL(k_adam, dim3(256), dim3(256), 0, U->ctx->stream,
+                     U->params.p, g, U->m1.p, U->m2.p, U->nparams, lr, b1, b2,
+                     1e-8f, c1, c2);
+  CPGPU_HIP(U->ctx, hipGetLastError());
+  return pack(U);
+}
+
+// SplitMix64 stream for sample selection
+struct SplitMix {
+  unsigned long long x;
+  unsigned long long next() {
+    unsigned long long z = (x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// internal entry points used by the learned solver
+namespace cpgpu {
+int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out) {
+  int rc = unet_alloc(U, H, W);
+  if (rc) return rc;
+  return apply_scaled(U, in, out, (size_t)H * W);
+}
+cpgpu_ctx* unet_context(const cpgpu_unet* U) { return U->ctx; }
+}  // namespace cpgpu
+
+extern "C" {
+
+int cpgpu_unet_create(cpgpu_ctx* ctx, unsigned long long seed,
+                      cpgpu_unet** out) {
+  if (!ctx || !out) return CPGPU_EINVAL;
+  *out = nullptr;
+  CPGPU_HIP(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<cpgpu_unet> U(new cpgpu_unet());
+  U->ctx = ctx;
+  size_t off = 0, boff = 0;
+  for (int l = 0; l < kNL; ++l) {
+    const LayerDef& d = kLayers[l];
+    const size_t nw = (size_t)ntaps(d.kind) * d.cout * d.cin;
+    U->woff[l] = off;
+    off += nw;
+    U->boff[l] = off;
+    off += d.cout;
+    U->wbf_off[l] = boff;
+    if (d.kind == K_CONV || d.kind == K_UP) boff += nw;
+  }
+  U->nparams = off;
+  U->nwbf = boff;
+  if (U->params.alloc(off) || U->grads.alloc(off) || U->m1.alloc(off) ||
+      U->m2.alloc(off) || U->wbf.alloc(boff) || U->wtbf.alloc(boff) ||
+      U->scale.alloc(4) ||
+      U->dpart.alloc(1024))
+    return set_error(ctx, CPGPU_ENOMEM, "U-Net parameters");
+  // Kaiming-normal weights (fan_in, gain sqrt 2; gain 1 for the linear
+  // output layer), zero biases, from xoshiro256** seeded by SplitMix64
+  std::vector<float> h(off, 0.f);
+  {
+    unsigned long long s[4], x = seed;
+    for (int k = 0; k < 4; ++k) {
+      unsigned long long z = (x += 0x9E3779B97F4A7C15ULL);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+      s[k] = z ^ (z >> 31);
+    }
+    auto rotl = [](unsigned long long v, int k) { return (v << k) | (v >> (64 - k)); };
+    auto next = [&]() {
+      const unsigned long long r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+      s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t;
+      s[3] = rotl(s[3], 45);
+      return r;
+    };
+    auto unif = [&]() { return (double)(next() >> 11) * 0x1.0p-53; };
+    for (int l = 0; l < kNL; ++l) {
+      const LayerDef& d = kLayers[l];
+      const double fan_in = d.kind == K_UP ? d.cin : (double)ntaps(d.kind) * d.cin;
+      const double gain = d.kind == K_OUT ? 1.0 : std::sqrt(2.0);
+      const double sd = gain / std::sqrt(fan_in);
+      const size_t nw = (size_t)ntaps(d.kind) * d.cout * d.cin;
+      for (size_t k = 0; k < nw; ++k) {
+        const double u1 = 1.0 - unif(), u2 = unif();
+        h[U->woff[l] + k] = (float)(sd * std::sqrt(-2.0 * std::log(u1)) *
+                                    std::cos(6.283185307179586 * u2));
+      }
+    }
+  }
+  // the context stream is non-blocking: every transfer goes on it
+  CPGPU_HIP(ctx, hipMemcpyAsync(U->params.p, h.data(), sizeof(float) * off,
+                                hipMemcpyHostToDevice, ctx->stream));
+  CPGPU_HIP(ctx, hipMemsetAsync(U->m1.p, 0, sizeof(float) * off, ctx->stream));
+  CPGPU_HIP(ctx, hipMemsetAsync(U->m2.p, 0, sizeof(float) * off, ctx->stream));
+  CPGPU_HIP(ctx, hipMemsetAsync(U->grads.p, 0, sizeof(float) * off, ctx->stream));
+  int rc = pack(U.get());
+  if (rc) return rc;
+  CPGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *out = U.release();
+  return CPGPU_OK;
+}
+
+int cpgpu_unet_destroy(cpgpu_unet* U) {
+  if (!U) return CPGPU_EINVAL;
+  (void)hipStreamSynchronize(U->ctx->stream);
+  delete U;
+  return CPGPU_OK;
+}
+
+long long cpgpu_unet_num_params(const cpgpu_unet* U) {
+  return U ? (long long)U->nparams : -1;
+}
+
+int cpgpu_unet_get_params(cpgpu_unet* U, float* host, int grads) {
+  if (!U || !host) return CPGPU_EINVAL;
+  CPGPU_HIP(U->ctx, hipStreamSynchronize(U->ctx->stream));
+  CPGPU_HIP(U->ctx, hipMemcpy(host, grads ? U->grads.p : U->params.p,
+                              sizeof(float) * U->nparams, hipMemcpyDeviceToHost));
+  return CPGPU_OK;
+}
+
+int cpgpu_unet_set_params(cpgpu_unet* U, const float* host) {
+  if (!U || !host) return CPGPU_EINVAL;
+  CPGPU_HIP(U->ctx, hipMemcpyAsync(U->params.p, host, sizeof(float) * U->nparams,
+                                   hipMemcpyHostToDevice, U->ctx->stream));
+  int rc = pack(U);
+  if (rc) return rc;
+  CPGPU_HIP(U->ctx, hipStreamSynchronize(U->ctx->stream));
+  return CPGPU_OK;
+}
+
+// Host in/out forward of the raw network (no input scaling): x [H*W] ->
+// y [H*W].  For tests.
+int cpgpu_unet_forward(cpgpu_unet* U, int H, int W, const float* x, float* y) {
+  if (!U || !x || !y) return CPGPU_EINVAL;
+  int rc = unet_alloc(U, H, W);
+  if (rc) return rc;
+  const size_t N = (size_t)H * W;
+  CPGPU_HIP(U->ctx, hipMemcpyAsync(U->x0.p, x, sizeof(float) * N,
+                                   hipMemcpyHostToDevice, U->ctx->stream));
+  if ((rc = forward(U))) return rc;
+  CPGPU_HIP(U->ctx, hipMemcpyAsync(y, U->y.p, sizeof(float) * N,
+                                   hipMemcpyDeviceToHost, U->ctx->stream));
+  CPGPU_HIP(U->ctx, hipStreamSynchronize(U->ctx->stream));
+  return CPGPU_OK;
+}
+
+// Internal tensor `id` of the last forward (grad = 0, BF16 widened to
+// FP32) or backward (grad != 0, FP32) as H x W x C into host (may be NULL:
+// dims only).  ids: 0 input (FP32), 1 a0, 2 C0, 3 d0a, 4 d0, 5 p0, 6 a1,
+// 7 C1, 8 d1a, 9 d1, 10 p1, 11 a2, 12 C2, 13 d2a, 14 d2, 15 p2, 16 a3,
+// 17 b3, 18 output (FP32).  For layer-by-layer tests.
+int cpgpu_unet_tensor(cpgpu_unet* U, int id, int grad, float* host, int* dims) {
+  if (!U || !dims || id < 0 || id > 18 || U->H == 0) return CPGPU_EINVAL;
+  const size_t N = (size_t)U->H * U->W;
+  CPGPU_HIP(U->ctx, hipStreamSynchronize(U->ctx->stream));
+  if (id == 0 || id == 18) {
+    if (grad && id == 0) return CPGPU_EINVAL;
+    dims[0] = U->H;
+    dims[1] = U->W;
+    dims[2] = 1;
+    const float* src = id == 0 ? U->x0.p : (grad ? U->dy.p : U->y.p);
+    if (host)
+      CPGPU_HIP(U->ctx, hipMemcpy(host, src, sizeof(float) * N, hipMemcpyDeviceToHost));
+    return CPGPU_OK;
+  }
+  const cpgpu_unet::T* tab[] = {nullptr, &U->a0, &U->C0, &U->d0a, &U->d0, &U->p0,
+                                &U->a1, &U->C1, &U->d1a, &U->d1, &U->p1, &U->a2,
+                                &U->C2, &U->d2a, &U->d2, &U->p2, &U->a3, &U->b3};
+  const cpgpu_unet::T& t = *tab[id];
+  dims[0] = t.H;
+  dims[1] = t.W;
+  dims[2] = t.C;
+  if (!host) return CPGPU_OK;
+  const size_t n = (size_t)t.H * t.W * t.C;
+  if (grad) {
+    CPGPU_HIP(U->ctx, hipMemcpy(host, U->gact.p + t.off, sizeof(float) * n,
+                                hipMemcpyDeviceToHost));
+  } else {
+    std::vector<bf16> h(n);
+    CPGPU_HIP(U->ctx, hipMemcpy(h.data(), U->act.p + t.off, sizeof(bf16) * n,
+                                hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < n; ++k) host[k] = bf2f(h[k]);
+  }
+  return CPGPU_OK;
+}
+
+// Gradients of sum(dy * y) w.r.t. every parameter after cpgpu_unet_forward
+// on the same input; grads (host, num_params) receives them.  For tests.
+int cpgpu_unet_backward(cpgpu_unet* U, const float* dy, float* grads) {
+  if (!U || !dy || !grads || U->H == 0) return CPGPU_EINVAL;
+  const size_t N = (size_t)U->H * U->W;
+  CPGPU_HIP(U->ctx, hipMemcpyAsync(U->dy.p, dy, sizeof(float) * N,
+                                   hipMemcpyHostToDevice, U->ctx->stream));
+  int rc = backward(U);
+  if (rc) return rc;
+  return cpgpu_unet_get_params(U, grads, 1);
+}
+
+// One Adam step on the current gradients.
+int cpgpu_unet_adam(cpgpu_unet* U, float lr) {
+  if (!U) return CPGPU_EINVAL;
+  return adam_step(U, lr, U->grads.p);
+}
+
+int cpgpu_unet_set_sample(cpgpu_unet* U, cpgpu_poisson* P, int slot,
+                          const double* e) {
+  if (!U || !P || !e || slot < 0 || slot > (1 << 20)) return CPGPU_EINVAL;
+  int rc = check_problem(U, P);
+  if (rc) return rc;
+  const size_t N = P->N;
+  if ((size_t)slot >= U->samples.size()) U->samples.resize(slot + 1);
+  if (!U->samples[slot]) U->samples[slot].reset(new cpgpu_unet::Sample());
+  cpgpu_unet::Sample& S = *U->samples[slot];
+  if (S.r.alloc(N) || S.e.alloc(N) || U->tmp64.alloc(N) ||
+      U->lpart.alloc(kMaxBlocks + 1))
+    return set_error(U->ctx, CPGPU_ENOMEM, "U-Net sample");
+  U->sH = P->nx;
+  U->sW = P->ny;
+  hipStream_t st = U->ctx->stream;
+  CPGPU_HIP(U->ctx, hipMemcpyAsync(U->tmp64.p, e, sizeof(double) * N,
+                                   hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_sample, dim3(P->tl.G), dim3(TJ, BY), 0, st, P->sten64(),
+                     U->tmp64.p, S.r.p, S.e.p, U->lpart.p, P->tl);
+  hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, st, U->lpart.p,
+                     (int)P->tl.G, U->lpart.p + kMaxBlocks);
+  CPGPU_HIP(U->ctx, hipGetLastError());
+  double eae = 0.0;
+  CPGPU_HIP(U->ctx, hipMemcpyAsync(&eae, U->lpart.p + kMaxBlocks, sizeof(double),
+                                   hipMemcpyDeviceToHost, st));
+  CPGPU_HIP(U->ctx, hipStreamSynchronize(st));
+  if (!(eae > 0.0))
+    return set_error(U->ctx, CPGPU_EINVAL, "sample %d has e'Ae = %g (need > 0)",
+                     slot, eae);
+  S.eAe = eae;
+  return CPGPU_OK;
+}
+
+int cpgpu_unet_num_samples(const cpgpu_unet* U) {
+  return U ? (int)U->samples.size() : -1;
+}
+
+static int check_samples(cpgpu_unet* U) {
+  if (U->samples.empty())
+    return set_error(U->ctx, CPGPU_EINVAL, "no training samples");
+  for (size_t k = 0; k < U->samples.size(); ++k)
+    if (!U->samples[k])
+      return set_error(U->ctx, CPGPU_EINVAL, "sample slot %zu is empty", k);
+  return CPGPU_OK;
+}
+
+static int loss_buffers(cpgpu_unet* U, size_t nloss) {
+  const size_t N = (size_t)U->H * U->W;
+  if (U->dbuf.alloc(N) || U->qbuf.alloc(N) || U->gacc.alloc(U->nparams) ||
+      U->lossdev.alloc(nloss) || U->lpart.alloc(kMaxBlocks + 1))
+    return set_error(U->ctx, CPGPU_ENOMEM, "U-Net training buffers");
+  return CPGPU_OK;
+}
+
+int cpgpu_unet_train(cpgpu_unet* U, cpgpu_poisson* P, int steps, int batch,
+                     float lr, unsigned long long seed, float* losses) {
+  if (!U || !P || steps < 0 || batch < 1 || !(lr > 0.f)) return CPGPU_EINVAL;
+  int rc = check_samples(U);
+  if (rc) return rc;
+  if ((rc = check_problem(U, P))) return rc;
+  if ((rc = loss_buffers(U, (size_t)std::max(steps, 1) * batch))) return rc;
+  hipStream_t st = U->ctx->stream;
+  SplitMix g{seed};
+  const int ns = (int)U->samples.size();
+  for (int s = 0; s < steps; ++s) {
+    for (int b = 0; b < batch; ++b) {
+      const int slot = (int)(g.next() % (unsigned long long)ns);
+      if ((rc = sample_loss(U, P, slot, 1.f / batch,
+                            U->lossdev.p + (size_t)s * batch + b)))
+        return rc;
+      if ((rc = backward(U))) return rc;
+      hipLaunchKernelGGL(k_accum, dim3(256), dim3(256), 0, st, U->gacc.p,
+                         U->grads.p, U->nparams, b == 0 ? 1 : 0);
+    }
+    if ((rc = adam_step(U, lr, U->gacc.p))) return rc;
+  }
+  if (losses && steps > 0) {
+    std::vector<float> h((size_t)steps * batch);
+    CPGPU_HIP(U->ctx, hipMemcpyAsync(h.data(), U->lossdev.p, sizeof(float) * h.size(),
+                                     hipMemcpyDeviceToHost, st));
+    CPGPU_HIP(U->ctx, hipStreamSynchronize(st));
+    for (int s = 0; s < steps; ++s) {
+      double m = 0;
+      for (int b = 0; b < batch; ++b) m += h[(size_t)s * batch + b];
+      losses[s] = (float)(m / batch);
+    }
+  }
+  CPGPU_HIP(U->ctx, hipStreamSynchronize(st));
+  return CPGPU_OK;
+}
+
+int cpgpu_unet_eval(cpgpu_unet* U, cpgpu_poisson* P, float* losses) {
+  if (!U || !P || !losses) return CPGPU_EINVAL;
+  int rc = check_samples(U);
+  if (rc) return rc;
+  if ((rc = check_problem(U, P))) return rc;
+  const int ns = (int)U->samples.size();
+  if ((rc = loss_buffers(U, ns))) return rc;
+  for (int k = 0; k < ns; ++k)
+    if ((rc = sample_loss(U, P, k, 0.f, U->lossdev.p + k))) return rc;
+  CPGPU_HIP(U->ctx, hipMemcpyAsync(losses, U->lossdev.p, sizeof(float) * ns,
+                                   hipMemcpyDeviceToHost, U->ctx->stream));
+  CPGPU_HIP(U->ctx, hipStreamSynchronize(U->ctx->stream));
+  return CPGPU_OK;
+}
+
+}  // extern "C"

@@ -45,6 +45,8 @@ extern "C" {
 #define CPGPU_JACOBI 3    /* carpio::Solver_Jacobi_, solver_matrix.hpp:820-910 */
 #define CPGPU_MIXED 4     /* FP32 J-PCG with FP64 residual replacement (new) */
 #define CPGPU_MIXED_RESTART 5 /* FP32 J-PCG restarted inside FP64 refinement (new) */
+#define CPGPU_FCG_UNET 6  /* FP64 flexible CG with the U-Net preconditioner,
+                             J-PCG fallback (new; cpgpu_poisson_set_unet) */
 
 /* ---- stopping-norm modes ---- */
 #define CPGPU_NORM_REF 0  /* carpio::nrm2: drops the last unknown (BLAS_Level1.hpp:569) */
@@ -230,6 +232,66 @@ int cpgpu_ns_step(cpgpu_ns *S, int nsteps, int *iters, double *resid);
 int cpgpu_ns_time(cpgpu_ns *S, double *t, double *dt);
 /* The projection problem (all-Neumann Poisson) of the step. */
 cpgpu_poisson *cpgpu_ns_projection(cpgpu_ns *S);
+
+/* ---- learned preconditioner: a BF16 U-Net on MFMA ----
+ * No counterpart in the reference: its Poisson solvers are the CSR Krylov
+ * methods of algebra/solver_matrix.hpp (Jacobi, CG, BiCGSTAB, IC-CG ...).
+ * This is the extension those solvers are selected beside, by name, in
+ * Poisson_::_init_solver (s_poisson.hpp:357-404).
+ *
+ * Network: 4 levels of 16-32-64-128 channels, two 3x3 convolutions + ReLU
+ * per level, 4x4 average pooling (stride 4), 4x4 stride-4 transposed
+ * convolutions, skip connections by concatenation, 1x1 linear output.
+ * Activations are NHWC BF16, accumulation FP32, master parameters FP32.
+ * A field is H = nx rows of W = ny contiguous values (the layout of
+ * cpgpu_poisson_device_vec); H and W must be multiples of 64. */
+typedef struct cpgpu_unet cpgpu_unet;
+
+/* Kaiming-normal weights (fan-in; gain sqrt(2), 1 for the output layer),
+ * zero biases, from xoshiro256** seeded through SplitMix64 by seed. */
+int cpgpu_unet_create(cpgpu_ctx *ctx, unsigned long long seed,
+                      cpgpu_unet **out);
+int cpgpu_unet_destroy(cpgpu_unet *U);
+long long cpgpu_unet_num_params(const cpgpu_unet *U);
+/* Flat FP32 parameter vector (grads != 0: the last computed gradients);
+ * per layer: weights [tap][cout][cin], then biases [cout]. */
+int cpgpu_unet_get_params(cpgpu_unet *U, float *host, int grads);
+int cpgpu_unet_set_params(cpgpu_unet *U, const float *host);
+/* y = net(x) on host arrays of H*W values (no input scaling). */
+int cpgpu_unet_forward(cpgpu_unet *U, int H, int W, const float *x, float *y);
+/* Gradients of sum(dy * y) after cpgpu_unet_forward into grads (host,
+ * num_params values). */
+int cpgpu_unet_backward(cpgpu_unet *U, const float *dy, float *grads);
+/* Internal tensor of the last forward (grad = 0; BF16 widened to FP32) or
+ * backward (grad != 0; FP32 gradient) as H x W x C floats into host (NULL:
+ * only dims[3] = {H, W, C}).  id: 0 input, 1 a0, 2 C0 (up-conv | skip), 3
+ * d0a, 4 d0, 5 p0, 6 a1, 7 C1, 8 d1a, 9 d1, 10 p1, 11 a2, 12 C2, 13 d2a,
+ * 14 d2, 15 p2, 16 a3, 17 b3, 18 output.  For layer-by-layer tests. */
+int cpgpu_unet_tensor(cpgpu_unet *U, int id, int grad, float *host, int *dims);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on the last gradients. */
+int cpgpu_unet_adam(cpgpu_unet *U, float lr);
+
+/* Training set for the problem P: slot k holds an error e (host, interior
+ * H*W doubles) and its residual r = A e, computed on the device.  Slots
+ * grow on demand; all samples share P's grid. */
+int cpgpu_unet_set_sample(cpgpu_unet *U, cpgpu_poisson *P, int slot,
+                          const double *e);
+int cpgpu_unet_num_samples(const cpgpu_unet *U);
+/* steps Adam updates, each on the mean energy-norm loss of batch samples
+ * drawn with a seeded generator:  L = (s y - e)' A (s y - e) / e' A e,
+ * y = net(r / s), s = rms(r).  losses (may be NULL) receives each step's
+ * mean loss. */
+int cpgpu_unet_train(cpgpu_unet *U, cpgpu_poisson *P, int steps, int batch,
+                     float lr, unsigned long long seed, float *losses);
+/* Loss of every sample (num_samples values) without updating. */
+int cpgpu_unet_eval(cpgpu_unet *U, cpgpu_poisson *P, float *losses);
+
+/* Attach a network to a problem (NULL detaches).  flags: CPGPU_UNET_WARM
+ * starts every solve from phi = s net(b / s); CPGPU_UNET_PRECOND makes
+ * CPGPU_FCG_UNET use it as the flexible-CG preconditioner. */
+#define CPGPU_UNET_WARM 1
+#define CPGPU_UNET_PRECOND 2
+int cpgpu_poisson_set_unet(cpgpu_poisson *P, cpgpu_unet *U, int flags);
 
 /* ---- kernel timing (HIP events on the context stream) ----
  * When enabled, every launch of the named kernel family is bracketed by

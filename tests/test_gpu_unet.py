@@ -1,0 +1,316 @@
+"""The HIP U-Net (forward, backward, Adam, training loss) against the numpy
+oracle, and the learned flexible-CG solver against Jacobi-PCG."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import carpios_amd as ca
+import oracle as O
+import oracle.unet as OU
+from carpios_amd import learned
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def random_params(seed):
+    rng = np.random.default_rng(seed)
+    lay, n = OU.layout()
+    p = np.zeros(n, dtype=np.float32)
+    for (wo, bo, shape), (kind, ci, co) in zip(lay, OU.LAYERS):
+        fan = ci if kind == "up" else OU.ntaps(kind) * ci
+        nw = int(np.prod(shape))
+        p[wo:wo + nw] = rng.standard_normal(nw) * np.sqrt(2.0 / fan)
+        p[bo:bo + co] = 0.05 * rng.standard_normal(co)
+    return p
+
+
+def test_init_matches_the_oracle_generator(ctx):
+    net = ca.UNet(ctx, seed=1)
+    assert net.num_params == OU.layout()[1]
+    p = net.get_params()
+    lay, _ = OU.layout()
+    ref = OU.kaiming_init(1, layers=3)
+    end = lay[2][1] + OU.LAYERS[2][2]
+    assert np.array_equal(p[:end], ref[:end])
+    for (wo, bo, shape), (kind, ci, co) in zip(lay, OU.LAYERS):
+        nw = int(np.prod(shape))
+        fan = ci if kind == "up" else OU.ntaps(kind) * ci
+        sd = (1.0 if kind == "out" else np.sqrt(2.0)) / np.sqrt(fan)
+        if nw >= 2000:
+            assert abs(p[wo:wo + nw].std() / sd - 1) < 0.05, kind
+        assert not p[bo:bo + co].any()
+    net.close()
+
+
+@pytest.mark.parametrize("H,W", [(64, 64), (128, 192)])
+def test_forward_and_backward_match_the_oracle(ctx, H, W):
+    say(f"unet fwd/bwd parity {H}x{W}")
+    net = ca.UNet(ctx, seed=1)
+    p = random_params(10 + H)
+    net.set_params(p)
+    rng = np.random.default_rng(H + W)
+    x = rng.standard_normal((H, W)).astype(np.float32)
+    y = net.forward(x)
+    yo, c = OU.forward(p.astype(np.float64), x, bf=True)
+    y64, c64 = OU.forward(p.astype(np.float64), x, bf=False)
+    e_bf, e_64 = rel_l2(y, yo), rel_l2(y, y64)
+    say(f"  forward rel L2: vs BF16 oracle {e_bf:.2e}, vs FP64 oracle {e_64:.2e}")
+    # A BF16 network is sensitive to rounding: a relative input perturbation
+    # of 1e-6 already moves its output by about 5e-3 (rel L2), so device
+    # and oracle agree to that level; a wrong tap, transposed weight or
+    # channel offset gives O(1) differences.
+    assert e_bf <= 2e-2
+    assert e_64 <= 5e-2
+    dy = rng.standard_normal((H, W)).astype(np.float32)
+    g = net.backward(dy)
+    go = OU.backward(c, dy)
+    g64 = OU.backward(c64, dy)
+    lay, _ = OU.layout()
+    worst = 0.0
+    for k, (wo, bo, shape) in enumerate(lay):
+        co = OU.LAYERS[k][2]
+        nw = int(np.prod(shape))
+        for lo, hi, what in ((wo, wo + nw, "w"), (bo, bo + co, "b")):
+            e = rel_l2(g[lo:hi], go[lo:hi])
+            worst = max(worst, e)
+            # end to end the BF16 rounding noise compounds (the layer-by-layer
+            # test below is the strict check of every kernel)
+            assert e <= 0.2, (k, what, e)
+            # BF16 rounding of the gradients compounds towards the input
+            # layers; the exact float64 gradient stays within a few percent
+            # to ~10% there
+            assert rel_l2(g[lo:hi], g64[lo:hi]) <= 0.25, (k, what)
+    say(f"  backward worst per-layer rel L2 vs BF16 oracle {worst:.2e}")
+    net.close()
+
+
+# (kind, input tensor, input channels, output tensor, output channels)
+LAYER_IO = [
+    ("first", "x", (0, 1), "a0", (0, 16)), ("conv", "a0", (0, 16), "C0", (16, 32)),
+    ("conv", "p0", (0, 16), "a1", (0, 32)), ("conv", "a1", (0, 32), "C1", (32, 64)),
+    ("conv", "p1", (0, 32), "a2", (0, 64)), ("conv", "a2", (0, 64), "C2", (64, 128)),
+    ("conv", "p2", (0, 64), "a3", (0, 128)), ("conv", "a3", (0, 128), "b3", (0, 128)),
+    ("up", "b3", (0, 128), "C2", (0, 64)), ("conv", "C2", (0, 128), "d2a", (0, 64)),
+    ("conv", "d2a", (0, 64), "d2", (0, 64)), ("up", "d2", (0, 64), "C1", (0, 32)),
+    ("conv", "C1", (0, 64), "d1a", (0, 32)), ("conv", "d1a", (0, 32), "d1", (0, 32)),
+    ("up", "d1", (0, 32), "C0", (0, 16)), ("conv", "C0", (0, 32), "d0a", (0, 16)),
+    ("conv", "d0a", (0, 16), "d0", (0, 16)), ("out", "d0", (0, 16), "y", (0, 1))]
+# input gradients that nothing else adds to after the layer's own dgrad
+EXACT_DX = {16: "d0a", 14: "d1", 13: "d1a", 11: "d2", 10: "d2a", 8: "b3",
+            7: "a3", 6: "p2", 5: "a2", 4: "p1", 3: "a1", 2: "p0", 1: "a0"}
+
+
+def test_layerwise_parity_on_device_activations(ctx):
+    """Every layer recomputed by the oracle from the device's own inputs
+    (teacher forcing), so BF16 rounding cannot compound across layers:
+    forward outputs, weight and bias gradients and input gradients agree to
+    FP32 accumulation order (plus rare one-ulp BF16 rounding flips)."""
+    H, W = 128, 192
+    say(f"unet layer-by-layer parity {H}x{W}")
+    net = ca.UNet(ctx, seed=1)
+    p = random_params(77).astype(np.float32)
+    net.set_params(p)
+    ws, bs = OU.unpack(p.astype(np.float64))
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((H, W)).astype(np.float32)
+    net.forward(x)
+    dy = rng.standard_normal((H, W)).astype(np.float32)
+    g = net.backward(dy)
+    gw, gb = OU.unpack(g.astype(np.float64))
+    act = {n: net.tensor(n).astype(np.float64) for n in ca.UNet.TENSORS}
+    grd = {n: net.tensor(n, grad=True).astype(np.float64) for n in ca.UNet.TENSORS[1:]}
+    R = OU.bf16
+    worst = {}
+    for k, (kind, nin, (i0, i1), nout, (o0, o1)) in enumerate(LAYER_IO):
+        xin = act[nin][:, :, i0:i1]
+        yout = act[nout][:, :, o0:o1]
+        w, b = ws[k], bs[k]
+        if kind == "first":
+            yo = R(np.maximum(OU.conv3(xin, w, b), 0))
+        elif kind == "conv":
+            yo = R(np.maximum(OU.conv3(xin, R(w), b), 0))
+        elif kind == "up":
+            yo = R(OU.up4(xin, R(w), b))
+        else:
+            yo = xin @ w[0, 0][:, None] + b[0]
+        ef = rel_l2(yout, yo)
+        # backward of the layer from the device's output gradient
+        gout = dy[:, :, None].astype(np.float64) if kind == "out" else grd[nout][:, :, o0:o1]
+        if kind == "out":
+            dw_o = (gout * xin).sum(axis=(0, 1)).reshape(1, 1, 16)
+            db_o = gout.sum(axis=(0, 1))
+            dx_o = None
+        elif kind == "up":
+            gq = R(gout)
+            dw_o, db_o, dx_o = OU.up4_dw(gq, xin), gout.sum(axis=(0, 1)), OU.up4_dx(gq, R(w))
+        else:
+            gm = gout * (yout > 0)
+            gq = gm if kind == "first" else R(gm)
+            dw_o, db_o = OU.conv3_dw(gq, xin), gm.sum(axis=(0, 1))
+            dx_o = None if kind == "first" else OU.conv3_dx(gq, R(w))
+        ew, eb = rel_l2(gw[k], dw_o), rel_l2(gb[k], db_o)
+        ex = 0.0
+        if k in EXACT_DX:
+            ex = rel_l2(grd[EXACT_DX[k]][:, :, i0:i1], dx_o)
+        worst[k] = (ef, ew, eb, ex)
+        assert ef <= 1e-3 and ew <= 1e-3 and eb <= 1e-3 and ex <= 1e-3, (k, kind, worst[k])
+    # skip channels: dgrad of the consumer plus the pooling backward
+    for big, lo, hi, pooled in (("C0", 16, 32, "p0"), ("C1", 32, 64, "p1"), ("C2", 64, 128, "p2")):
+        k = [j for j, l in enumerate(LAYER_IO) if l[1] == big][0]
+        gm = grd[LAYER_IO[k][3]] * (act[LAYER_IO[k][3]] > 0)
+        dx = OU.conv3_dx(R(gm), R(ws[k]))
+        ref = dx[:, :, lo:hi] + OU.pool4_dx(grd[pooled])
+        assert rel_l2(grd[big][:, :, lo:hi], ref) <= 1e-3, big
+        assert rel_l2(act[pooled], R(OU.pool4(act[big][:, :, lo:hi]))) <= 1e-3, pooled
+    say("  worst per layer (fwd, dW, db, dx): " +
+        " ".join(f"{k}:{max(v):.1e}" for k, v in worst.items()))
+    net.close()
+
+
+def test_adam_step_matches_the_oracle(ctx):
+    net = ca.UNet(ctx, seed=3)
+    p0 = net.get_params()
+    rng = np.random.default_rng(4)
+    net.forward(rng.standard_normal((64, 64)).astype(np.float32))
+    g = net.backward(rng.standard_normal((64, 64)).astype(np.float32))
+    net.adam(2e-3)
+    p1 = net.get_params()
+    z = np.zeros_like(p0)
+    pr, _, _ = OU.adam(p0, g, z, z, 1, 2e-3)
+    bad = ~np.isclose(p1, pr, rtol=1e-6, atol=1e-9)
+    for k in np.flatnonzero(bad)[:8]:
+        say(f"  adam mismatch [{k}]: p0 {p0[k]:.9g} g {g[k]:.9g} device {p1[k]:.9g} "
+            f"oracle {pr[k]:.9g}")
+    assert not bad.any(), int(bad.sum())
+    net.close()
+
+
+def dirichlet_problem(n):
+    og = O.StructGrid(n, n, 1)
+    desc = ca.uniform_grid_desc(n, n, 1, [1, 1, 1, 1])
+    return og, desc
+
+
+def test_training_lowers_the_loss_and_eval_matches_the_oracle(ctx):
+    n = 64
+    og, desc = dirichlet_problem(n)
+    P = ca.Poisson(ctx, desc)
+    net = ca.UNet(ctx, seed=1)
+    slots = learned.add_error_samples(P, net, seeds=[1, 2], checkpoints=(0, 8, 32))
+    assert slots == 6 == net.num_samples
+    l0 = net.eval(P)
+    # oracle loss of every sample with the device's parameters
+    A = O.Matrix(og, O.make_bc(og, [1, 1, 1, 1]), 1.0)
+    p = net.get_params().astype(np.float64)
+    P2 = ca.Poisson(ctx, desc)
+    for k, seed, ck in ((0, 1, 0), (4, 2, 8)):
+        P2.set_source_blobs(seed)
+        P2.fill_phi(0.0)
+        P2.solve("JPCG", max_iter=10 ** 6, tol=1e-10, norm=ca.NORM_TRUE)
+        phi = learned.interior(P2, P2.get_vec(0))
+        if ck:
+            P2.fill_phi(0.0)
+            P2.solve("JPCG", max_iter=ck, tol=1e-300, norm=ca.NORM_TRUE)
+            e = phi - learned.interior(P2, P2.get_vec(0))
+        else:
+            e = phi
+        r = A.mult(e.T.ravel()).reshape(n, n).T
+        Lo = OU.energy_loss(p, A, r, e, bf=True)[0]
+        say(f"  sample {k}: device loss {l0[k]:.6f} oracle {Lo:.6f}")
+        assert abs(l0[k] - Lo) <= 1e-2 * Lo
+    losses = net.train(P, 60, batch=3, lr=2e-3, seed=5)
+    l1 = net.eval(P)
+    say(f"  loss: first steps {losses[:5].mean():.4f} last steps {losses[-5:].mean():.4f};"
+        f" eval {l0.mean():.4f} -> {l1.mean():.4f}")
+    assert l1.mean() < 0.5 * l0.mean()
+    P2.close()
+    P.close()
+    net.close()
+
+
+def blob_neumann(ctx, n, seed=0):
+    desc = ca.uniform_grid_desc(n, n, 1, [ca.BC_NEUMANN] * 4)
+    P = ca.Poisson(ctx, desc)
+    P.set_source_blobs(seed)
+    return P
+
+
+def test_fcg_with_an_untrained_network_still_reaches_tolerance(ctx):
+    n = 128
+    og, desc = dirichlet_problem(n)
+    P = ca.Poisson(ctx, desc)
+    P.set_source_blobs(3)
+    net = ca.UNet(ctx, seed=1)
+    P.set_unet(net, ca.UNET_PRECOND)
+    P.fill_phi(0.0)
+    code, it, res, _ = P.solve("FCG_UNET", max_iter=100000, tol=1e-10, norm=ca.NORM_TRUE)
+    say(f"  untrained FCG: code {code} iterations {it} res {res:.2e}")
+    assert code == 0 and res <= 1e-10
+    assert P.residual() <= 1.01e-10
+    x = learned.interior(P, P.get_phi())
+    P.set_unet(None)
+    P.fill_phi(0.0)
+    P.solve("JPCG", max_iter=100000, tol=1e-12, norm=ca.NORM_TRUE)
+    assert rel_l2(x, learned.interior(P, P.get_phi())) <= 1e-8
+    P.close()
+    net.close()
+
+
+def test_learned_preconditioner_gains_over_jacobi_pcg(ctx):
+    """FCG with the trained network against plain FP64 Jacobi-PCG on an
+    unseen right-hand side (seed 0; training seeds 1..6) of the 256^2
+    all-Neumann blob problem, to the true relative residual 1e-8."""
+    n = 256
+    P = blob_neumann(ctx, n, seed=1)
+    net = ca.UNet(ctx, seed=1)
+    t0 = time.perf_counter()
+    learned.add_error_samples(P, net, seeds=range(1, 7),
+                              checkpoints=(0, 16, 64, 256, 1024))
+    t1 = time.perf_counter()
+    losses = learned.train(net, P, 300, batch=4, lr=1e-3, seed=0)
+    ctx.sync()
+    t2 = time.perf_counter()
+    say(f"  {net.num_samples} samples in {t1 - t0:.1f} s; 300 Adam steps in {t2 - t1:.1f} s; "
+        f"loss {losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}")
+    P.set_source_blobs(0)
+
+    def run(method, unet_flags):
+        P.set_unet(net if unet_flags else None, unet_flags)
+        P.fill_phi(0.0)
+        P.solve(method, max_iter=10 ** 6, tol=1e-8, norm=ca.NORM_TRUE,
+                opts=ca.OPT_PROJECT_MEAN)  # warm-up (graphs, buffers)
+        P.fill_phi(0.0)
+        ctx.sync()
+        t = time.perf_counter()
+        code, it, res, hist = P.solve(method, max_iter=10 ** 6, tol=1e-8,
+                                      norm=ca.NORM_TRUE, opts=ca.OPT_PROJECT_MEAN,
+                                      hist_cap=64)
+        ctx.sync()
+        dt = time.perf_counter() - t
+        if method == "FCG_UNET":
+            say(f"  FCG residuals ({len(hist)}): " + " ".join(f"{h:.2e}" for h in hist[:24]))
+        assert code == 0 and res <= 1e-8
+        return it, dt, learned.interior(P, P.get_phi())
+
+    it_j, t_j, x_j = run("JPCG", 0)
+    it_w, t_w, _ = run("JPCG", ca.UNET_WARM)
+    it_f, t_f, x_f = run("FCG_UNET", ca.UNET_PRECOND)
+    it_fw, t_fw, _ = run("FCG_UNET", ca.UNET_PRECOND | ca.UNET_WARM)
+    say(f"  J-PCG {it_j} it {t_j * 1e3:.1f} ms | warm J-PCG {it_w} it {t_w * 1e3:.1f} ms | "
+        f"FCG+U-Net {it_f} it {t_f * 1e3:.1f} ms | FCG+U-Net warm {it_fw} it {t_fw * 1e3:.1f} ms")
+    assert rel_l2(x_f - x_f.mean(), x_j - x_j.mean()) <= 1e-6
+    # the network must cut the Krylov iteration count
+    assert it_f * 4 <= it_j
+    P.close()
+    net.close()
