@@ -1,18 +1,25 @@
-"""Headline benchmark: Poisson solves/s to true relative residual 1e-8 on a
-1024x1024 grid, one MI355X.
+"""Headline benchmark: pressure-Poisson solves/s of a 1024^2 lid-driven cavity
+(Re 1000) to the true relative residual 1e-8, on one MI355X.
 
-A step is one complete pressure-Poisson solve from a zero initial guess on
-an all-Neumann 1024^2 grid whose right-hand side is resident in HBM.  The
-timed method is the fastest path that reaches the FP64 tolerance (FP32
-Jacobi-PCG inside FP64 iterative refinement); plain FP64 Jacobi-PCG on the
-same GPU and the CPU oracle are reported beside it.
+A step is one projection step of the cavity (NS_explicit_: TVD advection,
+explicit diffusion, the pressure-correction Poisson solve warm-started from
+the previous step, the velocity correction), all resident in HBM.  The
+Poisson solve dominates it.  The timed method is flexible CG preconditioned
+by the BF16 U-Net inside a Jacobi-smoothing cycle (FP64 Krylov vectors,
+FP32 smoothing, BF16/FP32 network), with FP64 Jacobi-PCG as its fallback.
+The network is trained first, on the GPU with the library's own kernels,
+from the plain solver's steps of the same run and from its own flexible-CG
+iterates (untimed; its cost is reported).  Then the same K steps are timed
+with the learned solver, with plain FP64 Jacobi-PCG and with FP32 Jacobi-PCG
+in FP64 refinement, each from the same saved state; their fields are
+compared.
 
-The roofline figure is the GPU time per launch of the dominant kernel of
-the timed method, measured in this process with HIP events around a
-replayed hipGraph of back-to-back launches of that kernel
-(cpgpu_poisson_kernel_times), against the kernel's algorithmic bytes.
+The roofline figure is the average duration of the dominant U-Net kernel
+(the 16 -> 16 3x3 convolution of the finest level) over one more learned
+step, from timestamps the dispatch itself carries (hipExtLaunchKernelGGL
+events), against its algorithmic bytes.
 
-Prints one JSON line.  --gpus N>1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 import argparse
 import json
@@ -20,30 +27,42 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--train-plain", type=int, default=100,
+                    help="plain-solver cavity steps whose errors seed training")
+    ap.add_argument("--rounds", type=int, default=3,
+                    help="flexible-CG sample rounds after the first fit")
+    ap.add_argument("--first-fit", type=int, default=800)
+    ap.add_argument("--round-fit", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hbm", action="store_true",
-                    help="skip the 4096^2 HBM roofline probe")
-    ap.add_argument("--no-fp64", action="store_true",
-                    help="skip the plain FP64 J-PCG comparison")
+                    help="skip the 4096^2 Krylov-kernel HBM probe")
     ap.add_argument("--no-graphs", action="store_true",
-                    help="launch the Krylov batches directly instead of as "
+                    help="launch Jacobi-PCG batches directly instead of as "
                          "replayed hipGraphs (same kernels; for profilers)")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -53,145 +72,179 @@ def main():
         sys.exit(2)
 
     import carpios_amd as ca
+    from carpios_amd import learned
 
-    n = a.size
+    n, K, tol = a.size, a.steps, a.tol
     N = n * n
     ctx = ca.Context(0)
-    desc = ca.uniform_grid_desc(n, n, 1, [ca.BC_NEUMANN] * 4)
-    P = ca.Poisson(ctx, desc)
-    P.set_source_blobs(seed=0, nblobs=64, wmin=0.01, wmax=0.1)
-    P.set_mixed(inner_tol=1e-4, max_outer=40)
+    ns = learned.cavity(ctx, n, tol=tol)
+    P = ns.projection()
     if a.no_graphs:
         P.set_batch(16, use_graphs=False)
     tune = P.autotune(reps=20)
-    log(f"bench: {n}^2 all-Neumann blob problem ready; autotune {tune}")
+    log(f"bench: {n}^2 cavity ready; Jacobi-PCG kernel autotune {tune}")
 
-    def solve(method):
-        P.fill_phi(0.0)
-        code, it, res, _ = P.solve(method, max_iter=10 ** 7, tol=a.tol,
-                                   norm=ca.NORM_TRUE,
-                                   opts=ca.OPT_PROJECT_MEAN if method == "MIXED" else 0)
-        if code != 0:
-            raise RuntimeError(f"{method} did not converge: code {code} res {res}")
-        return it, res
-
-    # ---- headline: FP32 inner + FP64 refinement ----
-    for _ in range(a.warmup):
-        solve("MIXED")
-    ctx.sync()
+    # ---- training (untimed) ----
+    net = ca.UNet(ctx, seed=1)
+    tr = learned.CavityTrainer(ns, net, nu=2, omega=0.8, tol=tol)
     t0 = time.perf_counter()
-    iters = []
-    for _ in range(a.steps):
-        it, res = solve("MIXED")
-        iters.append(it)
+    tr.run(plain=a.train_plain, first_fit=a.first_fit, rounds=a.rounds,
+           round_steps=40, round_fit=a.round_fit)
     ctx.sync()
-    wall = time.perf_counter() - t0
-    value = a.steps / wall
-    true_res = P.residual()
-    log(f"bench: mixed {value:.2f} solves/s, {iters[-1]} inner it, true res {true_res:.2e}")
+    t_train = time.perf_counter() - t0
+    log(f"bench: trained in {t_train:.1f} s on {tr.slot} samples: {tr.log}")
 
-    # ---- GPU baseline: plain FP64 Jacobi-PCG ----
-    fp64 = None
-    it64 = None
-    if not a.no_fp64:
-        nb = max(1, min(a.steps, 3))
-        solve("JPCG")
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(nb):
-            it64, res64 = solve("JPCG")
-        ctx.sync()
-        rate = nb / (time.perf_counter() - t0)
-        fp64 = {"solves_per_s": round(rate, 4), "iterations": it64,
-                "speedup_of_value": round(value / rate, 3)}
-        log(f"bench: fp64 J-PCG {rate:.2f} solves/s, {it64} it")
+    # ---- warm-up steps of the learned solver, then the common state ----
+    P.set_unet(net, ca.UNET_PRECOND)
+    ns.set_solver("FCG_UNET", max_iter=10 ** 7, tol=tol, norm=ca.NORM_TRUE)
+    for _ in range(a.warmup):
+        ns.step(1)
+    state = ns.save()
+    t_state = ns.time()[0]
 
-    # ---- dominant kernel of the headline solve: GPU time per launch ----
-    # One more solve of the same problem with HIP events around every
-    # launch of the two Krylov kernel families (direct launches on the
-    # context stream; the timed solves replay the same kernels as graphs).
-    ctx.timer_enable(0)
-    ctx.timer_enable(1)
-    solve("MIXED")
-    n_d, ms_d = ctx.timer_read(0)
-    n_u, ms_u = ctx.timer_read(1)
-    ctx.timer_enable(0, False)
-    ctx.timer_enable(1, False)
-    d32, u32 = ms_d * 1e3 / max(n_d, 1), ms_u * 1e3 / max(n_u, 1)
-    fam = [(ms_d, d32, n_d, 4, f"k_cg_dir (variant {tune['fp32'][0]}) <float>: "
-                                "p update + A p + p.q"),
-           (ms_u, u32, n_u, 6, "k_cg_upd<float>: x, r update + r.z, r.r")]
-    _, us, launches, nvec, kname = max(fam)
-    alg_bytes = nvec * N * 4
-    achieved = alg_bytes / (us * 1e-6) / 1e9
+    def timed(solver, flags):
+        ns.restore(state)
+        P.set_unet(net if flags else None, flags)
+        ns.set_solver(solver, max_iter=10 ** 7, tol=tol, norm=ca.NORM_TRUE)
+        its, res = [], []
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(K):
+            it, r = ns.step(1)
+            its.append(int(it[0]))
+            res.append(float(r[0]))
+        ctx.sync()
+        wall = time.perf_counter() - t
+        if max(res) > tol:
+            raise RuntimeError(f"{solver}: a step ended at residual {max(res):.3e}")
+        fields = {k: ns.get_field(k) for k in ("u", "v", "p")}
+        return wall, its, fields
+
+    wl, itl, fl = timed("FCG_UNET", ca.UNET_PRECOND)
+    value = K / wl
+    log(f"bench: learned {value:.2f} solves/s, iterations {itl}")
+    wp, itp, fp = timed("JPCG", 0)
+    log(f"bench: plain FP64 J-PCG {K / wp:.2f} solves/s, iterations {itp}")
+    wm, itm, fm = timed("MIXED", 0)
+    log(f"bench: FP32 J-PCG in FP64 refinement {K / wm:.2f} solves/s, iterations {itm}")
+    g = 2
+    inner = (slice(g, g + n), slice(g, g + n))
+    pl, pp = fl["p"][inner], fp["p"][inner]
+    field_err = {"u": rel_l2(fl["u"][inner], fp["u"][inner]),
+                 "v": rel_l2(fl["v"][inner], fp["v"][inner]),
+                 "p_zero_mean": rel_l2(pl - pl.mean(), pp - pp.mean())}
+    log(f"bench: learned vs plain FP64 fields after {K} steps: {field_err}")
+
+    # ---- dominant U-Net kernel: timestamps of its own dispatches ----
+    ns.restore(state)
+    P.set_unet(net, ca.UNET_PRECOND)
+    ns.set_solver("FCG_UNET", max_iter=10 ** 7, tol=tol, norm=ca.NORM_TRUE)
+    ctx.timer_enable(2)
+    ns.step(1)
+    n_c, ms_c = ctx.timer_read(2)
+    ctx.timer_enable(2, False)
+    us = ms_c * 1e3 / max(n_c, 1)
+    conv_bytes = 2 * N * 16 * 2          # read 16 BF16 channels, write 16
+    conv_flops = 2 * 9 * 16 * 16 * N
+    achieved = conv_bytes / (us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": kname, "bytes_per_launch": alg_bytes,
-                "avg_launch_us": round(us, 3), "launches": launches,
-                "other_kernel_us": round(min(d32, u32), 3),
-                "iteration_us": round(d32 + u32, 3),
-                "note": f"{n}^2 FP32 vectors ({N * 4 / 2**20:.0f} MiB each) are "
-                        "Infinity-Cache resident; see roofline_hbm_probe for HBM"}
+                "kernel": "k_conv3_fwd<16,16> (finest U-Net level, 3x3, BF16 MFMA)",
+                "bytes_per_launch": conv_bytes, "avg_launch_us": round(us, 3),
+                "launches": n_c,
+                "mfma_tflops": round(conv_flops / (us * 1e-6) / 1e12, 2),
+                "mfma_frac_bf16": round(conv_flops / (us * 1e-6) / 1e12 / BF16_PEAK_TFLOPS, 4)}
+    log(f"bench: conv16 {us:.2f} us/launch, {achieved:.0f} GB/s")
 
-    # ---- HBM-resident probe: the same kernels at 4096^2 in FP64 ----
+    # ---- Krylov kernels of the plain solve (same step, timed dispatches) ----
+    ns.restore(state)
+    P.set_unet(None, 0)
+    ns.set_solver("JPCG", max_iter=10 ** 7, tol=tol, norm=ca.NORM_TRUE)
+    ctx.timer_enable(0)
+    ctx.timer_enable(1)
+    ns.step(1)
+    nd, msd = ctx.timer_read(0)
+    nu_, msu = ctx.timer_read(1)
+    ctx.timer_enable(0, False)
+    ctx.timer_enable(1, False)
+    d64, u64 = msd * 1e3 / max(nd, 1), msu * 1e3 / max(nu_, 1)
+    krylov = {"k_cg_dir<double>_us": round(d64, 3),
+              "k_cg_dir<double>_GBs": round(4 * N * 8 / (d64 * 1e-6) / 1e9, 1),
+              "k_cg_upd<double>_us": round(u64, 3),
+              "k_cg_upd<double>_GBs": round(6 * N * 8 / (u64 * 1e-6) / 1e9, 1)}
+
+    # ---- HBM-resident probe of the Krylov kernels at 4096^2 FP64 ----
     hbm = None
     if not a.no_hbm:
         m = 4096
         Pm = ca.Poisson(ctx, ca.uniform_grid_desc(m, m, 1, [ca.BC_NEUMANN] * 4))
         Pm.set_source_blobs(seed=0)
         tune_m = Pm.autotune(reps=5)
-        d64, u64 = Pm.kernel_times(64, reps=20)
+        dd, uu = Pm.kernel_times(64, reps=20)
         Nm = m * m
-        d_gbs = 4 * Nm * 8 / (d64 * 1e-6) / 1e9
-        u_gbs = 6 * Nm * 8 / (u64 * 1e-6) / 1e9
+        d_gbs = 4 * Nm * 8 / (dd * 1e-6) / 1e9
+        u_gbs = 6 * Nm * 8 / (uu * 1e-6) / 1e9
         hbm = {"grid": "4096^2 FP64", "dir_variant": tune_m["fp64"][0],
-               "k_cg_dir_us": round(d64, 2), "k_cg_dir_GBs": round(d_gbs, 1),
+               "k_cg_dir_us": round(dd, 2), "k_cg_dir_GBs": round(d_gbs, 1),
                "k_cg_dir_frac": round(d_gbs / HBM_PEAK_GBS, 4),
-               "k_cg_upd_us": round(u64, 2), "k_cg_upd_GBs": round(u_gbs, 1),
+               "k_cg_upd_us": round(uu, 2), "k_cg_upd_GBs": round(u_gbs, 1),
                "k_cg_upd_frac": round(u_gbs / HBM_PEAK_GBS, 4)}
         Pm.close()
         log(f"bench: 4096^2 fp64 dir {d_gbs:.0f} GB/s, upd {u_gbs:.0f} GB/s")
 
-    # ---- CPU baseline: the oracle's FP64 J-PCG, bounded sample ----
+    # ---- CPU baseline: the oracle's FP64 J-PCG on the same problem ----
     cpu = None
-    if not a.no_cpu and it64:
+    if not a.no_cpu:
         import oracle as O
-        og = O.StructGrid(n, n, 1)
+        og = O.StructGrid(n, n, 1, -0.5, 0.5, -0.5, 0.5)
         A = O.Matrix(og, O.make_bc(og, [2, 2, 2, 2]), 1.0)
-        b = O.field_to_x(og, P.get_vec(1))
-        k = 150
+        b = O.field_to_x(og, np.pad(learned.interior(P, P.get_vec(1)), 1))
+        k = 100
         t0 = time.perf_counter()
         O.solve("JPCG", A, b, max_iter=k, tol=1e-300, norm=O.NORM_TRUE)
         t_it = (time.perf_counter() - t0) / k
-        cpu = {"value": round(1.0 / (t_it * it64), 5), "unit": "solves/s",
+        it_mean = float(np.mean(itp))
+        cpu = {"value": round(1.0 / (t_it * it_mean), 5), "unit": "solves/s",
                "cores": 1, "kind": "port",
-               "sample": f"{k} FP64 J-PCG iterations of the same {n}^2 problem "
-                         f"(oracle C, -O2, 1 thread), {t_it * 1e3:.2f} ms/iteration, "
-                         f"extrapolated to the {it64} iterations FP64 J-PCG needs"}
+               "sample": f"{k} FP64 Jacobi-PCG iterations on the {n}^2 cavity "
+                         f"projection problem (oracle C, -O2, 1 thread), "
+                         f"{t_it * 1e3:.2f} ms/iteration, extrapolated to the "
+                         f"{it_mean:.0f} iterations per step plain FP64 J-PCG needs"}
         log(f"bench: cpu {cpu['value']} solves/s")
 
     out = {
         "metric": "Poisson solves/s to rel-res 1e-8 @1024^2: learned+FP32/FP64-refine vs plain FP64",
         "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "steps": K, "warmup": a.warmup,
+        "ms_per_step": round(wl / K * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 inner / fp64 refinement",
-        "data": "synthetic",
-        "config": {"workload": f"{n}x{n} all-Neumann pressure Poisson, 64 Gaussian "
-                               f"blobs (seed 0) zero-mean RHS, zero initial guess, "
-                               f"true rel-res {a.tol:g}; FP32 Jacobi-PCG inner "
-                               f"(rel 1e-4) in FP64 iterative refinement",
-                   "method": "MIXED", "inner_iterations": iters[-1],
-                   "autotune": tune, "true_rel_residual": true_res},
-        "gpu_fp64_jpcg": fp64,
+        "dtype": "fp64 krylov / fp32 smoothing / bf16 u-net",
+        "data": "synthetic (cavity from rest, random-init U-Net trained in-run)",
+        "config": {"workload": f"{n}x{n} lid-driven cavity Re=1000 (CFL 0.5, van Leer), "
+                               f"pressure-correction Poisson solves of consecutive steps "
+                               f"warm-started from the previous phi, true rel-res {tol:g}",
+                   "method": "FCG_UNET: flexible CG, preconditioner = 2 Jacobi sweeps + "
+                             "U-Net + 2 Jacobi sweeps, FP64 J-PCG fallback",
+                   "cells": N, "start_time": t_state,
+                   "iterations_learned": itl, "iterations_plain_fp64": itp,
+                   "iterations_mixed": itm, "autotune": tune,
+                   "training": {"seconds": round(t_train, 1), "samples": tr.slot,
+                                "log": tr.log}},
+        "cells_per_s": round(value * N, 1),
+        "gpu_fp64_jpcg": {"solves_per_s": round(K / wp, 4),
+                          "speedup_of_value": round(wp / wl, 3)},
+        "gpu_mixed_jpcg": {"solves_per_s": round(K / wm, 4),
+                           "speedup_of_value": round(wm / wl, 3)},
+        "field_rel_l2_vs_fp64": field_err,
         "roofline": roofline,
+        "krylov_kernels_1024": krylov,
         "roofline_hbm_probe": hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    P.close()
+    net.close()
+    ns.close()
     ctx.close()
 
 

@@ -68,6 +68,7 @@ EXPORTS = [
     "cpgpu_unet_forward", "cpgpu_unet_backward", "cpgpu_unet_adam",
     "cpgpu_unet_set_sample", "cpgpu_unet_num_samples", "cpgpu_unet_train",
     "cpgpu_unet_eval", "cpgpu_poisson_set_unet", "cpgpu_unet_tensor",
+    "cpgpu_ns_set_time", "cpgpu_ns_set_solver", "cpgpu_poisson_set_fcg",
 ]
 
 
@@ -84,6 +85,7 @@ def lib():
         L.cpgpu_stream.restype = C.c_void_p
         L.cpgpu_poisson_device_vec.restype = C.c_void_p
         L.cpgpu_unet_num_params.restype = C.c_longlong
+        L.cpgpu_ns_projection.restype = C.c_void_p
         _lib = L
     return _lib
 
@@ -235,9 +237,9 @@ class Poisson:
         return code, mi.value, t.value, hist[:min(hl.value, hist_cap)].copy()
 
     def close(self):
-        if self.h and self.ctx.h:
+        if self.h and self.ctx.h and getattr(self, "_owned", True):
             lib().cpgpu_poisson_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -363,6 +365,38 @@ class NS:
         self.ctx.check(lib().cpgpu_ns_time(self.h, C.byref(t), C.byref(dt)))
         return t.value, dt.value
 
+    def set_time(self, t, dt):
+        self.ctx.check(lib().cpgpu_ns_set_time(self.h, C.c_double(t), C.c_double(dt)))
+
+    def set_solver(self, solver, max_iter=100000, tol=1e-10, norm=NORM_REF):
+        self.ctx.check(lib().cpgpu_ns_set_solver(self.h, METHODS[solver],
+                                                 int(max_iter), C.c_double(tol),
+                                                 int(norm)))
+
+    def projection(self):
+        """The step's projection problem (all-Neumann Poisson) as a Poisson
+        view that the stepper owns: b and phi of the last step, U-Net
+        attachment, solver options."""
+        P = Poisson.__new__(Poisson)
+        P.ctx, P.desc = self.ctx, None
+        P.nx, P.ny, P.gl = self.nx, self.ny, self.gl
+        P.h = C.c_void_p(lib().cpgpu_ns_projection(self.h))
+        P._owned = False
+        P._ns = self
+        return P
+
+    STATE = ("u", "v", "uf", "vf", "p", "phi")
+
+    def save(self):
+        """Host copy of the step state (fields, t, dt)."""
+        return {"fields": {k: self.get_field(k) for k in self.STATE},
+                "time": self.time()}
+
+    def restore(self, state):
+        for k, f in state["fields"].items():
+            self.set_field(k, f)
+        self.set_time(*state["time"])
+
     def close(self):
         if self.h and self.ctx.h:
             lib().cpgpu_ns_destroy(self.h)
@@ -469,12 +503,14 @@ class UNet:
                                                _p(out, _fp), dims))
         return out
 
-    def set_sample(self, P, slot, e):
-        """Training sample: error e (nx, ny) of problem P; r = A e on device."""
+    def set_sample(self, P, slot, e, smooth=0, omega=0.8):
+        """Training sample: error e (nx, ny) of problem P, after `smooth`
+        weighted-Jacobi sweeps; r = A e on device."""
         e = np.ascontiguousarray(e, dtype=np.float64)
         assert e.shape == (P.nx, P.ny)
         self.ctx.check(lib().cpgpu_unet_set_sample(self.h, P.h, int(slot),
-                                                   _p(e)))
+                                                   _p(e), int(smooth),
+                                                   C.c_double(omega)))
 
     @property
     def num_samples(self):
@@ -511,3 +547,11 @@ def _set_unet(self, unet, flags=UNET_PRECOND):
 
 
 Poisson.set_unet = _set_unet
+
+
+def _set_fcg(self, pre=2, post=2, omega=0.8):
+    self.ctx.check(lib().cpgpu_poisson_set_fcg(self.h, int(pre), int(post),
+                                               C.c_double(omega)))
+
+
+Poisson.set_fcg = _set_fcg

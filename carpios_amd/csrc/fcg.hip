@@ -67,22 +67,29 @@ __global__ __launch_bounds__(256) void k_fcg_start(const double* part, int G,
   }
 }
 
-// partials of z . q (the previous q)
+// partials of z . q (the previous q; 0 on the first iteration, q null)
+// and of sum(z)
 __global__ __launch_bounds__(256) void k_fcg_zq(const float* __restrict__ z,
                                                 const double* __restrict__ q,
                                                 double* part, size_t n) {
-  __shared__ double lds[4];
-  double v[1] = {0.0};
+  __shared__ double lds[8];
+  double v[2] = {0.0, 0.0};
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
-       k += (size_t)gridDim.x * blockDim.x)
-    v[0] += (double)z[k] * q[k];
-  store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
+       k += (size_t)gridDim.x * blockDim.x) {
+    const double zv = z[k];
+    if (q) v[0] += zv * q[k];
+    v[1] += zv;
+  }
+  store_partials<double, 2>(v, part, gridDim.x, blockIdx.x, lds);
 }
 
 // p_new = z + beta p_old (beta = -(z.q_prev)/(p.q)_prev, 0 on the first
 // iteration), q = A p_new, partials of p.q and p.r.  p_new at the four
 // neighbours is recomputed from z and p_old, so p_old and p_new are
-// different buffers.
+// different buffers.  For a singular (all-Neumann) operator z is first
+// projected to zero mean (project != 0): the constants are A's null space,
+// and a constant part of p would enter p.r through the rounding-level
+// mean of r and stall the iteration.
 __global__ __launch_bounds__(256) void k_fcg_dir(Sten<double> s,
                                                  const float* __restrict__ z,
                                                  const double* __restrict__ pold,
@@ -91,14 +98,14 @@ __global__ __launch_bounds__(256) void k_fcg_dir(Sten<double> s,
                                                  const double* __restrict__ r,
                                                  const double* partZ, int GZ,
                                                  const FState* st, int first,
-                                                 double* part, Tiling tl) {
+                                                 int project, double* part,
+                                                 Tiling tl) {
   __shared__ double lds[8];
-  double beta = 0.0;
-  if (!first) {
-    double t[1];
-    reduce_partials<double, 1>(partZ, GZ, t, lds);
-    beta = -t[0] / st->pq;
-  }
+  double t[2];
+  reduce_partials<double, 2>(partZ, GZ, t, lds);
+  // (z - m).q_prev = z.q_prev - m sum(q_prev), and sum(A p) = 0
+  const double m = project ? t[1] / ((double)s.nx * s.ny) : 0.0;
+  const double beta = first ? 0.0 : -t[0] / st->pq;
   double v[2] = {0.0, 0.0};
   const int ny = s.ny;
   for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
@@ -110,13 +117,14 @@ __global__ __launch_bounds__(256) void k_fcg_dir(Sten<double> s,
       const int i = i0 + threadIdx.y + BY * k;
       if (i < s.nx && j < ny) {
         const size_t idx = (size_t)i * ny + j;
-        const double pc = (double)z[idx] + beta * pold[idx];
+        const double pc = ((double)z[idx] - m) + beta * pold[idx];
         double acc = pc * diag_at(s, i, j);
-        if (i > 0) acc -= s.KX[i] * ((double)z[idx - ny] + beta * pold[idx - ny]);
-        if (j > 0) acc -= s.KY[j] * ((double)z[idx - 1] + beta * pold[idx - 1]);
-        if (j < ny - 1) acc -= s.KY[j + 1] * ((double)z[idx + 1] + beta * pold[idx + 1]);
+        if (i > 0) acc -= s.KX[i] * (((double)z[idx - ny] - m) + beta * pold[idx - ny]);
+        if (j > 0) acc -= s.KY[j] * (((double)z[idx - 1] - m) + beta * pold[idx - 1]);
+        if (j < ny - 1)
+          acc -= s.KY[j + 1] * (((double)z[idx + 1] - m) + beta * pold[idx + 1]);
         if (i < s.nx - 1)
-          acc -= s.KX[i + 1] * ((double)z[idx + ny] + beta * pold[idx + ny]);
+          acc -= s.KX[i + 1] * (((double)z[idx + ny] - m) + beta * pold[idx + ny]);
         pnew[idx] = pc;
         q[idx] = acc;
         v[0] += pc * acc;
@@ -207,6 +215,58 @@ __global__ __launch_bounds__(256) void k_sub_mean_part(double* __restrict__ x,
     x[k] = x[k] - mean;
 }
 
+// Weighted Jacobi sweep on A z = r (FP32): zout = zin + w D^-1 (r - A zin);
+// zin == nullptr means a zero initial guess (zout = w D^-1 r).
+__global__ __launch_bounds__(256) void k_jacobi32(Sten<float> s,
+                                                  const float* __restrict__ r,
+                                                  const float* __restrict__ zin,
+                                                  float* __restrict__ zout,
+                                                  float w, Tiling tl) {
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny) {
+        const size_t idx = (size_t)i * s.ny + j;
+        const float dg = diag_at(s, i, j);
+        const float rv = zin ? r[idx] - apply_global(s, zin, i, j, dg) : r[idx];
+        zout[idx] = (zin ? zin[idx] : 0.f) + w * (rv / dg);
+      }
+    }
+  }
+}
+
+// rout = r - A z (FP32)
+__global__ __launch_bounds__(256) void k_resid32(Sten<float> s,
+                                                 const float* __restrict__ r,
+                                                 const float* __restrict__ z,
+                                                 float* __restrict__ rout, Tiling tl) {
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny) {
+        const size_t idx = (size_t)i * s.ny + j;
+        rout[idx] = r[idx] - apply_global(s, z, i, j, diag_at(s, i, j));
+      }
+    }
+  }
+}
+
+// a += b
+__global__ __launch_bounds__(256) void k_add32(float* __restrict__ a,
+                                               const float* __restrict__ b, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    a[k] = a[k] + b[k];
+}
+
 int fcg_alloc(cpgpu_poisson* P) {
   int rc = poisson_prepare_f32(P);
   if (rc) return rc;
@@ -218,6 +278,57 @@ int fcg_alloc(cpgpu_poisson* P) {
     P->h_fst = nullptr;
     return set_error(P->ctx, CPGPU_ENOMEM, "pinned flexible-CG state");
   }
+  return CPGPU_OK;
+}
+
+// z (P->pa32) = M(r) for r in P->r32: `pre` weighted-Jacobi sweeps from
+// zero, the network on the remaining residual, `post` sweeps -- a two-level
+// cycle whose coarse correction is the network.  Work buffers: x32, pb32,
+// q32, b32.
+int apply_precond(cpgpu_poisson* P) {
+  hipStream_t st = P->ctx->stream;
+  const Tiling tl = P->tl;
+  const Sten<float> s = P->sten32();
+  const float w = (float)P->pc_omega;
+  const float* r = P->r32.p;
+  float* bufA = P->x32.p;
+  float* bufB = P->pb32.p;
+  float* z = nullptr;  // null: z = 0
+  for (int k = 0; k < P->pc_pre; ++k) {
+    float* out = (z == bufA) ? bufB : bufA;
+    hipLaunchKernelGGL(k_jacobi32, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
+                       (const float*)z, out, w, tl);
+    z = out;
+  }
+  const float* rn = r;
+  if (z) {
+    hipLaunchKernelGGL(k_resid32, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
+                       (const float*)z, P->q32.p, tl);
+    rn = P->q32.p;
+  }
+  float* c = P->b32.p;
+  int rc = unet_apply_device(P->unet, P->nx, P->ny, rn, c);
+  if (rc) return rc;
+  if (z) {
+    hipLaunchKernelGGL(k_add32, dim3(1024), dim3(256), 0, st, z, c, P->N);
+  } else {
+    z = c;
+  }
+  // post sweeps alternate between two buffers other than z's, the last one
+  // writing P->pa32
+  for (int k = 0; k < P->pc_post; ++k) {
+    const int left = P->pc_post - 1 - k;
+    float* spare = (z == bufA) ? bufB : bufA;
+    float* out = (left % 2 == 0) ? P->pa32.p : spare;
+    if (out == z) out = (z == bufA) ? bufB : bufA;
+    hipLaunchKernelGGL(k_jacobi32, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
+                       (const float*)z, out, w, tl);
+    z = out;
+  }
+  if (z != P->pa32.p)
+    CPGPU_HIP(P->ctx, hipMemcpyAsync(P->pa32.p, z, sizeof(float) * P->N,
+                                     hipMemcpyDeviceToDevice, st));
+  CPGPU_HIP(P->ctx, hipGetLastError());
   return CPGPU_OK;
 }
 
@@ -276,21 +387,26 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
   double res = fs.resid;
   std::vector<double> trace;
   trace.push_back(res);
+  constexpr int kStallWindow = 50;
+  bool singular = true;  // all-Neumann: constants are the null space
+  for (int sg = 0; sg < 4; ++sg)
+    singular = singular && P->seg[sg].type == CPGPU_BC_NEUMANN;
+  double best = res;
+  std::vector<double> best_hist(1, res);
   int k = 0;
   bool fallback = false;
   while (res > tol) {
     if (k >= max_iter) break;
     // z = preconditioned residual
-    if ((rc = unet_apply_device(P->unet, P->nx, P->ny, P->r32.p, P->pa32.p)))
-      return rc;
-    if (k > 0)
-      hipLaunchKernelGGL(k_fcg_zq, dim3(G1), dim3(256), 0, st, P->pa32.p,
-                         P->q.p, P->partA64.p, N);
+    if ((rc = apply_precond(P))) return rc;
+    hipLaunchKernelGGL(k_fcg_zq, dim3(G1), dim3(256), 0, st, P->pa32.p,
+                       k > 0 ? (const double*)P->q.p : nullptr, P->part64.p, N);  // 2 x G1 partials
     const double* pold = P->pf[k & 1].p;
     double* pnew = P->pf[(k + 1) & 1].p;
     hipLaunchKernelGGL(k_fcg_dir, dim3(tl.G), dim3(TJ, BY), 0, st, P->sten64(),
-                       P->pa32.p, pold, pnew, P->q.p, P->r.p, P->partA64.p, G1,
-                       P->fst.p, k == 0 ? 1 : 0, P->partB64.p, tl);
+                       P->pa32.p, pold, pnew, P->q.p, P->r.p, P->part64.p, G1,
+                       P->fst.p, k == 0 ? 1 : 0, singular ? 1 : 0, P->partB64.p,
+                       tl);
     hipLaunchKernelGGL(k_fcg_upd, dim3(G1), dim3(256), 0, st, pnew, P->q.p,
                        P->x.p, P->r.p, P->r32.p, P->partB64.p, (int)tl.G,
                        P->fst.p, P->partR64.p, N, norm_ref);
@@ -305,8 +421,12 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
       fallback = true;
       break;
     }
-    // stagnation: less than a factor 2 over the last 16 iterations
-    if (k >= 16 && res > 0.5 * trace[k - 16]) {
+    // stall: the best residual improved by less than 10% over the last
+    // kStallWindow iterations (the residual norm of a CG method is not
+    // monotone, so single-step or short-window tests misfire)
+    best = std::min(best, res);
+    best_hist.push_back(best);
+    if (k >= kStallWindow && best > 0.9 * best_hist[k - kStallWindow]) {
       fallback = true;
       break;
     }

@@ -512,6 +512,17 @@ int ns_one_step(cpgpu_ns* S) {
 
 }  // namespace
 
+namespace {
+bool ns_solver_ok(int m) {
+  return m == CPGPU_JPCG || m == CPGPU_MIXED || m == CPGPU_CG ||
+         m == CPGPU_FCG_UNET;
+}
+// the methods that are not the reference's report a zero-mean phi
+int ns_solver_opts(int m) {
+  return (m == CPGPU_MIXED || m == CPGPU_FCG_UNET) ? CPGPU_OPT_PROJECT_MEAN : 0;
+}
+}  // namespace
+
 extern "C" {
 
 int cpgpu_ns_create(cpgpu_ctx* ctx, const cpgpu_ns_desc* d, cpgpu_ns** out) {
@@ -521,8 +532,7 @@ int cpgpu_ns_create(cpgpu_ctx* ctx, const cpgpu_ns_desc* d, cpgpu_ns** out) {
   if (g.nx < 1 || g.ny < 1 || g.gl < 2 || !g.cx || !g.sx || !g.cy || !g.sy)
     return set_error(ctx, CPGPU_EINVAL, "NS needs gl >= 2 and axis arrays");
   if (d->limiter < 0 || d->limiter > 5 || !(d->rho > 0.0) ||
-      !(d->dt > 0.0) || (d->solver != CPGPU_JPCG && d->solver != CPGPU_MIXED &&
-                         d->solver != CPGPU_CG))
+      !(d->dt > 0.0) || !ns_solver_ok(d->solver))
     return set_error(ctx, CPGPU_EINVAL, "bad NS parameters");
   for (int s = 0; s < 4; ++s)
     if ((d->bcu_type[s] != 1 && d->bcu_type[s] != 2) ||
@@ -570,7 +580,7 @@ int cpgpu_ns_create(cpgpu_ctx* ctx, const cpgpu_ns_desc* d, cpgpu_ns** out) {
   S->rho = d->rho; S->mu = d->mu; S->cfl = d->cfl; S->dnum = d->dnum;
   S->dcoef = d->dcoef; S->limiter = d->limiter; S->solver = d->solver;
   S->max_iter = d->max_iter; S->tol = d->tol; S->norm_mode = d->norm_mode;
-  S->opts = d->solver == CPGPU_MIXED ? CPGPU_OPT_PROJECT_MEAN : 0;
+  S->opts = ns_solver_opts(d->solver);
   // Grid_::min_size over the ghost-inclusive size arrays; the diffusion
   // denominator sum over d of 1/(min_size(d)^2) (s_operation.hpp:856-867)
   double msx = g.sx[0], msy = g.sy[0];
@@ -643,5 +653,29 @@ int cpgpu_ns_time(cpgpu_ns* S, double* t, double* dt) {
 }
 
 cpgpu_poisson* cpgpu_ns_projection(cpgpu_ns* S) { return S ? S->P : nullptr; }
+
+int cpgpu_ns_set_time(cpgpu_ns* S, double t, double dt) {
+  if (!S || !(dt > 0.0)) return CPGPU_EINVAL;
+  CPGPU_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
+  S->h_st->t = t;
+  S->h_st->dt = dt;
+  S->h_st->maxmag = 0.0;
+  CPGPU_HIP(S->ctx, hipMemcpyAsync(S->st.p, S->h_st, sizeof(NSDev),
+                                   hipMemcpyHostToDevice, S->ctx->stream));
+  CPGPU_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
+  return CPGPU_OK;
+}
+
+int cpgpu_ns_set_solver(cpgpu_ns* S, int solver, int max_iter, double tol,
+                        int norm_mode) {
+  if (!S || !ns_solver_ok(solver) || max_iter < 1 || !(tol > 0.0))
+    return CPGPU_EINVAL;
+  S->solver = solver;
+  S->max_iter = max_iter;
+  S->tol = tol;
+  S->norm_mode = norm_mode;
+  S->opts = ns_solver_opts(solver);
+  return CPGPU_OK;
+}
 
 }  // extern "C"

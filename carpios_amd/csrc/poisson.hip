@@ -1089,6 +1089,16 @@ int cpgpu_poisson_set_batch(cpgpu_poisson* P, int iters, int use_graphs) {
   return CPGPU_OK;
 }
 
+int cpgpu_poisson_set_fcg(cpgpu_poisson* P, int pre, int post, double omega) {
+  if (!P || pre < 0 || post < 0 || pre > 64 || post > 64 || !(omega > 0.0) ||
+      !(omega < 2.0))
+    return CPGPU_EINVAL;
+  P->pc_pre = pre;
+  P->pc_post = post;
+  P->pc_omega = omega;
+  return CPGPU_OK;
+}
+
 int cpgpu_poisson_set_unet(cpgpu_poisson* P, cpgpu_unet* U, int flags) {
   if (!P || (flags & ~(CPGPU_UNET_WARM | CPGPU_UNET_PRECOND))) return CPGPU_EINVAL;
   if (U && unet_context(U) != P->ctx)

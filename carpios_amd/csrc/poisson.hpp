@@ -115,6 +115,10 @@ struct cpgpu_poisson {
   cpgpu_unet* unet = nullptr;
   int unet_flags = 0;
   int fcg_iters = 0;  // flexible-CG iterations of the last CPGPU_FCG_UNET solve
+  // preconditioner cycle of CPGPU_FCG_UNET: weighted-Jacobi sweeps before
+  // and after the network (its coarse correction)
+  int pc_pre = 2, pc_post = 2;
+  double pc_omega = 0.8;
   cpgpu::DevBuf<double> pf[2];
   cpgpu::DevBuf<cpgpu::FState> fst;
   cpgpu::FState* h_fst = nullptr;  // pinned

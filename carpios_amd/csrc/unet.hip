@@ -301,6 +301,27 @@ __global__ __launch_bounds__(256) void k_rms(const double* part, int G, size_t n
   }
 }
 
+// error propagation of one weighted-Jacobi sweep: eout = ein - w D^-1 A ein
+__global__ __launch_bounds__(256) void k_smooth_err(Sten<double> s,
+                                                    const double* __restrict__ ein,
+                                                    double* __restrict__ eout,
+                                                    double w, Tiling tl) {
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny) {
+        const size_t idx = (size_t)i * s.ny + j;
+        const double dg = diag_at(s, i, j);
+        eout[idx] = ein[idx] - w * (apply_global(s, ein, i, j, dg) / dg);
+      }
+    }
+  }
+}
+
 // training sample: r = A e in FP64, FP32 copies of r and e, partials of e.r
 __global__ __launch_bounds__(256) void k_sample(Sten<double> s, const double* __restrict__ e,
                                                 float* __restrict__ r32,
@@ -416,7 +437,7 @@ struct cpgpu_unet {
   std::vector<std::unique_ptr<Sample>> samples;
   int sH = 0, sW = 0;  // grid of the samples
   DevBuf<float> gacc, dbuf, qbuf, lossdev;
-  DevBuf<double> tmp64, lpart;
+  DevBuf<double> tmp64, tmp64b, lpart;
 
   View v(const T& t, int co = 0) { return View{act.p + t.off, t.H, t.W, t.C, co}; }
   GView g(const T& t, int co = 0) {
@@ -468,24 +489,44 @@ int unet_alloc(cpgpu_unet* U, int H, int W) {
   return CPGPU_OK;
 }
 
+// Wavefront tiles: 32 pixels x up to 64 channels on the large levels; on
+// levels with fewer than kSmallM pixels 16 x 16, so that the few pixels
+// still spread over many wavefronts.
+constexpr int kSmallM = 16384;
+
+template <int NB, int MP>
+inline dim3 tile_grid(int M, int ncols) {
+  return dim3((M + 64 * MP - 1) / (64 * MP), ncols / (16 * NB));
+}
+
 template <int CIN, int COUT>
 void conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu) {
   constexpr int NB = COUT / 16 < 4 ? COUT / 16 : 4;
   const int M = x.H * x.W;
-  dim3 grid((M + 64 * kMP - 1) / (64 * kMP), COUT / (16 * NB));
-  hipLaunchKernelGGL((k_conv3_fwd<CIN, COUT, NB>), grid, dim3(256), 0,
-                     U->ctx->stream, x, y, U->wbf.p + U->wbf_off[l],
-                     U->params.p + U->boff[l], relu);
+  const bf16* w = U->wbf.p + U->wbf_off[l];
+  const float* b = U->params.p + U->boff[l];
+  // timer family 2: the 16 -> 16 convolutions of the finest level
+  const EvPair ev = (CIN == 16 && COUT == 16 && M >= kSmallM)
+                        ? timer_events(U->ctx, 2) : EvPair();
+  if (M < kSmallM)
+    hipLaunchKernelGGL((k_conv3_fwd<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, COUT)),
+                       dim3(256), 0, U->ctx->stream, x, y, w, b, relu);
+  else
+    CPGPU_LAUNCH(ev, (k_conv3_fwd<CIN, COUT, NB, 2>), (tile_grid<NB, 2>(M, COUT)),
+                 dim3(256), 0, U->ctx->stream, x, y, w, b, relu);
 }
 
 template <int CIN, int COUT>
 void up_fwd(cpgpu_unet* U, View x, View y, int l) {
-  constexpr int NB = 4;
   const int M = x.H * x.W;
-  dim3 grid((M + 64 * kMP - 1) / (64 * kMP), 16 * COUT / (16 * NB));
-  hipLaunchKernelGGL((k_up_fwd<CIN, COUT, NB>), grid, dim3(256), 0,
-                     U->ctx->stream, x, y, U->wbf.p + U->wbf_off[l],
-                     U->params.p + U->boff[l]);
+  const bf16* w = U->wbf.p + U->wbf_off[l];
+  const float* b = U->params.p + U->boff[l];
+  if (M < kSmallM)
+    hipLaunchKernelGGL((k_up_fwd<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, 16 * COUT)),
+                       dim3(256), 0, U->ctx->stream, x, y, w, b);
+  else
+    hipLaunchKernelGGL((k_up_fwd<CIN, COUT, 4, 2>), (tile_grid<4, 2>(M, 16 * COUT)),
+                       dim3(256), 0, U->ctx->stream, x, y, w, b);
 }
 
 int forward(cpgpu_unet* U) {
@@ -534,9 +575,13 @@ int conv_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, int in_co,
   const int M = in.H * in.W;
   if (need_dx) {
     constexpr int NB = CIN / 16 < 4 ? CIN / 16 : 4;
-    dim3 grid((M + 64 * kMP - 1) / (64 * kMP), CIN / (16 * NB));
-    hipLaunchKernelGGL((k_conv3_dgrad<CIN, COUT, NB>), grid, dim3(256), 0, st,
-                       gout, yout, U->g(in, in_co), U->wtbf.p + U->wbf_off[l], 1);
+    const bf16* wt = U->wtbf.p + U->wbf_off[l];
+    if (M < kSmallM)
+      hipLaunchKernelGGL((k_conv3_dgrad<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, CIN)),
+                         dim3(256), 0, st, gout, yout, U->g(in, in_co), wt, 1);
+    else
+      hipLaunchKernelGGL((k_conv3_dgrad<CIN, COUT, NB, 2>), (tile_grid<NB, 2>(M, CIN)),
+                         dim3(256), 0, st, gout, yout, U->g(in, in_co), wt, 1);
   }
   const int nch = wg_chunks(M);
   const int ppc = (M + nch - 1) / nch;
@@ -561,9 +606,13 @@ int up_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, const cpgpu_unet::T& out,
   const int M = in.H * in.W;
   {
     constexpr int NB = CIN / 16 < 4 ? CIN / 16 : 4;
-    dim3 grid((M + 64 * kMP - 1) / (64 * kMP), CIN / (16 * NB));
-    hipLaunchKernelGGL((k_up_dgrad<CIN, COUT, NB>), grid, dim3(256), 0, st,
-                       gout, U->g(in), U->wtbf.p + U->wbf_off[l]);
+    const bf16* wt = U->wtbf.p + U->wbf_off[l];
+    if (M < kSmallM)
+      hipLaunchKernelGGL((k_up_dgrad<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, CIN)),
+                         dim3(256), 0, st, gout, U->g(in), wt);
+    else
+      hipLaunchKernelGGL((k_up_dgrad<CIN, COUT, NB, 2>), (tile_grid<NB, 2>(M, CIN)),
+                         dim3(256), 0, st, gout, U->g(in), wt);
   }
   const int nch = wg_chunks(M);
   const int ppc = (M + nch - 1) / nch;
@@ -902,8 +951,10 @@ int cpgpu_unet_adam(cpgpu_unet* U, float lr) {
 }
 
 int cpgpu_unet_set_sample(cpgpu_unet* U, cpgpu_poisson* P, int slot,
-                          const double* e) {
-  if (!U || !P || !e || slot < 0 || slot > (1 << 20)) return CPGPU_EINVAL;
+                          const double* e, int smooth, double omega) {
+  if (!U || !P || !e || slot < 0 || slot > (1 << 20) || smooth < 0 ||
+      smooth > 64 || (smooth > 0 && !(omega > 0.0 && omega < 2.0)))
+    return CPGPU_EINVAL;
   int rc = check_problem(U, P);
   if (rc) return rc;
   const size_t N = P->N;
@@ -911,15 +962,22 @@ int cpgpu_unet_set_sample(cpgpu_unet* U, cpgpu_poisson* P, int slot,
   if (!U->samples[slot]) U->samples[slot].reset(new cpgpu_unet::Sample());
   cpgpu_unet::Sample& S = *U->samples[slot];
   if (S.r.alloc(N) || S.e.alloc(N) || U->tmp64.alloc(N) ||
-      U->lpart.alloc(kMaxBlocks + 1))
+      (smooth > 0 && U->tmp64b.alloc(N)) || U->lpart.alloc(kMaxBlocks + 1))
     return set_error(U->ctx, CPGPU_ENOMEM, "U-Net sample");
   U->sH = P->nx;
   U->sW = P->ny;
   hipStream_t st = U->ctx->stream;
   CPGPU_HIP(U->ctx, hipMemcpyAsync(U->tmp64.p, e, sizeof(double) * N,
                                    hipMemcpyHostToDevice, st));
+  double* ecur = U->tmp64.p;
+  for (int k = 0; k < smooth; ++k) {
+    double* enew = ecur == U->tmp64.p ? U->tmp64b.p : U->tmp64.p;
+    hipLaunchKernelGGL(k_smooth_err, dim3(P->tl.G), dim3(TJ, BY), 0, st,
+                       P->sten64(), (const double*)ecur, enew, omega, P->tl);
+    ecur = enew;
+  }
   hipLaunchKernelGGL(k_sample, dim3(P->tl.G), dim3(TJ, BY), 0, st, P->sten64(),
-                     U->tmp64.p, S.r.p, S.e.p, U->lpart.p, P->tl);
+                     (const double*)ecur, S.r.p, S.e.p, U->lpart.p, P->tl);
   hipLaunchKernelGGL(k_total, dim3(1), dim3(256), 0, st, U->lpart.p,
                      (int)P->tl.G, U->lpart.p + kMaxBlocks);
   CPGPU_HIP(U->ctx, hipGetLastError());

@@ -122,13 +122,13 @@ struct Frag<32> {
   }
 };
 
-constexpr int kMP = 2;  // 16-pixel row blocks per wavefront
+
 
 // ---------------------------------------------------------------------
 // conv 3x3, pad 1: forward.  y = relu(bias + sum_tap W[tap] x[m + off]).
-// W layout [tap][co][ci].  grid: x = pixel tiles of 4 waves * 16*kMP,
+// W layout [tap][co][ci].  grid: x = pixel tiles of 4 waves * 16*MP,
 // y = column groups of 16*NB channels.
-template <int CIN, int COUT, int NB>
+template <int CIN, int COUT, int NB, int MP>
 __global__ __launch_bounds__(256) void k_conv3_fwd(View x, View y,
                                                    const bf16* __restrict__ w,
                                                    const float* __restrict__ bias,
@@ -137,17 +137,17 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(View x, View y,
   typedef Frag<KC> F;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int M = x.H * x.W;
-  const int m0 = (blockIdx.x * 4 + wave) * 16 * kMP;
+  const int m0 = (blockIdx.x * 4 + wave) * 16 * MP;
   const int n0 = blockIdx.y * 16 * NB;
   if (m0 >= M) return;
-  f32x4_t acc[kMP][NB];
+  f32x4_t acc[MP][NB];
 #pragma unroll
-  for (int a = 0; a < kMP; ++a)
+  for (int a = 0; a < MP; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-  int pi[kMP], pj[kMP];
+  int pi[MP], pj[MP];
 #pragma unroll
-  for (int a = 0; a < kMP; ++a) {
+  for (int a = 0; a < MP; ++a) {
     const int m = m0 + a * 16 + (lane & 15);
     pi[a] = m / x.W;
     pj[a] = m - pi[a] * x.W;
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(View x, View y,
         bf[b] = F::load(w + ((size_t)(tap * COUT + co) * CIN + c0 + kq));
       }
 #pragma unroll
-      for (int a = 0; a < kMP; ++a) {
+      for (int a = 0; a < MP; ++a) {
         const int si = pi[a] + dy, sj = pj[a] + dx;
         typename F::T af = F::zero();
         if (si >= 0 && si < x.H && sj >= 0 && sj < x.W)
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(View x, View y,
     }
   }
 #pragma unroll
-  for (int a = 0; a < kMP; ++a)
+  for (int a = 0; a < MP; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int co = n0 + b * 16 + (lane & 15);
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(View x, View y,
 
 // conv 3x3 input gradient: dx[m][ci] = sum_tap sum_co dyp[m - off][co]
 // WT[tap][ci][co], dyp = dy masked by the layer's ReLU output y.
-template <int CIN, int COUT, int NB>
+template <int CIN, int COUT, int NB, int MP>
 __global__ __launch_bounds__(256) void k_conv3_dgrad(GView dy, View y, GView dx,
                                                      const bf16* __restrict__ wt,
                                                      int relu) {
@@ -203,17 +203,17 @@ __global__ __launch_bounds__(256) void k_conv3_dgrad(GView dy, View y, GView dx,
   typedef Frag<KC> F;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int M = dy.H * dy.W;
-  const int m0 = (blockIdx.x * 4 + wave) * 16 * kMP;
+  const int m0 = (blockIdx.x * 4 + wave) * 16 * MP;
   const int n0 = blockIdx.y * 16 * NB;
   if (m0 >= M) return;
-  f32x4_t acc[kMP][NB];
+  f32x4_t acc[MP][NB];
 #pragma unroll
-  for (int a = 0; a < kMP; ++a)
+  for (int a = 0; a < MP; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-  int pi[kMP], pj[kMP];
+  int pi[MP], pj[MP];
 #pragma unroll
-  for (int a = 0; a < kMP; ++a) {
+  for (int a = 0; a < MP; ++a) {
     const int m = m0 + a * 16 + (lane & 15);
     pi[a] = m / dy.W;
     pj[a] = m - pi[a] * dy.W;
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) void k_conv3_dgrad(GView dy, View y, GView dx,
         bf[b] = F::load(wt + ((size_t)(tap * CIN + ci) * COUT + c0 + kq));
       }
 #pragma unroll
-      for (int a = 0; a < kMP; ++a) {
+      for (int a = 0; a < MP; ++a) {
         const int si = pi[a] - dyy, sj = pj[a] - dxx;
         typename F::T af = F::zero();
         if (si >= 0 && si < dy.H && sj >= 0 && sj < dy.W) {
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void k_conv3_dgrad(GView dy, View y, GView dx,
     }
   }
 #pragma unroll
-  for (int a = 0; a < kMP; ++a)
+  for (int a = 0; a < MP; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int ci = n0 + b * 16 + (lane & 15);
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_conv3_dgrad(GView dy, View y, GView dx,
 // up-convolution 4x4 stride 4 (transposed conv): out[4I+a][4J+b][co] =
 // bias[co] + sum_ci x[I][J][ci] Wu[ab][co][ci].  GEMM rows = coarse pixels,
 // columns = (ab, co).
-template <int CIN, int COUT, int NB>
+template <int CIN, int COUT, int NB, int MP>
 __global__ __launch_bounds__(256) void k_up_fwd(View x, View y,
                                                 const bf16* __restrict__ w,
                                                 const float* __restrict__ bias) {
@@ -269,12 +269,12 @@ __global__ __launch_bounds__(256) void k_up_fwd(View x, View y,
   typedef Frag<KC> F;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int M = x.H * x.W;
-  const int m0 = (blockIdx.x * 4 + wave) * 16 * kMP;
+  const int m0 = (blockIdx.x * 4 + wave) * 16 * MP;
   const int n0 = blockIdx.y * 16 * NB;  // over 16*COUT columns
   if (m0 >= M) return;
-  f32x4_t acc[kMP][NB];
+  f32x4_t acc[MP][NB];
 #pragma unroll
-  for (int a = 0; a < kMP; ++a)
+  for (int a = 0; a < MP; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
   const int kq = F::KPL * (lane >> 4);
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void k_up_fwd(View x, View y,
       bf[b] = F::load(w + ((size_t)n * CIN + c0 + kq));
     }
 #pragma unroll
-    for (int a = 0; a < kMP; ++a) {
+    for (int a = 0; a < MP; ++a) {
       const int m = m0 + a * 16 + (lane & 15);
       typename F::T af = F::zero();
       if (m < M) af = F::load(x.p + (size_t)m * x.cs + x.co + c0 + kq);
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) void k_up_fwd(View x, View y,
     }
   }
 #pragma unroll
-  for (int a = 0; a < kMP; ++a)
+  for (int a = 0; a < MP; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int n = n0 + b * 16 + (lane & 15);
@@ -316,19 +316,19 @@ __global__ __launch_bounds__(256) void k_up_fwd(View x, View y,
 
 // up-convolution input gradient: dx[m][ci] = sum_{ab,co} dy[fine(m,ab)][co]
 // WuT[ab][ci][co]
-template <int CIN, int COUT, int NB>
+template <int CIN, int COUT, int NB, int MP>
 __global__ __launch_bounds__(256) void k_up_dgrad(GView dy, GView dx,
                                                   const bf16* __restrict__ wt) {
   constexpr int KC = (COUT % 32 == 0) ? 32 : 16;
   typedef Frag<KC> F;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int M = dx.H * dx.W;
-  const int m0 = (blockIdx.x * 4 + wave) * 16 * kMP;
+  const int m0 = (blockIdx.x * 4 + wave) * 16 * MP;
   const int n0 = blockIdx.y * 16 * NB;
   if (m0 >= M) return;
-  f32x4_t acc[kMP][NB];
+  f32x4_t acc[MP][NB];
 #pragma unroll
-  for (int a = 0; a < kMP; ++a)
+  for (int a = 0; a < MP; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
   const int kq = F::KPL * (lane >> 4);
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void k_up_dgrad(GView dy, GView dx,
         bf[b] = F::load(wt + ((size_t)(ab * CIN + ci) * COUT + c0 + kq));
       }
 #pragma unroll
-      for (int a = 0; a < kMP; ++a) {
+      for (int a = 0; a < MP; ++a) {
         const int m = m0 + a * 16 + (lane & 15);
         typename F::T af = F::zero();
         if (m < M) {
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(256) void k_up_dgrad(GView dy, GView dx,
     }
   }
 #pragma unroll
-  for (int a = 0; a < kMP; ++a)
+  for (int a = 0; a < MP; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int ci = n0 + b * 16 + (lane & 15);

@@ -1,16 +1,27 @@
 """Training workflow of the learned preconditioner (no reference counterpart).
 
-The network is trained on errors of the problem family it will precondition:
-for a right-hand side b, FP64 Jacobi-PCG gives phi and its own iterates
-x_k; every e_k = phi - x_k is a sample whose residual A e_k = b - A x_k the
-library forms on the device.  The initial error (x_0 = 0) is smooth; later
-errors carry the slowly converging components Jacobi-PCG leaves behind --
-the spectrum a flexible-CG preconditioner has to cover.  Everything runs
-through libcpgpu.so.
+The network is trained on errors of the problems it will precondition.  A
+sample is an error e with its residual r = A e (formed on the device by
+cpgpu_unet_set_sample); the loss is the relative energy norm of the
+network's error on it.  Inside the flexible-CG preconditioner cycle
+(cpgpu_poisson_set_fcg) the network sees the residual left after `nu`
+weighted-Jacobi sweeps, so every sample is stored after the same sweeps.
+
+Samples come from the plain solver and from the learned solver itself:
+  * plain steps: phi_n from FP64 Jacobi-PCG; the error of the warm start
+    phi_n - phi_{n-1}, the error of a zero start phi_n, and Jacobi-PCG
+    iterate errors;
+  * flexible-CG rounds: the errors phi_n - x_k of the flexible-CG iterates
+    x_k run with the current network, which are exactly the inputs the
+    network meets at iteration k + 1 (the training distribution follows the
+    solver's own, round after round).
+Everything runs through libcpgpu.so.
 """
+import time
+
 import numpy as np
 
-from . import NORM_TRUE
+from . import NORM_TRUE, NS, UNET_PRECOND
 
 
 def interior(P, field):
@@ -19,13 +30,16 @@ def interior(P, field):
 
 
 def add_error_samples(P, net, seeds, checkpoints=(0, 16, 64, 256), nblobs=64,
-                      wmin=0.01, wmax=0.1, tol=1e-10, slot0=0):
+                      wmin=0.01, wmax=0.1, tol=1e-10, slot0=0, smooth=0,
+                      omega=0.8):
     """Samples e_k = phi - x_k of Jacobi-PCG on blob right-hand sides.
 
     For each seed: b from 'nblobs' Gaussian blobs, phi by FP64 Jacobi-PCG to
     the true relative residual 'tol' from zero, then the iterates x_k after
-    k = checkpoints Jacobi-PCG iterations from zero.  Returns the next free
-    slot.  Leaves the problem's right-hand side at the last seed's."""
+    k = checkpoints Jacobi-PCG iterations from zero; each stored after
+    `smooth` weighted-Jacobi sweeps (the preconditioner cycle's pre-smoothing,
+    cpgpu_poisson_set_fcg).  Returns the next free slot.  Leaves the
+    problem's right-hand side at the last seed's."""
     P.set_unet(None)
     slot = slot0
     for seed in seeds:
@@ -40,7 +54,7 @@ def add_error_samples(P, net, seeds, checkpoints=(0, 16, 64, 256), nblobs=64,
                 P.fill_phi(0.0)
                 P.solve("JPCG", max_iter=int(k), tol=1e-300, norm=NORM_TRUE)
                 e = phi - interior(P, P.get_vec(0))
-            net.set_sample(P, slot, e)
+            net.set_sample(P, slot, e, smooth=smooth, omega=omega)
             slot += 1
     return slot
 
@@ -52,3 +66,97 @@ def train(net, P, steps, batch=4, lr=1e-3, seed=0, decay_at=0.7):
     l1 = net.train(P, n1, batch=batch, lr=lr, seed=seed)
     l2 = net.train(P, steps - n1, batch=batch, lr=lr * 0.1, seed=seed + 1)
     return np.concatenate([l1, l2])
+
+
+# ------------------------------------------------------------------ cavity
+def cavity(ctx, n, re=1000.0, solver="JPCG", tol=1e-8, norm=NORM_TRUE):
+    """Lid-driven cavity on [-0.5, 0.5]^2, n x n cells, lid velocity 1 on the
+    y+ wall, no-slip elsewhere, Re = re (rho = Re, mu = 1), CFL 0.5,
+    diffusion number 0.25, van Leer TVD advection, dt0 = h / 2; the
+    projection solved to `tol`."""
+    return NS(ctx, n, n, 2, (-0.5, 0.5), (-0.5, 0.5),
+              bcu=((1, 1, 1, 1), [0.0, 0.0, 0.0, 1.0]), bcv=((1, 1, 1, 1), None),
+              rho=re, mu=1.0, cfl=0.5, dnum=0.25, dcoef=1.0 / re,
+              limiter="VanLeer", solver=solver, tol=tol, norm=norm,
+              dt=0.5 / n)
+
+
+class CavityTrainer:
+    """Collects samples from a cavity run and trains the network for the
+    flexible-CG cycle with `nu` pre- and post-sweeps of weight `omega`."""
+
+    def __init__(self, ns, net, nu=2, omega=0.8, tol=1e-8):
+        self.ns, self.net, self.nu, self.omega, self.tol = ns, net, nu, omega, tol
+        self.P = ns.projection()
+        self.P.set_fcg(nu, nu, omega)
+        self.slot = 0
+        self.log = []
+
+    def _add(self, e):
+        self.net.set_sample(self.P, self.slot, e, smooth=self.nu, omega=self.omega)
+        self.slot += 1
+
+    def _plain_step(self):
+        """One step with the plain FP64 solver; returns (phi_prev, phi)
+        as ghost-padded fields."""
+        P = self.P
+        prev = P.get_vec(0)
+        P.set_unet(None, 0)
+        self.ns.set_solver("JPCG", max_iter=10 ** 7, tol=self.tol, norm=NORM_TRUE)
+        self.ns.step(1)
+        return prev, P.get_vec(0)
+
+    def plain_steps(self, steps, every=5, jpcg_checkpoints=(30, 300)):
+        """Plain-solver steps: warm-start errors every step; every `every`
+        steps also the zero-start error and Jacobi-PCG iterate errors."""
+        P = self.P
+        t = time.perf_counter()
+        for s in range(steps):
+            prev, phi_f = self._plain_step()
+            phi = interior(P, phi_f)
+            self._add(phi - interior(P, prev))
+            if s % every == 0:
+                self._add(phi)
+                for k in jpcg_checkpoints:
+                    P.set_phi(prev)
+                    P.solve("JPCG", max_iter=int(k), tol=1e-300, norm=NORM_TRUE)
+                    self._add(phi - interior(P, P.get_vec(0)))
+                P.set_phi(phi_f)
+        self.log.append(("plain", steps, self.slot, time.perf_counter() - t))
+
+    def fcg_steps(self, steps, ks=(1, 2, 4, 8, 16)):
+        """Plain-solver steps whose phi scores the flexible-CG iterates of
+        the current network: errors phi - x_k after k iterations from the
+        warm start."""
+        P = self.P
+        t = time.perf_counter()
+        for _ in range(steps):
+            prev, phi_f = self._plain_step()
+            phi = interior(P, phi_f)
+            self._add(phi - interior(P, prev))
+            P.set_unet(self.net, UNET_PRECOND)
+            for k in ks:
+                P.set_phi(prev)
+                P.solve("FCG_UNET", max_iter=int(k), tol=1e-300, norm=NORM_TRUE)
+                self._add(phi - interior(P, P.get_vec(0)))
+            P.set_unet(None, 0)
+            P.set_phi(phi_f)
+        self.log.append(("fcg", steps, self.slot, time.perf_counter() - t))
+
+    def fit(self, steps, lr, seed, batch=4):
+        t = time.perf_counter()
+        losses = train(self.net, self.P, steps, batch=batch, lr=lr, seed=seed)
+        self.log.append(("train", steps, float(losses[-min(20, steps):].mean()),
+                         time.perf_counter() - t))
+        return losses
+
+    def run(self, plain=100, first_fit=800, rounds=3, round_steps=40,
+            round_fit=500):
+        """The schedule used by the benchmark: plain steps, a first fit, then
+        `rounds` of flexible-CG sample collection and refits."""
+        self.plain_steps(plain)
+        self.fit(first_fit, 1e-3, 0)
+        for r in range(rounds):
+            self.fcg_steps(round_steps)
+            self.fit(round_fit, 5e-4, r + 1)
+        return self.log

@@ -232,6 +232,14 @@ int cpgpu_ns_step(cpgpu_ns *S, int nsteps, int *iters, double *resid);
 int cpgpu_ns_time(cpgpu_ns *S, double *t, double *dt);
 /* The projection problem (all-Neumann Poisson) of the step. */
 cpgpu_poisson *cpgpu_ns_projection(cpgpu_ns *S);
+/* Restore the time state (Time_ t and dt) of a saved step sequence. */
+int cpgpu_ns_set_time(cpgpu_ns *S, double t, double dt);
+/* Projection solver of the following steps: the reference's name switch
+ * NS_::set_projection_solver (s_ns.hpp:120-130) over CPGPU_JPCG, CPGPU_CG,
+ * CPGPU_MIXED and CPGPU_FCG_UNET (a U-Net attached to the projection
+ * problem with cpgpu_poisson_set_unet). */
+int cpgpu_ns_set_solver(cpgpu_ns *S, int solver, int max_iter, double tol,
+                        int norm_mode);
 
 /* ---- learned preconditioner: a BF16 U-Net on MFMA ----
  * No counterpart in the reference: its Poisson solvers are the CSR Krylov
@@ -272,10 +280,13 @@ int cpgpu_unet_tensor(cpgpu_unet *U, int id, int grad, float *host, int *dims);
 int cpgpu_unet_adam(cpgpu_unet *U, float lr);
 
 /* Training set for the problem P: slot k holds an error e (host, interior
- * H*W doubles) and its residual r = A e, computed on the device.  Slots
- * grow on demand; all samples share P's grid. */
+ * H*W doubles) and its residual r = A e, computed on the device.  With
+ * smooth > 0 the stored error is e after that many weighted-Jacobi sweeps
+ * (e <- e - omega D^-1 A e), the error the network meets inside the
+ * preconditioner cycle of cpgpu_poisson_set_fcg.  Slots grow on demand;
+ * all samples share P's grid. */
 int cpgpu_unet_set_sample(cpgpu_unet *U, cpgpu_poisson *P, int slot,
-                          const double *e);
+                          const double *e, int smooth, double omega);
 int cpgpu_unet_num_samples(const cpgpu_unet *U);
 /* steps Adam updates, each on the mean energy-norm loss of batch samples
  * drawn with a seeded generator:  L = (s y - e)' A (s y - e) / e' A e,
@@ -292,12 +303,19 @@ int cpgpu_unet_eval(cpgpu_unet *U, cpgpu_poisson *P, float *losses);
 #define CPGPU_UNET_WARM 1
 #define CPGPU_UNET_PRECOND 2
 int cpgpu_poisson_set_unet(cpgpu_poisson *P, cpgpu_unet *U, int flags);
+/* Preconditioner cycle of CPGPU_FCG_UNET: z = M(r) is `pre` weighted-
+ * Jacobi sweeps (weight omega) from zero, plus the network applied to the
+ * remaining residual, followed by `post` sweeps -- a two-level cycle with
+ * the network as the coarse correction (default 2, 2, 0.8). */
+int cpgpu_poisson_set_fcg(cpgpu_poisson *P, int pre, int post, double omega);
 
 /* ---- kernel timing (HIP events on the context stream) ----
  * When enabled, every launch of the named kernel family is bracketed by
  * events; cpgpu_timer_read returns launches and summed milliseconds.
  * family: 0 = stencil apply (p update + A p + p.q), 1 = vector update
- * (x, r update + dots). */
+ * (x, r update + dots), 2 = the U-Net's 16 -> 16 3x3 convolutions on the
+ * finest level.  Timed launches carry their own start/stop timestamps
+ * (no extra packets between kernels). */
 int cpgpu_timer_enable(cpgpu_ctx *ctx, int family, int on);
 int cpgpu_timer_read(cpgpu_ctx *ctx, int family, int *launches, double *ms);
 

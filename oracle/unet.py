@@ -295,10 +295,12 @@ def adam(p, g, m1, m2, t, lr, b1=0.9, b2=0.999, eps=1e-8):
     """One Adam step (Kingma & Ba) with bias correction, in float32."""
     f = np.float32
     p, g, m1, m2 = (np.asarray(a, dtype=f) for a in (p, g, m1, m2))
-    m1 = f(b1) * m1 + f(1 - b1) * g
-    m2 = f(b2) * m2 + f(1 - b2) * g * g
-    c1 = f(1) - f(b1) ** f(t)
-    c2 = f(1) - f(b2) ** f(t)
+    b1, b2 = f(b1), f(b2)
+    # 1 - beta evaluated in float32, as the device does
+    m1 = b1 * m1 + (f(1) - b1) * g
+    m2 = b2 * m2 + (f(1) - b2) * g * g
+    c1 = f(1) - b1 ** f(t)
+    c2 = f(1) - b2 ** f(t)
     p = p - f(lr) * (m1 / c1) / (np.sqrt(m2 / c2) + f(eps))
     return p, m1, m2
 
