@@ -48,11 +48,15 @@ static const int kLevel[] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 2, 1, 1, 1, 0, 0, 0, 
 static_assert(sizeof(kLevel) / sizeof(kLevel[0]) == kNL, "one level per layer");
 inline int level_of(int l) { return kLevel[l]; }
 
-// chunks of the weight-gradient GEMM over pixels (K dimension)
+// chunks of the weight-gradient GEMM over pixels (its K dimension): at least
+// 256 pixels each, at most 256 chunks, so that small levels still launch
+// many blocks
 inline int wg_chunks(int M) {
-  int c = (M + 2047) / 2048;
-  return c < 256 ? c : 256;
+  int c = M / 256;
+  return c < 1 ? 1 : (c < 256 ? c : 256);
 }
+
+constexpr int kRedBlocks = 1024;  // blocks of the per-channel reductions
 
 // ---------------------------------------------------------------------
 // small kernels
@@ -199,11 +203,14 @@ __global__ __launch_bounds__(256) void k_pool_bwd(GView gout, GView gin, int C) 
       gout.p[mc * gout.cs + gout.co + c] * (1.f / 16.f);
 }
 
-// bias gradient partials: sum over pixels of g (masked by y when relu);
-// for up layers the fine pixels of the output.  C divides 256, so thread t
-// always accumulates channel t % C (coalesced over (pixel, channel)).
-__global__ __launch_bounds__(256) void k_bias_grad(GView g, View y, int relu,
-                                                   int C, float* __restrict__ part) {
+// Masked output gradient of a layer, once per backward: gq[m][c] =
+// bf16(g[m][c] * relu'(y[m][c])) (relu'= 1 when relu == 0), dense with C
+// channels, and per-block partial sums of the unrounded masked g per
+// channel (the bias gradient).  C divides 256, so thread t always handles
+// channel t % C over the (pixel, channel) elements.
+__global__ __launch_bounds__(256) void k_mask_grad(GView g, View y, int relu, int C,
+                                                   bf16* __restrict__ gq,
+                                                   float* __restrict__ part) {
   __shared__ float red[256];
   const size_t n = (size_t)g.H * g.W * C;
   float s = 0.f;
@@ -213,6 +220,7 @@ __global__ __launch_bounds__(256) void k_bias_grad(GView g, View y, int relu,
     const int c = (int)(e - m * C);
     float v = g.p[m * g.cs + g.co + c];
     if (relu && !(bf2f(y.p[m * y.cs + y.co + c]) > 0.f)) v = 0.f;
+    gq[e] = f2bf(v);
     s += v;
   }
   red[threadIdx.x] = s;
@@ -225,15 +233,27 @@ __global__ __launch_bounds__(256) void k_bias_grad(GView g, View y, int relu,
 }
 
 // out[e] = sum_k part[k*count + e], fixed order
+// (16 elements per block, 16 threads per element: thread q sums chunks
+// q, q + 16, ...; then a fixed tree over the 16)
 __global__ __launch_bounds__(256) void k_sum_chunks(const float* __restrict__ part,
                                                     int nchunks, int count,
                                                     float* __restrict__ out) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= count) return;
+  __shared__ float red[16][17];
+  const int el = threadIdx.x & 15, q = threadIdx.x >> 4;
+  const int e = blockIdx.x * 16 + el;
   float s = 0.f;
-  for (int k = 0; k < nchunks; ++k) s += part[(size_t)k * count + e];
-  out[e] = s;
+  if (e < count)
+    for (int k = q; k < nchunks; k += 16) s += part[(size_t)k * count + e];
+  red[q][el] = s;
+  __syncthreads();
+  for (int h = 8; h > 0; h >>= 1) {
+    if (q < h) red[q][el] += red[q + h][el];
+    __syncthreads();
+  }
+  if (q == 0 && e < count) out[e] = red[0][el];
 }
+
+inline dim3 sum_grid(int count) { return dim3((count + 15) / 16); }
 
 // Adam with bias correction on the FP32 master parameters
 __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p,
@@ -437,6 +457,8 @@ struct cpgpu_unet {
   std::vector<std::unique_ptr<Sample>> samples;
   int sH = 0, sW = 0;  // grid of the samples
   DevBuf<float> gacc, dbuf, qbuf, lossdev;
+  DevBuf<bf16> gq;       // masked BF16 output gradient of the current layer
+  DevBuf<float> bpart;   // per-block bias partials
   DevBuf<double> tmp64, tmp64b, lpart;
 
   View v(const T& t, int co = 0) { return View{act.p + t.off, t.H, t.W, t.C, co}; }
@@ -473,7 +495,7 @@ int unet_alloc(cpgpu_unet* U, int H, int W) {
   const size_t N = (size_t)H * W;
   // partial sums of the largest weight-gradient launch (chunks x taps x
   // cout x cin), of the bias/first/output reductions otherwise
-  size_t npart = 64 * 160;
+  size_t npart = (size_t)kRedBlocks * 160;
   for (int l = 0; l < kNL; ++l) {
     const LayerDef& d = kLayers[l];
     if (d.kind != K_CONV && d.kind != K_UP) continue;
@@ -482,7 +504,8 @@ int unet_alloc(cpgpu_unet* U, int H, int W) {
     npart = std::max(npart, (size_t)wg_chunks((int)M) * ntaps(d.kind) * d.cout * d.cin);
   }
   if (U->act.alloc(off) || U->gact.alloc(off) || U->x0.alloc(N) ||
-      U->y.alloc(N) || U->dy.alloc(N) || U->part.alloc(npart))
+      U->y.alloc(N) || U->dy.alloc(N) || U->part.alloc(npart) ||
+      U->gq.alloc(N * 16) || U->bpart.alloc((size_t)kRedBlocks * 128))
     return set_error(U->ctx, CPGPU_ENOMEM, "U-Net activations");
   U->H = H;
   U->W = W;
@@ -493,6 +516,13 @@ int unet_alloc(cpgpu_unet* U, int H, int W) {
 // levels with fewer than kSmallM pixels 16 x 16, so that the few pixels
 // still spread over many wavefronts.
 constexpr int kSmallM = 16384;
+
+// the LDS-tiled 3x3 convolution: sides multiples of 16, at most 64 input
+// channels (18 x 18 x 64 BF16 = 41.5 KB of LDS)
+template <int KCH>
+inline bool tiled(int H, int W) {
+  return KCH <= 64 && H % 16 == 0 && W % 16 == 0;
+}
 
 template <int NB, int MP>
 inline dim3 tile_grid(int M, int ncols) {
@@ -508,12 +538,20 @@ void conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu) {
   // timer family 2: the 16 -> 16 convolutions of the finest level
   const EvPair ev = (CIN == 16 && COUT == 16 && M >= kSmallM)
                         ? timer_events(U->ctx, 2) : EvPair();
-  if (M < kSmallM)
+  if (M < kSmallM) {
     hipLaunchKernelGGL((k_conv3_fwd<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, COUT)),
                        dim3(256), 0, U->ctx->stream, x, y, w, b, relu);
-  else
-    CPGPU_LAUNCH(ev, (k_conv3_fwd<CIN, COUT, NB, 2>), (tile_grid<NB, 2>(M, COUT)),
-                 dim3(256), 0, U->ctx->stream, x, y, w, b, relu);
+    return;
+  }
+  if constexpr (CIN <= 64) {
+    if (tiled<CIN>(x.H, x.W)) {
+      CPGPU_LAUNCH(ev, (k_conv3_tile<CIN, COUT, true>), dim3(x.W / 16, x.H / 16),
+                   dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu);
+      return;
+    }
+  }
+  CPGPU_LAUNCH(ev, (k_conv3_fwd<CIN, COUT, NB, 2>), (tile_grid<NB, 2>(M, COUT)),
+               dim3(256), 0, U->ctx->stream, x, y, w, b, relu);
 }
 
 template <int CIN, int COUT>
@@ -566,35 +604,66 @@ int forward(cpgpu_unet* U) {
   return CPGPU_OK;
 }
 
+// gq = masked BF16 output gradient (dense, COUT channels) of `out` at
+// channel offset out_co, and the bias gradient of layer l
+int mask_grad(cpgpu_unet* U, const cpgpu_unet::T& out, int out_co, int cout,
+              int relu, int l) {
+  hipStream_t st = U->ctx->stream;
+  hipLaunchKernelGGL(k_mask_grad, dim3(kRedBlocks), dim3(256), 0, st,
+                     U->g(out, out_co), U->v(out, out_co), relu, cout, U->gq.p,
+                     U->bpart.p);
+  hipLaunchKernelGGL(k_sum_chunks, sum_grid(cout), dim3(256), 0, st, U->bpart.p,
+                     kRedBlocks, cout, U->grads.p + U->boff[l]);
+  return CPGPU_OK;
+}
+
 template <int CIN, int COUT>
 int conv_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, int in_co,
              const cpgpu_unet::T& out, int out_co, int l, bool need_dx) {
   hipStream_t st = U->ctx->stream;
-  GView gout = U->g(out, out_co);
-  View yout = U->v(out, out_co);
+  mask_grad(U, out, out_co, COUT, 1, l);
+  const View gq{U->gq.p, out.H, out.W, COUT, 0};
   const int M = in.H * in.W;
   if (need_dx) {
     constexpr int NB = CIN / 16 < 4 ? CIN / 16 : 4;
     const bf16* wt = U->wtbf.p + U->wbf_off[l];
-    if (M < kSmallM)
+    bool done = false;
+    if (M < kSmallM) {
       hipLaunchKernelGGL((k_conv3_dgrad<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, CIN)),
-                         dim3(256), 0, st, gout, yout, U->g(in, in_co), wt, 1);
-    else
+                         dim3(256), 0, st, gq, U->g(in, in_co), wt);
+      done = true;
+    }
+    if constexpr (COUT <= 64) {
+      if (!done && tiled<COUT>(in.H, in.W)) {
+        hipLaunchKernelGGL((k_conv3_tile<COUT, CIN, false>), dim3(in.W / 16, in.H / 16),
+                           dim3(256), 0, st, gq, View{}, U->g(in, in_co), wt,
+                           (const float*)nullptr, 0);
+        done = true;
+      }
+    }
+    if (!done)
       hipLaunchKernelGGL((k_conv3_dgrad<CIN, COUT, NB, 2>), (tile_grid<NB, 2>(M, CIN)),
-                         dim3(256), 0, st, gout, yout, U->g(in, in_co), wt, 1);
+                         dim3(256), 0, st, gq, U->g(in, in_co), wt);
   }
-  const int nch = wg_chunks(M);
-  const int ppc = (M + nch - 1) / nch;
-  hipLaunchKernelGGL((k_wgrad<CIN, COUT, false>), dim3(nch, 9), dim3(256), 0, st,
-                     gout, yout, U->v(in, in_co), 1, ppc, U->part.p);
+  int nch = wg_chunks(M);
+  bool tiled_wg = false;
+  if constexpr (512 * COUT + 1728 * CIN <= 65536) {
+    if (M >= kSmallM && in.H % 16 == 0 && in.W % 16 == 0) {
+      const int ntiles = (in.H / 16) * (in.W / 16);
+      nch = ntiles < 256 ? ntiles : 256;
+      hipLaunchKernelGGL((k_wgrad_tile<CIN, COUT>), dim3(nch), dim3(256), 0, st, gq,
+                         U->v(in, in_co), U->part.p);
+      tiled_wg = true;
+    }
+  }
+  if (!tiled_wg) {
+    const int ppc = (M + nch - 1) / nch;
+    hipLaunchKernelGGL((k_wgrad<CIN, COUT, false>), dim3(nch, 9), dim3(256), 0, st,
+                       gq, U->v(in, in_co), ppc, U->part.p);
+  }
   const int count = 9 * COUT * CIN;
-  hipLaunchKernelGGL(k_sum_chunks, dim3((count + 255) / 256), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_sum_chunks, sum_grid(count), dim3(256), 0, st,
                      U->part.p, nch, count, U->grads.p + U->woff[l]);
-  const int nb = 64;
-  hipLaunchKernelGGL(k_bias_grad, dim3(nb), dim3(256), 0, st, gout, yout, 1,
-                     COUT, U->part.p);
-  hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, st, U->part.p, nb,
-                     COUT, U->grads.p + U->boff[l]);
   return CPGPU_OK;
 }
 
@@ -602,30 +671,27 @@ template <int CIN, int COUT>
 int up_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, const cpgpu_unet::T& out,
            int l) {
   hipStream_t st = U->ctx->stream;
-  GView gout = U->g(out, 0);  // channels [0, COUT) of the concat gradient
+  // channels [0, COUT) of the concat gradient; the up-conv has no ReLU
+  mask_grad(U, out, 0, COUT, 0, l);
+  const View gq{U->gq.p, out.H, out.W, COUT, 0};
   const int M = in.H * in.W;
   {
     constexpr int NB = CIN / 16 < 4 ? CIN / 16 : 4;
     const bf16* wt = U->wtbf.p + U->wbf_off[l];
     if (M < kSmallM)
       hipLaunchKernelGGL((k_up_dgrad<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, CIN)),
-                         dim3(256), 0, st, gout, U->g(in), wt);
+                         dim3(256), 0, st, gq, U->g(in), wt);
     else
       hipLaunchKernelGGL((k_up_dgrad<CIN, COUT, NB, 2>), (tile_grid<NB, 2>(M, CIN)),
-                         dim3(256), 0, st, gout, U->g(in), wt);
+                         dim3(256), 0, st, gq, U->g(in), wt);
   }
   const int nch = wg_chunks(M);
   const int ppc = (M + nch - 1) / nch;
   hipLaunchKernelGGL((k_wgrad<CIN, COUT, true>), dim3(nch, 16), dim3(256), 0, st,
-                     gout, U->v(out), U->v(in), 0, ppc, U->part.p);
+                     gq, U->v(in), ppc, U->part.p);
   const int count = 16 * COUT * CIN;
-  hipLaunchKernelGGL(k_sum_chunks, dim3((count + 255) / 256), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_sum_chunks, sum_grid(count), dim3(256), 0, st,
                      U->part.p, nch, count, U->grads.p + U->woff[l]);
-  const int nb = 64;
-  hipLaunchKernelGGL(k_bias_grad, dim3(nb), dim3(256), 0, st, gout, U->v(out),
-                     0, COUT, U->part.p);
-  hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, st, U->part.p, nb,
-                     COUT, U->grads.p + U->boff[l]);
   return CPGPU_OK;
 }
 
@@ -633,11 +699,11 @@ int up_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, const cpgpu_unet::T& out,
 int backward(cpgpu_unet* U) {
   hipStream_t st = U->ctx->stream;
   const int M = U->H * U->W;
-  const int nb = 64;
+  const int nb = kRedBlocks;
   hipLaunchKernelGGL(k_out_bwd, dim3(nb), dim3(256), 0, st, U->v(U->d0),
                      U->params.p + U->woff[17], U->dy.p, U->g(U->d0), M,
                      U->part.p);
-  hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, st, U->part.p, nb, 17,
+  hipLaunchKernelGGL(k_sum_chunks, sum_grid(17), dim3(256), 0, st, U->part.p, nb, 17,
                      U->grads.p + U->woff[17]);  // 16 weights then the bias
   conv_bwd<16, 16>(U, U->d0a, 0, U->d0, 0, 16, true);
   conv_bwd<32, 16>(U, U->C0, 0, U->d0a, 0, 15, true);
@@ -664,10 +730,10 @@ int backward(cpgpu_unet* U) {
   conv_bwd<16, 32>(U, U->p0, 0, U->a1, 0, 2, true);
   unpool(U->p0, U->C0, 16, 16);
   conv_bwd<16, 16>(U, U->a0, 0, U->C0, 16, 1, true);
-  const int nbf = 64;
+  const int nbf = kRedBlocks;
   hipLaunchKernelGGL(k_first_wgrad, dim3(nbf), dim3(256), 0, st, U->x0.p, U->H,
                      U->W, U->g(U->a0), U->v(U->a0), U->part.p);
-  hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, st, U->part.p, nbf,
+  hipLaunchKernelGGL(k_sum_chunks, sum_grid(160), dim3(256), 0, st, U->part.p, nbf,
                      160, U->grads.p + U->woff[0]);  // 144 weights then 16 biases
   CPGPU_HIP(U->ctx, hipGetLastError());
   return CPGPU_OK;

@@ -77,18 +77,6 @@ struct Frag<16> {
     T v = {0, 0, 0, 0};
     return v;
   }
-  __device__ static T from_f32(const float* g, const bf16* y, bool mask) {
-    T v;
-    short s[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      float x = g[t];
-      if (mask && !(bf2f(y[t]) > 0.f)) x = 0.f;
-      s[t] = (short)f2bf(x);
-    }
-    __builtin_memcpy(&v, s, 8);
-    return v;
-  }
 };
 template <>
 struct Frag<32> {
@@ -106,18 +94,6 @@ struct Frag<32> {
     T v;
     const uint32_t z[4] = {0, 0, 0, 0};
     __builtin_memcpy(&v, z, 16);
-    return v;
-  }
-  __device__ static T from_f32(const float* g, const bf16* y, bool mask) {
-    T v;
-    bf16 s[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      float x = g[t];
-      if (mask && !(bf2f(y[t]) > 0.f)) x = 0.f;
-      s[t] = f2bf(x);
-    }
-    __builtin_memcpy(&v, s, 16);
     return v;
   }
 };
@@ -193,12 +169,12 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(View x, View y,
     }
 }
 
-// conv 3x3 input gradient: dx[m][ci] = sum_tap sum_co dyp[m - off][co]
-// WT[tap][ci][co], dyp = dy masked by the layer's ReLU output y.
+// conv 3x3 input gradient: dx[m][ci] = sum_tap sum_co g[m - off][co]
+// WT[tap][ci][co], g = the layer's output gradient masked by its ReLU and
+// rounded to BF16 once (k_mask_grad), read like an activation.
 template <int CIN, int COUT, int NB, int MP>
-__global__ __launch_bounds__(256) void k_conv3_dgrad(GView dy, View y, GView dx,
-                                                     const bf16* __restrict__ wt,
-                                                     int relu) {
+__global__ __launch_bounds__(256) void k_conv3_dgrad(View dy, GView dx,
+                                                     const bf16* __restrict__ wt) {
   constexpr int KC = (COUT % 32 == 0) ? 32 : 16;
   typedef Frag<KC> F;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -235,11 +211,8 @@ __global__ __launch_bounds__(256) void k_conv3_dgrad(GView dy, View y, GView dx,
       for (int a = 0; a < MP; ++a) {
         const int si = pi[a] - dyy, sj = pj[a] - dxx;
         typename F::T af = F::zero();
-        if (si >= 0 && si < dy.H && sj >= 0 && sj < dy.W) {
-          const size_t sm = (size_t)(si * dy.W + sj);
-          af = F::from_f32(dy.p + sm * dy.cs + dy.co + c0 + kq,
-                           y.p + sm * y.cs + y.co + c0 + kq, relu != 0);
-        }
+        if (si >= 0 && si < dy.H && sj >= 0 && sj < dy.W)
+          af = F::load(dy.p + (size_t)(si * dy.W + sj) * dy.cs + dy.co + c0 + kq);
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(af, bf[b], acc[a][b]);
       }
@@ -256,6 +229,88 @@ __global__ __launch_bounds__(256) void k_conv3_dgrad(GView dy, View y, GView dx,
         if (m < M) dx.p[(size_t)m * dx.cs + dx.co + ci] = acc[a][b][t];
       }
     }
+}
+
+// 3x3 convolution (pad 1) of one 16 x 16 pixel tile per block, the input
+// tile and its halo staged once in LDS (pixel-major, KCH channels
+// contiguous, 16-byte chunks), for levels whose sides are multiples of 16.
+// Every tap's A fragment is then an aligned 16-byte LDS read.
+//   FWD:  y[p][n]  = relu(bias[n] + sum_tap sum_k w[tap][n][k] x[p + off][k])
+//         (BF16 out; off = (tap / 3 - 1, tap % 3 - 1))
+//   !FWD: dx[p][n] = sum_tap sum_k w[tap][n][k] x[p - off][k]
+//         (the input gradient: x = masked BF16 output gradient, w = the
+//         transposed weights [tap][ci][co], FP32 out)
+// Wave w computes tile rows 4w..4w+3 (16 pixels each) x NO channels.
+template <int KCH, int NO, bool FWD>
+__global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
+                                                    const bf16* __restrict__ w,
+                                                    const float* __restrict__ bias,
+                                                    int relu) {
+  constexpr int KC = (KCH % 32 == 0) ? 32 : 16;
+  constexpr int NB = NO / 16;
+  constexpr int TP = 18;       // tile side with halo
+  constexpr int CPP = KCH / 8;  // 16-byte chunks per pixel
+  typedef Frag<KC> F;
+  __shared__ __attribute__((aligned(16))) bf16 sx[TP * TP * KCH];
+  const int tj0 = blockIdx.x * 16, ti0 = blockIdx.y * 16;
+  const int H = x.H, W = x.W;
+  for (int e = threadIdx.x; e < TP * TP * CPP; e += 256) {
+    const int pix = e / CPP, ch = e - pix * CPP;
+    const int r = pix / TP, c = pix - r * TP;
+    const int gi = ti0 + r - 1, gj = tj0 + c - 1;
+    uint4 v = {0u, 0u, 0u, 0u};
+    if (gi >= 0 && gi < H && gj >= 0 && gj < W)
+      v = *reinterpret_cast<const uint4*>(x.p + ((size_t)gi * W + gj) * x.cs + x.co + ch * 8);
+    *reinterpret_cast<uint4*>(sx + pix * KCH + ch * 8) = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int px = lane & 15, kq = F::KPL * (lane >> 4);
+  f32x4_t acc[4][NB];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dy = FWD ? tap / 3 - 1 : 1 - tap / 3;
+    const int dx = FWD ? tap % 3 - 1 : 1 - tap % 3;
+#pragma unroll
+    for (int c0 = 0; c0 < KCH; c0 += KC) {
+      typename F::T bf[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        bf[b] = F::load(w + ((size_t)(tap * NO + b * 16 + px) * KCH + c0 + kq));
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int r = wave * 4 + a;
+        const typename F::T af =
+            F::load(sx + ((r + 1 + dy) * TP + px + 1 + dx) * KCH + c0 + kq);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(af, bf[b], acc[a][b]);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int gi = ti0 + wave * 4 + a;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int n = b * 16 + px;
+      const float bb = FWD ? bias[n] : 0.f;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const size_t m = (size_t)gi * W + tj0 + 4 * (lane >> 4) + t;
+        if (FWD) {
+          float v = acc[a][b][t] + bb;
+          if (relu) v = v > 0.f ? v : 0.f;
+          yb.p[m * yb.cs + yb.co + n] = f2bf(v);
+        } else {
+          yf.p[m * yf.cs + yf.co + n] = acc[a][b][t];
+        }
+      }
+    }
+  }
 }
 
 // up-convolution 4x4 stride 4 (transposed conv): out[4I+a][4J+b][co] =
@@ -317,7 +372,7 @@ __global__ __launch_bounds__(256) void k_up_fwd(View x, View y,
 // up-convolution input gradient: dx[m][ci] = sum_{ab,co} dy[fine(m,ab)][co]
 // WuT[ab][ci][co]
 template <int CIN, int COUT, int NB, int MP>
-__global__ __launch_bounds__(256) void k_up_dgrad(GView dy, GView dx,
+__global__ __launch_bounds__(256) void k_up_dgrad(View dy, GView dx,
                                                   const bf16* __restrict__ wt) {
   constexpr int KC = (COUT % 32 == 0) ? 32 : 16;
   typedef Frag<KC> F;
@@ -350,7 +405,7 @@ __global__ __launch_bounds__(256) void k_up_dgrad(GView dy, GView dx,
         if (m < M) {
           const int I = m / dx.W, J = m - I * dx.W;
           const size_t f = (size_t)(4 * I + ai) * dy.W + (4 * J + bj);
-          af = F::from_f32(dy.p + f * dy.cs + dy.co + c0 + kq, nullptr, false);
+          af = F::load(dy.p + f * dy.cs + dy.co + c0 + kq);
         }
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(af, bf[b], acc[a][b]);
@@ -376,11 +431,11 @@ __global__ __launch_bounds__(256) void k_up_dgrad(GView dy, GView dx,
 // stages 64-pixel slices of both operands in LDS transposed (channel-major,
 // pixels contiguous) so every fragment is one 16-byte LDS read, and writes
 // FP32 partial sums per chunk (reduced in a fixed order afterwards).
-// conv: g = dy[p] * relu'(y[p]), x = X[p + off(tap)], taps 0..8
-// up:   g = dy[fine(p, tap)],     x = X[p],           taps 0..15
+// conv: g = gq[p],           x = X[p + off(tap)], taps 0..8
+// up:   g = gq[fine(p, tap)], x = X[p],           taps 0..15
+// (gq: the masked BF16 output gradient of k_mask_grad)
 template <int CIN, int COUT, bool UP>
-__global__ __launch_bounds__(256) void k_wgrad(GView dy, View y, View x,
-                                               int relu, int pix_per_chunk,
+__global__ __launch_bounds__(256) void k_wgrad(View dy, View x, int pix_per_chunk,
                                                float* __restrict__ part) {
   constexpr int PS = 64;          // pixels per LDS slice
   constexpr int PITCH = PS + 8;   // bf16 per LDS row (16-byte aligned rows)
@@ -405,20 +460,16 @@ __global__ __launch_bounds__(256) void k_wgrad(GView dy, View y, View x,
     for (int e = threadIdx.x; e < PS * COUT; e += 256) {
       const int pp = e / COUT, c = e - pp * COUT;
       const int p = ps + pp;
-      float v = 0.f;
+      bf16 v = 0;
       if (p < p1) {
-        const int I = p / x.W, J = p - I * x.W;
-        size_t f;
+        size_t f = (size_t)p;
         if (UP) {
+          const int I = p / x.W, J = p - I * x.W;
           f = (size_t)(4 * I + ai) * dy.W + (4 * J + bj);
-          v = dy.p[f * dy.cs + dy.co + c];
-        } else {
-          f = (size_t)p;
-          v = dy.p[f * dy.cs + dy.co + c];
-          if (relu && !(bf2f(y.p[f * y.cs + y.co + c]) > 0.f)) v = 0.f;
         }
+        v = dy.p[f * dy.cs + dy.co + c];
       }
-      sg[c * PITCH + pp] = f2bf(v);
+      sg[c * PITCH + pp] = v;
     }
     for (int e = threadIdx.x; e < PS * CIN; e += 256) {
       const int pp = e / CIN, c = e - pp * CIN;
@@ -461,6 +512,92 @@ __global__ __launch_bounds__(256) void k_wgrad(GView dy, View y, View x,
     for (int r = 0; r < 4; ++r) {
       const int co = tco * 16 + 4 * (lane >> 4) + r;
       out[(size_t)co * CIN + ci] = acc[t][r];
+    }
+  }
+}
+
+// Weight gradient of a 3x3 convolution on 16 x 16 pixel tiles through LDS,
+// all nine taps per block (levels whose sides are multiples of 16 and whose
+// tiles fit: 512 COUT + 1728 CIN bytes <= 64 KiB):
+//   dW[tap][co][ci] += sum_p gq[p][co] x[p + off(tap)][ci]
+// gT holds the tile's masked BF16 output gradient channel-major; xs holds
+// the input tile with its halo rows in three copies shifted by one column
+// (dx = -1, 0, +1), channel-major, so that every MFMA operand fragment is an
+// aligned 16-byte LDS read.  Block b walks tiles b, b + G, ...; the (tap,
+// co-block, ci-block) products are spread over the four waves; partials
+// part[((b * 9 + tap) * COUT + co) * CIN + ci] are summed over blocks
+// afterwards in a fixed order.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_wgrad_tile(View gq, View x,
+                                                    float* __restrict__ part) {
+  constexpr int TCO = COUT / 16, TCI = CIN / 16, NCOMB = 9 * TCO * TCI;
+  constexpr int CPW = (NCOMB + 3) / 4;  // products per wave
+  __shared__ __attribute__((aligned(16))) bf16 gT[COUT * 256];
+  __shared__ __attribute__((aligned(16))) bf16 xs[3 * CIN * 18 * 16];
+  typedef Frag<32> F;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int H = x.H, W = x.W, ntj = W / 16, ntiles = (H / 16) * ntj;
+  f32x4_t acc[CPW];
+#pragma unroll
+  for (int q = 0; q < CPW; ++q) acc[q] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int ti0 = (t / ntj) * 16, tj0 = (t % ntj) * 16;
+    __syncthreads();  // the previous tile's reads are done
+    // gT[co][p]: 256 pixels x COUT channels, 8-channel chunks
+    for (int e = threadIdx.x; e < 256 * (COUT / 8); e += 256) {
+      const int p = e / (COUT / 8), ch = e - p * (COUT / 8);
+      const size_t m = (size_t)(ti0 + p / 16) * W + tj0 + (p % 16);
+      bf16 v[8];
+      __builtin_memcpy(v, gq.p + m * gq.cs + gq.co + ch * 8, 16);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) gT[(ch * 8 + k) * 256 + p] = v[k];
+    }
+    // xs[d][ci][row][c] = x[ti0 + row - 1][tj0 + c + d - 1], halo pixel
+    // (row, hc) with hc = c + d lands in up to three copies
+    for (int e = threadIdx.x; e < 18 * 18 * (CIN / 8); e += 256) {
+      const int pix = e / (CIN / 8), ch = e - pix * (CIN / 8);
+      const int row = pix / 18, hc = pix - row * 18;
+      const int gi = ti0 + row - 1, gj = tj0 + hc - 1;
+      bf16 v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (gi >= 0 && gi < H && gj >= 0 && gj < W)
+        __builtin_memcpy(v, x.p + ((size_t)gi * W + gj) * x.cs + x.co + ch * 8, 16);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int c = hc - d;
+        if (c < 0 || c >= 16) continue;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          xs[((d * CIN + ch * 8 + k) * 18 + row) * 16 + c] = v[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+      const int comb = wave + 4 * q;
+      if (comb >= NCOMB) break;
+      const int tap = comb / (TCO * TCI), rem = comb - tap * (TCO * TCI);
+      const int tco = rem / TCI, tci = rem - tco * TCI;
+      const int d = tap % 3, rshift = (tap / 3) * 16;  // rows dy + 1
+      const bf16* ga = gT + (tco * 16 + (lane & 15)) * 256 + 8 * (lane >> 4);
+      const bf16* xb = xs + ((d * CIN + tci * 16 + (lane & 15)) * 18) * 16 + rshift +
+                       8 * (lane >> 4);
+#pragma unroll
+      for (int k0 = 0; k0 < 256; k0 += 32)
+        acc[q] = F::mfma(F::load(ga + k0), F::load(xb + k0), acc[q]);
+    }
+  }
+  float* out = part + (size_t)blockIdx.x * 9 * COUT * CIN;
+#pragma unroll
+  for (int q = 0; q < CPW; ++q) {
+    const int comb = wave + 4 * q;
+    if (comb >= NCOMB) break;
+    const int tap = comb / (TCO * TCI), rem = comb - tap * (TCO * TCI);
+    const int tco = rem / TCI, tci = rem - tco * TCI;
+    const int ci = tci * 16 + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int co = tco * 16 + 4 * (lane >> 4) + r;
+      out[((size_t)tap * COUT + co) * CIN + ci] = acc[q][r];
     }
   }
 }

@@ -111,12 +111,15 @@ EXACT_DX = {16: "d0a", 14: "d1", 13: "d1a", 11: "d2", 10: "d2a", 8: "b3",
             7: "a3", 6: "p2", 5: "a2", 4: "p1", 3: "a1", 2: "p0", 1: "a0"}
 
 
-def test_layerwise_parity_on_device_activations(ctx):
+@pytest.mark.parametrize("H,W", [(128, 192), (512, 512)])
+def test_layerwise_parity_on_device_activations(ctx, H, W):
     """Every layer recomputed by the oracle from the device's own inputs
     (teacher forcing), so BF16 rounding cannot compound across layers:
     forward outputs, weight and bias gradients and input gradients agree to
-    FP32 accumulation order (plus rare one-ulp BF16 rounding flips)."""
-    H, W = 128, 192
+    FP32 accumulation order (plus rare one-ulp BF16 rounding flips).  The two
+    sizes cover every kernel path: LDS-tiled convolutions on the finest
+    level (both sizes) and on the second level (512^2), the wide and the
+    narrow wavefront tiles elsewhere."""
     say(f"unet layer-by-layer parity {H}x{W}")
     net = ca.UNet(ctx, seed=1)
     p = random_params(77).astype(np.float32)
