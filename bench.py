@@ -54,10 +54,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--train-plain", type=int, default=100,
                     help="plain-solver cavity steps whose errors seed training")
-    ap.add_argument("--rounds", type=int, default=3,
+    ap.add_argument("--rounds", type=int, default=5,
                     help="flexible-CG sample rounds after the first fit")
-    ap.add_argument("--first-fit", type=int, default=800)
-    ap.add_argument("--round-fit", type=int, default=500)
+    ap.add_argument("--first-fit", type=int, default=2000)
+    ap.add_argument("--round-fit", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hbm", action="store_true",
                     help="skip the 4096^2 Krylov-kernel HBM probe")

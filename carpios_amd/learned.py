@@ -150,8 +150,8 @@ class CavityTrainer:
                          time.perf_counter() - t))
         return losses
 
-    def run(self, plain=100, first_fit=800, rounds=3, round_steps=40,
-            round_fit=500):
+    def run(self, plain=100, first_fit=2000, rounds=5, round_steps=40,
+            round_fit=1500):
         """The schedule used by the benchmark: plain steps, a first fit, then
         `rounds` of flexible-CG sample collection and refits."""
         self.plain_steps(plain)
