@@ -1,0 +1,446 @@
+// The reference's other two structured solvers on the device, matrix-free:
+//   carpio::Dia_BiCGSTAB    (lib/algebra/solver_matrix.hpp:599-672)
+//   carpio::Solver_Jacobi_  (lib/algebra/solver_matrix.hpp:820-910)
+// with their quirks: BiCGSTAB's half-step exit on ||s|| leaves max_iter
+// unchanged (:642-647) and codes 2 (rho = 0) / 3 (omega = 0); Jacobi returns
+// the previous iterate on convergence (:898-903).  FP64, no FMA contraction
+// (the Makefile builds with -ffp-contract=off): the elementwise arithmetic
+// and the operator rows are the reference's; dot products are associated
+// per block (deterministic), so iterates agree with the reference to
+// rounding.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "poisson.hpp"
+
+namespace cpgpu {
+namespace {
+
+constexpr int kG1 = 1024;  // blocks of the 1-D kernels
+
+// partials of x.y (NV = 1) or of x.y and u.w (NV = 2); with norm_ref the
+// last element is left out of the sums (carpio::nrm2, BLAS_Level1.hpp:569)
+template <int NV>
+__global__ __launch_bounds__(256) void k_dots(const double* __restrict__ x,
+                                              const double* __restrict__ y,
+                                              const double* __restrict__ u,
+                                              const double* __restrict__ w,
+                                              size_t n, int skip_last,
+                                              double* part) {
+  __shared__ double lds[4 * NV];
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  const size_t m = skip_last ? n - 1 : n;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < m;
+       k += (size_t)gridDim.x * blockDim.x) {
+    v[0] += x[k] * y[k];
+    if constexpr (NV == 2) v[1] += u[k] * w[k];
+  }
+  store_partials<double, NV>(v, part, gridDim.x, blockIdx.x, lds);
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_total_n(const double* part, int G,
+                                                 double* out) {
+  __shared__ double lds[4 * NV];
+  double t[NV];
+  reduce_partials<double, NV>(part, G, t, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = t[k];
+  }
+}
+
+// p = r (first) or p = r + beta (p - omega v); ph = p * (1 / a_ii)
+__global__ __launch_bounds__(256) void k_bicg_p(Sten<double> s, const double* __restrict__ r,
+                                                double* __restrict__ p,
+                                                const double* __restrict__ v,
+                                                double* __restrict__ ph, double beta,
+                                                double omega, int first, Tiling tl) {
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny) {
+        const size_t idx = (size_t)i * s.ny + j;
+        const double pv = first ? r[idx] : r[idx] + beta * (p[idx] - omega * v[idx]);
+        p[idx] = pv;
+        ph[idx] = pv * (1. / diag_at(s, i, j));
+      }
+    }
+  }
+}
+
+// y = A x
+__global__ __launch_bounds__(256) void k_apply64(Sten<double> s, const double* __restrict__ x,
+                                                 double* __restrict__ y, Tiling tl) {
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny)
+        y[(size_t)i * s.ny + j] = apply_global(s, x, i, j, diag_at(s, i, j));
+    }
+  }
+}
+
+// s = r - alpha v
+__global__ __launch_bounds__(256) void k_bicg_s(const double* __restrict__ r,
+                                                const double* __restrict__ v,
+                                                double* __restrict__ sv, double alpha,
+                                                size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    sv[k] = r[k] - alpha * v[k];
+}
+
+// x += alpha ph (the half-step exit)
+__global__ __launch_bounds__(256) void k_axpy64(double* __restrict__ x,
+                                                const double* __restrict__ y,
+                                                double a, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    x[k] += a * y[k];
+}
+
+// sh = s * (1 / a_ii)
+__global__ __launch_bounds__(256) void k_scale_diag(Sten<double> s,
+                                                    const double* __restrict__ in,
+                                                    double* __restrict__ out, Tiling tl) {
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny) {
+        const size_t idx = (size_t)i * s.ny + j;
+        out[idx] = in[idx] * (1. / diag_at(s, i, j));
+      }
+    }
+  }
+}
+
+// x += alpha ph + omega sh; r = s - omega t
+__global__ __launch_bounds__(256) void k_bicg_x(double* __restrict__ x,
+                                                double* __restrict__ r,
+                                                const double* __restrict__ ph,
+                                                const double* __restrict__ sh,
+                                                const double* __restrict__ sv,
+                                                const double* __restrict__ tv,
+                                                double alpha, double omega, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    x[k] += alpha * ph[k] + omega * sh[k];
+    r[k] = sv[k] - omega * tv[k];
+  }
+}
+
+// r = b - A x
+__global__ __launch_bounds__(256) void k_resid64(Sten<double> s, const double* __restrict__ x,
+                                                 const double* __restrict__ b,
+                                                 double* __restrict__ r, Tiling tl) {
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny) {
+        const size_t idx = (size_t)i * s.ny + j;
+        r[idx] = b[idx] - apply_global(s, x, i, j, diag_at(s, i, j));
+      }
+    }
+  }
+}
+
+// Jacobi sweep of Solver_Jacobi_: newx = T x + C with T = -D^-1 (L + U)
+// stored row by row in CSR order (the diagonal entry zeroed, off-diagonals
+// -a_ik / a_ii) and C = b / a_ii; the product accumulates in the CSR order
+// (i-1,j), (i,j-1), (i,j), (i,j+1), (i+1,j) from 0 like MatrixSCR_::operator*.
+// Runs only while the device state says "not done".
+__global__ __launch_bounds__(256) void k_jacobi_sweep(Sten<double> s,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ b,
+                                                      double* __restrict__ newx,
+                                                      const JState* st, Tiling tl) {
+  if (st->done) return;
+  const int ny = s.ny;
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < ny) {
+        const size_t idx = (size_t)i * ny + j;
+        const double dv = diag_at(s, i, j);
+        double acc = 0.0;
+        if (i > 0) acc = acc + x[idx - ny] * (-(-s.KX[i]) / dv);
+        if (j > 0) acc = acc + x[idx - 1] * (-(-s.KY[j]) / dv);
+        acc = acc + x[idx] * 0.0;
+        if (j < ny - 1) acc = acc + x[idx + 1] * (-(-s.KY[j + 1]) / dv);
+        if (i < s.nx - 1) acc = acc + x[idx + ny] * (-(-s.KX[i + 1]) / dv);
+        newx[idx] = acc + b[idx] / dv;
+      }
+    }
+  }
+}
+
+// r = b - A newx; partials of r.r (norm_ref: without the last unknown)
+__global__ __launch_bounds__(256) void k_jacobi_resid(Sten<double> s,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ b,
+                                                      double* part, const JState* st,
+                                                      int norm_ref, Tiling tl) {
+  __shared__ double lds[4];
+  if (st->done) return;
+  double v[1] = {0.0};
+  const size_t nlast = (size_t)s.nx * s.ny - 1;
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny) {
+        const size_t idx = (size_t)i * s.ny + j;
+        const double rv = b[idx] - apply_global(s, x, i, j, diag_at(s, i, j));
+        if (!(norm_ref && idx == nlast)) v[0] += rv * rv;
+      }
+    }
+  }
+  store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
+}
+
+// resid = ||r|| / ||b||; stop at resid <= tol (the previous iterate stays
+// in x: the sweep that converged wrote newx only); history entry
+__global__ __launch_bounds__(256) void k_jacobi_check(const double* part, int G,
+                                                      JState* st, double tol,
+                                                      int max_iter, double* hist,
+                                                      int hist_cap) {
+  __shared__ double lds[4];
+  if (st->done) return;
+  double t[1];
+  reduce_partials<double, 1>(part, G, t, lds);
+  if (threadIdx.x == 0) {
+    const int it = st->iter + 1;
+    const double res = std::sqrt(t[0]) / st->normb;
+    st->resid = res;
+    st->iter = it;
+    if (hist && it <= hist_cap) hist[it - 1] = res;
+    if (res <= tol) st->done = 1;
+    else if (it >= max_iter) st->done = 2;
+  }
+}
+
+int total(cpgpu_poisson* P, int nv, double* out) {
+  hipStream_t st = P->ctx->stream;
+  if (nv == 1)
+    hipLaunchKernelGGL(k_total_n<1>, dim3(1), dim3(256), 0, st, P->part64.p, kG1,
+                       P->partB64.p + 2 * kMaxBlocks);
+  else
+    hipLaunchKernelGGL(k_total_n<2>, dim3(1), dim3(256), 0, st, P->part64.p, kG1,
+                       P->partB64.p + 2 * kMaxBlocks);
+  CPGPU_HIP(P->ctx, hipMemcpyAsync(out, P->partB64.p + 2 * kMaxBlocks,
+                                   sizeof(double) * nv, hipMemcpyDeviceToHost, st));
+  CPGPU_HIP(P->ctx, hipStreamSynchronize(st));
+  return CPGPU_OK;
+}
+
+// host-side value of dot(x, y) (and dot(u, w))
+int dots(cpgpu_poisson* P, const double* x, const double* y, const double* u,
+         const double* w, int skip_last, double* out) {
+  hipStream_t st = P->ctx->stream;
+  if (u)
+    hipLaunchKernelGGL(k_dots<2>, dim3(kG1), dim3(256), 0, st, x, y, u, w, P->N,
+                       skip_last, P->part64.p);
+  else
+    hipLaunchKernelGGL(k_dots<1>, dim3(kG1), dim3(256), 0, st, x, y, u, w, P->N,
+                       skip_last, P->part64.p);
+  CPGPU_HIP(P->ctx, hipGetLastError());
+  return total(P, u ? 2 : 1, out);
+}
+
+}  // namespace
+
+int run_bicgstab(cpgpu_poisson* P, int* max_iter, double* tol, int norm_ref,
+                 double* hist, int hist_cap, int* hist_len) {
+  cpgpu_ctx* ctx = P->ctx;
+  hipStream_t st = ctx->stream;
+  const size_t N = P->N;
+  const Tiling tl = P->tl;
+  const Sten<double> s = P->sten64();
+  for (auto& v : P->bv)
+    if (v.alloc(N)) return set_error(ctx, CPGPU_ENOMEM, "BiCGSTAB vectors");
+  double* x = P->x.p;
+  double* r = P->r.p;
+  double *rt = P->bv[0].p, *p = P->bv[1].p, *ph = P->bv[2].p, *v = P->bv[3].p;
+  double *sv = P->bv[4].p, *sh = P->bv[5].p, *tv = P->bv[6].p;
+  const dim3 g2(tl.G), b2(TJ, BY);
+  auto nrm = [&](const double* a, double* out) {
+    int rc = dots(P, a, a, nullptr, nullptr, norm_ref, out);
+    *out = std::sqrt(*out);
+    return rc;
+  };
+  int hl = 0;
+  auto push = [&](double val) {
+    if (hist && hl < hist_cap) hist[hl] = val;
+    ++hl;
+  };
+  double normb = 0, resid = 0;
+  int rc;
+  if ((rc = nrm(P->b.p, &normb))) return rc;
+  hipLaunchKernelGGL(k_resid64, g2, b2, 0, st, s, x, P->b.p, r, tl);
+  CPGPU_HIP(ctx, hipMemcpyAsync(rt, r, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+  if (normb == 0.0) normb = 1;
+  if ((rc = nrm(r, &resid))) return rc;
+  resid /= normb;
+  *hist_len = 0;
+  if (resid <= *tol) {
+    *tol = resid;
+    *max_iter = 0;
+    push(resid);
+    *hist_len = hl;
+    return CPGPU_OK;
+  }
+  double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0, omega = 0;
+  for (int i = 1; i <= *max_iter; ++i) {
+    if ((rc = dots(P, rt, r, nullptr, nullptr, 0, &rho_1))) return rc;
+    if (rho_1 == 0) {
+      if ((rc = nrm(r, &resid))) return rc;
+      *tol = resid / normb;
+      *hist_len = hl;
+      return CPGPU_BREAKDOWN_RHO;
+    }
+    if (i > 1) beta = (rho_1 / rho_2) * (alpha / omega);
+    hipLaunchKernelGGL(k_bicg_p, g2, b2, 0, st, s, (const double*)r, p,
+                       (const double*)v, ph, beta, omega, i == 1 ? 1 : 0, tl);
+    hipLaunchKernelGGL(k_apply64, g2, b2, 0, st, s, (const double*)ph, v, tl);
+    double rtv = 0;
+    if ((rc = dots(P, rt, v, nullptr, nullptr, 0, &rtv))) return rc;
+    alpha = rho_1 / rtv;
+    hipLaunchKernelGGL(k_bicg_s, dim3(kG1), dim3(256), 0, st, (const double*)r,
+                       (const double*)v, sv, alpha, N);
+    double ns = 0;
+    if ((rc = nrm(sv, &ns))) return rc;
+    if ((resid = ns / normb) < *tol) {
+      hipLaunchKernelGGL(k_axpy64, dim3(kG1), dim3(256), 0, st, x, (const double*)ph,
+                         alpha, N);
+      CPGPU_HIP(ctx, hipStreamSynchronize(st));
+      *tol = resid;
+      push(resid);
+      *hist_len = hl;
+      return CPGPU_OK;  // max_iter unchanged, as the reference
+    }
+    hipLaunchKernelGGL(k_scale_diag, g2, b2, 0, st, s, (const double*)sv, sh, tl);
+    hipLaunchKernelGGL(k_apply64, g2, b2, 0, st, s, (const double*)sh, tv, tl);
+    double ts_tt[2];
+    if ((rc = dots(P, tv, sv, tv, tv, 0, ts_tt))) return rc;
+    omega = ts_tt[0] / ts_tt[1];
+    hipLaunchKernelGGL(k_bicg_x, dim3(kG1), dim3(256), 0, st, x, r, (const double*)ph,
+                       (const double*)sh, (const double*)sv, (const double*)tv, alpha,
+                       omega, N);
+    rho_2 = rho_1;
+    if ((rc = nrm(r, &resid))) return rc;
+    resid /= normb;
+    push(resid);
+    if (resid < *tol) {
+      *tol = resid;
+      *max_iter = i;
+      *hist_len = hl;
+      return CPGPU_OK;
+    }
+    if (omega == 0) {
+      *tol = resid;
+      *hist_len = hl;
+      return CPGPU_BREAKDOWN_OMEGA;
+    }
+  }
+  *tol = resid;
+  *hist_len = hl;
+  return CPGPU_MAXITER;
+}
+
+int run_jacobi(cpgpu_poisson* P, int* max_iter, double* tol, int norm_ref,
+               double* hist, int hist_cap, int* hist_len) {
+  cpgpu_ctx* ctx = P->ctx;
+  hipStream_t st = ctx->stream;
+  const size_t N = P->N;
+  const Tiling tl = P->tl;
+  const Sten<double> s = P->sten64();
+  if (P->bv[0].alloc(N) || P->jst.alloc(1))
+    return set_error(ctx, CPGPU_ENOMEM, "Jacobi vectors");
+  if (hist && hist_cap > 0 && P->dhist.alloc(hist_cap))
+    return set_error(ctx, CPGPU_ENOMEM, "hist alloc");
+  double* dh = hist && hist_cap > 0 ? P->dhist.p : nullptr;
+  const dim3 g2(tl.G), b2(TJ, BY);
+  double normb = 0, resid = 0;
+  int rc;
+  if ((rc = dots(P, P->b.p, P->b.p, nullptr, nullptr, norm_ref, &normb))) return rc;
+  normb = std::sqrt(normb);
+  hipLaunchKernelGGL(k_resid64, g2, b2, 0, st, s, P->x.p, P->b.p, P->r.p, tl);
+  if ((rc = dots(P, P->r.p, P->r.p, nullptr, nullptr, norm_ref, &resid))) return rc;
+  if (normb == 0.0) normb = 1;
+  resid = std::sqrt(resid) / normb;
+  *hist_len = 0;
+  if (resid <= *tol) {
+    *tol = resid;
+    *max_iter = 0;
+    return CPGPU_OK;
+  }
+  JState h0{normb, resid, 0, 0};
+  CPGPU_HIP(ctx, hipMemcpyAsync(P->jst.p, &h0, sizeof(JState), hipMemcpyHostToDevice, st));
+  // iterate k lives in buf[k & 1] (buf[0] = x); sweeps after the stop return
+  double* buf[2] = {P->x.p, P->bv[0].p};
+  const int B = 16;
+  JState hs{};
+  for (int k0 = 0; k0 < *max_iter; k0 += B) {
+    for (int k = k0; k < std::min(k0 + B, *max_iter); ++k) {
+      hipLaunchKernelGGL(k_jacobi_sweep, g2, b2, 0, st, s, (const double*)buf[k & 1],
+                         (const double*)P->b.p, buf[(k + 1) & 1], (const JState*)P->jst.p, tl);
+      hipLaunchKernelGGL(k_jacobi_resid, g2, b2, 0, st, s, (const double*)buf[(k + 1) & 1],
+                         (const double*)P->b.p, P->part64.p, (const JState*)P->jst.p,
+                         norm_ref, tl);
+      hipLaunchKernelGGL(k_jacobi_check, dim3(1), dim3(256), 0, st,
+                         (const double*)P->part64.p, (int)tl.G, P->jst.p, *tol,
+                         *max_iter, dh, hist_cap);
+    }
+    CPGPU_HIP(ctx, hipGetLastError());
+    CPGPU_HIP(ctx, hipMemcpyAsync(&hs, P->jst.p, sizeof(JState), hipMemcpyDeviceToHost, st));
+    CPGPU_HIP(ctx, hipStreamSynchronize(st));
+    if (hs.done) break;
+  }
+  const int it = hs.iter;
+  // converged at iteration it: x is iterate it - 1 (in buf[(it - 1) & 1]);
+  // stopped at max_iter: x is iterate it (the reference copied newx)
+  const int keep = hs.done == 1 ? it - 1 : it;
+  if (keep & 1)
+    CPGPU_HIP(ctx, hipMemcpyAsync(P->x.p, buf[1], sizeof(double) * N,
+                                  hipMemcpyDeviceToDevice, st));
+  if (dh && it > 0)
+    CPGPU_HIP(ctx, hipMemcpyAsync(hist, dh, sizeof(double) * std::min(it, hist_cap),
+                                  hipMemcpyDeviceToHost, st));
+  CPGPU_HIP(ctx, hipStreamSynchronize(st));
+  *hist_len = it;
+  *tol = it > 0 ? hs.resid : resid;  // no sweep ran (max_iter <= 0)
+  if (hs.done == 1) {
+    *max_iter = it;
+    return CPGPU_OK;
+  }
+  return CPGPU_MAXITER;
+}
+
+}  // namespace cpgpu

@@ -772,6 +772,15 @@ int poisson_solve_internal(cpgpu_poisson* P, int method, int* max_iter,
                            int hist_cap, int* hist_len) {
   cpgpu_ctx* ctx = P->ctx;
   int hl = 0;
+  if (method == CPGPU_BICGSTAB || method == CPGPU_JACOBI) {
+    const int nref = norm_mode == CPGPU_NORM_REF && P->N > 1;
+    int code = method == CPGPU_BICGSTAB
+                   ? run_bicgstab(P, max_iter, tol, nref, hist, hist_cap, &hl)
+                   : run_jacobi(P, max_iter, tol, nref, hist, hist_cap, &hl);
+    if (code < 0) return code;
+    if (hist_len) *hist_len = hl;
+    return code;
+  }
   if (method == CPGPU_FCG_UNET) {
     const int nref = norm_mode == CPGPU_NORM_REF && P->N > 1;
     int iters = 0;

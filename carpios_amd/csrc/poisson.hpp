@@ -48,6 +48,12 @@ struct RState {
   double bb, rr, rsum, xsum;
 };
 
+// Solver_Jacobi_ state (device): done = 1 converged, 2 stopped at max_iter.
+struct JState {
+  double normb, resid;
+  int iter, done;
+};
+
 // Flexible-CG state (device, mirrored to pinned host memory).
 struct FState {
   double normb;   // ||b|| of the stopping test
@@ -120,6 +126,9 @@ struct cpgpu_poisson {
   int pc_pre = 2, pc_post = 2;
   double pc_omega = 0.8;
   cpgpu::DevBuf<double> pf[2];
+  // Dia_BiCGSTAB (rt, p, ph, v, s, sh, t) and Solver_Jacobi_ vectors
+  cpgpu::DevBuf<double> bv[7];
+  cpgpu::DevBuf<cpgpu::JState> jst;
   cpgpu::DevBuf<cpgpu::FState> fst;
   cpgpu::FState* h_fst = nullptr;  // pinned
 
@@ -165,6 +174,12 @@ int poisson_prepare_f32(cpgpu_poisson* P);
 // when norm_ref, else the Euclidean one); *iters in: limit, out: used.
 int poisson_jpcg64(cpgpu_poisson* P, int* iters, double tol, int norm_ref,
                    double* resid, int* done);
+// carpio::Dia_BiCGSTAB and carpio::Solver_Jacobi_ on the device (krylov.hip);
+// max_iter, tol in/out with the reference's meaning, reference status codes.
+int run_bicgstab(cpgpu_poisson* P, int* max_iter, double* tol, int norm_ref,
+                 double* hist, int hist_cap, int* hist_len);
+int run_jacobi(cpgpu_poisson* P, int* max_iter, double* tol, int norm_ref,
+               double* hist, int hist_cap, int* hist_len);
 // Flexible CG with the attached U-Net as preconditioner (fcg.hip).
 int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
                  int project, double* hist, int hist_cap, int* hist_len,
