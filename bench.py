@@ -45,6 +45,75 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 
 
+def batched(ctx, ca, tol, no_cpu):
+    """256 independent Poisson problems (sides 2^6..2^10 drawn from seed 2,
+    aspect <= 4, random Dirichlet/Neumann walls, Gaussian-blob sources)
+    solved to the true relative residual `tol` from zero by the batched
+    solver: plain FP64 Jacobi-PCG and FP32 Jacobi-PCG with FP64 residual
+    replacement, each after its tile autotune."""
+    t0 = time.perf_counter()
+    probs = ca.random_problems(ctx, seed=2, n=256)
+    ctx.sync()
+    t_setup = time.perf_counter() - t0
+    cells = np.array([P.nx * P.ny for P in probs], dtype=np.int64)
+    B = ca.Batch(ctx, probs)
+    res = {"problems": len(probs), "cells": int(cells.sum()),
+           "sides": {"min": int(min(min(P.nx, P.ny) for P in probs)),
+                     "max": int(max(max(P.nx, P.ny) for P in probs))},
+           "setup_s": round(t_setup, 2)}
+    sols, iters = {}, {}
+    for method, s in (("JPCG", 8), ("MIXED", 4)):
+        tpb, ms = B.autotune(method, reps=2)
+        for P in probs:
+            P.fill_phi(0.0)
+        ctx.sync()
+        t = time.perf_counter()
+        code, it, r, codes, lat = B.solve(method, max_iter=10 ** 6, tol=tol)
+        wall = time.perf_counter() - t
+        if code != ca.OK or r.max() > tol:
+            raise RuntimeError(f"batched {method}: code {code}, max residual {r.max():.3e}")
+        byts = float((it.astype(np.float64) * 11 * cells * s).sum())
+        res[method] = {"solves_per_s": round(len(probs) / wall, 2), "wall_s": round(wall, 4),
+                       "cells_per_s": round(float(cells.sum()) / wall, 1),
+                       "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+                       "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+                       "iterations_total": int(it.sum()), "iterations_max": int(it.max()),
+                       "tiles_per_block": tpb,
+                       "autotune_ms_per_iter": [round(float(v), 4) for v in ms],
+                       "algorithmic_GBs": round(byts / wall / 1e9, 1),
+                       "hbm_frac": round(byts / wall / 1e9 / HBM_PEAK_GBS, 4)}
+        sols[method] = [P.get_phi() for P in probs]
+        iters[method] = it.copy()
+        log(f"bench: batched {method}: {res[method]}")
+    errs = [rel_l2(a_, b_) for a_, b_ in zip(sols["MIXED"], sols["JPCG"])]
+    res["MIXED"]["rel_l2_vs_fp64_max"] = float(max(errs))
+    res["mixed_speedup"] = round(res["MIXED"]["solves_per_s"] / res["JPCG"]["solves_per_s"], 3)
+    if not no_cpu:
+        # the oracle's FP64 J-PCG time per cell-iteration on the member
+        # closest to 512^2 cells, times the batch's own FP64 iteration counts
+        import oracle as O
+        k = int(np.argmin(np.abs(cells - 512 * 512)))
+        Pk = probs[k]
+        og = O.StructGrid(Pk.nx, Pk.ny, 1)
+        A = O.Matrix(og, O.make_bc(og, [int(t) for t in Pk.desc.bc_type]), 1.0)
+        b = O.field_to_x(og, Pk.get_vec(1))
+        m = 40
+        t = time.perf_counter()
+        O.solve("JPCG", A, b, max_iter=m, tol=1e-300, norm=O.NORM_TRUE)
+        per = (time.perf_counter() - t) / (m * Pk.nx * Pk.ny)
+        total = float((iters["JPCG"].astype(np.float64) * cells).sum()) * per
+        res["cpu_baseline"] = {"value": round(len(probs) / total, 5), "unit": "solves/s",
+                               "kind": "port", "cores": 1,
+                               "sample": f"{m} oracle FP64 J-PCG iterations on member {k} "
+                                         f"({Pk.nx}x{Pk.ny}), {per * 1e9:.2f} ns per "
+                                         f"cell-iteration, times the batch's FP64 "
+                                         f"iteration counts ({total:.0f} s)"}
+    B.close()
+    for P in probs:
+        P.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +128,8 @@ def main():
     ap.add_argument("--first-fit", type=int, default=2000)
     ap.add_argument("--round-fit", type=int, default=1500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 256-problem batched solve")
     ap.add_argument("--no-hbm", action="store_true",
                     help="skip the 4096^2 Krylov-kernel HBM probe")
     ap.add_argument("--no-graphs", action="store_true",
@@ -213,6 +284,9 @@ def main():
                          f"{it_mean:.0f} iterations per step plain FP64 J-PCG needs"}
         log(f"bench: cpu {cpu['value']} solves/s")
 
+    batch = None
+    if not a.no_batch:
+        batch = batched(ctx, ca, tol, a.no_cpu)
     out = {
         "metric": "Poisson solves/s to rel-res 1e-8 @1024^2: learned+FP32/FP64-refine vs plain FP64",
         "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,
@@ -241,6 +315,7 @@ def main():
         "krylov_kernels_1024": krylov,
         "roofline_hbm_probe": hbm,
         "cpu_baseline": cpu,
+        "batched_256": batch,
     }
     print(json.dumps(out), flush=True)
     net.close()

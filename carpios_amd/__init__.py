@@ -69,6 +69,10 @@ EXPORTS = [
     "cpgpu_unet_set_sample", "cpgpu_unet_num_samples", "cpgpu_unet_train",
     "cpgpu_unet_eval", "cpgpu_poisson_set_unet", "cpgpu_unet_tensor",
     "cpgpu_ns_set_time", "cpgpu_ns_set_solver", "cpgpu_poisson_set_fcg",
+    "cpgpu_poisson_error_norms", "cpgpu_ns_error_norms", "cpgpu_batch_create",
+    "cpgpu_batch_destroy", "cpgpu_batch_set_tiles", "cpgpu_batch_set_graphs",
+    "cpgpu_batch_solve", "cpgpu_batch_autotune", "cpgpu_batch_draw",
+    "cpgpu_ns_field_change",
 ]
 
 
@@ -218,6 +222,15 @@ class Poisson:
         self.ctx.check(lib().cpgpu_poisson_residual(self.h, C.byref(r)))
         return r.value
 
+    def error_norms(self, exact):
+        """(Error1, Error2, Errori) of phi against a ghost-padded exact
+        field (Operation_::Error1/2/i, s_operation.hpp:869-908)."""
+        exact = np.ascontiguousarray(exact, dtype=np.float64)
+        assert exact.shape == self.shape
+        out = np.zeros(3)
+        self.ctx.check(lib().cpgpu_poisson_error_norms(self.h, _p(exact), _p(out)))
+        return out
+
     def set_mixed(self, inner_tol=1e-4, max_outer=40):
         self.ctx.check(lib().cpgpu_poisson_set_mixed(
             self.h, C.c_double(inner_tol), C.c_int(max_outer)))
@@ -353,6 +366,24 @@ class NS:
         self.ctx.check(lib().cpgpu_ns_get_field(self.h, FIELDS[name], _p(out)))
         return out
 
+    def error_norms(self, name, exact):
+        """(Error1, Error2, Errori) of field `name` against a ghost-padded
+        exact field (Operation_::Error1/2/i, s_operation.hpp:869-908)."""
+        exact = np.ascontiguousarray(exact, dtype=np.float64)
+        assert exact.shape == self.shape
+        out = np.zeros(3)
+        self.ctx.check(lib().cpgpu_ns_error_norms(self.h, FIELDS[name], _p(exact),
+                                                  _p(out)))
+        return out
+
+    def field_change(self, name):
+        """Error1/2/i of field `name` against its value at the previous call
+        (-1s on the first), the steady-state test of EventStopScalarError_
+        (s_event.hpp:520-566)."""
+        out = np.zeros(3)
+        self.ctx.check(lib().cpgpu_ns_field_change(self.h, FIELDS[name], _p(out)))
+        return out
+
     def step(self, n=1):
         it = np.zeros(max(n, 1), dtype=np.int32)
         res = np.zeros(max(n, 1))
@@ -440,6 +471,84 @@ def _set_batch(self, iters=16, use_graphs=True):
 
 Poisson.kernel_times = _kernel_times
 Poisson.set_batch = _set_batch
+
+
+# ------------------------------------------------------------ batched solves
+def draw_problems(seed, n, lo=6, hi=10):
+    """Random problem set (cpgpu_batch_draw): nx, ny, bc_type (n x 4),
+    blob seeds."""
+    nx = np.zeros(n, dtype=np.int32)
+    ny = np.zeros(n, dtype=np.int32)
+    bc = np.zeros((n, 4), dtype=np.int32)
+    bs = np.zeros(n, dtype=np.uint64)
+    code = lib().cpgpu_batch_draw(C.c_ulonglong(seed), C.c_int(n), C.c_int(lo),
+                                  C.c_int(hi), _p(nx, _ip), _p(ny, _ip), _p(bc, _ip),
+                                  bs.ctypes.data_as(C.POINTER(C.c_ulonglong)))
+    if code != OK:
+        raise CpgpuError(code, "cpgpu_batch_draw")
+    return nx, ny, bc, bs
+
+
+def random_problems(ctx, seed=2, n=256, lo=6, hi=10, nblobs=64, wmin=0.01, wmax=0.1):
+    """The drawn problem set on [0,1]^2 grids (gl 1, boundary values 0) with
+    zero-meaned Gaussian-blob sources; phi = 0."""
+    nx, ny, bc, bs = draw_problems(seed, n, lo, hi)
+    probs = []
+    for p in range(n):
+        P = Poisson(ctx, uniform_grid_desc(int(nx[p]), int(ny[p]), 1, [int(t) for t in bc[p]]))
+        P.set_source_blobs(int(bs[p]), nblobs=nblobs, wmin=wmin, wmax=wmax)
+        P.fill_phi(0.0)
+        probs.append(P)
+    return probs
+
+
+class Batch:
+    """Batched solves over Poisson problems of uneven sizes (cpgpu_batch)."""
+
+    def __init__(self, ctx, problems):
+        self.ctx, self.problems = ctx, list(problems)
+        self.n = len(self.problems)
+        arr = (C.c_void_p * self.n)(*[P.h for P in self.problems])
+        self.h = C.c_void_p()
+        ctx.check(lib().cpgpu_batch_create(ctx.h, C.c_int(self.n), arr, C.byref(self.h)))
+        ctx.adopt(self)
+
+    def set_tiles(self, tpb):
+        self.ctx.check(lib().cpgpu_batch_set_tiles(self.h, C.c_int(tpb)))
+
+    def set_graphs(self, on):
+        self.ctx.check(lib().cpgpu_batch_set_graphs(self.h, C.c_int(int(bool(on)))))
+
+    def autotune(self, method="JPCG", reps=2):
+        best = C.c_int(0)
+        ms = np.zeros(4, dtype=np.float32)
+        self.ctx.check(lib().cpgpu_batch_autotune(self.h, C.c_int(METHODS[method]),
+                                                  C.c_int(reps), C.byref(best),
+                                                  ms.ctypes.data_as(C.POINTER(C.c_float))))
+        return best.value, ms
+
+    def solve(self, method="JPCG", max_iter=100000, tol=1e-8):
+        """Returns (code, iters, resid, codes, latency_ms) per problem."""
+        it = np.zeros(self.n, dtype=np.int32)
+        res = np.zeros(self.n)
+        codes = np.zeros(self.n, dtype=np.int32)
+        lat = np.zeros(self.n)
+        code = lib().cpgpu_batch_solve(self.h, C.c_int(METHODS[method]), C.c_int(max_iter),
+                                       C.c_double(tol), _p(it, _ip), _p(res),
+                                       _p(codes, _ip), _p(lat))
+        self.ctx.check(code, allow=(OK, MAXITER))
+        return code, it, res, codes, lat
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().cpgpu_batch_destroy(self.h)
+        self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------- learned preconditioner

@@ -57,11 +57,11 @@ int cpgpu_create(int device, cpgpu_ctx** out) {
 
 int cpgpu_destroy(cpgpu_ctx* ctx) {
   if (!ctx) return CPGPU_EINVAL;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   for (auto& v : ctx->ev)
-    for (auto e : v) hipEventDestroy(e);
-  hipStreamDestroy(ctx->stream);
+    for (auto e : v) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CPGPU_OK;
 }

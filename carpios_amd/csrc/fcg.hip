@@ -285,7 +285,9 @@ int fcg_alloc(cpgpu_poisson* P) {
 // zero, the network on the remaining residual, `post` sweeps -- a two-level
 // cycle whose coarse correction is the network.  Work buffers: x32, pb32,
 // q32, b32.
-int apply_precond(cpgpu_poisson* P) {
+// z = M(r): pre-smoothing, the network's correction (unless use_net is
+// false: the smoothing-only cycle), post-smoothing; output in P->pa32.
+int apply_precond(cpgpu_poisson* P, bool use_net) {
   hipStream_t st = P->ctx->stream;
   const Tiling tl = P->tl;
   const Sten<float> s = P->sten32();
@@ -300,19 +302,21 @@ int apply_precond(cpgpu_poisson* P) {
                        (const float*)z, out, w, tl);
     z = out;
   }
-  const float* rn = r;
-  if (z) {
-    hipLaunchKernelGGL(k_resid32, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
-                       (const float*)z, P->q32.p, tl);
-    rn = P->q32.p;
-  }
-  float* c = P->b32.p;
-  int rc = unet_apply_device(P->unet, P->nx, P->ny, rn, c);
-  if (rc) return rc;
-  if (z) {
-    hipLaunchKernelGGL(k_add32, dim3(1024), dim3(256), 0, st, z, c, P->N);
-  } else {
-    z = c;
+  if (use_net) {
+    const float* rn = r;
+    if (z) {
+      hipLaunchKernelGGL(k_resid32, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
+                         (const float*)z, P->q32.p, tl);
+      rn = P->q32.p;
+    }
+    float* c = P->b32.p;
+    int rc = unet_apply_device(P->unet, P->nx, P->ny, rn, c);
+    if (rc) return rc;
+    if (z) {
+      hipLaunchKernelGGL(k_add32, dim3(1024), dim3(256), 0, st, z, c, P->N);
+    } else {
+      z = c;
+    }
   }
   // post sweeps alternate between two buffers other than z's, the last one
   // writing P->pa32
@@ -325,6 +329,7 @@ int apply_precond(cpgpu_poisson* P) {
                        (const float*)z, out, w, tl);
     z = out;
   }
+  if (!z) z = P->r32.p;  // no sweeps and no network: the identity
   if (z != P->pa32.p)
     CPGPU_HIP(P->ctx, hipMemcpyAsync(P->pa32.p, z, sizeof(float) * P->N,
                                      hipMemcpyDeviceToDevice, st));
@@ -387,25 +392,27 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
   double res = fs.resid;
   std::vector<double> trace;
   trace.push_back(res);
-  constexpr int kStallWindow = 50;
+  constexpr int kStallWindow = 30;
   bool singular = true;  // all-Neumann: constants are the null space
   for (int sg = 0; sg < 4; ++sg)
     singular = singular && P->seg[sg].type == CPGPU_BC_NEUMANN;
   double best = res;
   std::vector<double> best_hist(1, res);
   int k = 0;
+  int seg = 0;          // iterations since the last (re)start of the directions
+  bool use_net = true;  // false after a stall: the smoothing-only cycle
   bool fallback = false;
   while (res > tol) {
     if (k >= max_iter) break;
     // z = preconditioned residual
-    if ((rc = apply_precond(P))) return rc;
+    if ((rc = apply_precond(P, use_net))) return rc;
     hipLaunchKernelGGL(k_fcg_zq, dim3(G1), dim3(256), 0, st, P->pa32.p,
-                       k > 0 ? (const double*)P->q.p : nullptr, P->part64.p, N);  // 2 x G1 partials
+                       seg > 0 ? (const double*)P->q.p : nullptr, P->part64.p, N);  // 2 x G1 partials
     const double* pold = P->pf[k & 1].p;
     double* pnew = P->pf[(k + 1) & 1].p;
     hipLaunchKernelGGL(k_fcg_dir, dim3(tl.G), dim3(TJ, BY), 0, st, P->sten64(),
                        P->pa32.p, pold, pnew, P->q.p, P->r.p, P->part64.p, G1,
-                       P->fst.p, k == 0 ? 1 : 0, singular ? 1 : 0, P->partB64.p,
+                       P->fst.p, seg == 0 ? 1 : 0, singular ? 1 : 0, P->partB64.p,
                        tl);
     hipLaunchKernelGGL(k_fcg_upd, dim3(G1), dim3(256), 0, st, pnew, P->q.p,
                        P->x.p, P->r.p, P->r32.p, P->partB64.p, (int)tl.G,
@@ -414,6 +421,7 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
                        P->fst.p);
     CPGPU_HIP(ctx, hipGetLastError());
     ++k;
+    ++seg;
     if ((rc = read_state(P, &fs))) return rc;
     res = fs.resid;
     trace.push_back(res);
@@ -426,7 +434,18 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
     // monotone, so single-step or short-window tests misfire)
     best = std::min(best, res);
     best_hist.push_back(best);
-    if (k >= kStallWindow && best > 0.9 * best_hist[k - kStallWindow]) {
+    if (seg >= kStallWindow && best > 0.9 * best_hist[k - kStallWindow]) {
+      // The network maps the current residual to a correction that no
+      // longer reduces it (the residual stays put, and the deterministic
+      // network returns the same correction): continue from the current
+      // iterate with the smoothing-only cycle, a fixed symmetric positive
+      // definite preconditioner (damped Jacobi sweeps), directions
+      // restarted; Jacobi-PCG only if that stalls too.
+      if (use_net) {
+        use_net = false;
+        seg = 0;
+        continue;
+      }
       fallback = true;
       break;
     }

@@ -407,6 +407,10 @@ struct cpgpu_ns {
   DevBuf<double> F[11];
   int slot[11];  // field id -> buffer (u/us and uf/usf swap every step)
   DevBuf<double> div_int, part;
+  DevBuf<double> err_ref;  // exact field of cpgpu_ns_error_norms
+  DevBuf<double> prev[11];  // snapshots of cpgpu_ns_field_change, per field id
+  bool has_prev[11] = {false, false, false, false, false, false,
+                       false, false, false, false, false};
   DevBuf<NSDev> st;
   NSDev* h_st = nullptr;
   double rho, mu, cfl, dnum, dcoef, min_size, ddenom;
@@ -626,6 +630,39 @@ int cpgpu_ns_get_field(cpgpu_ns* S, int which, double* field) {
   CPGPU_HIP(S->ctx, hipMemcpyAsync(field, S->fld(which), sizeof(double) * S->NT,
                                    hipMemcpyDeviceToHost, S->ctx->stream));
   CPGPU_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
+  return CPGPU_OK;
+}
+
+int cpgpu_ns_error_norms(cpgpu_ns* S, int which, const double* exact_field,
+                         double* out) {
+  if (!S || !exact_field || !out || which < 0 || which > 10) return CPGPU_EINVAL;
+  CPGPU_HIP(S->ctx, hipSetDevice(S->ctx->device));
+  if (S->err_ref.alloc(S->NT)) return set_error(S->ctx, CPGPU_ENOMEM, "error norm buffer");
+  CPGPU_HIP(S->ctx, hipMemcpyAsync(S->err_ref.p, exact_field, sizeof(double) * S->NT,
+                                   hipMemcpyHostToDevice, S->ctx->stream));
+  const size_t off = (size_t)S->gl * S->NY + S->gl;
+  return error_norms_device(S->ctx, S->nx, S->ny, S->err_ref.p + off, S->NY,
+                            S->fld(which) + off, S->NY, S->sx.p + S->gl, S->sy.p + S->gl,
+                            S->part.p, S->part.n, out);
+}
+
+int cpgpu_ns_field_change(cpgpu_ns* S, int which, double* out) {
+  if (!S || !out || which < 0 || which > 10) return CPGPU_EINVAL;
+  cpgpu_ctx* ctx = S->ctx;
+  CPGPU_HIP(ctx, hipSetDevice(ctx->device));
+  if (S->prev[which].alloc(S->NT)) return set_error(ctx, CPGPU_ENOMEM, "field snapshot");
+  if (S->has_prev[which]) {
+    const size_t off = (size_t)S->gl * S->NY + S->gl;
+    int rc = error_norms_device(ctx, S->nx, S->ny, S->prev[which].p + off, S->NY,
+                                S->fld(which) + off, S->NY, S->sx.p + S->gl,
+                                S->sy.p + S->gl, S->part.p, S->part.n, out);
+    if (rc) return rc;
+  } else {
+    out[0] = out[1] = out[2] = -1.0;
+  }
+  CPGPU_HIP(ctx, hipMemcpyAsync(S->prev[which].p, S->fld(which), sizeof(double) * S->NT,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  S->has_prev[which] = true;
   return CPGPU_OK;
 }
 

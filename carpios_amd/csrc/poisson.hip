@@ -162,41 +162,6 @@ __global__ __launch_bounds__(256) void k_blobs(int nx, int ny,
   store_partials<double, 1>(v, partials, gridDim.x, blockIdx.x, lds);
 }
 
-// SplitMix64-seeded xoshiro256** (Blackman & Vigna); doubles from the top
-// 53 bits; normals by Box-Muller.  Per blob: cx, cy ~ U[0,1), w ~ U[wmin,
-// wmax), A ~ N(0,1).
-struct Xoshiro {
-  unsigned long long s[4];
-  static unsigned long long splitmix(unsigned long long& x) {
-    unsigned long long z = (x += 0x9E3779B97F4A7C15ULL);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-  }
-  explicit Xoshiro(unsigned long long seed) {
-    for (int k = 0; k < 4; ++k) s[k] = splitmix(seed);
-  }
-  static unsigned long long rotl(unsigned long long x, int k) {
-    return (x << k) | (x >> (64 - k));
-  }
-  unsigned long long next() {
-    const unsigned long long r = rotl(s[1] * 5, 7) * 9;
-    const unsigned long long t = s[1] << 17;
-    s[2] ^= s[0];
-    s[3] ^= s[1];
-    s[1] ^= s[2];
-    s[0] ^= s[3];
-    s[2] ^= t;
-    s[3] = rotl(s[3], 45);
-    return r;
-  }
-  double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
-  double normal() {
-    const double u1 = 1.0 - uniform(), u2 = uniform();
-    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-  }
-};
-
 void blob_params(unsigned long long seed, int nblobs, double wmin, double wmax,
                  double* out) {
   Xoshiro g(seed);
@@ -274,6 +239,8 @@ int build_descriptor(cpgpu_poisson* P, const cpgpu_grid2d* g) {
     P->seg[s].d1 = d1;
   }
   P->cxi.assign(g->cx + gl, g->cx + gl + nx);
+  P->sxi.assign(g->sx + gl, g->sx + gl + nx);
+  P->syi.assign(g->sy + gl, g->sy + gl + ny);
   P->cyi.assign(g->cy + gl, g->cy + gl + ny);
   P->vol.assign(nx, 0.0);
   for (int i = 0; i < nx; ++i) {

@@ -63,6 +63,46 @@ struct FState {
   int breakdown;  // p.q <= 0 or not finite
 };
 
+// SplitMix64-seeded xoshiro256** (Blackman & Vigna); doubles from the top
+// 53 bits; normals by Box-Muller.  The generator of every seeded input
+// (blob sources, batches of random problems).
+struct Xoshiro {
+  unsigned long long s[4];
+  static unsigned long long splitmix(unsigned long long& x) {
+    unsigned long long z = (x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }
+  explicit Xoshiro(unsigned long long seed) {
+    for (int k = 0; k < 4; ++k) s[k] = splitmix(seed);
+  }
+  static unsigned long long rotl(unsigned long long x, int k) {
+    return (x << k) | (x >> (64 - k));
+  }
+  unsigned long long next() {
+    const unsigned long long r = rotl(s[1] * 5, 7) * 9;
+    const unsigned long long t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+  double normal() {
+    const double u1 = 1.0 - uniform(), u2 = uniform();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  }
+};
+
+// Gaussian-blob parameters from Xoshiro(seed): per blob cx, cy ~ U[0,1),
+// w ~ U[wmin, wmax), A ~ N(0,1) (poisson.hip).
+void blob_params(unsigned long long seed, int nblobs, double wmin, double wmax,
+                 double* out);
+
 // Identity of a captured batch of Krylov iterations.
 struct GraphKey {
   const void* x;
@@ -88,6 +128,7 @@ struct cpgpu_poisson {
   // host copies of the descriptor
   std::vector<double> KX, KY, vol, bcv;
   std::vector<double> cxi, cyi;  // interior cell centres
+  std::vector<double> sxi, syi;  // interior cell sizes
   double adj[4] = {0, 0, 0, 0};
   cpgpu::RhsSeg seg[4];
   // device operator
@@ -131,6 +172,8 @@ struct cpgpu_poisson {
   cpgpu::DevBuf<cpgpu::JState> jst;
   cpgpu::DevBuf<cpgpu::FState> fst;
   cpgpu::FState* h_fst = nullptr;  // pinned
+  // error norms: the exact field and the interior cell sizes on the device
+  cpgpu::DevBuf<double> err_ref, dsxi, dsyi;
 
   cpgpu::Sten<double> sten64() const {
     cpgpu::Sten<double> s;
@@ -188,6 +231,12 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
 int unet_warm_start(cpgpu_poisson* P);
 // out = s net(in / s), s = rms(in), on H x W device FP32 arrays (unet.hip).
 int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out);
+// Error1 / Error2 / Errori of val against exa (interior arrays with row
+// strides lde / ldv, j contiguous) on the cells of the sizes sx (nx), sy (ny);
+// part holds at least 7 doubles (norms.hip).  out: host, 3 values.
+int error_norms_device(cpgpu_ctx* ctx, int nx, int ny, const double* exa, int lde,
+                       const double* val, int ldv, const double* sx, const double* sy,
+                       double* part, size_t part_cap, double* out);
 // The context a U-Net was created on (unet.hip).
 cpgpu_ctx* unet_context(const cpgpu_unet* U);
 }  // namespace cpgpu

@@ -179,7 +179,7 @@ __device__ __forceinline__ bool dir_prologue(KState* st,
   T tot[3];
   reduce_partials<T, 3>(partB, Gprev, tot, lds);
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0;
-  bool stop = false, conv = false;
+  bool stop = false, conv = false, replaced = false;
   double resid = 0.0;
   if (st->mode == 0) {
     double normb;
@@ -194,6 +194,7 @@ __device__ __forceinline__ bool dir_prologue(KState* st,
     if (lead && it >= 2 && it - 2 < hist_cap && hist) hist[it - 2] = resid;
     conv = resid <= st->tol;
   } else if (it > 1 && (it - 1) % st->refresh_every == 0) {
+    replaced = true;
     __shared__ double lds64[4];
     double rr[1];
     reduce_partials<double, 1>(partR64, Gref, rr, lds64);
@@ -212,7 +213,12 @@ __device__ __forceinline__ bool dir_prologue(KState* st,
     return false;
   }
   const T rho = tot[SLOT_RZ];
-  *first = it == 1;
+  // After a residual replacement the kept direction belongs to the FP32
+  // recurrence; when the true r.z exceeds twice the recurrence's last one,
+  // that recurrence had stagnated (its residual no longer the true one) and
+  // the direction restarts from z.
+  *first = it == 1 ||
+           (replaced && (double)rho > 2.0 * st->rho[(it - 1) & 1]);
   *beta = *first ? T(0) : rho / (T)st->rho[(it - 1) & 1];
   if (lead) {
     st->rho[it & 1] = (double)rho;

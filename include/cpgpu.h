@@ -158,6 +158,12 @@ int cpgpu_poisson_apply(cpgpu_poisson *P, const double *x_field,
                         double *y_field);
 /* True FP64 relative residual ||b - A phi|| / ||b|| of the current phi. */
 int cpgpu_poisson_residual(cpgpu_poisson *P, double *rel);
+/* Operation_<2>::Error1 / Error2 / Errori (s_operation.hpp:869-908) of the
+ * current phi against a host ghost-padded exact field, over the normal
+ * cells: out[0] = sum|e|v / sum v, out[1] = sqrt(sum e^2 v) / sum v (the
+ * reference's normalisation), out[2] = max|e|, v = sx_i * sy_j. */
+int cpgpu_poisson_error_norms(cpgpu_poisson *P, const double *exact_field,
+                              double *out);
 
 /* Device pointers of the problem's interior-only vectors (nx*ny doubles,
  * j contiguous): which = 0 phi, 1 b, 2 last residual.  For device-resident
@@ -173,6 +179,46 @@ int cpgpu_poisson2d_solve(cpgpu_ctx *ctx, const cpgpu_grid2d *g,
                           double *phi_field, int method, int *max_iter,
                           double *tol, int norm_mode, double *hist,
                           int hist_cap, int *hist_len);
+
+/* ---- batched solves over many problems of uneven sizes ----
+ * No counterpart in the reference, which solves one problem at a time
+ * (solver_matrix.hpp:264-374, the algorithm batched here).  A batch is a
+ * view over existing problems: their operators, right-hand sides and phi
+ * (the initial guess, overwritten by the solution) are used in place. */
+typedef struct cpgpu_batch cpgpu_batch;
+
+/* Batch of n problems of one context (not all-Neumann: singular problems
+ * are rejected).  The problems must outlive the batch. */
+int cpgpu_batch_create(cpgpu_ctx *ctx, int n, cpgpu_poisson *const *problems,
+                       cpgpu_batch **out);
+int cpgpu_batch_destroy(cpgpu_batch *B);
+/* Tiles (16 x 64 cells) per block; 1 gives each problem the block count of
+ * its single-problem solve, so results are bitwise those of CPGPU_JPCG with
+ * direction variant 0. */
+int cpgpu_batch_set_tiles(cpgpu_batch *B, int tiles_per_block);
+/* Graph-captured iteration batches on (default) or plain launches. */
+int cpgpu_batch_set_graphs(cpgpu_batch *B, int use_graphs);
+/* Solves every problem: method CPGPU_JPCG (FP64, the reference's J-PCG test
+ * on the true relative residual every iteration) or CPGPU_MIXED (FP32
+ * iterations, FP64 residual replacement and test every 16 iterations).
+ * Per problem (arrays of n, each may be NULL): iterations, final relative
+ * residual, status code, and the time from the start of the solve to the
+ * problem's stop in ms (device clock).  Returns CPGPU_OK when all
+ * converged, else CPGPU_MAXITER. */
+int cpgpu_batch_solve(cpgpu_batch *B, int method, int max_iter, double tol,
+                      int *iters, double *resid, int *codes,
+                      double *latency_ms);
+/* Times `reps` graph batches of `method` for 1, 2, 4 and 8 tiles per block
+ * (ms_per_iter[4], may be NULL), keeps the fastest; phi is restored. */
+int cpgpu_batch_autotune(cpgpu_batch *B, int method, int reps, int *best_tpb,
+                         float *ms_per_iter);
+/* Random problem set (sizes, wall types, blob seeds) from one seeded
+ * generator: log2 nx ~ U[lo, hi], log2 ny ~ U[max(lo, log2 nx - 2),
+ * min(hi, log2 nx + 2)], both rounded (aspect <= 4); each wall Dirichlet or
+ * Neumann with probability 1/2, redrawn while all four are Neumann;
+ * bc_type[4 p + s] for segments x-, x+, y-, y+. */
+int cpgpu_batch_draw(unsigned long long seed, int n, int lo, int hi, int *nx,
+                     int *ny, int *bc_type, unsigned long long *blob_seed);
 
 /* ---- NS_explicit_ projection step (device resident) ----
  * Replaces structure::NS_explicit_<2>::run_one_step + Time_::advance
@@ -226,6 +272,16 @@ int cpgpu_ns_destroy(cpgpu_ns *S);
 /* Host ghost-padded fields in/out (CPGPU_F_*). */
 int cpgpu_ns_set_field(cpgpu_ns *S, int which, const double *field);
 int cpgpu_ns_get_field(cpgpu_ns *S, int which, double *field);
+/* Error1 / Error2 / Errori (s_operation.hpp:869-908) of the device field
+ * `which` against a host ghost-padded exact field (the order and meaning
+ * of cpgpu_poisson_error_norms). */
+int cpgpu_ns_error_norms(cpgpu_ns *S, int which, const double *exact_field,
+                         double *out);
+/* The steady-state test of EventStopScalarError_ (s_event.hpp:520-566) on
+ * the device: Error1/2/i of field `which` against its snapshot from the
+ * previous call (out = -1, -1, -1 on the first call), then the snapshot
+ * is replaced by the current field. */
+int cpgpu_ns_field_change(cpgpu_ns *S, int which, double *out);
 /* nsteps projection steps; per-step projection iterations and final
  * residuals into iters/resid (may be NULL).  Returns the last solver code. */
 int cpgpu_ns_step(cpgpu_ns *S, int nsteps, int *iters, double *resid);

@@ -332,6 +332,15 @@ def grad(grid, phi):
     return gx, gy
 
 
+def error_norms(grid, exa, val):
+    """(Error1, Error2, Errori) of val against exa, s_operation.hpp:869-908."""
+    out = np.zeros(3)
+    exa = np.ascontiguousarray(exa, dtype=np.float64)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    lib().orc_error_norms(C.byref(grid.c), _ptr(exa), _ptr(val), _ptr(out))
+    return out
+
+
 def f2c(grid, gx, gy):
     cx, cy = grid.field(), grid.field()
     lib().orc_interp_f2c(C.byref(grid.c), _ptr(gx), _ptr(gy), _ptr(cx),

@@ -23,6 +23,10 @@
  *                             cavity centre-line vs the Botella table
  *   orc_jpcg_mixed ........... true FP64 residual reaches tol; solution
  *                             matches orc_jpcg to the conditioning bound
+ *   orc_error_norms .......... closed forms: a constant error c gives
+ *                             (|c|, |c| / sqrt(V), |c|), a single-cell
+ *                             error e gives (|e| v / V, |e| sqrt(v) / V,
+ *                             |e|) with V the domain volume; scaling
  */
 #include "oracle.h"
 
@@ -836,6 +840,33 @@ double orc_max_magnitude(const orc_grid *g, const double *u, const double *v) {
     }
   }
   return mx;
+}
+
+/* Operation_::Error1, Error2, Errori (s_operation.hpp:869-908) over the
+ * normal cells in i-fastest order, volume (1.0*sx)*sy (Grid_::volume,
+ * s_grid.hpp:97-104).  Error2 is sqrt(sum err^2 vol) / sum vol as the
+ * reference writes it (not the root of the volume mean); Errori keeps the
+ * first maximum ('>' test). */
+void orc_error_norms(const orc_grid *g, const double *exa, const double *val,
+                     double *out) {
+  double n1 = 0.0, n2 = 0.0, svol = 0.0, ni = 0.0;
+  int i, j;
+  for (j = 0; j < g->ny; ++j) {
+    for (i = 0; i < g->nx; ++i) {
+      double vol = 1.0, err, ae;
+      vol *= g->sx[i + g->gl];
+      vol *= g->sy[j + g->gl];
+      err = val[FIDX(g, i, j)] - exa[FIDX(g, i, j)];
+      n1 += fabs(err) * vol;
+      n2 += err * err * vol;
+      svol += vol;
+      ae = fabs(err);
+      if (ae > ni) ni = ae;
+    }
+  }
+  out[0] = n1 / svol;
+  out[1] = sqrt(n2) / svol;
+  out[2] = ni;
 }
 
 static double min_size(const double *s, int n) {

@@ -130,6 +130,10 @@ void orc_grad(const orc_grid *g, const double *phi, double *gx, double *gy);
 void orc_interp_f2c(const orc_grid *g, const double *gx, const double *gy,
                     double *cx, double *cy);
 double orc_max_magnitude(const orc_grid *g, const double *u, const double *v);
+/* Operation_::Error1 / Error2 / Errori of val against exa (both ghost-padded
+ * fields of g): out = {Error1, Error2, Errori}. */
+void orc_error_norms(const orc_grid *g, const double *exa, const double *val,
+                     double *out);
 
 /* ---- NS_explicit_ projection step ---- */
 typedef struct {

@@ -181,3 +181,43 @@ def test_cavity_dt_follows_restrict_time():
     h2 = 1.0 / 512
     assert st.dt == 0.25 / (1 / 1000.0) / (1.0 / (h2 * h2) + 1.0 / (h2 * h2))
     assert st.dt < 0.5 * h2
+
+
+def _stretched(nx=12, ny=7, gl=2):
+    """A grid whose cell sizes vary along both axes (sizes written in
+    place into the axis arrays the oracle grid points at)."""
+    g = O.StructGrid(nx, ny, gl, 0.0, 2.0, 0.0, 1.0)
+    g.ax.s[:] = 0.1 + 0.01 * np.arange(nx + 2 * gl)
+    g.ay.s[:] = 0.2 + 0.03 * np.arange(ny + 2 * gl)
+    return g
+
+
+def test_error_norms_closed_forms():
+    # s_operation.hpp:869-908: Error1 = sum |e| v / V, Error2 =
+    # sqrt(sum e^2 v) / V (the reference's normalisation), Errori = max |e|
+    g = _stretched()
+    gl = g.gl
+    sx = g.ax.s[gl:gl + g.nx]
+    sy = g.ay.s[gl:gl + g.ny]
+    V = sx.sum() * sy.sum()
+    exa = np.random.default_rng(0).standard_normal(g.shape)
+    val = exa.copy()
+    g.interior(val)[:] -= 0.3
+    n = O.error_norms(g, exa, val)
+    assert n == pytest.approx([0.3, 0.3 * np.sqrt(V) / V, 0.3], rel=1e-13)
+    # one cell: the volume is sx[i] * sy[j] (a transposed index fails)
+    val = exa.copy()
+    val[gl + 3, gl + 5] += 2.5
+    v = sx[3] * sy[5]
+    n = O.error_norms(g, exa, val)
+    assert n == pytest.approx([2.5 * v / V, 2.5 * np.sqrt(v) / V, 2.5], rel=1e-13)
+    # ghosts are not normal cells; Errori takes |e| of either sign
+    val[0, :] = 1e9
+    val[:, -1] = -1e9
+    val[gl, gl] -= 4.0
+    n2 = O.error_norms(g, exa, val)
+    assert n2[2] == pytest.approx(4.0, rel=1e-14)
+    assert n2[0] == pytest.approx((2.5 * v + 4.0 * sx[0] * sy[0]) / V, rel=1e-13)
+    # homogeneity: errors scaled by k scale every norm by |k|
+    n3 = O.error_norms(g, exa, exa + (-3.0) * (val - exa))
+    assert n3 == pytest.approx(3.0 * n2, rel=1e-12)
