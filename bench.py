@@ -45,12 +45,13 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 
 
-def batched(ctx, ca, tol, no_cpu):
+def batched(ctx, ca, tol, no_cpu, learned_net=True):
     """256 independent Poisson problems (sides 2^6..2^10 drawn from seed 2,
     aspect <= 4, random Dirichlet/Neumann walls, Gaussian-blob sources)
     solved to the true relative residual `tol` from zero by the batched
     solver: plain FP64 Jacobi-PCG and FP32 Jacobi-PCG with FP64 residual
-    replacement, each after its tile autotune."""
+    replacement, each after its tile autotune, and (learned_net) the trained
+    U-Net preconditioner on the largest members."""
     t0 = time.perf_counter()
     probs = ca.random_problems(ctx, seed=2, n=256)
     ctx.sync()
@@ -88,6 +89,39 @@ def batched(ctx, ca, tol, no_cpu):
     errs = [rel_l2(a_, b_) for a_, b_ in zip(sols["MIXED"], sols["JPCG"])]
     res["MIXED"]["rel_l2_vs_fp64_max"] = float(max(errs))
     res["mixed_speedup"] = round(res["MIXED"]["solves_per_s"] / res["JPCG"]["solves_per_s"], 3)
+    if learned_net:
+        # the U-Net preconditioner on the largest members (>= 2^19 cells),
+        # trained on members of another draw (seed 7); the rest, and learned
+        # members that stall or pass 150 flexible-CG iterations, batched
+        from carpios_amd import learned
+        net = ca.UNet(ctx, seed=1)
+        t_tr, n_s, curve = learned.train_problem_net(ctx, net, log=log)
+        thr = 1 << 19
+        for k, P in enumerate(probs):
+            P.set_fcg(2, 2, 0.8)
+            P.set_unet(net if cells[k] >= thr else None, ca.UNET_PRECOND if cells[k] >= thr else 0)
+            P.fill_phi(0.0)
+        ctx.sync()
+        t = time.perf_counter()
+        code, it, r, codes, lat = B.solve("FCG_UNET", max_iter=10 ** 6, tol=tol)
+        wall = time.perf_counter() - t
+        if code != ca.OK or r.max() > tol:
+            raise RuntimeError(f"batched FCG_UNET: code {code}, max residual {r.max():.3e}")
+        m = cells >= thr
+        errs = [rel_l2(P.get_phi(), s_) for P, s_ in zip(probs, sols["JPCG"])]
+        res["FCG_UNET"] = {"solves_per_s": round(len(probs) / wall, 2), "wall_s": round(wall, 4),
+                           "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+                           "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+                           "learned_members": int(m.sum()), "min_cells": thr,
+                           "learned_iterations": sorted(int(v) for v in it[m]),
+                           "speedup_vs_fp64": round(res["JPCG"]["wall_s"] / wall, 3),
+                           "rel_l2_vs_fp64_max": float(max(errs)),
+                           "training": {"seconds": round(t_tr, 1), "samples": n_s,
+                                        "loss_per_pass": [round(v, 3) for v in curve]}}
+        log(f"bench: batched FCG_UNET: {res['FCG_UNET']}")
+        for P in probs:
+            P.set_unet(None, 0)
+        net.close()
     if not no_cpu:
         # the oracle's FP64 J-PCG time per cell-iteration on the member
         # closest to 512^2 cells, times the batch's own FP64 iteration counts
@@ -130,6 +164,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 256-problem batched solve")
+    ap.add_argument("--no-batch-learned", action="store_true",
+                    help="skip the U-Net variant of the batched solve (its training)")
     ap.add_argument("--no-hbm", action="store_true",
                     help="skip the 4096^2 Krylov-kernel HBM probe")
     ap.add_argument("--no-graphs", action="store_true",
@@ -221,7 +257,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_conv3_fwd<16,16> (finest U-Net level, 3x3, BF16 MFMA)",
+                "kernel": "k_conv3_tile<16,16,true> (finest U-Net level, 3x3 forward, BF16 MFMA)",
                 "bytes_per_launch": conv_bytes, "avg_launch_us": round(us, 3),
                 "launches": n_c,
                 "mfma_tflops": round(conv_flops / (us * 1e-6) / 1e12, 2),
@@ -286,7 +322,7 @@ def main():
 
     batch = None
     if not a.no_batch:
-        batch = batched(ctx, ca, tol, a.no_cpu)
+        batch = batched(ctx, ca, tol, a.no_cpu, not a.no_batch_learned)
     out = {
         "metric": "Poisson solves/s to rel-res 1e-8 @1024^2: learned+FP32/FP64-refine vs plain FP64",
         "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,

@@ -72,7 +72,8 @@ EXPORTS = [
     "cpgpu_poisson_error_norms", "cpgpu_ns_error_norms", "cpgpu_batch_create",
     "cpgpu_batch_destroy", "cpgpu_batch_set_tiles", "cpgpu_batch_set_graphs",
     "cpgpu_batch_solve", "cpgpu_batch_autotune", "cpgpu_batch_draw",
-    "cpgpu_ns_field_change",
+    "cpgpu_ns_field_change", "cpgpu_unet_set_conv_rows", "cpgpu_unet_clear_samples",
+    "cpgpu_unet_set_walls",
 ]
 
 
@@ -578,6 +579,14 @@ class UNet:
         assert a.size == self.num_params
         self.ctx.check(lib().cpgpu_unet_set_params(self.h, _p(a, _fp)))
 
+    def set_walls(self, bc_type=None):
+        """Wall types (x-, x+, y-, y+) of the raw forward/backward input."""
+        arr = None if bc_type is None else (C.c_int * 4)(*[int(t) for t in bc_type])
+        self.ctx.check(lib().cpgpu_unet_set_walls(self.h, arr))
+
+    def set_conv_rows(self, rows):
+        self.ctx.check(lib().cpgpu_unet_set_conv_rows(self.h, C.c_int(rows)))
+
     def forward(self, x):
         """Raw network output for an (H, W) array (no input scaling)."""
         x = np.ascontiguousarray(x, dtype=np.float32)
@@ -624,6 +633,9 @@ class UNet:
     @property
     def num_samples(self):
         return int(lib().cpgpu_unet_num_samples(self.h))
+
+    def clear_samples(self):
+        self.ctx.check(lib().cpgpu_unet_clear_samples(self.h))
 
     def train(self, P, steps, batch=4, lr=1e-3, seed=0):
         losses = np.zeros(max(steps, 1), dtype=np.float32)

@@ -20,6 +20,7 @@
 // every K iterations (the scheme of the single-problem mixed solver).
 // Completion times are stamped on the device (wall clock) per problem.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -629,17 +630,14 @@ int cpgpu_batch_set_graphs(cpgpu_batch* B, int use_graphs) {
   return CPGPU_OK;
 }
 
-int cpgpu_batch_solve(cpgpu_batch* B, int method, int max_iter, double tol, int* iters,
-                      double* resid, int* codes, double* latency_ms) {
-  if (!B || max_iter < 0 || !(tol >= 0.0)) return CPGPU_EINVAL;
-  if (method != CPGPU_JPCG && method != CPGPU_MIXED)
-    return set_error(B->ctx, CPGPU_EUNSUPPORTED, "batched methods: CPGPU_JPCG, CPGPU_MIXED");
+// The batched iteration over the members (mode 0 FP64, 1 mixed); others
+// are marked stopped (done = 3) and left alone.  States and stamps end in
+// B->h_st and B->h_ts.
+static int run_batched(cpgpu_batch* B, int mode, int max_iter, double tol,
+                       const std::vector<char>& member) {
   cpgpu_ctx* ctx = B->ctx;
-  CPGPU_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  const int mode = method == CPGPU_MIXED ? 1 : 0;
   const int n = B->n;
-  // states
   for (int p = 0; p < n; ++p) {
     KState& k = B->h_st[p];
     std::memset(&k, 0, sizeof(k));
@@ -649,14 +647,20 @@ int cpgpu_batch_solve(cpgpu_batch* B, int method, int max_iter, double tol, int*
     k.mode = mode;
     k.refresh_every = B->K;
     k.normb = 1.0;
+    if (!member[p]) k.done = 3;
   }
   CPGPU_HIP(ctx, hipMemcpyAsync(B->st.p, B->h_st, sizeof(KState) * n, hipMemcpyHostToDevice, st));
+  std::vector<char> live(member);
   std::vector<int> bmap;
-  build_layout(B, nullptr, bmap);
+  build_layout(B, &live, bmap);
   int rc = upload_layout(B, bmap);
   if (rc) return rc;
-  const dim3 g(B->Bcur);
   hipLaunchKernelGGL(k_b_t0, dim3(1), dim3(64), 0, st, B->tstamp.p, n);
+  if (B->Bcur == 0) {
+    CPGPU_HIP(ctx, hipStreamSynchronize(st));
+    return CPGPU_OK;
+  }
+  const dim3 g(B->Bcur);
   if (mode == 0) {
     hipLaunchKernelGGL(k_b_init<double>, g, kBlock, 0, st, B->probs.p, B->bmap.p,
                        B->partB64.p, B->Btot);
@@ -669,11 +673,11 @@ int cpgpu_batch_solve(cpgpu_batch* B, int method, int max_iter, double tol, int*
                        (const KState*)B->st.p, B->partB32.p, B->partR64.p, B->Btot);
   }
   CPGPU_HIP(ctx, hipGetLastError());
-  // batches until every problem stopped (the kernels stop a problem at
+  // batches until every member stopped (the kernels stop a problem at
   // max_iter + 1 by themselves; one batch past that bounds the loop)
   const long long nbatch_max = ((long long)max_iter + 2) / B->K + 2;
-  std::vector<char> live(n, 1);
-  int nlive = n;
+  int nlive = 0;
+  for (int p = 0; p < n; ++p) nlive += live[p] ? 1 : 0;
   for (long long nb = 0; nb < nbatch_max && nlive > 0; ++nb) {
     hipGraphExec_t ge = B->use_graphs ? iter_graph(B, mode) : nullptr;
     if (ge) {
@@ -699,8 +703,8 @@ int cpgpu_batch_solve(cpgpu_batch* B, int method, int max_iter, double tol, int*
       if ((rc = upload_layout(B, bmap))) return rc;
     }
   }
-  if (mode == 1) {  // fold the increments of problems stopped at max_iter
-    build_layout(B, nullptr, bmap);
+  if (mode == 1) {  // fold the increments of members stopped at max_iter
+    build_layout(B, &member, bmap);
     if ((rc = upload_layout(B, bmap))) return rc;
     hipLaunchKernelGGL(k_b_merge, dim3(B->Bcur), kBlock, 0, st, B->probs.p, B->bmap.p,
                        (const KState*)B->st.p, 1);
@@ -710,17 +714,92 @@ int cpgpu_batch_solve(cpgpu_batch* B, int method, int max_iter, double tol, int*
   CPGPU_HIP(ctx, hipMemcpyAsync(B->h_ts, B->tstamp.p, sizeof(unsigned long long) * (n + 1),
                                 hipMemcpyDeviceToHost, st));
   CPGPU_HIP(ctx, hipStreamSynchronize(st));
+  return CPGPU_OK;
+}
+
+int cpgpu_batch_solve(cpgpu_batch* B, int method, int max_iter, double tol, int* iters,
+                      double* resid, int* codes, double* latency_ms) {
+  if (!B || max_iter < 0 || !(tol >= 0.0)) return CPGPU_EINVAL;
+  if (method != CPGPU_JPCG && method != CPGPU_MIXED && method != CPGPU_FCG_UNET)
+    return set_error(B->ctx, CPGPU_EUNSUPPORTED,
+                     "batched methods: CPGPU_JPCG, CPGPU_MIXED, CPGPU_FCG_UNET");
+  cpgpu_ctx* ctx = B->ctx;
+  CPGPU_HIP(ctx, hipSetDevice(ctx->device));
+  const int n = B->n;
+  const auto t0 = std::chrono::steady_clock::now();
+  const auto since = [&]() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  };
+  std::vector<int> it_sum(n, 0);
   int all_conv = 1;
-  for (int p = 0; p < n; ++p) {
-    const KState& k = B->h_st[p];
-    const int code = k.done == 1 ? CPGPU_OK : CPGPU_MAXITER;
-    if (code != CPGPU_OK) all_conv = 0;
-    if (iters) iters[p] = k.iter;
-    if (resid) resid[p] = k.resid;
-    if (codes) codes[p] = code;
-    if (latency_ms)
-      latency_ms[p] = k.done ? (double)(B->h_ts[p] - B->h_ts[n]) / B->clock_khz : -1.0;
+  // one batched pass over `member`, results recorded; latencies offset by
+  // the host time at the pass's start
+  auto batched = [&](const std::vector<char>& member, int mode) -> int {
+    const double off = since();
+    int rc = run_batched(B, mode, max_iter, tol, member);
+    if (rc) return rc;
+    for (int p = 0; p < n; ++p) {
+      if (!member[p]) continue;
+      const KState& k = B->h_st[p];
+      const int code = k.done == 1 ? CPGPU_OK : CPGPU_MAXITER;
+      if (code != CPGPU_OK) all_conv = 0;
+      it_sum[p] += k.iter;
+      if (iters) iters[p] = it_sum[p];
+      if (resid) resid[p] = k.resid;
+      if (codes) codes[p] = code;
+      if (latency_ms)
+        latency_ms[p] = k.done ? off + (double)(B->h_ts[p] - B->h_ts[n]) / B->clock_khz : -1.0;
+    }
+    return CPGPU_OK;
+  };
+  if (method != CPGPU_FCG_UNET) {
+    const int rc = batched(std::vector<char>(n, 1), method == CPGPU_JPCG ? 0 : 1);
+    if (rc) return rc;
+    return all_conv ? CPGPU_OK : CPGPU_MAXITER;
   }
+  // CPGPU_FCG_UNET: members with a U-Net attached as preconditioner run
+  // flexible CG one after another (smallest first, one shape after another:
+  // the network's activations are laid out per shape); the others, and the
+  // learned members whose flexible CG stalls, the batched mixed iteration
+  std::vector<char> member(n, 1);
+  std::vector<int> learned;
+  for (int p = 0; p < n; ++p)
+    if (B->P[p]->unet && (B->P[p]->unet_flags & CPGPU_UNET_PRECOND)) {
+      member[p] = 0;
+      learned.push_back(p);
+    }
+  int rc = batched(member, 1);
+  if (rc) return rc;
+  std::sort(learned.begin(), learned.end(), [&](int a, int b) {
+    const cpgpu_poisson *A = B->P[a], *Q = B->P[b];
+    return A->N != Q->N ? A->N < Q->N : (A->nx != Q->nx ? A->nx < Q->nx : a < b);
+  });
+  std::vector<char> rest(n, 0);
+  bool any_rest = false;
+  // flexible-CG iterations a learned member gets before the batched pass
+  // takes over from its iterate (an application of the network costs tens
+  // of batched iterations, so a slowly converging member is cheaper there)
+  constexpr int kFcgBudget = 150;
+  for (int p : learned) {
+    cpgpu_poisson* P = B->P[p];
+    int mi = std::min(max_iter, kFcgBudget), hl = 0;
+    double t = tol;
+    P->fcg_fallback = false;
+    const int code = poisson_solve_internal(P, CPGPU_FCG_UNET, &mi, &t, CPGPU_NORM_TRUE, 0,
+                                            nullptr, 0, &hl);
+    P->fcg_fallback = true;
+    if (code < 0) return code;
+    it_sum[p] = mi;
+    if (iters) iters[p] = mi;
+    if (resid) resid[p] = t;
+    if (codes) codes[p] = code;
+    if (latency_ms) latency_ms[p] = since();
+    if (code != CPGPU_OK) {
+      rest[p] = 1;
+      any_rest = true;
+    }
+  }
+  if (any_rest && (rc = batched(rest, 1))) return rc;  // from the flexible-CG iterates
   return all_conv ? CPGPU_OK : CPGPU_MAXITER;
 }
 

@@ -310,7 +310,7 @@ int apply_precond(cpgpu_poisson* P, bool use_net) {
       rn = P->q32.p;
     }
     float* c = P->b32.p;
-    int rc = unet_apply_device(P->unet, P->nx, P->ny, rn, c);
+    int rc = unet_apply_device(P->unet, P->nx, P->ny, rn, c, pack_walls(P));
     if (rc) return rc;
     if (z) {
       hipLaunchKernelGGL(k_add32, dim3(1024), dim3(256), 0, st, z, c, P->N);
@@ -355,7 +355,7 @@ int unet_warm_start(cpgpu_poisson* P) {
   hipStream_t st = P->ctx->stream;
   hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, st, P->b.p,
                      P->b32.p, P->N);
-  if ((rc = unet_apply_device(P->unet, P->nx, P->ny, P->b32.p, P->x32.p)))
+  if ((rc = unet_apply_device(P->unet, P->nx, P->ny, P->b32.p, P->x32.p, pack_walls(P))))
     return rc;
   hipLaunchKernelGGL(k_f32_to_f64, dim3(1024), dim3(256), 0, st, P->x32.p,
                      P->x.p, P->N);
@@ -392,27 +392,25 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
   double res = fs.resid;
   std::vector<double> trace;
   trace.push_back(res);
-  constexpr int kStallWindow = 30;
+  constexpr int kStallWindow = 40;
   bool singular = true;  // all-Neumann: constants are the null space
   for (int sg = 0; sg < 4; ++sg)
     singular = singular && P->seg[sg].type == CPGPU_BC_NEUMANN;
   double best = res;
   std::vector<double> best_hist(1, res);
   int k = 0;
-  int seg = 0;          // iterations since the last (re)start of the directions
-  bool use_net = true;  // false after a stall: the smoothing-only cycle
   bool fallback = false;
   while (res > tol) {
     if (k >= max_iter) break;
     // z = preconditioned residual
-    if ((rc = apply_precond(P, use_net))) return rc;
+    if ((rc = apply_precond(P, true))) return rc;
     hipLaunchKernelGGL(k_fcg_zq, dim3(G1), dim3(256), 0, st, P->pa32.p,
-                       seg > 0 ? (const double*)P->q.p : nullptr, P->part64.p, N);  // 2 x G1 partials
+                       k > 0 ? (const double*)P->q.p : nullptr, P->part64.p, N);  // 2 x G1 partials
     const double* pold = P->pf[k & 1].p;
     double* pnew = P->pf[(k + 1) & 1].p;
     hipLaunchKernelGGL(k_fcg_dir, dim3(tl.G), dim3(TJ, BY), 0, st, P->sten64(),
                        P->pa32.p, pold, pnew, P->q.p, P->r.p, P->part64.p, G1,
-                       P->fst.p, seg == 0 ? 1 : 0, singular ? 1 : 0, P->partB64.p,
+                       P->fst.p, k == 0 ? 1 : 0, singular ? 1 : 0, P->partB64.p,
                        tl);
     hipLaunchKernelGGL(k_fcg_upd, dim3(G1), dim3(256), 0, st, pnew, P->q.p,
                        P->x.p, P->r.p, P->r32.p, P->partB64.p, (int)tl.G,
@@ -421,7 +419,6 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
                        P->fst.p);
     CPGPU_HIP(ctx, hipGetLastError());
     ++k;
-    ++seg;
     if ((rc = read_state(P, &fs))) return rc;
     res = fs.resid;
     trace.push_back(res);
@@ -429,30 +426,26 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
       fallback = true;
       break;
     }
-    // stall: the best residual improved by less than 10% over the last
+    // stall: the best residual improved by less than 5% over the last
     // kStallWindow iterations (the residual norm of a CG method is not
-    // monotone, so single-step or short-window tests misfire)
+    // monotone and plateaus for tens of iterations while the error still
+    // falls, so short windows and steep thresholds misfire)
     best = std::min(best, res);
     best_hist.push_back(best);
-    if (seg >= kStallWindow && best > 0.9 * best_hist[k - kStallWindow]) {
-      // The network maps the current residual to a correction that no
-      // longer reduces it (the residual stays put, and the deterministic
-      // network returns the same correction): continue from the current
-      // iterate with the smoothing-only cycle, a fixed symmetric positive
-      // definite preconditioner (damped Jacobi sweeps), directions
-      // restarted; Jacobi-PCG only if that stalls too.
-      if (use_net) {
-        use_net = false;
-        seg = 0;
-        continue;
-      }
+    if (k >= kStallWindow && best > 0.95 * best_hist[k - kStallWindow]) {
+      // The cycle no longer reduces the residual: the network maps the
+      // residual to corrections that do not reduce it (a
+      // deterministic network returns the same correction for a residual
+      // that stays put).  Jacobi-PCG finishes from the current iterate: per
+      // unit of progress its graph-batched iterations are cheaper than the
+      // cycle without the network.
       fallback = true;
       break;
     }
   }
   const int kf = k;
   int code = res <= tol ? CPGPU_OK : CPGPU_MAXITER;
-  if (code != CPGPU_OK && (fallback || k < max_iter)) {
+  if (code != CPGPU_OK && P->fcg_fallback && (fallback || k < max_iter)) {
     // Jacobi-PCG from the current iterate with the remaining budget
     int it = std::max(0, max_iter - kf);
     int done = 0;

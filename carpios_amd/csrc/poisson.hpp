@@ -162,6 +162,9 @@ struct cpgpu_poisson {
   cpgpu_unet* unet = nullptr;
   int unet_flags = 0;
   int fcg_iters = 0;  // flexible-CG iterations of the last CPGPU_FCG_UNET solve
+  // CPGPU_FCG_UNET finishes a stalled solve with Jacobi-PCG (off: it returns
+  // CPGPU_MAXITER at the stall; the batched solver finishes such members)
+  bool fcg_fallback = true;
   // preconditioner cycle of CPGPU_FCG_UNET: weighted-Jacobi sweeps before
   // and after the network (its coarse correction)
   int pc_pre = 2, pc_post = 2;
@@ -229,8 +232,11 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
                  int* iters, double* resid);
 // phi = s net(b / s), s = rms(b), with the attached U-Net (fcg.hip).
 int unet_warm_start(cpgpu_poisson* P);
-// out = s net(in / s), s = rms(in), on H x W device FP32 arrays (unet.hip).
-int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out);
+// out = s net(in / s), s = rms(in), on H x W device FP32 arrays, with the
+// wall types `walls` (pack_walls) as the network's second input (unet.hip).
+int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out, int walls);
+// The wall types of a problem packed for unet_apply_device (unet.hip).
+int pack_walls(const cpgpu_poisson* P);
 // Error1 / Error2 / Errori of val against exa (interior arrays with row
 // strides lde / ldv, j contiguous) on the cells of the sizes sx (nx), sy (ny);
 // part holds at least 7 doubles (norms.hip).  out: host, 3 values.

@@ -34,7 +34,7 @@ struct LayerDef {
 
 // fixed layer list (order = parameter order)
 static const LayerDef kLayers[] = {
-    {K_FIRST, 1, 16},  {K_CONV, 16, 16},  {K_CONV, 16, 32},  {K_CONV, 32, 32},
+    {K_FIRST, 2, 16},  {K_CONV, 16, 16},  {K_CONV, 16, 32},  {K_CONV, 32, 32},
     {K_CONV, 32, 64},  {K_CONV, 64, 64},  {K_CONV, 64, 128}, {K_CONV, 128, 128},
     {K_UP, 128, 64},   {K_CONV, 128, 64}, {K_CONV, 64, 64},  {K_UP, 64, 32},
     {K_CONV, 64, 32},  {K_CONV, 32, 32},  {K_UP, 32, 16},    {K_CONV, 32, 16},
@@ -61,42 +61,63 @@ constexpr int kRedBlocks = 1024;  // blocks of the per-channel reductions
 // ---------------------------------------------------------------------
 // small kernels
 
-// first conv 1 -> 16 (FP32 input, direct): a[m][co] = relu(b + sum W x)
+// Wall code of cell (i, j), the network's second input channel: the sum
+// over the walls the cell touches of +1 (Dirichlet) or -1 (Neumann); walls
+// packs the types of the x-, x+, y-, y+ segments, two bits each (0: no
+// wall information).
+__device__ __forceinline__ float wall_code(int i, int j, int H, int W, int walls) {
+  auto v = [](int c) { return c == CPGPU_BC_DIRICHLET ? 1.f : (c == CPGPU_BC_NEUMANN ? -1.f : 0.f); };
+  float s = 0.f;
+  if (i == 0) s += v(walls & 3);
+  if (i == H - 1) s += v((walls >> 2) & 3);
+  if (j == 0) s += v((walls >> 4) & 3);
+  if (j == W - 1) s += v((walls >> 6) & 3);
+  return s;
+}
+
+// first conv 2 -> 16 (FP32 input and wall code, direct):
+// a[m][co] = relu(b + sum_tap W[tap][co][0] x + W[tap][co][1] wall)
 __global__ __launch_bounds__(256) void k_first_fwd(const float* __restrict__ x,
-                                                   int H, int W, View a,
+                                                   int H, int W, int walls, View a,
                                                    const float* __restrict__ w,
                                                    const float* __restrict__ b) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= H * W) return;
   const int i = m / W, j = m - i * W;
-  float xv[9];
+  float xv[9], wv[9];
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
     const int si = i + t / 3 - 1, sj = j + t % 3 - 1;
-    xv[t] = (si >= 0 && si < H && sj >= 0 && sj < W) ? x[si * W + sj] : 0.f;
+    const bool in = si >= 0 && si < H && sj >= 0 && sj < W;
+    xv[t] = in ? x[si * W + sj] : 0.f;
+    wv[t] = in ? wall_code(si, sj, H, W, walls) : 0.f;
   }
   bf16 out[16];
 #pragma unroll
   for (int co = 0; co < 16; ++co) {
     float s = b[co];
 #pragma unroll
-    for (int t = 0; t < 9; ++t) s += w[t * 16 + co] * xv[t];
+    for (int t = 0; t < 9; ++t) {
+      s += w[(t * 16 + co) * 2] * xv[t];
+      s += w[(t * 16 + co) * 2 + 1] * wv[t];
+    }
     out[co] = f2bf(s > 0.f ? s : 0.f);
   }
   __builtin_memcpy(a.p + (size_t)m * a.cs + a.co, out, 32);
 }
 
-// first conv weight/bias gradient partials: per block 9*16 + 16 values
+// first conv weight/bias gradient partials: per block kFirstPart values
+constexpr int kFirstPart = 9 * 16 * 2 + 16;
 __global__ __launch_bounds__(256) void k_first_wgrad(const float* __restrict__ x,
-                                                     int H, int W, GView g,
+                                                     int H, int W, int walls, GView g,
                                                      View a,
                                                      float* __restrict__ part) {
   // thread t owns output channel co = t % 16 and walks pixels 16 apart
-  __shared__ float red[10][256];
+  __shared__ float red[19][256];
   const int co = threadIdx.x & 15;
-  float acc[10];
+  float acc[19];
 #pragma unroll
-  for (int k = 0; k < 10; ++k) acc[k] = 0.f;
+  for (int k = 0; k < 19; ++k) acc[k] = 0.f;
   const int M = H * W;
   for (int m = blockIdx.x * 16 + (threadIdx.x >> 4); m < M; m += gridDim.x * 16) {
     const int i = m / W, j = m - i * W;
@@ -105,26 +126,33 @@ __global__ __launch_bounds__(256) void k_first_wgrad(const float* __restrict__ x
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
       const int si = i + t / 3 - 1, sj = j + t % 3 - 1;
-      const float xv = (si >= 0 && si < H && sj >= 0 && sj < W) ? x[si * W + sj] : 0.f;
-      acc[t] += gv * xv;
+      const bool in = si >= 0 && si < H && sj >= 0 && sj < W;
+      const float xv = in ? x[si * W + sj] : 0.f;
+      const float wv = in ? wall_code(si, sj, H, W, walls) : 0.f;
+      acc[2 * t] += gv * xv;
+      acc[2 * t + 1] += gv * wv;
     }
-    acc[9] += gv;
+    acc[18] += gv;
   }
 #pragma unroll
-  for (int k = 0; k < 10; ++k) red[k][threadIdx.x] = acc[k];
+  for (int k = 0; k < 19; ++k) red[k][threadIdx.x] = acc[k];
   __syncthreads();
   for (int s = 128; s >= 16; s >>= 1) {
     if ((int)threadIdx.x < s) {
 #pragma unroll
-      for (int k = 0; k < 10; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+      for (int k = 0; k < 19; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
     }
     __syncthreads();
   }
-  // part[blk*160 + tap*16 + co] (weights [tap][co]), then 16 biases
+  // part[blk*kFirstPart + (tap*16 + co)*2 + ci] (weights [tap][co][ci]),
+  // then 16 biases
   if (threadIdx.x < 16) {
+    float* q = part + (size_t)blockIdx.x * kFirstPart;
 #pragma unroll
-    for (int k = 0; k < 10; ++k)
-      part[(size_t)blockIdx.x * 160 + k * 16 + threadIdx.x] = red[k][threadIdx.x];
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+      for (int ci = 0; ci < 2; ++ci) q[(t * 16 + threadIdx.x) * 2 + ci] = red[2 * t + ci][threadIdx.x];
+    q[288 + threadIdx.x] = red[18][threadIdx.x];
   }
 }
 
@@ -432,6 +460,8 @@ using namespace cpgpu::unet;
 
 struct cpgpu_unet {
   cpgpu_ctx* ctx = nullptr;
+  int conv_rows = 4;  // pixel rows per wave of the 16-channel tile convolutions (4 or 8)
+  int walls = 0;      // wall types of the current input (2 bits per segment x-, x+, y-, y+)
   size_t woff[kNL], boff[kNL], wbf_off[kNL];
   size_t nparams = 0, nwbf = 0;
   DevBuf<float> params, grads, m1, m2;
@@ -495,7 +525,7 @@ int unet_alloc(cpgpu_unet* U, int H, int W) {
   const size_t N = (size_t)H * W;
   // partial sums of the largest weight-gradient launch (chunks x taps x
   // cout x cin), of the bias/first/output reductions otherwise
-  size_t npart = (size_t)kRedBlocks * 160;
+  size_t npart = (size_t)kRedBlocks * kFirstPart;
   for (int l = 0; l < kNL; ++l) {
     const LayerDef& d = kLayers[l];
     if (d.kind != K_CONV && d.kind != K_UP) continue;
@@ -544,9 +574,18 @@ void conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu) {
     return;
   }
   if constexpr (CIN <= 64) {
-    if (tiled<CIN>(x.H, x.W)) {
-      CPGPU_LAUNCH(ev, (k_conv3_tile<CIN, COUT, true>), dim3(x.W / 16, x.H / 16),
-                   dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu);
+    if (tiled<CIN>(x.H, x.W) && y.cs % 4 == 0 && y.co % 4 == 0) {  // 8-byte stores
+      if constexpr (CIN == 16) {
+        if (x.H % 32 == 0 && U->conv_rows == 8)
+          CPGPU_LAUNCH(ev, (k_conv3_tile16<COUT, true, 8>), dim3(x.W / 16, x.H / 32),
+                       dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu);
+        else
+          CPGPU_LAUNCH(ev, (k_conv3_tile16<COUT, true, 4>), dim3(x.W / 16, x.H / 16),
+                       dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu);
+      } else {
+        CPGPU_LAUNCH(ev, (k_conv3_tile<CIN, COUT, true>), dim3(x.W / 16, x.H / 16),
+                     dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu);
+      }
       return;
     }
   }
@@ -571,8 +610,8 @@ int forward(cpgpu_unet* U) {
   hipStream_t st = U->ctx->stream;
   const int M = U->H * U->W;
   hipLaunchKernelGGL(k_first_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
-                     U->x0.p, U->H, U->W, U->v(U->a0), U->params.p + U->woff[0],
-                     U->params.p + U->boff[0]);
+                     U->x0.p, U->H, U->W, U->walls, U->v(U->a0),
+                     U->params.p + U->woff[0], U->params.p + U->boff[0]);
   conv_fwd<16, 16>(U, U->v(U->a0), U->v(U->C0, 16), 1, 1);
   auto pool = [&](View in, View out, int C) {
     const int n = out.H * out.W * C;
@@ -634,10 +673,19 @@ int conv_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, int in_co,
       done = true;
     }
     if constexpr (COUT <= 64) {
-      if (!done && tiled<COUT>(in.H, in.W)) {
-        hipLaunchKernelGGL((k_conv3_tile<COUT, CIN, false>), dim3(in.W / 16, in.H / 16),
-                           dim3(256), 0, st, gq, View{}, U->g(in, in_co), wt,
-                           (const float*)nullptr, 0);
+      const GView gx = U->g(in, in_co);
+      if (!done && tiled<COUT>(in.H, in.W) && gx.cs % 4 == 0 && gx.co % 4 == 0) {
+        if constexpr (COUT == 16) {
+          if (in.H % 32 == 0 && U->conv_rows == 8)
+            hipLaunchKernelGGL((k_conv3_tile16<CIN, false, 8>), dim3(in.W / 16, in.H / 32),
+                               dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0);
+          else
+            hipLaunchKernelGGL((k_conv3_tile16<CIN, false, 4>), dim3(in.W / 16, in.H / 16),
+                               dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0);
+        }
+        else
+          hipLaunchKernelGGL((k_conv3_tile<COUT, CIN, false>), dim3(in.W / 16, in.H / 16),
+                             dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0);
         done = true;
       }
     }
@@ -732,9 +780,9 @@ int backward(cpgpu_unet* U) {
   conv_bwd<16, 16>(U, U->a0, 0, U->C0, 16, 1, true);
   const int nbf = kRedBlocks;
   hipLaunchKernelGGL(k_first_wgrad, dim3(nbf), dim3(256), 0, st, U->x0.p, U->H,
-                     U->W, U->g(U->a0), U->v(U->a0), U->part.p);
-  hipLaunchKernelGGL(k_sum_chunks, sum_grid(160), dim3(256), 0, st, U->part.p, nbf,
-                     160, U->grads.p + U->woff[0]);  // 144 weights then 16 biases
+                     U->W, U->walls, U->g(U->a0), U->v(U->a0), U->part.p);
+  hipLaunchKernelGGL(k_sum_chunks, sum_grid(kFirstPart), dim3(256), 0, st, U->part.p, nbf,
+                     kFirstPart, U->grads.p + U->woff[0]);  // 288 weights then 16 biases
   CPGPU_HIP(U->ctx, hipGetLastError());
   return CPGPU_OK;
 }
@@ -789,6 +837,7 @@ int sample_loss(cpgpu_unet* U, cpgpu_poisson* P, int slot, float dy_scale,
   hipStream_t st = U->ctx->stream;
   cpgpu_unet::Sample& S = *U->samples[slot];
   const size_t N = (size_t)U->H * U->W;
+  U->walls = pack_walls(P);
   int rc = apply_scaled(U, S.r.p, nullptr, N);
   if (rc) return rc;
   hipLaunchKernelGGL(k_loss_d, dim3(1024), dim3(256), 0, st, U->y.p, S.e.p,
@@ -833,9 +882,16 @@ struct SplitMix {
 // ---------------------------------------------------------------------
 // internal entry points used by the learned solver
 namespace cpgpu {
-int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out) {
+int pack_walls(const cpgpu_poisson* P) {
+  int w = 0;
+  for (int k = 0; k < 4; ++k) w |= (P->seg[k].type & 3) << (2 * k);
+  return w;
+}
+
+int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out, int walls) {
   int rc = unet_alloc(U, H, W);
   if (rc) return rc;
+  U->walls = walls;
   return apply_scaled(U, in, out, (size_t)H * W);
 }
 cpgpu_ctx* unet_context(const cpgpu_unet* U) { return U->ctx; }
@@ -910,6 +966,26 @@ int cpgpu_unet_create(cpgpu_ctx* ctx, unsigned long long seed,
   if (rc) return rc;
   CPGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *out = U.release();
+  return CPGPU_OK;
+}
+
+int cpgpu_unet_set_walls(cpgpu_unet* U, const int* bc_type) {
+  if (!U) return CPGPU_EINVAL;
+  int w = 0;
+  if (bc_type) {
+    for (int k = 0; k < 4; ++k) {
+      if (bc_type[k] != CPGPU_BC_DIRICHLET && bc_type[k] != CPGPU_BC_NEUMANN)
+        return CPGPU_EINVAL;
+      w |= bc_type[k] << (2 * k);
+    }
+  }
+  U->walls = w;
+  return CPGPU_OK;
+}
+
+int cpgpu_unet_set_conv_rows(cpgpu_unet* U, int rows) {
+  if (!U || (rows != 4 && rows != 8)) return CPGPU_EINVAL;
+  U->conv_rows = rows;
   return CPGPU_OK;
 }
 
@@ -1055,6 +1131,14 @@ int cpgpu_unet_set_sample(cpgpu_unet* U, cpgpu_poisson* P, int slot,
     return set_error(U->ctx, CPGPU_EINVAL, "sample %d has e'Ae = %g (need > 0)",
                      slot, eae);
   S.eAe = eae;
+  return CPGPU_OK;
+}
+
+int cpgpu_unet_clear_samples(cpgpu_unet* U) {
+  if (!U) return CPGPU_EINVAL;
+  CPGPU_HIP(U->ctx, hipStreamSynchronize(U->ctx->stream));  // no kernel reads them any more
+  U->samples.clear();
+  U->sH = U->sW = 0;
   return CPGPU_OK;
 }
 

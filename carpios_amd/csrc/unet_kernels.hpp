@@ -148,25 +148,32 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(View x, View y,
         if (si >= 0 && si < x.H && sj >= 0 && sj < x.W)
           af = F::load(x.p + (size_t)(si * x.W + sj) * x.cs + x.co + c0 + kq);
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(af, bf[b], acc[a][b]);
+        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(bf[b], af, acc[a][b]);
       }
     }
   }
+  // D[channel][pixel]: four consecutive channels of one pixel per lane
 #pragma unroll
-  for (int a = 0; a < MP; ++a)
+  for (int a = 0; a < MP; ++a) {
+    const int m = m0 + a * 16 + (lane & 15);
+    if (m >= M) continue;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const int co = n0 + b * 16 + (lane & 15);
-      const float bb = bias[co];
+      const int co = n0 + b * 16 + 4 * (lane >> 4);
+      uint32_t h[2];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int m = m0 + a * 16 + 4 * (lane >> 4) + t;
-        if (m >= M) continue;
-        float v = acc[a][b][t] + bb;
-        if (relu) v = v > 0.f ? v : 0.f;
-        y.p[(size_t)m * y.cs + y.co + co] = f2bf(v);
+      for (int t = 0; t < 4; t += 2) {
+        float v0 = acc[a][b][t] + bias[co + t];
+        float v1 = acc[a][b][t + 1] + bias[co + t + 1];
+        if (relu) {
+          v0 = v0 > 0.f ? v0 : 0.f;
+          v1 = v1 > 0.f ? v1 : 0.f;
+        }
+        h[t >> 1] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
       }
+      *reinterpret_cast<uint2*>(y.p + (size_t)m * y.cs + y.co + co) = uint2{h[0], h[1]};
     }
+  }
 }
 
 // conv 3x3 input gradient: dx[m][ci] = sum_tap sum_co g[m - off][co]
@@ -214,21 +221,22 @@ __global__ __launch_bounds__(256) void k_conv3_dgrad(View dy, GView dx,
         if (si >= 0 && si < dy.H && sj >= 0 && sj < dy.W)
           af = F::load(dy.p + (size_t)(si * dy.W + sj) * dy.cs + dy.co + c0 + kq);
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(af, bf[b], acc[a][b]);
+        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(bf[b], af, acc[a][b]);
       }
     }
   }
+  // D[channel][pixel]: four consecutive channels of one pixel per lane
 #pragma unroll
-  for (int a = 0; a < MP; ++a)
+  for (int a = 0; a < MP; ++a) {
+    const int m = m0 + a * 16 + (lane & 15);
+    if (m >= M) continue;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const int ci = n0 + b * 16 + (lane & 15);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int m = m0 + a * 16 + 4 * (lane >> 4) + t;
-        if (m < M) dx.p[(size_t)m * dx.cs + dx.co + ci] = acc[a][b][t];
-      }
+      const int ci = n0 + b * 16 + 4 * (lane >> 4);
+      *reinterpret_cast<float4*>(dx.p + (size_t)m * dx.cs + dx.co + ci) =
+          float4{acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
     }
+  }
 }
 
 // 3x3 convolution (pad 1) of one 16 x 16 pixel tile per block, the input
@@ -240,7 +248,11 @@ __global__ __launch_bounds__(256) void k_conv3_dgrad(View dy, GView dx,
 //   !FWD: dx[p][n] = sum_tap sum_k w[tap][n][k] x[p - off][k]
 //         (the input gradient: x = masked BF16 output gradient, w = the
 //         transposed weights [tap][ci][co], FP32 out)
-// Wave w computes tile rows 4w..4w+3 (16 pixels each) x NO channels.
+// Wave w computes tile rows 4w..4w+3 (16 pixels each) x NO channels.  The
+// product is formed as D[channel][pixel] (weights as the A operand): a lane
+// then holds four consecutive output channels of one pixel, stored as one
+// 8-byte (BF16) or 16-byte (FP32) write, and the 16 lanes of a group cover 16
+// consecutive pixels, so a wave's store is one contiguous row segment.
 template <int KCH, int NO, bool FWD>
 __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
                                                     const bf16* __restrict__ w,
@@ -287,27 +299,117 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
         const typename F::T af =
             F::load(sx + ((r + 1 + dy) * TP + px + 1 + dx) * KCH + c0 + kq);
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(af, bf[b], acc[a][b]);
+        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(bf[b], af, acc[a][b]);
       }
     }
   }
+  const int cq = 4 * (lane >> 4);  // first of the lane's four channels
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int gi = ti0 + wave * 4 + a;
+    const size_t m = (size_t)(ti0 + wave * 4 + a) * W + tj0 + px;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const int n = b * 16 + px;
-      const float bb = FWD ? bias[n] : 0.f;
+      const int n = b * 16 + cq;
+      if (FWD) {
+        uint32_t h[2];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const size_t m = (size_t)gi * W + tj0 + 4 * (lane >> 4) + t;
-        if (FWD) {
-          float v = acc[a][b][t] + bb;
-          if (relu) v = v > 0.f ? v : 0.f;
-          yb.p[m * yb.cs + yb.co + n] = f2bf(v);
-        } else {
-          yf.p[m * yf.cs + yf.co + n] = acc[a][b][t];
+        for (int t = 0; t < 4; t += 2) {
+          float v0 = acc[a][b][t] + bias[n + t];
+          float v1 = acc[a][b][t + 1] + bias[n + t + 1];
+          if (relu) {
+            v0 = v0 > 0.f ? v0 : 0.f;
+            v1 = v1 > 0.f ? v1 : 0.f;
+          }
+          h[t >> 1] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
         }
+        *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) = uint2{h[0], h[1]};
+      } else {
+        *reinterpret_cast<float4*>(yf.p + m * yf.cs + yf.co + n) =
+            float4{acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
+      }
+    }
+  }
+}
+
+// k_conv3_tile for 16 input channels: the nine taps are paired into five
+// K = 32 MFMAs (lanes 0-31 take the pair's first tap, 32-63 its second; the
+// fifth pair's second half is zero), and all weight fragments are loaded
+// before the tile is staged, so their latency overlaps the staging.  A block
+// covers 4*ROWS x 16 pixels (wave w: rows ROWS*w .. ROWS*w + ROWS - 1).
+template <int NO, bool FWD, int ROWS>
+__global__ __launch_bounds__(256) void k_conv3_tile16(View x, View yb, GView yf,
+                                                      const bf16* __restrict__ w,
+                                                      const float* __restrict__ bias,
+                                                      int relu) {
+  constexpr int KCH = 16, NB = NO / 16, TP = 18, TR = 4 * ROWS + 2, CPP = 2;
+  typedef Frag<32> F;
+  __shared__ __attribute__((aligned(16))) bf16 sx[TR * TP * KCH];
+  const int tj0 = blockIdx.x * 16, ti0 = blockIdx.y * 4 * ROWS;
+  const int H = x.H, W = x.W;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int px = lane & 15, half = lane >> 5, kc = 8 * ((lane >> 4) & 1);
+  typename F::T wf[5][NB];
+#pragma unroll
+  for (int pr = 0; pr < 5; ++pr) {
+    const int tap = 2 * pr + half;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      wf[pr][b] = tap < 9 ? F::load(w + ((size_t)(tap * NO + b * 16 + px) * KCH + kc))
+                          : F::zero();
+  }
+  for (int e = threadIdx.x; e < TR * TP * CPP; e += 256) {
+    const int pix = e / CPP, ch = e - pix * CPP;
+    const int r = pix / TP, c = pix - r * TP;
+    const int gi = ti0 + r - 1, gj = tj0 + c - 1;
+    uint4 v = {0u, 0u, 0u, 0u};
+    if (gi >= 0 && gi < H && gj >= 0 && gj < W)
+      v = *reinterpret_cast<const uint4*>(x.p + ((size_t)gi * W + gj) * x.cs + x.co + ch * 8);
+    *reinterpret_cast<uint4*>(sx + pix * KCH + ch * 8) = v;
+  }
+  __syncthreads();
+  f32x4_t acc[ROWS][NB];
+#pragma unroll
+  for (int a = 0; a < ROWS; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int pr = 0; pr < 5; ++pr) {
+    const int tap = 2 * pr + half;
+    const int tp = tap < 9 ? tap : 4;  // padded lanes: any in-tile position, zeroed below
+    const int dy = FWD ? tp / 3 - 1 : 1 - tp / 3;
+    const int dx = FWD ? tp % 3 - 1 : 1 - tp % 3;
+#pragma unroll
+    for (int a = 0; a < ROWS; ++a) {
+      const int r = wave * ROWS + a;
+      typename F::T xf = F::load(sx + ((r + 1 + dy) * TP + px + 1 + dx) * KCH + kc);
+      if (tap >= 9) xf = F::zero();
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(wf[pr][b], xf, acc[a][b]);
+    }
+  }
+  const int cq = 4 * (lane >> 4);
+#pragma unroll
+  for (int a = 0; a < ROWS; ++a) {
+    const size_t m = (size_t)(ti0 + wave * ROWS + a) * W + tj0 + px;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int n = b * 16 + cq;
+      if (FWD) {
+        uint32_t h[2];
+#pragma unroll
+        for (int t = 0; t < 4; t += 2) {
+          float v0 = acc[a][b][t] + bias[n + t];
+          float v1 = acc[a][b][t + 1] + bias[n + t + 1];
+          if (relu) {
+            v0 = v0 > 0.f ? v0 : 0.f;
+            v1 = v1 > 0.f ? v1 : 0.f;
+          }
+          h[t >> 1] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+        }
+        *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) = uint2{h[0], h[1]};
+      } else {
+        *reinterpret_cast<float4*>(yf.p + m * yf.cs + yf.co + n) =
+            float4{acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
       }
     }
   }
@@ -347,26 +449,30 @@ __global__ __launch_bounds__(256) void k_up_fwd(View x, View y,
       typename F::T af = F::zero();
       if (m < M) af = F::load(x.p + (size_t)m * x.cs + x.co + c0 + kq);
 #pragma unroll
-      for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(af, bf[b], acc[a][b]);
+      for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(bf[b], af, acc[a][b]);
     }
   }
+  // D[(ab, co)][coarse pixel]: four consecutive co of one fine pixel per
+  // lane (COUT is a multiple of 16, so the four share ab)
 #pragma unroll
-  for (int a = 0; a < MP; ++a)
+  for (int a = 0; a < MP; ++a) {
+    const int m = m0 + a * 16 + (lane & 15);
+    if (m >= M) continue;
+    const int I = m / x.W, J = m - I * x.W;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const int n = n0 + b * 16 + (lane & 15);
+      const int n = n0 + b * 16 + 4 * (lane >> 4);
       const int ab = n / COUT, co = n - ab * COUT;
       const int ai = ab >> 2, bj = ab & 3;
-      const float bb = bias[co];
+      const size_t f = (size_t)(4 * I + ai) * y.W + (4 * J + bj);
+      uint32_t h[2];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int m = m0 + a * 16 + 4 * (lane >> 4) + t;
-        if (m >= M) continue;
-        const int I = m / x.W, J = m - I * x.W;
-        const size_t f = (size_t)(4 * I + ai) * y.W + (4 * J + bj);
-        y.p[f * y.cs + y.co + co] = f2bf(acc[a][b][t] + bb);
-      }
+      for (int t = 0; t < 4; t += 2)
+        h[t >> 1] = (uint32_t)f2bf(acc[a][b][t] + bias[co + t]) |
+                    ((uint32_t)f2bf(acc[a][b][t + 1] + bias[co + t + 1]) << 16);
+      *reinterpret_cast<uint2*>(y.p + f * y.cs + y.co + co) = uint2{h[0], h[1]};
     }
+  }
 }
 
 // up-convolution input gradient: dx[m][ci] = sum_{ab,co} dy[fine(m,ab)][co]
@@ -408,21 +514,22 @@ __global__ __launch_bounds__(256) void k_up_dgrad(View dy, GView dx,
           af = F::load(dy.p + f * dy.cs + dy.co + c0 + kq);
         }
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(af, bf[b], acc[a][b]);
+        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(bf[b], af, acc[a][b]);
       }
     }
   }
+  // D[channel][pixel]: four consecutive channels of one pixel per lane
 #pragma unroll
-  for (int a = 0; a < MP; ++a)
+  for (int a = 0; a < MP; ++a) {
+    const int m = m0 + a * 16 + (lane & 15);
+    if (m >= M) continue;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const int ci = n0 + b * 16 + (lane & 15);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int m = m0 + a * 16 + 4 * (lane >> 4) + t;
-        if (m < M) dx.p[(size_t)m * dx.cs + dx.co + ci] = acc[a][b][t];
-      }
+      const int ci = n0 + b * 16 + 4 * (lane >> 4);
+      *reinterpret_cast<float4*>(dx.p + (size_t)m * dx.cs + dx.co + ci) =
+          float4{acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
     }
+  }
 }
 
 // ---------------------------------------------------------------------

@@ -160,3 +160,72 @@ class CavityTrainer:
             self.fcg_steps(round_steps)
             self.fit(round_fit, 5e-4, r + 1)
         return self.log
+
+
+# ------------------------------------------------------- batches of problems
+def train_problem_net(ctx, net, seed=7, count=16, min_cells=1 << 16, lo=6, hi=10,
+                      nu=2, omega=0.8, passes=16, fit=60, dagger_at=(4, 8, 12), lr=1e-3,
+                      tol=1e-10, checkpoints=(0, 16, 64, 256, 1024), fcg_ks=(2, 4, 8, 16),
+                      log=None):
+    """Trains `net` as the flexible-CG preconditioner of random problems of the
+    batch's kind: the problem set drawn from `seed` (cpgpu_batch_draw: sizes,
+    wall types, blob sources; a different seed from the measured set), members
+    with at least `min_cells` cells, up to `count` of them.
+
+    Per member: its FP64 solution phi (J-PCG from zero to `tol`) and the
+    errors phi - x_k of its J-PCG iterates (k = checkpoints) are kept on the
+    host.  Training interleaves the members: `passes` sweeps over them, each
+    member's errors set as the samples (stored after `nu` Jacobi sweeps; the
+    samples of one member share its operator) and `fit` Adam steps taken on
+    them, the learning rate falling to a tenth over the passes.  Before the
+    passes in `dagger_at`, the errors of the current network's flexible-CG
+    iterates (k = fcg_ks) are added to every member's set.
+    Returns (seconds, samples, mean loss per pass)."""
+    from . import Poisson, uniform_grid_desc, draw_problems
+    nx, ny, bc, bs = draw_problems(seed, 8 * count, lo, hi)
+    pick = [p for p in range(len(nx)) if int(nx[p]) * int(ny[p]) >= min_cells][:count]
+    t0 = time.perf_counter()
+    members = []
+    for p in pick:
+        P = Poisson(ctx, uniform_grid_desc(int(nx[p]), int(ny[p]), 1,
+                                           [int(t) for t in bc[p]]))
+        P.set_source_blobs(int(bs[p]))
+        P.set_fcg(nu, nu, omega)
+        P.fill_phi(0.0)
+        P.solve("JPCG", max_iter=10 ** 7, tol=tol, norm=NORM_TRUE)
+        phi = interior(P, P.get_vec(0))
+        errs = []
+        for c in checkpoints:
+            if c == 0:
+                errs.append(phi)
+                continue
+            P.fill_phi(0.0)
+            P.solve("JPCG", max_iter=int(c), tol=1e-300, norm=NORM_TRUE)
+            errs.append(phi - interior(P, P.get_vec(0)))
+        members.append((P, phi, errs))
+    curve = []
+    for ps in range(passes):
+        if ps in dagger_at:
+            for P, phi, errs in members:
+                P.set_unet(net, UNET_PRECOND)
+                for c in fcg_ks:
+                    P.fill_phi(0.0)
+                    P.solve("FCG_UNET", max_iter=int(c), tol=1e-300, norm=NORM_TRUE)
+                    errs.append(phi - interior(P, P.get_vec(0)))
+                P.set_unet(None, 0)
+        rate = lr * 0.1 ** (ps / max(passes - 1, 1))
+        losses = []
+        for k, (P, phi, errs) in enumerate(members):
+            net.clear_samples()
+            for slot, e in enumerate(errs):
+                net.set_sample(P, slot, e, smooth=nu, omega=omega)
+            losses.append(float(np.mean(net.train(P, fit, batch=4, lr=rate,
+                                                  seed=1000 * ps + k)[-5:])))
+        curve.append(float(np.mean(losses)))
+        if log:
+            log(f"train_problem_net: pass {ps} lr {rate:.1e} mean loss {curve[-1]:.3f}")
+    nsamp = sum(len(errs) for _, _, errs in members)
+    net.clear_samples()
+    for P, _, _ in members:
+        P.close()
+    return time.perf_counter() - t0, nsamp, curve

@@ -199,11 +199,16 @@ int cpgpu_batch_set_tiles(cpgpu_batch *B, int tiles_per_block);
 /* Graph-captured iteration batches on (default) or plain launches. */
 int cpgpu_batch_set_graphs(cpgpu_batch *B, int use_graphs);
 /* Solves every problem: method CPGPU_JPCG (FP64, the reference's J-PCG test
- * on the true relative residual every iteration) or CPGPU_MIXED (FP32
- * iterations, FP64 residual replacement and test every 16 iterations).
- * Per problem (arrays of n, each may be NULL): iterations, final relative
- * residual, status code, and the time from the start of the solve to the
- * problem's stop in ms (device clock).  Returns CPGPU_OK when all
+ * on the true relative residual every iteration), CPGPU_MIXED (FP32
+ * iterations, FP64 residual replacement and test every 16 iterations), or
+ * CPGPU_FCG_UNET: the members without a U-Net attached as preconditioner
+ * (cpgpu_poisson_set_unet, CPGPU_UNET_PRECOND) by the batched CPGPU_MIXED
+ * iteration, then the others by flexible CG one after another, smallest
+ * first; a member whose flexible CG stalls or takes more than 150
+ * iterations is finished, from its iterate, by a second batched pass.  Per problem (arrays of n, each may be NULL): iterations,
+ * final relative residual (true norm), status code, and the time from the
+ * start of the solve to the problem's stop in ms (device clock for batched
+ * members, host clock for flexible-CG ones).  Returns CPGPU_OK when all
  * converged, else CPGPU_MAXITER. */
 int cpgpu_batch_solve(cpgpu_batch *B, int method, int max_iter, double tol,
                       int *iters, double *resid, int *codes,
@@ -316,11 +321,20 @@ typedef struct cpgpu_unet cpgpu_unet;
 int cpgpu_unet_create(cpgpu_ctx *ctx, unsigned long long seed,
                       cpgpu_unet **out);
 int cpgpu_unet_destroy(cpgpu_unet *U);
+/* Launch shape of the 16-input-channel 3x3 convolutions: 4 or 8 pixel rows
+ * per wavefront (blocks of 16 x 16 or 32 x 16 pixels; default 4). */
+int cpgpu_unet_set_conv_rows(cpgpu_unet *U, int rows);
 long long cpgpu_unet_num_params(const cpgpu_unet *U);
 /* Flat FP32 parameter vector (grads != 0: the last computed gradients);
  * per layer: weights [tap][cout][cin], then biases [cout]. */
 int cpgpu_unet_get_params(cpgpu_unet *U, float *host, int grads);
 int cpgpu_unet_set_params(cpgpu_unet *U, const float *host);
+/* The network's second input channel: for each cell the sum over the walls
+ * it touches of +1 (Dirichlet) or -1 (Neumann).  bc_type[4] are the types of
+ * the x-, x+, y-, y+ segments (as cpgpu_grid2d.bc_type), NULL for none (all
+ * zero).  Used by cpgpu_unet_forward / _backward; solves and training take
+ * the wall types of their problem. */
+int cpgpu_unet_set_walls(cpgpu_unet *U, const int *bc_type);
 /* y = net(x) on host arrays of H*W values (no input scaling). */
 int cpgpu_unet_forward(cpgpu_unet *U, int H, int W, const float *x, float *y);
 /* Gradients of sum(dy * y) after cpgpu_unet_forward into grads (host,
@@ -344,6 +358,10 @@ int cpgpu_unet_adam(cpgpu_unet *U, float lr);
 int cpgpu_unet_set_sample(cpgpu_unet *U, cpgpu_poisson *P, int slot,
                           const double *e, int smooth, double omega);
 int cpgpu_unet_num_samples(const cpgpu_unet *U);
+/* Drops every sample (training then moves on to another problem: the
+ * samples and the problem passed to cpgpu_unet_train must share one
+ * operator); the parameters and the Adam state are kept. */
+int cpgpu_unet_clear_samples(cpgpu_unet *U);
 /* steps Adam updates, each on the mean energy-norm loss of batch samples
  * drawn with a seeded generator:  L = (s y - e)' A (s y - e) / e' A e,
  * y = net(r / s), s = rms(r).  losses (may be NULL) receives each step's

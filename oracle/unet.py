@@ -22,8 +22,9 @@ Arrays are NHWC: (H, W, C), H rows of W contiguous pixels.
 """
 import numpy as np
 
-# (kind, cin, cout); kinds: first 3x3 1->16 in FP32, conv 3x3, up 4x4/4, out 1x1
-LAYERS = [("first", 1, 16), ("conv", 16, 16), ("conv", 16, 32), ("conv", 32, 32),
+# (kind, cin, cout); kinds: first 3x3 2->16 in FP32 (residual, wall code),
+# conv 3x3, up 4x4/4, out 1x1
+LAYERS = [("first", 2, 16), ("conv", 16, 16), ("conv", 16, 32), ("conv", 32, 32),
           ("conv", 32, 64), ("conv", 64, 64), ("conv", 64, 128),
           ("conv", 128, 128), ("up", 128, 64), ("conv", 128, 64),
           ("conv", 64, 64), ("up", 64, 32), ("conv", 64, 32), ("conv", 32, 32),
@@ -145,8 +146,25 @@ def pool4_dx(g):
 
 
 # ------------------------------------------------------------------- network
-def forward(params, x, bf=False):
-    """Network output y (H, W) for input x (H, W); returns (y, cache)."""
+def wall_map(H, W, walls=None):
+    """The second input channel: per cell the sum over the walls it touches
+    of +1 (Dirichlet, type 1) or -1 (Neumann, type 2); walls = the types of
+    the x-, x+, y-, y+ segments (rows i = 0 and H-1, columns j = 0 and W-1),
+    None for no wall information (all zero)."""
+    m = np.zeros((H, W))
+    if walls is None:
+        return m
+    code = {0: 0.0, 1: 1.0, 2: -1.0}
+    m[0, :] += code[int(walls[0])]
+    m[H - 1, :] += code[int(walls[1])]
+    m[:, 0] += code[int(walls[2])]
+    m[:, W - 1] += code[int(walls[3])]
+    return m
+
+
+def forward(params, x, bf=False, walls=None):
+    """Network output y (H, W) for input x (H, W) on a domain whose walls
+    have the given types (wall_map); returns (y, cache)."""
     ws, bs = unpack(np.asarray(params, dtype=np.float64))
     R = bf16 if bf else (lambda a: a)
     wq = [R(w) if LAYERS[k][0] in ("conv", "up") else w for k, w in enumerate(ws)]
@@ -155,6 +173,7 @@ def forward(params, x, bf=False):
     x0 = np.asarray(x, dtype=np.float64).reshape(H, W, 1)
     if bf:
         x0 = x0.astype(np.float32).astype(np.float64)
+    x0 = np.concatenate([x0, wall_map(H, W, walls)[:, :, None]], axis=2)
     c = {"x0": x0, "wq": wq}
     c["a0"] = R(relu(conv3(x0, ws[0], bs[0])))
     e0 = R(relu(conv3(c["a0"], wq[1], bs[1])))

@@ -147,3 +147,33 @@ def test_all_neumann_problem_is_rejected(ctx):
     with pytest.raises(ca.CpgpuError):
         ca.Batch(ctx, [P])
     P.close()
+
+
+def test_batch_learned_members_reach_tolerance(ctx):
+    # members with a U-Net attached run flexible CG (an untrained network:
+    # some stall or pass the budget and are finished by the batched pass);
+    # the others run batched; every member ends at the tolerance
+    shapes = [(64, 64, [1, 2, 2, 1]), (128, 64, [2, 1, 1, 1]), (64, 192, [1, 1, 2, 2]),
+              (37, 53, [1, 2, 2, 1])]
+    ref = make(ctx, shapes, seed=11)
+    Br = ca.Batch(ctx, ref)
+    Br.solve("JPCG", max_iter=20000, tol=1e-12)
+    probs = make(ctx, shapes, seed=11)
+    net = ca.UNet(ctx, seed=1)
+    for P, (nx, ny, _) in zip(probs, shapes):
+        P.set_fcg(2, 2, 0.8)
+        if nx % 64 == 0 and ny % 64 == 0:
+            P.set_unet(net, ca.UNET_PRECOND)
+    B = ca.Batch(ctx, probs)
+    code, it, res, codes, lat = B.solve("FCG_UNET", max_iter=20000, tol=1e-8)
+    print("learned batch:", it, res, codes, lat)
+    assert code == 0 and (codes == 0).all() and (lat > 0).all()
+    for P, Q in zip(probs, ref):
+        assert P.residual() <= 1e-8 * 1.0001
+        assert rel_l2(P.get_phi(), Q.get_phi()) < 1e-5
+    B.close()
+    for P in probs:
+        P.set_unet(None, 0)
+    net.close()
+    for P in ref + probs:
+        P.close()

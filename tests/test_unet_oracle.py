@@ -68,7 +68,7 @@ def _random_params(seed):
 
 
 def test_parameter_count():
-    assert U.layout()[1] == 610769
+    assert U.layout()[1] == 610913  # the first layer reads 2 channels
 
 
 def test_backward_matches_finite_differences():
@@ -77,7 +77,8 @@ def test_backward_matches_finite_differences():
     p = _random_params(4)
     x = rng.standard_normal((H, W))
     dy = rng.standard_normal((H, W))
-    y, c = U.forward(p, x)
+    walls = (1, 2, 2, 1)  # the wall channel's weights get gradients too
+    y, c = U.forward(p, x, walls=walls)
     g = U.backward(c, dy)
     lay, _ = U.layout()
     # The network is piecewise smooth (about 10^6 ReLUs at this size): a
@@ -92,8 +93,8 @@ def test_backward_matches_finite_differences():
             an = g @ v
             errs = []
             for eps in (1e-7, 3e-8):
-                fp = np.sum(dy * U.forward(p + eps * v, x)[0])
-                fm = np.sum(dy * U.forward(p - eps * v, x)[0])
+                fp = np.sum(dy * U.forward(p + eps * v, x, walls=walls)[0])
+                fm = np.sum(dy * U.forward(p - eps * v, x, walls=walls)[0])
                 errs.append(abs((fp - fm) / (2 * eps) - an))
             assert min(errs) <= 1e-6 * max(1.0, abs(an)), (k, lo, errs, an)
     # a plausible mistake fails: the gradient of a transposed first conv
@@ -103,8 +104,8 @@ def test_backward_matches_finite_differences():
     v = np.zeros_like(p)
     v[wo:bo] = rng.standard_normal(bo - wo)
     eps = 1e-7
-    fd = (np.sum(dy * U.forward(p + eps * v, x)[0]) -
-          np.sum(dy * U.forward(p - eps * v, x)[0])) / (2 * eps)
+    fd = (np.sum(dy * U.forward(p + eps * v, x, walls=walls)[0]) -
+          np.sum(dy * U.forward(p - eps * v, x, walls=walls)[0])) / (2 * eps)
     assert abs(fd - gbad @ v) > 1e-3 * abs(fd)
 
 
@@ -177,3 +178,11 @@ def test_energy_loss_is_one_for_a_zero_network_and_zero_at_the_solution():
     p[bo] = 0.3 - h
     Lm = U.energy_loss(p, A, r, e)[0]
     assert (Lp - Lm) / (2 * h) == pytest.approx(dLdy1.sum(), rel=1e-6)
+
+
+def test_wall_map():
+    m = U.wall_map(4, 5, (1, 2, 2, 1))
+    # rows: x- Dirichlet (+1), x+ Neumann (-1); columns: y- Neumann, y+ Dirichlet
+    assert m[0, 2] == 1 and m[3, 2] == -1 and m[2, 0] == -1 and m[2, 4] == 1
+    assert m[0, 0] == 0 and m[0, 4] == 2 and m[3, 0] == -2 and m[3, 4] == 0
+    assert (m[1:3, 1:4] == 0).all() and (U.wall_map(4, 5) == 0).all()
