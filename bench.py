@@ -292,13 +292,21 @@ def main():
         Nm = m * m
         d_gbs = 4 * Nm * 8 / (dd * 1e-6) / 1e9
         u_gbs = 6 * Nm * 8 / (uu * 1e-6) / 1e9
+        dd32, uu32 = Pm.kernel_times(32, reps=20)
+        copy_gbs, copy_us = ctx.copy_bandwidth(1 << 30, reps=20)
         hbm = {"grid": "4096^2 FP64", "dir_variant": tune_m["fp64"][0],
                "k_cg_dir_us": round(dd, 2), "k_cg_dir_GBs": round(d_gbs, 1),
                "k_cg_dir_frac": round(d_gbs / HBM_PEAK_GBS, 4),
                "k_cg_upd_us": round(uu, 2), "k_cg_upd_GBs": round(u_gbs, 1),
-               "k_cg_upd_frac": round(u_gbs / HBM_PEAK_GBS, 4)}
+               "k_cg_upd_frac": round(u_gbs / HBM_PEAK_GBS, 4),
+               "fp32": {"dir_variant": tune_m["fp32"][0],
+                        "k_cg_dir_GBs": round(4 * Nm * 4 / (dd32 * 1e-6) / 1e9, 1),
+                        "k_cg_upd_GBs": round(6 * Nm * 4 / (uu32 * 1e-6) / 1e9, 1)},
+               "device_copy_1GiB_GBs": round(copy_gbs, 1),
+               "k_cg_dir_frac_of_copy": round(d_gbs / copy_gbs, 4),
+               "k_cg_upd_frac_of_copy": round(u_gbs / copy_gbs, 4)}
         Pm.close()
-        log(f"bench: 4096^2 fp64 dir {d_gbs:.0f} GB/s, upd {u_gbs:.0f} GB/s")
+        log(f"bench: 4096^2 fp64 dir {d_gbs:.0f} GB/s, upd {u_gbs:.0f} GB/s, copy {copy_gbs:.0f} GB/s")
 
     # ---- CPU baseline: the oracle's FP64 J-PCG on the same problem ----
     cpu = None

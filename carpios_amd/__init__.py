@@ -73,7 +73,7 @@ EXPORTS = [
     "cpgpu_batch_destroy", "cpgpu_batch_set_tiles", "cpgpu_batch_set_graphs",
     "cpgpu_batch_solve", "cpgpu_batch_autotune", "cpgpu_batch_draw",
     "cpgpu_ns_field_change", "cpgpu_unet_set_conv_rows", "cpgpu_unet_clear_samples",
-    "cpgpu_unet_set_walls",
+    "cpgpu_unet_set_walls", "cpgpu_copy_bandwidth",
 ]
 
 
@@ -121,6 +121,13 @@ class Context:
 
     def sync(self):
         self.check(lib().cpgpu_synchronize(self.h))
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=20):
+        """(GB/s, us per copy) of a plain device copy of nbytes."""
+        gbs, us = C.c_double(0), C.c_double(0)
+        self.check(lib().cpgpu_copy_bandwidth(self.h, C.c_size_t(nbytes), C.c_int(reps),
+                                              C.byref(gbs), C.byref(us)))
+        return gbs.value, us.value
 
     def timer_enable(self, family, on=True):
         self.check(lib().cpgpu_timer_enable(self.h, family, int(on)))

@@ -78,6 +78,55 @@ int cpgpu_synchronize(cpgpu_ctx* ctx) {
 
 void* cpgpu_stream(cpgpu_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+namespace {
+// y = x, 16 bytes per lane, grid-stride: the reference point of the HBM
+// figures (a plain device copy moves 2 bytes per byte copied)
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ x,
+                                                uint4* __restrict__ y, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    y[k] = x[k];
+}
+}  // namespace
+
+int cpgpu_copy_bandwidth(cpgpu_ctx* ctx, size_t bytes, int reps, double* gbs, double* us) {
+  if (!ctx || bytes < 16 || reps < 1 || !gbs) return CPGPU_EINVAL;
+  CPGPU_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const size_t n = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  CPGPU_HIP(ctx, hipMalloc(&a, n * 16));
+  if (hipMalloc(&b, n * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return cpgpu::set_error(ctx, CPGPU_ENOMEM, "copy probe buffers");
+  }
+  int rc = CPGPU_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float ms = 0.f;
+  const int G = 8192;
+  if (hipMemsetAsync(a, 0, n * 16, st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = cpgpu::set_error(ctx, CPGPU_EHIP, "copy probe setup");
+  } else {
+    hipLaunchKernelGGL(k_copy16, dim3(G), dim3(256), 0, st, (const uint4*)a, (uint4*)b, n);
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_copy16, dim3(G), dim3(256), 0, st, (const uint4*)a, (uint4*)b, n);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = cpgpu::set_error(ctx, CPGPU_EHIP, "copy probe timing");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (rc) return rc;
+  const double t = ms * 1e-3 / reps;
+  *gbs = 2.0 * (double)(n * 16) / t / 1e9;
+  if (us) *us = t * 1e6;
+  return CPGPU_OK;
+}
+
 int cpgpu_timer_enable(cpgpu_ctx* ctx, int family, int on) {
   if (!ctx || family < 0 || family >= 4) return CPGPU_EINVAL;
   ctx->timer_on[family] = on != 0;

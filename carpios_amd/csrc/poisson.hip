@@ -300,11 +300,15 @@ struct CGVecs {
 
 // Launch of the direction kernel of iteration `it`: variant 0 is the LDS
 // tile kernel, variants 1..4 the column march with H = 4, 8, 16, 32.
+// Variants 5 and 6 are the aligned march with H = 8, 16, paired with the
+// vectorised update kernel when ny is even.
+constexpr int kDirVariants = 7;
+
 template <typename T, bool PC>
 int dir_blocks(int variant, int nx, int ny, const Tiling& tl) {
   if (variant == 0) return tl.G;
-  static const int Hs[5] = {0, 4, 8, 16, 32};
-  const March m = make_march(nx, ny, Hs[variant]);
+  static const int Hs[kDirVariants] = {0, 4, 8, 16, 32, 8, 16};
+  const March m = variant >= 5 ? make_amarch(nx, ny, Hs[variant]) : make_march(nx, ny, Hs[variant]);
   return m.nunits < kMaxBlocks ? m.nunits : kMaxBlocks;
 }
 
@@ -325,13 +329,37 @@ void launch_dir(int variant, hipStream_t st, const Sten<T>& s, const T* r,
   CPGPU_LAUNCH(ev, (k_cg_dir_march<T, PC, HV>), dim3(G), block, 0, st, s, r,  \
                pold, pnew, q, kst, partB, partA, hist, hist_cap,              \
                make_march(s.nx, s.ny, HV), Gprev, partR64, Gref)
+#define CPGPU_DIR_AMARCH(HV)                                                  \
+  CPGPU_LAUNCH(ev, (k_cg_dir_amarch<T, PC, HV>), dim3(G), block, 0, st, s, r, \
+               pold, pnew, q, kst, partB, partA, hist, hist_cap,              \
+               make_amarch(s.nx, s.ny, HV), Gprev, partR64, Gref)
   switch (variant) {
     case 1: CPGPU_DIR_MARCH(4); break;
     case 2: CPGPU_DIR_MARCH(8); break;
     case 3: CPGPU_DIR_MARCH(16); break;
-    default: CPGPU_DIR_MARCH(32); break;
+    case 4: CPGPU_DIR_MARCH(32); break;
+    case 5: CPGPU_DIR_AMARCH(8); break;
+    default: CPGPU_DIR_AMARCH(16); break;
   }
 #undef CPGPU_DIR_MARCH
+#undef CPGPU_DIR_AMARCH
+}
+
+// The update kernel of a variant: the vectorised one for 5 and 6 when ny is
+// even, else k_cg_upd; both with tl.G blocks (so partial counts match).
+template <typename T, bool PC>
+void launch_upd(int variant, hipStream_t st, const Sten<T>& s, T* x, const T* p, T* r,
+                const T* q, KState* kst, const T* partA, T* partB, const Tiling& tl,
+                int norm_ref, int Gprev, EvPair ev = EvPair()) {
+  if (variant >= 5 && s.ny % 2 == 0) {
+    const int ntj = (s.ny + 127) / 128;
+    const int ntiles = ntj * ((s.nx + TI - 1) / TI);
+    CPGPU_LAUNCH(ev, (k_cg_upd2<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st, s, x, p, r, q,
+                 kst, partA, partB, ntiles, ntj, norm_ref, Gprev);
+  } else {
+    CPGPU_LAUNCH(ev, (k_cg_upd<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st, s, x, p, r, q,
+                 kst, partA, partB, tl, norm_ref, Gprev);
+  }
 }
 
 // One batch of `iters` CG iterations (iters even: the p buffers alternate
@@ -351,9 +379,8 @@ void launch_batch(cpgpu_poisson* P, hipStream_t st, const Sten<T>& s,
                       v.partB, v.partA, dhist, hist_cap, tl, tl.G, Gd, pr,
                       tl.G, timer_events(timed ? P->ctx : nullptr, 0));
     const EvPair eu = timer_events(timed ? P->ctx : nullptr, 1);
-    CPGPU_LAUNCH(eu, (k_cg_upd<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st, s,
-                 v.x, pnew, v.r, v.q, P->kst.p, v.partA, v.partB, tl,
-                 norm_ref, Gd);
+    launch_upd<T, PC>(variant, st, s, v.x, pnew, v.r, v.q, P->kst.p, v.partA, v.partB, tl,
+                      norm_ref, Gd, eu);
   }
   if constexpr (sizeof(T) == 4) {
     if (reliable) {
@@ -630,9 +657,8 @@ int time_kernels(cpgpu_poisson* P, const Sten<T>& s, const CGVecs<T>& v,
                      v.x, v.b, v.r, v.partB, tl, 0);
   launch_dir<T, PC>(variant, st, s, v.r, v.pa, v.pb, v.q, P->kst.p, v.partB,
                     v.partA, nullptr, 0, tl, tl.G, Gd);
-  hipLaunchKernelGGL((k_cg_upd<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st, s,
-                     v.x, v.pb, v.r, v.q, P->kst.p, v.partA, v.partB, tl, 0,
-                     Gd);
+  launch_upd<T, PC>(variant, st, s, v.x, v.pb, v.r, v.q, P->kst.p, v.partA, v.partB, tl, 0,
+                    Gd);
   CPGPU_HIP(ctx, hipGetLastError());
   CPGPU_HIP(ctx, hipStreamSynchronize(st));
   float out[2] = {0.f, 0.f};
@@ -647,9 +673,8 @@ int time_kernels(cpgpu_poisson* P, const Sten<T>& s, const CGVecs<T>& v,
         launch_dir<T, PC>(variant, cs, s, v.r, v.pb, v.pa, v.q, P->kst.p,
                           v.partB, v.partA, nullptr, 0, tl, tl.G, Gd);
       else
-        hipLaunchKernelGGL((k_cg_upd<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, cs,
-                           s, v.x, v.pa, v.r, v.q, P->kst.p, v.partA, v.partB,
-                           tl, 0, Gd);
+        launch_upd<T, PC>(variant, cs, s, v.x, v.pa, v.r, v.q, P->kst.p, v.partA, v.partB,
+                          tl, 0, Gd);
     }
     CPGPU_HIP(ctx, hipStreamEndCapture(cs, &graph));
     CPGPU_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -684,12 +709,13 @@ int tune_dir(cpgpu_poisson* P, const Sten<T>& s, CGVecs<T> v, int reps,
   const Tiling tl = P->tl;
   float bt = 1e30f, up = 0.f;
   int bv = 3;
-  for (int variant = 0; variant <= 4; ++variant) {
+  // the fastest (direction + update) pair: variants 5 and 6 change both
+  for (int variant = 0; variant < kDirVariants; ++variant) {
     float ud = 0.f, uu = 0.f;
     int rc = time_kernels<T, PC>(P, s, v, variant, reps, &ud, &uu);
     if (rc) return rc;
-    if (ud < bt) {
-      bt = ud;
+    if (ud + uu < bt) {
+      bt = ud + uu;
       bv = variant;
       up = uu;
     }
@@ -981,7 +1007,8 @@ int cpgpu_poisson_set_mixed(cpgpu_poisson* P, double inner_tol, int max_outer) {
 }
 
 int cpgpu_poisson_set_variant(cpgpu_poisson* P, int v64, int v32) {
-  if (!P || v64 < 0 || v64 > 4 || v32 < 0 || v32 > 4) return CPGPU_EINVAL;
+  if (!P || v64 < 0 || v64 >= kDirVariants || v32 < 0 || v32 >= kDirVariants)
+    return CPGPU_EINVAL;
   P->dir_variant64 = v64;
   P->dir_variant32 = v32;
   return CPGPU_OK;

@@ -410,6 +410,183 @@ __global__ __launch_bounds__(256) void k_cg_dir_march(
 }
 
 // ---------------------------------------------------------------------
+// Aligned column march: a wavefront owns 64 aligned output columns [c0,
+// c0+64) (c0 = 64 strip) and H rows, so every row load and store of the main
+// columns is one aligned segment (512 bytes in FP64) and whole cache lines
+// are written.  The halo columns c0-1 and c0+64 come from extra loads by
+// lanes 0 and 63 (lines the neighbouring strips load as well).
+inline March make_amarch(int nx, int ny, int H) {
+  March m;
+  m.nstrip = (ny + 63) / 64;
+  m.nchunk = (nx + H - 1) / H;
+  const int items = m.nstrip * m.nchunk;
+  m.nunits = (items + 3) / 4;
+  return m;
+}
+
+template <typename T, bool PC, int H>
+__global__ __launch_bounds__(256) void k_cg_dir_amarch(
+    Sten<T> s, const T* __restrict__ r, const T* __restrict__ pold,
+    T* __restrict__ pnew, T* __restrict__ q, KState* st,
+    const T* __restrict__ partB, T* __restrict__ partA, double* hist,
+    int hist_cap, March mr, int Gprev, const double* __restrict__ partR64,
+    int Gref) {
+  if (st->done) return;
+  __shared__ T lds[16];
+  const int nx = s.nx, ny = s.ny;
+  const int lane = threadIdx.x;  // blockDim = (64, 4)
+  const bool first0 = st->it_dir == 1;
+  T pn[H + 2], pv[H + 2], hn[H], hv[H];
+  auto load_unit = [&](int u) {
+    const int item = u * 4 + threadIdx.y;
+    const int strip = item / mr.nchunk;
+    const int chunk = item - strip * mr.nchunk;
+    const int r0 = chunk * H, c0 = strip * 64;
+    const int j = c0 + lane;
+    const bool jin = strip < mr.nstrip && j < ny;
+#pragma unroll
+    for (int h = 0; h < H + 2; ++h) {
+      const int i = r0 - 1 + h;
+      const bool ok = jin && i >= 0 && i < nx;
+      const size_t k = ok ? (size_t)i * ny + j : 0;
+      pn[h] = ok ? r[k] : T(0);
+      pv[h] = (ok && !first0) ? pold[k] : T(0);
+    }
+    const int jh = lane == 0 ? c0 - 1 : (lane == 63 ? c0 + 64 : -1);
+    const bool hin = strip < mr.nstrip && jh >= 0 && jh < ny;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const int i = r0 + h;
+      const bool ok = hin && i < nx;
+      const size_t k = ok ? (size_t)i * ny + jh : 0;
+      hn[h] = ok ? r[k] : T(0);
+      hv[h] = (ok && !first0) ? pold[k] : T(0);
+    }
+  };
+  if (blockIdx.x < mr.nunits) load_unit(blockIdx.x);
+  T beta;
+  bool first;
+  if (!dir_prologue<T>(st, partB, Gprev, partR64, Gref, hist, hist_cap, lds,
+                       &beta, &first))
+    return;
+
+  T acc_pq = T(0);
+  for (int u = blockIdx.x; u < mr.nunits; u += gridDim.x) {
+    if (u != (int)blockIdx.x) load_unit(u);
+    const int item = u * 4 + threadIdx.y;
+    const int strip = item / mr.nchunk;
+    if (strip >= mr.nstrip) continue;  // tail of the last unit
+    const int chunk = item - strip * mr.nchunk;
+    const int r0 = chunk * H, c0 = strip * 64;
+    const int j = c0 + lane;
+    const bool jin = j < ny;
+#pragma unroll
+    for (int h = 0; h < H + 2; ++h) {
+      const int i = r0 - 1 + h;
+      if (jin && i >= 0 && i < nx) {
+        const T z = PC ? pn[h] * (T(1) / diag_at(s, i, j)) : pn[h];
+        pn[h] = first ? z : z + beta * pv[h];
+      }
+    }
+    const int jh = lane == 0 ? c0 - 1 : (lane == 63 ? c0 + 64 : -1);
+    const bool hin = jh >= 0 && jh < ny;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const int i = r0 + h;
+      if (hin && i < nx) {
+        const T z = PC ? hn[h] * (T(1) / diag_at(s, i, jh)) : hn[h];
+        hn[h] = first ? z : z + beta * hv[h];
+      }
+    }
+    const T kyl = jin && j > 0 ? -s.KY[j] : T(0);
+    const T kyr = jin && j < ny - 1 ? -s.KY[j + 1] : T(0);
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const int i = r0 + h;
+      const T pc = pn[h + 1];
+      T left = __shfl_up(pc, 1, 64);
+      T right = __shfl_down(pc, 1, 64);
+      if (lane == 0) left = hn[h];
+      if (lane == 63) right = hn[h];
+      if (jin && i < nx) {
+        const T dg = diag_at(s, i, j);
+        T acc = T(0);
+        if (i > 0) acc = acc + pn[h] * (-s.KX[i]);
+        if (j > 0) acc = acc + left * kyl;
+        acc = acc + pc * dg;
+        if (j < ny - 1) acc = acc + right * kyr;
+        if (i < nx - 1) acc = acc + pn[h + 2] * (-s.KX[i + 1]);
+        const size_t k = (size_t)i * ny + j;
+        pnew[k] = pc;
+        q[k] = acc;
+        acc_pq += pc * acc;
+      }
+    }
+  }
+  T v[1] = {acc_pq};
+  store_partials<T, 1>(v, partA, gridDim.x, blockIdx.x, lds);
+}
+
+// Update kernel with two consecutive cells per lane (16-byte FP64 / 8-byte
+// FP32 accesses) over TI x 128 tiles; same block count and partial slots as
+// k_cg_upd.  Needs ny even (rows start at vector boundaries).
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  typedef double2 V;
+};
+template <>
+struct Vec2<float> {
+  typedef float2 V;
+};
+
+template <typename T, bool PC>
+__global__ __launch_bounds__(256) void k_cg_upd2(
+    Sten<T> s, T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r,
+    const T* __restrict__ q, KState* st, const T* __restrict__ partA,
+    T* __restrict__ partB, int ntiles, int ntj, int norm_ref, int Gprev) {
+  typedef typename Vec2<T>::V V;
+  if (st->done) return;
+  const int it = st->it_upd;
+  __shared__ T lds[16];
+  T pq[1];
+  reduce_partials<T, 1>(partA, Gprev, pq, lds);
+  const T alpha = (T)st->rho[it & 1] / pq[0];
+  const size_t nlast = (size_t)s.nx * s.ny - 1;
+  T v[3] = {T(0), T(0), T(0)};
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int ti = t / ntj;
+    const int i0 = ti * TI, j = (t - ti * ntj) * 128 + 2 * threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i >= s.nx || j >= s.ny) continue;
+      const size_t idx = (size_t)i * s.ny + j;
+      V xv = *reinterpret_cast<const V*>(x + idx);
+      const V pv = *reinterpret_cast<const V*>(p + idx);
+      V rv = *reinterpret_cast<const V*>(r + idx);
+      const V qv = *reinterpret_cast<const V*>(q + idx);
+      xv.x = xv.x + alpha * pv.x;
+      xv.y = xv.y + alpha * pv.y;
+      rv.x = rv.x - alpha * qv.x;
+      rv.y = rv.y - alpha * qv.y;
+      *reinterpret_cast<V*>(x + idx) = xv;
+      *reinterpret_cast<V*>(r + idx) = rv;
+      const T z0 = PC ? rv.x * (T(1) / diag_at(s, i, j)) : rv.x;
+      const T z1 = PC ? rv.y * (T(1) / diag_at(s, i, j + 1)) : rv.y;
+      v[SLOT_RZ] += rv.x * z0;
+      v[SLOT_RZ] += rv.y * z1;
+      v[SLOT_RR] += rv.x * rv.x;
+      if (!(norm_ref && idx + 1 == nlast)) v[SLOT_RR] += rv.y * rv.y;
+    }
+  }
+  store_partials<T, 3>(v, partB, gridDim.x, blockIdx.x, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0)
+    st->it_dir = it + 1;
+}
+
+// ---------------------------------------------------------------------
 // FP64 residual replacement of the mixed-precision iteration (no reference
 // counterpart).  Every refresh_every FP32 iterations the FP32 increment
 // x32 is folded into the FP64 solution and the recurrence residual is

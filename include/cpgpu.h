@@ -92,6 +92,11 @@ const char *cpgpu_last_error(const cpgpu_ctx *ctx);
 int cpgpu_synchronize(cpgpu_ctx *ctx);
 /* The context's HIP stream as an opaque pointer (hipStream_t). */
 void *cpgpu_stream(cpgpu_ctx *ctx);
+/* Measured device copy bandwidth (no reference counterpart): reps copies of
+ * `bytes` between two fresh buffers by a 16-byte-per-lane kernel, timed with
+ * events; *gbs = 2 * bytes / time (read + write), *us = time per copy. */
+int cpgpu_copy_bandwidth(cpgpu_ctx *ctx, size_t bytes, int reps, double *gbs,
+                         double *us);
 
 /* ---- Poisson problem (device resident) ----
  * Replaces the assembly + solve of structure::Solver_<2>::solve()
@@ -136,7 +141,9 @@ int cpgpu_poisson_solve(cpgpu_poisson *P, int method, int *max_iter,
 int cpgpu_poisson_set_mixed(cpgpu_poisson *P, double inner_tol, int max_outer);
 
 /* Direction-kernel variant of the Krylov iteration per precision: 0 LDS
- * tile, 1..4 column march with 4, 8, 16, 32 rows per wavefront. */
+ * tile, 1..4 column march with 4, 8, 16, 32 rows per wavefront, 5..6 the
+ * aligned column march (64 aligned columns per wavefront) with 8, 16 rows,
+ * paired with the two-cells-per-lane update kernel when ny is even. */
 int cpgpu_poisson_set_variant(cpgpu_poisson *P, int v64, int v32);
 /* Autotuner: times reps launches of every variant on this problem (FP64 and
  * FP32), keeps the fastest and reports it with its time per launch (us).
