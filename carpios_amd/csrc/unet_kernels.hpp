@@ -263,9 +263,22 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
   constexpr int TP = 18;       // tile side with halo
   constexpr int CPP = KCH / 8;  // 16-byte chunks per pixel
   typedef Frag<KC> F;
+  // one K = 32 step per tap and at most two column blocks: all nine taps'
+  // weight fragments fit in registers and are loaded before the staging
+  constexpr bool kPre = KCH == 32 && NB <= 2;
   __shared__ __attribute__((aligned(16))) bf16 sx[TP * TP * KCH];
   const int tj0 = blockIdx.x * 16, ti0 = blockIdx.y * 16;
   const int H = x.H, W = x.W;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int px = lane & 15, kq = F::KPL * (lane >> 4);
+  typename F::T wpre[kPre ? 9 : 1][NB];
+  if constexpr (kPre) {
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        wpre[tap][b] = F::load(w + ((size_t)(tap * NO + b * 16 + px) * KCH + kq));
+  }
   for (int e = threadIdx.x; e < TP * TP * CPP; e += 256) {
     const int pix = e / CPP, ch = e - pix * CPP;
     const int r = pix / TP, c = pix - r * TP;
@@ -276,15 +289,27 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
     *reinterpret_cast<uint4*>(sx + pix * KCH + ch * 8) = v;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int px = lane & 15, kq = F::KPL * (lane >> 4);
   f32x4_t acc[4][NB];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  if constexpr (kPre) {
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = FWD ? tap / 3 - 1 : 1 - tap / 3;
+      const int dx = FWD ? tap % 3 - 1 : 1 - tap % 3;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int r = wave * 4 + a;
+        const typename F::T af = F::load(sx + ((r + 1 + dy) * TP + px + 1 + dx) * KCH + kq);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(wpre[tap][b], af, acc[a][b]);
+      }
+    }
+  }
 #pragma unroll 1
-  for (int tap = 0; tap < 9; ++tap) {
+  for (int tap = 0; tap < (kPre ? 0 : 9); ++tap) {
     const int dy = FWD ? tap / 3 - 1 : 1 - tap / 3;
     const int dx = FWD ? tap % 3 - 1 : 1 - tap % 3;
 #pragma unroll

@@ -5,7 +5,9 @@ either
   --mode times: replays hipGraphs of back-to-back launches of the direction
                 and update kernels (cpgpu_poisson_kernel_times), or
   --mode solve: runs one solve (default MIXED) with directly launched
-                batches (no hipGraph), the launch pattern of a real solve;
+                batches (no hipGraph), the launch pattern of a real solve, or
+  --mode unet:  runs --reps forward passes of the U-Net (seed 1) on a
+                --size^2 input (the network's kernels alone);
 At --size 4096 with a loose --tol, the solve's k_merge_x launches
 (x64 += x32, x32 = 0: 12 bytes read and 12 written per cell, over vectors
 far larger than the caches) calibrate hardware byte counters.
@@ -20,7 +22,7 @@ import sys
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--mode", choices=("times", "solve"), default="times")
+    ap.add_argument("--mode", choices=("times", "solve", "unet"), default="times")
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--method", default="MIXED")
@@ -33,6 +35,20 @@ def main(argv=None):
 
     n = a.size
     ctx = ca.Context(0)
+    if a.mode == "unet":
+        import numpy as np
+        net = ca.UNet(ctx, seed=1)
+        x = np.random.default_rng(0).standard_normal((n, n)).astype(np.float32)
+        ctx.timer_enable(2)
+        for _ in range(a.reps):
+            net.forward(x)
+        k, ms = ctx.timer_read(2)
+        net.close()
+        ctx.close()
+        print(json.dumps({"size": n, "mode": "unet", "reps": a.reps,
+                          "conv16_launches": k, "conv16_us": round(ms * 1e3 / max(k, 1), 3)}),
+              flush=True)
+        return 0
     P = ca.Poisson(ctx, ca.uniform_grid_desc(n, n, 1, [ca.BC_NEUMANN] * 4))
     P.set_source_blobs(seed=0, nblobs=64, wmin=0.01, wmax=0.1)
     tune = None
