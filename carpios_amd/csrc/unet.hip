@@ -77,6 +77,8 @@ __device__ __forceinline__ float wall_code(int i, int j, int H, int W, int walls
 
 // first conv 2 -> 16 (FP32 input and wall code, direct):
 // a[m][co] = relu(b + sum_tap W[tap][co][0] x + W[tap][co][1] wall)
+// (branch-free: variants that skip the wall products away from the walls
+// measured slower)
 __global__ __launch_bounds__(256) void k_first_fwd(const float* __restrict__ x,
                                                    int H, int W, int walls, View a,
                                                    const float* __restrict__ w,
@@ -560,7 +562,9 @@ inline dim3 tile_grid(int M, int ncols) {
 }
 
 template <int CIN, int COUT>
-void conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu) {
+// With pout given, the 4 x 4 pooling of y into pout is fused where the
+// kernel supports it; returns whether it was.
+bool conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{}) {
   constexpr int NB = COUT / 16 < 4 ? COUT / 16 : 4;
   const int M = x.H * x.W;
   const bf16* w = U->wbf.p + U->wbf_off[l];
@@ -571,26 +575,31 @@ void conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu) {
   if (M < kSmallM) {
     hipLaunchKernelGGL((k_conv3_fwd<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, COUT)),
                        dim3(256), 0, U->ctx->stream, x, y, w, b, relu);
-    return;
+    return false;
   }
   if constexpr (CIN <= 64) {
     if (tiled<CIN>(x.H, x.W) && y.cs % 4 == 0 && y.co % 4 == 0) {  // 8-byte stores
       if constexpr (CIN == 16) {
-        if (x.H % 32 == 0 && U->conv_rows == 8)
+        const bool fuse = pout.p && pout.cs % 4 == 0 && pout.co % 4 == 0 &&
+                          pout.H * 4 == x.H && pout.W * 4 == x.W;
+        if (!fuse && x.H % 32 == 0 && U->conv_rows == 8)
           CPGPU_LAUNCH(ev, (k_conv3_tile16<COUT, true, 8>), dim3(x.W / 16, x.H / 32),
-                       dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu);
+                       dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu, View{});
         else
           CPGPU_LAUNCH(ev, (k_conv3_tile16<COUT, true, 4>), dim3(x.W / 16, x.H / 16),
-                       dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu);
+                       dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu,
+                       fuse ? pout : View{});
+        return fuse;
       } else {
         CPGPU_LAUNCH(ev, (k_conv3_tile<CIN, COUT, true>), dim3(x.W / 16, x.H / 16),
                      dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu);
       }
-      return;
+      return false;
     }
   }
   CPGPU_LAUNCH(ev, (k_conv3_fwd<CIN, COUT, NB, 2>), (tile_grid<NB, 2>(M, COUT)),
                dim3(256), 0, U->ctx->stream, x, y, w, b, relu);
+  return false;
 }
 
 template <int CIN, int COUT>
@@ -612,13 +621,13 @@ int forward(cpgpu_unet* U) {
   hipLaunchKernelGGL(k_first_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
                      U->x0.p, U->H, U->W, U->walls, U->v(U->a0),
                      U->params.p + U->woff[0], U->params.p + U->boff[0]);
-  conv_fwd<16, 16>(U, U->v(U->a0), U->v(U->C0, 16), 1, 1);
+  const bool pooled0 = conv_fwd<16, 16>(U, U->v(U->a0), U->v(U->C0, 16), 1, 1, U->v(U->p0));
   auto pool = [&](View in, View out, int C) {
     const int n = out.H * out.W * C;
     hipLaunchKernelGGL(k_pool_fwd, dim3((n + 255) / 256), dim3(256), 0, st, in,
                        out, C);
   };
-  pool(U->v(U->C0, 16), U->v(U->p0), 16);
+  if (!pooled0) pool(U->v(U->C0, 16), U->v(U->p0), 16);
   conv_fwd<16, 32>(U, U->v(U->p0), U->v(U->a1), 2, 1);
   conv_fwd<32, 32>(U, U->v(U->a1), U->v(U->C1, 32), 3, 1);
   pool(U->v(U->C1, 32), U->v(U->p1), 32);
@@ -678,10 +687,12 @@ int conv_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, int in_co,
         if constexpr (COUT == 16) {
           if (in.H % 32 == 0 && U->conv_rows == 8)
             hipLaunchKernelGGL((k_conv3_tile16<CIN, false, 8>), dim3(in.W / 16, in.H / 32),
-                               dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0);
+                               dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0,
+                               View{});
           else
             hipLaunchKernelGGL((k_conv3_tile16<CIN, false, 4>), dim3(in.W / 16, in.H / 16),
-                               dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0);
+                               dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0,
+                               View{});
         }
         else
           hipLaunchKernelGGL((k_conv3_tile<COUT, CIN, false>), dim3(in.W / 16, in.H / 16),

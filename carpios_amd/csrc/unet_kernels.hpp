@@ -361,11 +361,17 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
 // fifth pair's second half is zero), and all weight fragments are loaded
 // before the tile is staged, so their latency overlaps the staging.  A block
 // covers 4*ROWS x 16 pixels (wave w: rows ROWS*w .. ROWS*w + ROWS - 1).
+//
+// With pout.p set (forward, ROWS = 4: wave w's rows are one 4-row pooling
+// band) the epilogue also writes the 4 x 4 average pooling of the BF16
+// output into pout: rows summed in the lane, the four columns of a window by
+// lane shuffles, then / 16 and rounded (the sum of 16 non-negative BF16
+// values is exact in FP32, as in k_pool_fwd).
 template <int NO, bool FWD, int ROWS>
 __global__ __launch_bounds__(256) void k_conv3_tile16(View x, View yb, GView yf,
                                                       const bf16* __restrict__ w,
                                                       const float* __restrict__ bias,
-                                                      int relu) {
+                                                      int relu, View pout) {
   constexpr int KCH = 16, NB = NO / 16, TP = 18, TR = 4 * ROWS + 2, CPP = 2;
   typedef Frag<32> F;
   __shared__ __attribute__((aligned(16))) bf16 sx[TR * TP * KCH];
@@ -413,6 +419,11 @@ __global__ __launch_bounds__(256) void k_conv3_tile16(View x, View yb, GView yf,
     }
   }
   const int cq = 4 * (lane >> 4);
+  float ps[NB][4];
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) ps[b][t] = 0.f;
 #pragma unroll
   for (int a = 0; a < ROWS; ++a) {
     const size_t m = (size_t)(ti0 + wave * ROWS + a) * W + tj0 + px;
@@ -420,21 +431,43 @@ __global__ __launch_bounds__(256) void k_conv3_tile16(View x, View yb, GView yf,
     for (int b = 0; b < NB; ++b) {
       const int n = b * 16 + cq;
       if (FWD) {
-        uint32_t h[2];
+        bf16 q[4];
 #pragma unroll
-        for (int t = 0; t < 4; t += 2) {
-          float v0 = acc[a][b][t] + bias[n + t];
-          float v1 = acc[a][b][t + 1] + bias[n + t + 1];
-          if (relu) {
-            v0 = v0 > 0.f ? v0 : 0.f;
-            v1 = v1 > 0.f ? v1 : 0.f;
-          }
-          h[t >> 1] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+        for (int t = 0; t < 4; ++t) {
+          float v = acc[a][b][t] + bias[n + t];
+          if (relu) v = v > 0.f ? v : 0.f;
+          q[t] = f2bf(v);
+          ps[b][t] += bf2f(q[t]);
         }
-        *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) = uint2{h[0], h[1]};
+        const uint2 h = uint2{(uint32_t)q[0] | ((uint32_t)q[1] << 16),
+                              (uint32_t)q[2] | ((uint32_t)q[3] << 16)};
+        *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) = h;
       } else {
         *reinterpret_cast<float4*>(yf.p + m * yf.cs + yf.co + n) =
             float4{acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
+      }
+    }
+  }
+  if constexpr (FWD && ROWS == 4) {
+    if (pout.p) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          float v = ps[b][t];
+          v += __shfl_xor(v, 1, 64);
+          v += __shfl_xor(v, 2, 64);
+          ps[b][t] = v;
+        }
+      if ((px & 3) == 0) {
+        const size_t mp = (size_t)(ti0 / 4 + wave) * pout.W + tj0 / 4 + px / 4;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const bf16 q0 = f2bf(ps[b][0] * (1.f / 16.f)), q1 = f2bf(ps[b][1] * (1.f / 16.f));
+          const bf16 q2 = f2bf(ps[b][2] * (1.f / 16.f)), q3 = f2bf(ps[b][3] * (1.f / 16.f));
+          *reinterpret_cast<uint2*>(pout.p + mp * pout.cs + pout.co + b * 16 + cq) =
+              uint2{(uint32_t)q0 | ((uint32_t)q1 << 16), (uint32_t)q2 | ((uint32_t)q3 << 16)};
+        }
       }
     }
   }
