@@ -72,7 +72,7 @@ EXPORTS = [
     "cpgpu_poisson_error_norms", "cpgpu_ns_error_norms", "cpgpu_batch_create",
     "cpgpu_batch_destroy", "cpgpu_batch_set_tiles", "cpgpu_batch_set_graphs",
     "cpgpu_batch_solve", "cpgpu_batch_autotune", "cpgpu_batch_draw",
-    "cpgpu_ns_field_change", "cpgpu_unet_set_conv_rows", "cpgpu_unet_clear_samples",
+    "cpgpu_ns_field_change", "cpgpu_unet_set_conv_shape", "cpgpu_unet_clear_samples",
     "cpgpu_unet_set_walls", "cpgpu_copy_bandwidth",
 ]
 
@@ -591,8 +591,8 @@ class UNet:
         arr = None if bc_type is None else (C.c_int * 4)(*[int(t) for t in bc_type])
         self.ctx.check(lib().cpgpu_unet_set_walls(self.h, arr))
 
-    def set_conv_rows(self, rows):
-        self.ctx.check(lib().cpgpu_unet_set_conv_rows(self.h, C.c_int(rows)))
+    def set_conv_shape(self, rows=4, persist=4):
+        self.ctx.check(lib().cpgpu_unet_set_conv_shape(self.h, C.c_int(rows), C.c_int(persist)))
 
     def forward(self, x):
         """Raw network output for an (H, W) array (no input scaling)."""

@@ -462,7 +462,9 @@ using namespace cpgpu::unet;
 
 struct cpgpu_unet {
   cpgpu_ctx* ctx = nullptr;
-  int conv_rows = 4;  // pixel rows per wave of the 16-channel tile convolutions (4 or 8)
+  int conv_rows = 4;     // pixel rows per wave of the 16-channel tile convolutions (4 or 8)
+  int conv_persist = 4;  // > 0: persistent 16-channel tiles, blocks per compute unit
+  int ncu = 256;         // compute units of the device
   int walls = 0;      // wall types of the current input (2 bits per segment x-, x+, y-, y+)
   size_t woff[kNL], boff[kNL], wbf_off[kNL];
   size_t nparams = 0, nwbf = 0;
@@ -582,6 +584,14 @@ bool conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{}
       if constexpr (CIN == 16) {
         const bool fuse = pout.p && pout.cs % 4 == 0 && pout.co % 4 == 0 &&
                           pout.H * 4 == x.H && pout.W * 4 == x.W;
+        if (U->conv_persist > 0) {
+          const int ntx = x.W / 16, ntiles = ntx * (x.H / 16);
+          const int G = std::min(ntiles, U->ncu * U->conv_persist);
+          CPGPU_LAUNCH(ev, (k_conv3_tile16p<COUT, true>), dim3(G), dim3(256), 0,
+                       U->ctx->stream, x, y, GView{}, w, b, relu, fuse ? pout : View{}, ntx,
+                       ntiles);
+          return fuse;
+        }
         if (!fuse && x.H % 32 == 0 && U->conv_rows == 8)
           CPGPU_LAUNCH(ev, (k_conv3_tile16<COUT, true, 8>), dim3(x.W / 16, x.H / 32),
                        dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu, View{});
@@ -685,7 +695,12 @@ int conv_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, int in_co,
       const GView gx = U->g(in, in_co);
       if (!done && tiled<COUT>(in.H, in.W) && gx.cs % 4 == 0 && gx.co % 4 == 0) {
         if constexpr (COUT == 16) {
-          if (in.H % 32 == 0 && U->conv_rows == 8)
+          if (U->conv_persist > 0) {
+            const int ntx = in.W / 16, ntiles = ntx * (in.H / 16);
+            const int G = std::min(ntiles, U->ncu * U->conv_persist);
+            hipLaunchKernelGGL((k_conv3_tile16p<CIN, false>), dim3(G), dim3(256), 0, st, gq,
+                               View{}, gx, wt, (const float*)nullptr, 0, View{}, ntx, ntiles);
+          } else if (in.H % 32 == 0 && U->conv_rows == 8)
             hipLaunchKernelGGL((k_conv3_tile16<CIN, false, 8>), dim3(in.W / 16, in.H / 32),
                                dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0,
                                View{});
@@ -917,6 +932,10 @@ int cpgpu_unet_create(cpgpu_ctx* ctx, unsigned long long seed,
   CPGPU_HIP(ctx, hipSetDevice(ctx->device));
   std::unique_ptr<cpgpu_unet> U(new cpgpu_unet());
   U->ctx = ctx;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) ==
+          hipSuccess && ncu > 0)
+    U->ncu = ncu;
   size_t off = 0, boff = 0;
   for (int l = 0; l < kNL; ++l) {
     const LayerDef& d = kLayers[l];
@@ -994,9 +1013,10 @@ int cpgpu_unet_set_walls(cpgpu_unet* U, const int* bc_type) {
   return CPGPU_OK;
 }
 
-int cpgpu_unet_set_conv_rows(cpgpu_unet* U, int rows) {
-  if (!U || (rows != 4 && rows != 8)) return CPGPU_EINVAL;
+int cpgpu_unet_set_conv_shape(cpgpu_unet* U, int rows, int persist) {
+  if (!U || (rows != 4 && rows != 8) || persist < 0 || persist > 16) return CPGPU_EINVAL;
   U->conv_rows = rows;
+  U->conv_persist = persist;
   return CPGPU_OK;
 }
 

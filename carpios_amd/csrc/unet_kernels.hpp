@@ -473,6 +473,137 @@ __global__ __launch_bounds__(256) void k_conv3_tile16(View x, View yb, GView yf,
   }
 }
 
+// Persistent form of k_conv3_tile16 (ROWS = 4): a block walks 16 x 16 tiles
+// t = blockIdx.x + k gridDim.x (tiles numbered along rows) and loads tile
+// t + gridDim.x into registers while it computes tile t from LDS, so the
+// staging latency overlaps the MFMAs; the weights are loaded once per block.
+template <int NO, bool FWD>
+__global__ __launch_bounds__(256) void k_conv3_tile16p(View x, View yb, GView yf,
+                                                       const bf16* __restrict__ w,
+                                                       const float* __restrict__ bias,
+                                                       int relu, View pout, int ntx,
+                                                       int ntiles) {
+  constexpr int KCH = 16, NB = NO / 16, TP = 18, CPP = 2;
+  constexpr int NE = TP * TP * CPP, NPT = (NE + 255) / 256;
+  typedef Frag<32> F;
+  __shared__ __attribute__((aligned(16))) bf16 sx[TP * TP * KCH];
+  const int H = x.H, W = x.W;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int px = lane & 15, half = lane >> 5, kc = 8 * ((lane >> 4) & 1);
+  typename F::T wf[5][NB];
+#pragma unroll
+  for (int pr = 0; pr < 5; ++pr) {
+    const int tap = 2 * pr + half;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      wf[pr][b] = tap < 9 ? F::load(w + ((size_t)(tap * NO + b * 16 + px) * KCH + kc))
+                          : F::zero();
+  }
+  uint4 pre[NPT];
+  auto fetch = [&](int t) {
+    const int ty = t / ntx;
+    const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int e = threadIdx.x + 256 * k;
+      pre[k] = uint4{0u, 0u, 0u, 0u};
+      if (e < NE) {
+        const int pix = e / CPP, ch = e - pix * CPP;
+        const int r = pix / TP, c = pix - r * TP;
+        const int gi = ti0 + r - 1, gj = tj0 + c - 1;
+        if (gi >= 0 && gi < H && gj >= 0 && gj < W)
+          pre[k] = *reinterpret_cast<const uint4*>(x.p + ((size_t)gi * W + gj) * x.cs + x.co +
+                                                   ch * 8);
+      }
+    }
+  };
+  if ((int)blockIdx.x < ntiles) fetch(blockIdx.x);
+  const int cq = 4 * (lane >> 4);
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int ty = t / ntx;
+    const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
+    __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int e = threadIdx.x + 256 * k;
+      if (e < NE) *reinterpret_cast<uint4*>(sx + e * 8) = pre[k];
+    }
+    __syncthreads();
+    if (t + (int)gridDim.x < ntiles) fetch(t + gridDim.x);
+    f32x4_t acc[4][NB];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int pr = 0; pr < 5; ++pr) {
+      const int tap = 2 * pr + half;
+      const int tp = tap < 9 ? tap : 4;
+      const int dy = FWD ? tp / 3 - 1 : 1 - tp / 3;
+      const int dx = FWD ? tp % 3 - 1 : 1 - tp % 3;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int r = wave * 4 + a;
+        typename F::T xf = F::load(sx + ((r + 1 + dy) * TP + px + 1 + dx) * KCH + kc);
+        if (tap >= 9) xf = F::zero();
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(wf[pr][b], xf, acc[a][b]);
+      }
+    }
+    float ps[NB][4];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) ps[b][q] = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const size_t m = (size_t)(ti0 + wave * 4 + a) * W + tj0 + px;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int n = b * 16 + cq;
+        if (FWD) {
+          bf16 q[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            float v = acc[a][b][u] + bias[n + u];
+            if (relu) v = v > 0.f ? v : 0.f;
+            q[u] = f2bf(v);
+            ps[b][u] += bf2f(q[u]);
+          }
+          *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) =
+              uint2{(uint32_t)q[0] | ((uint32_t)q[1] << 16), (uint32_t)q[2] | ((uint32_t)q[3] << 16)};
+        } else {
+          *reinterpret_cast<float4*>(yf.p + m * yf.cs + yf.co + n) =
+              float4{acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
+        }
+      }
+    }
+    if constexpr (FWD) {
+      if (pout.p) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            float v = ps[b][u];
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            ps[b][u] = v;
+          }
+        if ((px & 3) == 0) {
+          const size_t mp = (size_t)(ti0 / 4 + wave) * pout.W + tj0 / 4 + px / 4;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const bf16 q0 = f2bf(ps[b][0] * (1.f / 16.f)), q1 = f2bf(ps[b][1] * (1.f / 16.f));
+            const bf16 q2 = f2bf(ps[b][2] * (1.f / 16.f)), q3 = f2bf(ps[b][3] * (1.f / 16.f));
+            *reinterpret_cast<uint2*>(pout.p + mp * pout.cs + pout.co + b * 16 + cq) =
+                uint2{(uint32_t)q0 | ((uint32_t)q1 << 16), (uint32_t)q2 | ((uint32_t)q3 << 16)};
+          }
+        }
+      }
+    }
+  }
+}
+
 // up-convolution 4x4 stride 4 (transposed conv): out[4I+a][4J+b][co] =
 // bias[co] + sum_ci x[I][J][ci] Wu[ab][co][ci].  GEMM rows = coarse pixels,
 // columns = (ab, co).

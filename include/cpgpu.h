@@ -329,8 +329,11 @@ int cpgpu_unet_create(cpgpu_ctx *ctx, unsigned long long seed,
                       cpgpu_unet **out);
 int cpgpu_unet_destroy(cpgpu_unet *U);
 /* Launch shape of the 16-input-channel 3x3 convolutions: 4 or 8 pixel rows
- * per wavefront (blocks of 16 x 16 or 32 x 16 pixels; default 4). */
-int cpgpu_unet_set_conv_rows(cpgpu_unet *U, int rows);
+ * per wavefront (blocks of 16 x 16 or 32 x 16 pixels), and with persist > 0
+ * a persistent grid of `persist` blocks per compute unit that walks 16 x 16
+ * tiles, staging the next tile while it computes one (rows 4).  Default
+ * rows 4, persist 4. */
+int cpgpu_unet_set_conv_shape(cpgpu_unet *U, int rows, int persist);
 long long cpgpu_unet_num_params(const cpgpu_unet *U);
 /* Flat FP32 parameter vector (grads != 0: the last computed gradients);
  * per layer: weights [tap][cout][cin], then biases [cout]. */
