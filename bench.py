@@ -257,7 +257,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_conv3_tile16<16,true,4> (finest U-Net level, 3x3 forward 16 -> 16, BF16 MFMA)",
+                "kernel": "k_conv3_tile16p<16,true> (finest U-Net level, 3x3 forward 16 -> 16, BF16 MFMA, persistent)",
                 "bytes_per_launch": conv_bytes, "avg_launch_us": round(us, 3),
                 "launches": n_c,
                 "mfma_tflops": round(conv_flops / (us * 1e-6) / 1e12, 2),
