@@ -564,9 +564,13 @@ inline dim3 tile_grid(int M, int ncols) {
 }
 
 template <int CIN, int COUT>
-// With pout given, the 4 x 4 pooling of y into pout is fused where the
-// kernel supports it; returns whether it was.
-bool conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{}) {
+// Fused epilogues where the kernel supports them (the persistent 16-channel
+// convolution): with pout given, the 4 x 4 pooling of y into pout (bit 1 of
+// the result); with wout given, the 1 x 1 output layer into yout scaled by
+// *yscale, y itself stored only if store_act (bit 2).
+int conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{},
+             const float* wout = nullptr, float* yout = nullptr,
+             const float* yscale = nullptr, int store_act = 1) {
   constexpr int NB = COUT / 16 < 4 ? COUT / 16 : 4;
   const int M = x.H * x.W;
   const bf16* w = U->wbf.p + U->wbf_off[l];
@@ -577,7 +581,7 @@ bool conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{}
   if (M < kSmallM) {
     hipLaunchKernelGGL((k_conv3_fwd<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, COUT)),
                        dim3(256), 0, U->ctx->stream, x, y, w, b, relu);
-    return false;
+    return 0;
   }
   if constexpr (CIN <= 64) {
     if (tiled<CIN>(x.H, x.W) && y.cs % 4 == 0 && y.co % 4 == 0) {  // 8-byte stores
@@ -587,10 +591,11 @@ bool conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{}
         if (U->conv_persist > 0) {
           const int ntx = x.W / 16, ntiles = ntx * (x.H / 16);
           const int G = std::min(ntiles, U->ncu * U->conv_persist);
+          const bool out = COUT == 16 && wout && yout && yscale;
           CPGPU_LAUNCH(ev, (k_conv3_tile16p<COUT, true>), dim3(G), dim3(256), 0,
                        U->ctx->stream, x, y, GView{}, w, b, relu, fuse ? pout : View{}, ntx,
-                       ntiles);
-          return fuse;
+                       ntiles, out ? wout : nullptr, yout, yscale, out ? store_act : 1);
+          return (fuse ? 1 : 0) | (out ? 2 : 0);
         }
         if (!fuse && x.H % 32 == 0 && U->conv_rows == 8)
           CPGPU_LAUNCH(ev, (k_conv3_tile16<COUT, true, 8>), dim3(x.W / 16, x.H / 32),
@@ -599,17 +604,17 @@ bool conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{}
           CPGPU_LAUNCH(ev, (k_conv3_tile16<COUT, true, 4>), dim3(x.W / 16, x.H / 16),
                        dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu,
                        fuse ? pout : View{});
-        return fuse;
+        return fuse ? 1 : 0;
       } else {
         CPGPU_LAUNCH(ev, (k_conv3_tile<CIN, COUT, true>), dim3(x.W / 16, x.H / 16),
                      dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu);
       }
-      return false;
+      return 0;
     }
   }
   CPGPU_LAUNCH(ev, (k_conv3_fwd<CIN, COUT, NB, 2>), (tile_grid<NB, 2>(M, COUT)),
                dim3(256), 0, U->ctx->stream, x, y, w, b, relu);
-  return false;
+  return 0;
 }
 
 template <int CIN, int COUT>
@@ -625,13 +630,16 @@ void up_fwd(cpgpu_unet* U, View x, View y, int l) {
                        dim3(256), 0, U->ctx->stream, x, y, w, b);
 }
 
-int forward(cpgpu_unet* U) {
+// final_out (applying only): the scaled output s y is written there directly
+// when the last layers fuse (*out_done), U->y and the last activation are
+// then not formed.
+int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr) {
   hipStream_t st = U->ctx->stream;
   const int M = U->H * U->W;
   hipLaunchKernelGGL(k_first_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
                      U->x0.p, U->H, U->W, U->walls, U->v(U->a0),
                      U->params.p + U->woff[0], U->params.p + U->boff[0]);
-  const bool pooled0 = conv_fwd<16, 16>(U, U->v(U->a0), U->v(U->C0, 16), 1, 1, U->v(U->p0));
+  const bool pooled0 = conv_fwd<16, 16>(U, U->v(U->a0), U->v(U->C0, 16), 1, 1, U->v(U->p0)) & 1;
   auto pool = [&](View in, View out, int C) {
     const int n = out.H * out.W * C;
     hipLaunchKernelGGL(k_pool_fwd, dim3((n + 255) / 256), dim3(256), 0, st, in,
@@ -654,10 +662,19 @@ int forward(cpgpu_unet* U) {
   conv_fwd<32, 32>(U, U->v(U->d1a), U->v(U->d1), 13, 1);
   up_fwd<32, 16>(U, U->v(U->d1), U->v(U->C0, 0), 14);
   conv_fwd<32, 16>(U, U->v(U->C0), U->v(U->d0a), 15, 1);
-  conv_fwd<16, 16>(U, U->v(U->d0a), U->v(U->d0), 16, 1);
-  hipLaunchKernelGGL(k_out_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
-                     U->v(U->d0), U->params.p + U->woff[17],
-                     U->params.p + U->boff[17], U->y.p, M);
+  // applied only (final_out): the output layer and the output scaling run in
+  // the last convolution's epilogue, which then stores nothing else
+  const float* wout = U->params.p + U->woff[17];  // 16 weights, then the bias
+  const int fused = conv_fwd<16, 16>(U, U->v(U->d0a), U->v(U->d0), 16, 1, View{},
+                                     final_out ? wout : nullptr, final_out, U->scale.p,
+                                     final_out ? 0 : 1);
+  if (final_out && (fused & 2)) {
+    *out_done = true;
+  } else {
+    hipLaunchKernelGGL(k_out_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
+                       U->v(U->d0), U->params.p + U->woff[17],
+                       U->params.p + U->boff[17], U->y.p, M);
+  }
   CPGPU_HIP(U->ctx, hipGetLastError());
   return CPGPU_OK;
 }
@@ -699,7 +716,8 @@ int conv_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, int in_co,
             const int ntx = in.W / 16, ntiles = ntx * (in.H / 16);
             const int G = std::min(ntiles, U->ncu * U->conv_persist);
             hipLaunchKernelGGL((k_conv3_tile16p<CIN, false>), dim3(G), dim3(256), 0, st, gq,
-                               View{}, gx, wt, (const float*)nullptr, 0, View{}, ntx, ntiles);
+                               View{}, gx, wt, (const float*)nullptr, 0, View{}, ntx, ntiles,
+                               (const float*)nullptr, (float*)nullptr, (const float*)nullptr, 1);
           } else if (in.H % 32 == 0 && U->conv_rows == 8)
             hipLaunchKernelGGL((k_conv3_tile16<CIN, false, 8>), dim3(in.W / 16, in.H / 32),
                                dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0,
@@ -835,9 +853,10 @@ int apply_scaled(cpgpu_unet* U, const float* in, float* out, size_t N) {
                      U->scale.p);
   hipLaunchKernelGGL(k_scale_copy, dim3(1024), dim3(256), 0, st, in, U->x0.p,
                      U->scale.p, 1, 1.f, N);
-  int rc = forward(U);
+  bool done = false;
+  int rc = forward(U, out, &done);
   if (rc) return rc;
-  if (out)
+  if (out && !done)
     hipLaunchKernelGGL(k_scale_copy, dim3(1024), dim3(256), 0, st, U->y.p, out,
                        U->scale.p, 0, 1.f, N);
   CPGPU_HIP(U->ctx, hipGetLastError());

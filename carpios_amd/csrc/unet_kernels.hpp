@@ -477,12 +477,21 @@ __global__ __launch_bounds__(256) void k_conv3_tile16(View x, View yb, GView yf,
 // t = blockIdx.x + k gridDim.x (tiles numbered along rows) and loads tile
 // t + gridDim.x into registers while it computes tile t from LDS, so the
 // staging latency overlaps the MFMAs; the weights are loaded once per block.
+//
+// With wout set (forward, NO = 16) the epilogue also applies the network's
+// 1 x 1 output layer to the BF16 output: y = (wout[16] + sum_c wout[c] out_c)
+// * (*yscale), the four channel groups of a pixel summed by lane shuffles,
+// into yout; store_act = 0 then skips storing the layer's own output (no
+// later kernel reads it when the network is only applied).
 template <int NO, bool FWD>
 __global__ __launch_bounds__(256) void k_conv3_tile16p(View x, View yb, GView yf,
                                                        const bf16* __restrict__ w,
                                                        const float* __restrict__ bias,
                                                        int relu, View pout, int ntx,
-                                                       int ntiles) {
+                                                       int ntiles, const float* __restrict__ wout,
+                                                       float* __restrict__ yout,
+                                                       const float* __restrict__ yscale,
+                                                       int store_act) {
   constexpr int KCH = 16, NB = NO / 16, TP = 18, CPP = 2;
   constexpr int NE = TP * TP * CPP, NPT = (NE + 255) / 256;
   typedef Frag<32> F;
@@ -570,8 +579,17 @@ __global__ __launch_bounds__(256) void k_conv3_tile16p(View x, View yb, GView yf
             q[u] = f2bf(v);
             ps[b][u] += bf2f(q[u]);
           }
-          *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) =
-              uint2{(uint32_t)q[0] | ((uint32_t)q[1] << 16), (uint32_t)q[2] | ((uint32_t)q[3] << 16)};
+          if (store_act)
+            *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) =
+                uint2{(uint32_t)q[0] | ((uint32_t)q[1] << 16), (uint32_t)q[2] | ((uint32_t)q[3] << 16)};
+          if (NO == 16 && wout) {
+            float part = 0.f;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) part += wout[n + u] * bf2f(q[u]);
+            part += __shfl_xor(part, 16, 64);
+            part += __shfl_xor(part, 32, 64);
+            if (lane < 16) yout[m] = (wout[16] + part) * *yscale;
+          }
         } else {
           *reinterpret_cast<float4*>(yf.p + m * yf.cs + yf.co + n) =
               float4{acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};

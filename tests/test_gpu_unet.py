@@ -365,3 +365,32 @@ def test_learned_preconditioner_gains_over_jacobi_pcg(ctx):
     zero.close()
     net.close()
     ns.close()
+
+
+def test_applied_network_fuses_the_output_layer(ctx):
+    # Applying the network inside a solve (here the warm start phi = s net(b / s))
+    # runs the output layer and the scaling in the last convolution's
+    # epilogue; the plain forward of the same input gives the same output up
+    # to the scale s and FP32 summation order.
+    n = 256
+    og = O.StructGrid(n, n, 1)
+    rng = np.random.default_rng(21)
+    src = og.field()
+    og.interior(src)[:] = rng.standard_normal((n, n))
+    types = [1, 2, 2, 1]
+    P = ca.Poisson(ctx, ca.uniform_grid_desc(n, n, 1, types, O.make_bc(og, types)._keep, 1.0))
+    P.set_source(src)
+    net = ca.UNet(ctx, seed=1)
+    net.set_params(random_params(31).astype(np.float32))
+    P.set_unet(net, ca.UNET_WARM)
+    P.fill_phi(0.0)
+    P.solve("JPCG", max_iter=0, tol=1e-300, norm=ca.NORM_TRUE)  # the warm start only
+    warm = og.interior(P.get_phi())
+    x0 = net.tensor("x")[:, :, 0]          # the scaled input the device used
+    y = net.forward(x0.astype(np.float32))  # same walls (kept from the apply)
+    c = float(np.vdot(warm, y) / np.vdot(y, y))
+    print(f"  fused output vs plain forward: scale {c:.6g}, rel L2 {rel_l2(warm, c * y):.2e}")
+    assert c > 0 and rel_l2(warm, c * y) <= 1e-5
+    P.set_unet(None, 0)
+    net.close()
+    P.close()
