@@ -103,18 +103,32 @@ int cpgpu_copy_bandwidth(cpgpu_ctx* ctx, size_t bytes, int reps, double* gbs, do
   int rc = CPGPU_OK;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   float ms = 0.f;
-  const int G = 8192;
+  int ncu = 256;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+          hipSuccess ||
+      ncu < 1)
+    ncu = 256;
   if (hipMemsetAsync(a, 0, n * 16, st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
       hipEventCreate(&e1) != hipSuccess) {
     rc = cpgpu::set_error(ctx, CPGPU_EHIP, "copy probe setup");
   } else {
-    hipLaunchKernelGGL(k_copy16, dim3(G), dim3(256), 0, st, (const uint4*)a, (uint4*)b, n);
-    (void)hipEventRecord(e0, st);
-    for (int r = 0; r < reps; ++r)
+    // the fastest of four grid sizes (2 .. 32 blocks per CU): the grid size
+    // changes a streaming copy's rate by up to 25% on gfx950
+    static const int per_cu[4] = {2, 4, 8, 32};
+    ms = 1e30f;
+    for (int c = 0; c < 4 && rc == CPGPU_OK; ++c) {
+      const int G = ncu * per_cu[c];
+      float t = 0.f;
       hipLaunchKernelGGL(k_copy16, dim3(G), dim3(256), 0, st, (const uint4*)a, (uint4*)b, n);
-    (void)hipEventRecord(e1, st);
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-      rc = cpgpu::set_error(ctx, CPGPU_EHIP, "copy probe timing");
+      (void)hipEventRecord(e0, st);
+      for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy16, dim3(G), dim3(256), 0, st, (const uint4*)a, (uint4*)b, n);
+      (void)hipEventRecord(e1, st);
+      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+        rc = cpgpu::set_error(ctx, CPGPU_EHIP, "copy probe timing");
+      else if (t < ms)
+        ms = t;
+    }
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

@@ -393,6 +393,7 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
   std::vector<double> trace;
   trace.push_back(res);
   constexpr int kStallWindow = 40;
+  constexpr int kProbe = 24;  // first projection of a batched member's progress
   bool singular = true;  // all-Neumann: constants are the null space
   for (int sg = 0; sg < 4; ++sg)
     singular = singular && P->seg[sg].type == CPGPU_BC_NEUMANN;
@@ -441,6 +442,16 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
       // cycle without the network.
       fallback = true;
       break;
+    }
+    // Without a fallback of its own (a member of a batched solve, whose
+    // batched pass takes over from this iterate), hand over as soon as the
+    // mean reduction so far projects past twice the iteration budget.
+    if (!P->fcg_fallback && k >= kProbe && k % 8 == 0) {
+      const double got = std::log(best / trace[0]), need = std::log(tol / trace[0]);
+      if (!(got < 0.0) || (double)k * need / got > 2.0 * max_iter) {
+        fallback = true;
+        break;
+      }
     }
   }
   const int kf = k;

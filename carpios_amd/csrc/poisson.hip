@@ -300,16 +300,22 @@ struct CGVecs {
 
 // Launch of the direction kernel of iteration `it`: variant 0 is the LDS
 // tile kernel, variants 1..4 the column march with H = 4, 8, 16, 32.
-// Variants 5 and 6 are the aligned march with H = 8, 16, paired with the
-// vectorised update kernel when ny is even.
-constexpr int kDirVariants = 7;
+// Variants 5 and 6 are the aligned march with H = 8, 16, variants 7..9 the
+// streaming aligned march over chunks of 16, 32, 64 rows; 5..9 are paired
+// with the vectorised update kernel when ny is even.
+constexpr int kDirVariants = 10;
+constexpr int kRollRows = 4;  // rows per group of the streaming march
 
 template <typename T, bool PC>
 int dir_blocks(int variant, int nx, int ny, const Tiling& tl) {
   if (variant == 0) return tl.G;
-  static const int Hs[kDirVariants] = {0, 4, 8, 16, 32, 8, 16};
-  const March m = variant >= 5 ? make_amarch(nx, ny, Hs[variant]) : make_march(nx, ny, Hs[variant]);
-  return m.nunits < kMaxBlocks ? m.nunits : kMaxBlocks;
+  static const int Hs[kDirVariants] = {0, 4, 8, 16, 32, 8, 16, 16, 32, 64};
+  int nunits;
+  if (variant >= 7)
+    nunits = make_roll(nx, ny, Hs[variant]).nunits;
+  else
+    nunits = (variant >= 5 ? make_amarch(nx, ny, Hs[variant]) : make_march(nx, ny, Hs[variant])).nunits;
+  return nunits < kMaxBlocks ? nunits : kMaxBlocks;
 }
 
 template <typename T, bool PC>
@@ -333,19 +339,27 @@ void launch_dir(int variant, hipStream_t st, const Sten<T>& s, const T* r,
   CPGPU_LAUNCH(ev, (k_cg_dir_amarch<T, PC, HV>), dim3(G), block, 0, st, s, r, \
                pold, pnew, q, kst, partB, partA, hist, hist_cap,              \
                make_amarch(s.nx, s.ny, HV), Gprev, partR64, Gref)
+#define CPGPU_DIR_ROLL(HC)                                                          \
+  CPGPU_LAUNCH(ev, (k_cg_dir_roll<T, PC, kRollRows>), dim3(G), block, 0, st, s, r, \
+               pold, pnew, q, kst, partB, partA, hist, hist_cap,                    \
+               make_roll(s.nx, s.ny, HC), Gprev, partR64, Gref)
   switch (variant) {
     case 1: CPGPU_DIR_MARCH(4); break;
     case 2: CPGPU_DIR_MARCH(8); break;
     case 3: CPGPU_DIR_MARCH(16); break;
     case 4: CPGPU_DIR_MARCH(32); break;
     case 5: CPGPU_DIR_AMARCH(8); break;
-    default: CPGPU_DIR_AMARCH(16); break;
+    case 6: CPGPU_DIR_AMARCH(16); break;
+    case 7: CPGPU_DIR_ROLL(16); break;
+    case 8: CPGPU_DIR_ROLL(32); break;
+    default: CPGPU_DIR_ROLL(64); break;
   }
 #undef CPGPU_DIR_MARCH
 #undef CPGPU_DIR_AMARCH
+#undef CPGPU_DIR_ROLL
 }
 
-// The update kernel of a variant: the vectorised one for 5 and 6 when ny is
+// The update kernel of a variant: the vectorised one for 5..9 when ny is
 // even, else k_cg_upd; both with tl.G blocks (so partial counts match).
 template <typename T, bool PC>
 void launch_upd(int variant, hipStream_t st, const Sten<T>& s, T* x, const T* p, T* r,

@@ -527,6 +527,184 @@ __global__ __launch_bounds__(256) void k_cg_dir_amarch(
   store_partials<T, 1>(v, partA, gridDim.x, blockIdx.x, lds);
 }
 
+// Streaming aligned march.  A wavefront owns 64 aligned columns [c0, c0+64)
+// of a chunk of hc rows and walks down it in groups of R rows of p: the
+// loads of group g+1 (r and p_old of the 64 columns, and of the halo columns
+// c0-1 / c0+64 on lanes 0 / 63) are in flight while group g is formed and
+// its rows of q are computed, so a wave keeps memory busy instead of
+// alternating between a load burst and a compute phase.  Work items
+// (strip, chunk) are numbered strip-fastest, and a block's logical index is
+// remapped so that the blocks a launch hands to one XCD (round robin by
+// block index) take consecutive items: neighbouring strips and chunks then
+// run together on one XCD and the halo lines they share come from its L2.
+struct Roll {
+  int nstrip, nchunk, hc, nunits;
+};
+
+inline Roll make_roll(int nx, int ny, int hc) {
+  Roll m;
+  m.nstrip = (ny + 63) / 64;
+  m.hc = hc;
+  m.nchunk = (nx + hc - 1) / hc;
+  m.nunits = (m.nstrip * m.nchunk + 3) / 4;
+  return m;
+}
+
+template <typename T, int R>
+struct RollBuf {
+  T r[R], p[R], hr[R], hp[R];
+  T kx[R + 2];  // KX of rows pbase-1 .. pbase+R
+};
+
+template <typename T, bool PC, int R>
+__global__ __launch_bounds__(256) void k_cg_dir_roll(
+    Sten<T> s, const T* __restrict__ r, const T* __restrict__ pold,
+    T* __restrict__ pnew, T* __restrict__ q, KState* st,
+    const T* __restrict__ partB, T* __restrict__ partA, double* hist,
+    int hist_cap, Roll mr, int Gprev, const double* __restrict__ partR64,
+    int Gref) {
+  if (st->done) return;
+  __shared__ T lds[16];
+  const int nx = s.nx, ny = s.ny;
+  const int lane = threadIdx.x;  // blockDim = (64, 4)
+  const int G = gridDim.x;
+  const int lb = (G & 7) ? (int)blockIdx.x
+                         : (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
+  const int items = mr.nstrip * mr.nchunk;
+
+  // Every load of a group is issued unconditionally from a clamped, valid
+  // address (values outside the chunk or the grid are masked when used), so
+  // the loads of the next group stay in flight across the current group's
+  // arithmetic: a load under a branch, or one issued in the middle of the
+  // arithmetic, makes the compiler drain all outstanding loads first.
+  // Rows past iend (the chunk's last p row + 1) re-read row iend - 1, a
+  // cache hit, so the groups past the chunk cost no memory traffic.
+  auto load = [&](RollBuf<T, R>& b, int c0, int pbase, int iend) {
+    const int j = min(c0 + lane, ny - 1);
+    const int jh = lane == 0 ? max(c0 - 1, 0) : min(c0 + 64, ny - 1);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const size_t row = (size_t)min(max(pbase + k, 0), iend - 1) * ny;
+      b.r[k] = r[row + j];
+      b.p[k] = pold[row + j];
+      b.hr[k] = r[row + jh];
+      b.hp[k] = pold[row + jh];
+    }
+#pragma unroll
+    for (int k = 0; k < R + 2; ++k) b.kx[k] = s.KX[min(max(pbase - 1 + k, 0), nx)];
+  };
+
+  RollBuf<T, R> bufA, bufB;
+  int u = lb;
+  {
+    const int item = u * 4 + threadIdx.y;
+    if (u < mr.nunits && item < items) {
+      const int chunk = item / mr.nstrip, strip = item - chunk * mr.nstrip;
+      load(bufA, strip * 64, chunk * mr.hc - 1, min(chunk * mr.hc + mr.hc + 1, nx));
+    }
+  }
+  T beta;
+  bool first;
+  if (!dir_prologue<T>(st, partB, Gprev, partR64, Gref, hist, hist_cap, lds,
+                       &beta, &first))
+    return;
+
+  T acc_pq = T(0);
+  for (; u < mr.nunits; u += G) {
+    const int item = u * 4 + threadIdx.y;
+    if (item >= items) continue;  // tail of the last unit
+    const int chunk = item / mr.nstrip, strip = item - chunk * mr.nstrip;
+    const int r0 = chunk * mr.hc, c0 = strip * 64;
+    const int j = c0 + lane;
+    const int jc = min(j, ny - 1);
+    const bool jin = j < ny;
+    const bool hin = (lane == 0 && c0 > 0) || (lane == 63 && c0 + 64 < ny);
+    const int jh = lane == 0 ? max(c0 - 1, 0) : min(c0 + 64, ny - 1);
+    const T ky0 = s.KY[jc], ky1 = s.KY[jc + 1];
+    const T hy0 = s.KY[jh], hy1 = s.KY[jh + 1];
+    const T kyl = jin && j > 0 ? -ky0 : T(0);
+    const T kyr = jin && j < ny - 1 ? -ky1 : T(0);
+    const int iend = min(r0 + mr.hc + 1, nx);  // p rows r0-1 .. iend-1
+    const int oend = min(r0 + mr.hc, nx);      // output rows r0 .. oend-1
+    if (u != lb) load(bufA, c0, r0 - 1, iend);
+    // groups of R p rows covering r0-1 .. r0+hc, rounded up to an even
+    // count so that the two-step loop below has no partial pass
+    const int ngroups = ((mr.hc + 2 + 2 * R - 1) / (2 * R)) * 2;
+    // window: w[0], w[1] carry p rows gR-2, gR-1, w[2 .. R+1] the group;
+    // dw the matching diagonals; hw[0] the halo p of row gR-1, hw[1 .. R]
+    // the group's.
+    T w[R + 2], dw[R + 2], hw[R + 1];
+    w[0] = w[1] = dw[0] = dw[1] = hw[0] = T(0);
+    auto step = [&](RollBuf<T, R>& cur, RollBuf<T, R>& nxt, int g) {
+      const int pbase = r0 - 1 + g * R;  // p row of cur slot 0
+      load(nxt, c0, pbase + R, iend);  // past the last group: cache hits
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int i = pbase + k;
+        // diag_at's arithmetic order
+        T d = cur.kx[k + 1] + cur.kx[k + 2];
+        T dh = d;
+        d = d + ky0;
+        d = d + ky1;
+        dh = dh + hy0;
+        dh = dh + hy1;
+        if (i == 0) {
+          d = d + s.adj[0];
+          dh = dh + s.adj[0];
+        }
+        if (j == 0) d = d + s.adj[2];
+        if (jh == 0) dh = dh + s.adj[2];
+        if (j == ny - 1) d = d + s.adj[3];
+        if (jh == ny - 1) dh = dh + s.adj[3];
+        if (i == nx - 1) {
+          d = d + s.adj[1];
+          dh = dh + s.adj[1];
+        }
+        const bool iok = i >= 0 && i < iend;
+        const T z = PC ? cur.r[k] * (T(1) / d) : cur.r[k];
+        const T zh = PC ? cur.hr[k] * (T(1) / dh) : cur.hr[k];
+        const T pv = first ? z : z + beta * cur.p[k];
+        const T hv = first ? zh : zh + beta * cur.hp[k];
+        w[k + 2] = iok && jin ? pv : T(0);
+        dw[k + 2] = d;
+        hw[k + 1] = iok && hin ? hv : T(0);
+      }
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const int i = pbase - 1 + m;  // output row, centre p = w[m + 1]
+        const T pc = w[m + 1];
+        T left = __shfl_up(pc, 1, 64);
+        T right = __shfl_down(pc, 1, 64);
+        if (lane == 0) left = hw[m];
+        if (lane == 63) right = hw[m];
+        if (jin && i >= r0 && i < oend) {
+          T acc = T(0);
+          if (i > 0) acc = acc + w[m] * (-cur.kx[m]);
+          if (j > 0) acc = acc + left * kyl;
+          acc = acc + pc * dw[m + 1];
+          if (j < ny - 1) acc = acc + right * kyr;
+          if (i < nx - 1) acc = acc + w[m + 2] * (-cur.kx[m + 1]);
+          const size_t k = (size_t)i * ny + j;
+          pnew[k] = pc;
+          q[k] = acc;
+          acc_pq += pc * acc;
+        }
+      }
+      w[0] = w[R];
+      w[1] = w[R + 1];
+      dw[0] = dw[R];
+      dw[1] = dw[R + 1];
+      hw[0] = hw[R];
+    };
+    for (int g = 0; g < ngroups; g += 2) {
+      step(bufA, bufB, g);
+      step(bufB, bufA, g + 1);
+    }
+  }
+  T v[1] = {acc_pq};
+  store_partials<T, 1>(v, partA, gridDim.x, blockIdx.x, lds);
+}
+
 // Update kernel with two consecutive cells per lane (16-byte FP64 / 8-byte
 // FP32 accesses) over TI x 128 tiles; same block count and partial slots as
 // k_cg_upd.  Needs ny even (rows start at vector boundaries).

@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void k_conv3_tile16(View x, View yb, GView yf,
 // into yout; store_act = 0 then skips storing the layer's own output (no
 // later kernel reads it when the network is only applied).
 template <int NO, bool FWD>
-__global__ __launch_bounds__(256) void k_conv3_tile16p(View x, View yb, GView yf,
+__global__ __launch_bounds__(256, NO == 16 ? 4 : 1) void k_conv3_tile16p(View x, View yb, GView yf,
                                                        const bf16* __restrict__ w,
                                                        const float* __restrict__ bias,
                                                        int relu, View pout, int ntx,
@@ -508,22 +508,38 @@ __global__ __launch_bounds__(256) void k_conv3_tile16p(View x, View yb, GView yf
       wf[pr][b] = tap < 9 ? F::load(w + ((size_t)(tap * NO + b * 16 + px) * KCH + kc))
                           : F::zero();
   }
+  // Every global load inside the tile loop is the next tile's prefetch,
+  // issued unconditionally from a clamped, valid address (masked when
+  // staged): the epilogue's constants are loaded before the loop, so the
+  // prefetch stays in flight through the MFMAs and the epilogue (a load
+  // under a branch, or a later load that must be waited for, makes the
+  // compiler drain it early).
+  float bias_r[NB][4];  // forward only (the input gradient has no bias)
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) bias_r[b][u] = FWD ? bias[b * 16 + 4 * (lane >> 4) + u] : 0.f;
+  float wo[4] = {0.f, 0.f, 0.f, 0.f}, wo16 = 0.f, ysc = 0.f;
+  if (NO == 16 && FWD && wout) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) wo[u] = wout[4 * (lane >> 4) + u];
+    wo16 = wout[16];
+    ysc = *yscale;
+  }
   uint4 pre[NPT];
+  bool pok[NPT];
   auto fetch = [&](int t) {
     const int ty = t / ntx;
     const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
-      const int e = threadIdx.x + 256 * k;
-      pre[k] = uint4{0u, 0u, 0u, 0u};
-      if (e < NE) {
-        const int pix = e / CPP, ch = e - pix * CPP;
-        const int r = pix / TP, c = pix - r * TP;
-        const int gi = ti0 + r - 1, gj = tj0 + c - 1;
-        if (gi >= 0 && gi < H && gj >= 0 && gj < W)
-          pre[k] = *reinterpret_cast<const uint4*>(x.p + ((size_t)gi * W + gj) * x.cs + x.co +
-                                                   ch * 8);
-      }
+      const int e = min((int)threadIdx.x + 256 * k, NE - 1);
+      const int pix = e / CPP, ch = e - pix * CPP;
+      const int r = pix / TP, c = pix - r * TP;
+      const int gi = ti0 + r - 1, gj = tj0 + c - 1;
+      pok[k] = gi >= 0 && gi < H && gj >= 0 && gj < W;
+      const int ci = min(max(gi, 0), H - 1), cj = min(max(gj, 0), W - 1);
+      pre[k] = *reinterpret_cast<const uint4*>(x.p + ((size_t)ci * W + cj) * x.cs + x.co + ch * 8);
     }
   };
   if ((int)blockIdx.x < ntiles) fetch(blockIdx.x);
@@ -535,10 +551,12 @@ __global__ __launch_bounds__(256) void k_conv3_tile16p(View x, View yb, GView yf
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
       const int e = threadIdx.x + 256 * k;
-      if (e < NE) *reinterpret_cast<uint4*>(sx + e * 8) = pre[k];
+      if (e < NE)
+        *reinterpret_cast<uint4*>(sx + e * 8) = pok[k] ? pre[k] : uint4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
-    if (t + (int)gridDim.x < ntiles) fetch(t + gridDim.x);
+    // the last tile of the block re-reads its own tile (cache hits)
+    fetch(t + (int)gridDim.x < ntiles ? t + (int)gridDim.x : t);
     f32x4_t acc[4][NB];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -574,7 +592,7 @@ __global__ __launch_bounds__(256) void k_conv3_tile16p(View x, View yb, GView yf
           bf16 q[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            float v = acc[a][b][u] + bias[n + u];
+            float v = acc[a][b][u] + bias_r[b][u];
             if (relu) v = v > 0.f ? v : 0.f;
             q[u] = f2bf(v);
             ps[b][u] += bf2f(q[u]);
@@ -585,10 +603,10 @@ __global__ __launch_bounds__(256) void k_conv3_tile16p(View x, View yb, GView yf
           if (NO == 16 && wout) {
             float part = 0.f;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) part += wout[n + u] * bf2f(q[u]);
+            for (int u = 0; u < 4; ++u) part += wo[u] * bf2f(q[u]);
             part += __shfl_xor(part, 16, 64);
             part += __shfl_xor(part, 32, 64);
-            if (lane < 16) yout[m] = (wout[16] + part) * *yscale;
+            if (lane < 16) yout[m] = (wo16 + part) * ysc;
           }
         } else {
           *reinterpret_cast<float4*>(yf.p + m * yf.cs + yf.co + n) =

@@ -93,8 +93,9 @@ int cpgpu_synchronize(cpgpu_ctx *ctx);
 /* The context's HIP stream as an opaque pointer (hipStream_t). */
 void *cpgpu_stream(cpgpu_ctx *ctx);
 /* Measured device copy bandwidth (no reference counterpart): reps copies of
- * `bytes` between two fresh buffers by a 16-byte-per-lane kernel, timed with
- * events; *gbs = 2 * bytes / time (read + write), *us = time per copy. */
+ * `bytes` between two fresh buffers by a 16-byte-per-lane grid-stride
+ * kernel at 2, 4, 8 and 32 blocks per CU, timed with events; the fastest
+ * grid counts: *gbs = 2 * bytes / time (read + write), *us = time per copy. */
 int cpgpu_copy_bandwidth(cpgpu_ctx *ctx, size_t bytes, int reps, double *gbs,
                          double *us);
 
@@ -143,6 +144,8 @@ int cpgpu_poisson_set_mixed(cpgpu_poisson *P, double inner_tol, int max_outer);
 /* Direction-kernel variant of the Krylov iteration per precision: 0 LDS
  * tile, 1..4 column march with 4, 8, 16, 32 rows per wavefront, 5..6 the
  * aligned column march (64 aligned columns per wavefront) with 8, 16 rows,
+ * 7..9 the streaming aligned march (loads of the next 4 rows in flight while
+ * the current ones are computed) over chunks of 16, 32, 64 rows; 5..9 are
  * paired with the two-cells-per-lane update kernel when ny is even. */
 int cpgpu_poisson_set_variant(cpgpu_poisson *P, int v64, int v32);
 /* Autotuner: times reps launches of every variant on this problem (FP64 and
