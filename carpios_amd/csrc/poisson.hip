@@ -300,16 +300,17 @@ struct CGVecs {
 
 // Launch of the direction kernel of iteration `it`: variant 0 is the LDS
 // tile kernel, variants 1..4 the column march with H = 4, 8, 16, 32.
-// Variants 5 and 6 are the aligned march with H = 8, 16, variants 7..9 the
-// streaming aligned march over chunks of 16, 32, 64 rows; 5..9 are paired
-// with the vectorised update kernel when ny is even.
-constexpr int kDirVariants = 10;
+// Variants 5 and 6 are the aligned march with H = 8, 16, variants 7..10 the
+// streaming aligned march over chunks of 32 and 64 rows with the XCD-aware
+// unit order, 64 and 128 rows in plain order; 5..10 are paired with the
+// vectorised update kernel when ny is even.
+constexpr int kDirVariants = 11;
 constexpr int kRollRows = 4;  // rows per group of the streaming march
 
 template <typename T, bool PC>
 int dir_blocks(int variant, int nx, int ny, const Tiling& tl) {
   if (variant == 0) return tl.G;
-  static const int Hs[kDirVariants] = {0, 4, 8, 16, 32, 8, 16, 16, 32, 64};
+  static const int Hs[kDirVariants] = {0, 4, 8, 16, 32, 8, 16, 32, 64, 64, 128};
   int nunits;
   if (variant >= 7)
     nunits = make_roll(nx, ny, Hs[variant]).nunits;
@@ -339,10 +340,10 @@ void launch_dir(int variant, hipStream_t st, const Sten<T>& s, const T* r,
   CPGPU_LAUNCH(ev, (k_cg_dir_amarch<T, PC, HV>), dim3(G), block, 0, st, s, r, \
                pold, pnew, q, kst, partB, partA, hist, hist_cap,              \
                make_amarch(s.nx, s.ny, HV), Gprev, partR64, Gref)
-#define CPGPU_DIR_ROLL(HC)                                                          \
+#define CPGPU_DIR_ROLL(HC, X)                                                       \
   CPGPU_LAUNCH(ev, (k_cg_dir_roll<T, PC, kRollRows>), dim3(G), block, 0, st, s, r, \
                pold, pnew, q, kst, partB, partA, hist, hist_cap,                    \
-               make_roll(s.nx, s.ny, HC), Gprev, partR64, Gref)
+               make_roll(s.nx, s.ny, HC, X), Gprev, partR64, Gref)
   switch (variant) {
     case 1: CPGPU_DIR_MARCH(4); break;
     case 2: CPGPU_DIR_MARCH(8); break;
@@ -350,16 +351,17 @@ void launch_dir(int variant, hipStream_t st, const Sten<T>& s, const T* r,
     case 4: CPGPU_DIR_MARCH(32); break;
     case 5: CPGPU_DIR_AMARCH(8); break;
     case 6: CPGPU_DIR_AMARCH(16); break;
-    case 7: CPGPU_DIR_ROLL(16); break;
-    case 8: CPGPU_DIR_ROLL(32); break;
-    default: CPGPU_DIR_ROLL(64); break;
+    case 7: CPGPU_DIR_ROLL(32, 1); break;
+    case 8: CPGPU_DIR_ROLL(64, 1); break;
+    case 9: CPGPU_DIR_ROLL(64, 0); break;
+    default: CPGPU_DIR_ROLL(128, 0); break;
   }
 #undef CPGPU_DIR_MARCH
 #undef CPGPU_DIR_AMARCH
 #undef CPGPU_DIR_ROLL
 }
 
-// The update kernel of a variant: the vectorised one for 5..9 when ny is
+// The update kernel of a variant: the vectorised one for 5..10 when ny is
 // even, else k_cg_upd; both with tl.G blocks (so partial counts match).
 template <typename T, bool PC>
 void launch_upd(int variant, hipStream_t st, const Sten<T>& s, T* x, const T* p, T* r,

@@ -70,6 +70,20 @@ __device__ __forceinline__ T diag_at(const Sten<T>& s, int i, int j) {
   return d;
 }
 
+// diag_at from coefficients already in registers (same arithmetic order).
+template <typename T>
+__device__ __forceinline__ T diag_from(const Sten<T>& s, int i, int j, T kx0, T kx1, T ky0,
+                                       T ky1) {
+  T d = kx0 + kx1;
+  d = d + ky0;
+  d = d + ky1;
+  if (i == 0) d = d + s.adj[0];
+  if (j == 0) d = d + s.adj[2];
+  if (j == s.ny - 1) d = d + s.adj[3];
+  if (i == s.nx - 1) d = d + s.adj[1];
+  return d;
+}
+
 template <typename T>
 __device__ __forceinline__ T apply_global(const Sten<T>& s,
                                           const T* __restrict__ x, int i,
@@ -533,16 +547,18 @@ __global__ __launch_bounds__(256) void k_cg_dir_amarch(
 // c0-1 / c0+64 on lanes 0 / 63) are in flight while group g is formed and
 // its rows of q are computed, so a wave keeps memory busy instead of
 // alternating between a load burst and a compute phase.  Work items
-// (strip, chunk) are numbered strip-fastest, and a block's logical index is
-// remapped so that the blocks a launch hands to one XCD (round robin by
-// block index) take consecutive items: neighbouring strips and chunks then
-// run together on one XCD and the halo lines they share come from its L2.
+// (strip, chunk) are numbered strip-fastest.  With xcd set, a block's
+// logical index is remapped so that the blocks a launch hands to one XCD
+// (round robin by block index) take consecutive items: neighbouring strips
+// and chunks then run together on one XCD and share halo lines in its L2
+// (measured faster in FP32 at 4096^2, slower in FP64, hence both orders).
 struct Roll {
-  int nstrip, nchunk, hc, nunits;
+  int nstrip, nchunk, hc, nunits, xcd;
 };
 
-inline Roll make_roll(int nx, int ny, int hc) {
+inline Roll make_roll(int nx, int ny, int hc, int xcd = 1) {
   Roll m;
+  m.xcd = xcd;
   m.nstrip = (ny + 63) / 64;
   m.hc = hc;
   m.nchunk = (nx + hc - 1) / hc;
@@ -568,8 +584,9 @@ __global__ __launch_bounds__(256) void k_cg_dir_roll(
   const int nx = s.nx, ny = s.ny;
   const int lane = threadIdx.x;  // blockDim = (64, 4)
   const int G = gridDim.x;
-  const int lb = (G & 7) ? (int)blockIdx.x
-                         : (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
+  const int lb = ((G & 7) || !mr.xcd)
+                     ? (int)blockIdx.x
+                     : (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
   const int items = mr.nstrip * mr.nchunk;
 
   // Every load of a group is issued unconditionally from a clamped, valid
@@ -725,38 +742,53 @@ __global__ __launch_bounds__(256) void k_cg_upd2(
     const T* __restrict__ q, KState* st, const T* __restrict__ partA,
     T* __restrict__ partB, int ntiles, int ntj, int norm_ref, int Gprev) {
   typedef typename Vec2<T>::V V;
+  constexpr int NR = TI / BY;
   if (st->done) return;
   const int it = st->it_upd;
   __shared__ T lds[16];
   T pq[1];
   reduce_partials<T, 1>(partA, Gprev, pq, lds);
   const T alpha = (T)st->rho[it & 1] / pq[0];
-  const size_t nlast = (size_t)s.nx * s.ny - 1;
+  const int nx = s.nx, ny = s.ny;
+  const size_t nlast = (size_t)nx * ny - 1;
   T v[3] = {T(0), T(0), T(0)};
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int ti = t / ntj;
     const int i0 = ti * TI, j = (t - ti * ntj) * 128 + 2 * threadIdx.x;
+    // every load of the tile first (clamped, valid addresses), then the rows
+    V xv[NR], pv[NR], rv[NR], qv[NR];
+    T kx0[NR], kx1[NR];
+    const int jc = min(j, ny - 2);
+    const T ky0 = s.KY[jc], ky1 = s.KY[jc + 1], ky2 = s.KY[jc + 2];
 #pragma unroll
-    for (int k = 0; k < TI / BY; ++k) {
+    for (int k = 0; k < NR; ++k) {
+      const int ic = min(i0 + (int)threadIdx.y + BY * k, nx - 1);
+      const size_t idx = (size_t)ic * ny + jc;
+      xv[k] = *reinterpret_cast<const V*>(x + idx);
+      pv[k] = *reinterpret_cast<const V*>(p + idx);
+      rv[k] = *reinterpret_cast<const V*>(r + idx);
+      qv[k] = *reinterpret_cast<const V*>(q + idx);
+      kx0[k] = s.KX[ic];
+      kx1[k] = s.KX[ic + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
       const int i = i0 + threadIdx.y + BY * k;
-      if (i >= s.nx || j >= s.ny) continue;
-      const size_t idx = (size_t)i * s.ny + j;
-      V xv = *reinterpret_cast<const V*>(x + idx);
-      const V pv = *reinterpret_cast<const V*>(p + idx);
-      V rv = *reinterpret_cast<const V*>(r + idx);
-      const V qv = *reinterpret_cast<const V*>(q + idx);
-      xv.x = xv.x + alpha * pv.x;
-      xv.y = xv.y + alpha * pv.y;
-      rv.x = rv.x - alpha * qv.x;
-      rv.y = rv.y - alpha * qv.y;
-      *reinterpret_cast<V*>(x + idx) = xv;
-      *reinterpret_cast<V*>(r + idx) = rv;
-      const T z0 = PC ? rv.x * (T(1) / diag_at(s, i, j)) : rv.x;
-      const T z1 = PC ? rv.y * (T(1) / diag_at(s, i, j + 1)) : rv.y;
-      v[SLOT_RZ] += rv.x * z0;
-      v[SLOT_RZ] += rv.y * z1;
-      v[SLOT_RR] += rv.x * rv.x;
-      if (!(norm_ref && idx + 1 == nlast)) v[SLOT_RR] += rv.y * rv.y;
+      if (i >= nx || j >= ny) continue;
+      const size_t idx = (size_t)i * ny + j;
+      V xo = xv[k], ro = rv[k];
+      xo.x = xo.x + alpha * pv[k].x;
+      xo.y = xo.y + alpha * pv[k].y;
+      ro.x = ro.x - alpha * qv[k].x;
+      ro.y = ro.y - alpha * qv[k].y;
+      *reinterpret_cast<V*>(x + idx) = xo;
+      *reinterpret_cast<V*>(r + idx) = ro;
+      const T z0 = PC ? ro.x * (T(1) / diag_from(s, i, j, kx0[k], kx1[k], ky0, ky1)) : ro.x;
+      const T z1 = PC ? ro.y * (T(1) / diag_from(s, i, j + 1, kx0[k], kx1[k], ky1, ky2)) : ro.y;
+      v[SLOT_RZ] += ro.x * z0;
+      v[SLOT_RZ] += ro.y * z1;
+      v[SLOT_RR] += ro.x * ro.x;
+      if (!(norm_ref && idx + 1 == nlast)) v[SLOT_RR] += ro.y * ro.y;
     }
   }
   store_partials<T, 3>(v, partB, gridDim.x, blockIdx.x, lds);
@@ -819,33 +851,61 @@ __global__ __launch_bounds__(256) void k_refresh(Sten<double> s64, Sten<float> s
 }
 
 // Iteration `it`, second kernel: alpha = rho_it / p.q; x += alpha p;
-// r -= alpha q; partials of r.z and r.r for the next prologue.
+// r -= alpha q; partials of r.z and r.r for the next prologue.  All loads
+// of a tile (TI/BY rows of x, p, r, q and the face coefficients) are issued
+// together from clamped valid addresses before any row is computed, so a
+// wave has a tile of loads in flight instead of one row's.
+template <typename T, int NR>
+struct UpdBuf {
+  T x[NR], p[NR], r[NR], q[NR], kx0[NR], kx1[NR];
+  T ky0, ky1;
+};
+
 template <typename T, bool PC>
 __global__ __launch_bounds__(256) void k_cg_upd(
     Sten<T> s, T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r,
     const T* __restrict__ q, KState* st, const T* __restrict__ partA,
     T* __restrict__ partB, Tiling tl, int norm_ref, int Gprev) {
   if (st->done) return;
+  constexpr int NR = TI / BY;
+  const int nx = s.nx, ny = s.ny;
   const int it = st->it_upd;
   __shared__ T lds[16];
   T pq[1];
   reduce_partials<T, 1>(partA, Gprev, pq, lds);
   const T alpha = (T)st->rho[it & 1] / pq[0];
-  const size_t nlast = (size_t)s.nx * s.ny - 1;
+  const size_t nlast = (size_t)nx * ny - 1;
   T v[3] = {T(0), T(0), T(0)};
   for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
     int i0, j0;
     tile_origin(t, tl.ntj, i0, j0);
+    UpdBuf<T, NR> b;
+    {
+      const int jc = min(j0 + (int)threadIdx.x, ny - 1);
+      b.ky0 = s.KY[jc];
+      b.ky1 = s.KY[jc + 1];
+#pragma unroll
+      for (int k = 0; k < NR; ++k) {
+        const int ic = min(i0 + (int)threadIdx.y + BY * k, nx - 1);
+        const size_t idx = (size_t)ic * ny + jc;
+        b.x[k] = x[idx];
+        b.p[k] = p[idx];
+        b.r[k] = r[idx];
+        b.q[k] = q[idx];
+        b.kx0[k] = s.KX[ic];
+        b.kx1[k] = s.KX[ic + 1];
+      }
+    }
     const int j = j0 + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < TI / BY; ++k) {
+    for (int k = 0; k < NR; ++k) {
       const int i = i0 + threadIdx.y + BY * k;
-      if (i >= s.nx || j >= s.ny) continue;
-      const size_t idx = (size_t)i * s.ny + j;
-      x[idx] = x[idx] + alpha * p[idx];
-      const T rv = r[idx] - alpha * q[idx];
+      if (i >= nx || j >= ny) continue;
+      const size_t idx = (size_t)i * ny + j;
+      x[idx] = b.x[k] + alpha * b.p[k];
+      const T rv = b.r[k] - alpha * b.q[k];
       r[idx] = rv;
-      const T z = PC ? rv * (T(1) / diag_at(s, i, j)) : rv;
+      const T z = PC ? rv * (T(1) / diag_from(s, i, j, b.kx0[k], b.kx1[k], b.ky0, b.ky1)) : rv;
       v[SLOT_RZ] += rv * z;
       if (!(norm_ref && idx == nlast)) v[SLOT_RR] += rv * rv;
     }

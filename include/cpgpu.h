@@ -144,9 +144,10 @@ int cpgpu_poisson_set_mixed(cpgpu_poisson *P, double inner_tol, int max_outer);
 /* Direction-kernel variant of the Krylov iteration per precision: 0 LDS
  * tile, 1..4 column march with 4, 8, 16, 32 rows per wavefront, 5..6 the
  * aligned column march (64 aligned columns per wavefront) with 8, 16 rows,
- * 7..9 the streaming aligned march (loads of the next 4 rows in flight while
- * the current ones are computed) over chunks of 16, 32, 64 rows; 5..9 are
- * paired with the two-cells-per-lane update kernel when ny is even. */
+ * 7..10 the streaming aligned march (loads of the next 4 rows in flight
+ * while the current ones are computed) over chunks of 32, 64 rows with the
+ * XCD-aware unit order and 64, 128 rows in plain order; 5..10 are paired
+ * with the two-cells-per-lane update kernel when ny is even. */
 int cpgpu_poisson_set_variant(cpgpu_poisson *P, int v64, int v32);
 /* Autotuner: times reps launches of every variant on this problem (FP64 and
  * FP32), keeps the fastest and reports it with its time per launch (us).
