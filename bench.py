@@ -15,9 +15,11 @@ in FP64 refinement, each from the same saved state; their fields are
 compared.
 
 The roofline figure is the average duration of the dominant U-Net kernel
-(the 16 -> 16 3x3 convolution of the finest level) over one more learned
-step, from timestamps the dispatch itself carries (hipExtLaunchKernelGGL
-events), against its algorithmic bytes.
+(the 16 -> 16 3x3 convolution of the finest level that stores its output and
+its 4 x 4 pooling) over one more learned step, from timestamps the dispatch
+itself carries (hipExtLaunchKernelGGL events), against its algorithmic
+bytes.  The same kernel's other launch in an applied network (its epilogue
+applies the output layer and stores no activation) is reported beside it.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -34,6 +36,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+# HBM bytes per launch of k_conv3_tile16p<16, true> storing its output and
+# its 4 x 4 pooling at 1024^2, from rocprofv3 PMC passes (FETCH_SIZE x 2.0
+# and WRITE_SIZE x 1.0, the factors a copy kernel of known size reads):
+# profiles/k_conv3_tile16p_1024_traffic_pmc.txt
+CONV16_TRAFFIC_1024 = (2 * 24734.9 + 34816.0) * 1024
 
 
 def log(msg):
@@ -247,22 +254,35 @@ def main():
     P.set_unet(net, ca.UNET_PRECOND)
     ns.set_solver("FCG_UNET", max_iter=10 ** 7, tol=tol, norm=ca.NORM_TRUE)
     ctx.timer_enable(2)
+    ctx.timer_enable(3)
     ns.step(1)
     n_c, ms_c = ctx.timer_read(2)
+    n_o, ms_o = ctx.timer_read(3)
     ctx.timer_enable(2, False)
+    ctx.timer_enable(3, False)
     us = ms_c * 1e3 / max(n_c, 1)
-    conv_bytes = 2 * N * 16 * 2          # read 16 BF16 channels, write 16
+    # read 16 BF16 channels, write 16 and their 4 x 4 pooling
+    conv_bytes = 2 * N * 16 * 2 + (N // 16) * 16 * 2
     conv_flops = 2 * 9 * 16 * 16 * N
     achieved = conv_bytes / (us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_conv3_tile16p<16,true> (finest U-Net level, 3x3 forward 16 -> 16, BF16 MFMA, persistent)",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": int(CONV16_TRAFFIC_1024) if n == 1024 else None,
+                "kernel": "k_conv3_tile16p<16,true> (finest U-Net level, 3x3 forward 16 -> 16 "
+                          "with fused 4x4 pooling, BF16 MFMA, persistent)",
                 "bytes_per_launch": conv_bytes, "avg_launch_us": round(us, 3),
                 "launches": n_c,
                 "mfma_tflops": round(conv_flops / (us * 1e-6) / 1e12, 2),
                 "mfma_frac_bf16": round(conv_flops / (us * 1e-6) / 1e12 / BF16_PEAK_TFLOPS, 4)}
-    log(f"bench: conv16 {us:.2f} us/launch, {achieved:.0f} GB/s")
+    if n_o:
+        us_o = ms_o * 1e3 / n_o
+        bytes_o = N * 16 * 2 + N * 4   # read 16 BF16 channels, write the FP32 output
+        roofline["fused_output_launch"] = {
+            "avg_launch_us": round(us_o, 3), "launches": n_o, "bytes_per_launch": bytes_o,
+            "achieved_GBs": round(bytes_o / (us_o * 1e-6) / 1e9, 1)}
+    log(f"bench: conv16 {us:.2f} us/launch, {achieved:.0f} GB/s; fused-output launch "
+        f"{(ms_o * 1e3 / max(n_o, 1)):.2f} us")
 
     # ---- Krylov kernels of the plain solve (same step, timed dispatches) ----
     ns.restore(state)

@@ -73,7 +73,8 @@ EXPORTS = [
     "cpgpu_batch_destroy", "cpgpu_batch_set_tiles", "cpgpu_batch_set_graphs",
     "cpgpu_batch_solve", "cpgpu_batch_autotune", "cpgpu_batch_draw",
     "cpgpu_ns_field_change", "cpgpu_unet_set_conv_shape", "cpgpu_unet_clear_samples",
-    "cpgpu_unet_set_walls", "cpgpu_copy_bandwidth",
+    "cpgpu_unet_set_walls", "cpgpu_copy_bandwidth", "cpgpu_csr_create",
+    "cpgpu_csr_destroy", "cpgpu_csr_mult", "cpgpu_csr_solve",
 ]
 
 
@@ -508,6 +509,57 @@ def random_problems(ctx, seed=2, n=256, lo=6, hi=10, nblobs=64, wmin=0.01, wmax=
         P.fill_phi(0.0)
         probs.append(P)
     return probs
+
+
+EZERODIAG = -10
+
+
+class CSR:
+    """A general CSR matrix on the device (cpgpu_csr): the algebra solvers'
+    MatrixSCR_ (lib/algebra/matrix_SparCompRow.hpp) with the free-function
+    solvers of lib/algebra/solver_matrix.hpp on it."""
+
+    def __init__(self, ctx, row_ptr, col, val):
+        self.ctx = ctx
+        self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.n = len(self.row_ptr) - 1
+        self.h = C.c_void_p()
+        ctx.check(lib().cpgpu_csr_create(ctx.h, C.c_int(self.n), _p(self.row_ptr, _ip),
+                                         _p(self.col, _ip), _p(self.val), C.byref(self.h)))
+        ctx.adopt(self)
+
+    def mult(self, x):
+        """y = A x (MatrixSCR_::operator*, rows summed in CSR order)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n)
+        self.ctx.check(lib().cpgpu_csr_mult(self.h, _p(x), _p(y)))
+        return y
+
+    def solve(self, method, x, b, max_iter=1000, tol=1e-4, norm=NORM_REF, hist_cap=100000):
+        """The reference's int F(A, x, b, max_iter, tol, lresid): returns
+        (code, x, max_iter, tol, residual history); x is not modified."""
+        x = np.array(x, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        mi, t, hl = C.c_int(max_iter), C.c_double(tol), C.c_int(0)
+        hist = np.zeros(max(hist_cap, 1))
+        code = lib().cpgpu_csr_solve(self.h, C.c_int(METHODS[method]), _p(x), _p(b),
+                                     C.byref(mi), C.byref(t), C.c_int(norm), _p(hist),
+                                     C.c_int(hist_cap), C.byref(hl))
+        self.ctx.check(code, allow=(OK, MAXITER, 2, 3))
+        return code, x, mi.value, t.value, hist[:min(hl.value, hist_cap)]
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().cpgpu_csr_destroy(self.h)
+        self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

@@ -662,7 +662,9 @@ int time_kernels(cpgpu_poisson* P, const Sten<T>& s, const CGVecs<T>& v,
   hipStream_t st = ctx->stream;
   const Tiling tl = P->tl;
   KState init{};
-  init.tol = 0.0;
+  // a negative tolerance never converges: a zero right-hand side (resid 0)
+  // would otherwise stop the state at once and time empty launches
+  init.tol = -1.0;
   init.max_iter = 1 << 30;
   init.it_dir = 1;
   std::memcpy(&P->h_kst[0], &init, sizeof(KState));

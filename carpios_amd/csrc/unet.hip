@@ -575,9 +575,13 @@ int conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{},
   const int M = x.H * x.W;
   const bf16* w = U->wbf.p + U->wbf_off[l];
   const float* b = U->params.p + U->boff[l];
-  // timer family 2: the 16 -> 16 convolutions of the finest level
+  // timer families 2 and 3: the 16 -> 16 convolutions of the finest level
+  // that store their output (2) and those whose epilogue applies the output
+  // layer instead (3): their memory traffic differs
+  const bool out_fusable = U->conv_persist > 0 && tiled<CIN>(x.H, x.W) && y.cs % 4 == 0 &&
+                           y.co % 4 == 0 && wout && yout && yscale && !store_act;
   const EvPair ev = (CIN == 16 && COUT == 16 && M >= kSmallM)
-                        ? timer_events(U->ctx, 2) : EvPair();
+                        ? timer_events(U->ctx, out_fusable ? 3 : 2) : EvPair();
   if (M < kSmallM) {
     hipLaunchKernelGGL((k_conv3_fwd<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, COUT)),
                        dim3(256), 0, U->ctx->stream, x, y, w, b, relu);

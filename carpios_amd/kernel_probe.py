@@ -45,6 +45,8 @@ def main(argv=None):
         k, ms = ctx.timer_read(2)
         net.close()
         ctx.close()
+        # net.forward keeps every activation: both finest-level 16 -> 16
+        # convolutions store their output (one also its 4 x 4 pooling)
         print(json.dumps({"size": n, "mode": "unet", "reps": a.reps,
                           "conv16_launches": k, "conv16_us": round(ms * 1e3 / max(k, 1), 3)}),
               flush=True)

@@ -397,13 +397,39 @@ int cpgpu_poisson_set_unet(cpgpu_poisson *P, cpgpu_unet *U, int flags);
  * the network as the coarse correction (default 2, 2, 0.8). */
 int cpgpu_poisson_set_fcg(cpgpu_poisson *P, int pre, int post, double omega);
 
+/* ---- general CSR matrices (the algebra solvers' MatrixSCR_) ----
+ * Replaces the free functions of lib/algebra/solver_matrix.hpp on a
+ * MatrixSCR_<double> (matrix_SparCompRow.hpp): int F(const MatrixSCR_& A,
+ * ArrayListV& x, const ArrayListV& b, int& max_iter, Float& tol,
+ * std::list<Float>& lresid) for F = CG (:264-316), IC_CG's loop with
+ * DiaPre(A,1) (:318-374), Dia_BiCGSTAB (:599-672) and Solver_Jacobi_
+ * (:820-910), and MatrixSCR_::operator* (:236-249).  The matrix is copied to
+ * the device once; row_ptr has n + 1 entries starting at 0, col_ind and val
+ * row_ptr[n] entries (0-based columns). */
+typedef struct cpgpu_csr cpgpu_csr;
+int cpgpu_csr_create(cpgpu_ctx *ctx, int n, const int *row_ptr, const int *col_ind,
+                     const double *val, cpgpu_csr **out);
+int cpgpu_csr_destroy(cpgpu_csr *A);
+/* y = A x (host vectors of n doubles), each row summed in CSR order. */
+int cpgpu_csr_mult(cpgpu_csr *A, const double *x, double *y);
+/* method: CPGPU_CG, CPGPU_JPCG, CPGPU_BICGSTAB or CPGPU_JACOBI (other
+ * methods: CPGPU_EUNSUPPORTED).  x (host, n doubles) is the start on entry
+ * and the result on exit; max_iter and tol in/out as the reference; hist
+ * receives the residual after each iteration (hist_len = their count, hist
+ * may be NULL).  Returns 0 converged, 1 max_iter, 2 / 3 BiCGSTAB breakdowns,
+ * CPGPU_EZERODIAG when DiaPre meets a zero or missing diagonal entry. */
+int cpgpu_csr_solve(cpgpu_csr *A, int method, double *x, const double *b, int *max_iter,
+                    double *tol, int norm_mode, double *hist, int hist_cap, int *hist_len);
+
 /* ---- kernel timing (HIP events on the context stream) ----
  * When enabled, every launch of the named kernel family is bracketed by
  * events; cpgpu_timer_read returns launches and summed milliseconds.
  * family: 0 = stencil apply (p update + A p + p.q), 1 = vector update
  * (x, r update + dots), 2 = the U-Net's 16 -> 16 3x3 convolutions on the
- * finest level.  Timed launches carry their own start/stop timestamps
- * (no extra packets between kernels). */
+ * finest level that store their output, 3 = the same convolution when its
+ * epilogue applies the 1 x 1 output layer instead (a network only applied).
+ * Timed launches carry their own start/stop timestamps (no extra packets
+ * between kernels). */
 int cpgpu_timer_enable(cpgpu_ctx *ctx, int family, int on);
 int cpgpu_timer_read(cpgpu_ctx *ctx, int family, int *launches, double *ms);
 
