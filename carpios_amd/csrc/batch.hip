@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void k_b_dir(const BProb* __restrict__ probs,
   const int rank = blockIdx.x - P.blk0;
   __shared__ T lds[16];
   __shared__ double lds64[4];
-  __shared__ T tile[TI + 2][TJ + 2];
+  __shared__ TileLds<T> L;
   const int it = st->it_dir;
   T tot[3];
   reduce_prob<T, 3>(partB + P.pofs, Btot, P.nblk, tot, lds);
@@ -244,42 +244,16 @@ __global__ __launch_bounds__(256) void k_b_dir(const BProb* __restrict__ probs,
   const BVecs<T> v = bvecs<T>(P);
   const T* __restrict__ pold = flip ? v.pb : v.pa;
   T* __restrict__ pnew = flip ? v.pa : v.pb;
-  const int nx = s.nx, ny = s.ny;
-  const int tid = threadIdx.x + TJ * threadIdx.y;
   T acc[1] = {T(0)};
   for (int t = rank; t < P.ntiles; t += P.nblk) {
     int i0, j0;
     tile_origin(t, P.ntj, i0, j0);
+    TileStage<T> b;
+    stage_load<T>(s, v.r, pold, i0, j0, b);
     __syncthreads();  // previous tile fully consumed
-    for (int e = tid; e < (TI + 2) * (TJ + 2); e += TJ * BY) {
-      const int ii = e / (TJ + 2), jj = e - ii * (TJ + 2);
-      const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
-      T pv = T(0);
-      if (gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
-        const size_t k = (size_t)gi * ny + gj;
-        const T rv = v.r[k];
-        const T z = rv * (T(1) / diag_at(s, gi, gj));
-        pv = first ? z : z + beta * pold[k];
-        if (ii >= 1 && ii <= TI && jj >= 1 && jj <= TJ) pnew[k] = pv;
-      }
-      tile[ii][jj] = pv;
-    }
+    stage_store<T, true>(s, b, i0, j0, beta, first, pnew, L);
     __syncthreads();
-    const int lj = threadIdx.x + 1, j = j0 + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < TI / BY; ++k) {
-      const int li = threadIdx.y + BY * k + 1, i = i0 + li - 1;
-      if (i >= nx || j >= ny) continue;
-      const T dg = diag_at(s, i, j);
-      T a = T(0);
-      if (i > 0) a = a + tile[li - 1][lj] * (-s.KX[i]);
-      if (j > 0) a = a + tile[li][lj - 1] * (-s.KY[j]);
-      a = a + tile[li][lj] * dg;
-      if (j < ny - 1) a = a + tile[li][lj + 1] * (-s.KY[j + 1]);
-      if (i < nx - 1) a = a + tile[li + 1][lj] * (-s.KX[i + 1]);
-      v.q[(size_t)i * ny + j] = a;
-      acc[0] += tile[li][lj] * a;
-    }
+    tile_apply<T>(s, i0, j0, L, v.q, acc[0]);
   }
   store_partials<T, 1>(acc, partA + P.pofs, Btot, rank, lds);
 }
@@ -306,19 +280,36 @@ __global__ __launch_bounds__(256) void k_b_upd(const BProb* __restrict__ probs,
   const BVecs<T> v = bvecs<T>(P);
   const T* __restrict__ pv = flip ? v.pa : v.pb;
   T acc[3] = {T(0), T(0), T(0)};
+  constexpr int NR = TI / BY;
+  const int nx = s.nx, ny = s.ny;
   for (int t = rank; t < P.ntiles; t += P.nblk) {
     int i0, j0;
     tile_origin(t, P.ntj, i0, j0);
+    // every load of the tile first (clamped, valid addresses), as k_cg_upd
+    T xv[NR], pw[NR], rw[NR], qv[NR], kx0[NR], kx1[NR];
+    const int jc = min(j0 + (int)threadIdx.x, ny - 1);
+    const T ky0 = s.KY[jc], ky1 = s.KY[jc + 1];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int ic = min(i0 + (int)threadIdx.y + BY * k, nx - 1);
+      const size_t idx = (size_t)ic * ny + jc;
+      xv[k] = v.x[idx];
+      pw[k] = pv[idx];
+      rw[k] = v.r[idx];
+      qv[k] = v.q[idx];
+      kx0[k] = s.KX[ic];
+      kx1[k] = s.KX[ic + 1];
+    }
     const int j = j0 + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < TI / BY; ++k) {
+    for (int k = 0; k < NR; ++k) {
       const int i = i0 + threadIdx.y + BY * k;
-      if (i >= s.nx || j >= s.ny) continue;
-      const size_t idx = (size_t)i * s.ny + j;
-      v.x[idx] = v.x[idx] + alpha * pv[idx];
-      const T rv = v.r[idx] - alpha * v.q[idx];
+      if (i >= nx || j >= ny) continue;
+      const size_t idx = (size_t)i * ny + j;
+      v.x[idx] = xv[k] + alpha * pw[k];
+      const T rv = rw[k] - alpha * qv[k];
       v.r[idx] = rv;
-      const T z = rv * (T(1) / diag_at(s, i, j));
+      const T z = rv * (T(1) / diag_from(s, i, j, kx0[k], kx1[k], ky0, ky1));
       acc[SLOT_RZ] += rv * z;
       acc[SLOT_RR] += rv * rv;
     }

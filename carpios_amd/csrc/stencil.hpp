@@ -241,6 +241,103 @@ __device__ __forceinline__ bool dir_prologue(KState* st,
   return true;
 }
 
+// Staging of the LDS tile kernels (k_cg_dir here, the batched k_b_dir):
+// the (TI+2) x (TJ+2) tile of p = z + beta p_old with its halo.  Every
+// global load of a tile (r and p_old of the staged elements, the face
+// coefficients of the tile's rows and columns) is issued first,
+// unconditionally from a clamped valid address (masked when used), so a
+// thread has all of them in flight at once instead of one round trip per
+// staged element (a load under a branch makes the compiler drain the loads
+// before it).  The coefficients go through LDS (skx, sky), which keeps them
+// out of the registers and out of the vector-memory wait counts.  The
+// arithmetic is diag_at's and k_cg_dir's exactly.
+constexpr int kStageE = (TI + 2) * (TJ + 2);
+constexpr int kStageK = (kStageE + TJ * BY - 1) / (TJ * BY);
+
+template <typename T>
+struct TileStage {
+  T r[kStageK], p[kStageK];
+  T ckx, cky;  // KX[i0 - 1 + tid] (tid < TI + 3), KY[j0 - 1 + tid] (tid < TJ + 3)
+};
+
+template <typename T>
+struct TileLds {
+  T tile[TI + 2][TJ + 2];
+  T skx[TI + 3], sky[TJ + 3];
+};
+
+template <typename T>
+__device__ __forceinline__ void stage_load(const Sten<T>& s, const T* __restrict__ r,
+                                           const T* __restrict__ pold, int i0, int j0,
+                                           TileStage<T>& b) {
+  const int nx = s.nx, ny = s.ny;
+  const int tid = threadIdx.x + TJ * threadIdx.y;
+#pragma unroll
+  for (int k = 0; k < kStageK; ++k) {
+    const int e = min(tid + TJ * BY * k, kStageE - 1);
+    const int ii = e / (TJ + 2), jj = e - ii * (TJ + 2);
+    const int gi = min(max(i0 - 1 + ii, 0), nx - 1), gj = min(max(j0 - 1 + jj, 0), ny - 1);
+    const size_t m = (size_t)gi * ny + gj;
+    b.r[k] = r[m];
+    b.p[k] = pold[m];
+  }
+  b.ckx = s.KX[min(max(i0 - 1 + tid, 0), nx)];
+  b.cky = s.KY[min(max(j0 - 1 + tid, 0), ny)];
+}
+
+// after a barrier that ends the previous tile's reads: the coefficients to
+// LDS, a barrier, then p into the tile (and p_new for the tile's interior)
+template <typename T, bool PC>
+__device__ __forceinline__ void stage_store(const Sten<T>& s, const TileStage<T>& b, int i0,
+                                            int j0, T beta, bool first, T* __restrict__ pnew,
+                                            TileLds<T>& L) {
+  const int nx = s.nx, ny = s.ny;
+  const int tid = threadIdx.x + TJ * threadIdx.y;
+  if (tid < TI + 3) L.skx[tid] = b.ckx;
+  if (tid < TJ + 3) L.sky[tid] = b.cky;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kStageK; ++k) {
+    const int e = tid + TJ * BY * k;
+    if (e >= kStageE) continue;
+    const int ii = e / (TJ + 2), jj = e - ii * (TJ + 2);
+    const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
+    T pv = T(0);
+    if (gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
+      const T dg = diag_from(s, gi, gj, L.skx[ii], L.skx[ii + 1], L.sky[jj], L.sky[jj + 1]);
+      const T z = PC ? b.r[k] * (T(1) / dg) : b.r[k];
+      pv = first ? z : z + beta * b.p[k];
+      if (ii >= 1 && ii <= TI && jj >= 1 && jj <= TJ) pnew[(size_t)gi * ny + gj] = pv;
+    }
+    L.tile[ii][jj] = pv;
+  }
+}
+
+// q = A p for the block's output rows from the staged tile, q stored and
+// the thread's partial p.q accumulated row by row into acc
+template <typename T>
+__device__ __forceinline__ void tile_apply(const Sten<T>& s, int i0, int j0, const TileLds<T>& L,
+                                           T* __restrict__ q, T& acc) {
+  const int nx = s.nx, ny = s.ny;
+  const int lj = threadIdx.x + 1, j = j0 + threadIdx.x;
+  const T ky0 = L.sky[lj], ky1 = L.sky[lj + 1];
+#pragma unroll
+  for (int k = 0; k < TI / BY; ++k) {
+    const int li = threadIdx.y + BY * k + 1, i = i0 + li - 1;
+    if (i >= nx || j >= ny) continue;
+    const T kx0 = L.skx[li], kx1 = L.skx[li + 1];
+    const T dg = diag_from(s, i, j, kx0, kx1, ky0, ky1);
+    T a = T(0);
+    if (i > 0) a = a + L.tile[li - 1][lj] * (-kx0);
+    if (j > 0) a = a + L.tile[li][lj - 1] * (-ky0);
+    a = a + L.tile[li][lj] * dg;
+    if (j < ny - 1) a = a + L.tile[li][lj + 1] * (-ky1);
+    if (i < nx - 1) a = a + L.tile[li + 1][lj] * (-kx1);
+    q[(size_t)i * ny + j] = a;
+    acc += L.tile[li][lj] * a;
+  }
+}
+
 // Iteration `it` (1-based), first kernel.  Prologue: totals of the previous
 // kernel's partials -> residual of iteration it-1, stopping test
 // (solver_matrix.hpp:356-372: resid <= tol), rho_it = r.z.  Body: p = z +
@@ -256,49 +353,23 @@ __global__ __launch_bounds__(256) void k_cg_dir(
     int Gref) {
   if (st->done) return;
   __shared__ T lds[16];
-  __shared__ T tile[TI + 2][TJ + 2];
+  __shared__ TileLds<T> L;
   T beta;
   bool first;
   if (!dir_prologue<T>(st, partB, Gprev, partR64, Gref, hist, hist_cap, lds,
                        &beta, &first))
     return;
 
-  const int nx = s.nx, ny = s.ny;
-  const int tid = threadIdx.x + TJ * threadIdx.y;
   T v[1] = {T(0)};
   for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
     int i0, j0;
     tile_origin(t, tl.ntj, i0, j0);
+    TileStage<T> b;
+    stage_load<T>(s, r, pold, i0, j0, b);
     __syncthreads();  // previous tile fully consumed
-    for (int e = tid; e < (TI + 2) * (TJ + 2); e += TJ * BY) {
-      const int ii = e / (TJ + 2), jj = e - ii * (TJ + 2);
-      const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
-      T pv = T(0);
-      if (gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
-        const size_t k = (size_t)gi * ny + gj;
-        const T rv = r[k];
-        const T z = PC ? rv * (T(1) / diag_at(s, gi, gj)) : rv;
-        pv = first ? z : z + beta * pold[k];
-        if (ii >= 1 && ii <= TI && jj >= 1 && jj <= TJ) pnew[k] = pv;
-      }
-      tile[ii][jj] = pv;
-    }
+    stage_store<T, PC>(s, b, i0, j0, beta, first, pnew, L);
     __syncthreads();
-    const int lj = threadIdx.x + 1, j = j0 + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < TI / BY; ++k) {
-      const int li = threadIdx.y + BY * k + 1, i = i0 + li - 1;
-      if (i >= nx || j >= ny) continue;
-      const T dg = diag_at(s, i, j);
-      T acc = T(0);
-      if (i > 0) acc = acc + tile[li - 1][lj] * (-s.KX[i]);
-      if (j > 0) acc = acc + tile[li][lj - 1] * (-s.KY[j]);
-      acc = acc + tile[li][lj] * dg;
-      if (j < ny - 1) acc = acc + tile[li][lj + 1] * (-s.KY[j + 1]);
-      if (i < nx - 1) acc = acc + tile[li + 1][lj] * (-s.KX[i + 1]);
-      q[(size_t)i * ny + j] = acc;
-      v[0] += tile[li][lj] * acc;
-    }
+    tile_apply<T>(s, i0, j0, L, q, v[0]);
   }
   store_partials<T, 1>(v, partA, gridDim.x, blockIdx.x, lds);
 }
