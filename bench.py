@@ -155,6 +155,61 @@ def batched(ctx, ca, tol, no_cpu, learned_net=True):
     return res
 
 
+def csr_probe(ctx, ca, no_cpu, m=2048, iters=200):
+    """The general CSR path (cpgpu_csr: the algebra solvers' MatrixSCR_ on
+    the device) on the m^2 five-point Dirichlet Laplacian as an explicit CSR
+    matrix (sorted columns, 5 m^2 entries): Jacobi-PCG iterations timed
+    without convergence, against the algorithmic bytes of one iteration
+    (the product: 12 B per entry + row pointers + x and y once; the vector
+    steps: 136 B per row) and the oracle's CPU iterations on the same
+    matrix."""
+    import scipy.sparse as sp
+    n = m * m
+    T = sp.diags([-np.ones(m - 1), 4.0 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+    E = sp.diags([-np.ones(m - 1), -np.ones(m - 1)], [-1, 1])
+    A = (sp.kron(sp.identity(m), T) + sp.kron(E, sp.identity(m))).tocsr()
+    A.sort_indices()
+    rp, col, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    nnz = int(rp[-1])
+    b = np.random.default_rng(5).standard_normal(n)
+    M = ca.CSR(ctx, rp, col, val)
+    M.solve("JPCG", np.zeros(n), b, max_iter=5, tol=1e-300, hist_cap=8)  # warm
+    t0 = time.perf_counter()
+    code, x, it, t, _ = M.solve("JPCG", np.zeros(n), b, max_iter=iters, tol=1e-300,
+                                hist_cap=iters + 2)
+    wall = time.perf_counter() - t0
+    M.close()
+    t_it = wall / iters
+    alg = 12 * nnz + 4 * (n + 1) + 16 * n + 136 * n
+    out = {"matrix": f"{m}^2 five-point Laplacian, CSR, {n} rows, {nnz} entries",
+           "method": "JPCG (IC_CG loop with DiaPre), host-driven, every dot read back",
+           "iterations": iters, "ms_per_iteration": round(t_it * 1e3, 4),
+           "algorithmic_bytes_per_iteration": alg,
+           "algorithmic_GBs": round(alg / t_it / 1e9, 1),
+           "hbm_frac": round(alg / t_it / 1e9 / HBM_PEAK_GBS, 4),
+           "note": "includes the host round trip of each dot product and the host-to-device "
+                   "copies of x and b"}
+    if not no_cpu:
+        import ctypes as C
+        import oracle as O
+        H = O.CSR(n, rp.ctypes.data_as(C.POINTER(C.c_int)), col.ctypes.data_as(C.POINTER(C.c_int)),
+                  val.ctypes.data_as(C.POINTER(C.c_double)))
+
+        class Wrap:
+            pass
+        w = Wrap()
+        w.c, w.n = H, n
+        k = 10
+        c0 = time.perf_counter()
+        O.solve("JPCG", w, b, max_iter=k, tol=1e-300)
+        t_cpu = (time.perf_counter() - c0) / k
+        out["cpu_baseline"] = {"ms_per_iteration": round(t_cpu * 1e3, 3), "cores": 1,
+                               "kind": "port", "sample": f"{k} oracle JPCG iterations on the same CSR"}
+        out["speedup_vs_cpu"] = round(t_cpu / t_it, 1)
+    log(f"bench: CSR {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +228,8 @@ def main():
                     help="skip the 256-problem batched solve")
     ap.add_argument("--no-batch-learned", action="store_true",
                     help="skip the U-Net variant of the batched solve (its training)")
+    ap.add_argument("--no-csr", action="store_true",
+                    help="skip the general-CSR Jacobi-PCG probe")
     ap.add_argument("--no-hbm", action="store_true",
                     help="skip the 4096^2 Krylov-kernel HBM probe")
     ap.add_argument("--no-graphs", action="store_true",
@@ -351,6 +408,7 @@ def main():
     batch = None
     if not a.no_batch:
         batch = batched(ctx, ca, tol, a.no_cpu, not a.no_batch_learned)
+    csr = None if a.no_csr else csr_probe(ctx, ca, a.no_cpu)
     out = {
         "metric": "Poisson solves/s to rel-res 1e-8 @1024^2: learned+FP32/FP64-refine vs plain FP64",
         "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,
@@ -380,6 +438,7 @@ def main():
         "roofline_hbm_probe": hbm,
         "cpu_baseline": cpu,
         "batched_256": batch,
+        "csr_jpcg": csr,
     }
     print(json.dumps(out), flush=True)
     net.close()

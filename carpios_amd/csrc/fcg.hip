@@ -108,27 +108,37 @@ __global__ __launch_bounds__(256) void k_fcg_dir(Sten<double> s,
   const double beta = first ? 0.0 : -t[0] / st->pq;
   double v[2] = {0.0, 0.0};
   const int ny = s.ny;
+  constexpr int NR = TI / BY;
   for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
     int i0, j0;
     tile_origin(t, tl.ntj, i0, j0);
     const int j = j0 + threadIdx.x;
+    // every load of the thread's rows first (clamped addresses)
+    Nbr<float> zn[NR];
+    Nbr<double> pn[NR], kc[NR];
+    double rr[NR];
 #pragma unroll
-    for (int k = 0; k < TI / BY; ++k) {
+    for (int k = 0; k < NR; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      load_nbr(s, z, i, j, zn[k], kc[k]);
+      load_nbr(s, pold, i, j, pn[k], kc[k]);
+      rr[k] = r[(size_t)min(i, s.nx - 1) * ny + min(j, ny - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
       const int i = i0 + threadIdx.y + BY * k;
       if (i < s.nx && j < ny) {
         const size_t idx = (size_t)i * ny + j;
-        const double pc = ((double)z[idx] - m) + beta * pold[idx];
-        double acc = pc * diag_at(s, i, j);
-        if (i > 0) acc -= s.KX[i] * (((double)z[idx - ny] - m) + beta * pold[idx - ny]);
-        if (j > 0) acc -= s.KY[j] * (((double)z[idx - 1] - m) + beta * pold[idx - 1]);
-        if (j < ny - 1)
-          acc -= s.KY[j + 1] * (((double)z[idx + 1] - m) + beta * pold[idx + 1]);
-        if (i < s.nx - 1)
-          acc -= s.KX[i + 1] * (((double)z[idx + ny] - m) + beta * pold[idx + ny]);
+        const double pc = ((double)zn[k].c - m) + beta * pn[k].c;
+        double acc = pc * diag_from(s, i, j, kc[k].kx0, kc[k].kx1, kc[k].ky0, kc[k].ky1);
+        if (i > 0) acc -= kc[k].kx0 * (((double)zn[k].n - m) + beta * pn[k].n);
+        if (j > 0) acc -= kc[k].ky0 * (((double)zn[k].w - m) + beta * pn[k].w);
+        if (j < ny - 1) acc -= kc[k].ky1 * (((double)zn[k].e - m) + beta * pn[k].e);
+        if (i < s.nx - 1) acc -= kc[k].kx1 * (((double)zn[k].s - m) + beta * pn[k].s);
         pnew[idx] = pc;
         q[idx] = acc;
         v[0] += pc * acc;
-        v[1] += pc * r[idx];
+        v[1] += pc * rr[k];
       }
     }
   }
@@ -226,14 +236,24 @@ __global__ __launch_bounds__(256) void k_jacobi32(Sten<float> s,
     int i0, j0;
     tile_origin(t, tl.ntj, i0, j0);
     const int j = j0 + threadIdx.x;
+    constexpr int NR = TI / BY;
+    Nbr<float> zn[NR], kc[NR];
+    float rr[NR];
 #pragma unroll
-    for (int k = 0; k < TI / BY; ++k) {
+    for (int k = 0; k < NR; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      // (a zero guess reads r in place of zin: the values are not used)
+      load_nbr(s, zin ? zin : r, i, j, zn[k], kc[k]);
+      rr[k] = r[(size_t)min(i, s.nx - 1) * s.ny + min(j, s.ny - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
       const int i = i0 + threadIdx.y + BY * k;
       if (i < s.nx && j < s.ny) {
         const size_t idx = (size_t)i * s.ny + j;
-        const float dg = diag_at(s, i, j);
-        const float rv = zin ? r[idx] - apply_global(s, zin, i, j, dg) : r[idx];
-        zout[idx] = (zin ? zin[idx] : 0.f) + w * (rv / dg);
+        const float dg = diag_from(s, i, j, kc[k].kx0, kc[k].kx1, kc[k].ky0, kc[k].ky1);
+        const float rv = zin ? rr[k] - apply_regs(s, i, j, dg, zn[k], kc[k]) : rr[k];
+        zout[idx] = (zin ? zn[k].c : 0.f) + w * (rv / dg);
       }
     }
   }
@@ -248,12 +268,21 @@ __global__ __launch_bounds__(256) void k_resid32(Sten<float> s,
     int i0, j0;
     tile_origin(t, tl.ntj, i0, j0);
     const int j = j0 + threadIdx.x;
+    constexpr int NR = TI / BY;
+    Nbr<float> zn[NR], kc[NR];
+    float rr[NR];
 #pragma unroll
-    for (int k = 0; k < TI / BY; ++k) {
+    for (int k = 0; k < NR; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      load_nbr(s, z, i, j, zn[k], kc[k]);
+      rr[k] = r[(size_t)min(i, s.nx - 1) * s.ny + min(j, s.ny - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
       const int i = i0 + threadIdx.y + BY * k;
       if (i < s.nx && j < s.ny) {
-        const size_t idx = (size_t)i * s.ny + j;
-        rout[idx] = r[idx] - apply_global(s, z, i, j, diag_at(s, i, j));
+        const float dg = diag_from(s, i, j, kc[k].kx0, kc[k].kx1, kc[k].ky0, kc[k].ky1);
+        rout[(size_t)i * s.ny + j] = rr[k] - apply_regs(s, i, j, dg, zn[k], kc[k]);
       }
     }
   }

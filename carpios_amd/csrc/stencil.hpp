@@ -99,6 +99,47 @@ __device__ __forceinline__ T apply_global(const Sten<T>& s,
   return acc;
 }
 
+// The five values of x around (i, j) and the row's and column's face
+// coefficients, loaded from clamped valid addresses (values outside the grid
+// are never used: apply_regs masks them as apply_global does), so that a
+// kernel can issue every load of its rows before any arithmetic.
+template <typename T>
+struct Nbr {
+  T c, n, s, w, e;      // (i, j), (i-1, j), (i+1, j), (i, j-1), (i, j+1)
+  T kx0, kx1, ky0, ky1;  // KX[i], KX[i+1], KY[j], KY[j+1]
+};
+
+template <typename T, typename X>
+__device__ __forceinline__ void load_nbr(const Sten<T>& s, const X* __restrict__ x, int i,
+                                         int j, Nbr<X>& o, Nbr<T>& k) {
+  const int nx = s.nx, ny = s.ny;
+  const int ic = min(i, nx - 1), jc = min(j, ny - 1);
+  const int im = max(ic - 1, 0), ip = min(ic + 1, nx - 1);
+  const int jm = max(jc - 1, 0), jp = min(jc + 1, ny - 1);
+  o.c = x[(size_t)ic * ny + jc];
+  o.n = x[(size_t)im * ny + jc];
+  o.s = x[(size_t)ip * ny + jc];
+  o.w = x[(size_t)ic * ny + jm];
+  o.e = x[(size_t)ic * ny + jp];
+  k.kx0 = s.KX[ic];
+  k.kx1 = s.KX[ic + 1];
+  k.ky0 = s.KY[jc];
+  k.ky1 = s.KY[jc + 1];
+}
+
+// apply_global's arithmetic from registers
+template <typename T>
+__device__ __forceinline__ T apply_regs(const Sten<T>& s, int i, int j, T dg, const Nbr<T>& x,
+                                        const Nbr<T>& k) {
+  T acc = T(0);
+  if (i > 0) acc = acc + x.n * (-k.kx0);
+  if (j > 0) acc = acc + x.w * (-k.ky0);
+  acc = acc + x.c * dg;
+  if (j < s.ny - 1) acc = acc + x.e * (-k.ky1);
+  if (i < s.nx - 1) acc = acc + x.s * (-k.kx1);
+  return acc;
+}
+
 // Device state of one Krylov solve.  FP32 solves store FP32 values exactly.
 // rho is double-buffered by iteration parity: block 0 of iteration it writes
 // rho[it & 1] while every block reads rho[(it - 1) & 1].
