@@ -362,17 +362,33 @@ __global__ __launch_bounds__(256) void k_b_refresh(const BProb* __restrict__ pro
     int i0, j0;
     tile_origin(t, P.ntj, i0, j0);
     const int j = j0 + threadIdx.x;
+    constexpr int NR = TI / BY;
+    // every load of the rows first (clamped addresses)
+    Nbr<double> xn[NR], kc[NR];
+    Nbr<float> kf[NR];
+    double bv[NR];
 #pragma unroll
-    for (int k = 0; k < TI / BY; ++k) {
+    for (int k = 0; k < NR; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      load_nbr(s64, (const double*)P.x, i, j, xn[k], kc[k]);
+      const int ic = min(i, P.nx - 1), jc = min(j, P.ny - 1);
+      bv[k] = P.b[(size_t)ic * P.ny + jc];
+      kf[k].kx0 = s32.KX[ic];
+      kf[k].kx1 = s32.KX[ic + 1];
+      kf[k].ky0 = s32.KY[jc];
+      kf[k].ky1 = s32.KY[jc + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
       const int i = i0 + threadIdx.y + BY * k;
       if (i >= P.nx || j >= P.ny) continue;
       const size_t idx = (size_t)i * P.ny + j;
-      const double rv = P.b[idx] - apply_global(s64, (const double*)P.x, i, j,
-                                                diag_at(s64, i, j));
+      const double dg = diag_from(s64, i, j, kc[k].kx0, kc[k].kx1, kc[k].ky0, kc[k].ky1);
+      const double rv = bv[k] - apply_regs(s64, i, j, dg, xn[k], kc[k]);
       w[0] += rv * rv;
       const float rf = (float)rv;
       P.r32[idx] = rf;
-      const float z = rf * (1.f / diag_at(s32, i, j));
+      const float z = rf * (1.f / diag_from(s32, i, j, kf[k].kx0, kf[k].kx1, kf[k].ky0, kf[k].ky1));
       v[SLOT_RZ] += rf * z;
       v[SLOT_RR] += rf * rf;
     }
