@@ -40,7 +40,7 @@ BF16_PEAK_TFLOPS = 2500.0
 # its 4 x 4 pooling at 1024^2, from rocprofv3 PMC passes (FETCH_SIZE x 2.0
 # and WRITE_SIZE x 1.0, the factors a copy kernel of known size reads):
 # profiles/k_conv3_tile16p_1024_traffic_pmc.txt
-CONV16_TRAFFIC_1024 = (2 * 24734.9 + 34816.0) * 1024
+CONV16_TRAFFIC_1024 = (2 * 16787.8 + 34816.0) * 1024
 
 
 def log(msg):

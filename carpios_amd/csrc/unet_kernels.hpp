@@ -542,9 +542,20 @@ __global__ __launch_bounds__(256, NO == 16 ? 4 : 1) void k_conv3_tile16p(View x,
       pre[k] = *reinterpret_cast<const uint4*>(x.p + ((size_t)ci * W + cj) * x.cs + x.co + ch * 8);
     }
   };
-  if ((int)blockIdx.x < ntiles) fetch(blockIdx.x);
+  // XCD-aware tile order: a launch hands blocks to the 8 XCDs round robin,
+  // so with the grid a multiple of 8 the blocks of XCD x walk the x-th
+  // contiguous eighth of the tiles together (neighbouring tiles, whose halo
+  // lines they share, then sit in one L2); otherwise the plain order
+  const int G = gridDim.x;
+  const bool xo = (G & 7) == 0;
+  const int per = xo ? G >> 3 : G;                        // blocks walking one range
+  const int span = xo ? (ntiles + 7) >> 3 : ntiles;       // tiles of one range
+  const int base = xo ? (int)(blockIdx.x & 7) * span : 0;  // first tile of the range
+  const int end = min(base + span, ntiles);
+  const int first = base + (xo ? (int)(blockIdx.x >> 3) : (int)blockIdx.x);
+  if (first < end) fetch(first);
   const int cq = 4 * (lane >> 4);
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t = first; t < end; t += per) {
     const int ty = t / ntx;
     const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
     __syncthreads();  // the previous tile's LDS reads are done
@@ -556,7 +567,7 @@ __global__ __launch_bounds__(256, NO == 16 ? 4 : 1) void k_conv3_tile16p(View x,
     }
     __syncthreads();
     // the last tile of the block re-reads its own tile (cache hits)
-    fetch(t + (int)gridDim.x < ntiles ? t + (int)gridDim.x : t);
+    fetch(t + per < end ? t + per : t);
     f32x4_t acc[4][NB];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
