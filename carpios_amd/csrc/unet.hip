@@ -87,21 +87,30 @@ __global__ __launch_bounds__(256) void k_first_fwd(const float* __restrict__ x,
   if (m >= H * W) return;
   const int i = m / W, j = m - i * W;
   float xv[9], wv[9];
+  // the nine loads from clamped valid addresses, all in flight together
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int si = min(max(i + t / 3 - 1, 0), H - 1), sj = min(max(j + t % 3 - 1, 0), W - 1);
+    xv[t] = x[si * W + sj];
+  }
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
     const int si = i + t / 3 - 1, sj = j + t % 3 - 1;
     const bool in = si >= 0 && si < H && sj >= 0 && sj < W;
-    xv[t] = in ? x[si * W + sj] : 0.f;
+    xv[t] = in ? xv[t] : 0.f;
     wv[t] = in ? wall_code(si, sj, H, W, walls) : 0.f;
   }
+  // explicit fused multiply-adds: the library is built without contraction
+  // (for the FP64 solver's bitwise parity), and 288 separate products and
+  // sums per pixel made this layer VALU-bound
   bf16 out[16];
 #pragma unroll
   for (int co = 0; co < 16; ++co) {
     float s = b[co];
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
-      s += w[(t * 16 + co) * 2] * xv[t];
-      s += w[(t * 16 + co) * 2 + 1] * wv[t];
+      s = __builtin_fmaf(w[(t * 16 + co) * 2], xv[t], s);
+      s = __builtin_fmaf(w[(t * 16 + co) * 2 + 1], wv[t], s);
     }
     out[co] = f2bf(s > 0.f ? s : 0.f);
   }

@@ -353,8 +353,11 @@ def test_learned_preconditioner_gains_over_jacobi_pcg(ctx):
     g = 2
     ui, uj = u_f[g:-g, g:-g], u_j[g:-g, g:-g]
     pi, pj = p_f[g:-g, g:-g], p_j[g:-g, g:-g]
+    # both pressure solves stop at a true relative residual of 1e-8, so
+    # their solutions differ by up to the condition number (~1e4 for the
+    # 256^2 Neumann operator) times that; the fields agree far closer
     assert rel_l2(ui, uj) <= 1e-6
-    assert rel_l2(pi - pi.mean(), pj - pj.mean()) <= 1e-6
+    assert rel_l2(pi - pi.mean(), pj - pj.mean()) <= 1e-5
     # the network cuts the Krylov iterations several-fold against both on a
     # typical step; a step whose residual the network stops reducing hands
     # over to Jacobi-PCG after a 40-iteration window, so no step costs much
