@@ -226,10 +226,13 @@ __global__ __launch_bounds__(256) void k_sub_mean_part(double* __restrict__ x,
 }
 
 // Weighted Jacobi sweep on A z = r (FP32): zout = zin + w D^-1 (r - A zin);
-// zin == nullptr means a zero initial guess (zout = w D^-1 r).
+// zin == nullptr means a zero initial guess (zout = w D^-1 r).  With zadd,
+// the sweep starts from zin + zadd (the sum formed on the fly, as a
+// separate addition kernel would store it).
 __global__ __launch_bounds__(256) void k_jacobi32(Sten<float> s,
                                                   const float* __restrict__ r,
                                                   const float* __restrict__ zin,
+                                                  const float* __restrict__ zadd,
                                                   float* __restrict__ zout,
                                                   float w, Tiling tl) {
   for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
@@ -237,13 +240,14 @@ __global__ __launch_bounds__(256) void k_jacobi32(Sten<float> s,
     tile_origin(t, tl.ntj, i0, j0);
     const int j = j0 + threadIdx.x;
     constexpr int NR = TI / BY;
-    Nbr<float> zn[NR], kc[NR];
+    Nbr<float> zn[NR], za[NR], kc[NR];
     float rr[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const int i = i0 + threadIdx.y + BY * k;
       // (a zero guess reads r in place of zin: the values are not used)
       load_nbr(s, zin ? zin : r, i, j, zn[k], kc[k]);
+      if (zadd) load_nbr(s, zadd, i, j, za[k], kc[k]);
       rr[k] = r[(size_t)min(i, s.nx - 1) * s.ny + min(j, s.ny - 1)];
     }
 #pragma unroll
@@ -251,9 +255,69 @@ __global__ __launch_bounds__(256) void k_jacobi32(Sten<float> s,
       const int i = i0 + threadIdx.y + BY * k;
       if (i < s.nx && j < s.ny) {
         const size_t idx = (size_t)i * s.ny + j;
+        Nbr<float> z = zn[k];
+        if (zadd) {
+          z.c = z.c + za[k].c;
+          z.n = z.n + za[k].n;
+          z.s = z.s + za[k].s;
+          z.w = z.w + za[k].w;
+          z.e = z.e + za[k].e;
+        }
         const float dg = diag_from(s, i, j, kc[k].kx0, kc[k].kx1, kc[k].ky0, kc[k].ky1);
-        const float rv = zin ? rr[k] - apply_regs(s, i, j, dg, zn[k], kc[k]) : rr[k];
-        zout[idx] = (zin ? zn[k].c : 0.f) + w * (rv / dg);
+        const float rv = zin ? rr[k] - apply_regs(s, i, j, dg, z, kc[k]) : rr[k];
+        zout[idx] = (zin ? z.c : 0.f) + w * (rv / dg);
+      }
+    }
+  }
+}
+
+// Two weighted Jacobi sweeps from a zero guess in one pass: the first
+// sweep's values z1 = 0 + w D^-1 r at the five points are formed on the fly
+// (the same arithmetic as k_jacobi32 storing them), then
+// zout = z1 + w D^-1 (r - A z1).
+__global__ __launch_bounds__(256) void k_jacobi32_2z(Sten<float> s,
+                                                     const float* __restrict__ r,
+                                                     float* __restrict__ zout, float w,
+                                                     Tiling tl) {
+  const int nx = s.nx, ny = s.ny;
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+    constexpr int NR = TI / BY;
+    Nbr<float> rn[NR], kc[NR];
+    float kxm[NR], kxp[NR];
+    const int jc = min(j, ny - 1);
+    const float kym = s.KY[max(jc - 1, 0)], kyp = s.KY[min(jc + 2, ny)];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      load_nbr(s, r, i, j, rn[k], kc[k]);
+      const int ic = min(i, nx - 1);
+      kxm[k] = s.KX[max(ic - 1, 0)];
+      kxp[k] = s.KX[min(ic + 2, nx)];
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < nx && j < ny) {
+        const float dg = diag_from(s, i, j, kc[k].kx0, kc[k].kx1, kc[k].ky0, kc[k].ky1);
+        Nbr<float> z1;
+        z1.c = 0.f + w * (rn[k].c / dg);
+        z1.n = i > 0 ? 0.f + w * (rn[k].n / diag_from(s, i - 1, j, kxm[k], kc[k].kx0,
+                                                      kc[k].ky0, kc[k].ky1))
+                     : 0.f;
+        z1.s = i < nx - 1 ? 0.f + w * (rn[k].s / diag_from(s, i + 1, j, kc[k].kx1, kxp[k],
+                                                           kc[k].ky0, kc[k].ky1))
+                          : 0.f;
+        z1.w = j > 0 ? 0.f + w * (rn[k].w / diag_from(s, i, j - 1, kc[k].kx0, kc[k].kx1,
+                                                      kym, kc[k].ky0))
+                     : 0.f;
+        z1.e = j < ny - 1 ? 0.f + w * (rn[k].e / diag_from(s, i, j + 1, kc[k].kx0, kc[k].kx1,
+                                                           kc[k].ky1, kyp))
+                          : 0.f;
+        const float rv = rn[k].c - apply_regs(s, i, j, dg, z1, kc[k]);
+        zout[(size_t)i * ny + j] = z1.c + w * (rv / dg);
       }
     }
   }
@@ -325,12 +389,19 @@ int apply_precond(cpgpu_poisson* P, bool use_net) {
   float* bufA = P->x32.p;
   float* bufB = P->pb32.p;
   float* z = nullptr;  // null: z = 0
-  for (int k = 0; k < P->pc_pre; ++k) {
+  int k0 = 0;
+  if (P->pc_pre >= 2) {  // the first two sweeps from zero in one pass
+    hipLaunchKernelGGL(k_jacobi32_2z, dim3(tl.G), dim3(TJ, BY), 0, st, s, r, bufA, w, tl);
+    z = bufA;
+    k0 = 2;
+  }
+  for (int k = k0; k < P->pc_pre; ++k) {
     float* out = (z == bufA) ? bufB : bufA;
     hipLaunchKernelGGL(k_jacobi32, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
-                       (const float*)z, out, w, tl);
+                       (const float*)z, (const float*)nullptr, out, w, tl);
     z = out;
   }
+  const float* zadd = nullptr;  // a correction the next sweep adds to z
   if (use_net) {
     const float* rn = r;
     if (z) {
@@ -341,11 +412,12 @@ int apply_precond(cpgpu_poisson* P, bool use_net) {
     float* c = P->b32.p;
     int rc = unet_apply_device(P->unet, P->nx, P->ny, rn, c, pack_walls(P));
     if (rc) return rc;
-    if (z) {
-      hipLaunchKernelGGL(k_add32, dim3(1024), dim3(256), 0, st, z, c, P->N);
-    } else {
+    if (!z)
       z = c;
-    }
+    else if (P->pc_post > 0)
+      zadd = c;  // the first post sweep forms z + c
+    else
+      hipLaunchKernelGGL(k_add32, dim3(1024), dim3(256), 0, st, z, c, P->N);
   }
   // post sweeps alternate between two buffers other than z's, the last one
   // writing P->pa32
@@ -355,8 +427,9 @@ int apply_precond(cpgpu_poisson* P, bool use_net) {
     float* out = (left % 2 == 0) ? P->pa32.p : spare;
     if (out == z) out = (z == bufA) ? bufB : bufA;
     hipLaunchKernelGGL(k_jacobi32, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
-                       (const float*)z, out, w, tl);
+                       (const float*)z, zadd, out, w, tl);
     z = out;
+    zadd = nullptr;
   }
   if (!z) z = P->r32.p;  // no sweeps and no network: the identity
   if (z != P->pa32.p)
