@@ -279,14 +279,29 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
       for (int b = 0; b < NB; ++b)
         wpre[tap][b] = F::load(w + ((size_t)(tap * NO + b * 16 + px) * KCH + kq));
   }
-  for (int e = threadIdx.x; e < TP * TP * CPP; e += 256) {
+  // staging: every load of the tile first, from clamped valid addresses
+  // (masked when stored), so a thread's loads are in flight together
+  constexpr int NE = TP * TP * CPP, NPT = (NE + 255) / 256;
+  uint4 pre[NPT];
+  bool pok[NPT];
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int e = min((int)threadIdx.x + 256 * k, NE - 1);
     const int pix = e / CPP, ch = e - pix * CPP;
     const int r = pix / TP, c = pix - r * TP;
     const int gi = ti0 + r - 1, gj = tj0 + c - 1;
-    uint4 v = {0u, 0u, 0u, 0u};
-    if (gi >= 0 && gi < H && gj >= 0 && gj < W)
-      v = *reinterpret_cast<const uint4*>(x.p + ((size_t)gi * W + gj) * x.cs + x.co + ch * 8);
-    *reinterpret_cast<uint4*>(sx + pix * KCH + ch * 8) = v;
+    pok[k] = gi >= 0 && gi < H && gj >= 0 && gj < W;
+    const int ci = min(max(gi, 0), H - 1), cj = min(max(gj, 0), W - 1);
+    pre[k] = *reinterpret_cast<const uint4*>(x.p + ((size_t)ci * W + cj) * x.cs + x.co + ch * 8);
+  }
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int e = threadIdx.x + 256 * k;
+    if (e < NE) {
+      const int pix = e / CPP, ch = e - pix * CPP;
+      *reinterpret_cast<uint4*>(sx + pix * KCH + ch * 8) =
+          pok[k] ? pre[k] : uint4{0u, 0u, 0u, 0u};
+    }
   }
   __syncthreads();
   f32x4_t acc[4][NB];
