@@ -130,27 +130,42 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(View x, View y,
     if (m >= M) pi[a] = -1000000;  // never in range
   }
   const int kq = F::KPL * (lane >> 4);
+  constexpr int NC = CIN / KC;
+  // per tap, every load of the tap (all channel chunks, weights and
+  // activations) is issued before its MFMAs, the activations from clamped
+  // valid addresses and zeroed when outside the image: one memory round trip
+  // per tap instead of one per chunk
 #pragma unroll 1
   for (int tap = 0; tap < 9; ++tap) {
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    typename F::T bf[NC][NB], af[NC][MP];
+    bool ok[MP];
+    size_t off[MP];
 #pragma unroll
-    for (int c0 = 0; c0 < CIN; c0 += KC) {
-      typename F::T bf[NB];
+    for (int a = 0; a < MP; ++a) {
+      const int si = pi[a] + dy, sj = pj[a] + dx;
+      ok[a] = si >= 0 && si < x.H && sj >= 0 && sj < x.W;
+      const int ci = min(max(si, 0), x.H - 1), cj = min(max(sj, 0), x.W - 1);
+      off[a] = (size_t)(ci * x.W + cj) * x.cs + x.co + kq;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         const int co = n0 + b * 16 + (lane & 15);
-        bf[b] = F::load(w + ((size_t)(tap * COUT + co) * CIN + c0 + kq));
+        bf[c][b] = F::load(w + ((size_t)(tap * COUT + co) * CIN + c * KC + kq));
       }
+#pragma unroll
+      for (int a = 0; a < MP; ++a) af[c][a] = F::load(x.p + off[a] + c * KC);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int a = 0; a < MP; ++a) {
-        const int si = pi[a] + dy, sj = pj[a] + dx;
-        typename F::T af = F::zero();
-        if (si >= 0 && si < x.H && sj >= 0 && sj < x.W)
-          af = F::load(x.p + (size_t)(si * x.W + sj) * x.cs + x.co + c0 + kq);
+        const typename F::T av = ok[a] ? af[c][a] : F::zero();
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(bf[b], af, acc[a][b]);
+        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(bf[c][b], av, acc[a][b]);
       }
-    }
   }
   // D[channel][pixel]: four consecutive channels of one pixel per lane
 #pragma unroll
