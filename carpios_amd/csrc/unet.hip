@@ -79,10 +79,13 @@ __device__ __forceinline__ float wall_code(int i, int j, int H, int W, int walls
 // a[m][co] = relu(b + sum_tap W[tap][co][0] x + W[tap][co][1] wall)
 // (branch-free: variants that skip the wall products away from the walls
 // measured slower)
+// With scale given, x is the unscaled input and each value is multiplied by
+// 1 / *scale on the fly (as k_scale_copy would have stored it).
 __global__ __launch_bounds__(256) void k_first_fwd(const float* __restrict__ x,
                                                    int H, int W, int walls, View a,
                                                    const float* __restrict__ w,
-                                                   const float* __restrict__ b) {
+                                                   const float* __restrict__ b,
+                                                   const float* __restrict__ scale) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= H * W) return;
   const int i = m / W, j = m - i * W;
@@ -92,6 +95,11 @@ __global__ __launch_bounds__(256) void k_first_fwd(const float* __restrict__ x,
   for (int t = 0; t < 9; ++t) {
     const int si = min(max(i + t / 3 - 1, 0), H - 1), sj = min(max(j + t % 3 - 1, 0), W - 1);
     xv[t] = x[si * W + sj];
+  }
+  if (scale) {
+    const float sc = 1.f / *scale;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) xv[t] = xv[t] * sc;
   }
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
@@ -646,12 +654,16 @@ void up_fwd(cpgpu_unet* U, View x, View y, int l) {
 // final_out (applying only): the scaled output s y is written there directly
 // when the last layers fuse (*out_done), U->y and the last activation are
 // then not formed.
-int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr) {
+// raw (applying only): the unscaled input, scaled by 1 / U->scale inside the
+// first layer instead of through U->x0.
+int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
+            const float* raw = nullptr) {
   hipStream_t st = U->ctx->stream;
   const int M = U->H * U->W;
   hipLaunchKernelGGL(k_first_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
-                     U->x0.p, U->H, U->W, U->walls, U->v(U->a0),
-                     U->params.p + U->woff[0], U->params.p + U->boff[0]);
+                     raw ? raw : (const float*)U->x0.p, U->H, U->W, U->walls, U->v(U->a0),
+                     U->params.p + U->woff[0], U->params.p + U->boff[0],
+                     raw ? (const float*)U->scale.p : nullptr);
   const bool pooled0 = conv_fwd<16, 16>(U, U->v(U->a0), U->v(U->C0, 16), 1, 1, U->v(U->p0)) & 1;
   auto pool = [&](View in, View out, int C) {
     const int n = out.H * out.W * C;
@@ -864,10 +876,13 @@ int apply_scaled(cpgpu_unet* U, const float* in, float* out, size_t N) {
   hipLaunchKernelGGL(k_sumsq, dim3(1024), dim3(256), 0, st, in, N, U->dpart.p);
   hipLaunchKernelGGL(k_rms, dim3(1), dim3(256), 0, st, U->dpart.p, 1024, N,
                      U->scale.p);
-  hipLaunchKernelGGL(k_scale_copy, dim3(1024), dim3(256), 0, st, in, U->x0.p,
-                     U->scale.p, 1, 1.f, N);
+  // a network only applied (out given) scales its input inside the first
+  // layer; one that a backward pass follows keeps the scaled input in x0
+  if (!out)
+    hipLaunchKernelGGL(k_scale_copy, dim3(1024), dim3(256), 0, st, in, U->x0.p,
+                       U->scale.p, 1, 1.f, N);
   bool done = false;
-  int rc = forward(U, out, &done);
+  int rc = forward(U, out, &done, out ? in : nullptr);
   if (rc) return rc;
   if (out && !done)
     hipLaunchKernelGGL(k_scale_copy, dim3(1024), dim3(256), 0, st, U->y.p, out,

@@ -389,7 +389,11 @@ def test_applied_network_fuses_the_output_layer(ctx):
     P.fill_phi(0.0)
     P.solve("JPCG", max_iter=0, tol=1e-300, norm=ca.NORM_TRUE)  # the warm start only
     warm = og.interior(P.get_phi())
-    x0 = net.tensor("x")[:, :, 0]          # the scaled input the device used
+    # the scaled input of the warm start: the FP32 right-hand side over its
+    # rms (an applied network scales inside its first layer and keeps no copy)
+    b32 = og.interior(P.get_vec(1)).astype(np.float32)
+    s32 = np.float32(np.sqrt(np.sum(b32.astype(np.float64) ** 2) / b32.size))
+    x0 = b32 * (np.float32(1) / s32)
     y = net.forward(x0.astype(np.float32))  # same walls (kept from the apply)
     c = float(np.vdot(warm, y) / np.vdot(y, y))
     print(f"  fused output vs plain forward: scale {c:.6g}, rel L2 {rel_l2(warm, c * y):.2e}")
