@@ -75,6 +75,8 @@ EXPORTS = [
     "cpgpu_ns_field_change", "cpgpu_unet_set_conv_shape", "cpgpu_unet_clear_samples",
     "cpgpu_unet_set_walls", "cpgpu_copy_bandwidth", "cpgpu_csr_create",
     "cpgpu_csr_destroy", "cpgpu_csr_mult", "cpgpu_csr_solve",
+    "cpgpu_poisson_set_alpha", "cpgpu_poisson_set_bc_values", "cpgpu_ns_set_kim",
+    "cpgpu_ns_kim_iters", "cpgpu_ns_set_bc_values",
 ]
 
 
@@ -219,6 +221,14 @@ class Poisson:
         out = np.zeros(self.shape) if out is None else out
         self.ctx.check(lib().cpgpu_poisson_get_phi(self.h, _p(out)))
         return out
+
+    def set_alpha(self, alpha):
+        """Helmholtz rows (Poisson_::set_uniform_alpha, StencilHelmholtz_)."""
+        self.ctx.check(lib().cpgpu_poisson_set_alpha(self.h, C.c_double(alpha)))
+
+    def set_bc_values(self, vals):
+        arr = pack_bc_values(self.nx, self.ny, self.gl, vals)
+        self.ctx.check(lib().cpgpu_poisson_set_bc_values(self.h, _p(arr)))
 
     def apply(self, x_field):
         x = np.ascontiguousarray(x_field, dtype=np.float64)
@@ -407,6 +417,27 @@ class NS:
 
     def set_time(self, t, dt):
         self.ctx.check(lib().cpgpu_ns_set_time(self.h, C.c_double(t), C.c_double(dt)))
+
+    def set_kim(self, enable=True, helm="Jacobi", helm_max_iter=1000,
+                helm_tol=1e-4):
+        """NS_kim_ stepping (s_ns_kim.hpp:130-287): fixed dt, upwind
+        advection, a Helmholtz predictor per component solved by `helm`
+        (Solver_Jacobi_ and the Solver_ limits by default)."""
+        self.ctx.check(lib().cpgpu_ns_set_kim(self.h, int(bool(enable)),
+                                              METHODS[helm], int(helm_max_iter),
+                                              C.c_double(helm_tol)))
+
+    def kim_iters(self):
+        it = np.zeros(2, dtype=np.int32)
+        self.ctx.check(lib().cpgpu_ns_kim_iters(self.h, _p(it, _ip)))
+        return int(it[0]), int(it[1])
+
+    def set_bc_values(self, comp, vals):
+        """New boundary values of velocity component comp ("u"/"v" or 0/1):
+        4 scalars or the packed per-ghost-line array."""
+        c = {"u": 0, "v": 1}.get(comp, comp)
+        arr = pack_bc_values(self.nx, self.ny, self.gl, vals)
+        self.ctx.check(lib().cpgpu_ns_set_bc_values(self.h, int(c), _p(arr)))
 
     def set_solver(self, solver, max_iter=100000, tol=1e-10, norm=NORM_REF):
         self.ctx.check(lib().cpgpu_ns_set_solver(self.h, METHODS[solver],

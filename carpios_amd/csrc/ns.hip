@@ -337,6 +337,8 @@ __device__ __forceinline__ double phi_at(const Axes& a, const double* __restrict
   return x[(size_t)i * a.ny + j];
 }
 
+// KIM: p also loses (0.5/rho) * NablaMuNabla(mu, phi) (s_ns_kim.hpp:180-192).
+template <bool KIM>
 __global__ __launch_bounds__(256) void k_grad_correct(Axes a, const double* __restrict__ x,
                                                       double* __restrict__ usf,
                                                       double* __restrict__ vsf,
@@ -344,7 +346,8 @@ __global__ __launch_bounds__(256) void k_grad_correct(Axes a, const double* __re
                                                       double* __restrict__ vs,
                                                       double* __restrict__ p,
                                                       double* __restrict__ phi,
-                                                      const NSDev* st) {
+                                                      const NSDev* st, double mu,
+                                                      double rho) {
   const int J = blockIdx.x * blockDim.x + threadIdx.x;
   const int I = blockIdx.y * blockDim.y + threadIdx.y;
   if (I >= a.NX || J >= a.NY) return;
@@ -380,11 +383,100 @@ __global__ __launch_bounds__(256) void k_grad_correct(Axes a, const double* __re
   const double gxm = (ph - pxm) / (cax(a, 0, i) - cax(a, 0, i - 1));
   const double gym = (ph - pym) / (cax(a, 1, j) - cax(a, 1, j - 1));
   phi[f] = ph;
-  p[f] = ph * (1.0 / st->dt);
+  double pv = ph * (1.0 / st->dt);
+  if (KIM) {
+    // NablaMuNabla(mu, phi) over phi's Neumann ghosts (s_operation.hpp:664-688)
+    const double dmx = mu * (ph - pxm) / (cax(a, 0, i) - cax(a, 0, i - 1));
+    const double dpx = mu * (pxp - ph) / (cax(a, 0, i + 1) - cax(a, 0, i));
+    const double dmy = mu * (ph - pym) / (cax(a, 1, j) - cax(a, 1, j - 1));
+    const double dpy = mu * (pyp - ph) / (cax(a, 1, j + 1) - cax(a, 1, j));
+    double lap = 0.0;
+    lap += (dpx - dmx) / sax(a, 0, i);
+    lap += (dpy - dmy) / sax(a, 1, j);
+    pv = pv - lap * (0.5 / rho);
+  }
+  p[f] = pv;
   usf[f] = usf[f] - gxp;
   vsf[f] = vsf[f] - gyp;
   us[f] = us[f] - (gxp + gxm) * 0.5;
   vs[f] = vs[f] - (gyp + gym) * 0.5;
+}
+
+// VdotNabla_upwind1 at one cell (s_operation.hpp:400-422), the NS_ default
+// advection (s_ns.hpp:225); faces as in k_c2f_div (uf at (i, j) is the
+// x+ face of cell (i, j)).
+__device__ __forceinline__ double upwind1_cell(const Axes& a, const double* __restrict__ uf,
+                                               const double* __restrict__ vf,
+                                               const double* __restrict__ s, int i, int j) {
+  double t[2] = {0.0, 0.0};
+  const double c = s[fidx(a, i, j)];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1, k = d == 0 ? i : j;
+    const double* f = d == 0 ? uf : vf;
+    const double veo = (f[fidx(a, i, j)] + f[fidx(a, i - ei, j - ej)]) * 0.5;
+    if (veo > 0.0)
+      t[d] = veo * (c - s[fidx(a, i - ei, j - ej)]) / (cax(a, d, k) - cax(a, d, k - 1));
+    else if (veo < 0.0)
+      t[d] = veo * (s[fidx(a, i + ei, j + ej)] - c) / (cax(a, d, k + 1) - cax(a, d, k));
+  }
+  double r = 0.0;
+  r += t[0];
+  r += t[1];
+  return r;
+}
+
+// NS_kim_ predictor source of both components, interior-only (i*ny + j)
+// (s_ns_kim.hpp:238-265): (adv - (0.5*dif + 0)/rho)*dt - u with adv the
+// upwind term at the first step and 1.5*a - 0.5*a' after; a' is the same
+// value because run_one_step copies u into um before the predictor.
+__global__ __launch_bounds__(256) void k_kim_src(Axes a, const double* __restrict__ u,
+                                                 const double* __restrict__ v,
+                                                 const double* __restrict__ uf,
+                                                 const double* __restrict__ vf,
+                                                 double* __restrict__ su,
+                                                 double* __restrict__ sv,
+                                                 const NSDev* st, double rho, double mu,
+                                                 int first) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= a.nx || j >= a.ny) return;
+  const double dt = st->dt;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const double* s = c == 0 ? u : v;
+    const double advn = upwind1_cell(a, uf, vf, s, i, j);
+    const double adv = 1.5 * advn - 0.5 * advn;
+    const double dif = diffusion_cell(a, mu, s, i, j);
+    const double src = 0.0;
+    const double cur = s[fidx(a, i, j)];
+    const double r = first ? (advn - (0.5 * dif + src) / rho) * dt - cur
+                           : (adv - (0.5 * dif + src) / rho) * dt - cur;
+    (c == 0 ? su : sv)[(size_t)i * a.ny + j] = r;
+  }
+}
+
+// interior of two ghost-padded fields <-> two interior-only vectors
+__global__ __launch_bounds__(256) void k_field_to_x2(Axes a, const double* __restrict__ f0,
+                                                     const double* __restrict__ f1,
+                                                     double* __restrict__ x0,
+                                                     double* __restrict__ x1) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= a.nx || j >= a.ny) return;
+  x0[(size_t)i * a.ny + j] = f0[fidx(a, i, j)];
+  x1[(size_t)i * a.ny + j] = f1[fidx(a, i, j)];
+}
+
+__global__ __launch_bounds__(256) void k_x_to_field2(Axes a, const double* __restrict__ x0,
+                                                     const double* __restrict__ x1,
+                                                     double* __restrict__ f0,
+                                                     double* __restrict__ f1) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= a.nx || j >= a.ny) return;
+  f0[fidx(a, i, j)] = x0[(size_t)i * a.ny + j];
+  f1[fidx(a, i, j)] = x1[(size_t)i * a.ny + j];
 }
 
 __global__ void k_advance(NSDev* st) {
@@ -419,6 +511,17 @@ struct cpgpu_ns {
   int last_iters = 0;
   double last_resid = 0.0;
   long long total_iters = 0;
+  // host copies of the axes and velocity boundary values (Helmholtz set-up)
+  std::vector<double> hcx, hsx, hcy, hsy, hbc[2];
+  // NS_kim_ mode: Helmholtz predictor problems (alpha 1, beta dt*0.5/rho),
+  // rebuilt when dt changes
+  bool kim = false;
+  int kim_solver = CPGPU_JACOBI, kim_max_iter = 1000;
+  double kim_tol = 1e-4, kim_dt = -1.0;
+  long long kim_step = 0;
+  cpgpu_poisson* H[2] = {nullptr, nullptr};
+  DevBuf<double> ksrc[2];
+  int helm_iters[2] = {0, 0};
 
   double* fld(int id) { return F[slot[id]].p; }
   Axes axes() const {
@@ -429,6 +532,8 @@ struct cpgpu_ns {
   }
   ~cpgpu_ns() {
     if (P) cpgpu_poisson_destroy(P);
+    for (int c = 0; c < 2; ++c)
+      if (H[c]) cpgpu_poisson_destroy(H[c]);
     if (h_st) (void)hipHostFree(h_st);
   }
 };
@@ -449,6 +554,8 @@ void launch_veo(cpgpu_ns* S, dim3 g, dim3 b) {
                      S->fld(F_U), S->fld(F_V), S->fld(F_UF), S->fld(F_VF),
                      S->fld(F_US), S->fld(F_VS), S->st.p, S->rho, S->mu);
 }
+
+int ns_project(cpgpu_ns* S, bool kim);
 
 int ns_one_step(cpgpu_ns* S) {
   cpgpu_ctx* ctx = S->ctx;
@@ -483,7 +590,17 @@ int ns_one_step(cpgpu_ns* S) {
                      bu, S->fld(F_VS), bv);
   hipLaunchKernelGGL(k_ghost_corners, gcorner, dim3(256), 0, st, a,
                      S->fld(F_US), bu, S->fld(F_VS), bv);
-  // C2F + Div
+  CPGPU_HIP(ctx, hipGetLastError());
+  return ns_project(S, false);
+}
+
+// C2F + Div of u*, the projection solve, grad phi, p, the corrections and
+// the time advance; u <- u*, uf <- u*f.
+int ns_project(cpgpu_ns* S, bool kim) {
+  cpgpu_ctx* ctx = S->ctx;
+  hipStream_t st = ctx->stream;
+  const Axes a = S->axes();
+  const dim3 b2(64, 4), g2((S->ny + 63) / 64, (S->nx + 3) / 4);
   hipLaunchKernelGGL(k_c2f_div, g2, b2, 0, st, a, S->fld(F_US), S->fld(F_VS),
                      S->fld(F_USF), S->fld(F_VSF), S->fld(F_DIV), S->div_int.p);
   CPGPU_HIP(ctx, hipGetLastError());
@@ -501,9 +618,14 @@ int ns_one_step(cpgpu_ns* S) {
   S->total_iters += mi;
   // grad, corrections, p
   const dim3 g3((S->NY + 63) / 64, (S->NX + 3) / 4);
-  hipLaunchKernelGGL(k_grad_correct, g3, b2, 0, st, a, S->P->x.p,
-                     S->fld(F_USF), S->fld(F_VSF), S->fld(F_US), S->fld(F_VS),
-                     S->fld(F_P), S->fld(F_PHI), S->st.p);
+  if (kim)
+    hipLaunchKernelGGL(k_grad_correct<true>, g3, b2, 0, st, a, S->P->x.p,
+                       S->fld(F_USF), S->fld(F_VSF), S->fld(F_US), S->fld(F_VS),
+                       S->fld(F_P), S->fld(F_PHI), S->st.p, S->mu, S->rho);
+  else
+    hipLaunchKernelGGL(k_grad_correct<false>, g3, b2, 0, st, a, S->P->x.p,
+                       S->fld(F_USF), S->fld(F_VSF), S->fld(F_US), S->fld(F_VS),
+                       S->fld(F_P), S->fld(F_PHI), S->st.p, S->mu, S->rho);
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, S->st.p);
   CPGPU_HIP(ctx, hipGetLastError());
   // u <- u*, uf <- u*f by exchanging the buffers
@@ -511,6 +633,91 @@ int ns_one_step(cpgpu_ns* S) {
   std::swap(S->slot[F_V], S->slot[F_VS]);
   std::swap(S->slot[F_UF], S->slot[F_USF]);
   std::swap(S->slot[F_VF], S->slot[F_VSF]);
+  return code;
+}
+
+// The two Helmholtz predictor problems of the current dt (Poisson_ with
+// set_uniform_alpha(1) and uniform beta dt*0.5/rho, s_ns_kim.hpp:267-283),
+// each with its component's boundary conditions.
+int ns_kim_build(cpgpu_ns* S) {
+  const double dt = S->h_st->dt;
+  if (S->H[0] && S->H[1] && S->kim_dt == dt) return CPGPU_OK;
+  for (int c = 0; c < 2; ++c)
+    if (S->H[c]) {
+      cpgpu_poisson_destroy(S->H[c]);
+      S->H[c] = nullptr;
+    }
+  cpgpu_grid2d gd;
+  gd.nx = S->nx; gd.ny = S->ny; gd.gl = S->gl;
+  gd.cx = S->hcx.data(); gd.sx = S->hsx.data();
+  gd.cy = S->hcy.data(); gd.sy = S->hsy.data();
+  gd.beta = dt * 0.5 / S->rho;
+  for (int c = 0; c < 2; ++c) {
+    const int* bt = c == 0 ? S->bcu_type : S->bcv_type;
+    for (int k = 0; k < 4; ++k) gd.bc_type[k] = bt[k];
+    gd.bc_val = S->hbc[c].data();
+    int rc = cpgpu_poisson_create(S->ctx, &gd, &S->H[c]);
+    if (rc) return rc;
+    rc = cpgpu_poisson_set_alpha(S->H[c], 1.0);
+    if (rc) return rc;
+  }
+  S->kim_dt = dt;
+  return CPGPU_OK;
+}
+
+// NS_kim_::run_one_step + Time_::advance (s_ns_kim.hpp:130-205): no
+// restrict_time; the predictor solves start from the current u*.
+int ns_kim_step(cpgpu_ns* S) {
+  cpgpu_ctx* ctx = S->ctx;
+  hipStream_t st = ctx->stream;
+  const Axes a = S->axes();
+  int rc = ns_kim_build(S);
+  if (rc) return rc;
+  const size_t n = (size_t)S->nx * S->ny;
+  if (S->ksrc[0].alloc(n) || S->ksrc[1].alloc(n))
+    return set_error(ctx, CPGPU_ENOMEM, "kim source");
+  BCDev bu{{S->bcu_type[0], S->bcu_type[1], S->bcu_type[2], S->bcu_type[3]},
+           S->bcu_val.p};
+  BCDev bv{{S->bcv_type[0], S->bcv_type[1], S->bcv_type[2], S->bcv_type[3]},
+           S->bcv_val.p};
+  const dim3 gedge(64, 2), gcorner(1, 2);
+  const dim3 b2(64, 4), g2((S->ny + 63) / 64, (S->nx + 3) / 4);
+  hipLaunchKernelGGL(k_ghost_edges, gedge, dim3(256), 0, st, a, S->fld(F_U), bu,
+                     S->fld(F_V), bv);
+  hipLaunchKernelGGL(k_ghost_corners, gcorner, dim3(256), 0, st, a, S->fld(F_U),
+                     bu, S->fld(F_V), bv);
+  hipLaunchKernelGGL(k_kim_src, g2, b2, 0, st, a, S->fld(F_U), S->fld(F_V),
+                     S->fld(F_UF), S->fld(F_VF), S->ksrc[0].p, S->ksrc[1].p,
+                     S->st.p, S->rho, S->mu, S->kim_step == 0 ? 1 : 0);
+  hipLaunchKernelGGL(k_field_to_x2, g2, b2, 0, st, a, S->fld(F_US), S->fld(F_VS),
+                     S->H[0]->x.p, S->H[1]->x.p);
+  CPGPU_HIP(ctx, hipGetLastError());
+  for (int c = 0; c < 2; ++c) {
+    rc = poisson_rhs_from_device(S->H[c], S->ksrc[c].p, 0.0);
+    if (rc) return rc;
+    int mi = S->kim_max_iter;
+    double tol = S->kim_tol;
+    int hl = 0;
+    const int code = poisson_solve_internal(S->H[c], S->kim_solver, &mi, &tol,
+                                            S->norm_mode, 0, nullptr, 0, &hl);
+    if (code < 0) return code;
+    S->helm_iters[c] = mi;
+  }
+  hipLaunchKernelGGL(k_x_to_field2, g2, b2, 0, st, a, S->H[0]->x.p, S->H[1]->x.p,
+                     S->fld(F_US), S->fld(F_VS));
+  hipLaunchKernelGGL(k_ghost_edges, gedge, dim3(256), 0, st, a, S->fld(F_US),
+                     bu, S->fld(F_VS), bv);
+  hipLaunchKernelGGL(k_ghost_corners, gcorner, dim3(256), 0, st, a,
+                     S->fld(F_US), bu, S->fld(F_VS), bv);
+  CPGPU_HIP(ctx, hipGetLastError());
+  const int code = ns_project(S, true);
+  if (code < 0) return code;
+  // the reference copies u* into u (whole arrays); u* keeps the same values
+  const size_t bytes = sizeof(double) * S->NT;
+  for (int k = 0; k < 4; ++k)
+    CPGPU_HIP(ctx, hipMemcpyAsync(S->fld(F_US + k), S->fld(F_U + k), bytes,
+                                  hipMemcpyDeviceToDevice, st));
+  S->kim_step++;
   return code;
 }
 
@@ -581,6 +788,14 @@ int cpgpu_ns_create(cpgpu_ctx* ctx, const cpgpu_ns_desc* d, cpgpu_ns** out) {
     S->bcu_type[s] = d->bcu_type[s];
     S->bcv_type[s] = d->bcv_type[s];
   }
+  S->hcx.assign(g.cx, g.cx + S->NX);
+  S->hsx.assign(g.sx, g.sx + S->NX);
+  S->hcy.assign(g.cy, g.cy + S->NY);
+  S->hsy.assign(g.sy, g.sy + S->NY);
+  S->hbc[0].assign(d->bcu_val ? d->bcu_val : zero.data(),
+                   (d->bcu_val ? d->bcu_val : zero.data()) + nbc);
+  S->hbc[1].assign(d->bcv_val ? d->bcv_val : zero.data(),
+                   (d->bcv_val ? d->bcv_val : zero.data()) + nbc);
   S->rho = d->rho; S->mu = d->mu; S->cfl = d->cfl; S->dnum = d->dnum;
   S->dcoef = d->dcoef; S->limiter = d->limiter; S->solver = d->solver;
   S->max_iter = d->max_iter; S->tol = d->tol; S->norm_mode = d->norm_mode;
@@ -671,7 +886,7 @@ int cpgpu_ns_step(cpgpu_ns* S, int nsteps, int* iters, double* resid) {
   CPGPU_HIP(S->ctx, hipSetDevice(S->ctx->device));
   int code = CPGPU_OK;
   for (int k = 0; k < nsteps; ++k) {
-    code = ns_one_step(S);
+    code = S->kim ? ns_kim_step(S) : ns_one_step(S);
     if (code < 0) return code;
     if (iters) iters[k] = S->last_iters;
     if (resid) resid[k] = S->last_resid;
@@ -700,6 +915,45 @@ int cpgpu_ns_set_time(cpgpu_ns* S, double t, double dt) {
   CPGPU_HIP(S->ctx, hipMemcpyAsync(S->st.p, S->h_st, sizeof(NSDev),
                                    hipMemcpyHostToDevice, S->ctx->stream));
   CPGPU_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
+  return CPGPU_OK;
+}
+
+int cpgpu_ns_set_kim(cpgpu_ns* S, int enable, int helm_solver,
+                     int helm_max_iter, double helm_tol) {
+  if (!S || helm_max_iter < 1 || !(helm_tol > 0.0)) return CPGPU_EINVAL;
+  if (helm_solver != CPGPU_CG && helm_solver != CPGPU_JPCG &&
+      helm_solver != CPGPU_BICGSTAB && helm_solver != CPGPU_JACOBI)
+    return CPGPU_EINVAL;
+  // the explicit steps move dt on the device only: take the device's
+  CPGPU_HIP(S->ctx, hipSetDevice(S->ctx->device));
+  CPGPU_HIP(S->ctx, hipMemcpyAsync(S->h_st, S->st.p, sizeof(NSDev),
+                                   hipMemcpyDeviceToHost, S->ctx->stream));
+  CPGPU_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
+  S->kim = enable != 0;
+  S->kim_solver = helm_solver;
+  S->kim_max_iter = helm_max_iter;
+  S->kim_tol = helm_tol;
+  S->kim_step = 0;
+  return CPGPU_OK;
+}
+
+int cpgpu_ns_kim_iters(cpgpu_ns* S, int* iters) {
+  if (!S || !iters) return CPGPU_EINVAL;
+  iters[0] = S->helm_iters[0];
+  iters[1] = S->helm_iters[1];
+  return CPGPU_OK;
+}
+
+int cpgpu_ns_set_bc_values(cpgpu_ns* S, int comp, const double* bc_val) {
+  if (!S || !bc_val || comp < 0 || comp > 1) return CPGPU_EINVAL;
+  const size_t nbc = 2 * (size_t)S->NY + 2 * (size_t)S->NX;
+  S->hbc[comp].assign(bc_val, bc_val + nbc);
+  CPGPU_HIP(S->ctx, hipSetDevice(S->ctx->device));
+  CPGPU_HIP(S->ctx, hipMemcpyAsync(comp == 0 ? S->bcu_val.p : S->bcv_val.p,
+                                   S->hbc[comp].data(), sizeof(double) * nbc,
+                                   hipMemcpyHostToDevice, S->ctx->stream));
+  CPGPU_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
+  if (S->H[comp]) return cpgpu_poisson_set_bc_values(S->H[comp], bc_val);
   return CPGPU_OK;
 }
 

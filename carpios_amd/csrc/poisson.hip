@@ -999,6 +999,46 @@ int cpgpu_poisson_set_phi(cpgpu_poisson* P, const double* phi_field) {
   return field_to_dev(P, phi_field, P->x.p);
 }
 
+int cpgpu_poisson_set_alpha(cpgpu_poisson* P, double alpha) {
+  if (!P) return CPGPU_EINVAL;
+  // uniform cells only: one shift for every row
+  for (double v : P->sxi)
+    if (v != P->sxi[0]) return set_error(P->ctx, CPGPU_EUNSUPPORTED, "alpha: non-uniform x sizes");
+  for (double v : P->syi)
+    if (v != P->syi[0]) return set_error(P->ctx, CPGPU_EUNSUPPORTED, "alpha: non-uniform y sizes");
+  // v = product of the sizes over d (StencilHelmholtz_::coe_row,
+  // s_poisson.hpp:248-258): the shift v*alpha and the source weight v
+  double v = 1.0;
+  v *= P->sxi[0];
+  v *= P->syi[0];
+  for (int i = 0; i < P->nx; ++i) P->vol[i] = v;
+  P->shift = v * alpha;
+  CPGPU_HIP(P->ctx, hipSetDevice(P->ctx->device));
+  int rc = upload(P->ctx, P->dvol.p, P->vol.data(), sizeof(double) * P->vol.size());
+  if (rc) return rc;
+  CPGPU_HIP(P->ctx, hipStreamSynchronize(P->ctx->stream));
+  return CPGPU_OK;
+}
+
+int cpgpu_poisson_set_bc_values(cpgpu_poisson* P, const double* bc_val) {
+  if (!P || !bc_val) return CPGPU_EINVAL;
+  const int nx = P->nx, ny = P->ny, gl = P->gl;
+  const int NX = nx + 2 * gl, NY = ny + 2 * gl;
+  for (int j = 0; j < ny; ++j) {
+    P->bcv[j] = bc_val[j + gl];
+    P->bcv[ny + j] = bc_val[NY + j + gl];
+  }
+  for (int i = 0; i < nx; ++i) {
+    P->bcv[2 * ny + i] = bc_val[2 * NY + i + gl];
+    P->bcv[2 * ny + nx + i] = bc_val[2 * NY + NX + i + gl];
+  }
+  CPGPU_HIP(P->ctx, hipSetDevice(P->ctx->device));
+  int rc = upload(P->ctx, P->dbcv.p, P->bcv.data(), sizeof(double) * P->bcv.size());
+  if (rc) return rc;
+  CPGPU_HIP(P->ctx, hipStreamSynchronize(P->ctx->stream));
+  return CPGPU_OK;
+}
+
 int cpgpu_poisson_get_phi(cpgpu_poisson* P, double* phi_field) {
   if (!P || !phi_field) return CPGPU_EINVAL;
   return dev_to_field(P, P->x.p, phi_field);

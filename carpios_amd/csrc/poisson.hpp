@@ -124,6 +124,7 @@ struct cpgpu_poisson {
   int nx = 0, ny = 0, gl = 0;
   size_t N = 0;
   double beta = 1.0;
+  double shift = 0.0;  // Helmholtz vol*alpha (0: Poisson)
   cpgpu::Tiling tl{};
   // host copies of the descriptor
   std::vector<double> KX, KY, vol, bcv;
@@ -185,6 +186,7 @@ struct cpgpu_poisson {
     s.KX = dKX.p;
     s.KY = dKY.p;
     for (int k = 0; k < 4; ++k) s.adj[k] = adj[k];
+    s.shift = shift;
     return s;
   }
   cpgpu::Sten<float> sten32() const {
@@ -194,6 +196,7 @@ struct cpgpu_poisson {
     s.KX = fKX.p;
     s.KY = fKY.p;
     for (int k = 0; k < 4; ++k) s.adj[k] = (float)adj[k];
+    s.shift = (float)shift;
     return s;
   }
   ~cpgpu_poisson() {

@@ -32,6 +32,10 @@ struct Sten {
   const T* KX;  // nx+1 face coefficients along x: face f between cells f-1, f
   const T* KY;  // ny+1
   T adj[4];     // diagonal adjustment per boundary segment x-, x+, y-, y+
+  // Helmholtz term vol*alpha added to the centre coefficient before the ghost
+  // substitutions (StencilHelmholtz_::coe_row, s_poisson.hpp:230-259); 0 for
+  // the Poisson row, where d + 0 leaves every diagonal bit-identical
+  T shift = T(0);
 };
 
 constexpr int TJ = 64;  // tile width (j, contiguous) = one wavefront
@@ -63,6 +67,7 @@ __device__ __forceinline__ T diag_at(const Sten<T>& s, int i, int j) {
   T d = s.KX[i] + s.KX[i + 1];
   d = d + s.KY[j];
   d = d + s.KY[j + 1];
+  d = d + s.shift;
   if (i == 0) d = d + s.adj[0];
   if (j == 0) d = d + s.adj[2];
   if (j == s.ny - 1) d = d + s.adj[3];
@@ -77,6 +82,7 @@ __device__ __forceinline__ T diag_from(const Sten<T>& s, int i, int j, T kx0, T 
   T d = kx0 + kx1;
   d = d + ky0;
   d = d + ky1;
+  d = d + s.shift;
   if (i == 0) d = d + s.adj[0];
   if (j == 0) d = d + s.adj[2];
   if (j == s.ny - 1) d = d + s.adj[3];
@@ -777,6 +783,8 @@ __global__ __launch_bounds__(256) void k_cg_dir_roll(
         d = d + ky1;
         dh = dh + hy0;
         dh = dh + hy1;
+        d = d + s.shift;
+        dh = dh + s.shift;
         if (i == 0) {
           d = d + s.adj[0];
           dh = dh + s.adj[0];

@@ -124,6 +124,16 @@ int cpgpu_poisson_set_source_blobs(cpgpu_poisson *P, unsigned long long seed,
 int cpgpu_blob_params(unsigned long long seed, int nblobs, double wmin,
                       double wmax, double *out);
 /* Warm start / result: host cell fields (interior read or written). */
+/* Replaces Poisson_::set_uniform_alpha (s_poisson.hpp:461-467): the rows
+ * become StencilHelmholtz_'s (s_poisson.hpp:185-260): the centre gets
+ * v*alpha before the ghost substitutions and the source weight is v, with
+ * v the product of the cell sizes.  Uniform cells only (else
+ * CPGPU_EUNSUPPORTED).  Call before setting the source. */
+int cpgpu_poisson_set_alpha(cpgpu_poisson *P, double alpha);
+/* New boundary values (packed as cpgpu_grid2d.bc_val) for time-dependent
+ * boundary conditions: the reference samples BoundaryCondition::get_val at
+ * each assembly (s_ghost.hpp:161-165).  Takes effect at the next source. */
+int cpgpu_poisson_set_bc_values(cpgpu_poisson *P, const double *bc_val);
 int cpgpu_poisson_set_phi(cpgpu_poisson *P, const double *phi_field);
 int cpgpu_poisson_get_phi(cpgpu_poisson *P, double *phi_field);
 /* Device-side fill of phi (a zero initial guess without a host transfer). */
@@ -302,6 +312,23 @@ int cpgpu_ns_field_change(cpgpu_ns *S, int which, double *out);
  * residuals into iters/resid (may be NULL).  Returns the last solver code. */
 int cpgpu_ns_step(cpgpu_ns *S, int nsteps, int *iters, double *resid);
 int cpgpu_ns_time(cpgpu_ns *S, double *t, double *dt);
+/* NS_kim_ mode (s_ns_kim.hpp:130-287): when enable is set, cpgpu_ns_step
+ * runs NS_kim_::run_one_step + Time_::advance: no time-step restriction,
+ * the upwind advection of the NS_ default (VdotNabla_upwind1,
+ * s_operation.hpp:400-422) in the Adams-Bashforth form of the reference, a
+ * Helmholtz predictor per component (alpha 1, beta dt*0.5/rho) solved by
+ * helm_solver (CPGPU_JACOBI, the reference's default, or CPGPU_CG /
+ * CPGPU_JPCG / CPGPU_BICGSTAB) within helm_max_iter / helm_tol (Solver_
+ * defaults 1000 and 1e-4, s_solver.hpp:73-74), then the projection with
+ * p = phi/dt - (0.5/rho) mu lap(phi).  The step counter restarts at 0. */
+int cpgpu_ns_set_kim(cpgpu_ns *S, int enable, int helm_solver,
+                     int helm_max_iter, double helm_tol);
+/* Iterations of the last step's u and v Helmholtz solves. */
+int cpgpu_ns_kim_iters(cpgpu_ns *S, int *iters);
+/* New velocity boundary values of component comp (0 u, 1 v), packed as
+ * cpgpu_grid2d.bc_val, for time-dependent BoundaryCondition functions
+ * (sampled by the caller where the reference calls get_val). */
+int cpgpu_ns_set_bc_values(cpgpu_ns *S, int comp, const double *bc_val);
 /* The projection problem (all-Neumann Poisson) of the step. */
 cpgpu_poisson *cpgpu_ns_projection(cpgpu_ns *S);
 /* Restore the time state (Time_ t and dt) of a saved step sequence. */

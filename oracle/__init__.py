@@ -68,6 +68,7 @@ def lib():
         L.orc_nrm2_true.restype = C.c_double
         L.orc_tvd_cell.restype = C.c_double
         L.orc_diffusion_cell.restype = C.c_double
+        L.orc_upwind1_cell.restype = C.c_double
         L.orc_max_magnitude.restype = C.c_double
         _lib = L
     return _lib
@@ -150,7 +151,8 @@ def make_bc(grid, types, values=None):
 class Matrix:
     """Assembled CSR (i-fastest unknowns) plus its right-hand side."""
 
-    def __init__(self, grid, bc, beta=1.0, src_field=None, src_uniform=0.0):
+    def __init__(self, grid, bc, beta=1.0, src_field=None, src_uniform=0.0,
+                 alpha=None):
         n = grid.nx * grid.ny
         nnz_max = 5 * n
         self.n = n
@@ -162,8 +164,12 @@ class Matrix:
         if src_field is not None:
             self._src = np.ascontiguousarray(src_field, dtype=np.float64)
             sp = _ptr(self._src)
-        nnz = lib().orc_poisson_assemble(
-            C.byref(grid.c), C.c_double(beta), sp, C.c_double(src_uniform),
+        # alpha: the StencilHelmholtz_ rows of Poisson_::set_uniform_alpha
+        head = ((lib().orc_poisson_assemble,) if alpha is None else
+                (lib().orc_helmholtz_assemble, C.c_double(alpha)))
+        nnz = head[0](
+            C.byref(grid.c), *head[1:], C.c_double(beta), sp,
+            C.c_double(src_uniform),
             C.byref(bc), _ptr(self.row_ptr, _ip), _ptr(self.col, _ip),
             _ptr(self.val), _ptr(self.b))
         if nnz < 0:
@@ -290,6 +296,55 @@ class NS:
                                  C.byref(it), C.byref(res))
         self.iters.append(it.value)
         return code, it.value, res.value
+
+
+class KimParams(C.Structure):
+    _fields_ = [("method", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double)]
+
+
+SOLVE = {"CG": 0, "JPCG": 1, "Dia_BiCGSTAB": 2, "Jacobi": 3}
+
+
+class NSKim(NS):
+    """NS_kim_ state driven by the oracle (s_ns_kim.hpp:130-287): fixed dt,
+    upwind1 advection, a Helmholtz predictor per component (Solver_Jacobi_
+    with the Solver_ defaults unless told otherwise) and the projection."""
+
+    def __init__(self, grid, bcu, bcv, helm="Jacobi", helm_max_iter=1000,
+                 helm_tol=1e-4, **kw):
+        super().__init__(grid, bcu, bcv, **kw)
+        self.kp = KimParams(SOLVE[helm], helm_max_iter, helm_tol)
+        self.nstep = 0
+        self.helm_iters = []
+        self._build()
+
+    def _build(self):
+        beta = self.dt * 0.5 / self.prm.rho
+        self.Hu = Matrix(self.g, self.bcu, beta=beta, alpha=1.0)
+        self.Hv = Matrix(self.g, self.bcv, beta=beta, alpha=1.0)
+
+    def step(self):
+        it = (C.c_int * 3)()
+        res = C.c_double(0)
+        code = lib().orc_ns_kim_step(
+            C.byref(self.g.c), C.byref(self.A.c), C.byref(self.Hu.c),
+            C.byref(self.Hv.c), C.byref(self.bcu), C.byref(self.bcv),
+            C.byref(self.prm), C.byref(self.kp), self.nstep, C.byref(self.st),
+            it, C.byref(res))
+        self.nstep += 1
+        self.iters.append(it[0])
+        self.helm_iters.append((it[1], it[2]))
+        return code, it[0], res.value
+
+
+def upwind1_field(grid, uf, vf, phi):
+    out = np.zeros((grid.nx, grid.ny))
+    L = lib()
+    for i in range(grid.nx):
+        for j in range(grid.ny):
+            out[i, j] = L.orc_upwind1_cell(C.byref(grid.c), _ptr(uf), _ptr(vf),
+                                           _ptr(phi), i, j)
+    return out
 
 
 def tvd_field(grid, uf, vf, phi, limiter="VanLeer"):

@@ -152,7 +152,7 @@ typedef struct {
  * (s_define.hpp:227-241): (i-1,j) (i,j-1) (i,j) (i,j+1) (i+1,j). */
 static int build_row(const orc_grid *g, double beta, const double *src_field,
                      double src_uniform, const orc_bc *bc, int i, int j,
-                     term_t *t, double *cst) {
+                     term_t *t, double *cst, int helm, double alpha) {
   int nx = g->nx, ny = g->ny, d, n = 0, k;
   double coec = 0.0, cm[2], cp[2], v, src;
   double constant;
@@ -173,6 +173,15 @@ static int build_row(const orc_grid *g, double beta, const double *src_field,
   for (d = 0; d < 2; ++d) v *= sax(g, 0, i);
   src = src_field ? src_field[FIDX(g, i, j)] : src_uniform;
   constant = v * src;
+  if (helm) {
+    /* StencilHelmholtz_::coe_row (s_poisson.hpp:230-259): v is the product
+     * of the sizes over d (not dim 0 twice), the centre term gets v*alpha
+     * before the ghost substitutions and the constant is v*source */
+    v = 1.0;
+    for (d = 0; d < 2; ++d) v *= sax(g, d, d == 0 ? i : j);
+    constant = v * src;
+    coec = coec + v * alpha;
+  }
   /* map-ordered terms, ghost ones flagged by coordinates out of range */
   t[n].di = -1; t[n].dj = 0; t[n].coe = cm[0]; n++;
   t[n].di = 0; t[n].dj = -1; t[n].coe = cm[1]; n++;
@@ -216,10 +225,10 @@ static int build_row(const orc_grid *g, double beta, const double *src_field,
 /* Solver_::_build_matrix (s_solver.hpp:94-161): non-constant terms become
  * CSR entries in map order, b = -constant ("negative added here"); zero
  * coefficients were trimmed by concise. */
-int orc_poisson_assemble(const orc_grid *g, double beta,
+static int assemble_rows(const orc_grid *g, double beta,
                          const double *src_field, double src_uniform,
                          const orc_bc *bc, int *row_ptr, int *col, double *val,
-                         double *b) {
+                         double *b, int helm, double alpha) {
   int nx = g->nx, ny = g->ny, i, j, k, nz = 0;
   term_t t[5];
   double cst;
@@ -228,7 +237,7 @@ int orc_poisson_assemble(const orc_grid *g, double beta,
   for (j = 0; j < ny; ++j) {
     for (i = 0; i < nx; ++i) {
       int row = i + nx * j;
-      build_row(g, beta, src_field, src_uniform, bc, i, j, t, &cst);
+      build_row(g, beta, src_field, src_uniform, bc, i, j, t, &cst, helm, alpha);
       for (k = 0; k < 5; ++k) {
         if (t[k].di == 99 || t[k].coe == 0.0) continue;
         col[nz] = (i + t[k].di) + nx * (j + t[k].dj);
@@ -243,17 +252,47 @@ int orc_poisson_assemble(const orc_grid *g, double beta,
 }
 
 /* Right-hand side only: the constant column of the same rows. */
-void orc_poisson_rhs(const orc_grid *g, double beta, const double *src_field,
-                     double src_uniform, const orc_bc *bc, double *b) {
+static void rhs_rows(const orc_grid *g, double beta, const double *src_field,
+                     double src_uniform, const orc_bc *bc, double *b, int helm,
+                     double alpha) {
   int i, j;
   term_t t[5];
   double cst;
   for (j = 0; j < g->ny; ++j) {
     for (i = 0; i < g->nx; ++i) {
-      build_row(g, beta, src_field, src_uniform, bc, i, j, t, &cst);
+      build_row(g, beta, src_field, src_uniform, bc, i, j, t, &cst, helm, alpha);
       b[i + g->nx * j] = cst == 0.0 ? 0.0 : -cst;
     }
   }
+}
+
+int orc_poisson_assemble(const orc_grid *g, double beta,
+                         const double *src_field, double src_uniform,
+                         const orc_bc *bc, int *row_ptr, int *col, double *val,
+                         double *b) {
+  return assemble_rows(g, beta, src_field, src_uniform, bc, row_ptr, col, val,
+                       b, 0, 0.0);
+}
+
+void orc_poisson_rhs(const orc_grid *g, double beta, const double *src_field,
+                     double src_uniform, const orc_bc *bc, double *b) {
+  rhs_rows(g, beta, src_field, src_uniform, bc, b, 0, 0.0);
+}
+
+/* Poisson_ with set_uniform_alpha (s_poisson.hpp:461-467, :576-592):
+ * StencilHelmholtz_ rows with uniform alpha and beta. */
+int orc_helmholtz_assemble(const orc_grid *g, double alpha, double beta,
+                           const double *src_field, double src_uniform,
+                           const orc_bc *bc, int *row_ptr, int *col,
+                           double *val, double *b) {
+  return assemble_rows(g, beta, src_field, src_uniform, bc, row_ptr, col, val,
+                       b, 1, alpha);
+}
+
+void orc_helmholtz_rhs(const orc_grid *g, double alpha, double beta,
+                       const double *src_field, double src_uniform,
+                       const orc_bc *bc, double *b) {
+  rhs_rows(g, beta, src_field, src_uniform, bc, b, 1, alpha);
 }
 
 void orc_field_to_x(const orc_grid *g, const double *field, double *x) {
@@ -902,33 +941,18 @@ void orc_ns_restrict_time(const orc_grid *g, const orc_ns_params *prm,
  * 61-64).  The projection is the reference's Poisson_ with an empty
  * BoundaryIndex (all-Neumann, value 0), source = div(u*), warm-started from
  * the previous phi, solved here with Jacobi-PCG. */
-int orc_ns_step(const orc_grid *g, const orc_csr *A, const orc_bc *bcu,
-                const orc_bc *bcv, const orc_ns_params *prm, orc_ns_state *s,
-                int *iters, double *resid) {
+/* The projection shared by both steps: ghosts of u*, C2F + Div, the
+ * all-Neumann Poisson solve (J-PCG), grad phi, p and the corrections
+ * (s_ns_explicit.hpp:134-153, s_ns_kim.hpp:160-203). */
+static int ns_project(const orc_grid *g, const orc_csr *A, const orc_bc *bcu,
+                      const orc_bc *bcv, const orc_ns_params *prm,
+                      orc_ns_state *s, int *iters, double *resid, int kim) {
   int nx = g->nx, ny = g->ny, i, j, NT, n = nx * ny, code, k;
   int NX = nx + 2 * g->gl, NY = ny + 2 * g->gl;
-  double dt, *x, *b, *gx, *gy, *cgx, *cgy, tol;
+  double dt = s->dt, *x, *b, *gx, *gy, *cgx, *cgy, tol;
   orc_bc bcphi = {{ORC_BC2, ORC_BC2, ORC_BC2, ORC_BC2}, NULL};
   int mi;
   NT = NX * NY;
-  orc_ns_restrict_time(g, prm, s);
-  dt = s->dt;
-  /* u* at centres */
-  orc_apply_bc(g, s->u, bcu);
-  orc_apply_bc(g, s->v, bcv);
-  for (k = 0; k < 2; ++k) {
-    double *sc = k == 0 ? s->u : s->v;
-    double *ss = k == 0 ? s->us : s->vs;
-    for (j = 0; j < ny; ++j) {
-      for (i = 0; i < nx; ++i) {
-        double adv = orc_tvd_cell(g, s->uf, s->vf, sc, i, j, prm->limiter);
-        double dif = orc_diffusion_cell(g, prm->mu, sc, i, j);
-        double src = 0.0;
-        ss[FIDX(g, i, j)] =
-            (-adv + (dif + src) / prm->rho) * dt + sc[FIDX(g, i, j)];
-      }
-    }
-  }
   orc_apply_bc(g, s->us, bcu);
   orc_apply_bc(g, s->vs, bcv);
   orc_interp_c2f(g, s->us, s->vs, s->usf, s->vsf);
@@ -958,6 +982,15 @@ int orc_ns_step(const orc_grid *g, const orc_csr *A, const orc_bc *bcu,
   memcpy(s->p, s->phi, sizeof(double) * NT);
   for (j = 0; j < ny; ++j)
     for (i = 0; i < nx; ++i) s->p[FIDX(g, i, j)] = s->p[FIDX(g, i, j)] * (1.0 / dt);
+  if (kim) {
+    /* NS_kim_: p -= (0.5/rho) * NablaMuNabla(mu, phi) over the interior
+     * (s_ns_kim.hpp:183-192) */
+    for (j = 0; j < ny; ++j)
+      for (i = 0; i < nx; ++i)
+        s->p[FIDX(g, i, j)] = s->p[FIDX(g, i, j)] -
+                              orc_diffusion_cell(g, prm->mu, s->phi, i, j) *
+                                  (0.5 / prm->rho);
+  }
   for (k = 0; k < NT; ++k) {
     s->usf[k] = s->usf[k] - gx[k];
     s->vsf[k] = s->vsf[k] - gy[k];
@@ -975,3 +1008,121 @@ int orc_ns_step(const orc_grid *g, const orc_csr *A, const orc_bc *bcu,
   s->t += dt;
   return code;
 }
+
+int orc_ns_step(const orc_grid *g, const orc_csr *A, const orc_bc *bcu,
+                const orc_bc *bcv, const orc_ns_params *prm, orc_ns_state *s,
+                int *iters, double *resid) {
+  int nx = g->nx, ny = g->ny, i, j, k;
+  double dt;
+  orc_ns_restrict_time(g, prm, s);
+  dt = s->dt;
+  /* u* at centres */
+  orc_apply_bc(g, s->u, bcu);
+  orc_apply_bc(g, s->v, bcv);
+  for (k = 0; k < 2; ++k) {
+    double *sc = k == 0 ? s->u : s->v;
+    double *ss = k == 0 ? s->us : s->vs;
+    for (j = 0; j < ny; ++j) {
+      for (i = 0; i < nx; ++i) {
+        double adv = orc_tvd_cell(g, s->uf, s->vf, sc, i, j, prm->limiter);
+        double dif = orc_diffusion_cell(g, prm->mu, sc, i, j);
+        double src = 0.0;
+        ss[FIDX(g, i, j)] =
+            (-adv + (dif + src) / prm->rho) * dt + sc[FIDX(g, i, j)];
+      }
+    }
+  }
+  return ns_project(g, A, bcu, bcv, prm, s, iters, resid, 0);
+}
+
+/* VdotNabla_upwind1 at one cell (s_operation.hpp:400-422), the NS_ default
+ * advection (s_ns.hpp:225). */
+double orc_upwind1_cell(const orc_grid *g, const double *uf, const double *vf,
+                        const double *phi, int i, int j) {
+  double t[2] = {0.0, 0.0}, r = 0.0;
+  int d;
+  for (d = 0; d < 2; ++d) {
+    const double *f = d == 0 ? uf : vf;
+    int mi = d == 0 ? i - 1 : i, mj = d == 0 ? j : j - 1;
+    int pi = d == 0 ? i + 1 : i, pj = d == 0 ? j : j + 1;
+    int c = d == 0 ? i : j;
+    double veo = (f[FIDX(g, i, j)] + f[FIDX(g, mi, mj)]) * 0.5;
+    if (veo > 0.0)
+      t[d] = veo * (phi[FIDX(g, i, j)] - phi[FIDX(g, mi, mj)]) /
+             (cax(g, d, c) - cax(g, d, c - 1));
+    else if (veo < 0.0)
+      t[d] = veo * (phi[FIDX(g, pi, pj)] - phi[FIDX(g, i, j)]) /
+             (cax(g, d, c + 1) - cax(g, d, c));
+  }
+  r += t[0];
+  r += t[1];
+  return r;
+}
+
+static int solve_by(int method, const orc_csr *A, double *x, const double *b,
+                    int *mi, double *tol, int norm_mode) {
+  int hl;
+  switch (method) {
+    case ORC_SOLVE_CG: return orc_cg(A, x, b, mi, tol, norm_mode, NULL, 0, &hl);
+    case ORC_SOLVE_JPCG: return orc_jpcg(A, x, b, mi, tol, norm_mode, NULL, 0, &hl);
+    case ORC_SOLVE_BICGSTAB:
+      return orc_dia_bicgstab(A, x, b, mi, tol, norm_mode, NULL, 0, &hl);
+    default: return orc_jacobi(A, x, b, mi, tol, norm_mode, NULL, 0, &hl);
+  }
+}
+
+/* NS_kim_::run_one_step + Time_::advance (s_ns_kim.hpp:130-205, :222-287).
+ * No restrict_time (commented out there).  The predictor's source per
+ * component is (adv - (0.5*dif + 0)/rho)*dt - u with adv = upwind1 at step
+ * 0 and 1.5*a - 0.5*a' after, where a' == a: run_one_step copies u into um
+ * (and uf into umf) before the predictor.  Each component then solves the
+ * Helmholtz problem alpha = 1, beta = dt*0.5/rho (mu does not enter,
+ * :267-272) from the current u* as the first guess; H[k] is that matrix,
+ * assembled by the caller with the component's boundary conditions. */
+int orc_ns_kim_step(const orc_grid *g, const orc_csr *A, const orc_csr *Hu,
+                    const orc_csr *Hv, const orc_bc *bcu, const orc_bc *bcv,
+                    const orc_ns_params *prm, const orc_kim_params *kp,
+                    int step, orc_ns_state *s, int *iters, double *resid) {
+  int nx = g->nx, ny = g->ny, i, j, k, n = nx * ny, code;
+  int NT = (nx + 2 * g->gl) * (ny + 2 * g->gl);
+  double dt = s->dt, beta = dt * 0.5 / prm->rho;
+  double *sscr = (double *)calloc((size_t)NT, sizeof(double));
+  double *x = (double *)malloc(sizeof(double) * n);
+  double *b = (double *)malloc(sizeof(double) * n);
+  orc_apply_bc(g, s->u, bcu);
+  orc_apply_bc(g, s->v, bcv);
+  for (k = 0; k < 2; ++k) {
+    double *sc = k == 0 ? s->u : s->v;
+    double *ss = k == 0 ? s->us : s->vs;
+    int mi = kp->max_iter;
+    double tol = kp->tol;
+    for (j = 0; j < ny; ++j) {
+      for (i = 0; i < nx; ++i) {
+        double advn = orc_upwind1_cell(g, s->uf, s->vf, sc, i, j);
+        double advnm1 = orc_upwind1_cell(g, s->uf, s->vf, sc, i, j);
+        double adv = 1.5 * advn - 0.5 * advnm1;
+        double dif = orc_diffusion_cell(g, prm->mu, sc, i, j);
+        double src = 0.0;
+        double cur = sc[FIDX(g, i, j)];
+        sscr[FIDX(g, i, j)] =
+            step == 0 ? (advn - (0.5 * dif + src) / prm->rho) * dt - cur
+                      : (adv - (0.5 * dif + src) / prm->rho) * dt - cur;
+      }
+    }
+    orc_helmholtz_rhs(g, 1.0, beta, sscr, 0.0, k == 0 ? bcu : bcv, b);
+    orc_field_to_x(g, ss, x);
+    code = solve_by(kp->method, k == 0 ? Hu : Hv, x, b, &mi, &tol,
+                    prm->norm_mode);
+    if (iters) iters[1 + k] = mi;
+    (void)code;
+    orc_x_to_field(g, x, ss);
+  }
+  free(sscr);
+  free(x);
+  free(b);
+  code = ns_project(g, A, bcu, bcv, prm, s, iters, resid, 1);
+  memcpy(s->us, s->u, sizeof(double) * NT);
+  memcpy(s->vs, s->v, sizeof(double) * NT);
+  return code;
+}
+

@@ -89,6 +89,13 @@ int orc_poisson_assemble(const orc_grid *g, double beta,
                          double *b);
 void orc_poisson_rhs(const orc_grid *g, double beta, const double *src_field,
                      double src_uniform, const orc_bc *bc, double *b);
+int orc_helmholtz_assemble(const orc_grid *g, double alpha, double beta,
+                           const double *src_field, double src_uniform,
+                           const orc_bc *bc, int *row_ptr, int *col,
+                           double *val, double *b);
+void orc_helmholtz_rhs(const orc_grid *g, double alpha, double beta,
+                       const double *src_field, double src_uniform,
+                       const orc_bc *bc, double *b);
 void orc_field_to_x(const orc_grid *g, const double *field, double *x);
 void orc_x_to_field(const orc_grid *g, const double *x, double *field);
 
@@ -164,6 +171,24 @@ int orc_ns_step(const orc_grid *g, const orc_csr *A, const orc_bc *bcu,
                 int *iters, double *resid);
 void orc_ns_restrict_time(const orc_grid *g, const orc_ns_params *prm,
                           orc_ns_state *s);
+
+/* ---- NS_kim_ semi-implicit step ---- */
+#define ORC_SOLVE_CG 0
+#define ORC_SOLVE_JPCG 1
+#define ORC_SOLVE_BICGSTAB 2
+#define ORC_SOLVE_JACOBI 3 /* Solver_Jacobi_, the predictor's default */
+typedef struct {
+  int method;   /* ORC_SOLVE_* of the Helmholtz predictor */
+  int max_iter; /* Solver_ defaults: 1000 and 1e-4 (s_solver.hpp:73-74) */
+  double tol;
+} orc_kim_params;
+double orc_upwind1_cell(const orc_grid *g, const double *uf, const double *vf,
+                        const double *phi, int i, int j);
+/* iters[0] projection, iters[1..2] the u and v Helmholtz solves */
+int orc_ns_kim_step(const orc_grid *g, const orc_csr *A, const orc_csr *Hu,
+                    const orc_csr *Hv, const orc_bc *bcu, const orc_bc *bcv,
+                    const orc_ns_params *prm, const orc_kim_params *kp,
+                    int step, orc_ns_state *s, int *iters, double *resid);
 
 #ifdef __cplusplus
 }

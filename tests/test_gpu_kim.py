@@ -1,0 +1,119 @@
+"""Parity of the device NS_kim_ step and the Helmholtz rows (through the
+C-ABI) with the oracle's restatement (s_ns_kim.hpp:130-287,
+s_poisson.hpp:185-260, :461-467)."""
+import numpy as np
+import pytest
+
+import carpios_amd as ca
+import oracle as O
+from test_kim_oracle import tg_bc_values, tg_exact
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+@pytest.mark.parametrize("nx,ny,alpha,beta,method", [
+    (40, 24, 1.0, 0.02, "Jacobi"), (33, 57, -0.5, 0.3, "JPCG"),
+    (64, 64, 2.0, 5e-4, "CG"), (20, 36, 1.0, 0.1, "Dia_BiCGSTAB")])
+def test_helmholtz_matches_oracle(ctx, nx, ny, alpha, beta, method):
+    types = [O.BC1, O.BC2, O.BC1, O.BC2]
+    vals = [0.3, -0.2, 1.0, 0.5]
+    g = O.StructGrid(nx, ny, 1, 0.0, 2.0, 0.0, 1.0)
+    rng = np.random.default_rng(nx * 100 + ny)
+    f = g.field()
+    g.interior(f)[:] = rng.standard_normal((nx, ny))
+    M = O.Matrix(g, O.make_bc(g, types, vals), beta=beta, src_field=f,
+                 alpha=alpha)
+    fn = {"Jacobi": O.lib().orc_jacobi, "JPCG": O.lib().orc_jpcg,
+          "CG": O.lib().orc_cg, "Dia_BiCGSTAB": O.lib().orc_dia_bicgstab}[method]
+    import ctypes as C
+    x = np.zeros(nx * ny)
+    mi, tol, hl = C.c_int(20000), C.c_double(1e-11), C.c_int(0)
+    code = fn(C.byref(M.c), O._ptr(x), O._ptr(M.b), C.byref(mi), C.byref(tol),
+              O.NORM_REF, None, 0, C.byref(hl))
+    desc = ca.uniform_grid_desc(nx, ny, 1, types, vals, beta=beta,
+                                xlim=(0.0, 2.0), ylim=(0.0, 1.0))
+    P = ca.Poisson(ctx, desc)
+    P.set_alpha(alpha)
+    P.set_source(f)
+    gcode, git, gtol, _ = P.solve(method, max_iter=20000, tol=1e-11)
+    print(f"helmholtz {nx}x{ny} {method}: oracle {code}/{mi.value}, gpu {gcode}/{git}",
+          flush=True)
+    assert gcode == code
+    assert abs(git - mi.value) <= 1
+    assert rel(g.interior(P.get_phi()), O.x_to_field(g, x)[1:-1, 1:-1]) < 1e-10
+
+
+def kim_pair(ctx, n, mu=0.01, dt=2e-3, helm="Jacobi", bc="cavity"):
+    if bc == "cavity":
+        g = O.StructGrid(n, n, 2, 0.0, 1.0, 0.0, 1.0)
+        uv, vv = [0, 0, 0, 1.0], None
+        lim = (0.0, 1.0)
+    else:
+        g = O.StructGrid(n, n, 2, 0.0, np.pi, 0.0, np.pi)
+        uv, vv = tg_bc_values(g, 0.0)
+        lim = (0.0, np.pi)
+    bcu = O.make_bc(g, [O.BC1] * 4, uv)
+    bcv = O.make_bc(g, [O.BC1] * 4, vv)
+    ora = O.NSKim(g, bcu, bcv, helm=helm, rho=1.0, mu=mu, tol=1e-10, dt=dt)
+    gpu = ca.NS(ctx, n, n, 2, lim, lim, bcu=((1, 1, 1, 1), uv),
+                bcv=((1, 1, 1, 1), vv), rho=1.0, mu=mu, solver="JPCG",
+                tol=1e-10, dt=dt)
+    gpu.set_kim(True, helm=helm)
+    return g, ora, gpu
+
+
+@pytest.mark.parametrize("n,helm", [(32, "Jacobi"), (48, "JPCG")])
+def test_kim_cavity_matches_oracle(ctx, n, helm):
+    g, ora, gpu = kim_pair(ctx, n, helm=helm)
+    for k in range(8):
+        code, it, res = ora.step()
+        git, gres = gpu.step(1)
+        hi = gpu.kim_iters()
+        print(f"kim cavity n={n} {helm} step {k}: proj {it}/{git[0]} "
+              f"helm {ora.helm_iters[-1]}/{hi}", flush=True)
+        assert abs(int(git[0]) - it) <= 1
+        assert all(abs(a - b) <= 1 for a, b in zip(hi, ora.helm_iters[-1]))
+    for name in ("u", "v", "p", "phi", "us", "vs"):
+        r = rel(g.interior(gpu.get_field(name)), g.interior(ora.f[name]))
+        assert r < 1e-10, (name, r)
+    t, dt = gpu.time()
+    assert t == pytest.approx(ora.st.t) and dt == ora.st.dt
+
+
+def test_kim_taylor_green_matches_oracle_and_exact(ctx):
+    n, dt, steps = 32, 0.01, 10
+    g, ora, gpu = kim_pair(ctx, n, mu=1.0, dt=dt, bc="tg")
+    X, Y = g.centres()
+    XF, YF = np.meshgrid(g.ax.c, g.ay.c, indexing="ij")
+    h = np.pi / n
+    u0, v0, _ = tg_exact(X, Y, 0.0)
+    for name, val in (("u", u0), ("v", v0)):
+        f = g.field()
+        g.interior(f)[:] = val
+        g.interior(ora.f[name])[:] = val
+        gpu.set_field(name, f)
+    uf = tg_exact(XF + h / 2, YF, 0.0)[0]
+    vf = tg_exact(XF, YF + h / 2, 0.0)[1]
+    ora.f["uf"][:] = uf
+    ora.f["vf"][:] = vf
+    gpu.set_field("uf", uf)
+    gpu.set_field("vf", vf)
+    for k in range(steps):
+        vu, vv = tg_bc_values(g, (k + 1) * dt)
+        ora.bcu = O.make_bc(g, [O.BC1] * 4, vu)
+        ora.bcv = O.make_bc(g, [O.BC1] * 4, vv)
+        gpu.set_bc_values("u", vu)
+        gpu.set_bc_values("v", vv)
+        ora.step()
+        gpu.step(1)
+    ue, ve, _ = tg_exact(X, Y, steps * dt)
+    for name in ("u", "v", "p"):
+        r = rel(g.interior(gpu.get_field(name)), g.interior(ora.f[name]))
+        assert r < 1e-10, (name, r)
+    err = np.max(np.abs(g.interior(gpu.get_field("u")) - ue))
+    assert err < 0.02 * np.exp(-2 * steps * dt)
