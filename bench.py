@@ -155,6 +155,61 @@ def batched(ctx, ca, tol, no_cpu, learned_net=True):
     return res
 
 
+def kim_probe(ctx, ca, no_cpu, n=512, steps=10):
+    """The NS_kim_ step (cpgpu_ns_set_kim) on the n^2 lid-driven cavity,
+    Re 1000 (rho 1, mu 1e-3), fixed dt = 0.5 h: upwind predictor source, two
+    Helmholtz solves with the reference's default Solver_Jacobi_ (1000
+    iterations, 1e-4) and the J-PCG projection to 1e-10.  The first step
+    from rest is timed alone, then `steps` more; the oracle's first step at
+    the same size is the CPU baseline."""
+    import time as _t
+    h = 1.0 / n
+    kw = dict(bcu=((1, 1, 1, 1), [0, 0, 0, 1.0]), bcv=((1, 1, 1, 1), None),
+              rho=1.0, mu=1e-3, solver="JPCG", tol=1e-10, dt=0.5 * h)
+    warm = ca.NS(ctx, 64, 64, 2, **kw)
+    warm.set_kim(True)
+    warm.step(2)
+    ns = ca.NS(ctx, n, n, 2, **kw)
+    ns.set_kim(True)
+    ctx.sync()
+    t0 = _t.perf_counter()
+    it0, _ = ns.step(1)
+    ctx.sync()
+    first_ms = (_t.perf_counter() - t0) * 1e3
+    helm0 = ns.kim_iters()
+    t0 = _t.perf_counter()
+    its = []
+    helm = []
+    for _ in range(steps):
+        it, _ = ns.step(1)
+        its.append(int(it[0]))
+        helm.append(ns.kim_iters())
+    ctx.sync()
+    ms = (_t.perf_counter() - t0) * 1e3 / steps
+    out = {"workload": f"{n}x{n} cavity Re=1000, NS_kim_ step, dt=0.5h fixed",
+           "first_step_ms": round(first_ms, 3), "ms_per_step": round(ms, 3),
+           "steps_per_s": round(1e3 / ms, 2),
+           "projection_iterations": [int(it0[0])] + its,
+           "helmholtz_iterations": [list(helm0)] + [list(x) for x in helm]}
+    if not no_cpu:
+        import oracle as O
+        g = O.StructGrid(n, n, 2, 0.0, 1.0, 0.0, 1.0)
+        ora = O.NSKim(g, O.make_bc(g, [O.BC1] * 4, [0, 0, 0, 1.0]),
+                      O.make_bc(g, [O.BC1] * 4), rho=1.0, mu=1e-3, tol=1e-10,
+                      dt=0.5 * h)
+        t0 = _t.perf_counter()
+        ora.step()
+        cpu_ms = (_t.perf_counter() - t0) * 1e3
+        out["cpu_baseline"] = {"first_step_ms": round(cpu_ms, 1), "cores": 1,
+                               "kind": "port",
+                               "sample": "the oracle's first NS_kim_ step from rest at the same size"}
+        out["first_step_speedup_vs_cpu"] = round(cpu_ms / first_ms, 1)
+        out["first_step_iterations_match"] = (ora.iters[0] == int(it0[0]) and
+                                              tuple(ora.helm_iters[0]) == tuple(helm0))
+    log(f"bench: kim {out}")
+    return out
+
+
 def csr_probe(ctx, ca, no_cpu, m=2048, iters=200):
     """The general CSR path (cpgpu_csr: the algebra solvers' MatrixSCR_ on
     the device) on the m^2 five-point Dirichlet Laplacian as an explicit CSR
@@ -228,6 +283,8 @@ def main():
                     help="skip the 256-problem batched solve")
     ap.add_argument("--no-batch-learned", action="store_true",
                     help="skip the U-Net variant of the batched solve (its training)")
+    ap.add_argument("--no-kim", action="store_true",
+                    help="skip the NS_kim_ step probe")
     ap.add_argument("--no-csr", action="store_true",
                     help="skip the general-CSR Jacobi-PCG probe")
     ap.add_argument("--no-hbm", action="store_true",
@@ -409,6 +466,7 @@ def main():
     if not a.no_batch:
         batch = batched(ctx, ca, tol, a.no_cpu, not a.no_batch_learned)
     csr = None if a.no_csr else csr_probe(ctx, ca, a.no_cpu)
+    kim = None if a.no_kim else kim_probe(ctx, ca, a.no_cpu)
     out = {
         "metric": "Poisson solves/s to rel-res 1e-8 @1024^2: learned+FP32/FP64-refine vs plain FP64",
         "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,
@@ -439,6 +497,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_256": batch,
         "csr_jpcg": csr,
+        "ns_kim_512": kim,
     }
     print(json.dumps(out), flush=True)
     net.close()
