@@ -165,7 +165,9 @@ def make_grid_desc(cx, sx, cy, sy, nx, ny, gl, bc_type, bc_val=None,
     for k in range(4):
         g.bc_type[k] = int(bc_type[k])
     if bc_val is not None:
-        bv = np.ascontiguousarray(bc_val, dtype=np.float64)
+        # 4 per-segment scalars or the packed per-ghost-line array: the
+        # library reads 2*(nx+2gl) + 2*(ny+2gl) values
+        bv = pack_bc_values(nx, ny, gl, bc_val)
         keep.append(bv)
         g.bc_val = _p(bv)
     g._keep = keep
