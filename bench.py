@@ -210,6 +210,120 @@ def kim_probe(ctx, ca, no_cpu, n=512, steps=10):
     return out
 
 
+def ns_explicit_probe(ctx, ca, no_cpu, n=512, steps=100, warm=2):
+    """The plain method on the GPU: the n^2 lid-driven cavity (Re 1000,
+    rho = Re, mu = 1, van Leer TVD, CFL 0.5, dt0 = 0.5 h, diffusion number
+    0.25) stepped by NS_explicit_ with FP64 Jacobi-PCG projections to 1e-10
+    (the reference's nrm2 test), from rest.  `warm` steps, then `steps`
+    timed; then one more step with the NS kernels' own dispatch timestamps
+    against their algorithmic bytes; the oracle's first step at the same
+    size is the CPU baseline."""
+    h = 1.0 / n
+    kw = dict(bcu=((1, 1, 1, 1), [0, 0, 0, 1.0]), bcv=((1, 1, 1, 1), None),
+              rho=1000.0, mu=1.0, cfl=0.5, dnum=0.25, dcoef=1e-3,
+              limiter="VanLeer", solver="JPCG", tol=1e-10, norm=ca.NORM_REF,
+              dt=0.5 * h)
+    ns = ca.NS(ctx, n, n, 2, (-0.5, 0.5), (-0.5, 0.5), **kw)
+    P = ns.projection()
+    tune = P.autotune(reps=20)
+    it0, _ = ns.step(warm)
+    ctx.sync()
+    t0 = time.perf_counter()
+    its, res = ns.step(steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    N = n * n
+    fam = {4: ("k_veo_star", 48), 5: ("k_c2f_div", 48), 6: ("k_grad_correct", 88)}
+    for f in fam:
+        ctx.timer_enable(f)
+    ns.step(1)
+    kern = {}
+    for f, (name, bpc) in fam.items():
+        k, ms = ctx.timer_read(f)
+        ctx.timer_enable(f, False)
+        us = ms * 1e3 / max(k, 1)
+        gbs = bpc * N / (us * 1e-6) / 1e9
+        kern[name] = {"us": round(us, 3), "bytes_per_cell": bpc, "GBs": round(gbs, 1),
+                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    t, dt = ns.time()
+    it_sum = int(np.sum(its))
+    out = {"workload": f"{n}x{n} lid-driven cavity Re=1000, NS_explicit_ (van Leer, CFL 0.5, "
+                       f"dt0=0.5h), FP64 J-PCG projection to 1e-10 (reference nrm2)",
+           "steps": steps, "warmup": warm, "ms_per_step": round(wall / steps * 1e3, 3),
+           "steps_per_s": round(steps / wall, 2), "dt": dt, "t_end": t,
+           "projection_iterations": {"first": [int(v) for v in it0], "mean": round(it_sum / steps, 1),
+                                     "last": int(its[-1])},
+           "jpcg_algorithmic_GBs": round(it_sum * 10 * N * 8 / wall / 1e9, 1),
+           "autotune": tune, "kernels": kern,
+           "note": "kernel figures: L2/MALL-resident at this size (16 fields of 2.1 MB)"}
+    if not no_cpu:
+        import oracle as O
+        g = O.StructGrid(n, n, 2, -0.5, 0.5, -0.5, 0.5)
+        ora = O.NS(g, O.make_bc(g, [1] * 4, [0, 0, 0, 1.0]), O.make_bc(g, [1] * 4),
+                   rho=1000.0, mu=1.0, cfl=0.5, dnum=0.25, dcoef=1e-3, limiter="VanLeer",
+                   tol=1e-10, dt=0.5 * h)
+        c0 = time.perf_counter()
+        _, it_cpu, _ = ora.step()
+        cpu_s = time.perf_counter() - c0
+        out["cpu_baseline"] = {"first_step_ms": round(cpu_s * 1e3, 1), "cores": 1, "kind": "port",
+                               "iterations": it_cpu,
+                               "sample": "the oracle's first NS_explicit_ step from rest at the "
+                                         "same size (C, -O2, one thread)"}
+        out["first_step_iterations_match"] = abs(int(it0[0]) - it_cpu) <= 1
+    ns.close()
+    log(f"bench: NS_explicit_ {n}^2: {out}")
+    return out
+
+
+def blob_probe(ctx, ca, no_cpu, n=4096, tol=1e-10):
+    """The precision configuration: n^2 all-Neumann Poisson, 64 zero-meaned
+    Gaussian blobs (seed 0), solved from zero to the true relative residual
+    `tol` by plain FP64 Jacobi-PCG and by FP32 Jacobi-PCG inside FP64
+    residual replacement with the zero-mean projection; time to solution,
+    iterations, algorithmic GB/s (11 N s bytes per iteration, the scope
+    table's figure), and the zero-mean L2 difference of the two."""
+    P = ca.Poisson(ctx, ca.uniform_grid_desc(n, n, 1, [ca.BC_NEUMANN] * 4))
+    P.set_source_blobs(seed=0, nblobs=64, wmin=0.01, wmax=0.1)
+    tune = P.autotune(reps=5)
+    N = n * n
+    res = {"grid": f"{n}x{n}", "tol": tol, "autotune": tune}
+    sols = {}
+    for method, s in (("JPCG", 8), ("MIXED", 4)):
+        P.fill_phi(0.0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        code, it, r, _ = P.solve(method, max_iter=10 ** 7, tol=tol, norm=ca.NORM_TRUE,
+                                 opts=ca.OPT_PROJECT_MEAN)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        true_res = P.residual()
+        if code != ca.OK or true_res > tol:
+            raise RuntimeError(f"blob {method}: code {code}, true residual {true_res:.3e}")
+        x = P.get_phi()[1:-1, 1:-1]
+        sols[method] = x - x.mean()
+        gbs = it * 11 * N * s / wall / 1e9
+        res[method] = {"seconds": round(wall, 4), "iterations": it, "true_residual": true_res,
+                       "algorithmic_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"bench: blob {n}^2 {method}: {res[method]}")
+    res["MIXED"]["speedup_vs_fp64"] = round(res["JPCG"]["seconds"] / res["MIXED"]["seconds"], 3)
+    res["MIXED"]["rel_l2_vs_fp64"] = rel_l2(sols["MIXED"], sols["JPCG"])
+    if not no_cpu:
+        import oracle as O
+        og = O.StructGrid(n, n, 1)
+        A = O.Matrix(og, O.make_bc(og, [2] * 4), 1.0)
+        b = O.field_to_x(og, P.get_vec(1))
+        k = 10
+        c0 = time.perf_counter()
+        O.solve("JPCG", A, b, max_iter=k, tol=1e-300, norm=O.NORM_TRUE)
+        t_it = (time.perf_counter() - c0) / k
+        res["cpu_baseline"] = {"seconds_to_solution": round(t_it * res["JPCG"]["iterations"], 1),
+                               "ms_per_iteration": round(t_it * 1e3, 2), "cores": 1, "kind": "port",
+                               "sample": f"{k} oracle FP64 J-PCG iterations at {n}^2 (C, -O2, one "
+                                         f"thread), times the GPU FP64 iteration count"}
+    P.close()
+    return res
+
+
 def csr_probe(ctx, ca, no_cpu, m=2048, iters=200):
     """The general CSR path (cpgpu_csr: the algebra solvers' MatrixSCR_ on
     the device) on the m^2 five-point Dirichlet Laplacian as an explicit CSR
@@ -289,6 +403,12 @@ def main():
                     help="skip the general-CSR Jacobi-PCG probe")
     ap.add_argument("--no-hbm", action="store_true",
                     help="skip the 4096^2 Krylov-kernel HBM probe")
+    ap.add_argument("--only", default="",
+                    help="comma-separated legs to run alone (ns, blob, kim, csr, batch)")
+    ap.add_argument("--no-ns", action="store_true",
+                    help="skip the 512^2 NS_explicit_ cavity run")
+    ap.add_argument("--no-blob", action="store_true",
+                    help="skip the 4096^2 blob solve (FP64 vs FP32 + FP64 refinement)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch Jacobi-PCG batches directly instead of as "
                          "replayed hipGraphs (same kernels; for profilers)")
@@ -301,6 +421,21 @@ def main():
 
     import carpios_amd as ca
     from carpios_amd import learned
+
+    if a.only:
+        # the named legs alone, one JSON line of their results (no headline)
+        ctx = ca.Context(0)
+        legs = {"ns": lambda: ns_explicit_probe(ctx, ca, a.no_cpu),
+                "blob": lambda: blob_probe(ctx, ca, a.no_cpu),
+                "kim": lambda: kim_probe(ctx, ca, a.no_cpu),
+                "csr": lambda: csr_probe(ctx, ca, a.no_cpu),
+                "batch": lambda: batched(ctx, ca, a.tol, a.no_cpu, not a.no_batch_learned)}
+        out = {}
+        for name in a.only.split(","):
+            out[name] = legs[name]()
+        print(json.dumps(out), flush=True)
+        ctx.close()
+        return
 
     n, K, tol = a.size, a.steps, a.tol
     N = n * n
@@ -467,6 +602,8 @@ def main():
         batch = batched(ctx, ca, tol, a.no_cpu, not a.no_batch_learned)
     csr = None if a.no_csr else csr_probe(ctx, ca, a.no_cpu)
     kim = None if a.no_kim else kim_probe(ctx, ca, a.no_cpu)
+    nsx = None if a.no_ns else ns_explicit_probe(ctx, ca, a.no_cpu)
+    blob = None if a.no_blob else blob_probe(ctx, ca, a.no_cpu)
     out = {
         "metric": "Poisson solves/s to rel-res 1e-8 @1024^2: learned+FP32/FP64-refine vs plain FP64",
         "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,
@@ -498,6 +635,8 @@ def main():
         "batched_256": batch,
         "csr_jpcg": csr,
         "ns_kim_512": kim,
+        "ns_explicit_512": nsx,
+        "blob_4096": blob,
     }
     print(json.dumps(out), flush=True)
     net.close()

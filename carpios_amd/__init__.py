@@ -341,7 +341,12 @@ def pack_bc_values(nx, ny, gl, values):
     if np.ndim(values) == 1 and len(values) == 4:
         return np.concatenate([np.full(NY, values[0]), np.full(NY, values[1]),
                                np.full(NX, values[2]), np.full(NX, values[3])])
-    return np.ascontiguousarray(values, dtype=np.float64)
+    arr = np.ascontiguousarray(values, dtype=np.float64)
+    # the library reads exactly 2*(ny+2gl) + 2*(nx+2gl) values
+    if arr.ndim != 1 or arr.size != 2 * NY + 2 * NX:
+        raise ValueError(f"boundary values: expected 4 scalars or a 1-D array of "
+                         f"{2 * NY + 2 * NX} per-ghost-line values, got shape {arr.shape}")
+    return arr
 
 
 class NS:

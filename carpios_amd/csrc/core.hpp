@@ -24,9 +24,10 @@ struct cpgpu_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   // kernel timers: per family a list of (start, stop) event pairs
-  bool timer_on[4] = {false, false, false, false};
-  std::vector<hipEvent_t> ev[4];
-  size_t ev_used[4] = {0, 0, 0, 0};
+  static constexpr int kFamilies = 8;
+  bool timer_on[kFamilies] = {};
+  std::vector<hipEvent_t> ev[kFamilies];
+  size_t ev_used[kFamilies] = {};
 };
 
 namespace cpgpu {

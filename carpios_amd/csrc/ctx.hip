@@ -142,14 +142,14 @@ int cpgpu_copy_bandwidth(cpgpu_ctx* ctx, size_t bytes, int reps, double* gbs, do
 }
 
 int cpgpu_timer_enable(cpgpu_ctx* ctx, int family, int on) {
-  if (!ctx || family < 0 || family >= 4) return CPGPU_EINVAL;
+  if (!ctx || family < 0 || family >= cpgpu_ctx::kFamilies) return CPGPU_EINVAL;
   ctx->timer_on[family] = on != 0;
   ctx->ev_used[family] = 0;
   return CPGPU_OK;
 }
 
 int cpgpu_timer_read(cpgpu_ctx* ctx, int family, int* launches, double* ms) {
-  if (!ctx || family < 0 || family >= 4 || !launches || !ms)
+  if (!ctx || family < 0 || family >= cpgpu_ctx::kFamilies || !launches || !ms)
     return CPGPU_EINVAL;
   CPGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
   double tot = 0.0;

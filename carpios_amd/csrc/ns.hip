@@ -428,10 +428,15 @@ __device__ __forceinline__ double upwind1_cell(const Axes& a, const double* __re
 
 // NS_kim_ predictor source of both components, interior-only (i*ny + j)
 // (s_ns_kim.hpp:238-265): (adv - (0.5*dif + 0)/rho)*dt - u with adv the
-// upwind term at the first step and 1.5*a - 0.5*a' after; a' is the same
-// value because run_one_step copies u into um before the predictor.
+// upwind term at the first step and 1.5*a - 0.5*a' after.  a' is the upwind
+// term of um on umf: run_one_step copies u (ghosts included) into um and uf
+// into umf before apply_bc(u) refreshes u's ghosts (:132-133, :225-227), so
+// um differs from u in its ghost cells only (they keep the walls applied to
+// the previous step's u* before its correction) and umf equals uf.
 __global__ __launch_bounds__(256) void k_kim_src(Axes a, const double* __restrict__ u,
                                                  const double* __restrict__ v,
+                                                 const double* __restrict__ um,
+                                                 const double* __restrict__ vm,
                                                  const double* __restrict__ uf,
                                                  const double* __restrict__ vf,
                                                  double* __restrict__ su,
@@ -446,7 +451,8 @@ __global__ __launch_bounds__(256) void k_kim_src(Axes a, const double* __restric
   for (int c = 0; c < 2; ++c) {
     const double* s = c == 0 ? u : v;
     const double advn = upwind1_cell(a, uf, vf, s, i, j);
-    const double adv = 1.5 * advn - 0.5 * advn;
+    const double advnm1 = upwind1_cell(a, uf, vf, c == 0 ? um : vm, i, j);
+    const double adv = 1.5 * advn - 0.5 * advnm1;
     const double dif = diffusion_cell(a, mu, s, i, j);
     const double src = 0.0;
     const double cur = s[fidx(a, i, j)];
@@ -521,6 +527,7 @@ struct cpgpu_ns {
   long long kim_step = 0;
   cpgpu_poisson* H[2] = {nullptr, nullptr};
   DevBuf<double> ksrc[2];
+  DevBuf<double> kum[2];  // u, v as the step found them (ghosts included)
   int helm_iters[2] = {0, 0};
 
   double* fld(int id) { return F[slot[id]].p; }
@@ -550,9 +557,10 @@ int ns_upload(cpgpu_ctx* ctx, void* dst, const void* src, size_t bytes) {
 
 template <int LIM>
 void launch_veo(cpgpu_ns* S, dim3 g, dim3 b) {
-  hipLaunchKernelGGL(k_veo_star<LIM>, g, b, 0, S->ctx->stream, S->axes(),
-                     S->fld(F_U), S->fld(F_V), S->fld(F_UF), S->fld(F_VF),
-                     S->fld(F_US), S->fld(F_VS), S->st.p, S->rho, S->mu);
+  const EvPair ev = timer_events(S->ctx, 4);
+  CPGPU_LAUNCH(ev, k_veo_star<LIM>, g, b, 0, S->ctx->stream, S->axes(),
+               S->fld(F_U), S->fld(F_V), S->fld(F_UF), S->fld(F_VF),
+               S->fld(F_US), S->fld(F_VS), S->st.p, S->rho, S->mu);
 }
 
 int ns_project(cpgpu_ns* S, bool kim);
@@ -601,8 +609,9 @@ int ns_project(cpgpu_ns* S, bool kim) {
   hipStream_t st = ctx->stream;
   const Axes a = S->axes();
   const dim3 b2(64, 4), g2((S->ny + 63) / 64, (S->nx + 3) / 4);
-  hipLaunchKernelGGL(k_c2f_div, g2, b2, 0, st, a, S->fld(F_US), S->fld(F_VS),
-                     S->fld(F_USF), S->fld(F_VSF), S->fld(F_DIV), S->div_int.p);
+  const EvPair ev5 = timer_events(ctx, 5);
+  CPGPU_LAUNCH(ev5, k_c2f_div, g2, b2, 0, st, a, S->fld(F_US), S->fld(F_VS),
+               S->fld(F_USF), S->fld(F_VSF), S->fld(F_DIV), S->div_int.p);
   CPGPU_HIP(ctx, hipGetLastError());
   // projection
   int rc = poisson_rhs_from_device(S->P, S->div_int.p, 0.0);
@@ -618,14 +627,15 @@ int ns_project(cpgpu_ns* S, bool kim) {
   S->total_iters += mi;
   // grad, corrections, p
   const dim3 g3((S->NY + 63) / 64, (S->NX + 3) / 4);
+  const EvPair ev6 = timer_events(ctx, 6);
   if (kim)
-    hipLaunchKernelGGL(k_grad_correct<true>, g3, b2, 0, st, a, S->P->x.p,
-                       S->fld(F_USF), S->fld(F_VSF), S->fld(F_US), S->fld(F_VS),
-                       S->fld(F_P), S->fld(F_PHI), S->st.p, S->mu, S->rho);
+    CPGPU_LAUNCH(ev6, k_grad_correct<true>, g3, b2, 0, st, a, S->P->x.p,
+                 S->fld(F_USF), S->fld(F_VSF), S->fld(F_US), S->fld(F_VS),
+                 S->fld(F_P), S->fld(F_PHI), S->st.p, S->mu, S->rho);
   else
-    hipLaunchKernelGGL(k_grad_correct<false>, g3, b2, 0, st, a, S->P->x.p,
-                       S->fld(F_USF), S->fld(F_VSF), S->fld(F_US), S->fld(F_VS),
-                       S->fld(F_P), S->fld(F_PHI), S->st.p, S->mu, S->rho);
+    CPGPU_LAUNCH(ev6, k_grad_correct<false>, g3, b2, 0, st, a, S->P->x.p,
+                 S->fld(F_USF), S->fld(F_VSF), S->fld(F_US), S->fld(F_VS),
+                 S->fld(F_P), S->fld(F_PHI), S->st.p, S->mu, S->rho);
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, S->st.p);
   CPGPU_HIP(ctx, hipGetLastError());
   // u <- u*, uf <- u*f by exchanging the buffers
@@ -637,8 +647,11 @@ int ns_project(cpgpu_ns* S, bool kim) {
 }
 
 // The two Helmholtz predictor problems of the current dt (Poisson_ with
-// set_uniform_alpha(1) and uniform beta dt*0.5/rho, s_ns_kim.hpp:267-283),
-// each with its component's boundary conditions.
+// set_uniform_alpha(1) and uniform beta dt*0.5/rho, s_ns_kim.hpp:267-283).
+// The predictor is a Poisson_ of its own (:105) whose BoundaryIndex stays
+// empty (s_equation.hpp:104), so every wall of its rows takes the default
+// condition, BC2 with value 0 (s_boundary.hpp:29-36, :195-207); the
+// velocity walls reach u* only through apply_bc(us) after the solves.
 int ns_kim_build(cpgpu_ns* S) {
   const double dt = S->h_st->dt;
   if (S->H[0] && S->H[1] && S->kim_dt == dt) return CPGPU_OK;
@@ -652,10 +665,9 @@ int ns_kim_build(cpgpu_ns* S) {
   gd.cx = S->hcx.data(); gd.sx = S->hsx.data();
   gd.cy = S->hcy.data(); gd.sy = S->hsy.data();
   gd.beta = dt * 0.5 / S->rho;
+  for (int k = 0; k < 4; ++k) gd.bc_type[k] = CPGPU_BC_NEUMANN;
+  gd.bc_val = nullptr;
   for (int c = 0; c < 2; ++c) {
-    const int* bt = c == 0 ? S->bcu_type : S->bcv_type;
-    for (int k = 0; k < 4; ++k) gd.bc_type[k] = bt[k];
-    gd.bc_val = S->hbc[c].data();
     int rc = cpgpu_poisson_create(S->ctx, &gd, &S->H[c]);
     if (rc) return rc;
     rc = cpgpu_poisson_set_alpha(S->H[c], 1.0);
@@ -674,8 +686,13 @@ int ns_kim_step(cpgpu_ns* S) {
   int rc = ns_kim_build(S);
   if (rc) return rc;
   const size_t n = (size_t)S->nx * S->ny;
-  if (S->ksrc[0].alloc(n) || S->ksrc[1].alloc(n))
+  if (S->ksrc[0].alloc(n) || S->ksrc[1].alloc(n) || S->kum[0].alloc(S->NT) ||
+      S->kum[1].alloc(S->NT))
     return set_error(ctx, CPGPU_ENOMEM, "kim source");
+  // um <- u, vm <- v before the ghost refresh (Operation_::Copy, IJK sweep)
+  for (int c = 0; c < 2; ++c)
+    CPGPU_HIP(ctx, hipMemcpyAsync(S->kum[c].p, S->fld(F_U + c), sizeof(double) * S->NT,
+                                  hipMemcpyDeviceToDevice, st));
   BCDev bu{{S->bcu_type[0], S->bcu_type[1], S->bcu_type[2], S->bcu_type[3]},
            S->bcu_val.p};
   BCDev bv{{S->bcv_type[0], S->bcv_type[1], S->bcv_type[2], S->bcv_type[3]},
@@ -687,6 +704,7 @@ int ns_kim_step(cpgpu_ns* S) {
   hipLaunchKernelGGL(k_ghost_corners, gcorner, dim3(256), 0, st, a, S->fld(F_U),
                      bu, S->fld(F_V), bv);
   hipLaunchKernelGGL(k_kim_src, g2, b2, 0, st, a, S->fld(F_U), S->fld(F_V),
+                     S->kum[0].p, S->kum[1].p,
                      S->fld(F_UF), S->fld(F_VF), S->ksrc[0].p, S->ksrc[1].p,
                      S->st.p, S->rho, S->mu, S->kim_step == 0 ? 1 : 0);
   hipLaunchKernelGGL(k_field_to_x2, g2, b2, 0, st, a, S->fld(F_US), S->fld(F_VS),
@@ -953,7 +971,7 @@ int cpgpu_ns_set_bc_values(cpgpu_ns* S, int comp, const double* bc_val) {
                                    S->hbc[comp].data(), sizeof(double) * nbc,
                                    hipMemcpyHostToDevice, S->ctx->stream));
   CPGPU_HIP(S->ctx, hipStreamSynchronize(S->ctx->stream));
-  if (S->H[comp]) return cpgpu_poisson_set_bc_values(S->H[comp], bc_val);
+  // the NS_kim_ predictor rows keep their own walls (BC2 value 0)
   return CPGPU_OK;
 }
 

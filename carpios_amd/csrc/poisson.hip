@@ -937,13 +937,17 @@ int cpgpu_poisson_set_source(cpgpu_poisson* P, const double* src_field,
     if (P->src.alloc(P->N)) return set_error(P->ctx, CPGPU_ENOMEM, "src");
     int rc = field_to_dev(P, src_field, P->src.p);
     if (rc) return rc;
+    P->src_kind = 1;
     return poisson_rhs_from_device(P, P->src.p, 0.0);
   }
+  P->src_kind = 2;
+  P->src_uni = src_uniform;
   return poisson_rhs_from_device(P, nullptr, src_uniform);
 }
 
 int cpgpu_poisson_set_source_device(cpgpu_poisson* P, const double* d_src) {
   if (!P || !d_src) return CPGPU_EINVAL;
+  P->src_kind = 3;
   return poisson_rhs_from_device(P, d_src, 0.0);
 }
 
@@ -987,6 +991,7 @@ int cpgpu_poisson_set_source_blobs(cpgpu_poisson* P, unsigned long long seed,
   hipLaunchKernelGGL(k_sub_mean, dim3(1024), dim3(256), 0, ctx->stream,
                      P->src.p, &P->rst.p->rsum, P->N);
   CPGPU_HIP(ctx, hipGetLastError());
+  P->src_kind = 1;
   rc = poisson_rhs_from_device(P, P->src.p, 0.0);
   if (rc) return rc;
   // the parameter buffers must outlive the kernels
@@ -1001,6 +1006,10 @@ int cpgpu_poisson_set_phi(cpgpu_poisson* P, const double* phi_field) {
 
 int cpgpu_poisson_set_alpha(cpgpu_poisson* P, double alpha) {
   if (!P) return CPGPU_EINVAL;
+  // b of a caller's device source cannot be rebuilt with the new weight
+  if (P->src_kind == 3)
+    return set_error(P->ctx, CPGPU_EINVAL,
+                     "alpha after a device source: set alpha first, then the source");
   // uniform cells only: one shift for every row
   for (double v : P->sxi)
     if (v != P->sxi[0]) return set_error(P->ctx, CPGPU_EUNSUPPORTED, "alpha: non-uniform x sizes");
@@ -1014,7 +1023,16 @@ int cpgpu_poisson_set_alpha(cpgpu_poisson* P, double alpha) {
   for (int i = 0; i < P->nx; ++i) P->vol[i] = v;
   P->shift = v * alpha;
   CPGPU_HIP(P->ctx, hipSetDevice(P->ctx->device));
+  CPGPU_HIP(P->ctx, hipStreamSynchronize(P->ctx->stream));
+  // the captured iteration graphs hold the old diagonal in their kernel
+  // arguments: drop them (the next solve captures new ones)
+  for (auto& g : P->graphs) (void)hipGraphExecDestroy(g.second);
+  P->graphs.clear();
   int rc = upload(P->ctx, P->dvol.p, P->vol.data(), sizeof(double) * P->vol.size());
+  if (rc) return rc;
+  // b carries the source weight v: rebuild it from the kept source
+  if (P->src_kind == 1) rc = poisson_rhs_from_device(P, P->src.p, 0.0);
+  else if (P->src_kind == 2) rc = poisson_rhs_from_device(P, nullptr, P->src_uni);
   if (rc) return rc;
   CPGPU_HIP(P->ctx, hipStreamSynchronize(P->ctx->stream));
   return CPGPU_OK;

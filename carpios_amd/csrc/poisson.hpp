@@ -125,6 +125,11 @@ struct cpgpu_poisson {
   size_t N = 0;
   double beta = 1.0;
   double shift = 0.0;  // Helmholtz vol*alpha (0: Poisson)
+  // where b came from, so that a new alpha can rebuild it: 0 none, 1 the
+  // problem's own source field (src), 2 a uniform value (src_uni), 3 a
+  // caller's device field (not kept)
+  int src_kind = 0;
+  double src_uni = 0.0;
   cpgpu::Tiling tl{};
   // host copies of the descriptor
   std::vector<double> KX, KY, vol, bcv;

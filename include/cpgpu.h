@@ -123,17 +123,21 @@ int cpgpu_poisson_set_source_blobs(cpgpu_poisson *P, unsigned long long seed,
 /* The blob parameters of that generator: out[4k..4k+3] = cx, cy, w, A. */
 int cpgpu_blob_params(unsigned long long seed, int nblobs, double wmin,
                       double wmax, double *out);
-/* Warm start / result: host cell fields (interior read or written). */
 /* Replaces Poisson_::set_uniform_alpha (s_poisson.hpp:461-467): the rows
  * become StencilHelmholtz_'s (s_poisson.hpp:185-260): the centre gets
  * v*alpha before the ghost substitutions and the source weight is v, with
  * v the product of the cell sizes.  Uniform cells only (else
- * CPGPU_EUNSUPPORTED).  Call before setting the source. */
+ * CPGPU_EUNSUPPORTED).  May follow solves: the captured iteration graphs
+ * are dropped, and b is rebuilt when its source was a host field, a
+ * uniform value or the blob generator; after cpgpu_poisson_set_source_device
+ * it returns CPGPU_EINVAL (set alpha first, then the device source).  A
+ * batch (cpgpu_batch_create) over this problem must be created after it. */
 int cpgpu_poisson_set_alpha(cpgpu_poisson *P, double alpha);
 /* New boundary values (packed as cpgpu_grid2d.bc_val) for time-dependent
  * boundary conditions: the reference samples BoundaryCondition::get_val at
  * each assembly (s_ghost.hpp:161-165).  Takes effect at the next source. */
 int cpgpu_poisson_set_bc_values(cpgpu_poisson *P, const double *bc_val);
+/* Warm start / result: host cell fields (interior read or written). */
 int cpgpu_poisson_set_phi(cpgpu_poisson *P, const double *phi_field);
 int cpgpu_poisson_get_phi(cpgpu_poisson *P, double *phi_field);
 /* Device-side fill of phi (a zero initial guess without a host transfer). */
@@ -454,7 +458,10 @@ int cpgpu_csr_solve(cpgpu_csr *A, int method, double *x, const double *b, int *m
  * family: 0 = stencil apply (p update + A p + p.q), 1 = vector update
  * (x, r update + dots), 2 = the U-Net's 16 -> 16 3x3 convolutions on the
  * finest level that store their output, 3 = the same convolution when its
- * epilogue applies the 1 x 1 output layer instead (a network only applied).
+ * epilogue applies the 1 x 1 output layer instead (a network only applied),
+ * 4 = the NS_explicit_ u* kernel (TVD advection + diffusion of both
+ * components), 5 = C2F + Div of u*, 6 = grad phi + corrections + p,
+ * 7 = the fused u* + C2F + Div kernel.
  * Timed launches carry their own start/stop timestamps (no extra packets
  * between kernels). */
 int cpgpu_timer_enable(cpgpu_ctx *ctx, int family, int on);

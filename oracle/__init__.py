@@ -46,7 +46,7 @@ class NSParams(C.Structure):
 class NSState(C.Structure):
     _fields_ = [(n, _dp) for n in ("u", "v", "uf", "vf", "us", "vs", "usf",
                                    "vsf", "p", "phi", "div")] + \
-               [("dt", C.c_double), ("t", C.c_double)]
+               [("dt", C.c_double), ("t", C.c_double), ("kim_src", _dp)]
 
 
 def _ptr(a, t=_dp):
@@ -314,14 +314,21 @@ class NSKim(NS):
                  helm_tol=1e-4, **kw):
         super().__init__(grid, bcu, bcv, **kw)
         self.kp = KimParams(SOLVE[helm], helm_max_iter, helm_tol)
+        # the last step's predictor sources of u and v (ghost-padded)
+        self.kim_src = np.zeros((2,) + grid.shape)
+        self.st.kim_src = _ptr(self.kim_src)
         self.nstep = 0
         self.helm_iters = []
         self._build()
 
     def _build(self):
+        # the predictor Poisson_ has its own empty BoundaryIndex
+        # (s_ns_kim.hpp:105, s_equation.hpp:104): BC2 value 0 on every wall
         beta = self.dt * 0.5 / self.prm.rho
-        self.Hu = Matrix(self.g, self.bcu, beta=beta, alpha=1.0)
-        self.Hv = Matrix(self.g, self.bcv, beta=beta, alpha=1.0)
+        walls = _plant == PLANT["kim_pred_walls"]
+        bp = make_bc(self.g, [BC2] * 4)
+        self.Hu = Matrix(self.g, self.bcu if walls else bp, beta=beta, alpha=1.0)
+        self.Hv = Matrix(self.g, self.bcv if walls else bp, beta=beta, alpha=1.0)
 
     def step(self):
         it = (C.c_int * 3)()
@@ -335,6 +342,19 @@ class NSKim(NS):
         self.iters.append(it[0])
         self.helm_iters.append((it[1], it[2]))
         return code, it[0], res.value
+
+
+PLANT = {"none": 0, "adv_sign": 1, "dif_rho": 2, "face_upwind": 3,
+         "neumann_sign": 4, "kim_pred_walls": 5, "kim_ab2_ghosts": 6}
+_plant = 0
+
+
+def set_plant(name):
+    """Switch on one deliberate mistake (ORC_PLANT_*) so that a test can show
+    its pin fails on a wrong restatement; "none" restores the oracle."""
+    global _plant
+    _plant = PLANT[name]
+    lib().orc_set_plant(C.c_int(_plant))
 
 
 def upwind1_field(grid, uf, vf, phi):

@@ -15,12 +15,26 @@
  *                             histories from the reference's lib/algebra
  *                             built in oracle/_ref; dense Gaussian
  *                             elimination on tiny grids
+ *   Neumann ghost constant .. a linear phi with nonzero Neumann data is
+ *                             reproduced exactly by the assembled rows
+ *                             (a flipped sign fails)
  *   TVD / diffusion / C2F / Div / Grad / F2C  exactness on constant and
  *                             linear fields, limiter special values,
- *                             Div(Grad phi) == -A phi / vol consistency
+ *                             Div(Grad phi) == -A phi / vol consistency,
+ *                             hand-computed per-cell upwind values on a
+ *                             cell whose face velocities differ in sign
+ *                             (face-centric upwinding fails)
  *   orc_ns_step .............. discrete divergence after projection at the
- *                             solver tolerance, Taylor-Green decay order,
- *                             cavity centre-line vs the Botella table
+ *                             solver tolerance; Taylor-Green vortex with
+ *                             nu = mu/rho = 1: second-order u and the
+ *                             pressure (a flipped advection sign or dif*rho
+ *                             fails)
+ *   orc_ns_kim_step .......... Taylor-Green decay; predictor rows against
+ *                             the all-Neumann Helmholtz matrix built by
+ *                             Kronecker products; a hand-computed AB2
+ *                             source on a wall cell at step 1
+ *   orc_set_plant ............ switches on one deliberate mistake so the
+ *                             tests can show each pin fails on it
  *   orc_jpcg_mixed ........... true FP64 residual reaches tol; solution
  *                             matches orc_jpcg to the conditioning bound
  *   orc_error_norms .......... closed forms: a constant error c gives
@@ -35,6 +49,10 @@
 #include <string.h>
 
 #define FIDX(g, i, j) (((i) + (g)->gl) * ((g)->ny + 2 * (g)->gl) + ((j) + (g)->gl))
+
+/* Planted-mistake switch (ORC_PLANT_*, tests only; 0 = faithful). */
+static int g_plant = 0;
+void orc_set_plant(int plant) { g_plant = plant; }
 
 /* lib/structure/s_grid.hpp:372-395 (_set_uniform_grid): sizes are l/n,
  * centres by accumulation csum + 0.5*cs, then shifted by min - sum(ghost
@@ -212,7 +230,10 @@ static int build_row(const orc_grid *g, double beta, const double *src_field,
         constant = constant + (coe * dl * val / d1);
       } else {
         t[2].coe = t[2].coe + coe;
-        constant = constant + (-val * dl * coe);
+        if (g_plant == ORC_PLANT_NEUMANN_SIGN)
+          constant = constant + (val * dl * coe);
+        else
+          constant = constant + (-val * dl * coe);
       }
     }
     t[k].coe = 0.0;
@@ -751,6 +772,13 @@ double orc_tvd_cell(const orc_grid *g, const double *uf, const double *vf,
       Dm = -1; Cm = 0; Um = 1;
       Dp = 0; Cp = 1; Up = 2;
     }
+    if (g_plant == ORC_PLANT_FACE_UPWIND) {
+      /* planted: each face upwinded by its own velocity */
+      if (vfm >= 0.0) { Dm = 0; Cm = -1; Um = -2; }
+      else { Dm = -1; Cm = 0; Um = 1; }
+      if (vfp >= 0.0) { Dp = 1; Cp = 0; Up = -1; }
+      else { Dp = 0; Cp = 1; Up = 2; }
+    }
 #define VAL(o) phi[FIDX(g, i + ei * (o), j + ej * (o))]
 #define SZ(o) sax(g, d, c + (o))
     sC = SZ(Cp); sD = SZ(Dp); sU = SZ(Up);
@@ -1027,8 +1055,13 @@ int orc_ns_step(const orc_grid *g, const orc_csr *A, const orc_bc *bcu,
         double adv = orc_tvd_cell(g, s->uf, s->vf, sc, i, j, prm->limiter);
         double dif = orc_diffusion_cell(g, prm->mu, sc, i, j);
         double src = 0.0;
-        ss[FIDX(g, i, j)] =
-            (-adv + (dif + src) / prm->rho) * dt + sc[FIDX(g, i, j)];
+        if (g_plant == ORC_PLANT_ADV_SIGN) adv = -adv;
+        if (g_plant == ORC_PLANT_DIF_RHO)
+          ss[FIDX(g, i, j)] =
+              (-adv + (dif + src) * prm->rho) * dt + sc[FIDX(g, i, j)];
+        else
+          ss[FIDX(g, i, j)] =
+              (-adv + (dif + src) / prm->rho) * dt + sc[FIDX(g, i, j)];
       }
     }
   }
@@ -1072,13 +1105,22 @@ static int solve_by(int method, const orc_csr *A, double *x, const double *b,
 }
 
 /* NS_kim_::run_one_step + Time_::advance (s_ns_kim.hpp:130-205, :222-287).
- * No restrict_time (commented out there).  The predictor's source per
- * component is (adv - (0.5*dif + 0)/rho)*dt - u with adv = upwind1 at step
- * 0 and 1.5*a - 0.5*a' after, where a' == a: run_one_step copies u into um
- * (and uf into umf) before the predictor.  Each component then solves the
- * Helmholtz problem alpha = 1, beta = dt*0.5/rho (mu does not enter,
- * :267-272) from the current u* as the first guess; H[k] is that matrix,
- * assembled by the caller with the component's boundary conditions. */
+ * No restrict_time (commented out there).  The step first copies u (ghosts
+ * included, Operation_::Copy sweeps IJK, s_operation.hpp:818-845) into um
+ * and uf into umf (:132-133), then apply_bc(u) refreshes u's ghosts
+ * (:225-227).  um therefore keeps the ghost values u carried out of the
+ * previous step: the velocity walls applied to u* before its correction
+ * (the correction's InterpolateF2C leaves ghosts at 0, s_operation.hpp:
+ * 141-164).  The predictor's source per component is
+ * (adv - (0.5*dif + 0)/rho)*dt - u with adv = upwind1(uf, u) at step 0 and
+ * 1.5*upwind1(uf, u) - 0.5*upwind1(umf, um) after, umf == uf.  Each
+ * component then solves the Helmholtz problem alpha = 1,
+ * beta = dt*0.5/rho (mu does not enter, :267-272) from the current u* as the
+ * first guess.  The predictor is its own Poisson_ (:105) with its own empty
+ * BoundaryIndex (s_equation.hpp:104), so its rows see the default boundary
+ * condition, BC2 with value 0 (s_boundary.hpp:29-36, :195-207), on every
+ * wall: H[k] is that matrix, assembled by the caller; the velocity walls
+ * reach u* only through apply_bc(us) afterwards (:285-287). */
 int orc_ns_kim_step(const orc_grid *g, const orc_csr *A, const orc_csr *Hu,
                     const orc_csr *Hv, const orc_bc *bcu, const orc_bc *bcv,
                     const orc_ns_params *prm, const orc_kim_params *kp,
@@ -1087,19 +1129,29 @@ int orc_ns_kim_step(const orc_grid *g, const orc_csr *A, const orc_csr *Hu,
   int NT = (nx + 2 * g->gl) * (ny + 2 * g->gl);
   double dt = s->dt, beta = dt * 0.5 / prm->rho;
   double *sscr = (double *)calloc((size_t)NT, sizeof(double));
+  double *um = (double *)malloc(sizeof(double) * NT);
+  double *vm = (double *)malloc(sizeof(double) * NT);
   double *x = (double *)malloc(sizeof(double) * n);
   double *b = (double *)malloc(sizeof(double) * n);
+  orc_bc bcpred = {{ORC_BC2, ORC_BC2, ORC_BC2, ORC_BC2}, NULL};
+  memcpy(um, s->u, sizeof(double) * NT);
+  memcpy(vm, s->v, sizeof(double) * NT);
   orc_apply_bc(g, s->u, bcu);
   orc_apply_bc(g, s->v, bcv);
+  if (g_plant == ORC_PLANT_KIM_AB2_GHOSTS) {
+    memcpy(um, s->u, sizeof(double) * NT);
+    memcpy(vm, s->v, sizeof(double) * NT);
+  }
   for (k = 0; k < 2; ++k) {
     double *sc = k == 0 ? s->u : s->v;
+    double *sm = k == 0 ? um : vm;
     double *ss = k == 0 ? s->us : s->vs;
     int mi = kp->max_iter;
     double tol = kp->tol;
     for (j = 0; j < ny; ++j) {
       for (i = 0; i < nx; ++i) {
         double advn = orc_upwind1_cell(g, s->uf, s->vf, sc, i, j);
-        double advnm1 = orc_upwind1_cell(g, s->uf, s->vf, sc, i, j);
+        double advnm1 = orc_upwind1_cell(g, s->uf, s->vf, sm, i, j);
         double adv = 1.5 * advn - 0.5 * advnm1;
         double dif = orc_diffusion_cell(g, prm->mu, sc, i, j);
         double src = 0.0;
@@ -1109,7 +1161,11 @@ int orc_ns_kim_step(const orc_grid *g, const orc_csr *A, const orc_csr *Hu,
                       : (adv - (0.5 * dif + src) / prm->rho) * dt - cur;
       }
     }
-    orc_helmholtz_rhs(g, 1.0, beta, sscr, 0.0, k == 0 ? bcu : bcv, b);
+    if (s->kim_src) memcpy(s->kim_src + (size_t)k * NT, sscr, sizeof(double) * NT);
+    orc_helmholtz_rhs(g, 1.0, beta, sscr, 0.0,
+                      g_plant == ORC_PLANT_KIM_PRED_WALLS ? (k == 0 ? bcu : bcv)
+                                                          : &bcpred,
+                      b);
     orc_field_to_x(g, ss, x);
     code = solve_by(kp->method, k == 0 ? Hu : Hv, x, b, &mi, &tol,
                     prm->norm_mode);
@@ -1118,6 +1174,8 @@ int orc_ns_kim_step(const orc_grid *g, const orc_csr *A, const orc_csr *Hu,
     orc_x_to_field(g, x, ss);
   }
   free(sscr);
+  free(um);
+  free(vm);
   free(x);
   free(b);
   code = ns_project(g, A, bcu, bcv, prm, s, iters, resid, 1);
@@ -1125,4 +1183,3 @@ int orc_ns_kim_step(const orc_grid *g, const orc_csr *A, const orc_csr *Hu,
   memcpy(s->vs, s->v, sizeof(double) * NT);
   return code;
 }
-

@@ -161,6 +161,8 @@ typedef struct {
   double *p, *phi;   /* pressure and projection potential */
   double *div;       /* divergence scratch */
   double dt, t;
+  double *kim_src;   /* NULL, or 2 fields: NS_kim_'s predictor sources of
+                        u and v from the last step (sscr, s_ns_kim.hpp:262) */
 } orc_ns_state;
 
 /* One NS_explicit_::run_one_step followed by Time_::advance.  A is the
@@ -189,6 +191,23 @@ int orc_ns_kim_step(const orc_grid *g, const orc_csr *A, const orc_csr *Hu,
                     const orc_csr *Hv, const orc_bc *bcu, const orc_bc *bcv,
                     const orc_ns_params *prm, const orc_kim_params *kp,
                     int step, orc_ns_state *s, int *iters, double *resid);
+
+/* ---- planted mistakes (tests only) ----
+ * A test switches one deliberate error on to show that a pin fails when the
+ * restatement is wrong; 0 (the default) is the faithful restatement.
+ *   1  the advection term enters u* with the wrong sign
+ *   2  the diffusion term is multiplied by rho instead of divided
+ *   3  TVD picks the upwind side per face from the face velocity
+ *   4  the Neumann ghost constant enters a Poisson row with the wrong sign
+ *   5  NS_kim_'s predictor rows take the velocity walls, not BC2 value 0
+ *   6  NS_kim_'s AB2 term reads u with refreshed ghosts for both levels */
+#define ORC_PLANT_ADV_SIGN 1
+#define ORC_PLANT_DIF_RHO 2
+#define ORC_PLANT_FACE_UPWIND 3
+#define ORC_PLANT_NEUMANN_SIGN 4
+#define ORC_PLANT_KIM_PRED_WALLS 5
+#define ORC_PLANT_KIM_AB2_GHOSTS 6
+void orc_set_plant(int plant);
 
 #ifdef __cplusplus
 }

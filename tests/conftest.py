@@ -18,3 +18,15 @@ def ctx():
     c = ca.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture
+def planted():
+    """Switch on one of the oracle's planted mistakes (oracle.PLANT) for the
+    rest of a test; the faithful oracle is restored afterwards."""
+    import oracle as O
+
+    def plant(name):
+        O.set_plant(name)
+    yield plant
+    O.set_plant("none")

@@ -41,3 +41,17 @@ def test_uniform_axis_matches_grid_construction():
         c, s = ca.uniform_axis(n, gl, lo, hi)
         a = O.Axis(n, gl, lo, hi)
         assert np.array_equal(c, a.c) and np.array_equal(s, a.s)
+
+
+def test_packed_boundary_values_are_checked_on_the_host():
+    # the library reads 2*(ny+2gl) + 2*(nx+2gl) values: anything else is
+    # refused before it reaches the C side
+    nx, ny, gl = 6, 4, 2
+    full = ca.pack_bc_values(nx, ny, gl, [1.0, 2.0, 3.0, 4.0])
+    assert full.shape == (2 * (ny + 2 * gl) + 2 * (nx + 2 * gl),)
+    assert np.array_equal(ca.pack_bc_values(nx, ny, gl, full), full)
+    import pytest
+    for bad in (full[:-1], np.concatenate([full, [0.0]]), full.reshape(2, -1),
+                [1.0, 2.0, 3.0]):
+        with pytest.raises(ValueError):
+            ca.pack_bc_values(nx, ny, gl, bad)

@@ -40,14 +40,38 @@ def test_helmholtz_matches_oracle(ctx, nx, ny, alpha, beta, method):
     P = ca.Poisson(ctx, desc)
     P.set_alpha(alpha)
     P.set_source(f)
-    gcode, git, gtol, _ = P.solve(method, max_iter=20000, tol=1e-11)
+    gcode, git, gtol, ghist = P.solve(method, max_iter=20000, tol=1e-11,
+                                      hist_cap=20000)
     print(f"helmholtz {nx}x{ny} {method}: oracle {code}/{mi.value}, gpu {gcode}/{git}",
           flush=True)
     assert gcode == code
     if method == "Dia_BiCGSTAB":
-        # the oracle's Dia_BiCGSTAB leaves its in/out max_iter at the limit
-        # on this return, so only the fields are compared
-        assert rel(g.interior(P.get_phi()), O.x_to_field(g, x)[1:-1, 1:-1]) < 1e-8
+        # Dia_BiCGSTAB's stop is as sensitive to rounding as the
+        # reference's own: the device run is held to the oracle's band under
+        # 1e-15 perturbations of b (exit code, history length, residual
+        # prefix, solution spread).  The half-step exit leaves max_iter at
+        # its input (solver_matrix.hpp:642-647); when the device takes the
+        # same kind of exit as the unperturbed oracle run, the 1e-10 field
+        # bound holds as for the other solvers.
+        from test_gpu_krylov import check_against_band, oracle_band
+        band = oracle_band("Dia_BiCGSTAB", M, M.b, max_iter=20000, tol=1e-11)
+        xg = O.field_to_x(g, P.get_phi())
+        check_against_band((gcode, xg, git, gtol, ghist), band, rel_floor=1e-10)
+        half_o, half_g = band[0][2] == 20000, git == 20000
+        print(f"  oracle band lengths {band[1]}..{band[2]}, spread {band[3]:.2e}; "
+              f"half-step exit oracle {half_o}, gpu {half_g}", flush=True)
+        if half_o == half_g:
+            assert rel(xg, band[0][1]) < 1e-10
+        else:
+            # the other exit must itself occur among the oracle's perturbed
+            # runs
+            rng = np.random.default_rng(12345)
+            kinds = set()
+            for _ in range(8):
+                o = O.solve("Dia_BiCGSTAB", M, M.b * (1 + 1e-15 * rng.standard_normal(M.n)),
+                            max_iter=20000, tol=1e-11)
+                kinds.add(o[2] == 20000)
+            assert half_g in kinds, kinds
         return
     assert abs(git - mi.value) <= 1
     assert rel(g.interior(P.get_phi()), O.x_to_field(g, x)[1:-1, 1:-1]) < 1e-10
@@ -120,5 +144,41 @@ def test_kim_taylor_green_matches_oracle_and_exact(ctx):
     for name in ("u", "v", "p"):
         r = rel(g.interior(gpu.get_field(name)), g.interior(ora.f[name]))
         assert r < 1e-10, (name, r)
+    # the predictor's Neumann walls keep the scheme near first order at the
+    # walls (the oracle's own Taylor-Green pin)
     err = np.max(np.abs(g.interior(gpu.get_field("u")) - ue))
-    assert err < 0.02 * np.exp(-2 * steps * dt)
+    assert err < 0.05 * np.exp(-2 * steps * dt)
+
+
+def test_set_alpha_after_a_solve_uses_the_new_rows(ctx):
+    # a solve captures its iteration graphs with the operator's diagonal in
+    # the kernel arguments; set_alpha afterwards must drop them and rebuild b
+    # with the new source weight, so the next solve is the Helmholtz one
+    import ctypes as C
+    nx, ny, beta, alpha = 48, 40, 0.05, 2.0
+    types, vals = [O.BC1, O.BC2, O.BC1, O.BC2], [0.3, -0.2, 1.0, 0.5]
+    g = O.StructGrid(nx, ny, 1, 0.0, 1.2, 0.0, 1.0)
+    f = g.field()
+    g.interior(f)[:] = np.random.default_rng(3).standard_normal((nx, ny))
+    bc = O.make_bc(g, types, vals)
+    desc = ca.uniform_grid_desc(nx, ny, 1, types, vals, beta=beta,
+                                xlim=(0.0, 1.2), ylim=(0.0, 1.0))
+    P = ca.Poisson(ctx, desc)
+    P.set_source(f)
+    P.solve("JPCG", max_iter=20000, tol=1e-11)
+    P.set_alpha(alpha)
+    P.fill_phi(0.0)
+    code, it, tol, _ = P.solve("JPCG", max_iter=20000, tol=1e-11)
+    M = O.Matrix(g, bc, beta=beta, src_field=f, alpha=alpha)
+    assert np.array_equal(O.field_to_x(g, P.get_vec(1)), M.b)
+    code_o, x_o, it_o, _, _ = O.solve("JPCG", M, M.b, tol=1e-11, max_iter=20000)
+    print(f"alpha after solve: gpu {code}/{it}, oracle {code_o}/{it_o}", flush=True)
+    assert code == code_o == 0 and abs(it - it_o) <= 1
+    assert rel(O.field_to_x(g, P.get_phi()), x_o) < 1e-10
+    # b of a caller's device source cannot be rebuilt: refused, state kept
+    Q = ca.Poisson(ctx, desc)
+    dsrc = ca.lib().cpgpu_poisson_device_vec(P.h, 1)
+    assert ca.lib().cpgpu_poisson_set_source_device(Q.h, C.c_void_p(dsrc)) == ca.OK
+    assert ca.lib().cpgpu_poisson_set_alpha(Q.h, C.c_double(1.0)) == ca.EINVAL
+    Q.close()
+    P.close()

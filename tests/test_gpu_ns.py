@@ -51,7 +51,8 @@ def compare(g, ora, gpu, tol=1e-12):
 
 
 @pytest.mark.parametrize("nx,ny,lim", [(48, 48, "VanLeer"), (37, 53, "superbee"),
-                                       (64, 32, "QUICK"), (40, 40, "upwind2")])
+                                       (64, 32, "QUICK"), (40, 40, "upwind2"),
+                                       (44, 36, "WAHYD"), (30, 50, "CUI")])
 def test_cavity_steps_match_oracle(ctx, nx, ny, lim):
     g, ora, gpu = pair(ctx, nx, ny, lim=lim)
     steps = 12
@@ -104,4 +105,27 @@ def test_projection_divergence_free_at_solver_tolerance(ctx):
     dv = og.interior(O.div(og, gpu.get_field("uf"), gpu.get_field("vf")))
     dstar = og.interior(gpu.get_field("div"))
     assert np.abs(dv).max() < 1e-7 * np.abs(dstar).max()
+    gpu.close()
+
+
+def test_cavity_512_three_steps_match_oracle(ctx):
+    # the 512^2 cavity of the plain-method configuration (Re 1000, van Leer,
+    # CFL 0.5, dt0 = 0.5 h, diffusion number 0.25, J-PCG to 1e-10) for three
+    # steps from rest, device against the oracle
+    g, ora, gpu = pair(ctx, 512, 512)
+    it_o = []
+    for k in range(3):
+        code, it, res = ora.step()
+        assert code == 0
+        it_o.append(it)
+        print(f"512^2 cavity: oracle step {k}: {it} iterations", flush=True)
+    it_g, res_g = gpu.step(3)
+    print(f"512^2 cavity: gpu its {list(it_g)} oracle {it_o}", flush=True)
+    assert np.all(np.abs(np.array(it_g) - np.array(it_o)) <= 1)
+    errs = compare(g, ora, gpu)
+    print("rel L2:", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-12, (k, v)
+    t, dt = gpu.time()
+    assert dt == ora.dt
     gpu.close()

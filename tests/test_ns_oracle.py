@@ -221,3 +221,128 @@ def test_error_norms_closed_forms():
     # homogeneity: errors scaled by k scale every norm by |k|
     n3 = O.error_norms(g, exa, exa + (-3.0) * (val - exa))
     assert n3 == pytest.approx(3.0 * n2, rel=1e-12)
+
+
+# ---- NS_explicit_ pins with planted-mistake twins ----
+
+def _tvd_line(uf_m, uf_p, lim="VanLeer"):
+    """Advection of one cell (i = 3, j = 1) of a 7 x 3 grid (h = 0.1) whose
+    phi varies along x only: phi(i) = -1, 0, 1, 3, 4, 4.5, 5; its m face
+    carries uf_m, its p face uf_p, and vf = 0."""
+    g = O.StructGrid(7, 3, 2, 0.0, 0.7, 0.0, 0.3)
+    phi = g.field()
+    G = g.gl
+    for i, val in enumerate((-1.0, 0.0, 1.0, 3.0, 4.0, 4.5, 5.0)):
+        phi[G + i, :] = val
+    uf = np.zeros(g.shape)
+    uf[G + 2, :] = uf_m
+    uf[G + 3, :] = uf_p
+    return O.tvd_field(g, uf, np.zeros(g.shape), phi, lim)[3, 1]
+
+
+def test_tvd_upwind_side_is_chosen_once_per_cell(planted):
+    # s_operation.hpp:556-577: the cell-averaged velocity veo picks U/C/D
+    # for both faces.  van Leer on a uniform grid (R = 2):
+    # psi = 2r / (1 + r), face = C + psi/2 (D - C), r = (C - U) / (D - C).
+    # (a) uf_m = -1, uf_p = 3: veo = 1 > 0, both faces from the left:
+    #     p face C=3 D=4 U=1: r = 2, psi = 4/3, f = 3 + 2/3 = 11/3
+    #     m face C=1 D=3 U=0: r = 1/2, psi = 2/3, f = 1 + 2/3 = 5/3
+    #     adv = (3 * 11/3 + 5/3) / 0.1 = 380/3
+    #     (face-centric: m face from the right, C=3 D=1 U=4, f = 7/3,
+    #      adv = 400/3)
+    # (b) uf_m = 2, uf_p = -5: veo = -1.5 < 0, both faces from the right:
+    #     p face C=4 D=3 U=4.5: r = 1/2, f = 4 - 1/3 = 11/3
+    #     m face C=3 D=1 U=4: r = 1/2, f = 3 - 2/3 = 7/3
+    #     adv = (-5 * 11/3 - 2 * 7/3) / 0.1 = -230
+    #     (face-centric: m face from the left, f = 5/3, adv = -650/3)
+    assert _tvd_line(-1.0, 3.0) == pytest.approx(380 / 3, rel=1e-8)
+    assert _tvd_line(2.0, -5.0) == pytest.approx(-230.0, rel=1e-8)
+    planted("face_upwind")
+    assert _tvd_line(-1.0, 3.0) == pytest.approx(400 / 3, rel=1e-8)
+    assert _tvd_line(2.0, -5.0) == pytest.approx(-650 / 3, rel=1e-8)
+
+
+def _linear_rows_error():
+    # phi = a x + b y + c with a Dirichlet x- wall (the face value) and
+    # Neumann walls elsewhere.  The ghost of a Neumann wall is
+    # phi_o - val * dl (s_ghost.hpp:171-175, s_equation.hpp:370-374), exact
+    # for a linear phi when val = -dphi/dn (outward normal): x+ -a, y- +b,
+    # y+ -b.  The 5-point rows are then exact, so A^-1 b is phi at the
+    # centres to rounding.
+    a, b, c = 0.7, -1.3, 0.4
+    g = O.StructGrid(6, 5, 1, 0.0, 0.6, 0.0, 0.5)
+    NX, NY = g.shape
+    yc = g.ay.c
+    vals = np.concatenate([b * yc + c, np.full(NY, -a), np.full(NX, b),
+                           np.full(NX, -b)])
+    bc = O.make_bc(g, [O.BC1, O.BC2, O.BC2, O.BC2], vals)
+    M = O.Matrix(g, bc, beta=1.0)
+    x = np.linalg.solve(M.dense(), M.b)
+    X, Y = g.centres()
+    want = O.field_to_x(g, np.pad(a * X + b * Y + c, 1))
+    return np.max(np.abs(x - want))
+
+
+def test_neumann_rows_exact_for_linear_field(planted):
+    assert _linear_rows_error() < 1e-12
+    planted("neumann_sign")
+    assert _linear_rows_error() > 1e-2
+
+
+def _tg_explicit(n, dt, steps, rho=2.0, mu=2.0):
+    """Taylor-Green vortex (example/ns_sin/main.cpp:40-51, nu = mu/rho = 1)
+    stepped by NS_explicit_ with van Leer; exact Dirichlet velocity walls at
+    the time each step starts (apply_bc uses Time_::current_time,
+    s_equation.hpp:352-357).  Returns the errors at the end relative to
+    the amplitudes: max |u - ue|, and the interior pressure up to its
+    constant (rel L2)."""
+    from test_kim_oracle import tg_bc_values, tg_exact
+    g = O.StructGrid(n, n, 2, 0.0, np.pi, 0.0, np.pi)
+    vu, vv = tg_bc_values(g, 0.0)
+    ns = O.NS(g, O.make_bc(g, [O.BC1] * 4, vu), O.make_bc(g, [O.BC1] * 4, vv),
+              rho=rho, mu=mu, cfl=10.0, dnum=1e10, limiter="VanLeer",
+              tol=1e-12, dt=dt)
+    X, Y = g.centres()
+    u0, v0, _ = tg_exact(X, Y, 0.0)
+    g.interior(ns.f["u"])[:] = u0
+    g.interior(ns.f["v"])[:] = v0
+    XF, YF = np.meshgrid(g.ax.c, g.ay.c, indexing="ij")
+    h = np.pi / n
+    ns.f["uf"][:] = tg_exact(XF + h / 2, YF, 0.0)[0]
+    ns.f["vf"][:] = tg_exact(XF, YF + h / 2, 0.0)[1]
+    with np.errstate(all="ignore"):
+        for k in range(steps):
+            vu, vv = tg_bc_values(g, k * dt)
+            ns.bcu = O.make_bc(g, [O.BC1] * 4, vu)
+            ns.bcv = O.make_bc(g, [O.BC1] * 4, vv)
+            ns.step()
+            if not np.isfinite(ns.f["u"]).all():
+                return np.inf, np.inf
+        t = ns.st.t
+        ue, ve, pe = tg_exact(X, Y, t)
+        u, v = g.interior(ns.f["u"]), g.interior(ns.f["v"])
+        eu = max(np.max(np.abs(u - ue)), np.max(np.abs(v - ve))) / np.exp(-2 * t)
+        p = g.interior(ns.f["p"])[2:-2, 2:-2]
+        pe = pe[2:-2, 2:-2]
+        pd, ped = p - p.mean(), pe - pe.mean()
+        ep = np.linalg.norm(pd - ped) / np.linalg.norm(ped)
+    return eu, ep
+
+
+def test_explicit_step_taylor_green(planted):
+    # u* = (-adv + (dif + src)/rho) dt + u (s_ns_explicit.hpp:196-211),
+    # projection, p = phi/dt: second order in u (measured 2.1e-2 at 16^2,
+    # 5.6e-3 at 32^2, t = 0.2, dt = h^2/5 scale) and the pressure, whose
+    # balance with the advection term carries the sign of adv (the TG
+    # nonlinear term is a gradient, so a flipped adv leaves u but flips p)
+    eu32, ep32 = _tg_explicit(32, 0.002, 100)
+    eu16, ep16 = _tg_explicit(16, 0.008, 25)
+    assert eu32 < 8e-3, eu32
+    assert ep32 < 0.04, ep32
+    assert eu16 / eu32 > 3.5, (eu16, eu32)
+    planted("adv_sign")
+    eu, ep = _tg_explicit(32, 0.002, 100)
+    assert ep > 1.5, ep
+    planted("dif_rho")
+    eu, ep = _tg_explicit(32, 0.002, 100)
+    assert not eu < 8e-3, eu
