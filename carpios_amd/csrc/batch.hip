@@ -383,8 +383,7 @@ __global__ __launch_bounds__(256) void k_b_refresh(const BProb* __restrict__ pro
       const int i = i0 + threadIdx.y + BY * k;
       if (i >= P.nx || j >= P.ny) continue;
       const size_t idx = (size_t)i * P.ny + j;
-      const double dg = diag_from(s64, i, j, kc[k].kx0, kc[k].kx1, kc[k].ky0, kc[k].ky1);
-      const double rv = bv[k] - apply_regs(s64, i, j, dg, xn[k], kc[k]);
+      const double rv = bv[k] - apply_regs_diff(s64, i, j, xn[k], kc[k]);
       w[0] += rv * rv;
       const float rf = (float)rv;
       P.r32[idx] = rf;

@@ -58,9 +58,11 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
   }
 }
 
-// r = b - A x (FP64, reference order); partials of b.b, r.r, sum(r)
+// r = b - A x [- A xlo] (FP64, difference form: the true residual of the
+// mixed solves and of cpgpu_poisson_residual); partials of b.b, r.r, sum(r)
 __global__ __launch_bounds__(256) void k_residual64(Sten<double> s,
                                                     const double* __restrict__ x,
+                                                    const double* __restrict__ xlo,
                                                     const double* __restrict__ b,
                                                     double* __restrict__ r,
                                                     double* partials, Tiling tl) {
@@ -75,7 +77,8 @@ __global__ __launch_bounds__(256) void k_residual64(Sten<double> s,
       if (i >= s.nx || j >= s.ny) continue;
       const size_t idx = (size_t)i * s.ny + j;
       const double bv = b[idx];
-      const double rv = bv - apply_global(s, x, i, j, diag_at(s, i, j));
+      double rv = bv - apply_global_diff(s, x, i, j);
+      if (xlo) rv = rv - apply_global_diff(s, xlo, i, j);
       r[idx] = rv;
       v[0] += bv * bv;
       v[1] += rv * rv;
@@ -99,17 +102,20 @@ __global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ r,
   }
 }
 
-// x += (double)e; partial sum of the new x
+// (x, xlo) += (double)e exactly (two-sum); partial sum of the new x
 __global__ __launch_bounds__(256) void k_add_f32(double* __restrict__ x,
+                                                 double* __restrict__ xlo,
                                                  const float* __restrict__ e,
                                                  double* partials, size_t n) {
   __shared__ double lds[4];
   double v[1] = {0.0};
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
        k += (size_t)gridDim.x * blockDim.x) {
-    const double nv = x[k] + (double)e[k];
-    x[k] = nv;
-    v[0] += nv;
+    double hi, lo;
+    two_sum(x[k], xlo[k] + (double)e[k], hi, lo);
+    x[k] = hi;
+    xlo[k] = lo;
+    v[0] += hi;
   }
   store_partials<double, 1>(v, partials, gridDim.x, blockIdx.x, lds);
 }
@@ -121,6 +127,30 @@ __global__ __launch_bounds__(256) void k_sub_mean(double* __restrict__ x,
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
        k += (size_t)gridDim.x * blockDim.x)
     x[k] = x[k] - mean;
+}
+
+// (x, xlo) -= total/n exactly (two-sum)
+__global__ __launch_bounds__(256) void k_sub_mean2(double* __restrict__ x,
+                                                   double* __restrict__ xlo,
+                                                   const double* sum, size_t n) {
+  const double mean = *sum / (double)n;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    double hi, lo;
+    two_sum(x[k], xlo[k] - mean, hi, lo);
+    x[k] = hi;
+    xlo[k] = lo;
+  }
+}
+
+// x = x + xlo (the FP64 value nearest the pair), xlo = 0
+__global__ __launch_bounds__(256) void k_round_x(double* __restrict__ x,
+                                                 double* __restrict__ xlo, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    x[k] = x[k] + xlo[k];
+    xlo[k] = 0.0;
+  }
 }
 
 template <typename T>
@@ -400,10 +430,10 @@ void launch_batch(cpgpu_poisson* P, hipStream_t st, const Sten<T>& s,
   }
   if constexpr (sizeof(T) == 4) {
     if (reliable) {
-      hipLaunchKernelGGL(k_merge_x, dim3(1024), dim3(256), 0, st, P->x.p, v.x,
+      hipLaunchKernelGGL(k_merge_x, dim3(1024), dim3(256), 0, st, P->x.p, P->xlo.p, v.x,
                          P->N);
       hipLaunchKernelGGL((k_refresh<PC>), dim3(tl.G), dim3(TJ, BY), 0, st,
-                         P->sten64(), P->sten32(), P->x.p, P->b.p, v.r,
+                         P->sten64(), P->sten32(), P->x.p, P->xlo.p, P->b.p, v.r,
                          v.partB, P->partR64.p, tl);
     }
   }
@@ -512,7 +542,7 @@ int alloc_f32(cpgpu_poisson* P) {
   if (P->b32.alloc(N) || P->x32.alloc(N) || P->r32.alloc(N) ||
       P->pa32.alloc(N) || P->pb32.alloc(N) || P->q32.alloc(N) ||
       P->partA32.alloc(kMaxBlocks) || P->partB32.alloc(3 * kMaxBlocks) ||
-      P->partR64.alloc(kMaxBlocks) ||
+      P->partR64.alloc(kMaxBlocks) || P->xlo.alloc(N) ||
       P->fKX.alloc(P->nx + 1) || P->fKY.alloc(P->ny + 1))
     return set_error(P->ctx, CPGPU_ENOMEM, "device allocation failed");
   std::vector<float> kx(P->KX.begin(), P->KX.end()), ky(P->KY.begin(), P->KY.end());
@@ -524,13 +554,14 @@ int alloc_f32(cpgpu_poisson* P) {
   return CPGPU_OK;
 }
 
-// r = b - A x in FP64 with b.b, r.r, sum(r) into P->rst (device) and its
-// pinned mirror.
-int residual64(cpgpu_poisson* P, bool to_host) {
+// r = b - A x [- A xlo] in FP64 with b.b, r.r, sum(r) into P->rst (device)
+// and its pinned mirror.
+int residual64(cpgpu_poisson* P, bool to_host, bool with_lo = false) {
   cpgpu_ctx* ctx = P->ctx;
   hipStream_t st = ctx->stream;
   hipLaunchKernelGGL(k_residual64, dim3(P->tl.G), dim3(TJ, BY), 0, st,
-                     P->sten64(), P->x.p, P->b.p, P->r.p, P->part64.p, P->tl);
+                     P->sten64(), P->x.p, with_lo ? P->xlo.p : nullptr, P->b.p, P->r.p,
+                     P->part64.p, P->tl);
   hipLaunchKernelGGL(k_finalize<3>, dim3(1), dim3(kThreads), 0, st,
                      P->part64.p, (int)P->tl.G, (double*)P->rst.p);
   CPGPU_HIP(ctx, hipGetLastError());
@@ -553,8 +584,9 @@ int run_mixed(cpgpu_poisson* P, int max_inner_total, double tol, int project,
   const int nb1 = 1024;
   int total_inner = 0, code = 1, hl = 0;
   double resid = 0;
+  hipLaunchKernelGGL(k_fill<double>, dim3(nb1), dim3(256), 0, st, P->xlo.p, 0.0, P->N);
   for (int o = 0; o <= P->max_outer; ++o) {
-    if ((rc = residual64(P, true))) return rc;
+    if ((rc = residual64(P, true, true))) return rc;
     double normb = std::sqrt(P->h_rst->bb);
     if (normb == 0.0) normb = 1.0;
     double rr = P->h_rst->rr;
@@ -579,16 +611,18 @@ int run_mixed(cpgpu_poisson* P, int max_inner_total, double tol, int project,
                              P->inner_tol, 0, nullptr, 0, &ks);
     if (rc) return rc;
     total_inner += ks.iter;
-    hipLaunchKernelGGL(k_add_f32, dim3(nb1), dim3(256), 0, st, P->x.p,
+    hipLaunchKernelGGL(k_add_f32, dim3(nb1), dim3(256), 0, st, P->x.p, P->xlo.p,
                        P->x32.p, P->part64.p, P->N);
     if (project) {
       hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kThreads), 0, st,
                          P->part64.p, nb1, &P->rst.p->xsum);
-      hipLaunchKernelGGL(k_sub_mean, dim3(nb1), dim3(256), 0, st, P->x.p,
+      hipLaunchKernelGGL(k_sub_mean2, dim3(nb1), dim3(256), 0, st, P->x.p, P->xlo.p,
                          &P->rst.p->xsum, P->N);
     }
     CPGPU_HIP(ctx, hipGetLastError());
   }
+  hipLaunchKernelGGL(k_round_x, dim3(nb1), dim3(256), 0, st, P->x.p, P->xlo.p, P->N);
+  CPGPU_HIP(ctx, hipGetLastError());
   *hist_len = hl;
   *iters = total_inner;
   *resid_out = resid;
@@ -616,6 +650,7 @@ int run_reliable(cpgpu_poisson* P, int max_iter, double tol, int project,
   *hist_len = 0;
   *resid_out = resid0;
   if (resid0 <= tol) return CPGPU_OK;
+  hipLaunchKernelGGL(k_fill<double>, dim3(1024), dim3(256), 0, st, P->xlo.p, 0.0, P->N);
   // r32 = (float)r64 as the right-hand side of a zero-start FP32 solve
   hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, st, P->r.p, P->b32.p,
                      P->x32.p, P->rst.p, P->N, project);
@@ -629,16 +664,17 @@ int run_reliable(cpgpu_poisson* P, int max_iter, double tol, int project,
                            &ks, true, normb);
   if (rc) return rc;
   // increments since the last replacement (none when it converged there)
-  hipLaunchKernelGGL(k_merge_x, dim3(1024), dim3(256), 0, st, P->x.p,
+  hipLaunchKernelGGL(k_merge_x, dim3(1024), dim3(256), 0, st, P->x.p, P->xlo.p,
                      P->x32.p, P->N);
   if (project) {
-    hipLaunchKernelGGL(k_add_f32, dim3(1024), dim3(256), 0, st, P->x.p,
+    hipLaunchKernelGGL(k_add_f32, dim3(1024), dim3(256), 0, st, P->x.p, P->xlo.p,
                        P->x32.p, P->part64.p, P->N);  // x32 is zero: sums x
     hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kThreads), 0, st,
                        P->part64.p, 1024, &P->rst.p->xsum);
-    hipLaunchKernelGGL(k_sub_mean, dim3(1024), dim3(256), 0, st, P->x.p,
+    hipLaunchKernelGGL(k_sub_mean2, dim3(1024), dim3(256), 0, st, P->x.p, P->xlo.p,
                        &P->rst.p->xsum, P->N);
   }
+  hipLaunchKernelGGL(k_round_x, dim3(1024), dim3(256), 0, st, P->x.p, P->xlo.p, P->N);
   CPGPU_HIP(ctx, hipGetLastError());
   const int nh = (ks.iter + P->batch - 1) / P->batch;
   if (dh && nh > 0)

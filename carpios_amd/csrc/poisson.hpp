@@ -140,8 +140,8 @@ struct cpgpu_poisson {
   // device operator
   cpgpu::DevBuf<double> dKX, dKY, dvol, dbcv;
   cpgpu::DevBuf<float> fKX, fKY;
-  // FP64 vectors
-  cpgpu::DevBuf<double> x, b, r, p0, p1, q, src;
+  // FP64 vectors; xlo: the low part of the mixed solves' solution x + xlo
+  cpgpu::DevBuf<double> x, b, r, p0, p1, q, src, xlo;
   // FP32 vectors of the inner solve
   cpgpu::DevBuf<float> b32, x32, r32, pa32, pb32, q32;
   // reductions and states

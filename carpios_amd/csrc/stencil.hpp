@@ -90,19 +90,88 @@ __device__ __forceinline__ T diag_from(const Sten<T>& s, int i, int j, T kx0, T 
   return d;
 }
 
+// The part of a row's centre coefficient that no interior difference
+// carries: the Helmholtz shift and, on each boundary face, K_f plus the
+// segment's ghost substitution (K_f d0/d1 + K_f for a Dirichlet wall, 0 for
+// a Neumann wall, whose substitution is -K_f).
+template <typename T>
+__device__ __forceinline__ T row_extra(const Sten<T>& s, int i, int j, T kx0, T kx1, T ky0,
+                                       T ky1) {
+  T e = s.shift;
+  if (i == 0) e = e + (kx0 + s.adj[0]);
+  if (j == 0) e = e + (ky0 + s.adj[2]);
+  if (j == s.ny - 1) e = e + (ky1 + s.adj[3]);
+  if (i == s.nx - 1) e = e + (kx1 + s.adj[1]);
+  return e;
+}
+
+// One row of A x in difference form: the sum over interior faces of
+// K_f (c - x_f), then c times row_extra.  Its rounding scales with the
+// differences of neighbouring values instead of with c.  The Laplacian of a
+// smooth field is about h^2 times the field: at 4096^2 that is below FP32's
+// epsilon, so the term order of the reference's CSR product leaves no
+// correct digit of A x in FP32 (and ~1e-11 relative noise in FP64), while
+// the differences keep ~5 digits in FP32.  Used where there is no reference
+// counterpart to reproduce: every FP32 iteration and the FP64 residuals of
+// the mixed solves' refinement.
+template <typename T>
+__device__ __forceinline__ T row_apply_diff(const Sten<T>& s, int i, int j, T c, T n, T so,
+                                            T w, T e, T kx0, T kx1, T ky0, T ky1) {
+  T acc = T(0);
+  if (i > 0) acc = acc + kx0 * (c - n);
+  if (j > 0) acc = acc + ky0 * (c - w);
+  if (j < s.ny - 1) acc = acc + ky1 * (c - e);
+  if (i < s.nx - 1) acc = acc + kx1 * (c - so);
+  return acc + c * row_extra(s, i, j, kx0, kx1, ky0, ky1);
+}
+
+// One row of A x from the centre value c, its neighbours n (i-1), so (i+1),
+// w (j-1), e (j+1) and the four face coefficients (values outside the grid
+// are never used).  FP64: the reference's CSR term order with the centre
+// coefficient dg, bitwise its SpMV (matrix_SparCompRow.hpp:236-249).  FP32:
+// the difference form.
+template <typename T>
+__device__ __forceinline__ T row_apply(const Sten<T>& s, int i, int j, T dg, T c, T n, T so,
+                                       T w, T e, T kx0, T kx1, T ky0, T ky1) {
+  if constexpr (sizeof(T) == 8) {
+    T acc = T(0);
+    if (i > 0) acc = acc + n * (-kx0);
+    if (j > 0) acc = acc + w * (-ky0);
+    acc = acc + c * dg;
+    if (j < s.ny - 1) acc = acc + e * (-ky1);
+    if (i < s.nx - 1) acc = acc + so * (-kx1);
+    return acc;
+  } else {
+    return row_apply_diff(s, i, j, c, n, so, w, e, kx0, kx1, ky0, ky1);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T apply_global(const Sten<T>& s,
                                           const T* __restrict__ x, int i,
                                           int j, T dg) {
   const int ny = s.ny;
   const size_t k = (size_t)i * ny + j;
-  T acc = T(0);
-  if (i > 0) acc = acc + x[k - ny] * (-s.KX[i]);
-  if (j > 0) acc = acc + x[k - 1] * (-s.KY[j]);
-  acc = acc + x[k] * dg;
-  if (j < ny - 1) acc = acc + x[k + 1] * (-s.KY[j + 1]);
-  if (i < s.nx - 1) acc = acc + x[k + ny] * (-s.KX[i + 1]);
-  return acc;
+  const T c = x[k];
+  const T n = i > 0 ? x[k - ny] : T(0);
+  const T w = j > 0 ? x[k - 1] : T(0);
+  const T e = j < ny - 1 ? x[k + 1] : T(0);
+  const T so = i < s.nx - 1 ? x[k + ny] : T(0);
+  return row_apply(s, i, j, dg, c, n, so, w, e, s.KX[i], s.KX[i + 1], s.KY[j], s.KY[j + 1]);
+}
+
+// apply_global in difference form (any precision)
+template <typename T>
+__device__ __forceinline__ T apply_global_diff(const Sten<T>& s, const T* __restrict__ x,
+                                               int i, int j) {
+  const int ny = s.ny;
+  const size_t k = (size_t)i * ny + j;
+  const T c = x[k];
+  const T n = i > 0 ? x[k - ny] : T(0);
+  const T w = j > 0 ? x[k - 1] : T(0);
+  const T e = j < ny - 1 ? x[k + 1] : T(0);
+  const T so = i < s.nx - 1 ? x[k + ny] : T(0);
+  return row_apply_diff(s, i, j, c, n, so, w, e, s.KX[i], s.KX[i + 1], s.KY[j], s.KY[j + 1]);
 }
 
 // The five values of x around (i, j) and the row's and column's face
@@ -137,13 +206,14 @@ __device__ __forceinline__ void load_nbr(const Sten<T>& s, const X* __restrict__
 template <typename T>
 __device__ __forceinline__ T apply_regs(const Sten<T>& s, int i, int j, T dg, const Nbr<T>& x,
                                         const Nbr<T>& k) {
-  T acc = T(0);
-  if (i > 0) acc = acc + x.n * (-k.kx0);
-  if (j > 0) acc = acc + x.w * (-k.ky0);
-  acc = acc + x.c * dg;
-  if (j < s.ny - 1) acc = acc + x.e * (-k.ky1);
-  if (i < s.nx - 1) acc = acc + x.s * (-k.kx1);
-  return acc;
+  return row_apply(s, i, j, dg, x.c, x.n, x.s, x.w, x.e, k.kx0, k.kx1, k.ky0, k.ky1);
+}
+
+// apply_global_diff's arithmetic from registers
+template <typename T>
+__device__ __forceinline__ T apply_regs_diff(const Sten<T>& s, int i, int j, const Nbr<T>& x,
+                                             const Nbr<T>& k) {
+  return row_apply_diff(s, i, j, x.c, x.n, x.s, x.w, x.e, k.kx0, k.kx1, k.ky0, k.ky1);
 }
 
 // Device state of one Krylov solve.  FP32 solves store FP32 values exactly.
@@ -374,12 +444,8 @@ __device__ __forceinline__ void tile_apply(const Sten<T>& s, int i0, int j0, con
     if (i >= nx || j >= ny) continue;
     const T kx0 = L.skx[li], kx1 = L.skx[li + 1];
     const T dg = diag_from(s, i, j, kx0, kx1, ky0, ky1);
-    T a = T(0);
-    if (i > 0) a = a + L.tile[li - 1][lj] * (-kx0);
-    if (j > 0) a = a + L.tile[li][lj - 1] * (-ky0);
-    a = a + L.tile[li][lj] * dg;
-    if (j < ny - 1) a = a + L.tile[li][lj + 1] * (-ky1);
-    if (i < nx - 1) a = a + L.tile[li + 1][lj] * (-kx1);
+    const T a = row_apply(s, i, j, dg, L.tile[li][lj], L.tile[li - 1][lj], L.tile[li + 1][lj],
+                          L.tile[li][lj - 1], L.tile[li][lj + 1], kx0, kx1, ky0, ky1);
     q[(size_t)i * ny + j] = a;
     acc += L.tile[li][lj] * a;
   }
@@ -523,13 +589,19 @@ __global__ __launch_bounds__(256) void k_cg_dir_march(
       const T left = __shfl_up(pc, 1, 64);
       const T right = __shfl_down(pc, 1, 64);
       if (out && i < nx) {
-        const T dg = diag_at(s, i, j);
-        T acc = T(0);
-        if (i > 0) acc = acc + pn[h] * (-s.KX[i]);
-        if (j > 0) acc = acc + left * kyl;
-        acc = acc + pc * dg;
-        if (j < ny - 1) acc = acc + right * kyr;
-        if (i < nx - 1) acc = acc + pn[h + 2] * (-s.KX[i + 1]);
+        T acc;
+        if constexpr (sizeof(T) == 8) {
+          const T dg = diag_at(s, i, j);
+          acc = T(0);
+          if (i > 0) acc = acc + pn[h] * (-s.KX[i]);
+          if (j > 0) acc = acc + left * kyl;
+          acc = acc + pc * dg;
+          if (j < ny - 1) acc = acc + right * kyr;
+          if (i < nx - 1) acc = acc + pn[h + 2] * (-s.KX[i + 1]);
+        } else {
+          acc = row_apply_diff(s, i, j, pc, pn[h], pn[h + 2], left, right, s.KX[i],
+                               s.KX[i + 1], s.KY[j], s.KY[j + 1]);
+        }
         const size_t k = (size_t)i * ny + j;
         pnew[k] = pc;
         q[k] = acc;
@@ -641,13 +713,19 @@ __global__ __launch_bounds__(256) void k_cg_dir_amarch(
       if (lane == 0) left = hn[h];
       if (lane == 63) right = hn[h];
       if (jin && i < nx) {
-        const T dg = diag_at(s, i, j);
-        T acc = T(0);
-        if (i > 0) acc = acc + pn[h] * (-s.KX[i]);
-        if (j > 0) acc = acc + left * kyl;
-        acc = acc + pc * dg;
-        if (j < ny - 1) acc = acc + right * kyr;
-        if (i < nx - 1) acc = acc + pn[h + 2] * (-s.KX[i + 1]);
+        T acc;
+        if constexpr (sizeof(T) == 8) {
+          const T dg = diag_at(s, i, j);
+          acc = T(0);
+          if (i > 0) acc = acc + pn[h] * (-s.KX[i]);
+          if (j > 0) acc = acc + left * kyl;
+          acc = acc + pc * dg;
+          if (j < ny - 1) acc = acc + right * kyr;
+          if (i < nx - 1) acc = acc + pn[h + 2] * (-s.KX[i + 1]);
+        } else {
+          acc = row_apply_diff(s, i, j, pc, pn[h], pn[h + 2], left, right, s.KX[i],
+                               s.KX[i + 1], s.KY[j], s.KY[j + 1]);
+        }
         const size_t k = (size_t)i * ny + j;
         pnew[k] = pc;
         q[k] = acc;
@@ -815,12 +893,18 @@ __global__ __launch_bounds__(256) void k_cg_dir_roll(
         if (lane == 0) left = hw[m];
         if (lane == 63) right = hw[m];
         if (jin && i >= r0 && i < oend) {
-          T acc = T(0);
-          if (i > 0) acc = acc + w[m] * (-cur.kx[m]);
-          if (j > 0) acc = acc + left * kyl;
-          acc = acc + pc * dw[m + 1];
-          if (j < ny - 1) acc = acc + right * kyr;
-          if (i < nx - 1) acc = acc + w[m + 2] * (-cur.kx[m + 1]);
+          T acc;
+          if constexpr (sizeof(T) == 8) {
+            acc = T(0);
+            if (i > 0) acc = acc + w[m] * (-cur.kx[m]);
+            if (j > 0) acc = acc + left * kyl;
+            acc = acc + pc * dw[m + 1];
+            if (j < ny - 1) acc = acc + right * kyr;
+            if (i < nx - 1) acc = acc + w[m + 2] * (-cur.kx[m + 1]);
+          } else {
+            acc = row_apply_diff(s, i, j, pc, w[m], w[m + 2], left, right, cur.kx[m],
+                                 cur.kx[m + 1], ky0, ky1);
+          }
           const size_t k = (size_t)i * ny + j;
           pnew[k] = pc;
           q[k] = acc;
@@ -923,22 +1007,41 @@ __global__ __launch_bounds__(256) void k_cg_upd2(
 // replaced by the true FP64 residual b - A x rounded to FP32; the search
 // direction is kept, so the Krylov information is not thrown away.
 
-// x64 += (double)x32; x32 = 0
+// s + e = a + b exactly (Knuth's two-sum; needs fp contract off)
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// The mixed solves keep their FP64 solution as an unevaluated sum
+// x64 + xlo: on fine grids b = -h^2 f is so small against A x that rounding
+// x to FP64 alone leaves a relative residual of about eps ||A|| ||x|| / ||b||
+// (~7e-11 at 4096^2 for the blob right-hand sides); the pair carries the
+// rounding of every update, and the final rounding x64 + xlo is the FP64
+// value nearest the refined solution.
+// (x64, xlo) += (double)x32 exactly; x32 = 0
 static __global__ __launch_bounds__(256) void k_merge_x(double* __restrict__ x64,
+                                                 double* __restrict__ xlo,
                                                  float* __restrict__ x32,
                                                  size_t n) {
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
        k += (size_t)gridDim.x * blockDim.x) {
-    x64[k] = x64[k] + (double)x32[k];
+    double hi, lo;
+    two_sum(x64[k], xlo[k] + (double)x32[k], hi, lo);
+    x64[k] = hi;
+    xlo[k] = lo;
     x32[k] = 0.f;
   }
 }
 
-// r32 = (float)(b - A x64) with the FP64 operator; partials of the FP32
-// r.z, r.r (partB, same slots as the update kernel) and of the FP64 r.r.
+// r32 = (float)(b - A x64 - A xlo) with the FP64 operator in difference
+// form; partials of the FP32 r.z, r.r (partB, same slots as the update
+// kernel) and of the FP64 r.r.
 template <bool PC>
 __global__ __launch_bounds__(256) void k_refresh(Sten<double> s64, Sten<float> s32,
                                                  const double* __restrict__ x64,
+                                                 const double* __restrict__ xlo,
                                                  const double* __restrict__ b64,
                                                  float* __restrict__ r32,
                                                  float* __restrict__ partB,
@@ -954,13 +1057,14 @@ __global__ __launch_bounds__(256) void k_refresh(Sten<double> s64, Sten<float> s
     const int j = j0 + threadIdx.x;
     constexpr int NR = TI / BY;
     // every load of the rows first (clamped addresses)
-    Nbr<double> xn[NR], kc[NR];
+    Nbr<double> xn[NR], xl[NR], kc[NR];
     Nbr<float> kf[NR];
     double bv[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const int i = i0 + threadIdx.y + BY * k;
       load_nbr(s64, x64, i, j, xn[k], kc[k]);
+      load_nbr(s64, xlo, i, j, xl[k], kc[k]);
       const int ic = min(i, s64.nx - 1), jc = min(j, s64.ny - 1);
       bv[k] = b64[(size_t)ic * s64.ny + jc];
       kf[k].kx0 = s32.KX[ic];
@@ -973,8 +1077,8 @@ __global__ __launch_bounds__(256) void k_refresh(Sten<double> s64, Sten<float> s
       const int i = i0 + threadIdx.y + BY * k;
       if (i >= s64.nx || j >= s64.ny) continue;
       const size_t idx = (size_t)i * s64.ny + j;
-      const double dg = diag_from(s64, i, j, kc[k].kx0, kc[k].kx1, kc[k].ky0, kc[k].ky1);
-      const double rv = bv[k] - apply_regs(s64, i, j, dg, xn[k], kc[k]);
+      const double rv = (bv[k] - apply_regs_diff(s64, i, j, xn[k], kc[k])) -
+                        apply_regs_diff(s64, i, j, xl[k], kc[k]);
       w[0] += rv * rv;
       const float rf = (float)rv;
       r32[idx] = rf;
