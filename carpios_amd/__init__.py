@@ -76,7 +76,7 @@ EXPORTS = [
     "cpgpu_unet_set_walls", "cpgpu_copy_bandwidth", "cpgpu_csr_create",
     "cpgpu_csr_destroy", "cpgpu_csr_mult", "cpgpu_csr_solve",
     "cpgpu_poisson_set_alpha", "cpgpu_poisson_set_bc_values", "cpgpu_ns_set_kim",
-    "cpgpu_ns_kim_iters", "cpgpu_ns_set_bc_values",
+    "cpgpu_ns_kim_iters", "cpgpu_ns_set_bc_values", "cpgpu_poisson2d_solve_static",
 ]
 
 

@@ -23,6 +23,8 @@ struct cpgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // device address of this GPU's static scratch (cpgpu_poisson2d_solve_static)
+  void* scratch = nullptr;
   // kernel timers: per family a list of (start, stop) event pairs
   static constexpr int kFamilies = 8;
   bool timer_on[kFamilies] = {};
@@ -33,6 +35,10 @@ struct cpgpu_ctx {
 namespace cpgpu {
 
 int set_error(cpgpu_ctx* ctx, int code, const char* fmt, ...);
+
+// Device address of the static scratch array on `device` (current), looked
+// up once per device (poisson.hip).
+int static_scratch_setup(int device, void** out);
 
 #define CPGPU_HIP(ctx, call)                                                  \
   do {                                                                        \

@@ -51,6 +51,11 @@ int cpgpu_create(int device, cpgpu_ctx** out) {
     delete c;
     return CPGPU_EHIP;
   }
+  if (cpgpu::static_scratch_setup(device, &c->scratch) != CPGPU_OK) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CPGPU_EHIP;
+  }
   *out = c;
   return CPGPU_OK;
 }

@@ -6,6 +6,8 @@
 #include <cstring>
 #include <memory>
 
+#include <mutex>
+
 #include "poisson.hpp"
 
 #pragma clang fp contract(off)
@@ -216,8 +218,11 @@ bool all_equal(const double* a, int n) {
 
 // Face coefficients K = -(-beta*a/d) (s_poisson.hpp:98-135), diagonal
 // adjustments and rhs constants of the ghost substitution
-// (s_ghost.hpp:140-179).
-int build_descriptor(cpgpu_poisson* P, const cpgpu_grid2d* g) {
+// (s_ghost.hpp:140-179), source weights and the boundary values of the
+// interior lines, into caller arrays: KX (nx+1), KY (ny+1), adj[4], seg[4],
+// vol (nx), bcv (2 ny + 2 nx).
+int describe_grid(cpgpu_ctx* ctx, const cpgpu_grid2d* g, double* KX, double* KY,
+                  double* adj, RhsSeg* seg, double* vol, double* bcv) {
   const int nx = g->nx, ny = g->ny, gl = g->gl;
   auto cx = [&](int i) { return g->cx[i + gl]; };
   auto sx = [&](int i) { return g->sx[i + gl]; };
@@ -226,71 +231,80 @@ int build_descriptor(cpgpu_poisson* P, const cpgpu_grid2d* g) {
   // the kernels use 1-D face coefficients: a = sy[j] (x faces) and sx[i]
   // (y faces) must not vary along the face's own line
   if (!all_equal(g->sy + gl, ny) || !all_equal(g->sx + gl, nx))
-    return set_error(P->ctx, CPGPU_EUNSUPPORTED,
+    return set_error(ctx, CPGPU_EUNSUPPORTED,
                      "non-uniform cell sizes are not supported");
   const double beta = g->beta;
-  P->KX.assign(nx + 1, 0.0);
-  P->KY.assign(ny + 1, 0.0);
   for (int f = 0; f <= nx; ++f) {
     const double a = sy(0) * 1.0;
     const double d = std::fabs(cx(f) - cx(f - 1));
     const double coe = -beta * a / d;
-    P->KX[f] = -coe;
+    KX[f] = -coe;
   }
   for (int f = 0; f <= ny; ++f) {
     const double a = 1.0 * sx(0);
     const double d = std::fabs(cy(f) - cy(f - 1));
     const double coe = -beta * a / d;
-    P->KY[f] = -coe;
+    KY[f] = -coe;
   }
   // segments: 0 x- (ghost i=-1), 1 x+ (i=nx), 2 y- (j=-1), 3 y+ (j=ny)
   for (int s = 0; s < 4; ++s) {
     const int type = g->bc_type[s];
     if (type != CPGPU_BC_DIRICHLET && type != CPGPU_BC_NEUMANN)
-      return set_error(P->ctx, CPGPU_EINVAL, "bad bc type %d on segment %d",
+      return set_error(ctx, CPGPU_EINVAL, "bad bc type %d on segment %d",
                        type, s);
     double c0, c1, cf, coe;
     if (s == 0) {
-      c0 = cx(-1); c1 = cx(0); cf = c1 - sx(0) * 0.5; coe = -P->KX[0];
+      c0 = cx(-1); c1 = cx(0); cf = c1 - sx(0) * 0.5; coe = -KX[0];
     } else if (s == 1) {
-      c0 = cx(nx); c1 = cx(nx - 1); cf = c1 + sx(nx - 1) * 0.5; coe = -P->KX[nx];
+      c0 = cx(nx); c1 = cx(nx - 1); cf = c1 + sx(nx - 1) * 0.5; coe = -KX[nx];
     } else if (s == 2) {
-      c0 = cy(-1); c1 = cy(0); cf = c1 - sy(0) * 0.5; coe = -P->KY[0];
+      c0 = cy(-1); c1 = cy(0); cf = c1 - sy(0) * 0.5; coe = -KY[0];
     } else {
-      c0 = cy(ny); c1 = cy(ny - 1); cf = c1 + sy(ny - 1) * 0.5; coe = -P->KY[ny];
+      c0 = cy(ny); c1 = cy(ny - 1); cf = c1 + sy(ny - 1) * 0.5; coe = -KY[ny];
     }
     const double dl = std::fabs(c1 - c0);
     const double d0 = std::fabs(cf - c0);
     const double d1 = std::fabs(c1 - cf);
-    P->adj[s] = type == CPGPU_BC_DIRICHLET ? (-coe * d0 / d1) : coe;
-    P->seg[s].type = type;
-    P->seg[s].coe = coe;
-    P->seg[s].dl = dl;
-    P->seg[s].d1 = d1;
+    adj[s] = type == CPGPU_BC_DIRICHLET ? (-coe * d0 / d1) : coe;
+    seg[s].type = type;
+    seg[s].coe = coe;
+    seg[s].dl = dl;
+    seg[s].d1 = d1;
   }
+  for (int i = 0; i < nx; ++i) {
+    double v = 1.0;
+    for (int d = 0; d < 2; ++d) v *= sx(i);  // s_poisson.hpp:71-77 uses dim 0
+    vol[i] = v;
+  }
+  // boundary values of the interior lines
+  const int NX = nx + 2 * gl, NY = ny + 2 * gl;
+  for (int k = 0; k < 2 * ny + 2 * nx; ++k) bcv[k] = 0.0;
+  if (g->bc_val) {
+    for (int j = 0; j < ny; ++j) {
+      bcv[j] = g->bc_val[j + gl];
+      bcv[ny + j] = g->bc_val[NY + j + gl];
+    }
+    for (int i = 0; i < nx; ++i) {
+      bcv[2 * ny + i] = g->bc_val[2 * NY + i + gl];
+      bcv[2 * ny + nx + i] = g->bc_val[2 * NY + NX + i + gl];
+    }
+  }
+  return CPGPU_OK;
+}
+
+int build_descriptor(cpgpu_poisson* P, const cpgpu_grid2d* g) {
+  const int nx = g->nx, ny = g->ny, gl = g->gl;
+  P->KX.assign(nx + 1, 0.0);
+  P->KY.assign(ny + 1, 0.0);
+  P->vol.assign(nx, 0.0);
+  P->bcv.assign(2 * ny + 2 * nx, 0.0);
+  int rc = describe_grid(P->ctx, g, P->KX.data(), P->KY.data(), P->adj, P->seg,
+                         P->vol.data(), P->bcv.data());
+  if (rc) return rc;
   P->cxi.assign(g->cx + gl, g->cx + gl + nx);
   P->sxi.assign(g->sx + gl, g->sx + gl + nx);
   P->syi.assign(g->sy + gl, g->sy + gl + ny);
   P->cyi.assign(g->cy + gl, g->cy + gl + ny);
-  P->vol.assign(nx, 0.0);
-  for (int i = 0; i < nx; ++i) {
-    double v = 1.0;
-    for (int d = 0; d < 2; ++d) v *= sx(i);  // s_poisson.hpp:71-77 uses dim 0
-    P->vol[i] = v;
-  }
-  // boundary values of the interior lines
-  const int NX = nx + 2 * gl, NY = ny + 2 * gl;
-  P->bcv.assign(2 * ny + 2 * nx, 0.0);
-  if (g->bc_val) {
-    for (int j = 0; j < ny; ++j) {
-      P->bcv[j] = g->bc_val[j + gl];
-      P->bcv[ny + j] = g->bc_val[NY + j + gl];
-    }
-    for (int i = 0; i < nx; ++i) {
-      P->bcv[2 * ny + i] = g->bc_val[2 * NY + i + gl];
-      P->bcv[2 * ny + nx + i] = g->bc_val[2 * NY + NX + i + gl];
-    }
-  }
   return CPGPU_OK;
 }
 
@@ -780,6 +794,173 @@ int tune_dir(cpgpu_poisson* P, const Sten<T>& s, CGVecs<T> v, int reps,
   *best = bv;
   *best_ms = bt / 1e3f;
   return CPGPU_OK;
+}
+
+// ---------------------------------------------------------------------
+// Scratch-free solve (cpgpu_poisson2d_solve_static; no reference
+// counterpart).  Every work vector, coefficient array, reduction partial and
+// the Krylov state live in one file-scope device array per GPU, `scratch`;
+// the caller's device arrays are the only other memory touched.
+
+constexpr size_t kStaticMaxCells = 512 * 512;  // cells of the largest grid
+constexpr int kStaticMaxSide = 4096;           // cells per side
+constexpr size_t kAlign = 256;
+constexpr size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+// byte offsets: b, r, p0, p1, q | KX, KY | vol | bcv | partA | partB | state
+constexpr size_t kOffVec = 0;
+constexpr size_t kVecBytes = align_up(kStaticMaxCells * sizeof(double));
+constexpr size_t kOffKX = kOffVec + 5 * kVecBytes;
+constexpr size_t kOffKY = kOffKX + align_up((kStaticMaxSide + 1) * sizeof(double));
+constexpr size_t kOffVol = kOffKY + align_up((kStaticMaxSide + 1) * sizeof(double));
+constexpr size_t kOffBcv = kOffVol + align_up(kStaticMaxSide * sizeof(double));
+constexpr size_t kOffPartA = kOffBcv + align_up(4 * kStaticMaxSide * sizeof(double));
+constexpr size_t kOffPartB = kOffPartA + align_up(kMaxBlocks * sizeof(double));
+constexpr size_t kOffState = kOffPartB + align_up(3 * kMaxBlocks * sizeof(double));
+constexpr size_t kScratchBytes = kOffState + align_up(sizeof(KState));
+
+// Host staging of the descriptor arrays and the state, guarded by g_stage_mu
+// (the copies complete before the lock is released).
+struct StaticStage {
+  double KX[kStaticMaxSide + 1], KY[kStaticMaxSide + 1], vol[kStaticMaxSide],
+      bcv[4 * kStaticMaxSide];
+  KState st;
+};
+StaticStage g_stage;
+std::mutex g_stage_mu;
+
+}  // namespace
+
+// The file-scope device array (one copy per GPU, held until the process
+// ends).  In host code its name is a host-side stand-in: every use goes
+// through the per-device address from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+}  // namespace
+
+int static_scratch_setup(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return CPGPU_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    // the device is current (cpgpu_create set it)
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+      return CPGPU_EHIP;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return CPGPU_OK;
+}
+
+namespace {
+
+int solve_static(cpgpu_ctx* ctx, const cpgpu_grid2d* g, const double* d_src,
+                 double src_uniform, double* d_phi, int method, int* max_iter,
+                 double* tol, int norm_mode) {
+  const int nx = g->nx, ny = g->ny;
+  const size_t N = (size_t)nx * ny;
+  unsigned char* base = (unsigned char*)ctx->scratch;
+  if (!base) return set_error(ctx, CPGPU_EHIP, "static scratch not set up on this device");
+  hipStream_t st = ctx->stream;
+  double* b = (double*)(base + kOffVec);
+  double* r = (double*)(base + kOffVec + kVecBytes);
+  double* pa = (double*)(base + kOffVec + 2 * kVecBytes);
+  double* pb = (double*)(base + kOffVec + 3 * kVecBytes);
+  double* q = (double*)(base + kOffVec + 4 * kVecBytes);
+  double* dKX = (double*)(base + kOffKX);
+  double* dKY = (double*)(base + kOffKY);
+  double* dvol = (double*)(base + kOffVol);
+  double* dbcv = (double*)(base + kOffBcv);
+  double* partA = (double*)(base + kOffPartA);
+  double* partB = (double*)(base + kOffPartB);
+  KState* kst = (KState*)(base + kOffState);
+  Sten<double> s;
+  s.nx = nx;
+  s.ny = ny;
+  s.KX = dKX;
+  s.KY = dKY;
+  RhsDesc d;
+  d.nx = nx;
+  d.ny = ny;
+  d.vol = dvol;
+  d.bcval = dbcv;
+  const Tiling tl = make_tiling(nx, ny);
+  const int PCm = method == CPGPU_JPCG;
+  const int nref = norm_mode == CPGPU_NORM_REF && N > 1;
+  // a new problem's direction kernel: the dot products are partitioned as
+  // in cpgpu_poisson2d_solve, so the results are bitwise its own
+  const int variant = kDefaultDirVariant;
+  const int Gd = PCm ? dir_blocks<double, true>(variant, nx, ny, tl)
+                     : dir_blocks<double, false>(variant, nx, ny, tl);
+  {
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    StaticStage& h = g_stage;
+    int rc = describe_grid(ctx, g, h.KX, h.KY, s.adj, d.seg, h.vol, h.bcv);
+    if (rc) return rc;
+    std::memset(&h.st, 0, sizeof(KState));
+    h.st.tol = *tol;
+    h.st.max_iter = *max_iter;
+    h.st.it_dir = 1;
+    h.st.refresh_every = 16;
+    h.st.normb = 1.0;
+    CPGPU_HIP(ctx, hipMemcpyAsync(dKX, h.KX, sizeof(double) * (nx + 1),
+                                  hipMemcpyHostToDevice, st));
+    CPGPU_HIP(ctx, hipMemcpyAsync(dKY, h.KY, sizeof(double) * (ny + 1),
+                                  hipMemcpyHostToDevice, st));
+    CPGPU_HIP(ctx, hipMemcpyAsync(dvol, h.vol, sizeof(double) * nx,
+                                  hipMemcpyHostToDevice, st));
+    CPGPU_HIP(ctx, hipMemcpyAsync(dbcv, h.bcv, sizeof(double) * (2 * nx + 2 * ny),
+                                  hipMemcpyHostToDevice, st));
+    CPGPU_HIP(ctx, hipMemcpyAsync(kst, &h.st, sizeof(KState), hipMemcpyHostToDevice, st));
+    CPGPU_HIP(ctx, hipStreamSynchronize(st));
+  }
+  s.shift = 0.0;
+  hipLaunchKernelGGL(k_build_rhs, dim3(tl.G), dim3(TJ, BY), 0, st, d, d_src, src_uniform,
+                     b, tl);
+  if (PCm)
+    hipLaunchKernelGGL((k_cg_init<double, true>), dim3(tl.G), dim3(TJ, BY), 0, st, s,
+                       d_phi, b, r, partB, tl, nref);
+  else
+    hipLaunchKernelGGL((k_cg_init<double, false>), dim3(tl.G), dim3(TJ, BY), 0, st, s,
+                       d_phi, b, r, partB, tl, nref);
+  CPGPU_HIP(ctx, hipGetLastError());
+  // batches of B iterations; the kernels stop by themselves at the test,
+  // later launches return at entry; the host reads the state after each
+  // batch (the direction kernel of iteration max_iter + 1 makes the final
+  // test, so the loop ends at most one batch past it)
+  const int B = 16;
+  const long long total = (long long)*max_iter + 2;
+  KState hs{};
+  for (long long launched = 0;; launched += B) {
+    for (int k = 0; k < B; ++k) {
+      double* pold = (k & 1) ? pb : pa;
+      double* pnew = (k & 1) ? pa : pb;
+      if (PCm) {
+        launch_dir<double, true>(variant, st, s, r, pold, pnew, q, kst, partB, partA,
+                                 nullptr, 0, tl, tl.G, Gd, nullptr, tl.G);
+        launch_upd<double, true>(variant, st, s, d_phi, pnew, r, q, kst, partA, partB, tl,
+                                 nref, Gd);
+      } else {
+        launch_dir<double, false>(variant, st, s, r, pold, pnew, q, kst, partB, partA,
+                                  nullptr, 0, tl, tl.G, Gd, nullptr, tl.G);
+        launch_upd<double, false>(variant, st, s, d_phi, pnew, r, q, kst, partA, partB,
+                                  tl, nref, Gd);
+      }
+    }
+    CPGPU_HIP(ctx, hipGetLastError());
+    CPGPU_HIP(ctx, hipMemcpyAsync(&hs, kst, sizeof(KState), hipMemcpyDeviceToHost, st));
+    CPGPU_HIP(ctx, hipStreamSynchronize(st));
+    if (hs.done || launched + B >= total + B) break;
+  }
+  *tol = hs.resid;
+  if (hs.done == 1) {
+    *max_iter = hs.iter;
+    return CPGPU_OK;
+  }
+  return CPGPU_MAXITER;
 }
 
 }  // namespace
@@ -1281,6 +1462,25 @@ int cpgpu_poisson2d_solve(cpgpu_ctx* ctx, const cpgpu_grid2d* g,
   }
   cpgpu_poisson_destroy(P);
   return code;
+}
+
+int cpgpu_poisson2d_solve_static(cpgpu_ctx* ctx, const cpgpu_grid2d* g,
+                                 const double* d_src, double src_uniform, double* d_phi,
+                                 int method, int* max_iter, double* tol, int norm_mode) {
+  if (!ctx || !g || !d_phi || !max_iter || !tol || !g->cx || !g->sx || !g->cy || !g->sy)
+    return set_error(ctx, CPGPU_EINVAL, "invalid arguments");
+  if (method != CPGPU_JPCG && method != CPGPU_CG)
+    return set_error(ctx, CPGPU_EUNSUPPORTED, "static solve: CPGPU_JPCG or CPGPU_CG only");
+  if (g->nx < 1 || g->ny < 1 || g->gl < 1 || *max_iter < 0)
+    return set_error(ctx, CPGPU_EINVAL, "invalid grid");
+  if ((size_t)g->nx * g->ny > kStaticMaxCells || g->nx > kStaticMaxSide ||
+      g->ny > kStaticMaxSide)
+    return set_error(ctx, CPGPU_EINVAL,
+                     "static solve: %d x %d cells exceed the static scratch (at most %zu "
+                     "cells and %d per side)",
+                     g->nx, g->ny, kStaticMaxCells, kStaticMaxSide);
+  CPGPU_HIP(ctx, hipSetDevice(ctx->device));
+  return solve_static(ctx, g, d_src, src_uniform, d_phi, method, max_iter, tol, norm_mode);
 }
 
 }  // extern "C"

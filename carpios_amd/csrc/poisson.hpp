@@ -103,6 +103,10 @@ struct Xoshiro {
 void blob_params(unsigned long long seed, int nblobs, double wmin, double wmax,
                  double* out);
 
+// Direction-kernel variant of a new problem (the column march with 16 rows
+// per wavefront) until cpgpu_poisson_set_variant or the autotuner changes it.
+constexpr int kDefaultDirVariant = 3;
+
 // Identity of a captured batch of Krylov iterations.
 struct GraphKey {
   const void* x;
@@ -160,7 +164,7 @@ struct cpgpu_poisson {
   // direction-kernel variant per precision (0 LDS tile, 1..4 column march
   // with H = 4, 8, 16, 32); set by heuristic, cpgpu_poisson_set_variant or
   // the autotuner
-  int dir_variant64 = 3, dir_variant32 = 3;
+  int dir_variant64 = cpgpu::kDefaultDirVariant, dir_variant32 = cpgpu::kDefaultDirVariant;
   double inner_tol = 1e-4;
   int max_outer = 40;
   // attached U-Net (warm start / flexible-CG preconditioner) and the

@@ -205,6 +205,25 @@ int cpgpu_poisson2d_solve(cpgpu_ctx *ctx, const cpgpu_grid2d *g,
                           double *tol, int norm_mode, double *hist,
                           int hist_cap, int *hist_len);
 
+/* Scratch-free variant of cpgpu_poisson2d_solve, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  d_src (interior-only nx*ny device array, j contiguous, or
+ * NULL for src_uniform) and d_phi (the same layout: initial guess in,
+ * solution out) are the caller's device arrays; every other vector,
+ * coefficient array, reduction partial and the solver state live in one
+ * static device array per GPU of 10.8 MB, so grids are limited to
+ * 262144 cells (512^2) and 4096 cells per side (larger: CPGPU_EINVAL).
+ * method CPGPU_JPCG or CPGPU_CG; max_iter, tol and the codes as
+ * cpgpu_poisson2d_solve (results bitwise equal to it).  No hipMalloc,
+ * hipFree or other allocation on any call; it synchronizes the context's
+ * stream.  All calls on one GPU share that array: calls that overlap on one
+ * GPU (several contexts or host threads) race. */
+int cpgpu_poisson2d_solve_static(cpgpu_ctx *ctx, const cpgpu_grid2d *g,
+                                 const double *d_src, double src_uniform,
+                                 double *d_phi, int method, int *max_iter,
+                                 double *tol, int norm_mode);
+
 /* ---- batched solves over many problems of uneven sizes ----
  * No counterpart in the reference, which solves one problem at a time
  * (solver_matrix.hpp:264-374, the algorithm batched here).  A batch is a
