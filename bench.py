@@ -390,6 +390,54 @@ def csr_probe(ctx, ca, no_cpu, m=2048, iters=200):
     return out
 
 
+def kernel_probe(ca, mode, size, precision=32, reps=50, method="MIXED", tol=1e-8,
+                 variant=-1):
+    """Kernels alone, for profilers, on the size^2 all-Neumann blob problem
+    (64 Gaussian blobs, seed 0):
+      times: replayed graphs of back-to-back launches of the Krylov
+             direction and update kernels (cpgpu_poisson_kernel_times);
+      solve: one solve (default MIXED) with directly launched batches (the
+             launch pattern of a real solve); at 4096^2 with a loose tol its
+             k_merge_x launches (x64 += x32 as an exact pair, x32 = 0) stream
+             vectors far larger than the caches;
+      unet:  `reps` forward passes of the U-Net (seed 1) on a size^2 input."""
+    n = size
+    ctx = ca.Context(0)
+    if mode == "unet":
+        net = ca.UNet(ctx, seed=1)
+        x = np.random.default_rng(0).standard_normal((n, n)).astype(np.float32)
+        ctx.timer_enable(2)
+        for _ in range(reps):
+            net.forward(x)
+        k, ms = ctx.timer_read(2)
+        net.close()
+        ctx.close()
+        # net.forward keeps every activation: both finest-level 16 -> 16
+        # convolutions store their output (one also its 4 x 4 pooling)
+        return {"size": n, "mode": "unet", "reps": reps, "conv16_launches": k,
+                "conv16_us": round(ms * 1e3 / max(k, 1), 3)}
+    P = ca.Poisson(ctx, ca.uniform_grid_desc(n, n, 1, [ca.BC_NEUMANN] * 4))
+    P.set_source_blobs(seed=0, nblobs=64, wmin=0.01, wmax=0.1)
+    tune = None
+    if variant >= 0:
+        P.set_variant(variant, variant)
+    else:
+        tune = P.autotune(reps=10)
+    out = {"size": n, "mode": mode, "autotune": tune, "variant": variant}
+    if mode == "times":
+        d, u = P.kernel_times(precision, reps=reps)
+        out.update(precision=precision, reps=reps, dir_us=round(d, 3), upd_us=round(u, 3))
+    else:
+        P.set_batch(16, use_graphs=False)
+        P.fill_phi(0.0)
+        code, it, res, _ = P.solve(method, max_iter=10 ** 7, tol=tol, norm=ca.NORM_TRUE,
+                                   opts=ca.OPT_PROJECT_MEAN)
+        out.update(method=method, tol=tol, code=code, iterations=it, residual=res)
+    P.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -403,6 +451,13 @@ def main():
                     help="flexible-CG sample rounds after the first fit")
     ap.add_argument("--first-fit", type=int, default=2000)
     ap.add_argument("--round-fit", type=int, default=1500)
+    ap.add_argument("--round-steps", type=int, default=40)
+    ap.add_argument("--save-net", default="",
+                    help="save the trained network and the cavity state (npz)")
+    ap.add_argument("--load-net", default="",
+                    help="skip training: load a network and cavity state saved by --save-net")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="time the learned steps alone and stop (for profilers)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 256-problem batched solve")
@@ -414,6 +469,13 @@ def main():
                     help="skip the general-CSR Jacobi-PCG probe")
     ap.add_argument("--no-hbm", action="store_true",
                     help="skip the 4096^2 Krylov-kernel HBM probe")
+    ap.add_argument("--probe", choices=("times", "solve", "unet"), default="",
+                    help="kernels alone on the --size^2 blob problem, for profilers")
+    ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
+    ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--method", default="MIXED")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="direction-kernel variant of --probe (default: autotune)")
     ap.add_argument("--only", default="",
                     help="comma-separated legs to run alone (ns, blob, kim, csr, batch)")
     ap.add_argument("--no-ns", action="store_true",
@@ -432,6 +494,11 @@ def main():
 
     import carpios_amd as ca
     from carpios_amd import learned
+
+    if a.probe:
+        print(json.dumps(kernel_probe(ca, a.probe, a.size, a.precision, a.reps, a.method,
+                                      a.tol, a.variant)), flush=True)
+        return
 
     if a.only:
         # the named legs alone, one JSON line of their results (no headline)
@@ -454,19 +521,37 @@ def main():
     ns = learned.cavity(ctx, n, tol=tol)
     P = ns.projection()
     if a.no_graphs:
+        # the autotuner times its variants with replayed graphs: profiled runs
+        # keep the default direction kernel
         P.set_batch(16, use_graphs=False)
-    tune = P.autotune(reps=20)
+        tune = None
+    else:
+        tune = P.autotune(reps=20)
     log(f"bench: {n}^2 cavity ready; Jacobi-PCG kernel autotune {tune}")
 
     # ---- training (untimed) ----
     net = ca.UNet(ctx, seed=1)
     tr = learned.CavityTrainer(ns, net, nu=2, omega=0.8, tol=tol)
     t0 = time.perf_counter()
-    tr.run(plain=a.train_plain, first_fit=a.first_fit, rounds=a.rounds,
-           round_steps=40, round_fit=a.round_fit)
+    if a.load_net:
+        # a network and cavity state saved by an earlier --save-net run (a
+        # profiler run then holds the learned solves without the training)
+        z = np.load(a.load_net, allow_pickle=False)
+        net.set_params(z["params"])
+        ns.restore({"fields": {k: z["f_" + k] for k in ns.STATE},
+                    "time": (float(z["t"]), float(z["dt"]))})
+        tr.slot = int(z["samples"])
+        tr.log = [("loaded", a.load_net)]
+    else:
+        tr.run(plain=a.train_plain, first_fit=a.first_fit, rounds=a.rounds,
+               round_steps=a.round_steps, round_fit=a.round_fit)
     ctx.sync()
     t_train = time.perf_counter() - t0
     log(f"bench: trained in {t_train:.1f} s on {tr.slot} samples: {tr.log}")
+    if a.save_net:
+        st = ns.save()
+        np.savez(a.save_net, params=net.get_params(), t=st["time"][0], dt=st["time"][1],
+                 samples=tr.slot, **{"f_" + k: v for k, v in st["fields"].items()})
 
     # ---- warm-up steps of the learned solver, then the common state ----
     P.set_unet(net, ca.UNET_PRECOND)
@@ -497,6 +582,11 @@ def main():
     wl, itl, fl = timed("FCG_UNET", ca.UNET_PRECOND)
     value = K / wl
     log(f"bench: learned {value:.2f} solves/s, iterations {itl}")
+    if a.learned_only:
+        print(json.dumps({"metric": "learned solves/s (timed steps only)", "value": round(value, 4),
+                          "ms_per_step": round(wl / K * 1e3, 3), "steps": K,
+                          "iterations_learned": itl}), flush=True)
+        return
     wp, itp, fp = timed("JPCG", 0)
     log(f"bench: plain FP64 J-PCG {K / wp:.2f} solves/s, iterations {itp}")
     wm, itm, fm = timed("MIXED", 0)
