@@ -451,7 +451,9 @@ def main():
                     help="flexible-CG sample rounds after the first fit")
     ap.add_argument("--first-fit", type=int, default=2000)
     ap.add_argument("--round-fit", type=int, default=1500)
-    ap.add_argument("--round-steps", type=int, default=40)
+    ap.add_argument("--round-steps", type=int, default=67,
+                    help="cavity steps per flexible-CG sample round (6 samples each: with "
+                         "the plain steps' 160, 2170 training samples by default)")
     ap.add_argument("--save-net", default="",
                     help="save the trained network and the cavity state (npz)")
     ap.add_argument("--load-net", default="",
@@ -561,10 +563,10 @@ def main():
     state = ns.save()
     t_state = ns.time()[0]
 
-    def timed(solver, flags):
+    def timed(solver, flags, tol_=tol):
         ns.restore(state)
         P.set_unet(net if flags else None, flags)
-        ns.set_solver(solver, max_iter=10 ** 7, tol=tol, norm=ca.NORM_TRUE)
+        ns.set_solver(solver, max_iter=10 ** 7, tol=tol_, norm=ca.NORM_TRUE)
         its, res = [], []
         ctx.sync()
         t = time.perf_counter()
@@ -574,10 +576,18 @@ def main():
             res.append(float(r[0]))
         ctx.sync()
         wall = time.perf_counter() - t
-        if max(res) > tol:
+        if max(res) > tol_:
             raise RuntimeError(f"{solver}: a step ended at residual {max(res):.3e}")
         fields = {k: ns.get_field(k) for k in ("u", "v", "p")}
         return wall, its, fields
+
+    def field_diff(fa, fb):
+        g = 2
+        inner = (slice(g, g + n), slice(g, g + n))
+        pa_, pb_ = fa["p"][inner], fb["p"][inner]
+        return {"u": rel_l2(fa["u"][inner], fb["u"][inner]),
+                "v": rel_l2(fa["v"][inner], fb["v"][inner]),
+                "p_zero_mean": rel_l2(pa_ - pa_.mean(), pb_ - pb_.mean())}
 
     wl, itl, fl = timed("FCG_UNET", ca.UNET_PRECOND)
     value = K / wl
@@ -591,13 +601,21 @@ def main():
     log(f"bench: plain FP64 J-PCG {K / wp:.2f} solves/s, iterations {itp}")
     wm, itm, fm = timed("MIXED", 0)
     log(f"bench: FP32 J-PCG in FP64 refinement {K / wm:.2f} solves/s, iterations {itm}")
-    g = 2
-    inner = (slice(g, g + n), slice(g, g + n))
-    pl, pp = fl["p"][inner], fp["p"][inner]
-    field_err = {"u": rel_l2(fl["u"][inner], fp["u"][inner]),
-                 "v": rel_l2(fl["v"][inner], fp["v"][inner]),
-                 "p_zero_mean": rel_l2(pl - pl.mean(), pp - pp.mean())}
+    field_err = field_diff(fl, fp)
     log(f"bench: learned vs plain FP64 fields after {K} steps: {field_err}")
+    # the same steps with both solvers at 1e-10: two solutions that each
+    # stop at a relative residual tau differ by up to the condition number
+    # times tau (~4e5 at 1024^2), so fields that agree to 1e-8 need the
+    # tighter stop
+    tight = 1e-10
+    wl2, itl2, fl2 = timed("FCG_UNET", ca.UNET_PRECOND, tight)
+    wp2, itp2, fp2 = timed("JPCG", 0, tight)
+    field_tight = {"tol": tight, "rel_l2": field_diff(fl2, fp2),
+                   "learned_solves_per_s": round(K / wl2, 3),
+                   "plain_fp64_solves_per_s": round(K / wp2, 3),
+                   "speedup": round(wp2 / wl2, 3),
+                   "iterations_learned": itl2, "iterations_plain_fp64": itp2}
+    log(f"bench: at {tight:g}: {field_tight}")
 
     # ---- dominant U-Net kernel: timestamps of its own dispatches ----
     ns.restore(state)
@@ -719,6 +737,13 @@ def main():
                    "method": "FCG_UNET: flexible CG, preconditioner = 2 Jacobi sweeps + "
                              "U-Net + 2 Jacobi sweeps, FP64 J-PCG fallback",
                    "cells": N, "start_time": t_state,
+                   "unet": "4 levels 16-32-64-128, 4x4 average pooling and 4x4 stride-4 "
+                           "up-convolutions (1024^2 -> 16^2 bottleneck), 610913 parameters, "
+                           "about 25 GFLOP per 1024^2 application",
+                   "training_samples": tr.slot,
+                   "training_set": "error samples (warm-start, zero-start and Jacobi-PCG/"
+                                   "flexible-CG iterate errors with their residuals) from "
+                                   "the plain solver's cavity steps",
                    "iterations_learned": itl, "iterations_plain_fp64": itp,
                    "iterations_mixed": itm, "autotune": tune,
                    "training": {"seconds": round(t_train, 1), "samples": tr.slot,
@@ -729,6 +754,7 @@ def main():
         "gpu_mixed_jpcg": {"solves_per_s": round(K / wm, 4),
                            "speedup_of_value": round(wm / wl, 3)},
         "field_rel_l2_vs_fp64": field_err,
+        "field_rel_l2_vs_fp64_at_1e-10": field_tight,
         "roofline": roofline,
         "krylov_kernels_1024": krylov,
         "roofline_hbm_probe": hbm,
