@@ -55,3 +55,26 @@ def test_packed_boundary_values_are_checked_on_the_host():
                 [1.0, 2.0, 3.0]):
         with pytest.raises(ValueError):
             ca.pack_bc_values(nx, ny, gl, bad)
+
+
+def test_cpp_bindings_compile():
+    # the C++ host side a maintainer adds to the reference: the structured
+    # helper needs only the C-ABI header; the algebra solvers' counterparts
+    # compile against the reference's own lib/algebra where that tree is
+    import shutil
+    import subprocess
+    import pytest
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("no g++")
+    inc = os.path.join(ROOT, "include")
+    r = subprocess.run([gxx, "-std=c++11", "-fsyntax-only", "-Wall", "-I" + inc,
+                        os.path.join(ROOT, "tests", "cpp", "caller.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    ref = "/root/reference/lib"
+    if os.path.isdir(os.path.join(ref, "algebra")):
+        r = subprocess.run([gxx, "-std=c++11", "-fsyntax-only", "-w", "-I" + ref, "-I" + inc,
+                            os.path.join(ROOT, "tests", "cpp", "algebra_shim.cpp")],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
