@@ -77,6 +77,7 @@ EXPORTS = [
     "cpgpu_csr_destroy", "cpgpu_csr_mult", "cpgpu_csr_solve",
     "cpgpu_poisson_set_alpha", "cpgpu_poisson_set_bc_values", "cpgpu_ns_set_kim",
     "cpgpu_ns_kim_iters", "cpgpu_ns_set_bc_values", "cpgpu_poisson2d_solve_static",
+    "cpgpu_grid_axis_sizes",
 ]
 
 
@@ -184,6 +185,27 @@ def uniform_axis(n, gl, vmin, vmax):
     if rc != OK:
         raise CpgpuError(rc, "bad axis")
     return c, s
+
+
+def axis_from_sizes(sizes, gl, vmin=0.0):
+    """(centres, sizes) of an axis of given interior cell sizes, ghosts
+    included (structure::Grid_ from size arrays, s_grid.hpp:397-434)."""
+    sizes = np.ascontiguousarray(sizes, dtype=np.float64)
+    n = len(sizes)
+    c = np.zeros(n + 2 * gl)
+    s = np.zeros(n + 2 * gl)
+    rc = lib().cpgpu_grid_axis_sizes(C.c_int(n), C.c_int(gl), C.c_double(vmin), _p(sizes),
+                                     _p(c), _p(s))
+    if rc != OK:
+        raise CpgpuError(rc, "bad axis sizes")
+    return c, s
+
+
+def sized_grid_desc(xsizes, ysizes, gl, bc_type, bc_val=None, beta=1.0, x0=0.0, y0=0.0):
+    """Descriptor of a grid of given cell sizes along x and y."""
+    cx, sx = axis_from_sizes(xsizes, gl, x0)
+    cy, sy = axis_from_sizes(ysizes, gl, y0)
+    return make_grid_desc(cx, sx, cy, sy, len(xsizes), len(ysizes), gl, bc_type, bc_val, beta)
 
 
 def uniform_grid_desc(nx, ny, gl, bc_type, bc_val=None, beta=1.0,
@@ -356,9 +378,15 @@ class NS:
                  bcu=((1, 1, 1, 1), None), bcv=((1, 1, 1, 1), None), rho=1.0,
                  mu=1.0, cfl=0.5, dnum=1e10, dcoef=1.0, limiter="VanLeer",
                  solver="JPCG", max_iter=100000, tol=1e-10, norm=NORM_REF,
-                 dt=1e-3):
+                 dt=1e-3, xsizes=None, ysizes=None):
         self.ctx, self.nx, self.ny, self.gl = ctx, nx, ny, gl
-        g = uniform_grid_desc(nx, ny, gl, [2, 2, 2, 2], xlim=xlim, ylim=ylim)
+        if xsizes is None and ysizes is None:
+            g = uniform_grid_desc(nx, ny, gl, [2, 2, 2, 2], xlim=xlim, ylim=ylim)
+        else:
+            # cell sizes from arrays (starting at xlim[0], ylim[0])
+            xs = np.full(nx, (xlim[1] - xlim[0]) / nx) if xsizes is None else xsizes
+            ys = np.full(ny, (ylim[1] - ylim[0]) / ny) if ysizes is None else ysizes
+            g = sized_grid_desc(xs, ys, gl, [2, 2, 2, 2], x0=xlim[0], y0=ylim[0])
         d = NSDesc()
         d.grid = g
         self._keep = [g, g._keep]

@@ -571,6 +571,8 @@ int cpgpu_batch_create(cpgpu_ctx* ctx, int n, cpgpu_poisson* const* problems,
   for (int p = 0; p < n; ++p) {
     cpgpu_poisson* Q = problems[p];
     if (!Q || Q->ctx != ctx) return set_error(ctx, CPGPU_EINVAL, "problem %d: not on this context", p);
+    if (Q->general)
+      return set_error(ctx, CPGPU_EUNSUPPORTED, "problem %d: cell sizes vary (single solves only)", p);
     int neumann = 0;
     for (int k = 0; k < 4; ++k) neumann += Q->seg[k].type == CPGPU_BC_NEUMANN;
     if (neumann == 4)

@@ -17,11 +17,6 @@ namespace cpgpu {
 // ---------------------------------------------------------------------
 // kernels other than the Krylov iteration (stencil.hpp)
 
-__device__ __forceinline__ double ghost_const(const RhsSeg& g, double cst,
-                                              double val) {
-  if (g.type == CPGPU_BC_DIRICHLET) return cst + (g.coe * g.dl * val / g.d1);
-  return cst + (-val * g.dl * g.coe);
-}
 
 __global__ __launch_bounds__(256) void k_build_rhs(RhsDesc d,
                                                    const double* __restrict__ src,
@@ -228,11 +223,9 @@ int describe_grid(cpgpu_ctx* ctx, const cpgpu_grid2d* g, double* KX, double* KY,
   auto sx = [&](int i) { return g->sx[i + gl]; };
   auto cy = [&](int j) { return g->cy[j + gl]; };
   auto sy = [&](int j) { return g->sy[j + gl]; };
-  // the kernels use 1-D face coefficients: a = sy[j] (x faces) and sx[i]
-  // (y faces) must not vary along the face's own line
-  if (!all_equal(g->sy + gl, ny) || !all_equal(g->sx + gl, nx))
-    return set_error(ctx, CPGPU_EUNSUPPORTED,
-                     "non-uniform cell sizes are not supported");
+  // the 1-D face coefficients are exact only when a = sy[j] (x faces) and
+  // sx[i] (y faces) do not vary along the face's own line; other grids take
+  // the per-face arrays of general.hip (KX, KY and adj here are then unused)
   const double beta = g->beta;
   for (int f = 0; f <= nx; ++f) {
     const double a = sy(0) * 1.0;
@@ -292,6 +285,10 @@ int describe_grid(cpgpu_ctx* ctx, const cpgpu_grid2d* g, double* KX, double* KY,
   return CPGPU_OK;
 }
 
+bool uniform_cells(const cpgpu_grid2d* g) {
+  return all_equal(g->sy + g->gl, g->ny) && all_equal(g->sx + g->gl, g->nx);
+}
+
 int build_descriptor(cpgpu_poisson* P, const cpgpu_grid2d* g) {
   const int nx = g->nx, ny = g->ny, gl = g->gl;
   P->KX.assign(nx + 1, 0.0);
@@ -301,6 +298,7 @@ int build_descriptor(cpgpu_poisson* P, const cpgpu_grid2d* g) {
   int rc = describe_grid(P->ctx, g, P->KX.data(), P->KY.data(), P->adj, P->seg,
                          P->vol.data(), P->bcv.data());
   if (rc) return rc;
+  if (!uniform_cells(g) && (rc = general_build(P, g))) return rc;
   P->cxi.assign(g->cx + gl, g->cx + gl + nx);
   P->sxi.assign(g->sx + gl, g->sx + gl + nx);
   P->syi.assign(g->sy + gl, g->sy + gl + ny);
@@ -898,6 +896,8 @@ int solve_static(cpgpu_ctx* ctx, const cpgpu_grid2d* g, const double* d_src,
   {
     std::lock_guard<std::mutex> lock(g_stage_mu);
     StaticStage& h = g_stage;
+    if (!uniform_cells(g))
+      return set_error(ctx, CPGPU_EUNSUPPORTED, "static solve: uniform cell sizes only");
     int rc = describe_grid(ctx, g, h.KX, h.KY, s.adj, d.seg, h.vol, h.bcv);
     if (rc) return rc;
     std::memset(&h.st, 0, sizeof(KState));
@@ -967,6 +967,7 @@ int solve_static(cpgpu_ctx* ctx, const cpgpu_grid2d* g, const double* d_src,
 
 int poisson_rhs_from_device(cpgpu_poisson* P, const double* dsrc,
                             double src_uniform) {
+  if (P->general) return general_rhs(P, dsrc, src_uniform);
   RhsDesc d;
   d.nx = P->nx;
   d.ny = P->ny;
@@ -1000,6 +1001,9 @@ int poisson_solve_internal(cpgpu_poisson* P, int method, int* max_iter,
                            int hist_cap, int* hist_len) {
   cpgpu_ctx* ctx = P->ctx;
   int hl = 0;
+  if (P->general)
+    return general_solve(P, method, max_iter, tol, norm_mode == CPGPU_NORM_REF && P->N > 1,
+                         hist, hist_cap, hist_len);
   if (method == CPGPU_BICGSTAB || method == CPGPU_JACOBI) {
     const int nref = norm_mode == CPGPU_NORM_REF && P->N > 1;
     int code = method == CPGPU_BICGSTAB
@@ -1084,6 +1088,25 @@ int cpgpu_grid_uniform_axis(int n, int gl, double vmin, double vmax, double* c,
   const int ng = n + 2 * gl;
   const double ds = (vmax - vmin) / n;
   for (int k = 0; k < ng; ++k) s[k] = ds;
+  double csum = 0.0;
+  for (int k = 0; k < ng; ++k) {
+    c[k] = csum + 0.5 * s[k];
+    csum += s[k];
+  }
+  double llg = 0.0;
+  for (int k = 0; k < gl; ++k) llg += s[k];
+  const double sh = vmin - llg;
+  for (int k = 0; k < ng; ++k) c[k] += sh;
+  return CPGPU_OK;
+}
+
+int cpgpu_grid_axis_sizes(int n, int gl, double vmin, const double* sizes, double* c,
+                          double* s) {
+  if (n < 1 || gl < 0 || !sizes || !c || !s) return CPGPU_EINVAL;
+  for (int k = 0; k < n; ++k)
+    if (!(sizes[k] > 0.0)) return CPGPU_EINVAL;
+  const int ng = n + 2 * gl;
+  for (int k = 0; k < ng; ++k) s[k] = k < gl ? sizes[0] : (k - gl >= n ? sizes[n - 1] : sizes[k - gl]);
   double csum = 0.0;
   for (int k = 0; k < ng; ++k) {
     c[k] = csum + 0.5 * s[k];
@@ -1311,6 +1334,13 @@ int cpgpu_poisson_autotune(cpgpu_poisson* P, int reps, int* v64, int* v32,
                            float* us64, float* us32) {
   if (!P || reps < 1) return CPGPU_EINVAL;
   cpgpu_ctx* ctx = P->ctx;
+  if (P->general) {  // one kernel family: nothing to choose
+    if (v64) *v64 = -1;
+    if (v32) *v32 = -1;
+    if (us64) *us64 = 0.f;
+    if (us32) *us32 = 0.f;
+    return CPGPU_OK;
+  }
   CPGPU_HIP(ctx, hipSetDevice(ctx->device));
   // keep phi: the tuning runs Krylov kernels on the problem's vectors
   DevBuf<double> save;
@@ -1348,6 +1378,8 @@ int cpgpu_poisson_kernel_times(cpgpu_poisson* P, int precision, int reps,
       (precision != 32 && precision != 64))
     return CPGPU_EINVAL;
   cpgpu_ctx* ctx = P->ctx;
+  if (P->general)
+    return set_error(ctx, CPGPU_EUNSUPPORTED, "kernel times: uniform cell sizes only");
   CPGPU_HIP(ctx, hipSetDevice(ctx->device));
   DevBuf<double> save;
   if (save.alloc(P->N)) return set_error(ctx, CPGPU_ENOMEM, "timing buffer");
@@ -1413,14 +1445,19 @@ int cpgpu_poisson_apply(cpgpu_poisson* P, const double* x_field,
   if (!P || !x_field || !y_field) return CPGPU_EINVAL;
   int rc = field_to_dev(P, x_field, P->p0.p);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_apply<double>, dim3(P->tl.G), dim3(TJ, BY), 0,
-                     P->ctx->stream, P->sten64(), P->p0.p, P->q.p, P->tl);
-  CPGPU_HIP(P->ctx, hipGetLastError());
+  if (P->general) {
+    if ((rc = general_apply(P, P->p0.p, P->q.p))) return rc;
+  } else {
+    hipLaunchKernelGGL(k_apply<double>, dim3(P->tl.G), dim3(TJ, BY), 0,
+                       P->ctx->stream, P->sten64(), P->p0.p, P->q.p, P->tl);
+    CPGPU_HIP(P->ctx, hipGetLastError());
+  }
   return dev_to_field(P, P->q.p, y_field);
 }
 
 int cpgpu_poisson_residual(cpgpu_poisson* P, double* rel) {
   if (!P || !rel) return CPGPU_EINVAL;
+  if (P->general) return general_residual(P, rel);
   int rc = residual64(P, true);
   if (rc) return rc;
   double nb = std::sqrt(P->h_rst->bb);

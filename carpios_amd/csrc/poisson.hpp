@@ -36,6 +36,13 @@ struct RhsSeg {
   double coe;     // coefficient of the ghost term (negative)
   double dl, d1;  // distances of the substitution
 };
+// The constant a ghost term leaves in its row (s_ghost.hpp:171-176):
+// Dirichlet coe*dl*val/d1, Neumann -val*dl*coe, added to cst.
+__device__ __forceinline__ double ghost_const(const RhsSeg& g, double cst, double val) {
+  if (g.type == CPGPU_BC_DIRICHLET) return cst + (g.coe * g.dl * val / g.d1);
+  return cst + (-val * g.dl * g.coe);
+}
+
 struct RhsDesc {
   int nx, ny;
   const double* vol;    // nx: (1.0*sx)*sx
@@ -129,6 +136,12 @@ struct cpgpu_poisson {
   size_t N = 0;
   double beta = 1.0;
   double shift = 0.0;  // Helmholtz vol*alpha (0: Poisson)
+  // cell sizes that vary along an axis: the operator is held as per-face
+  // coefficient arrays and per-line ghost data (general.hip), and only the
+  // FP64 Jacobi-PCG / CG of the reference run on it
+  bool general = false;
+  cpgpu::DevBuf<double> gFX, gFY, gD;
+  cpgpu::DevBuf<cpgpu::RhsSeg> gseg;
   // where b came from, so that a new alpha can rebuild it: 0 none, 1 the
   // problem's own source field (src), 2 a uniform value (src_uni), 3 a
   // caller's device field (not kept)
@@ -226,6 +239,15 @@ int poisson_solve_internal(cpgpu_poisson* P, int method, int* max_iter,
 // b from a device source field (interior-only, j contiguous).
 int poisson_rhs_from_device(cpgpu_poisson* P, const double* dsrc,
                             double src_uniform);
+// Grids whose cell sizes vary (general.hip): the operator from the
+// descriptor, b, y = A x on device vectors, the reference's Jacobi-PCG / CG,
+// and the true relative residual.
+int general_build(cpgpu_poisson* P, const cpgpu_grid2d* g);
+int general_rhs(cpgpu_poisson* P, const double* dsrc, double src_uniform);
+int general_apply(cpgpu_poisson* P, const double* dx, double* dy);
+int general_solve(cpgpu_poisson* P, int method, int* max_iter, double* tol, int norm_ref,
+                  double* hist, int hist_cap, int* hist_len);
+int general_residual(cpgpu_poisson* P, double* rel);
 // FP32 operator coefficients and FP32 work vectors of the problem.
 int poisson_prepare_f32(cpgpu_poisson* P);
 // Plain FP64 Jacobi-PCG from the current phi (the reference's test norm

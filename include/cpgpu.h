@@ -82,6 +82,14 @@ typedef struct {
  * 372-395): sizes (vmax-vmin)/n, centres accumulated cell by cell. */
 int cpgpu_grid_uniform_axis(int n, int gl, double vmin, double vmax, double *c,
                             double *s);
+/* The same arrays of an axis given its n interior cell sizes
+ * (Grid_::_set_non_uniform_grid, s_grid.hpp:397-434): each ghost layer
+ * repeats the nearest interior size, centres accumulated from vmin.  Grids
+ * whose sizes vary take per-face coefficient arrays (FX, FY, centre
+ * coefficients: 3 N doubles more) and solve with CPGPU_JPCG or CPGPU_CG;
+ * the projection step (cpgpu_ns_*) runs on them. */
+int cpgpu_grid_axis_sizes(int n, int gl, double vmin, const double *sizes,
+                          double *c, double *s);
 
 /* ---- context ---- */
 /* Replaces: nothing in the reference (it is single-threaded host code); owns

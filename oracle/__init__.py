@@ -101,13 +101,28 @@ class Axis:
                                C.c_double(vmax), _ptr(self.c), _ptr(self.s))
 
 
-class StructGrid:
-    """Uniform structured grid as lib/structure/s_grid.hpp builds it."""
+class SizedAxis:
+    """Axis of given interior cell sizes (Grid_::_set_non_uniform_grid)."""
 
-    def __init__(self, nx, ny, gl=1, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0):
+    def __init__(self, sizes, gl, vmin):
+        sizes = np.ascontiguousarray(sizes, dtype=np.float64)
+        self.n, self.gl = len(sizes), gl
+        self.c = np.zeros(self.n + 2 * gl)
+        self.s = np.zeros(self.n + 2 * gl)
+        lib().orc_axis_sizes(C.c_int(self.n), C.c_int(gl), C.c_double(vmin),
+                             _ptr(sizes), _ptr(self.c), _ptr(self.s))
+
+
+class StructGrid:
+    """Structured grid as lib/structure/s_grid.hpp builds it: uniform on
+    [xmin, xmax] x [ymin, ymax], or from interior cell sizes (xsizes,
+    ysizes) starting at (xmin, ymin)."""
+
+    def __init__(self, nx, ny, gl=1, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0,
+                 xsizes=None, ysizes=None):
         self.nx, self.ny, self.gl = nx, ny, gl
-        self.ax = Axis(nx, gl, xmin, xmax)
-        self.ay = Axis(ny, gl, ymin, ymax)
+        self.ax = Axis(nx, gl, xmin, xmax) if xsizes is None else SizedAxis(xsizes, gl, xmin)
+        self.ay = Axis(ny, gl, ymin, ymax) if ysizes is None else SizedAxis(ysizes, gl, ymin)
         self.c = Grid(nx, ny, gl, _ptr(self.ax.c), _ptr(self.ax.s),
                       _ptr(self.ay.c), _ptr(self.ay.s))
 

@@ -74,6 +74,27 @@ void orc_axis_uniform(int n, int gl, double vmin, double vmax, double *c,
   for (ii = 0; ii < ng; ++ii) c[ii] += sh;
 }
 
+/* lib/structure/s_grid.hpp:397-434 (_set_non_uniform_grid): interior sizes
+ * from the caller, each ghost layer repeats the nearest interior size, then
+ * the centres exactly as for the uniform axis. */
+void orc_axis_sizes(int n, int gl, double vmin, const double *sizes, double *c,
+                    double *s) {
+  int ng = n + 2 * gl, ii;
+  double csum = 0.0, llg = 0.0, sh;
+  for (ii = 0; ii < ng; ++ii) {
+    if (ii < gl) s[ii] = sizes[0];
+    else if (ii - gl >= n) s[ii] = sizes[n - 1];
+    else s[ii] = sizes[ii - gl];
+  }
+  for (ii = 0; ii < ng; ++ii) {
+    c[ii] = csum + 0.5 * s[ii];
+    csum += s[ii];
+  }
+  for (ii = 0; ii < gl; ++ii) llg += s[ii];
+  sh = vmin - llg;
+  for (ii = 0; ii < ng; ++ii) c[ii] += sh;
+}
+
 /* lib/algebra/BLAS_Level1.hpp:500-544: the 5-way unrolled loop still adds
  * left to right, so it is a sequential sum. */
 double orc_dot(int n, const double *x, const double *y) {

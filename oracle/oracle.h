@@ -74,6 +74,9 @@ typedef struct {
 /* ---- grid ---- */
 void orc_axis_uniform(int n, int gl, double vmin, double vmax, double *c,
                       double *s);
+/* Grid_::_set_non_uniform_grid: n interior sizes, ghosts repeat the ends */
+void orc_axis_sizes(int n, int gl, double vmin, const double *sizes, double *c,
+                    double *s);
 
 /* ---- BLAS-1 ---- */
 double orc_dot(int n, const double *x, const double *y);
