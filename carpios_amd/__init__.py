@@ -72,7 +72,7 @@ EXPORTS = [
     "cpgpu_poisson_error_norms", "cpgpu_ns_error_norms", "cpgpu_batch_create",
     "cpgpu_batch_destroy", "cpgpu_batch_set_tiles", "cpgpu_batch_set_graphs",
     "cpgpu_batch_solve", "cpgpu_batch_autotune", "cpgpu_batch_draw",
-    "cpgpu_ns_field_change", "cpgpu_unet_set_conv_shape", "cpgpu_unet_clear_samples",
+    "cpgpu_ns_field_change", "cpgpu_unet_set_conv_shape", "cpgpu_unet_set_fusion", "cpgpu_unet_clear_samples",
     "cpgpu_unet_set_walls", "cpgpu_copy_bandwidth", "cpgpu_csr_create",
     "cpgpu_csr_destroy", "cpgpu_csr_mult", "cpgpu_csr_solve",
     "cpgpu_poisson_set_alpha", "cpgpu_poisson_set_bc_values", "cpgpu_ns_set_kim",
@@ -711,6 +711,10 @@ class UNet:
 
     def set_conv_shape(self, rows=4, persist=4):
         self.ctx.check(lib().cpgpu_unet_set_conv_shape(self.h, C.c_int(rows), C.c_int(persist)))
+
+    def set_fusion(self, level0=True):
+        """Run an applied network's finest level in the two fused kernels."""
+        self.ctx.check(lib().cpgpu_unet_set_fusion(self.h, C.c_int(int(bool(level0)))))
 
     def forward(self, x):
         """Raw network output for an (H, W) array (no input scaling)."""

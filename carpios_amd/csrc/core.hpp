@@ -26,7 +26,7 @@ struct cpgpu_ctx {
   // device address of this GPU's static scratch (cpgpu_poisson2d_solve_static)
   void* scratch = nullptr;
   // kernel timers: per family a list of (start, stop) event pairs
-  static constexpr int kFamilies = 8;
+  static constexpr int kFamilies = 10;
   bool timer_on[kFamilies] = {};
   std::vector<hipEvent_t> ev[kFamilies];
   size_t ev_used[kFamilies] = {};

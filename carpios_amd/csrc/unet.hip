@@ -125,6 +125,316 @@ __global__ __launch_bounds__(256) void k_first_fwd(const float* __restrict__ x,
   __builtin_memcpy(a.p + (size_t)m * a.cs + a.co, out, 32);
 }
 
+// ---------------------------------------------------------------------
+// Level 0 of an applied network (no backward pass follows) in two passes.
+// Training keeps the layer-per-kernel forward, whose activations the
+// backward pass reads; an application needs only the skip tensor between
+// the encoder and the decoder at full resolution, so the two fused kernels
+// below keep every other level-0 activation in LDS.  Each walks 16 x 16
+// pixel tiles persistently (TileWalk order), recomputes the halo an inner
+// layer needs, and repeats the per-value arithmetic of the layer-per-kernel
+// path (same products, same MFMA operand assignment and tap order, same
+// BF16 rounding points), so an applied network's output equals the plain
+// forward's with the output layer fused.
+
+// Encoder: the first layer (2 -> 16, FP32, as k_first_fwd with the input
+// scaled by 1 / *scale as loaded) on the tile's 18 x 18 halo into LDS, then
+// the 16 -> 16 convolution with k_conv3_tile16p's pooled epilogue into skip
+// (the level-0 skip channels) and pout (the level-1 input).  The first
+// layer's input window is 20 x 20.  Where that window avoids the boundary
+// rows and columns the wall channel is zero there and its products are
+// skipped: fma(w, 0, s) = s up to the sign of a zero, which the ReLU and
+// every later product and sum leave without effect.  Item (pixel, channel
+// half h) per thread, h wave-uniform so that the weights are scalar loads.
+__global__ __launch_bounds__(256, 4) void k_enc0_apply(const float* __restrict__ x,
+                                                      const float* __restrict__ scale, int H,
+                                                      int W, int walls,
+                                                      const float* __restrict__ w0,
+                                                      const float* __restrict__ b0,
+                                                      const bf16* __restrict__ w1,
+                                                      const float* __restrict__ b1,
+                                                      View skip, View pout, int ntx,
+                                                      int ntiles) {
+  constexpr int XP = 20, TP = 18, NX = XP * XP;
+  __shared__ float sxf[NX];
+  __shared__ __attribute__((aligned(16))) bf16 sa[TP * TP * 16];
+  typedef Frag<32> F;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  F::T wf[5][1];
+  conv16_weights<16, 1>(w1, lane, wf);
+  Conv16Epi<1> ep;
+  ep.load(b1, nullptr, nullptr, lane);
+  const float sc = 1.f / *scale;
+  const int h = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 7);
+  const float* w0h = w0 + 16 * h;  // this half's weights: w0h[q * 32 + 2 u + {0, 1}]
+  const float* b0h = b0 + 8 * h;
+  const int pl = threadIdx.x & 127;
+  float pre[2];
+  auto fetch = [&](int t) {
+    const int ty = t / ntx;
+    const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int e = min((int)threadIdx.x + 256 * k, NX - 1);
+      const int r = e / XP, c = e - r * XP;
+      const int ci = min(max(ti0 - 2 + r, 0), H - 1), cj = min(max(tj0 - 2 + c, 0), W - 1);
+      pre[k] = x[(size_t)ci * W + cj];
+    }
+  };
+  const TileWalk tw(ntiles);
+  if (tw.first < tw.end) fetch(tw.first);
+  for (int t = tw.first; t < tw.end; t += tw.per) {
+    const int ty = t / ntx;
+    const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
+    __syncthreads();  // the previous tile's reads of sxf and sa are done
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int e = threadIdx.x + 256 * k;
+      if (e < NX) {
+        const int r = e / XP, c = e - r * XP;
+        const int gi = ti0 - 2 + r, gj = tj0 - 2 + c;
+        const bool in = gi >= 0 && gi < H && gj >= 0 && gj < W;
+        sxf[e] = in ? pre[k] * sc : 0.f;
+      }
+    }
+    __syncthreads();
+    fetch(t + tw.per < tw.end ? t + tw.per : t);
+    const bool edge = walls != 0 && (ti0 == 0 || ti0 + 16 == H || tj0 == 0 || tj0 + 16 == W);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int p = pl + 128 * k;
+      if (p < TP * TP) {
+        const int r = p / TP, c = p - r * TP;
+        const int gi = ti0 - 1 + r, gj = tj0 - 1 + c;
+        uint32_t o[4] = {0u, 0u, 0u, 0u};
+        if (gi >= 0 && gi < H && gj >= 0 && gj < W) {
+          // taps in order, one row of three per iteration (not unrolled,
+          // so that only that row's weights occupy scalar registers)
+          float s[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) s[u] = b0h[u];
+#pragma unroll 1
+          for (int dy = 0; dy < 3; ++dy) {
+            const float* wr = w0h + dy * 96;  // taps 3 dy .. 3 dy + 2
+            float xv[3], wv[3];
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+              xv[dx] = sxf[(r + dy) * XP + c + dx];
+              const int si = gi + dy - 1, sj = gj + dx - 1;
+              const bool in = si >= 0 && si < H && sj >= 0 && sj < W;
+              wv[dx] = edge && in ? wall_code(si, sj, H, W, walls) : 0.f;
+            }
+            if (edge) {
+#pragma unroll
+              for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                  s[u] = __builtin_fmaf(wr[dx * 32 + 2 * u], xv[dx], s[u]);
+                  s[u] = __builtin_fmaf(wr[dx * 32 + 2 * u + 1], wv[dx], s[u]);
+                }
+            } else {
+#pragma unroll
+              for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                  s[u] = __builtin_fmaf(wr[dx * 32 + 2 * u], xv[dx], s[u]);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            o[u] = (uint32_t)f2bf(s[2 * u] > 0.f ? s[2 * u] : 0.f) |
+                   ((uint32_t)f2bf(s[2 * u + 1] > 0.f ? s[2 * u + 1] : 0.f) << 16);
+        }
+        *reinterpret_cast<uint4*>(sa + p * 16 + 8 * h) = uint4{o[0], o[1], o[2], o[3]};
+      }
+    }
+    __syncthreads();
+    f32x4_t acc[4][1];
+    conv16_mma<1, true>(sa, wf, lane, wave, acc);
+    conv16_epilogue<1>(acc, ep, 1, ti0, tj0, W, skip, 1, pout, nullptr, lane, wave);
+  }
+}
+
+// Decoder: the 4 x 4 stride-4 up-convolution 32 -> 16 of the level-1 output
+// d1 (as k_up_fwd) on the tile's 20 x 20 window, beside the skip channels
+// (the concatenation [up | skip], 32 channels, in LDS), then the 32 -> 16
+// convolution (as k_conv3_tile, nine K = 32 MFMAs in tap order) on the
+// 18 x 18 halo into LDS, then the 16 -> 16 convolution with the output layer
+// in k_conv3_tile16p's epilogue into yout, scaled by *yscale.  Zero padding:
+// window pixels outside the grid are zero in both LDS tiles.
+//   up:  the window's 6 x 6 coarse pixels x 256 columns (ab, co), K = 32;
+//        wave w takes the columns of ab = 4w .. 4w + 3
+//   mid: 18 rows of 16 pixels (wave w: rows R0(w) .. R0(w) + 4 or + 3, so
+//        that a row's fragments serve the three rows beside it) and the
+//        two right halo columns as three 16-pixel groups
+__global__ __launch_bounds__(256, 2) void k_dec0_apply(View d1, View skip,
+                                                      const bf16* __restrict__ wu,
+                                                      const float* __restrict__ bu,
+                                                      const bf16* __restrict__ wm,
+                                                      const float* __restrict__ bm,
+                                                      const bf16* __restrict__ wl,
+                                                      const float* __restrict__ bl,
+                                                      const float* __restrict__ wout,
+                                                      const float* __restrict__ yscale,
+                                                      float* __restrict__ yout, int ntx,
+                                                      int ntiles) {
+  constexpr int XP = 20, TP = 18, NC = 36;   // window side, halo side, coarse pixels
+  constexpr int NS = XP * XP * 2;            // 16-byte skip chunks of the window
+  constexpr int NSP = (NS + 255) / 256;
+  __shared__ __attribute__((aligned(16))) bf16 sc[XP * XP * 32];
+  __shared__ __attribute__((aligned(16))) bf16 sd[TP * TP * 16];
+  __shared__ __attribute__((aligned(16))) bf16 su[NC * 32];
+  __shared__ __attribute__((aligned(16))) bf16 swu[256 * 32];  // up weights [(ab, co)][ci]
+  __shared__ __attribute__((aligned(16))) bf16 swl[9 * 16 * 16];  // last conv [tap][co][ci]
+  typedef Frag<32> F;
+  const int H = skip.H, W = skip.W, Hc = d1.H, Wc = d1.W;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int px = lane & 15, kq = 8 * (lane >> 4), cq = 4 * (lane >> 4);
+  // up-convolution weights (A: rows (ab, co) = n) in LDS, read once per tile
+  for (int e = threadIdx.x; e < 256 * 4; e += 256)
+    *reinterpret_cast<uint4*>(swu + e * 8) = *reinterpret_cast<const uint4*>(wu + e * 8);
+  // the last convolution's weights too, taken into registers for the last
+  // stage only (their registers serve the middle stage's fragments before)
+  for (int e = threadIdx.x; e < 9 * 16 * 2; e += 256)
+    *reinterpret_cast<uint4*>(swl + e * 8) = *reinterpret_cast<const uint4*>(wl + e * 8);
+  float bur[4], bmr[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    bur[u] = bu[cq + u];
+    bmr[u] = bm[cq + u];
+  }
+  // 32 -> 16 weights, one K = 32 fragment per tap
+  F::T wmf[9];
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) wmf[tap] = F::load(wm + (size_t)(tap * 16 + px) * 32 + kq);
+  Conv16Epi<1> ep;
+  ep.load(bl, wout, yscale, lane);
+  uint4 pre[NSP], prc;
+  auto fetch = [&](int t) {
+    const int ty = t / ntx;
+    const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) {
+      const int e = min((int)threadIdx.x + 256 * k, NS - 1);
+      const int pix = e >> 1, ch = e & 1;
+      const int r = pix / XP, c = pix - r * XP;
+      const int ci = min(max(ti0 - 2 + r, 0), H - 1), cj = min(max(tj0 - 2 + c, 0), W - 1);
+      pre[k] = *reinterpret_cast<const uint4*>(skip.p + ((size_t)ci * W + cj) * skip.cs +
+                                               skip.co + ch * 8);
+    }
+    const int e = min((int)threadIdx.x, NC * 4 - 1);
+    const int cp = e >> 2, ch = e & 3;
+    const int I = min(max(ti0 / 4 - 1 + cp / 6, 0), Hc - 1);
+    const int J = min(max(tj0 / 4 - 1 + cp % 6, 0), Wc - 1);
+    prc = *reinterpret_cast<const uint4*>(d1.p + ((size_t)I * Wc + J) * d1.cs + d1.co + ch * 8);
+  };
+  // mid-layer rows of this wave and its group of the right halo columns
+  const int R0 = wave < 2 ? 5 * wave : 10 + 4 * (wave - 2);
+  const int nr = wave < 2 ? 5 : 4;
+  const int sg = wave == 0 ? 2 : (wave == 1 ? -1 : wave - 2);
+  const TileWalk tw(ntiles);
+  if (tw.first < tw.end) fetch(tw.first);
+  for (int t = tw.first; t < tw.end; t += tw.per) {
+    const int ty = t / ntx;
+    const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
+    // A: stage the skip window and the coarse pixels.  No barrier is needed
+    // before it: the previous tile's last reads of sc and su precede its
+    // barrier before the final convolution.
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) {
+      const int e = threadIdx.x + 256 * k;
+      if (e < NS) {
+        const int pix = e >> 1, ch = e & 1;
+        const int r = pix / XP, c = pix - r * XP;
+        const int gi = ti0 - 2 + r, gj = tj0 - 2 + c;
+        const bool in = gi >= 0 && gi < H && gj >= 0 && gj < W;
+        *reinterpret_cast<uint4*>(sc + pix * 32 + 16 + ch * 8) = in ? pre[k] : uint4{0u, 0u, 0u, 0u};
+      }
+    }
+    if ((int)threadIdx.x < NC * 4) {
+      const int cp = threadIdx.x >> 2, ch = threadIdx.x & 3;
+      *reinterpret_cast<uint4*>(su + cp * 32 + ch * 8) = prc;
+    }
+    __syncthreads();
+    fetch(t + tw.per < tw.end ? t + tw.per : t);
+    // B: up-convolution into channels 0..15 of the window
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      const int cpl = g * 16 + px;  // this lane's coarse pixel (B column)
+      const F::T bfr = F::load(su + min(cpl, NC - 1) * 32 + kq);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const F::T wuf = F::load(swu + ((4 * wave + j) * 16 + px) * 32 + kq);
+        const f32x4_t a = F::mfma(wuf, bfr, f32x4_t{0.f, 0.f, 0.f, 0.f});
+        const int ab = 4 * wave + j, ai = ab >> 2, bj = ab & 3;
+        const int fr = 4 * (cpl / 6) + ai - 2, fc = 4 * (cpl % 6) + bj - 2;
+        const int I = ti0 / 4 - 1 + cpl / 6, J = tj0 / 4 - 1 + cpl % 6;
+        if (cpl < NC && fr >= 0 && fr < XP && fc >= 0 && fc < XP) {
+          uint2 v = {0u, 0u};
+          if (I >= 0 && I < Hc && J >= 0 && J < Wc)
+            v = uint2{(uint32_t)f2bf(a[0] + bur[0]) | ((uint32_t)f2bf(a[1] + bur[1]) << 16),
+                      (uint32_t)f2bf(a[2] + bur[2]) | ((uint32_t)f2bf(a[3] + bur[3]) << 16)};
+          *reinterpret_cast<uint2*>(sc + (fr * XP + fc) * 32 + cq) = v;
+        }
+      }
+    }
+    __syncthreads();
+    // C: the 32 -> 16 convolution on the 18 x 18 halo into sd
+    auto mid_store = [&](const f32x4_t& a, int R, int c) {
+      const int gi = ti0 - 1 + R, gj = tj0 - 1 + c;
+      uint2 v = {0u, 0u};
+      if (gi >= 0 && gi < H && gj >= 0 && gj < W) {
+        float q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float s = a[u] + bmr[u];
+          q[u] = s > 0.f ? s : 0.f;
+        }
+        v = uint2{(uint32_t)f2bf(q[0]) | ((uint32_t)f2bf(q[1]) << 16),
+                  (uint32_t)f2bf(q[2]) | ((uint32_t)f2bf(q[3]) << 16)};
+      }
+      *reinterpret_cast<uint2*>(sd + (R * TP + c) * 16 + cq) = v;
+    };
+    {
+      f32x4_t acc[5];
+#pragma unroll
+      for (int a = 0; a < 5; ++a) acc[a] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+        for (int a = 0; a < 5; ++a) {
+          const int R = min(R0 + a, TP - 1);
+          const F::T af = F::load(sc + ((R + 1 + dy) * XP + px + 1 + dx) * 32 + kq);
+          acc[a] = F::mfma(wmf[tap], af, acc[a]);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 5; ++a)
+        if (a < nr) mid_store(acc[a], R0 + a, px);
+    }
+    if (sg >= 0) {
+      const int idx = sg * 16 + px;
+      const int ii = min(idx, 2 * TP - 1);
+      const int R = ii >> 1, c = 16 + (ii & 1);
+      f32x4_t a = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        a = F::mfma(wmf[tap], F::load(sc + ((R + 1 + dy) * XP + c + 1 + dx) * 32 + kq), a);
+      }
+      if (idx < 2 * TP) mid_store(a, R, c);
+    }
+    __syncthreads();
+    // D: the 16 -> 16 convolution and the output layer
+    F::T wf[5][1];
+    conv16_weights<16, 1>(swl, lane, wf);
+    f32x4_t acc[4][1];
+    conv16_mma<1, true>(sd, wf, lane, wave, acc);
+    conv16_epilogue<1>(acc, ep, 1, ti0, tj0, W, View{}, 0, View{}, yout, lane, wave);
+  }
+}
+
 // first conv weight/bias gradient partials: per block kFirstPart values
 constexpr int kFirstPart = 9 * 16 * 2 + 16;
 __global__ __launch_bounds__(256) void k_first_wgrad(const float* __restrict__ x,
@@ -481,6 +791,7 @@ struct cpgpu_unet {
   cpgpu_ctx* ctx = nullptr;
   int conv_rows = 4;     // pixel rows per wave of the 16-channel tile convolutions (4 or 8)
   int conv_persist = 4;  // > 0: persistent 16-channel tiles, blocks per compute unit
+  int fuse_level0 = 1;   // applied networks: level 0 in two fused kernels (needs conv_persist > 0)
   int ncu = 256;         // compute units of the device
   int walls = 0;      // wall types of the current input (2 bits per segment x-, x+, y-, y+)
   size_t woff[kNL], boff[kNL], wbf_off[kNL];
@@ -660,11 +971,27 @@ int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
             const float* raw = nullptr) {
   hipStream_t st = U->ctx->stream;
   const int M = U->H * U->W;
-  hipLaunchKernelGGL(k_first_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
-                     raw ? raw : (const float*)U->x0.p, U->H, U->W, U->walls, U->v(U->a0),
-                     U->params.p + U->woff[0], U->params.p + U->boff[0],
-                     raw ? (const float*)U->scale.p : nullptr);
-  const bool pooled0 = conv_fwd<16, 16>(U, U->v(U->a0), U->v(U->C0, 16), 1, 1, U->v(U->p0)) & 1;
+  // an applied network (output and unscaled input given) runs level 0 in
+  // the two fused kernels; a0, d0a, d0 and the up-convolution's half of C0
+  // are then not formed.  Only where the layer-per-kernel path takes the
+  // tiled convolutions (M >= kSmallM), whose summation order they repeat.
+  const bool fused0 = final_out && raw && U->fuse_level0 && U->conv_persist > 0 && M >= kSmallM;
+  const int ntx = U->W / 16, ntiles = ntx * (U->H / 16);
+  bool pooled0 = false;
+  if (fused0) {
+    const EvPair ev = timer_events(U->ctx, 8);
+    CPGPU_LAUNCH(ev, k_enc0_apply, dim3(std::min(ntiles, U->ncu * U->conv_persist)), dim3(256), 0,
+                 st, raw, U->scale.p, U->H, U->W, U->walls, U->params.p + U->woff[0],
+                 U->params.p + U->boff[0], U->wbf.p + U->wbf_off[1], U->params.p + U->boff[1],
+                 U->v(U->C0, 16), U->v(U->p0), ntx, ntiles);
+    pooled0 = true;
+  } else {
+    hipLaunchKernelGGL(k_first_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
+                       raw ? raw : (const float*)U->x0.p, U->H, U->W, U->walls, U->v(U->a0),
+                       U->params.p + U->woff[0], U->params.p + U->boff[0],
+                       raw ? (const float*)U->scale.p : nullptr);
+    pooled0 = conv_fwd<16, 16>(U, U->v(U->a0), U->v(U->C0, 16), 1, 1, U->v(U->p0)) & 1;
+  }
   auto pool = [&](View in, View out, int C) {
     const int n = out.H * out.W * C;
     hipLaunchKernelGGL(k_pool_fwd, dim3((n + 255) / 256), dim3(256), 0, st, in,
@@ -685,6 +1012,18 @@ int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
   up_fwd<64, 32>(U, U->v(U->d2), U->v(U->C1, 0), 11);
   conv_fwd<64, 32>(U, U->v(U->C1), U->v(U->d1a), 12, 1);
   conv_fwd<32, 32>(U, U->v(U->d1a), U->v(U->d1), 13, 1);
+  if (fused0) {
+    const EvPair ev = timer_events(U->ctx, 9);
+    CPGPU_LAUNCH(ev, k_dec0_apply, dim3(std::min(ntiles, U->ncu * 2)), dim3(256), 0, st,
+                 U->v(U->d1), U->v(U->C0, 16), U->wbf.p + U->wbf_off[14],
+                 U->params.p + U->boff[14], U->wbf.p + U->wbf_off[15],
+                 U->params.p + U->boff[15], U->wbf.p + U->wbf_off[16],
+                 U->params.p + U->boff[16], U->params.p + U->woff[17], U->scale.p, final_out,
+                 ntx, ntiles);
+    *out_done = true;
+    CPGPU_HIP(U->ctx, hipGetLastError());
+    return CPGPU_OK;
+  }
   up_fwd<32, 16>(U, U->v(U->d1), U->v(U->C0, 0), 14);
   conv_fwd<32, 16>(U, U->v(U->C0), U->v(U->d0a), 15, 1);
   // applied only (final_out): the output layer and the output scaling run in
@@ -1064,6 +1403,12 @@ int cpgpu_unet_set_conv_shape(cpgpu_unet* U, int rows, int persist) {
   if (!U || (rows != 4 && rows != 8) || persist < 0 || persist > 16) return CPGPU_EINVAL;
   U->conv_rows = rows;
   U->conv_persist = persist;
+  return CPGPU_OK;
+}
+
+int cpgpu_unet_set_fusion(cpgpu_unet* U, int level0) {
+  if (!U || level0 < 0 || level0 > 1) return CPGPU_EINVAL;
+  U->fuse_level0 = level0;
   return CPGPU_OK;
 }
 

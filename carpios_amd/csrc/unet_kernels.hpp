@@ -36,18 +36,10 @@ __device__ __host__ __forceinline__ float bf2f(bf16 b) {
   x.u = (uint32_t)b << 16;
   return x.f;
 }
-// round to nearest even; NaN stays NaN
-__device__ __host__ __forceinline__ bf16 f2bf(float f) {
-  union {
-    uint32_t u;
-    float f;
-  } x;
-  x.f = f;
-  if ((x.u & 0x7f800000u) == 0x7f800000u && (x.u & 0x7fffffu))
-    return (bf16)((x.u >> 16) | 0x40);
-  const uint32_t r = x.u + 0x7fffu + ((x.u >> 16) & 1u);
-  return (bf16)(r >> 16);
-}
+// round to nearest even (the hardware conversion, v_cvt_pk_bf16_f32; two
+// conversions side by side become one instruction): branch-free, where the
+// bit-level formula compiled to a branch per value for its NaN case
+__device__ __forceinline__ bf16 f2bf(float f) { return __builtin_bit_cast(bf16, (__bf16)f); }
 
 // A tensor view: element (m, c) at p[m*cs + co + c]
 struct View {
@@ -503,16 +495,178 @@ __global__ __launch_bounds__(256) void k_conv3_tile16(View x, View yb, GView yf,
   }
 }
 
+// Weight fragments of a 3x3 convolution with 16 input channels for the
+// paired-tap MFMAs of k_conv3_tile16: pair pr holds tap 2pr in lanes 0-31
+// and tap 2pr + 1 in lanes 32-63 (the fifth pair's second half is zero).
+template <int NO, int NB>
+__device__ __forceinline__ void conv16_weights(const bf16* __restrict__ w, int lane,
+                                               typename Frag<32>::T (&wf)[5][NB]) {
+  typedef Frag<32> F;
+  const int px = lane & 15, half = lane >> 5, kc = 8 * ((lane >> 4) & 1);
+#pragma unroll
+  for (int pr = 0; pr < 5; ++pr) {
+    const int tap = 2 * pr + half;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      wf[pr][b] = tap < 9 ? F::load(w + ((size_t)(tap * NO + b * 16 + px) * 16 + kc))
+                          : F::zero();
+  }
+}
+
+// The convolution sums of tile rows 4 wave .. 4 wave + 3 (16 pixels each)
+// from an LDS tile sx of 18 x 18 pixels x 16 channels (halo included):
+// acc[a][b] = D[channels 16b..16b+15][pixels of row 4 wave + a], taps in
+// pair order.
+template <int NB, bool FWD>
+__device__ __forceinline__ void conv16_mma(const bf16* sx,
+                                           const typename Frag<32>::T (&wf)[5][NB],
+                                           int lane, int wave, f32x4_t (&acc)[4][NB]) {
+  typedef Frag<32> F;
+  constexpr int TP = 18, KCH = 16;
+  const int px = lane & 15, half = lane >> 5, kc = 8 * ((lane >> 4) & 1);
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int pr = 0; pr < 5; ++pr) {
+    const int tap = 2 * pr + half;
+    const int tp = tap < 9 ? tap : 4;
+    const int dy = FWD ? tp / 3 - 1 : 1 - tp / 3;
+    const int dx = FWD ? tp % 3 - 1 : 1 - tp % 3;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int r = wave * 4 + a;
+      typename F::T xf = F::load(sx + ((r + 1 + dy) * TP + px + 1 + dx) * KCH + kc);
+      if (tap >= 9) xf = F::zero();
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(wf[pr][b], xf, acc[a][b]);
+    }
+  }
+}
+
+// Constants of the forward epilogue: the lane's four biases per column block
+// and, for the fused output layer, its four weights, the output bias and
+// the output scale (zero when wout is null).
+template <int NB>
+struct Conv16Epi {
+  float bias[NB][4];
+  float wo[4], wo16, ysc;
+  __device__ __forceinline__ void load(const float* __restrict__ b, const float* __restrict__ wout,
+                                       const float* __restrict__ yscale, int lane) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) bias[k][u] = b[k * 16 + 4 * (lane >> 4) + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) wo[u] = 0.f;
+    wo16 = 0.f;
+    ysc = 0.f;
+    if (wout) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) wo[u] = wout[4 * (lane >> 4) + u];
+      wo16 = wout[16];
+      ysc = *yscale;
+    }
+  }
+};
+
+// Forward epilogue of a 16 x 16 tile at (ti0, tj0) of a W-wide level:
+// relu(acc + bias) rounded to BF16 and stored into yb (store_act); with
+// pout.p the 4 x 4 average pooling into pout (rows summed in the lane, the
+// four columns of a window by lane shuffles, then / 16 and rounded: the sum
+// of 16 non-negative BF16 values is exact in FP32, as in k_pool_fwd); with
+// yout (NB = 1) the 1 x 1 output layer y = (wo16 + sum_c wo[c] out_c) * ysc,
+// the four channel groups of a pixel summed by lane shuffles.
+template <int NB>
+__device__ __forceinline__ void conv16_epilogue(const f32x4_t (&acc)[4][NB],
+                                                const Conv16Epi<NB>& ep, int relu, int ti0,
+                                                int tj0, int W, View yb, int store_act,
+                                                View pout, float* __restrict__ yout,
+                                                int lane, int wave) {
+  const int px = lane & 15, cq = 4 * (lane >> 4);
+  float ps[NB][4];
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ps[b][q] = 0.f;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const size_t m = (size_t)(ti0 + wave * 4 + a) * W + tj0 + px;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int n = b * 16 + cq;
+      bf16 q[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        float v = acc[a][b][u] + ep.bias[b][u];
+        if (relu) v = v > 0.f ? v : 0.f;
+        q[u] = f2bf(v);
+        ps[b][u] += bf2f(q[u]);
+      }
+      if (store_act)
+        *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) =
+            uint2{(uint32_t)q[0] | ((uint32_t)q[1] << 16), (uint32_t)q[2] | ((uint32_t)q[3] << 16)};
+      if (NB == 1 && yout) {
+        float part = 0.f;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) part += ep.wo[u] * bf2f(q[u]);
+        part += __shfl_xor(part, 16, 64);
+        part += __shfl_xor(part, 32, 64);
+        if (lane < 16) yout[m] = (ep.wo16 + part) * ep.ysc;
+      }
+    }
+  }
+  if (pout.p) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        float v = ps[b][u];
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        ps[b][u] = v;
+      }
+    if ((px & 3) == 0) {
+      const size_t mp = (size_t)(ti0 / 4 + wave) * pout.W + tj0 / 4 + px / 4;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const bf16 q0 = f2bf(ps[b][0] * (1.f / 16.f)), q1 = f2bf(ps[b][1] * (1.f / 16.f));
+        const bf16 q2 = f2bf(ps[b][2] * (1.f / 16.f)), q3 = f2bf(ps[b][3] * (1.f / 16.f));
+        *reinterpret_cast<uint2*>(pout.p + mp * pout.cs + pout.co + b * 16 + cq) =
+            uint2{(uint32_t)q0 | ((uint32_t)q1 << 16), (uint32_t)q2 | ((uint32_t)q3 << 16)};
+      }
+    }
+  }
+}
+
+// The tiles of a persistent launch: XCD-aware order.  A launch hands blocks
+// to the 8 XCDs round robin, so with the grid a multiple of 8 the blocks of
+// XCD x walk the x-th contiguous eighth of the tiles together (neighbouring
+// tiles, whose halo lines they share, then sit in one L2); otherwise the
+// plain order.  Block b takes first, first + per, ... below end.
+struct TileWalk {
+  int first, per, end;
+  __device__ __forceinline__ TileWalk(int ntiles) {
+    const int G = gridDim.x;
+    const bool xo = (G & 7) == 0;
+    per = xo ? G >> 3 : G;
+    const int span = xo ? (ntiles + 7) >> 3 : ntiles;
+    const int base = xo ? (int)(blockIdx.x & 7) * span : 0;
+    end = min(base + span, ntiles);
+    first = base + (xo ? (int)(blockIdx.x >> 3) : (int)blockIdx.x);
+  }
+};
+
 // Persistent form of k_conv3_tile16 (ROWS = 4): a block walks 16 x 16 tiles
-// t = blockIdx.x + k gridDim.x (tiles numbered along rows) and loads tile
-// t + gridDim.x into registers while it computes tile t from LDS, so the
-// staging latency overlaps the MFMAs; the weights are loaded once per block.
+// (numbered along rows, in TileWalk order) and loads its next tile into
+// registers while it computes the current one from LDS, so the staging
+// latency overlaps the MFMAs; the weights are loaded once per block.
 //
 // With wout set (forward, NO = 16) the epilogue also applies the network's
-// 1 x 1 output layer to the BF16 output: y = (wout[16] + sum_c wout[c] out_c)
-// * (*yscale), the four channel groups of a pixel summed by lane shuffles,
-// into yout; store_act = 0 then skips storing the layer's own output (no
-// later kernel reads it when the network is only applied).
+// 1 x 1 output layer to the BF16 output into yout; store_act = 0 then skips
+// storing the layer's own output (no later kernel reads it when the network
+// is only applied).
 template <int NO, bool FWD>
 __global__ __launch_bounds__(256, NO == 16 ? 4 : 1) void k_conv3_tile16p(View x, View yb, GView yf,
                                                        const bf16* __restrict__ w,
@@ -528,34 +682,16 @@ __global__ __launch_bounds__(256, NO == 16 ? 4 : 1) void k_conv3_tile16p(View x,
   __shared__ __attribute__((aligned(16))) bf16 sx[TP * TP * KCH];
   const int H = x.H, W = x.W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int px = lane & 15, half = lane >> 5, kc = 8 * ((lane >> 4) & 1);
   typename F::T wf[5][NB];
-#pragma unroll
-  for (int pr = 0; pr < 5; ++pr) {
-    const int tap = 2 * pr + half;
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-      wf[pr][b] = tap < 9 ? F::load(w + ((size_t)(tap * NO + b * 16 + px) * KCH + kc))
-                          : F::zero();
-  }
+  conv16_weights<NO, NB>(w, lane, wf);
   // Every global load inside the tile loop is the next tile's prefetch,
   // issued unconditionally from a clamped, valid address (masked when
   // staged): the epilogue's constants are loaded before the loop, so the
   // prefetch stays in flight through the MFMAs and the epilogue (a load
   // under a branch, or a later load that must be waited for, makes the
   // compiler drain it early).
-  float bias_r[NB][4];  // forward only (the input gradient has no bias)
-#pragma unroll
-  for (int b = 0; b < NB; ++b)
-#pragma unroll
-    for (int u = 0; u < 4; ++u) bias_r[b][u] = FWD ? bias[b * 16 + 4 * (lane >> 4) + u] : 0.f;
-  float wo[4] = {0.f, 0.f, 0.f, 0.f}, wo16 = 0.f, ysc = 0.f;
-  if (NO == 16 && FWD && wout) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) wo[u] = wout[4 * (lane >> 4) + u];
-    wo16 = wout[16];
-    ysc = *yscale;
-  }
+  Conv16Epi<NB> ep;  // forward only (the input gradient has no bias)
+  if (FWD) ep.load(bias, NO == 16 ? wout : nullptr, yscale, lane);
   uint4 pre[NPT];
   bool pok[NPT];
   auto fetch = [&](int t) {
@@ -572,20 +708,9 @@ __global__ __launch_bounds__(256, NO == 16 ? 4 : 1) void k_conv3_tile16p(View x,
       pre[k] = *reinterpret_cast<const uint4*>(x.p + ((size_t)ci * W + cj) * x.cs + x.co + ch * 8);
     }
   };
-  // XCD-aware tile order: a launch hands blocks to the 8 XCDs round robin,
-  // so with the grid a multiple of 8 the blocks of XCD x walk the x-th
-  // contiguous eighth of the tiles together (neighbouring tiles, whose halo
-  // lines they share, then sit in one L2); otherwise the plain order
-  const int G = gridDim.x;
-  const bool xo = (G & 7) == 0;
-  const int per = xo ? G >> 3 : G;                        // blocks walking one range
-  const int span = xo ? (ntiles + 7) >> 3 : ntiles;       // tiles of one range
-  const int base = xo ? (int)(blockIdx.x & 7) * span : 0;  // first tile of the range
-  const int end = min(base + span, ntiles);
-  const int first = base + (xo ? (int)(blockIdx.x >> 3) : (int)blockIdx.x);
-  if (first < end) fetch(first);
-  const int cq = 4 * (lane >> 4);
-  for (int t = first; t < end; t += per) {
+  const TileWalk tw(ntiles);
+  if (tw.first < tw.end) fetch(tw.first);
+  for (int t = tw.first; t < tw.end; t += tw.per) {
     const int ty = t / ntx;
     const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
     __syncthreads();  // the previous tile's LDS reads are done
@@ -597,85 +722,21 @@ __global__ __launch_bounds__(256, NO == 16 ? 4 : 1) void k_conv3_tile16p(View x,
     }
     __syncthreads();
     // the last tile of the block re-reads its own tile (cache hits)
-    fetch(t + per < end ? t + per : t);
+    fetch(t + tw.per < tw.end ? t + tw.per : t);
     f32x4_t acc[4][NB];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int pr = 0; pr < 5; ++pr) {
-      const int tap = 2 * pr + half;
-      const int tp = tap < 9 ? tap : 4;
-      const int dy = FWD ? tp / 3 - 1 : 1 - tp / 3;
-      const int dx = FWD ? tp % 3 - 1 : 1 - tp % 3;
+    conv16_mma<NB, FWD>(sx, wf, lane, wave, acc);
+    if constexpr (FWD) {
+      conv16_epilogue<NB>(acc, ep, relu, ti0, tj0, W, yb, store_act, pout,
+                          NO == 16 && wout ? yout : nullptr, lane, wave);
+    } else {
+      const int px = lane & 15, cq = 4 * (lane >> 4);
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
-        const int r = wave * 4 + a;
-        typename F::T xf = F::load(sx + ((r + 1 + dy) * TP + px + 1 + dx) * KCH + kc);
-        if (tap >= 9) xf = F::zero();
-#pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(wf[pr][b], xf, acc[a][b]);
-      }
-    }
-    float ps[NB][4];
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) ps[b][q] = 0.f;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const size_t m = (size_t)(ti0 + wave * 4 + a) * W + tj0 + px;
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const int n = b * 16 + cq;
-        if (FWD) {
-          bf16 q[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            float v = acc[a][b][u] + bias_r[b][u];
-            if (relu) v = v > 0.f ? v : 0.f;
-            q[u] = f2bf(v);
-            ps[b][u] += bf2f(q[u]);
-          }
-          if (store_act)
-            *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) =
-                uint2{(uint32_t)q[0] | ((uint32_t)q[1] << 16), (uint32_t)q[2] | ((uint32_t)q[3] << 16)};
-          if (NO == 16 && wout) {
-            float part = 0.f;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) part += wo[u] * bf2f(q[u]);
-            part += __shfl_xor(part, 16, 64);
-            part += __shfl_xor(part, 32, 64);
-            if (lane < 16) yout[m] = (wo16 + part) * ysc;
-          }
-        } else {
-          *reinterpret_cast<float4*>(yf.p + m * yf.cs + yf.co + n) =
-              float4{acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
-        }
-      }
-    }
-    if constexpr (FWD) {
-      if (pout.p) {
+        const size_t m = (size_t)(ti0 + wave * 4 + a) * W + tj0 + px;
 #pragma unroll
         for (int b = 0; b < NB; ++b)
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            float v = ps[b][u];
-            v += __shfl_xor(v, 1, 64);
-            v += __shfl_xor(v, 2, 64);
-            ps[b][u] = v;
-          }
-        if ((px & 3) == 0) {
-          const size_t mp = (size_t)(ti0 / 4 + wave) * pout.W + tj0 / 4 + px / 4;
-#pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            const bf16 q0 = f2bf(ps[b][0] * (1.f / 16.f)), q1 = f2bf(ps[b][1] * (1.f / 16.f));
-            const bf16 q2 = f2bf(ps[b][2] * (1.f / 16.f)), q3 = f2bf(ps[b][3] * (1.f / 16.f));
-            *reinterpret_cast<uint2*>(pout.p + mp * pout.cs + pout.co + b * 16 + cq) =
-                uint2{(uint32_t)q0 | ((uint32_t)q1 << 16), (uint32_t)q2 | ((uint32_t)q3 << 16)};
-          }
-        }
+          *reinterpret_cast<float4*>(yf.p + m * yf.cs + yf.co + b * 16 + cq) =
+              float4{acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
       }
     }
   }

@@ -396,6 +396,13 @@ int cpgpu_unet_destroy(cpgpu_unet *U);
  * tiles, staging the next tile while it computes one (rows 4).  Default
  * rows 4, persist 4. */
 int cpgpu_unet_set_conv_shape(cpgpu_unet *U, int rows, int persist);
+/* Level-0 fusion of an applied network (a preconditioner or warm start, no
+ * backward pass): level0 = 1 (default) runs the finest level in two kernels
+ * (first layer + convolution + pooling; up-convolution + both decoder
+ * convolutions + output layer) whose inner activations stay in LDS, with
+ * the same per-value arithmetic as the layer-per-kernel path (level0 = 0).
+ * Needs the persistent shape (persist > 0).  No reference counterpart. */
+int cpgpu_unet_set_fusion(cpgpu_unet *U, int level0);
 long long cpgpu_unet_num_params(const cpgpu_unet *U);
 /* Flat FP32 parameter vector (grads != 0: the last computed gradients);
  * per layer: weights [tap][cout][cin], then biases [cout]. */
@@ -488,7 +495,8 @@ int cpgpu_csr_solve(cpgpu_csr *A, int method, double *x, const double *b, int *m
  * epilogue applies the 1 x 1 output layer instead (a network only applied),
  * 4 = the NS_explicit_ u* kernel (TVD advection + diffusion of both
  * components), 5 = C2F + Div of u*, 6 = grad phi + corrections + p,
- * 7 = the fused u* + C2F + Div kernel.
+ * 7 = the fused u* + C2F + Div kernel, 8 = the fused level-0 encoder of an
+ * applied U-Net, 9 = its fused level-0 decoder.
  * Timed launches carry their own start/stop timestamps (no extra packets
  * between kernels). */
 int cpgpu_timer_enable(cpgpu_ctx *ctx, int family, int on);
