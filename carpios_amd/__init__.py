@@ -540,8 +540,10 @@ def _kernel_times(self, precision=32, reps=50):
 
 
 def _set_batch(self, iters=16, use_graphs=True):
-    self.ctx.check(lib().cpgpu_poisson_set_batch(self.h, int(iters),
-                                                 int(bool(use_graphs))))
+    """use_graphs: False, True (replayed Krylov batches) or 2 (each
+    flexible-CG iteration replayed as one graph as well)."""
+    g = 2 if (not isinstance(use_graphs, bool) and use_graphs == 2) else int(bool(use_graphs))
+    self.ctx.check(lib().cpgpu_poisson_set_batch(self.h, int(iters), g))
 
 
 Poisson.kernel_times = _kernel_times

@@ -10,6 +10,8 @@
 // Jacobi-PCG.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "poisson.hpp"
@@ -439,6 +441,60 @@ int apply_precond(cpgpu_poisson* P, bool use_net) {
   return CPGPU_OK;
 }
 
+// One flexible-CG iteration: the preconditioner cycle, then the Krylov
+// kernels.  first: the first iteration (no previous direction); the
+// direction buffers alternate with the iteration's parity.
+int launch_fcg_iter(cpgpu_poisson* P, bool first, int parity, bool singular,
+                    int norm_ref) {
+  hipStream_t st = P->ctx->stream;
+  const Tiling tl = P->tl;
+  const size_t N = P->N;
+  const int G1 = 1024;  // blocks of the 1-D kernels
+  int rc = apply_precond(P, true);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fcg_zq, dim3(G1), dim3(256), 0, st, P->pa32.p,
+                     first ? nullptr : (const double*)P->q.p, P->part64.p, N);  // 2 x G1 partials
+  const double* pold = P->pf[parity].p;
+  double* pnew = P->pf[parity ^ 1].p;
+  hipLaunchKernelGGL(k_fcg_dir, dim3(tl.G), dim3(TJ, BY), 0, st, P->sten64(), P->pa32.p,
+                     pold, pnew, P->q.p, P->r.p, P->part64.p, G1, P->fst.p, first ? 1 : 0,
+                     singular ? 1 : 0, P->partB64.p, tl);
+  hipLaunchKernelGGL(k_fcg_upd, dim3(G1), dim3(256), 0, st, pnew, P->q.p, P->x.p, P->r.p,
+                     P->r32.p, P->partB64.p, (int)tl.G, P->fst.p, P->partR64.p, N, norm_ref);
+  hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(256), 0, st, P->partR64.p, G1, P->fst.p);
+  CPGPU_HIP(P->ctx, hipGetLastError());
+  return CPGPU_OK;
+}
+
+// The iteration (not the first) captured once per configuration and
+// replayed: one graph launch in place of the cycle's ~30 kernel launches.
+// Captured on the context stream itself, which
+// apply_precond and the network launch on; the network's activations are
+// already allocated (the first iteration ran directly).  Null: launch
+// directly.
+hipGraphExec_t fcg_graph(cpgpu_poisson* P, int parity, bool singular, int norm_ref) {
+  long long om;
+  std::memcpy(&om, &P->pc_omega, sizeof(om));
+  const std::vector<long long> key{parity, singular ? 1 : 0, norm_ref, (long long)(uintptr_t)P->unet,
+                                   unet_generation(P->unet), P->pc_pre, P->pc_post, om,
+                                   pack_walls(P), (long long)(uintptr_t)P->pf[0].p};
+  for (auto& g : P->fcg_graphs)
+    if (g.first == key) return g.second;
+  hipStream_t st = P->ctx->stream;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  const int rc = launch_fcg_iter(P, false, parity, singular, norm_ref);
+  const hipError_t ec = hipStreamEndCapture(st, &graph);
+  if (rc == CPGPU_OK && ec == hipSuccess && graph &&
+      hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+    exec = nullptr;
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipGetLastError();
+  if (exec) P->fcg_graphs.push_back({key, exec});
+  return exec;
+}
+
 int read_state(cpgpu_poisson* P, FState* out) {
   hipStream_t st = P->ctx->stream;
   CPGPU_HIP(P->ctx, hipMemcpyAsync(P->h_fst, P->fst.p, sizeof(FState),
@@ -503,24 +559,19 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
   std::vector<double> best_hist(1, res);
   int k = 0;
   bool fallback = false;
+  // replayed iteration graphs when asked for, unless kernel timers are on
+  // (their events bracket direct launches)
+  bool timed = false;
+  for (int f = 0; f < cpgpu_ctx::kFamilies; ++f) timed = timed || ctx->timer_on[f];
+  const bool use_graphs = P->fcg_graphs_on && !timed;
   while (res > tol) {
     if (k >= max_iter) break;
-    // z = preconditioned residual
-    if ((rc = apply_precond(P, true))) return rc;
-    hipLaunchKernelGGL(k_fcg_zq, dim3(G1), dim3(256), 0, st, P->pa32.p,
-                       k > 0 ? (const double*)P->q.p : nullptr, P->part64.p, N);  // 2 x G1 partials
-    const double* pold = P->pf[k & 1].p;
-    double* pnew = P->pf[(k + 1) & 1].p;
-    hipLaunchKernelGGL(k_fcg_dir, dim3(tl.G), dim3(TJ, BY), 0, st, P->sten64(),
-                       P->pa32.p, pold, pnew, P->q.p, P->r.p, P->part64.p, G1,
-                       P->fst.p, k == 0 ? 1 : 0, singular ? 1 : 0, P->partB64.p,
-                       tl);
-    hipLaunchKernelGGL(k_fcg_upd, dim3(G1), dim3(256), 0, st, pnew, P->q.p,
-                       P->x.p, P->r.p, P->r32.p, P->partB64.p, (int)tl.G,
-                       P->fst.p, P->partR64.p, N, norm_ref);
-    hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(256), 0, st, P->partR64.p, G1,
-                       P->fst.p);
-    CPGPU_HIP(ctx, hipGetLastError());
+    hipGraphExec_t ge = (use_graphs && k > 0) ? fcg_graph(P, k & 1, singular, norm_ref) : nullptr;
+    if (ge) {
+      CPGPU_HIP(ctx, hipGraphLaunch(ge, st));
+    } else if ((rc = launch_fcg_iter(P, k == 0, k & 1, singular, norm_ref))) {
+      return rc;
+    }
     ++k;
     if ((rc = read_state(P, &fs))) return rc;
     res = fs.resid;

@@ -828,6 +828,13 @@ std::mutex g_stage_mu;
 
 }  // namespace
 
+void drop_graphs(cpgpu_poisson* P) {
+  for (auto& g : P->graphs) (void)hipGraphExecDestroy(g.second);
+  P->graphs.clear();
+  for (auto& g : P->fcg_graphs) (void)hipGraphExecDestroy(g.second);
+  P->fcg_graphs.clear();
+}
+
 // The file-scope device array (one copy per GPU, held until the process
 // ends).  In host code its name is a host-side stand-in: every use goes
 // through the per-device address from hipGetSymbolAddress.
@@ -1266,8 +1273,7 @@ int cpgpu_poisson_set_alpha(cpgpu_poisson* P, double alpha) {
   CPGPU_HIP(P->ctx, hipStreamSynchronize(P->ctx->stream));
   // the captured iteration graphs hold the old diagonal in their kernel
   // arguments: drop them (the next solve captures new ones)
-  for (auto& g : P->graphs) (void)hipGraphExecDestroy(g.second);
-  P->graphs.clear();
+  drop_graphs(P);
   int rc = upload(P->ctx, P->dvol.p, P->vol.data(), sizeof(double) * P->vol.size());
   if (rc) return rc;
   // b carries the source weight v: rebuild it from the kept source
@@ -1413,7 +1419,9 @@ int cpgpu_poisson_kernel_times(cpgpu_poisson* P, int precision, int reps,
 int cpgpu_poisson_set_batch(cpgpu_poisson* P, int iters, int use_graphs) {
   if (!P || iters < 2 || (iters & 1)) return CPGPU_EINVAL;
   P->batch = iters;
+  if (use_graphs < 0 || use_graphs > 2) return CPGPU_EINVAL;
   P->use_graphs = use_graphs != 0;
+  P->fcg_graphs_on = use_graphs == 2;
   return CPGPU_OK;
 }
 

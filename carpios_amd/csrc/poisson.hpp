@@ -171,8 +171,12 @@ struct cpgpu_poisson {
   cpgpu::RState* h_rst = nullptr;
   // batched Krylov iterations: replayed graphs and the polling events
   bool use_graphs = true;
+  bool fcg_graphs_on = false;  // flexible-CG iterations as replayed graphs too
   int batch = 16;  // iterations per batch (even)
   std::vector<std::pair<cpgpu::GraphKey, hipGraphExec_t>> graphs;
+  // flexible-CG iterations (preconditioner cycle + Krylov kernels):
+  // configuration key -> replayed graph
+  std::vector<std::pair<std::vector<long long>, hipGraphExec_t>> fcg_graphs;
   hipEvent_t ev_batch[2] = {nullptr, nullptr};
   // direction-kernel variant per precision (0 LDS tile, 1..4 column march
   // with H = 4, 8, 16, 32); set by heuristic, cpgpu_poisson_set_variant or
@@ -223,6 +227,7 @@ struct cpgpu_poisson {
   }
   ~cpgpu_poisson() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : fcg_graphs) (void)hipGraphExecDestroy(g.second);
     for (auto e : ev_batch)
       if (e) (void)hipEventDestroy(e);
     if (h_kst) (void)hipHostFree(h_kst);
@@ -279,4 +284,9 @@ int error_norms_device(cpgpu_ctx* ctx, int nx, int ny, const double* exa, int ld
                        double* part, size_t part_cap, double* out);
 // The context a U-Net was created on (unet.hip).
 cpgpu_ctx* unet_context(const cpgpu_unet* U);
+// changes whenever an application captured in a graph would be stale
+long long unet_generation(const cpgpu_unet* U);
+// drop every captured iteration graph of P (their kernel arguments hold
+// the operator's coefficients)
+void drop_graphs(cpgpu_poisson* P);
 }  // namespace cpgpu

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256, 4) void k_enc0_apply(const float* __restrict__
     __syncthreads();
     f32x4_t acc[4][1];
     conv16_mma<1, true>(sa, wf, lane, wave, acc);
-    conv16_epilogue<1>(acc, ep, 1, ti0, tj0, W, skip, 1, pout, nullptr, lane, wave);
+    conv16_epilogue<1, EPI_POOL>(acc, ep, 1, ti0, tj0, W, skip, pout, nullptr, lane, wave);
   }
 }
 
@@ -284,29 +284,25 @@ __global__ __launch_bounds__(256, 2) void k_dec0_apply(View d1, View skip,
   __shared__ __attribute__((aligned(16))) bf16 sc[XP * XP * 32];
   __shared__ __attribute__((aligned(16))) bf16 sd[TP * TP * 16];
   __shared__ __attribute__((aligned(16))) bf16 su[NC * 32];
-  __shared__ __attribute__((aligned(16))) bf16 swu[256 * 32];  // up weights [(ab, co)][ci]
   __shared__ __attribute__((aligned(16))) bf16 swl[9 * 16 * 16];  // last conv [tap][co][ci]
+  __shared__ __attribute__((aligned(16))) bf16 swm[9 * 16 * 32];  // middle conv [tap][co][ci]
+  __shared__ __attribute__((aligned(16))) bf16 junk[4 * 64];      // discarded up-conv stores
   typedef Frag<32> F;
   const int H = skip.H, W = skip.W, Hc = d1.H, Wc = d1.W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int px = lane & 15, kq = 8 * (lane >> 4), cq = 4 * (lane >> 4);
-  // up-convolution weights (A: rows (ab, co) = n) in LDS, read once per tile
-  for (int e = threadIdx.x; e < 256 * 4; e += 256)
-    *reinterpret_cast<uint4*>(swu + e * 8) = *reinterpret_cast<const uint4*>(wu + e * 8);
-  // the last convolution's weights too, taken into registers for the last
-  // stage only (their registers serve the middle stage's fragments before)
+  // the two convolutions' weights in LDS, each taken into registers in its
+  // own stage
   for (int e = threadIdx.x; e < 9 * 16 * 2; e += 256)
     *reinterpret_cast<uint4*>(swl + e * 8) = *reinterpret_cast<const uint4*>(wl + e * 8);
+  for (int e = threadIdx.x; e < 9 * 16 * 4; e += 256)
+    *reinterpret_cast<uint4*>(swm + e * 8) = *reinterpret_cast<const uint4*>(wm + e * 8);
   float bur[4], bmr[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     bur[u] = bu[cq + u];
     bmr[u] = bm[cq + u];
   }
-  // 32 -> 16 weights, one K = 32 fragment per tap
-  F::T wmf[9];
-#pragma unroll
-  for (int tap = 0; tap < 9; ++tap) wmf[tap] = F::load(wm + (size_t)(tap * 16 + px) * 32 + kq);
   Conv16Epi<1> ep;
   ep.load(bl, wout, yscale, lane);
   uint4 pre[NSP], prc;
@@ -328,9 +324,10 @@ __global__ __launch_bounds__(256, 2) void k_dec0_apply(View d1, View skip,
     const int J = min(max(tj0 / 4 - 1 + cp % 6, 0), Wc - 1);
     prc = *reinterpret_cast<const uint4*>(d1.p + ((size_t)I * Wc + J) * d1.cs + d1.co + ch * 8);
   };
-  // mid-layer rows of this wave and its group of the right halo columns
+  // mid-layer rows R0 .. R0 + 4 of this wave (rows past the halo clamp to
+  // its last row: a repeated row stores the same values again) and its
+  // 16-pixel group of the two right halo columns (none for wave 1)
   const int R0 = wave < 2 ? 5 * wave : 10 + 4 * (wave - 2);
-  const int nr = wave < 2 ? 5 : 4;
   const int sg = wave == 0 ? 2 : (wave == 1 ? -1 : wave - 2);
   const TileWalk tw(ntiles);
   if (tw.first < tw.end) fetch(tw.first);
@@ -356,74 +353,75 @@ __global__ __launch_bounds__(256, 2) void k_dec0_apply(View d1, View skip,
       *reinterpret_cast<uint4*>(su + cp * 32 + ch * 8) = prc;
     }
     __syncthreads();
+    // up-convolution weights of this wave's columns (ab = 4 wave + j: rows
+    // (ab, co)), loaded before the prefetch so that waiting for them does
+    // not wait for it
+    F::T wuf[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) wuf[j] = F::load(wu + (size_t)((4 * wave + j) * 16 + px) * 32 + kq);
     fetch(t + tw.per < tw.end ? t + tw.per : t);
-    // B: up-convolution into channels 0..15 of the window
+    // B: up-convolution into channels 0..15 of the window.  Fine pixel
+    // (4 I + wave, 4 J + j) of coarse pixel (I, J) sits at window row
+    // 4 pi + wave - 2, column 4 pj + j - 2; stores outside the window go to
+    // this lane's junk slot.  Lanes past the 36 coarse pixels repeat the last.
 #pragma unroll
     for (int g = 0; g < 3; ++g) {
-      const int cpl = g * 16 + px;  // this lane's coarse pixel (B column)
-      const F::T bfr = F::load(su + min(cpl, NC - 1) * 32 + kq);
+      const int cpl = min(g * 16 + px, NC - 1);
+      const int pi = cpl / 6, pj = cpl - 6 * pi;
+      const F::T bfr = F::load(su + cpl * 32 + kq);
+      const int I = ti0 / 4 - 1 + pi, J = tj0 / 4 - 1 + pj;
+      const bool cin = I >= 0 && I < Hc && J >= 0 && J < Wc;
+      const int fr = 4 * pi + wave - 2;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const F::T wuf = F::load(swu + ((4 * wave + j) * 16 + px) * 32 + kq);
-        const f32x4_t a = F::mfma(wuf, bfr, f32x4_t{0.f, 0.f, 0.f, 0.f});
-        const int ab = 4 * wave + j, ai = ab >> 2, bj = ab & 3;
-        const int fr = 4 * (cpl / 6) + ai - 2, fc = 4 * (cpl % 6) + bj - 2;
-        const int I = ti0 / 4 - 1 + cpl / 6, J = tj0 / 4 - 1 + cpl % 6;
-        if (cpl < NC && fr >= 0 && fr < XP && fc >= 0 && fc < XP) {
-          uint2 v = {0u, 0u};
-          if (I >= 0 && I < Hc && J >= 0 && J < Wc)
-            v = uint2{(uint32_t)f2bf(a[0] + bur[0]) | ((uint32_t)f2bf(a[1] + bur[1]) << 16),
-                      (uint32_t)f2bf(a[2] + bur[2]) | ((uint32_t)f2bf(a[3] + bur[3]) << 16)};
-          *reinterpret_cast<uint2*>(sc + (fr * XP + fc) * 32 + cq) = v;
-        }
+        const f32x4_t a = F::mfma(wuf[j], bfr, f32x4_t{0.f, 0.f, 0.f, 0.f});
+        const int fc = 4 * pj + j - 2;
+        const bool ok = fr >= 0 && fr < XP && fc >= 0 && fc < XP;
+        const uint32_t h0 = (uint32_t)f2bf(a[0] + bur[0]) | ((uint32_t)f2bf(a[1] + bur[1]) << 16);
+        const uint32_t h1 = (uint32_t)f2bf(a[2] + bur[2]) | ((uint32_t)f2bf(a[3] + bur[3]) << 16);
+        bf16* dst = ok ? sc + (fr * XP + fc) * 32 + cq : junk + 4 * lane;
+        *reinterpret_cast<uint2*>(dst) = uint2{cin ? h0 : 0u, cin ? h1 : 0u};
       }
     }
     __syncthreads();
-    // C: the 32 -> 16 convolution on the 18 x 18 halo into sd
+    // C: the 32 -> 16 convolution on the 18 x 18 halo into sd (zero outside
+    // the grid)
     auto mid_store = [&](const f32x4_t& a, int R, int c) {
       const int gi = ti0 - 1 + R, gj = tj0 - 1 + c;
-      uint2 v = {0u, 0u};
-      if (gi >= 0 && gi < H && gj >= 0 && gj < W) {
-        float q[4];
+      const bool in = gi >= 0 && gi < H && gj >= 0 && gj < W;
+      float q[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const float s = a[u] + bmr[u];
-          q[u] = s > 0.f ? s : 0.f;
-        }
-        v = uint2{(uint32_t)f2bf(q[0]) | ((uint32_t)f2bf(q[1]) << 16),
-                  (uint32_t)f2bf(q[2]) | ((uint32_t)f2bf(q[3]) << 16)};
+      for (int u = 0; u < 4; ++u) {
+        const float s = a[u] + bmr[u];
+        q[u] = s > 0.f ? s : 0.f;
       }
-      *reinterpret_cast<uint2*>(sd + (R * TP + c) * 16 + cq) = v;
+      const uint32_t h0 = (uint32_t)f2bf(q[0]) | ((uint32_t)f2bf(q[1]) << 16);
+      const uint32_t h1 = (uint32_t)f2bf(q[2]) | ((uint32_t)f2bf(q[3]) << 16);
+      *reinterpret_cast<uint2*>(sd + (R * TP + c) * 16 + cq) = uint2{in ? h0 : 0u, in ? h1 : 0u};
     };
     {
-      f32x4_t acc[5];
+      // five row slots and (waves 0, 2, 3) one group of the right halo
+      // columns; wave 1's sixth slot repeats its last row
+      const int ii = min(max(sg, 0) * 16 + px, 2 * TP - 1);
+      const int Rs = sg >= 0 ? ii >> 1 : min(R0 + 4, TP - 1);
+      const int cs = sg >= 0 ? 16 + (ii & 1) : px;
+      f32x4_t acc[6];
 #pragma unroll
-      for (int a = 0; a < 5; ++a) acc[a] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      for (int a = 0; a < 6; ++a) acc[a] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
         const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const F::T wt = F::load(swm + (tap * 16 + px) * 32 + kq);
 #pragma unroll
-        for (int a = 0; a < 5; ++a) {
-          const int R = min(R0 + a, TP - 1);
-          const F::T af = F::load(sc + ((R + 1 + dy) * XP + px + 1 + dx) * 32 + kq);
-          acc[a] = F::mfma(wmf[tap], af, acc[a]);
+        for (int a = 0; a < 6; ++a) {
+          const int R = a < 5 ? min(R0 + a, TP - 1) : Rs, c = a < 5 ? px : cs;
+          const F::T af = F::load(sc + ((R + 1 + dy) * XP + c + 1 + dx) * 32 + kq);
+          acc[a] = F::mfma(wt, af, acc[a]);
         }
       }
 #pragma unroll
-      for (int a = 0; a < 5; ++a)
-        if (a < nr) mid_store(acc[a], R0 + a, px);
-    }
-    if (sg >= 0) {
-      const int idx = sg * 16 + px;
-      const int ii = min(idx, 2 * TP - 1);
-      const int R = ii >> 1, c = 16 + (ii & 1);
-      f32x4_t a = f32x4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int tap = 0; tap < 9; ++tap) {
-        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-        a = F::mfma(wmf[tap], F::load(sc + ((R + 1 + dy) * XP + c + 1 + dx) * 32 + kq), a);
-      }
-      if (idx < 2 * TP) mid_store(a, R, c);
+      for (int a = 0; a < 5; ++a) mid_store(acc[a], min(R0 + a, TP - 1), px);
+      mid_store(acc[5], Rs, cs);
     }
     __syncthreads();
     // D: the 16 -> 16 convolution and the output layer
@@ -431,7 +429,7 @@ __global__ __launch_bounds__(256, 2) void k_dec0_apply(View d1, View skip,
     conv16_weights<16, 1>(swl, lane, wf);
     f32x4_t acc[4][1];
     conv16_mma<1, true>(sd, wf, lane, wave, acc);
-    conv16_epilogue<1>(acc, ep, 1, ti0, tj0, W, View{}, 0, View{}, yout, lane, wave);
+    conv16_epilogue<1, EPI_OUT>(acc, ep, 1, ti0, tj0, W, View{}, View{}, yout, lane, wave);
   }
 }
 
@@ -792,6 +790,9 @@ struct cpgpu_unet {
   int conv_rows = 4;     // pixel rows per wave of the 16-channel tile convolutions (4 or 8)
   int conv_persist = 4;  // > 0: persistent 16-channel tiles, blocks per compute unit
   int fuse_level0 = 1;   // applied networks: level 0 in two fused kernels (needs conv_persist > 0)
+  // bumped whenever the activations move or the launch configuration
+  // changes: a captured application is then stale
+  long long generation = 0;
   int ncu = 256;         // compute units of the device
   int walls = 0;      // wall types of the current input (2 bits per segment x-, x+, y-, y+)
   size_t woff[kNL], boff[kNL], wbf_off[kNL];
@@ -871,6 +872,7 @@ int unet_alloc(cpgpu_unet* U, int H, int W) {
     return set_error(U->ctx, CPGPU_ENOMEM, "U-Net activations");
   U->H = H;
   U->W = W;
+  ++U->generation;
   return CPGPU_OK;
 }
 
@@ -911,7 +913,7 @@ int conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{},
   const EvPair ev = (CIN == 16 && COUT == 16 && M >= kSmallM)
                         ? timer_events(U->ctx, out_fusable ? 3 : 2) : EvPair();
   if (M < kSmallM) {
-    hipLaunchKernelGGL((k_conv3_fwd<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, COUT)),
+    hipLaunchKernelGGL((k_conv3_fwd_ks<CIN, COUT>), dim3((M + 15) / 16, COUT / 16),
                        dim3(256), 0, U->ctx->stream, x, y, w, b, relu);
     return 0;
   }
@@ -923,11 +925,24 @@ int conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{},
         if (U->conv_persist > 0) {
           const int ntx = x.W / 16, ntiles = ntx * (x.H / 16);
           const int G = std::min(ntiles, U->ncu * U->conv_persist);
-          const bool out = COUT == 16 && wout && yout && yscale;
-          CPGPU_LAUNCH(ev, (k_conv3_tile16p<COUT, true>), dim3(G), dim3(256), 0,
-                       U->ctx->stream, x, y, GView{}, w, b, relu, fuse ? pout : View{}, ntx,
-                       ntiles, out ? wout : nullptr, yout, yscale, out ? store_act : 1);
-          return (fuse ? 1 : 0) | (out ? 2 : 0);
+          const bool out = COUT == 16 && wout && yout && yscale && !store_act;
+          if constexpr (COUT == 16) {
+            if (out) {
+              CPGPU_LAUNCH(ev, (k_conv3_tile16p<COUT, true, EPI_OUT>), dim3(G), dim3(256), 0,
+                           U->ctx->stream, x, y, GView{}, w, b, relu, View{}, ntx, ntiles, wout,
+                           yout, yscale);
+              return 2;
+            }
+          }
+          if (fuse)
+            CPGPU_LAUNCH(ev, (k_conv3_tile16p<COUT, true, EPI_POOL>), dim3(G), dim3(256), 0,
+                         U->ctx->stream, x, y, GView{}, w, b, relu, pout, ntx, ntiles, nullptr,
+                         nullptr, nullptr);
+          else
+            CPGPU_LAUNCH(ev, (k_conv3_tile16p<COUT, true, EPI_STORE>), dim3(G), dim3(256), 0,
+                         U->ctx->stream, x, y, GView{}, w, b, relu, View{}, ntx, ntiles, nullptr,
+                         nullptr, nullptr);
+          return fuse ? 1 : 0;
         }
         if (!fuse && x.H % 32 == 0 && U->conv_rows == 8)
           CPGPU_LAUNCH(ev, (k_conv3_tile16<COUT, true, 8>), dim3(x.W / 16, x.H / 32),
@@ -1079,9 +1094,10 @@ int conv_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, int in_co,
           if (U->conv_persist > 0) {
             const int ntx = in.W / 16, ntiles = ntx * (in.H / 16);
             const int G = std::min(ntiles, U->ncu * U->conv_persist);
-            hipLaunchKernelGGL((k_conv3_tile16p<CIN, false>), dim3(G), dim3(256), 0, st, gq,
-                               View{}, gx, wt, (const float*)nullptr, 0, View{}, ntx, ntiles,
-                               (const float*)nullptr, (float*)nullptr, (const float*)nullptr, 1);
+            hipLaunchKernelGGL((k_conv3_tile16p<CIN, false, EPI_STORE>), dim3(G), dim3(256), 0,
+                               st, gq, View{}, gx, wt, (const float*)nullptr, 0, View{}, ntx,
+                               ntiles, (const float*)nullptr, (float*)nullptr,
+                               (const float*)nullptr);
           } else if (in.H % 32 == 0 && U->conv_rows == 8)
             hipLaunchKernelGGL((k_conv3_tile16<CIN, false, 8>), dim3(in.W / 16, in.H / 32),
                                dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0,
@@ -1307,6 +1323,7 @@ int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out, 
   return apply_scaled(U, in, out, (size_t)H * W);
 }
 cpgpu_ctx* unet_context(const cpgpu_unet* U) { return U->ctx; }
+long long unet_generation(const cpgpu_unet* U) { return U->generation; }
 }  // namespace cpgpu
 
 extern "C" {
@@ -1403,12 +1420,14 @@ int cpgpu_unet_set_conv_shape(cpgpu_unet* U, int rows, int persist) {
   if (!U || (rows != 4 && rows != 8) || persist < 0 || persist > 16) return CPGPU_EINVAL;
   U->conv_rows = rows;
   U->conv_persist = persist;
+  ++U->generation;
   return CPGPU_OK;
 }
 
 int cpgpu_unet_set_fusion(cpgpu_unet* U, int level0) {
   if (!U || level0 < 0 || level0 > 1) return CPGPU_EINVAL;
   U->fuse_level0 = level0;
+  ++U->generation;
   return CPGPU_OK;
 }
 

@@ -183,6 +183,80 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(View x, View y,
   }
 }
 
+// conv 3x3 forward for small levels (a few thousand pixels or fewer), the
+// nine taps split over the block's four waves: wave w sums taps
+// [kT0[w], kT0[w + 1]) = {0,1,2}, {3,4}, {5,6}, {7,8} for the block's 16
+// pixels x 16 output channels, every load of its taps issued before its
+// MFMAs (one memory round trip per wave, where a wave walking all nine taps
+// makes nine), and wave 0 adds the four partial sums in wave order from LDS
+// and applies the epilogue.  grid: x = 16-pixel groups, y = 16-channel groups.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_conv3_fwd_ks(View x, View y, const bf16* __restrict__ w,
+                                                      const float* __restrict__ bias, int relu) {
+  constexpr int KC = (CIN % 32 == 0) ? 32 : 16;
+  constexpr int NC = CIN / KC;
+  typedef Frag<KC> F;
+  __shared__ float red[3][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int M = x.H * x.W;
+  const int m = blockIdx.x * 16 + (lane & 15);
+  const int n0 = blockIdx.y * 16;
+  const int pi = m < M ? m / x.W : -1000000, pj = m < M ? m - (m / x.W) * x.W : 0;
+  const int kq = F::KPL * (lane >> 4);
+  const int t0 = wave == 0 ? 0 : 1 + 2 * wave, nt = wave == 0 ? 3 : 2;
+  typename F::T bf[3][NC], af[3][NC];
+  bool ok[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int tap = min(t0 + k, 8);
+    const int si = pi + tap / 3 - 1, sj = pj + tap % 3 - 1;
+    ok[k] = k < nt && si >= 0 && si < x.H && sj >= 0 && sj < x.W;
+    const int ci = min(max(si, 0), x.H - 1), cj = min(max(sj, 0), x.W - 1);
+    const size_t off = (size_t)(ci * x.W + cj) * x.cs + x.co + kq;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      bf[k][c] = F::load(w + ((size_t)(tap * COUT + n0 + (lane & 15)) * CIN + c * KC + kq));
+      af[k][c] = F::load(x.p + off + c * KC);
+    }
+  }
+  f32x4_t acc = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (k < nt) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc = F::mfma(bf[k][c], ok[k] ? af[k][c] : F::zero(), acc);
+    }
+  }
+  const int e = (4 * (lane >> 4)) * 16 + (lane & 15);  // + 16 t: D[4 (lane / 16) + t][lane % 16]
+  if (wave > 0) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) red[wave - 1][e + 16 * t] = acc[t];
+  }
+  __syncthreads();
+  if (wave > 0 || m >= M) return;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    float v = acc[t];
+    v += red[0][e + 16 * t];
+    v += red[1][e + 16 * t];
+    v += red[2][e + 16 * t];
+    acc[t] = v;
+  }
+  const int co = n0 + 4 * (lane >> 4);
+  uint32_t h[2];
+#pragma unroll
+  for (int t = 0; t < 4; t += 2) {
+    float v0 = acc[t] + bias[co + t];
+    float v1 = acc[t + 1] + bias[co + t + 1];
+    if (relu) {
+      v0 = v0 > 0.f ? v0 : 0.f;
+      v1 = v1 > 0.f ? v1 : 0.f;
+    }
+    h[t >> 1] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+  }
+  *reinterpret_cast<uint2*>(y.p + (size_t)m * y.cs + y.co + co) = uint2{h[0], h[1]};
+}
+
 // conv 3x3 input gradient: dx[m][ci] = sum_tap sum_co g[m - off][co]
 // WT[tap][ci][co], g = the layer's output gradient masked by its ReLU and
 // rounded to BF16 once (k_mask_grad), read like an activation.
@@ -572,18 +646,20 @@ struct Conv16Epi {
 };
 
 // Forward epilogue of a 16 x 16 tile at (ti0, tj0) of a W-wide level:
-// relu(acc + bias) rounded to BF16 and stored into yb (store_act); with
-// pout.p the 4 x 4 average pooling into pout (rows summed in the lane, the
-// four columns of a window by lane shuffles, then / 16 and rounded: the sum
-// of 16 non-negative BF16 values is exact in FP32, as in k_pool_fwd); with
-// yout (NB = 1) the 1 x 1 output layer y = (wo16 + sum_c wo[c] out_c) * ysc,
+// relu(acc + bias) rounded to BF16 and stored into yb (STORE); the 4 x 4
+// average pooling into pout (POOL: rows summed in the lane, the four
+// columns of a window by lane shuffles, then / 16 and rounded: the sum of 16
+// non-negative BF16 values is exact in FP32, as in k_pool_fwd); the 1 x 1
+// output layer y = (wo16 + sum_c wo[c] out_c) * ysc into yout (OUT, NB = 1),
 // the four channel groups of a pixel summed by lane shuffles.
-template <int NB>
+enum { EPI_STORE = 0, EPI_POOL = 1, EPI_OUT = 2 };
+template <int NB, int EPI>
 __device__ __forceinline__ void conv16_epilogue(const f32x4_t (&acc)[4][NB],
                                                 const Conv16Epi<NB>& ep, int relu, int ti0,
-                                                int tj0, int W, View yb, int store_act,
-                                                View pout, float* __restrict__ yout,
-                                                int lane, int wave) {
+                                                int tj0, int W, View yb, View pout,
+                                                float* __restrict__ yout, int lane, int wave) {
+  constexpr bool STORE = EPI != EPI_OUT, POOL = EPI == EPI_POOL, OUT = EPI == EPI_OUT;
+  static_assert(!OUT || NB == 1, "the output layer follows a 16-channel convolution");
   const int px = lane & 15, cq = 4 * (lane >> 4);
   float ps[NB][4];
 #pragma unroll
@@ -592,7 +668,7 @@ __device__ __forceinline__ void conv16_epilogue(const f32x4_t (&acc)[4][NB],
     for (int q = 0; q < 4; ++q) ps[b][q] = 0.f;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const size_t m = (size_t)(ti0 + wave * 4 + a) * W + tj0 + px;
+    const int m = (ti0 + wave * 4 + a) * W + tj0 + px;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int n = b * 16 + cq;
@@ -602,12 +678,12 @@ __device__ __forceinline__ void conv16_epilogue(const f32x4_t (&acc)[4][NB],
         float v = acc[a][b][u] + ep.bias[b][u];
         if (relu) v = v > 0.f ? v : 0.f;
         q[u] = f2bf(v);
-        ps[b][u] += bf2f(q[u]);
+        if (POOL) ps[b][u] += bf2f(q[u]);
       }
-      if (store_act)
-        *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) =
+      if (STORE)
+        *reinterpret_cast<uint2*>(yb.p + (size_t)m * yb.cs + yb.co + n) =
             uint2{(uint32_t)q[0] | ((uint32_t)q[1] << 16), (uint32_t)q[2] | ((uint32_t)q[3] << 16)};
-      if (NB == 1 && yout) {
+      if (OUT) {
         float part = 0.f;
 #pragma unroll
         for (int u = 0; u < 4; ++u) part += ep.wo[u] * bf2f(q[u]);
@@ -617,7 +693,7 @@ __device__ __forceinline__ void conv16_epilogue(const f32x4_t (&acc)[4][NB],
       }
     }
   }
-  if (pout.p) {
+  if (POOL) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
@@ -628,12 +704,12 @@ __device__ __forceinline__ void conv16_epilogue(const f32x4_t (&acc)[4][NB],
         ps[b][u] = v;
       }
     if ((px & 3) == 0) {
-      const size_t mp = (size_t)(ti0 / 4 + wave) * pout.W + tj0 / 4 + px / 4;
+      const int mp = (ti0 / 4 + wave) * pout.W + tj0 / 4 + px / 4;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         const bf16 q0 = f2bf(ps[b][0] * (1.f / 16.f)), q1 = f2bf(ps[b][1] * (1.f / 16.f));
         const bf16 q2 = f2bf(ps[b][2] * (1.f / 16.f)), q3 = f2bf(ps[b][3] * (1.f / 16.f));
-        *reinterpret_cast<uint2*>(pout.p + mp * pout.cs + pout.co + b * 16 + cq) =
+        *reinterpret_cast<uint2*>(pout.p + (size_t)mp * pout.cs + pout.co + b * 16 + cq) =
             uint2{(uint32_t)q0 | ((uint32_t)q1 << 16), (uint32_t)q2 | ((uint32_t)q3 << 16)};
       }
     }
@@ -663,19 +739,18 @@ struct TileWalk {
 // registers while it computes the current one from LDS, so the staging
 // latency overlaps the MFMAs; the weights are loaded once per block.
 //
-// With wout set (forward, NO = 16) the epilogue also applies the network's
-// 1 x 1 output layer to the BF16 output into yout; store_act = 0 then skips
-// storing the layer's own output (no later kernel reads it when the network
-// is only applied).
-template <int NO, bool FWD>
+// The forward epilogue EPI: the output stored (EPI_STORE), stored and pooled
+// into pout (EPI_POOL), or, NO = 16, only the network's 1 x 1 output layer
+// applied to it into yout (EPI_OUT: no later kernel reads the layer's own
+// output when the network is only applied).
+template <int NO, bool FWD, int EPI>
 __global__ __launch_bounds__(256, NO == 16 ? 4 : 1) void k_conv3_tile16p(View x, View yb, GView yf,
                                                        const bf16* __restrict__ w,
                                                        const float* __restrict__ bias,
                                                        int relu, View pout, int ntx,
                                                        int ntiles, const float* __restrict__ wout,
                                                        float* __restrict__ yout,
-                                                       const float* __restrict__ yscale,
-                                                       int store_act) {
+                                                       const float* __restrict__ yscale) {
   constexpr int KCH = 16, NB = NO / 16, TP = 18, CPP = 2;
   constexpr int NE = TP * TP * CPP, NPT = (NE + 255) / 256;
   typedef Frag<32> F;
@@ -691,7 +766,7 @@ __global__ __launch_bounds__(256, NO == 16 ? 4 : 1) void k_conv3_tile16p(View x,
   // under a branch, or a later load that must be waited for, makes the
   // compiler drain it early).
   Conv16Epi<NB> ep;  // forward only (the input gradient has no bias)
-  if (FWD) ep.load(bias, NO == 16 ? wout : nullptr, yscale, lane);
+  if (FWD) ep.load(bias, EPI == EPI_OUT ? wout : nullptr, yscale, lane);
   uint4 pre[NPT];
   bool pok[NPT];
   auto fetch = [&](int t) {
@@ -726,8 +801,7 @@ __global__ __launch_bounds__(256, NO == 16 ? 4 : 1) void k_conv3_tile16p(View x,
     f32x4_t acc[4][NB];
     conv16_mma<NB, FWD>(sx, wf, lane, wave, acc);
     if constexpr (FWD) {
-      conv16_epilogue<NB>(acc, ep, relu, ti0, tj0, W, yb, store_act, pout,
-                          NO == 16 && wout ? yout : nullptr, lane, wave);
+      conv16_epilogue<NB, EPI>(acc, ep, relu, ti0, tj0, W, yb, pout, yout, lane, wave);
     } else {
       const int px = lane & 15, cq = 4 * (lane >> 4);
 #pragma unroll

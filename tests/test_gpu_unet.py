@@ -326,7 +326,7 @@ def test_learned_preconditioner_gains_over_jacobi_pcg(ctx):
     net = ca.UNet(ctx, seed=1)
     tr = learned.CavityTrainer(ns, net, nu=2, omega=0.8)
     t0 = time.perf_counter()
-    tr.run(plain=60, first_fit=300, rounds=2, round_steps=20, round_fit=200)
+    tr.run(plain=60, first_fit=600, rounds=2, round_steps=20, round_fit=400)
     ctx.sync()
     say(f"  trained in {time.perf_counter() - t0:.1f} s: {tr.log}")
     P = ns.projection()

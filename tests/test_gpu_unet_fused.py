@@ -135,3 +135,28 @@ def test_device_bf16_rounding_is_round_to_nearest_even(ctx):
         assert bad.size == 0, (co, [hex(int(b)) for b in bits[bad[:5]]])
     assert np.isinf(want.ravel()[4 * k + 5]) and np.count_nonzero(want.ravel()[:k] == 0) < k
     net.close()
+
+
+def test_replayed_iteration_graphs_take_the_same_iterates(ctx):
+    # with use_graphs = 2 the flexible-CG iterations after the first replay
+    # one captured graph per direction parity; otherwise the same kernels
+    # launch directly
+    n = 256
+    og, P = problem(ctx, n, n, [2, 2, 1, 2], 9)
+    net = ca.UNet(ctx, seed=1)
+    net.set_params(random_params(23).astype(np.float32))
+    out = []
+    for graphs in (True, 2, 2):  # the second graphed solve replays cached graphs
+        P.set_batch(16, use_graphs=graphs)
+        P.set_unet(net, ca.UNET_PRECOND)
+        P.fill_phi(0.0)
+        code, it, res, _ = P.solve("FCG_UNET", max_iter=5000, tol=1e-10, norm=ca.NORM_TRUE)
+        print(f"  FCG graphs={graphs}: code {code} iterations {it} res {res:.3e}", flush=True)
+        out.append((code, it, res, og.interior(P.get_phi()).copy()))
+        P.set_unet(None, 0)
+    for o in out[1:]:
+        assert o[:3] == out[0][:3]
+        assert np.array_equal(o[3], out[0][3])
+    assert out[0][0] == 0 and out[0][1] > 2
+    net.close()
+    P.close()
