@@ -36,11 +36,18 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
-# HBM bytes per launch of k_conv3_tile16p<16, true> storing its output and
-# its 4 x 4 pooling at 1024^2, from rocprofv3 PMC passes (FETCH_SIZE x 2.0
-# and WRITE_SIZE x 1.0, the factors a copy kernel of known size reads):
-# profiles/k_conv3_tile16p_1024_traffic_pmc.txt
-CONV16_TRAFFIC_1024 = (2 * 16787.8 + 34816.0) * 1024
+# HBM bytes per launch of k_dec0_apply at 1024^2, from rocprofv3 PMC passes
+# (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a copy kernel of known
+# size reads): profiles/k_dec0_apply_1024_traffic_pmc.txt
+DEC0_TRAFFIC_1024 = (2 * 19210.3 + 4096.0) * 1024
+
+
+def cpu_threads():
+    """Host threads of the CPU baseline: the GPU's share of the host (16 per
+    GPU), or fewer where the machine has fewer."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = int(env) if env and env.isdigit() else 16
+    return max(1, min(n, os.cpu_count() or 1))
 
 
 def log(msg):
@@ -265,10 +272,25 @@ def ns_explicit_probe(ctx, ca, no_cpu, n=512, steps=100, warm=2):
         c0 = time.perf_counter()
         _, it_cpu, _ = ora.step()
         cpu_s = time.perf_counter() - c0
-        out["cpu_baseline"] = {"first_step_ms": round(cpu_s * 1e3, 1), "cores": 1, "kind": "port",
-                               "iterations": it_cpu,
+        # the same step on every thread of the GPU's host share (another
+        # summation order in the dots, so its iteration count may differ by one)
+        threads = cpu_threads()
+        ora2 = O.NS(g, O.make_bc(g, [1] * 4, [0, 0, 0, 1.0]), O.make_bc(g, [1] * 4),
+                    rho=1000.0, mu=1.0, cfl=0.5, dnum=0.25, dcoef=1e-3, limiter="VanLeer",
+                    tol=1e-10, dt=0.5 * h)
+        O.set_threads(threads)
+        try:
+            c0 = time.perf_counter()
+            _, it_cpu2, _ = ora2.step()
+            cpu_s2 = time.perf_counter() - c0
+        finally:
+            O.set_threads(1)
+        out["cpu_baseline"] = {"first_step_ms": round(cpu_s2 * 1e3, 1), "cores": threads,
+                               "kind": "port", "iterations": it_cpu2,
+                               "single_thread_first_step_ms": round(cpu_s * 1e3, 1),
                                "sample": "the oracle's first NS_explicit_ step from rest at the "
-                                         "same size (C, -O2, one thread)"}
+                                         f"same size (C, -O2, OpenMP on {threads} threads; "
+                                         "one thread beside it)"}
         out["first_step_iterations_match"] = abs(int(it0[0]) - it_cpu) <= 1
     ns.close()
     log(f"bench: NS_explicit_ {n}^2: {out}")
@@ -327,10 +349,22 @@ def blob_probe(ctx, ca, no_cpu, n=4096, tol=1e-10):
         c0 = time.perf_counter()
         O.solve("JPCG", A, b, max_iter=k, tol=1e-300, norm=O.NORM_TRUE)
         t_it = (time.perf_counter() - c0) / k
-        res["cpu_baseline"] = {"seconds_to_solution": round(t_it * res["JPCG"]["iterations"], 1),
-                               "ms_per_iteration": round(t_it * 1e3, 2), "cores": 1, "kind": "port",
-                               "sample": f"{k} oracle FP64 J-PCG iterations at {n}^2 (C, -O2, one "
-                                         f"thread), times the GPU FP64 iteration count"}
+        threads = cpu_threads()
+        k2 = 100
+        O.set_threads(threads)
+        try:
+            c0 = time.perf_counter()
+            O.solve("JPCG", A, b, max_iter=k2, tol=1e-300, norm=O.NORM_TRUE)
+            t_it2 = (time.perf_counter() - c0) / k2
+        finally:
+            O.set_threads(1)
+        res["cpu_baseline"] = {"seconds_to_solution": round(t_it2 * res["JPCG"]["iterations"], 1),
+                               "ms_per_iteration": round(t_it2 * 1e3, 2), "cores": threads,
+                               "kind": "port",
+                               "single_thread_ms_per_iteration": round(t_it * 1e3, 2),
+                               "sample": f"{k2} oracle FP64 J-PCG iterations at {n}^2 (C, -O2, "
+                                         f"OpenMP on {threads} threads; {k} on one thread), "
+                                         f"times the GPU FP64 iteration count"}
     P.close()
     return res
 
@@ -400,9 +434,31 @@ def kernel_probe(ca, mode, size, precision=32, reps=50, method="MIXED", tol=1e-8
              launch pattern of a real solve); at 4096^2 with a loose tol its
              k_merge_x launches (x64 += x32 as an exact pair, x32 = 0) stream
              vectors far larger than the caches;
-      unet:  `reps` forward passes of the U-Net (seed 1) on a size^2 input."""
+      unet:  `reps` forward passes of the U-Net (seed 1) on a size^2 input;
+      apply: `reps` applications of the same network as a preconditioner
+             applies it (the fused level-0 kernels), as warm starts of a
+             size^2 Poisson problem."""
     n = size
     ctx = ca.Context(0)
+    if mode == "apply":
+        net = ca.UNet(ctx, seed=1)
+        types = [ca.BC_DIRICHLET] * 3 + [ca.BC_NEUMANN]
+        P = ca.Poisson(ctx, ca.uniform_grid_desc(n, n, 1, types))
+        P.set_source_blobs(seed=0, nblobs=64, wmin=0.01, wmax=0.1)
+        P.set_unet(net, ca.UNET_WARM)
+        for f in (8, 9):
+            ctx.timer_enable(f)
+        for _ in range(reps):
+            P.fill_phi(0.0)
+            P.solve("JPCG", max_iter=0, tol=1e-300)
+        (ke, mse), (kd, msd) = ctx.timer_read(8), ctx.timer_read(9)
+        P.set_unet(None, 0)
+        P.close()
+        net.close()
+        ctx.close()
+        return {"size": n, "mode": "apply", "reps": reps, "enc0_launches": ke,
+                "enc0_us": round(mse * 1e3 / max(ke, 1), 3), "dec0_launches": kd,
+                "dec0_us": round(msd * 1e3 / max(kd, 1), 3)}
     if mode == "unet":
         net = ca.UNet(ctx, seed=1)
         x = np.random.default_rng(0).standard_normal((n, n)).astype(np.float32)
@@ -471,7 +527,7 @@ def main():
                     help="skip the general-CSR Jacobi-PCG probe")
     ap.add_argument("--no-hbm", action="store_true",
                     help="skip the 4096^2 Krylov-kernel HBM probe")
-    ap.add_argument("--probe", choices=("times", "solve", "unet"), default="",
+    ap.add_argument("--probe", choices=("times", "solve", "unet", "apply"), default="",
                     help="kernels alone on the --size^2 blob problem, for profilers")
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--reps", type=int, default=50)
@@ -617,40 +673,53 @@ def main():
                    "iterations_learned": itl2, "iterations_plain_fp64": itp2}
     log(f"bench: at {tight:g}: {field_tight}")
 
-    # ---- dominant U-Net kernel: timestamps of its own dispatches ----
+    # ---- dominant kernels of the learned step: timestamps of their own
+    # dispatches (the applied network's fused level-0 decoder and encoder)
     ns.restore(state)
     P.set_unet(net, ca.UNET_PRECOND)
     ns.set_solver("FCG_UNET", max_iter=10 ** 7, tol=tol, norm=ca.NORM_TRUE)
-    ctx.timer_enable(2)
-    ctx.timer_enable(3)
+    for f in (8, 9):
+        ctx.timer_enable(f)
     ns.step(1)
-    n_c, ms_c = ctx.timer_read(2)
-    n_o, ms_o = ctx.timer_read(3)
-    ctx.timer_enable(2, False)
-    ctx.timer_enable(3, False)
-    us = ms_c * 1e3 / max(n_c, 1)
-    # read 16 BF16 channels, write 16 and their 4 x 4 pooling
-    conv_bytes = 2 * N * 16 * 2 + (N // 16) * 16 * 2
-    conv_flops = 2 * 9 * 16 * 16 * N
-    achieved = conv_bytes / (us * 1e-6) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": int(CONV16_TRAFFIC_1024) if n == 1024 else None,
-                "kernel": "k_conv3_tile16p<16,true> (finest U-Net level, 3x3 forward 16 -> 16 "
-                          "with fused 4x4 pooling, BF16 MFMA, persistent)",
-                "bytes_per_launch": conv_bytes, "avg_launch_us": round(us, 3),
-                "launches": n_c,
-                "mfma_tflops": round(conv_flops / (us * 1e-6) / 1e12, 2),
-                "mfma_frac_bf16": round(conv_flops / (us * 1e-6) / 1e12 / BF16_PEAK_TFLOPS, 4)}
-    if n_o:
-        us_o = ms_o * 1e3 / n_o
-        bytes_o = N * 16 * 2 + N * 4   # read 16 BF16 channels, write the FP32 output
-        roofline["fused_output_launch"] = {
-            "avg_launch_us": round(us_o, 3), "launches": n_o, "bytes_per_launch": bytes_o,
-            "achieved_GBs": round(bytes_o / (us_o * 1e-6) / 1e9, 1)}
-    log(f"bench: conv16 {us:.2f} us/launch, {achieved:.0f} GB/s; fused-output launch "
-        f"{(ms_o * 1e3 / max(n_o, 1)):.2f} us")
+    n_e, ms_e = ctx.timer_read(8)
+    n_d, ms_d = ctx.timer_read(9)
+    for f in (8, 9):
+        ctx.timer_enable(f, False)
+    us_d = ms_d * 1e3 / max(n_d, 1)
+    us_e = ms_e * 1e3 / max(n_e, 1)
+    # decoder: read the level-1 output (N/16 pixels x 32 BF16) and the skip
+    # channels (N x 16 BF16), write the FP32 output: 40 B per cell; MFMA
+    # work per cell: up-conv 16x32, 3x3 32 -> 16, 3x3 16 -> 16 (2 flop/MAC)
+    dec_bytes = 40 * N
+    dec_flops = 2 * (16 * 32 + 9 * 16 * 32 + 9 * 16 * 16) * N
+    dec_tf = dec_flops / (us_d * 1e-6) / 1e12 if n_d else 0.0
+    dec_gbs = dec_bytes / (us_d * 1e-6) / 1e9 if n_d else 0.0
+    # encoder: read the FP32 input, write the skip channels and the pooled
+    # level-1 input: 4 + 32 + 2 B per cell; first layer 2 -> 16 (FP32 FMA)
+    # and 3x3 16 -> 16 on MFMA
+    enc_bytes = 38 * N
+    enc_flops = 2 * (9 * 2 * 16 + 9 * 16 * 16) * N
+    enc_gbs = enc_bytes / (us_e * 1e-6) / 1e9 if n_e else 0.0
+    roofline = {"bound": "mfma", "achieved": round(dec_tf, 2), "peak": BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(dec_tf / BF16_PEAK_TFLOPS, 4),
+                "traffic": int(DEC0_TRAFFIC_1024) if (n == 1024 and DEC0_TRAFFIC_1024) else None,
+                "kernel": "k_dec0_apply (fused finest-level decoder of the applied U-Net: "
+                          "4x4 up-convolution 32 -> 16, 3x3 32 -> 16, 3x3 16 -> 16, 1x1 "
+                          "output layer; BF16 MFMA, persistent 16 x 16 tiles)",
+                "flops_per_launch": dec_flops, "bytes_per_launch": dec_bytes,
+                "flop_per_byte": round(dec_flops / dec_bytes, 1),
+                "avg_launch_us": round(us_d, 3), "launches": n_d,
+                "achieved_GBs": round(dec_gbs, 1),
+                "hbm_frac": round(dec_gbs / HBM_PEAK_GBS, 4),
+                "encoder": {"kernel": "k_enc0_apply (first layer 2 -> 16 + 3x3 16 -> 16 + "
+                                      "4x4 pooling)", "avg_launch_us": round(us_e, 3),
+                            "launches": n_e, "bytes_per_launch": enc_bytes,
+                            "achieved_GBs": round(enc_gbs, 1),
+                            "hbm_frac": round(enc_gbs / HBM_PEAK_GBS, 4),
+                            "mfma_plus_fma_tflops": round(enc_flops / (us_e * 1e-6) / 1e12, 2)
+                            if n_e else None}}
+    log(f"bench: fused decoder {us_d:.2f} us/launch ({dec_tf:.0f} TFLOP/s), "
+        f"encoder {us_e:.2f} us ({enc_gbs:.0f} GB/s)")
 
     # ---- Krylov kernels of the plain solve (same step, timed dispatches) ----
     ns.restore(state)
@@ -696,25 +765,46 @@ def main():
         Pm.close()
         log(f"bench: 4096^2 fp64 dir {d_gbs:.0f} GB/s, upd {u_gbs:.0f} GB/s, copy {copy_gbs:.0f} GB/s")
 
-    # ---- CPU baseline: the oracle's FP64 J-PCG on the same problem ----
+    # ---- CPU baseline: whole NS_explicit_ steps of the oracle (the
+    # reference's plain FP64 Jacobi-PCG) from the timed region's start state,
+    # on every host thread of the GPU's share ----
     cpu = None
     if not a.no_cpu:
         import oracle as O
-        og = O.StructGrid(n, n, 1, -0.5, 0.5, -0.5, 0.5)
-        A = O.Matrix(og, O.make_bc(og, [2, 2, 2, 2]), 1.0)
-        b = O.field_to_x(og, np.pad(learned.interior(P, P.get_vec(1)), 1))
-        k = 100
+        threads = cpu_threads()
+        og = O.StructGrid(n, n, 2, -0.5, 0.5, -0.5, 0.5)
+        ora = O.NS(og, O.make_bc(og, [1, 1, 1, 1], [0.0, 0.0, 0.0, 1.0]),
+                   O.make_bc(og, [1, 1, 1, 1]), rho=1000.0, mu=1.0, cfl=0.5, dnum=0.25,
+                   dcoef=1e-3, limiter="VanLeer", tol=tol, norm=O.NORM_TRUE,
+                   max_iter=10 ** 7)
+        ns.restore(state)
+        for name in ("u", "v", "uf", "vf", "p", "phi"):
+            ora.f[name][:] = ns.get_field(name)
+        ora.st.t, ora.st.dt = ns.time()
+        O.set_threads(threads)
+        try:
+            t0 = time.perf_counter()
+            code_c, it_c, res_c = ora.step()
+            t_step = time.perf_counter() - t0
+        finally:
+            O.set_threads(1)
+        # one thread: a bounded sample of the same solve's iterations
+        k1 = 40
         t0 = time.perf_counter()
-        O.solve("JPCG", A, b, max_iter=k, tol=1e-300, norm=O.NORM_TRUE)
-        t_it = (time.perf_counter() - t0) / k
-        it_mean = float(np.mean(itp))
-        cpu = {"value": round(1.0 / (t_it * it_mean), 5), "unit": "solves/s",
-               "cores": 1, "kind": "port",
-               "sample": f"{k} FP64 Jacobi-PCG iterations on the {n}^2 cavity "
-                         f"projection problem (oracle C, -O2, 1 thread), "
-                         f"{t_it * 1e3:.2f} ms/iteration, extrapolated to the "
-                         f"{it_mean:.0f} iterations per step plain FP64 J-PCG needs"}
-        log(f"bench: cpu {cpu['value']} solves/s")
+        O.solve("JPCG", ora.A, np.random.default_rng(0).standard_normal(n * n), max_iter=k1,
+                tol=1e-300, norm=O.NORM_TRUE)
+        t_it1 = (time.perf_counter() - t0) / k1
+        cpu = {"value": round(1.0 / t_step, 5), "unit": "solves/s", "cores": threads,
+               "kind": "port",
+               "sample": f"one whole NS_explicit_ step of the {n}^2 cavity from the timed "
+                         f"region's start state: TVD predictor, FP64 Jacobi-PCG projection "
+                         f"to true rel-res {tol:g} ({it_c} iterations, the device's plain "
+                         f"FP64 solve took {itp[0]}), correction; oracle C (-O2, OpenMP) "
+                         f"on {threads} threads, {t_step:.2f} s",
+               "ms_per_iteration": round(t_step / max(it_c, 1) * 1e3, 3),
+               "single_thread_ms_per_iteration": round(t_it1 * 1e3, 3),
+               "single_thread_solves_per_s_est": round(1.0 / (t_it1 * max(it_c, 1)), 5)}
+        log(f"bench: cpu {cpu['value']} solves/s on {threads} threads ({it_c} iterations)")
 
     batch = None
     if not a.no_batch:

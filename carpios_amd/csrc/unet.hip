@@ -987,9 +987,12 @@ int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
   hipStream_t st = U->ctx->stream;
   const int M = U->H * U->W;
   // an applied network (output and unscaled input given) runs level 0 in
-  // the two fused kernels; a0, d0a, d0 and the up-convolution's half of C0
-  // are then not formed.  Only where the layer-per-kernel path takes the
-  // tiled convolutions (M >= kSmallM), whose summation order they repeat.
+  // the two fused kernels; the first layer's output, d0a, d0 and C0 are
+  // then not formed, and the skip channels (C0's second half in the
+  // layer-per-kernel path) go densely into a0's buffer: 32 B per pixel,
+  // where reading them out of C0's 64-byte pixels fetched whole lines.
+  // Only where the layer-per-kernel path takes the tiled convolutions
+  // (M >= kSmallM), whose summation order they repeat.
   const bool fused0 = final_out && raw && U->fuse_level0 && U->conv_persist > 0 && M >= kSmallM;
   const int ntx = U->W / 16, ntiles = ntx * (U->H / 16);
   bool pooled0 = false;
@@ -998,7 +1001,7 @@ int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
     CPGPU_LAUNCH(ev, k_enc0_apply, dim3(std::min(ntiles, U->ncu * U->conv_persist)), dim3(256), 0,
                  st, raw, U->scale.p, U->H, U->W, U->walls, U->params.p + U->woff[0],
                  U->params.p + U->boff[0], U->wbf.p + U->wbf_off[1], U->params.p + U->boff[1],
-                 U->v(U->C0, 16), U->v(U->p0), ntx, ntiles);
+                 U->v(U->a0), U->v(U->p0), ntx, ntiles);
     pooled0 = true;
   } else {
     hipLaunchKernelGGL(k_first_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
@@ -1030,7 +1033,7 @@ int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
   if (fused0) {
     const EvPair ev = timer_events(U->ctx, 9);
     CPGPU_LAUNCH(ev, k_dec0_apply, dim3(std::min(ntiles, U->ncu * 2)), dim3(256), 0, st,
-                 U->v(U->d1), U->v(U->C0, 16), U->wbf.p + U->wbf_off[14],
+                 U->v(U->d1), U->v(U->a0), U->wbf.p + U->wbf_off[14],
                  U->params.p + U->boff[14], U->wbf.p + U->wbf_off[15],
                  U->params.p + U->boff[15], U->wbf.p + U->wbf_off[16],
                  U->params.p + U->boff[16], U->params.p + U->woff[17], U->scale.p, final_out,

@@ -372,6 +372,12 @@ def set_plant(name):
     lib().orc_set_plant(C.c_int(_plant))
 
 
+def set_threads(n):
+    """Threads of the CPU baseline; 1 (the default) keeps every routine in
+    the reference's serial order, as the parity checks need."""
+    lib().orc_set_threads(C.c_int(int(n)))
+
+
 def upwind1_field(grid, uf, vf, phi):
     out = np.zeros((grid.nx, grid.ny))
     L = lib()

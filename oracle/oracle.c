@@ -54,6 +54,38 @@
 static int g_plant = 0;
 void orc_set_plant(int plant) { g_plant = plant; }
 
+/* Threads of the CPU baseline (orc_set_threads; default 1).  With one
+ * thread every routine runs its serial loop, in the reference's order.
+ * With more, the row loops of the matrix product, the vector updates and
+ * the cell loops of the NS predictor split over threads (each value is
+ * computed as before), and dots and norms sum fixed contiguous chunks in
+ * parallel, then the chunk sums in order: deterministic for a given thread
+ * count, but not the serial order, so parity checks run on one thread. */
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int orc_get_threads(void) { return g_threads; }
+#define ORC_PAR _Pragma("omp parallel for if (g_threads > 1) num_threads(g_threads) schedule(static)")
+
+/* sum over i of f(x[i], y[i]) in g_threads contiguous chunks
+ * (mode 0: x*y, 1: x*x, 2: x*x for x != 0) */
+static double par_sum(int n, const double *x, const double *y, int mode) {
+  double part[256];
+  int T = g_threads < 256 ? g_threads : 256, t;
+  double s = 0.0;
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+  for (t = 0; t < T; ++t) {
+    long lo = (long)n * t / T, hi = (long)n * (t + 1) / T, i;
+    double a = 0.0;
+    for (i = lo; i < hi; ++i) {
+      if (mode == 0) a = a + x[i] * y[i];
+      else if (mode == 1 || x[i] != 0.0) a += x[i] * x[i];
+    }
+    part[t] = a;
+  }
+  for (t = 0; t < T; ++t) s += part[t];
+  return s;
+}
+
 /* lib/structure/s_grid.hpp:372-395 (_set_uniform_grid): sizes are l/n,
  * centres by accumulation csum + 0.5*cs, then shifted by min - sum(ghost
  * sizes). */
@@ -100,6 +132,7 @@ void orc_axis_sizes(int n, int gl, double vmin, const double *sizes, double *c,
 double orc_dot(int n, const double *x, const double *y) {
   double d = 0.0;
   int i;
+  if (g_threads > 1) return par_sum(n, x, y, 0);
   for (i = 0; i < n; ++i) d = d + x[i] * y[i];
   return d;
 }
@@ -111,6 +144,7 @@ double orc_nrm2_ref(int n, const double *x) {
   int i;
   if (n <= 0) return 0.0;
   if (n == 1) return fabs(x[0]);
+  if (g_threads > 1) return sqrt(par_sum(n - 1, x, NULL, 2));
   for (i = 0; i < n - 1; ++i)
     if (x[i] != 0.0) norm += x[i] * x[i];
   return sqrt(norm);
@@ -119,6 +153,7 @@ double orc_nrm2_ref(int n, const double *x) {
 double orc_nrm2_true(int n, const double *x) {
   double norm = 0.0;
   int i;
+  if (g_threads > 1) return sqrt(par_sum(n, x, NULL, 1));
   for (i = 0; i < n; ++i) norm += x[i] * x[i];
   return sqrt(norm);
 }
@@ -130,9 +165,11 @@ static double nrm(int mode, int n, const double *x) {
 /* lib/algebra/matrix_SparCompRow.hpp:236-249: res[i] starts at 0 and adds
  * x[col]*val in CSR order. */
 void orc_csr_mult(const orc_csr *A, const double *x, double *y) {
-  int i, k;
+  int i;
+  ORC_PAR
   for (i = 0; i < A->n; ++i) {
     double acc = 0.0;
+    int k;
     for (k = A->row_ptr[i]; k < A->row_ptr[i + 1]; ++k)
       acc += x[A->col[k]] * A->val[k];
     y[i] = acc;
@@ -377,6 +414,7 @@ static void residual(const orc_csr *A, const double *x, const double *b,
                      double *r, double *tmp) {
   int i;
   orc_csr_mult(A, x, tmp);
+  ORC_PAR
   for (i = 0; i < A->n; ++i) r[i] = b[i] - tmp[i];
 }
 
@@ -412,21 +450,26 @@ static int cg_body(const orc_csr *A, double *x, const double *b, int *max_iter,
     goto done;
   }
   for (i = 1; i <= *max_iter; ++i) {
-    if (precond)
+    if (precond) {
+      ORC_PAR
       for (k = 0; k < n; ++k) z[k] = r[k] * d[k];
-    else
+    } else {
       memcpy(z, r, sizeof(double) * n);
+    }
     rho = orc_dot(n, r, z);
     if (i == 1) {
       memcpy(p, z, sizeof(double) * n);
     } else {
       beta = rho / rho_1;
+      ORC_PAR
       for (k = 0; k < n; ++k) p[k] = z[k] + beta * p[k];
     }
     orc_csr_mult(A, p, q);
     pq = orc_dot(n, p, q);
     alpha = rho / pq;
+    ORC_PAR
     for (k = 0; k < n; ++k) x[k] += alpha * p[k];
+    ORC_PAR
     for (k = 0; k < n; ++k) r[k] -= alpha * q[k];
     resid = nrm(norm_mode, n, r) / normb;
     hist_push(hist, hist_cap, hist_len, resid);
@@ -1071,7 +1114,9 @@ int orc_ns_step(const orc_grid *g, const orc_csr *A, const orc_bc *bcu,
   for (k = 0; k < 2; ++k) {
     double *sc = k == 0 ? s->u : s->v;
     double *ss = k == 0 ? s->us : s->vs;
+    ORC_PAR
     for (j = 0; j < ny; ++j) {
+      int i;
       for (i = 0; i < nx; ++i) {
         double adv = orc_tvd_cell(g, s->uf, s->vf, sc, i, j, prm->limiter);
         double dif = orc_diffusion_cell(g, prm->mu, sc, i, j);

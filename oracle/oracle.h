@@ -211,6 +211,9 @@ int orc_ns_kim_step(const orc_grid *g, const orc_csr *A, const orc_csr *Hu,
 #define ORC_PLANT_KIM_PRED_WALLS 5
 #define ORC_PLANT_KIM_AB2_GHOSTS 6
 void orc_set_plant(int plant);
+/* threads of the CPU baseline (1: the serial reference order) */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 
 #ifdef __cplusplus
 }

@@ -27,13 +27,15 @@ def problem(ctx, nx, ny, types, seed):
 
 
 def warm_start(P, og, net, fused):
-    # the warm start phi = s net(b / s) is one application of the network
+    # the warm start phi = s net(b / s) is one application of the network;
+    # the fused encoder stores the skip channels densely in a0's buffer, the
+    # layer-per-kernel path as the second half of C0
     net.set_fusion(fused)
     P.set_unet(net, ca.UNET_WARM)
     P.fill_phi(0.0)
     P.solve("JPCG", max_iter=0, tol=1e-300, norm=ca.NORM_TRUE)
     phi = og.interior(P.get_phi()).copy()
-    skip = net.tensor("C0")[:, :, 16:].copy()
+    skip = (net.tensor("a0") if fused else net.tensor("C0")[:, :, 16:]).copy()
     p0 = net.tensor("p0").copy()
     d1 = net.tensor("d1").copy()
     P.set_unet(None, 0)

@@ -208,3 +208,24 @@ def test_mixed_precision_reaches_fp64_tolerance():
     assert code == 0 and res <= 1e-10 and outer >= 2
     xd = O.solve("JPCG", A, A.b, tol=1e-13, norm=O.NORM_TRUE)[1]
     assert np.max(np.abs(x - xd)) < 1e-7 * np.max(np.abs(xd))
+
+
+def test_threaded_baseline_agrees_and_one_thread_stays_serial():
+    # The CPU baseline runs the oracle on many threads: row loops and
+    # vector updates split over threads compute every value as before; dots
+    # and norms sum chunks in parallel (another order).  One thread must
+    # reproduce the serial result bit for bit.
+    n = 96
+    g = O.StructGrid(n, n, 1)
+    A = O.Matrix(g, O.make_bc(g, [1, 2, 1, 2]), 1.0)
+    b = np.random.default_rng(4).standard_normal(n * n)
+    ser = O.solve("JPCG", A, b, max_iter=5000, tol=1e-10)
+    try:
+        O.set_threads(4)
+        par = O.solve("JPCG", A, b, max_iter=5000, tol=1e-10)
+    finally:
+        O.set_threads(1)
+    again = O.solve("JPCG", A, b, max_iter=5000, tol=1e-10)
+    assert ser[0] == par[0] == 0 and abs(ser[2] - par[2]) <= 2
+    assert np.linalg.norm(par[1] - ser[1]) <= 1e-8 * np.linalg.norm(ser[1])
+    assert np.array_equal(again[1], ser[1]) and again[2] == ser[2]
