@@ -72,7 +72,10 @@ EXPORTS = [
     "cpgpu_poisson_error_norms", "cpgpu_ns_error_norms", "cpgpu_batch_create",
     "cpgpu_batch_destroy", "cpgpu_batch_set_tiles", "cpgpu_batch_set_graphs",
     "cpgpu_batch_solve", "cpgpu_batch_autotune", "cpgpu_batch_draw",
-    "cpgpu_ns_field_change", "cpgpu_unet_set_conv_shape", "cpgpu_unet_set_fusion", "cpgpu_unet_clear_samples",
+    "cpgpu_ns_field_change", "cpgpu_unet_set_conv_shape", "cpgpu_unet_set_fusion", "cpgpu_advection_create",
+    "cpgpu_advection_destroy", "cpgpu_advection_set_field", "cpgpu_advection_get_field",
+    "cpgpu_advection_set_scheme", "cpgpu_advection_initial", "cpgpu_advection_step",
+    "cpgpu_advection_time", "cpgpu_unet_clear_samples",
     "cpgpu_unet_set_walls", "cpgpu_copy_bandwidth", "cpgpu_csr_create",
     "cpgpu_csr_destroy", "cpgpu_csr_mult", "cpgpu_csr_solve",
     "cpgpu_poisson_set_alpha", "cpgpu_poisson_set_bc_values", "cpgpu_ns_set_kim",
@@ -355,6 +358,17 @@ class NSDesc(C.Structure):
                 ("norm_mode", C.c_int), ("dt", C.c_double)]
 
 
+class AdvectionDesc(C.Structure):
+    _fields_ = [("grid", Grid2D), ("bcu_type", C.c_int * 4), ("bcu_val", _dp),
+                ("bcv_type", C.c_int * 4), ("bcv_val", _dp), ("scheme", C.c_int),
+                ("dt", C.c_double)]
+
+
+# Advection_::set_advection_scheme names (s_advection.hpp:140-182)
+ADV_SCHEMES = {"VanLeer": 0, "superbee": 1, "WAHYD": 2, "QUICK": 3, "CUI": 4, "upwind2": 5,
+               "upwind1": 10, "center": 11, "center4": 12}
+
+
 def pack_bc_values(nx, ny, gl, values):
     """None, 4 per-segment scalars, or an already packed array."""
     NX, NY = nx + 2 * gl, ny + 2 * gl
@@ -581,6 +595,69 @@ def random_problems(ctx, seed=2, n=256, lo=6, hi=10, nblobs=64, wmin=0.01, wmax=
 
 EZERODIAG = -10
 
+
+
+class Advection:
+    """Device-resident Advection_ scalar driver (cpgpu_advection): phi's
+    boundary (types, values), the centre velocities' boundaries, the scheme
+    name and the Time_ dt."""
+
+    def __init__(self, ctx, nx, ny, gl=2, xlim=(0.0, 1.0), ylim=(0.0, 1.0),
+                 bc_phi=((1, 1, 1, 1), None), bcu=((2, 2, 2, 2), None),
+                 bcv=((2, 2, 2, 2), None), scheme="upwind1", dt=1e-3):
+        self.ctx, self.nx, self.ny, self.gl = ctx, nx, ny, gl
+        types, vals = bc_phi
+        pv = pack_bc_values(nx, ny, gl, vals)
+        g = uniform_grid_desc(nx, ny, gl, list(types), pv, xlim=xlim, ylim=ylim)
+        d = AdvectionDesc()
+        d.grid = g
+        self._keep = [g, g._keep, pv]
+        for name, (t_, v_) in (("bcu", bcu), ("bcv", bcv)):
+            arr = pack_bc_values(nx, ny, gl, v_)
+            self._keep.append(arr)
+            t = getattr(d, name + "_type")
+            for k in range(4):
+                t[k] = int(t_[k])
+            setattr(d, name + "_val", _p(arr))
+        d.scheme = ADV_SCHEMES[scheme]
+        d.dt = dt
+        self.desc = d
+        self.h = C.c_void_p()
+        ctx.check(lib().cpgpu_advection_create(ctx.h, C.byref(d), C.byref(self.h)))
+        ctx.adopt(self)
+
+    @property
+    def shape(self):
+        return (self.nx + 2 * self.gl, self.ny + 2 * self.gl)
+
+    def set_field(self, name, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.shape == self.shape
+        self.ctx.check(lib().cpgpu_advection_set_field(self.h, FIELDS[name], _p(f)))
+
+    def get_field(self, name):
+        out = np.zeros(self.shape)
+        self.ctx.check(lib().cpgpu_advection_get_field(self.h, FIELDS[name], _p(out)))
+        return out
+
+    def set_scheme(self, name):
+        self.ctx.check(lib().cpgpu_advection_set_scheme(self.h, ADV_SCHEMES[name]))
+
+    def initial(self):
+        self.ctx.check(lib().cpgpu_advection_initial(self.h))
+
+    def step(self, n=1):
+        self.ctx.check(lib().cpgpu_advection_step(self.h, C.c_int(n)))
+
+    def time(self):
+        t, k = C.c_double(0), C.c_longlong(0)
+        self.ctx.check(lib().cpgpu_advection_time(self.h, C.byref(t), C.byref(k)))
+        return t.value, k.value
+
+    def close(self):
+        if self.h:
+            lib().cpgpu_advection_destroy(self.h)
+            self.h = C.c_void_p()
 
 class CSR:
     """A general CSR matrix on the device (cpgpu_csr): the algebra solvers'

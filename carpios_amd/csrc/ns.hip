@@ -196,13 +196,28 @@ __global__ __launch_bounds__(256) void k_ghost_corners(Axes a, double* f0, BCDev
   }
 }
 
+// A scalar field read at cell (i, j): a ghost-padded device field, or a
+// tile of it staged in LDS (the in-place advection sweep).
+struct FieldAt {
+  const double* __restrict__ p;
+  Axes a;
+  __device__ __forceinline__ double operator()(int i, int j) const { return p[fidx(a, i, j)]; }
+};
+struct TileAt {
+  const double* t;  // LDS window of pitch TP: cell (i0 - H + r, j0 - H + c) at t[r * TP + c]
+  int i0, j0;
+  static constexpr int H = 2, TP = 20;
+  __device__ __forceinline__ double operator()(int i, int j) const {
+    return t[(i - i0 + H) * TP + (j - j0 + H)];
+  }
+};
+
 // VdotNabla_TVD for one cell and component (s_operation.hpp:549-596)
-template <int LIM>
+template <int LIM, class PHI>
 __device__ __forceinline__ double tvd_cell(const Axes& a,
                                            const double* __restrict__ uf,
                                            const double* __restrict__ vf,
-                                           const double* __restrict__ phi,
-                                           int i, int j) {
+                                           const PHI& phi, int i, int j) {
   double vdd[2];
 #pragma unroll
   for (int d = 0; d < 2; ++d) {
@@ -220,7 +235,7 @@ __device__ __forceinline__ double tvd_cell(const Axes& a,
       Dm = -1; Cm = 0; Um = 1;
       Dp = 0; Cp = 1; Up = 2;
     }
-#define VAL(o) phi[fidx(a, i + ei * (o), j + ej * (o))]
+#define VAL(o) phi(i + ei * (o), j + ej * (o))
 #define SZ(o) sax(a, d, c + (o))
     double sC = SZ(Cp), sD = SZ(Dp), sU = SZ(Up);
     double vC = VAL(Cp), vD = VAL(Dp), vU = VAL(Up);
@@ -279,13 +294,13 @@ __global__ __launch_bounds__(256) void k_veo_star(Axes a, const double* __restri
   if (i >= a.nx || j >= a.ny) return;
   const size_t f = fidx(a, i, j);
   {
-    const double adv = tvd_cell<LIM>(a, uf, vf, u, i, j);
+    const double adv = tvd_cell<LIM>(a, uf, vf, FieldAt{u, a}, i, j);
     const double dif = diffusion_cell(a, mu, u, i, j);
     const double src = 0.0;
     us[f] = (-adv + (dif + src) / rho) * dt + u[f];
   }
   {
-    const double adv = tvd_cell<LIM>(a, uf, vf, v, i, j);
+    const double adv = tvd_cell<LIM>(a, uf, vf, FieldAt{v, a}, i, j);
     const double dif = diffusion_cell(a, mu, v, i, j);
     const double src = 0.0;
     vs[f] = (-adv + (dif + src) / rho) * dt + v[f];
@@ -405,20 +420,21 @@ __global__ __launch_bounds__(256) void k_grad_correct(Axes a, const double* __re
 // VdotNabla_upwind1 at one cell (s_operation.hpp:400-422), the NS_ default
 // advection (s_ns.hpp:225); faces as in k_c2f_div (uf at (i, j) is the
 // x+ face of cell (i, j)).
+template <class PHI>
 __device__ __forceinline__ double upwind1_cell(const Axes& a, const double* __restrict__ uf,
                                                const double* __restrict__ vf,
-                                               const double* __restrict__ s, int i, int j) {
+                                               const PHI& s, int i, int j) {
   double t[2] = {0.0, 0.0};
-  const double c = s[fidx(a, i, j)];
+  const double c = s(i, j);
 #pragma unroll
   for (int d = 0; d < 2; ++d) {
     const int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1, k = d == 0 ? i : j;
     const double* f = d == 0 ? uf : vf;
     const double veo = (f[fidx(a, i, j)] + f[fidx(a, i - ei, j - ej)]) * 0.5;
     if (veo > 0.0)
-      t[d] = veo * (c - s[fidx(a, i - ei, j - ej)]) / (cax(a, d, k) - cax(a, d, k - 1));
+      t[d] = veo * (c - s(i - ei, j - ej)) / (cax(a, d, k) - cax(a, d, k - 1));
     else if (veo < 0.0)
-      t[d] = veo * (s[fidx(a, i + ei, j + ej)] - c) / (cax(a, d, k + 1) - cax(a, d, k));
+      t[d] = veo * (s(i + ei, j + ej) - c) / (cax(a, d, k + 1) - cax(a, d, k));
   }
   double r = 0.0;
   r += t[0];
@@ -450,8 +466,8 @@ __global__ __launch_bounds__(256) void k_kim_src(Axes a, const double* __restric
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const double* s = c == 0 ? u : v;
-    const double advn = upwind1_cell(a, uf, vf, s, i, j);
-    const double advnm1 = upwind1_cell(a, uf, vf, c == 0 ? um : vm, i, j);
+    const double advn = upwind1_cell(a, uf, vf, FieldAt{s, a}, i, j);
+    const double advnm1 = upwind1_cell(a, uf, vf, FieldAt{c == 0 ? um : vm, a}, i, j);
     const double adv = 1.5 * advn - 0.5 * advnm1;
     const double dif = diffusion_cell(a, mu, s, i, j);
     const double src = 0.0;
@@ -487,6 +503,130 @@ __global__ __launch_bounds__(256) void k_x_to_field2(Axes a, const double* __res
 
 __global__ void k_advance(NSDev* st) {
   if (threadIdx.x == 0) st->t += st->dt;
+}
+
+// ---------------------------------------------------------------------
+// Advection_ (s_advection.hpp:20-217): explicit Euler steps of
+// d phi/dt + v . grad phi = 0 with a fixed face velocity.
+
+// VdotNabla_center (s_operation.hpp:442-459)
+template <class PHI>
+__device__ __forceinline__ double center_cell(const Axes& a, const double* __restrict__ uf,
+                                              const double* __restrict__ vf, const PHI& s,
+                                              int i, int j) {
+  double t[2];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1, k = d == 0 ? i : j;
+    const double* f = d == 0 ? uf : vf;
+    const double veo = (f[fidx(a, i, j)] + f[fidx(a, i - ei, j - ej)]) * 0.5;
+    t[d] = veo * (s(i + ei, j + ej) - s(i - ei, j - ej)) / (cax(a, d, k + 1) - cax(a, d, k - 1));
+  }
+  double r = 0.0;
+  r += t[0];
+  r += t[1];
+  return r;
+}
+
+// VdotNabla_center4 (s_operation.hpp:461-480).  The reference forms the
+// averaged velocity and leaves it out of the result: the term is the
+// fourth-order gradient alone, reproduced as such.
+template <class PHI>
+__device__ __forceinline__ double center4_cell(const Axes& a, const PHI& s, int i, int j) {
+  double t[2];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1, k = d == 0 ? i : j;
+    const double inn = (s(i + ei, j + ej) - s(i - ei, j - ej)) /
+                       (cax(a, d, k + 1) - cax(a, d, k - 1));
+    const double out = (s(i + 2 * ei, j + 2 * ej) - s(i - 2 * ei, j - 2 * ej)) /
+                       (cax(a, d, k + 2) - cax(a, d, k - 2));
+    t[d] = (4.0 * inn - out) / 3.0;
+  }
+  double r = 0.0;
+  r += t[0];
+  r += t[1];
+  return r;
+}
+
+// the advection term of scheme SCH (CPGPU_ADV_*: the limiter ids select the
+// TVD family)
+template <int SCH, class PHI>
+__device__ __forceinline__ double adv_cell(const Axes& a, const double* __restrict__ uf,
+                                           const double* __restrict__ vf, const PHI& s, int i,
+                                           int j) {
+  if constexpr (SCH == CPGPU_ADV_UPWIND1) return upwind1_cell(a, uf, vf, s, i, j);
+  else if constexpr (SCH == CPGPU_ADV_CENTER) return center_cell(a, uf, vf, s, i, j);
+  else if constexpr (SCH == CPGPU_ADV_CENTER4) return center4_cell(a, s, i, j);
+  else return tvd_cell<SCH>(a, uf, vf, s, i, j);
+}
+
+// One tile diagonal of the in-place sweep of Advection_::run_one_step
+// (s_advection.hpp:200-210).  The reference visits cells i-fastest and
+// overwrites phi as it goes, so a cell sees the new values of the cells
+// before it in that order and the old values of those after it.  Its
+// stencil is a cross of radius two, whose cells before it are (i - 1, j),
+// (i - 2, j), (i, j - 1), (i, j - 2): along anti-diagonals i + j those lie on
+// the two previous diagonals and the cells after it on the next two.
+// Sweeping anti-diagonals in order therefore reproduces the reference's
+// values exactly; at the tile level, tiles (ti, tj) of one diagonal
+// ti + tj = D are independent (no cross stencil spans two of them), and the
+// tiles of diagonals D - 1 (updated) and D + 1 (untouched) are what the
+// cross reads.  One wavefront per 16 x 16 tile stages its cells and a
+// two-cell halo in LDS, then walks the tile's 31 local anti-diagonals.
+template <int SCH>
+__global__ __launch_bounds__(64) void k_adv_sweep(Axes a, const double* __restrict__ uf,
+                                                  const double* __restrict__ vf,
+                                                  double* __restrict__ phi, double dt, int D,
+                                                  int nti, int ntj) {
+  constexpr int T = 16, H = TileAt::H, TP = TileAt::TP;
+  __shared__ double w[TP * TP];
+  const int ti = max(0, D - (ntj - 1)) + (int)blockIdx.x;
+  const int tj = D - ti;
+  if (ti >= nti || tj < 0) return;
+  const int i0 = ti * T, j0 = tj * T;
+  // stage the window (cells outside the field's ghost frame are never read)
+  for (int e = threadIdx.x; e < TP * TP; e += 64) {
+    const int r = e / TP, c = e - r * TP;
+    const int i = i0 - H + r, j = j0 - H + c;
+    double v = 0.0;
+    if (i >= -a.gl && i < a.nx + a.gl && j >= -a.gl && j < a.ny + a.gl) v = phi[fidx(a, i, j)];
+    w[e] = v;
+  }
+  __syncthreads();
+  const TileAt at{w, i0, j0};
+  for (int d = 0; d < 2 * T - 1; ++d) {
+    const int li = threadIdx.x, lj = d - li;
+    const int i = i0 + li, j = j0 + lj;
+    if (li < T && lj >= 0 && lj < T && i < a.nx && j < a.ny) {
+      const double adv = adv_cell<SCH>(a, uf, vf, at, i, j);
+      const double phin = at(i, j);
+      w[(li + H) * TP + (lj + H)] = phin - dt * adv;
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < T * T; e += 64) {
+    const int li = e / T, lj = e - li * T;
+    const int i = i0 + li, j = j0 + lj;
+    if (i < a.nx && j < a.ny) phi[fidx(a, i, j)] = w[(li + H) * TP + (lj + H)];
+  }
+}
+
+// face velocities from the centre velocities (InterpolateC2F,
+// s_operation.hpp:100-120): every face of the field, the m-face of the first
+// cell included
+__global__ __launch_bounds__(256) void k_adv_faces(Axes a, const double* __restrict__ u,
+                                                   const double* __restrict__ v,
+                                                   double* __restrict__ uf,
+                                                   double* __restrict__ vf) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= a.nx || j >= a.ny) return;
+  const double uc = u[fidx(a, i, j)], vc = v[fidx(a, i, j)];
+  uf[fidx(a, i, j)] = c2f(a, 0, i, uc, u[fidx(a, i + 1, j)]);
+  vf[fidx(a, i, j)] = c2f(a, 1, j, vc, v[fidx(a, i, j + 1)]);
+  if (i == 0) uf[fidx(a, -1, j)] = c2f(a, 0, -1, u[fidx(a, -1, j)], uc);
+  if (j == 0) vf[fidx(a, i, -1)] = c2f(a, 1, -1, v[fidx(a, i, -1)], vc);
 }
 
 }  // namespace cpgpu
@@ -984,6 +1124,202 @@ int cpgpu_ns_set_solver(cpgpu_ns* S, int solver, int max_iter, double tol,
   S->tol = tol;
   S->norm_mode = norm_mode;
   S->opts = ns_solver_opts(solver);
+  return CPGPU_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// Advection_ driver: fields u, v, uf, vf, phi resident on the device.
+struct cpgpu_advection {
+  cpgpu_ctx* ctx = nullptr;
+  int nx = 0, ny = 0, gl = 0, NX = 0, NY = 0;
+  size_t NT = 0;
+  DevBuf<double> cx, sx, cy, sy;
+  DevBuf<double> bc_val[3];  // phi, u, v: packed per ghost line
+  int bc_type[3][4];
+  DevBuf<double> F[5];       // u, v, uf, vf, phi
+  int scheme = CPGPU_ADV_UPWIND1;
+  double dt = 0.0, t = 0.0;
+  long long steps = 0;
+  Axes axes() const {
+    Axes a;
+    a.nx = nx; a.ny = ny; a.gl = gl; a.NX = NX; a.NY = NY;
+    a.cx = cx.p; a.sx = sx.p; a.cy = cy.p; a.sy = sy.p;
+    return a;
+  }
+  BCDev bcdev(int k) const {
+    return BCDev{{bc_type[k][0], bc_type[k][1], bc_type[k][2], bc_type[k][3]}, bc_val[k].p};
+  }
+};
+
+namespace {
+
+bool adv_scheme_ok(int s) {
+  return (s >= CPGPU_LIM_VANLEER && s <= CPGPU_LIM_UPWIND2) || s == CPGPU_ADV_UPWIND1 ||
+         s == CPGPU_ADV_CENTER || s == CPGPU_ADV_CENTER4;
+}
+
+int adv_slot(int which) {
+  switch (which) {
+    case CPGPU_F_U: return 0;
+    case CPGPU_F_V: return 1;
+    case CPGPU_F_UF: return 2;
+    case CPGPU_F_VF: return 3;
+    case CPGPU_F_PHI: return 4;
+    default: return -1;
+  }
+}
+
+template <int SCH>
+void adv_sweep(cpgpu_advection* A, hipStream_t st, int nti, int ntj) {
+  const Axes a = A->axes();
+  for (int D = 0; D < nti + ntj - 1; ++D) {
+    const int lo = std::max(0, D - (ntj - 1)), hi = std::min(D, nti - 1);
+    hipLaunchKernelGGL((k_adv_sweep<SCH>), dim3(hi - lo + 1), dim3(64), 0, st, a, A->F[2].p,
+                       A->F[3].p, A->F[4].p, A->dt, D, nti, ntj);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpgpu_advection_create(cpgpu_ctx* ctx, const cpgpu_advection_desc* d,
+                           cpgpu_advection** out) {
+  if (!ctx || !d || !out) return CPGPU_EINVAL;
+  *out = nullptr;
+  const cpgpu_grid2d& g = d->grid;
+  if (g.nx < 1 || g.ny < 1 || g.gl < 2 || !g.cx || !g.sx || !g.cy || !g.sy)
+    return set_error(ctx, CPGPU_EINVAL, "Advection needs gl >= 2 and axis arrays");
+  if (!adv_scheme_ok(d->scheme) || !(d->dt > 0.0))
+    return set_error(ctx, CPGPU_EINVAL, "bad advection scheme or dt");
+  const int* types[3] = {g.bc_type, d->bcu_type, d->bcv_type};
+  for (int k = 0; k < 3; ++k)
+    for (int s = 0; s < 4; ++s)
+      if (types[k][s] != CPGPU_BC_DIRICHLET && types[k][s] != CPGPU_BC_NEUMANN)
+        return set_error(ctx, CPGPU_EINVAL, "bad advection boundary type");
+  CPGPU_HIP(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<cpgpu_advection> A(new cpgpu_advection());
+  A->ctx = ctx;
+  A->nx = g.nx; A->ny = g.ny; A->gl = g.gl;
+  A->NX = g.nx + 2 * g.gl; A->NY = g.ny + 2 * g.gl;
+  A->NT = (size_t)A->NX * A->NY;
+  const size_t nbc = 2 * (size_t)A->NY + 2 * (size_t)A->NX;
+  for (int k = 0; k < 5; ++k) {
+    if (A->F[k].alloc(A->NT)) return set_error(ctx, CPGPU_ENOMEM, "advection fields");
+    CPGPU_HIP(ctx, hipMemsetAsync(A->F[k].p, 0, sizeof(double) * A->NT, ctx->stream));
+  }
+  if (A->cx.alloc(A->NX) || A->sx.alloc(A->NX) || A->cy.alloc(A->NY) || A->sy.alloc(A->NY) ||
+      A->bc_val[0].alloc(nbc) || A->bc_val[1].alloc(nbc) || A->bc_val[2].alloc(nbc))
+    return set_error(ctx, CPGPU_ENOMEM, "advection buffers");
+  std::vector<double> zero(nbc, 0.0);
+  const double* vals[3] = {g.bc_val, d->bcu_val, d->bcv_val};
+  int rc;
+  if ((rc = ns_upload(ctx, A->cx.p, g.cx, sizeof(double) * A->NX)) ||
+      (rc = ns_upload(ctx, A->sx.p, g.sx, sizeof(double) * A->NX)) ||
+      (rc = ns_upload(ctx, A->cy.p, g.cy, sizeof(double) * A->NY)) ||
+      (rc = ns_upload(ctx, A->sy.p, g.sy, sizeof(double) * A->NY)))
+    return rc;
+  for (int k = 0; k < 3; ++k) {
+    if ((rc = ns_upload(ctx, A->bc_val[k].p, vals[k] ? vals[k] : zero.data(),
+                        sizeof(double) * nbc)))
+      return rc;
+    for (int s = 0; s < 4; ++s) A->bc_type[k][s] = types[k][s];
+  }
+  A->scheme = d->scheme;
+  A->dt = d->dt;
+  CPGPU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *out = A.release();
+  return CPGPU_OK;
+}
+
+int cpgpu_advection_destroy(cpgpu_advection* A) {
+  if (!A) return CPGPU_EINVAL;
+  (void)hipStreamSynchronize(A->ctx->stream);
+  delete A;
+  return CPGPU_OK;
+}
+
+int cpgpu_advection_set_field(cpgpu_advection* A, int which, const double* field) {
+  const int k = adv_slot(which);
+  if (!A || !field || k < 0) return CPGPU_EINVAL;
+  CPGPU_HIP(A->ctx, hipSetDevice(A->ctx->device));
+  return ns_upload(A->ctx, A->F[k].p, field, sizeof(double) * A->NT);
+}
+
+int cpgpu_advection_get_field(cpgpu_advection* A, int which, double* field) {
+  const int k = adv_slot(which);
+  if (!A || !field || k < 0) return CPGPU_EINVAL;
+  CPGPU_HIP(A->ctx, hipSetDevice(A->ctx->device));
+  CPGPU_HIP(A->ctx, hipMemcpyAsync(field, A->F[k].p, sizeof(double) * A->NT,
+                                   hipMemcpyDeviceToHost, A->ctx->stream));
+  CPGPU_HIP(A->ctx, hipStreamSynchronize(A->ctx->stream));
+  return CPGPU_OK;
+}
+
+int cpgpu_advection_set_scheme(cpgpu_advection* A, int scheme) {
+  if (!A || !adv_scheme_ok(scheme)) return CPGPU_EINVAL;
+  A->scheme = scheme;
+  return CPGPU_OK;
+}
+
+int cpgpu_advection_initial(cpgpu_advection* A) {
+  if (!A) return CPGPU_EINVAL;
+  CPGPU_HIP(A->ctx, hipSetDevice(A->ctx->device));
+  hipStream_t st = A->ctx->stream;
+  const Axes a = A->axes();
+  hipLaunchKernelGGL(k_ghost_edges, dim3(64, 2), dim3(256), 0, st, a, A->F[0].p, A->bcdev(1),
+                     A->F[1].p, A->bcdev(2));
+  hipLaunchKernelGGL(k_ghost_corners, dim3(1, 2), dim3(256), 0, st, a, A->F[0].p, A->bcdev(1),
+                     A->F[1].p, A->bcdev(2));
+  const dim3 b2(64, 4), g2((A->ny + 63) / 64, (A->nx + 3) / 4);
+  hipLaunchKernelGGL(k_adv_faces, g2, b2, 0, st, a, A->F[0].p, A->F[1].p, A->F[2].p, A->F[3].p);
+  CPGPU_HIP(A->ctx, hipGetLastError());
+  CPGPU_HIP(A->ctx, hipStreamSynchronize(st));
+  return CPGPU_OK;
+}
+
+int cpgpu_advection_step(cpgpu_advection* A, int nsteps) {
+  if (!A || nsteps < 0) return CPGPU_EINVAL;
+  CPGPU_HIP(A->ctx, hipSetDevice(A->ctx->device));
+  hipStream_t st = A->ctx->stream;
+  const Axes a = A->axes();
+  const int nti = (A->nx + 15) / 16, ntj = (A->ny + 15) / 16;
+  for (int s = 0; s < nsteps; ++s) {
+    // run_one_step: apply_bc of u, v (their faces are kept) and phi, then
+    // the in-place sweep
+    hipLaunchKernelGGL(k_ghost_edges, dim3(64, 2), dim3(256), 0, st, a, A->F[0].p,
+                       A->bcdev(1), A->F[1].p, A->bcdev(2));
+    hipLaunchKernelGGL(k_ghost_corners, dim3(1, 2), dim3(256), 0, st, a, A->F[0].p,
+                       A->bcdev(1), A->F[1].p, A->bcdev(2));
+    hipLaunchKernelGGL(k_ghost_edges, dim3(64, 1), dim3(256), 0, st, a, A->F[4].p,
+                       A->bcdev(0), A->F[4].p, A->bcdev(0));
+    hipLaunchKernelGGL(k_ghost_corners, dim3(1, 1), dim3(256), 0, st, a, A->F[4].p,
+                       A->bcdev(0), A->F[4].p, A->bcdev(0));
+    switch (A->scheme) {
+      case CPGPU_LIM_VANLEER: adv_sweep<CPGPU_LIM_VANLEER>(A, st, nti, ntj); break;
+      case CPGPU_LIM_SUPERBEE: adv_sweep<CPGPU_LIM_SUPERBEE>(A, st, nti, ntj); break;
+      case CPGPU_LIM_WAHYD: adv_sweep<CPGPU_LIM_WAHYD>(A, st, nti, ntj); break;
+      case CPGPU_LIM_QUICK: adv_sweep<CPGPU_LIM_QUICK>(A, st, nti, ntj); break;
+      case CPGPU_LIM_CUI: adv_sweep<CPGPU_LIM_CUI>(A, st, nti, ntj); break;
+      case CPGPU_LIM_UPWIND2: adv_sweep<CPGPU_LIM_UPWIND2>(A, st, nti, ntj); break;
+      case CPGPU_ADV_UPWIND1: adv_sweep<CPGPU_ADV_UPWIND1>(A, st, nti, ntj); break;
+      case CPGPU_ADV_CENTER: adv_sweep<CPGPU_ADV_CENTER>(A, st, nti, ntj); break;
+      default: adv_sweep<CPGPU_ADV_CENTER4>(A, st, nti, ntj); break;
+    }
+    CPGPU_HIP(A->ctx, hipGetLastError());
+    A->t += A->dt;
+    ++A->steps;
+  }
+  CPGPU_HIP(A->ctx, hipStreamSynchronize(st));
+  return CPGPU_OK;
+}
+
+int cpgpu_advection_time(cpgpu_advection* A, double* t, long long* steps) {
+  if (!A || !t || !steps) return CPGPU_EINVAL;
+  *t = A->t;
+  *steps = A->steps;
   return CPGPU_OK;
 }
 

@@ -376,6 +376,51 @@ int cpgpu_ns_set_time(cpgpu_ns *S, double t, double dt);
 int cpgpu_ns_set_solver(cpgpu_ns *S, int solver, int max_iter, double tol,
                         int norm_mode);
 
+/* ---- Advection_ scalar driver (device resident) ----
+ * Replaces structure::Advection_<2> (lib/structure/s_advection.hpp:20-217):
+ * explicit Euler steps of phi_t + v . grad phi = 0 with face velocities
+ * fixed at initial().  run_one_step (:185-212) overwrites phi in place in
+ * Grid_::begin_ijk order (i fastest), so a cell reads the new values of
+ * the cells before it; the device reproduces that order exactly by
+ * sweeping anti-diagonals of 16 x 16 tiles. */
+typedef struct cpgpu_advection cpgpu_advection;
+
+/* schemes of Advection_::set_advection_scheme (s_advection.hpp:140-182):
+ * the TVD family by limiter id (CPGPU_LIM_VANLEER .. CPGPU_LIM_UPWIND2:
+ * "VanLeer", "superbee", "WAHYD", "QUICK", "CUI", "upwind2") and */
+#define CPGPU_ADV_UPWIND1 10 /* "upwind1", VdotNabla_upwind1, the default */
+#define CPGPU_ADV_CENTER 11  /* "center", VdotNabla_center */
+#define CPGPU_ADV_CENTER4 12 /* "center4", VdotNabla_center4 (the reference
+                                leaves the velocity out of this term) */
+
+typedef struct {
+  cpgpu_grid2d grid;       /* axes (gl >= 2); bc_type / bc_val: phi's boundary
+                              (add_bc of "phi"); beta and source unused */
+  int bcu_type[4];         /* centre velocity boundaries ("u", "v") */
+  const double *bcu_val;   /* packed per ghost line, as cpgpu_grid2d.bc_val */
+  int bcv_type[4];
+  const double *bcv_val;
+  int scheme;              /* CPGPU_ADV_* or a CPGPU_LIM_* id */
+  double dt;               /* Time_ dt (set_time) */
+} cpgpu_advection_desc;
+
+int cpgpu_advection_create(cpgpu_ctx *ctx, const cpgpu_advection_desc *d,
+                           cpgpu_advection **out);
+int cpgpu_advection_destroy(cpgpu_advection *A);
+/* Host ghost-padded fields in/out: CPGPU_F_U, CPGPU_F_V (centre velocities,
+ * initial_CS), CPGPU_F_UF, CPGPU_F_VF (face velocities), CPGPU_F_PHI. */
+int cpgpu_advection_set_field(cpgpu_advection *A, int which, const double *field);
+int cpgpu_advection_get_field(cpgpu_advection *A, int which, double *field);
+int cpgpu_advection_set_scheme(cpgpu_advection *A, int scheme);
+/* Advection_::initial (s_advection.hpp:103-135): the centre velocities'
+ * ghosts from their boundaries, then the faces by InterpolateC2F. */
+int cpgpu_advection_initial(cpgpu_advection *A);
+/* nsteps of run_one_step: apply_bc of u, v and phi, then the in-place sweep
+ * phi <- phi - dt * VdotNabla(v, phi). */
+int cpgpu_advection_step(cpgpu_advection *A, int nsteps);
+/* time t = steps * dt and the step count */
+int cpgpu_advection_time(cpgpu_advection *A, double *t, long long *steps);
+
 /* ---- learned preconditioner: a BF16 U-Net on MFMA ----
  * No counterpart in the reference: its Poisson solvers are the CSR Krylov
  * methods of algebra/solver_matrix.hpp (Jacobi, CG, BiCGSTAB, IC-CG ...).

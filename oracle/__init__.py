@@ -70,6 +70,11 @@ def lib():
         L.orc_diffusion_cell.restype = C.c_double
         L.orc_upwind1_cell.restype = C.c_double
         L.orc_max_magnitude.restype = C.c_double
+        L.orc_center_cell.restype = C.c_double
+        L.orc_center4_cell.restype = C.c_double
+        L.orc_adv_cell.restype = C.c_double
+        L.orc_advection_step.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_void_p, C.c_int,
+                                         C.c_double]
         _lib = L
     return _lib
 
@@ -360,7 +365,8 @@ class NSKim(NS):
 
 
 PLANT = {"none": 0, "adv_sign": 1, "dif_rho": 2, "face_upwind": 3,
-         "neumann_sign": 4, "kim_pred_walls": 5, "kim_ab2_ghosts": 6}
+         "neumann_sign": 4, "kim_pred_walls": 5, "kim_ab2_ghosts": 6,
+         "adv_two_buffer": 7}
 _plant = 0
 
 
@@ -376,6 +382,45 @@ def set_threads(n):
     """Threads of the CPU baseline; 1 (the default) keeps every routine in
     the reference's serial order, as the parity checks need."""
     lib().orc_set_threads(C.c_int(int(n)))
+
+
+# Advection_::set_advection_scheme names (s_advection.hpp:140-182): the TVD
+# family by limiter id, and the three plain schemes
+ADV = dict(LIM, upwind1=10, center=11, center4=12)
+
+
+def adv_field(grid, uf, vf, phi, scheme):
+    """The advection term of every cell, evaluated on phi as it is."""
+    out = np.zeros((grid.nx, grid.ny))
+    L = lib()
+    for i in range(grid.nx):
+        for j in range(grid.ny):
+            out[i, j] = L.orc_adv_cell(C.byref(grid.c), _ptr(uf), _ptr(vf), _ptr(phi), i, j,
+                                       ADV[scheme])
+    return out
+
+
+class Advection:
+    """Advection_ state driven by the oracle (s_advection.hpp:20-217): centre
+    and face velocities, phi and its boundary, the scheme and dt."""
+
+    def __init__(self, grid, bc_phi, scheme="upwind1", dt=1e-3):
+        self.g, self.bc, self.dt = grid, bc_phi, dt
+        self.scheme = scheme
+        self.f = {n: grid.field() for n in ("u", "v", "uf", "vf", "phi")}
+
+    def initial(self, bcu, bcv):
+        """Advection_::initial (s_advection.hpp:103-135): the centre
+        velocities' ghosts, then the faces by InterpolateC2F."""
+        apply_bc(self.g, self.f["u"], bcu)
+        apply_bc(self.g, self.f["v"], bcv)
+        c2f(self.g, self.f["u"], self.f["v"], self.f["uf"], self.f["vf"])
+
+    def step(self, n=1):
+        for _ in range(n):
+            lib().orc_advection_step(C.byref(self.g.c), _ptr(self.f["uf"]), _ptr(self.f["vf"]),
+                                     _ptr(self.f["phi"]), C.byref(self.bc), ADV[self.scheme],
+                                     self.dt)
 
 
 def upwind1_field(grid, uf, vf, phi):

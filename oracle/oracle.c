@@ -863,6 +863,79 @@ double orc_tvd_cell(const orc_grid *g, const double *uf, const double *vf,
   return res;
 }
 
+/* VdotNabla_center (s_operation.hpp:442-459) */
+double orc_center_cell(const orc_grid *g, const double *uf, const double *vf,
+                       const double *phi, int i, int j) {
+  double t[2], res = 0.0;
+  int d;
+  for (d = 0; d < 2; ++d) {
+    const double *f = d == 0 ? uf : vf;
+    int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1, k = d == 0 ? i : j;
+    double veo = (f[FIDX(g, i, j)] + f[FIDX(g, i - ei, j - ej)]) * 0.5;
+    t[d] = veo * (phi[FIDX(g, i + ei, j + ej)] - phi[FIDX(g, i - ei, j - ej)]) /
+           (cax(g, d, k + 1) - cax(g, d, k - 1));
+  }
+  for (d = 0; d < 2; ++d) res += t[d];
+  return res;
+}
+
+/* VdotNabla_center4 (s_operation.hpp:461-480): the averaged velocity is
+ * formed and not applied, so the term is the fourth-order gradient alone */
+double orc_center4_cell(const orc_grid *g, const double *phi, int i, int j) {
+  double t[2], res = 0.0;
+  int d;
+  for (d = 0; d < 2; ++d) {
+    int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1, k = d == 0 ? i : j;
+    double inn = (phi[FIDX(g, i + ei, j + ej)] - phi[FIDX(g, i - ei, j - ej)]) /
+                 (cax(g, d, k + 1) - cax(g, d, k - 1));
+    double out = (phi[FIDX(g, i + 2 * ei, j + 2 * ej)] -
+                  phi[FIDX(g, i - 2 * ei, j - 2 * ej)]) /
+                 (cax(g, d, k + 2) - cax(g, d, k - 2));
+    t[d] = (4.0 * inn - out) / 3.0;
+  }
+  for (d = 0; d < 2; ++d) res += t[d];
+  return res;
+}
+
+/* the advection term of Advection_'s scheme (set_advection_scheme,
+ * s_advection.hpp:140-182): ORC_ADV_UPWIND1 / CENTER / CENTER4, or a TVD
+ * limiter id */
+double orc_adv_cell(const orc_grid *g, const double *uf, const double *vf,
+                    const double *phi, int i, int j, int scheme) {
+  if (scheme == ORC_ADV_UPWIND1) return orc_upwind1_cell(g, uf, vf, phi, i, j);
+  if (scheme == ORC_ADV_CENTER) return orc_center_cell(g, uf, vf, phi, i, j);
+  if (scheme == ORC_ADV_CENTER4) return orc_center4_cell(g, phi, i, j);
+  return orc_tvd_cell(g, uf, vf, phi, i, j, scheme);
+}
+
+/* Advection_::run_one_step (s_advection.hpp:185-212): phi's ghosts from its
+ * boundary, then every cell in Grid_::begin_ijk order (i fastest,
+ * Ijk_::_incr, s_define.hpp:394-410) overwritten in place by
+ * phi - dt * adv, so later cells read the new values of earlier ones.  The
+ * centre velocities' apply_bc of the step leaves the faces unchanged. */
+void orc_advection_step(const orc_grid *g, const double *uf, const double *vf,
+                        double *phi, const orc_bc *bc, int scheme, double dt) {
+  int i, j;
+  const double *src = phi;
+  double *old = NULL;
+  orc_apply_bc(g, phi, bc);
+  if (g_plant == ORC_PLANT_ADV_TWO_BUFFER) {
+    /* planted: every cell reads the values from before the sweep */
+    size_t nt = (size_t)(g->nx + 2 * g->gl) * (g->ny + 2 * g->gl);
+    old = (double *)malloc(sizeof(double) * nt);
+    memcpy(old, phi, sizeof(double) * nt);
+    src = old;
+  }
+  for (j = 0; j < g->ny; ++j) {
+    for (i = 0; i < g->nx; ++i) {
+      double adv = orc_adv_cell(g, uf, vf, src, i, j, scheme);
+      double phin = phi[FIDX(g, i, j)];
+      phi[FIDX(g, i, j)] = phin - dt * adv;
+    }
+  }
+  free(old);
+}
+
 /* NablaMuNabla with uniform mu (s_operation.hpp:664-688) */
 double orc_diffusion_cell(const orc_grid *g, double mu, const double *phi,
                           int i, int j) {

@@ -187,6 +187,17 @@ typedef struct {
   int max_iter; /* Solver_ defaults: 1000 and 1e-4 (s_solver.hpp:73-74) */
   double tol;
 } orc_kim_params;
+/* Advection_ (s_advection.hpp:20-217): schemes beyond the TVD limiter ids */
+#define ORC_ADV_UPWIND1 10
+#define ORC_ADV_CENTER 11
+#define ORC_ADV_CENTER4 12
+double orc_center_cell(const orc_grid *g, const double *uf, const double *vf,
+                       const double *phi, int i, int j);
+double orc_center4_cell(const orc_grid *g, const double *phi, int i, int j);
+double orc_adv_cell(const orc_grid *g, const double *uf, const double *vf,
+                    const double *phi, int i, int j, int scheme);
+void orc_advection_step(const orc_grid *g, const double *uf, const double *vf,
+                        double *phi, const orc_bc *bc, int scheme, double dt);
 double orc_upwind1_cell(const orc_grid *g, const double *uf, const double *vf,
                         const double *phi, int i, int j);
 /* iters[0] projection, iters[1..2] the u and v Helmholtz solves */
@@ -210,6 +221,7 @@ int orc_ns_kim_step(const orc_grid *g, const orc_csr *A, const orc_csr *Hu,
 #define ORC_PLANT_NEUMANN_SIGN 4
 #define ORC_PLANT_KIM_PRED_WALLS 5
 #define ORC_PLANT_KIM_AB2_GHOSTS 6
+#define ORC_PLANT_ADV_TWO_BUFFER 7
 void orc_set_plant(int plant);
 /* threads of the CPU baseline (1: the serial reference order) */
 void orc_set_threads(int n);
