@@ -59,3 +59,19 @@ def test_residual_file_appends_with_an_empty_line():
 def test_step_line():
     assert out.step_line(42, 0.0125, 0.5, 1.25) == \
         "step:       42 t: 1.25000e-02 cpu:  0.50000 wall:  1.25000\n"
+
+
+def test_reader_on_a_file_the_reference_holds():
+    # tests/golden/textfile_pointchain_shui.txt is the data file
+    # example/geometry/ear_clipping/shui of the reference tree, written in
+    # the same TextFile layout as Output_Scalar ("## key : value" header
+    # lines, then comma-separated numbers): the header reads as
+    # TextFile::parse_config reads it and every data line parses
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "textfile_pointchain_shui.txt")
+    cfg, data = out.read_scalar(path)
+    # the file's "Size" header (67) is as its writer left it; it holds 81
+    # points, and neither reader cross-checks the two
+    assert cfg == {"Dim": "2", "Size": "67", "Type": "PointChain", "Closed": "1"}
+    assert data.shape == (81, 2)
+    assert tuple(data[0]) == (128.20655244, 815.0344063)
