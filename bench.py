@@ -217,7 +217,7 @@ def kim_probe(ctx, ca, no_cpu, n=512, steps=10):
     return out
 
 
-def ns_explicit_probe(ctx, ca, no_cpu, n=512, steps=100, warm=2):
+def ns_explicit_probe(ctx, ca, no_cpu, n=512, steps=100, warm=2, graphs=True):
     """The plain method on the GPU: the n^2 lid-driven cavity (Re 1000,
     rho = Re, mu = 1, van Leer TVD, CFL 0.5, dt0 = 0.5 h, diffusion number
     0.25) stepped by NS_explicit_ with FP64 Jacobi-PCG projections to 1e-10
@@ -232,7 +232,12 @@ def ns_explicit_probe(ctx, ca, no_cpu, n=512, steps=100, warm=2):
               dt=0.5 * h)
     ns = ca.NS(ctx, n, n, 2, (-0.5, 0.5), (-0.5, 0.5), **kw)
     P = ns.projection()
-    tune = P.autotune(reps=20)
+    if graphs:
+        tune = P.autotune(reps=20)
+    else:
+        # profiled runs: direct launches, the default direction kernel
+        P.set_batch(16, use_graphs=False)
+        tune = None
     it0, _ = ns.step(warm)
     ctx.sync()
     t0 = time.perf_counter()
@@ -536,6 +541,8 @@ def main():
                     help="direction-kernel variant of --probe (default: autotune)")
     ap.add_argument("--only", default="",
                     help="comma-separated legs to run alone (ns, blob, kim, csr, batch)")
+    ap.add_argument("--ns-steps", type=int, default=100,
+                    help="timed steps of the 512^2 NS_explicit_ cavity leg")
     ap.add_argument("--no-ns", action="store_true",
                     help="skip the 512^2 NS_explicit_ cavity run")
     ap.add_argument("--no-blob", action="store_true",
@@ -561,7 +568,8 @@ def main():
     if a.only:
         # the named legs alone, one JSON line of their results (no headline)
         ctx = ca.Context(0)
-        legs = {"ns": lambda: ns_explicit_probe(ctx, ca, a.no_cpu),
+        legs = {"ns": lambda: ns_explicit_probe(ctx, ca, a.no_cpu, steps=a.ns_steps,
+                                                graphs=not a.no_graphs),
                 "blob": lambda: blob_probe(ctx, ca, a.no_cpu),
                 "kim": lambda: kim_probe(ctx, ca, a.no_cpu),
                 "csr": lambda: csr_probe(ctx, ca, a.no_cpu),
@@ -811,7 +819,7 @@ def main():
         batch = batched(ctx, ca, tol, a.no_cpu, not a.no_batch_learned)
     csr = None if a.no_csr else csr_probe(ctx, ca, a.no_cpu)
     kim = None if a.no_kim else kim_probe(ctx, ca, a.no_cpu)
-    nsx = None if a.no_ns else ns_explicit_probe(ctx, ca, a.no_cpu)
+    nsx = None if a.no_ns else ns_explicit_probe(ctx, ca, a.no_cpu, steps=a.ns_steps)
     blob = None if a.no_blob else blob_probe(ctx, ca, a.no_cpu)
     out = {
         "metric": "Poisson solves/s to rel-res 1e-8 @1024^2: learned+FP32/FP64-refine vs plain FP64",
