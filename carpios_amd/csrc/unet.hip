@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_first_fwd(const float* __restrict__ x,
 // skipped: fma(w, 0, s) = s up to the sign of a zero, which the ReLU and
 // every later product and sum leave without effect.  Item (pixel, channel
 // half h) per thread, h wave-uniform so that the weights are scalar loads.
-__global__ __launch_bounds__(256, 4) void k_enc0_apply(const float* __restrict__ x,
+__global__ __launch_bounds__(256, 5) void k_enc0_apply(const float* __restrict__ x,
                                                       const float* __restrict__ scale, int H,
                                                       int W, int walls,
                                                       const float* __restrict__ w0,
@@ -186,10 +186,15 @@ __global__ __launch_bounds__(256, 4) void k_enc0_apply(const float* __restrict__
   for (int t = tw.first; t < tw.end; t += tw.per) {
     const int ty = t / ntx;
     const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
+    // the thread's indices recomputed per tile (held across the loop, the
+    // addresses derived from them outgrow four waves' registers)
+    int tv = threadIdx.x;
+    asm volatile("" : "+v"(tv));
+    const int lane = tv & 63, pl = tv & 127;
     __syncthreads();  // the previous tile's reads of sxf and sa are done
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      const int e = threadIdx.x + 256 * k;
+      const int e = tv + 256 * k;
       if (e < NX) {
         const int r = e / XP, c = e - r * XP;
         const int gi = ti0 - 2 + r, gj = tj0 - 2 + c;
@@ -267,7 +272,7 @@ __global__ __launch_bounds__(256, 4) void k_enc0_apply(const float* __restrict__
 //   mid: 18 rows of 16 pixels (wave w: rows R0(w) .. R0(w) + 4 or + 3, so
 //        that a row's fragments serve the three rows beside it) and the
 //        two right halo columns as three 16-pixel groups
-__global__ __launch_bounds__(256, 2) void k_dec0_apply(View d1, View skip,
+__global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
                                                       const bf16* __restrict__ wu,
                                                       const float* __restrict__ bu,
                                                       const bf16* __restrict__ wm,
@@ -279,9 +284,10 @@ __global__ __launch_bounds__(256, 2) void k_dec0_apply(View d1, View skip,
                                                       float* __restrict__ yout, int ntx,
                                                       int ntiles) {
   constexpr int XP = 20, TP = 18, NC = 36;   // window side, halo side, coarse pixels
+  constexpr int SP = 32;                     // window pixel pitch (BF16 values)
   constexpr int NS = XP * XP * 2;            // 16-byte skip chunks of the window
   constexpr int NSP = (NS + 255) / 256;
-  __shared__ __attribute__((aligned(16))) bf16 sc[XP * XP * 32];
+  __shared__ __attribute__((aligned(16))) bf16 sc[XP * XP * SP];
   __shared__ __attribute__((aligned(16))) bf16 sd[TP * TP * 16];
   __shared__ __attribute__((aligned(16))) bf16 su[NC * 32];
   __shared__ __attribute__((aligned(16))) bf16 swl[9 * 16 * 16];  // last conv [tap][co][ci]
@@ -334,22 +340,29 @@ __global__ __launch_bounds__(256, 2) void k_dec0_apply(View d1, View skip,
   for (int t = tw.first; t < tw.end; t += tw.per) {
     const int ty = t / ntx;
     const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
+    // the thread's indices are recomputed per tile: held across the loop,
+    // every address derived from them stayed live and the kernel needed 240
+    // registers (two waves per SIMD); recomputed, three waves fit
+    int tv = threadIdx.x;
+    asm volatile("" : "+v"(tv));
+    const int px = tv & 15, kq = 8 * ((tv >> 4) & 3), cq = 4 * ((tv >> 4) & 3);
+    const int lane = tv & 63;
     // A: stage the skip window and the coarse pixels.  No barrier is needed
     // before it: the previous tile's last reads of sc and su precede its
     // barrier before the final convolution.
 #pragma unroll
     for (int k = 0; k < NSP; ++k) {
-      const int e = threadIdx.x + 256 * k;
+      const int e = tv + 256 * k;
       if (e < NS) {
         const int pix = e >> 1, ch = e & 1;
         const int r = pix / XP, c = pix - r * XP;
         const int gi = ti0 - 2 + r, gj = tj0 - 2 + c;
         const bool in = gi >= 0 && gi < H && gj >= 0 && gj < W;
-        *reinterpret_cast<uint4*>(sc + pix * 32 + 16 + ch * 8) = in ? pre[k] : uint4{0u, 0u, 0u, 0u};
+        *reinterpret_cast<uint4*>(sc + pix * SP + 16 + ch * 8) = in ? pre[k] : uint4{0u, 0u, 0u, 0u};
       }
     }
-    if ((int)threadIdx.x < NC * 4) {
-      const int cp = threadIdx.x >> 2, ch = threadIdx.x & 3;
+    if (tv < NC * 4) {
+      const int cp = tv >> 2, ch = tv & 3;
       *reinterpret_cast<uint4*>(su + cp * 32 + ch * 8) = prc;
     }
     __syncthreads();
@@ -379,7 +392,7 @@ __global__ __launch_bounds__(256, 2) void k_dec0_apply(View d1, View skip,
         const bool ok = fr >= 0 && fr < XP && fc >= 0 && fc < XP;
         const uint32_t h0 = (uint32_t)f2bf(a[0] + bur[0]) | ((uint32_t)f2bf(a[1] + bur[1]) << 16);
         const uint32_t h1 = (uint32_t)f2bf(a[2] + bur[2]) | ((uint32_t)f2bf(a[3] + bur[3]) << 16);
-        bf16* dst = ok ? sc + (fr * XP + fc) * 32 + cq : junk + 4 * lane;
+        bf16* dst = ok ? sc + (fr * XP + fc) * SP + cq : junk + 4 * lane;
         *reinterpret_cast<uint2*>(dst) = uint2{cin ? h0 : 0u, cin ? h1 : 0u};
       }
     }
@@ -408,16 +421,32 @@ __global__ __launch_bounds__(256, 2) void k_dec0_apply(View d1, View skip,
       f32x4_t acc[6];
 #pragma unroll
       for (int a = 0; a < 6; ++a) acc[a] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      // software-pipelined: the next tap's weight and six fragments are
+      // read from LDS before this tap's six MFMAs, so the reads' latency
+      // overlaps them (read just before use, every MFMA waited for its read)
+      auto frag = [&](int tap, int a) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const int R = a < 5 ? min(R0 + a, TP - 1) : Rs, c = a < 5 ? px : cs;
+        return F::load(sc + ((R + 1 + dy) * XP + c + 1 + dx) * SP + kq);
+      };
+      F::T wt = F::load(swm + px * 32 + kq), af[6];
+#pragma unroll
+      for (int a = 0; a < 6; ++a) af[a] = frag(0, a);
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap) {
-        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-        const F::T wt = F::load(swm + (tap * 16 + px) * 32 + kq);
+        F::T wn = wt, an[6];
 #pragma unroll
-        for (int a = 0; a < 6; ++a) {
-          const int R = a < 5 ? min(R0 + a, TP - 1) : Rs, c = a < 5 ? px : cs;
-          const F::T af = F::load(sc + ((R + 1 + dy) * XP + c + 1 + dx) * 32 + kq);
-          acc[a] = F::mfma(wt, af, acc[a]);
+        for (int a = 0; a < 6; ++a) an[a] = af[a];
+        if (tap < 8) {
+          wn = F::load(swm + ((tap + 1) * 16 + px) * 32 + kq);
+#pragma unroll
+          for (int a = 0; a < 6; ++a) an[a] = frag(tap + 1, a);
         }
+#pragma unroll
+        for (int a = 0; a < 6; ++a) acc[a] = F::mfma(wt, af[a], acc[a]);
+        wt = wn;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) af[a] = an[a];
       }
 #pragma unroll
       for (int a = 0; a < 5; ++a) mid_store(acc[a], min(R0 + a, TP - 1), px);
@@ -998,7 +1027,7 @@ int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
   bool pooled0 = false;
   if (fused0) {
     const EvPair ev = timer_events(U->ctx, 8);
-    CPGPU_LAUNCH(ev, k_enc0_apply, dim3(std::min(ntiles, U->ncu * U->conv_persist)), dim3(256), 0,
+    CPGPU_LAUNCH(ev, k_enc0_apply, dim3(std::min(ntiles, U->ncu * 5)), dim3(256), 0,
                  st, raw, U->scale.p, U->H, U->W, U->walls, U->params.p + U->woff[0],
                  U->params.p + U->boff[0], U->wbf.p + U->wbf_off[1], U->params.p + U->boff[1],
                  U->v(U->a0), U->v(U->p0), ntx, ntiles);
@@ -1032,7 +1061,7 @@ int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
   conv_fwd<32, 32>(U, U->v(U->d1a), U->v(U->d1), 13, 1);
   if (fused0) {
     const EvPair ev = timer_events(U->ctx, 9);
-    CPGPU_LAUNCH(ev, k_dec0_apply, dim3(std::min(ntiles, U->ncu * 2)), dim3(256), 0, st,
+    CPGPU_LAUNCH(ev, k_dec0_apply, dim3(std::min(ntiles, U->ncu * 3)), dim3(256), 0, st,
                  U->v(U->d1), U->v(U->a0), U->wbf.p + U->wbf_off[14],
                  U->params.p + U->boff[14], U->wbf.p + U->wbf_off[15],
                  U->params.p + U->boff[15], U->wbf.p + U->wbf_off[16],

@@ -602,20 +602,36 @@ __device__ __forceinline__ void conv16_mma(const bf16* sx,
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int pr = 0; pr < 5; ++pr) {
-    const int tap = 2 * pr + half;
-    const int tp = tap < 9 ? tap : 4;
+  // row by row, the next row's five fragments read from LDS before this
+  // row's MFMAs (each accumulator still takes the pairs in order 0..4)
+  // the pair's two taps are compile-time constants: a lane selects its
+  // tap's offset (a runtime tap cost a division by 3 per fragment)
+  auto tap_off = [](int tap) {
+    const int tp = tap < 9 ? tap : 4;  // the padded tap reads any in-tile value, zeroed
     const int dy = FWD ? tp / 3 - 1 : 1 - tp / 3;
     const int dx = FWD ? tp % 3 - 1 : 1 - tp % 3;
+    return (dy * TP + dx) * KCH;
+  };
+  const bf16* base = sx + ((wave * 4 + 1) * TP + px + 1) * KCH + kc;
+  auto frag = [&](int pr, int a) {
+    const int off = half ? tap_off(2 * pr + 1) : tap_off(2 * pr);
+    typename F::T xf = F::load(base + a * TP * KCH + off);
+    return (half && 2 * pr + 1 >= 9) ? F::zero() : xf;
+  };
+  typename F::T xf[5];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int r = wave * 4 + a;
-      typename F::T xf = F::load(sx + ((r + 1 + dy) * TP + px + 1 + dx) * KCH + kc);
-      if (tap >= 9) xf = F::zero();
+  for (int pr = 0; pr < 5; ++pr) xf[pr] = frag(pr, 0);
 #pragma unroll
-      for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(wf[pr][b], xf, acc[a][b]);
-    }
+  for (int a = 0; a < 4; ++a) {
+    typename F::T xn[5];
+#pragma unroll
+    for (int pr = 0; pr < 5; ++pr) xn[pr] = a < 3 ? frag(pr, a + 1) : xf[pr];
+#pragma unroll
+    for (int pr = 0; pr < 5; ++pr)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(wf[pr][b], xf[pr], acc[a][b]);
+#pragma unroll
+    for (int pr = 0; pr < 5; ++pr) xf[pr] = xn[pr];
   }
 }
 
