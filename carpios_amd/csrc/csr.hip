@@ -17,6 +17,7 @@
 #include <cmath>
 #include <vector>
 
+#include "bicg.hpp"
 #include "poisson.hpp"
 
 struct cpgpu_csr {
@@ -29,6 +30,11 @@ struct cpgpu_csr {
   cpgpu::DevBuf<double> vec[10];
   cpgpu::DevBuf<double> part;   // block partials + totals
   cpgpu::DevBuf<int> flag;
+  cpgpu::DevBuf<double> dhist;  // device residual history of the device-scalar CG
+  void* hst = nullptr;          // pinned host copy of its state
+  ~cpgpu_csr() {
+    if (hst) (void)hipHostFree(hst);
+  }
 };
 
 namespace cpgpu {
@@ -39,6 +45,26 @@ constexpr double kSmall = 1e-10;  // SMALL, lib/type_define.hpp:16
 
 inline int csr_grid(int n) { return std::max(1, std::min(kCsrG, (n + 255) / 256)); }
 
+// sum over k in [k, e) of f(ci[k]) * w[k], left to right as the reference
+// adds a row: the entries are taken four at a time so that the four
+// gathers are in flight together
+template <typename F>
+__device__ __forceinline__ double row_sum(int k, int e, const int* __restrict__ ci,
+                                          const double* __restrict__ w, F f) {
+  double acc = 0.0;
+  for (; k + 4 <= e; k += 4) {
+    const int c0 = ci[k], c1 = ci[k + 1], c2 = ci[k + 2], c3 = ci[k + 3];
+    const double w0 = w[k], w1 = w[k + 1], w2 = w[k + 2], w3 = w[k + 3];
+    const double x0 = f(c0), x1 = f(c1), x2 = f(c2), x3 = f(c3);
+    acc += x0 * w0;
+    acc += x1 * w1;
+    acc += x2 * w2;
+    acc += x3 * w3;
+  }
+  for (; k < e; ++k) acc += f(ci[k]) * w[k];
+  return acc;
+}
+
 // y = A x: row i summed in CSR order from 0 (matrix_SparCompRow.hpp:236-249)
 __global__ __launch_bounds__(256) void k_csr_mult(int n, const int* __restrict__ rp,
                                                   const int* __restrict__ ci,
@@ -46,9 +72,7 @@ __global__ __launch_bounds__(256) void k_csr_mult(int n, const int* __restrict__
                                                   const double* __restrict__ x,
                                                   double* __restrict__ y) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) acc += x[ci[k]] * val[k];
-    y[i] = acc;
+    y[i] = row_sum(rp[i], rp[i + 1], ci, val, [&](int c) { return x[c]; });
   }
 }
 
@@ -60,9 +84,7 @@ __global__ __launch_bounds__(256) void k_csr_resid(int n, const int* __restrict_
                                                    const double* __restrict__ b,
                                                    double* __restrict__ r) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) acc += x[ci[k]] * val[k];
-    r[i] = b[i] - acc;
+    r[i] = b[i] - row_sum(rp[i], rp[i + 1], ci, val, [&](int c) { return x[c]; });
   }
 }
 
@@ -120,9 +142,7 @@ __global__ __launch_bounds__(256) void k_csr_jacobi_sweep(int n, const int* __re
                                                           const double* __restrict__ x,
                                                           double* __restrict__ newx) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) acc += x[ci[k]] * tv[k];
-    newx[i] = acc + c[i];
+    newx[i] = row_sum(rp[i], rp[i + 1], ci, tv, [&](int j) { return x[j]; }) + c[i];
   }
 }
 
@@ -157,73 +177,187 @@ __global__ __launch_bounds__(256) void k_csr_total(const double* part, int G, do
 }
 
 // z = r (no preconditioner) or r * d
-__global__ __launch_bounds__(256) void k_csr_precond(int n, const double* __restrict__ r,
-                                                     const double* __restrict__ d,
-                                                     double* __restrict__ z) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-    z[k] = d ? r[k] * d[k] : r[k];
-}
 
 // p = z (first) or z + beta p
-__global__ __launch_bounds__(256) void k_csr_dir(int n, const double* __restrict__ z,
-                                                 double* __restrict__ p, double beta, int first) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-    p[k] = first ? z[k] : z[k] + beta * p[k];
-}
 
 // x += alpha p; r -= alpha q
-__global__ __launch_bounds__(256) void k_csr_cg_upd(int n, double* __restrict__ x,
-                                                    double* __restrict__ r,
-                                                    const double* __restrict__ p,
-                                                    const double* __restrict__ q, double alpha) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    x[k] += alpha * p[k];
-    r[k] -= alpha * q[k];
-  }
-}
 
 // p = r (first) or r + beta (p - omega v); ph = p * d
-__global__ __launch_bounds__(256) void k_csr_bicg_p(int n, const double* __restrict__ r,
-                                                    double* __restrict__ p,
-                                                    const double* __restrict__ v,
-                                                    const double* __restrict__ d,
-                                                    double* __restrict__ ph, double beta,
-                                                    double omega, int first) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const double pv = first ? r[k] : r[k] + beta * (p[k] - omega * v[k]);
-    p[k] = pv;
-    ph[k] = pv * d[k];
-  }
-}
 
 // s = r - alpha v
-__global__ __launch_bounds__(256) void k_csr_bicg_s(int n, const double* __restrict__ r,
-                                                    const double* __restrict__ v,
-                                                    double* __restrict__ s, double alpha) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-    s[k] = r[k] - alpha * v[k];
-}
 
 // x += a y
-__global__ __launch_bounds__(256) void k_csr_axpy(int n, double* __restrict__ x,
-                                                  const double* __restrict__ y, double a) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-    x[k] += a * y[k];
-}
 
 // x += alpha ph + omega sh; r = s - omega t
-__global__ __launch_bounds__(256) void k_csr_bicg_x(int n, double* __restrict__ x,
-                                                    double* __restrict__ r,
-                                                    const double* __restrict__ ph,
-                                                    const double* __restrict__ sh,
-                                                    const double* __restrict__ s,
-                                                    const double* __restrict__ t, double alpha,
-                                                    double omega) {
+
+// ---------------------------------------------------------------------
+// CG / Jacobi-PCG with its scalars on the device.  An iteration of
+// solver_matrix.hpp:264-374 (z = M r; rho = r.z; p = z + (rho/rho_1) p;
+// q = A p; alpha = rho/p.q; x += alpha p; r -= alpha q; resid = nrm(r)/|b|)
+// runs as four launches that read and write the state below; iterations
+// after the stop return at entry, so the host launches batches and reads
+// the state once per batch.  z and r.z of the next iteration are formed in
+// the update kernel from the updated r, the values the next iteration's
+// first two steps compute.
+struct CState {
+  double normb, tol, resid, rho, rho_1, alpha;
+  int it, max_iter, done, hist_cap;
+};
+static_assert(sizeof(CState) <= 8 * sizeof(double), "the CSR state slot holds 8 doubles");
+
+// p_new = z (first) or z + beta p_old, q = A p_new with p_new at the
+// column entries recomputed from z and p_old (the stored p's bits), and
+// partials of p.q
+__global__ __launch_bounds__(256) void k_csr_dirq(int n, const int* __restrict__ rp,
+                                                  const int* __restrict__ ci,
+                                                  const double* __restrict__ val,
+                                                  const double* __restrict__ z,
+                                                  const double* __restrict__ pold,
+                                                  double* __restrict__ pnew,
+                                                  double* __restrict__ q, const CState* st,
+                                                  double* part) {
+  __shared__ double lds[4];
+  if (st->done) return;
+  const bool first = st->it == 0;
+  const double beta = first ? 0.0 : st->rho / st->rho_1;
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double acc = row_sum(rp[i], rp[i + 1], ci, val, [&](int c) {
+      return first ? z[c] : z[c] + beta * pold[c];
+    });
+    const double pi = first ? z[i] : z[i] + beta * pold[i];
+    pnew[i] = pi;
+    q[i] = acc;
+    v[0] += pi * acc;
+  }
+  store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
+}
+
+__global__ __launch_bounds__(256) void k_csr_alpha(const double* part, int G, CState* st) {
+  __shared__ double lds[4];
+  if (st->done) return;
+  double t[1];
+  reduce_partials<double, 1>(part, G, t, lds);
+  if (threadIdx.x == 0) st->alpha = st->rho / t[0];
+}
+
+// x += alpha p; r -= alpha q; z = M r; partials of r.r (the reference nrm2
+// leaves the last element out when skip_last) and r.z
+__global__ __launch_bounds__(256) void k_csr_updz(int n, double* __restrict__ x,
+                                                  double* __restrict__ r,
+                                                  const double* __restrict__ p,
+                                                  const double* __restrict__ q,
+                                                  const double* __restrict__ d,
+                                                  double* __restrict__ z, const CState* st,
+                                                  int skip_last, double* part) {
+  __shared__ double lds[8];
+  if (st->done) return;
+  const double alpha = st->alpha;
+  double v[2] = {0.0, 0.0};
+  const int m = skip_last ? n - 1 : n;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    x[k] += alpha * ph[k] + omega * sh[k];
-    r[k] = s[k] - omega * t[k];
+    x[k] += alpha * p[k];
+    const double rk = r[k] - alpha * q[k];
+    r[k] = rk;
+    const double zk = d ? rk * d[k] : rk;
+    z[k] = zk;
+    if (k < m) v[0] += rk * rk;
+    v[1] += rk * zk;
+  }
+  store_partials<double, 2>(v, part, gridDim.x, blockIdx.x, lds);
+}
+
+// the stopping test: resid = sqrt(r.r)/|b| into the history; converged,
+// or max_iter reached; rho_1 <- rho, rho <- r.z of the next iteration
+__global__ __launch_bounds__(256) void k_csr_test(const double* part, int G, CState* st,
+                                                  double* hist) {
+  __shared__ double lds[8];
+  if (st->done) return;
+  double t[2];
+  reduce_partials<double, 2>(part, G, t, lds);
+  if (threadIdx.x == 0) {
+    const double resid = std::sqrt(t[0]) / st->normb;
+    if (hist && st->it < st->hist_cap) hist[st->it] = resid;
+    st->it += 1;
+    st->resid = resid;
+    if (resid <= st->tol) st->done = 1;
+    else if (st->it >= st->max_iter) st->done = 2;
+    st->rho_1 = st->rho;
+    st->rho = t[1];
   }
 }
+
+// the first z = M r and rho = r.z
+__global__ __launch_bounds__(256) void k_csr_pz(int n, const double* __restrict__ r,
+                                                const double* __restrict__ d,
+                                                double* __restrict__ z, double* part) {
+  __shared__ double lds[4];
+  double v[1] = {0.0};
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const double zk = d ? r[k] * d[k] : r[k];
+    z[k] = zk;
+    v[0] += r[k] * zk;
+  }
+  store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
+}
+
+__global__ __launch_bounds__(256) void k_csr_rho0(const double* part, int G, CState* st) {
+  __shared__ double lds[4];
+  double t[1];
+  reduce_partials<double, 1>(part, G, t, lds);
+  if (threadIdx.x == 0) st->rho = t[0];
+}
+
+// Dia_BiCGSTAB's operator kernels on the CSR matrix (the scalar kernels and
+// the batch loop are shared with the structured operator).
+// p = r (first) or r + beta (p_old - omega v_old), v = A (p D^-1) with the
+// columns' p formed in place; partials of rt.v
+__global__ __launch_bounds__(256) void k_csr_bicg_dir(
+    int n, const int* __restrict__ rp, const int* __restrict__ ci,
+    const double* __restrict__ val, const double* __restrict__ d, const double* __restrict__ r,
+    const double* __restrict__ rt, const double* __restrict__ pold,
+    const double* __restrict__ vold, double* __restrict__ pnew, double* __restrict__ vnew,
+    const BState* st, double* part) {
+  __shared__ double lds[4];
+  if (st->stop) return;
+  const bool first = st->it == 0;
+  const double beta = st->beta, omega = st->omega;
+  auto pat = [&](int c) { return first ? r[c] : r[c] + beta * (pold[c] - omega * vold[c]); };
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double vi = row_sum(rp[i], rp[i + 1], ci, val, [&](int c) { return pat(c) * d[c]; });
+    pnew[i] = pat(i);
+    vnew[i] = vi;
+    v[0] += rt[i] * vi;
+  }
+  store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
+}
+
+// s = r - alpha v, t = A (s D^-1) with the columns' s formed in place;
+// partials of s.s (without the last unknown when skip_last), t.s, t.t
+__global__ __launch_bounds__(256) void k_csr_bicg_st(
+    int n, const int* __restrict__ rp, const int* __restrict__ ci,
+    const double* __restrict__ val, const double* __restrict__ d, const double* __restrict__ r,
+    const double* __restrict__ vv, double* __restrict__ sv, double* __restrict__ tv,
+    const BState* st, int skip_last, double* part) {
+  __shared__ double lds[12];
+  if (st->stop) return;
+  const double alpha = st->alpha;
+  double v[3] = {0.0, 0.0, 0.0};
+  const int m = skip_last ? n - 1 : n;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double ti =
+        row_sum(rp[i], rp[i + 1], ci, val, [&](int c) { return (r[c] - alpha * vv[c]) * d[c]; });
+    const double si = r[i] - alpha * vv[i];
+    sv[i] = si;
+    tv[i] = ti;
+    if (i < m) v[0] += si * si;
+    v[1] += ti * si;
+    v[2] += ti * ti;
+  }
+  store_partials<double, 3>(v, part, gridDim.x, blockIdx.x, lds);
+}
+
 
 struct CsrRun {
   cpgpu_csr* A;
@@ -286,12 +420,15 @@ struct CsrRun {
   }
 };
 
-// CG / Jacobi-PCG (solver_matrix.hpp:264-374 with DiaPre(A,1))
+// CG / Jacobi-PCG (solver_matrix.hpp:264-374 with DiaPre(A,1)): the initial
+// residual test on the host, then batches of kBatch device-scalar
+// iterations, the state read back once per batch
 int csr_cg(CsrRun& R, double* x, const double* b, int* max_iter, double* tol, bool precond) {
   cpgpu_csr* A = R.A;
   const int n = A->n, G = R.G;
   hipStream_t st = R.st;
-  double *r = A->vec[0].p, *z = A->vec[1].p, *p = A->vec[2].p, *q = A->vec[3].p;
+  double *r = A->vec[0].p, *z = A->vec[1].p, *q = A->vec[3].p;
+  double* P[2] = {A->vec[2].p, A->vec[4].p};
   int rc;
   if (precond && (rc = R.diapre())) return rc;
   double normb = 0, resid = 0;
@@ -304,46 +441,112 @@ int csr_cg(CsrRun& R, double* x, const double* b, int* max_iter, double* tol, bo
     *max_iter = 0;
     return CPGPU_OK;
   }
-  double rho = 0, rho_1 = 0;
-  for (int i = 1; i <= *max_iter; ++i) {
-    hipLaunchKernelGGL(k_csr_precond, dim3(G), dim3(256), 0, st, n, (const double*)r,
-                       precond ? (const double*)A->dinv.p : nullptr, z);
-    if ((rc = R.dots(r, z, nullptr, nullptr, 0, &rho))) return rc;
-    const double beta = i == 1 ? 0.0 : rho / rho_1;
-    hipLaunchKernelGGL(k_csr_dir, dim3(G), dim3(256), 0, st, n, (const double*)z, p, beta,
-                       i == 1 ? 1 : 0);
-    R.mult(p, q);
-    double pq = 0;
-    if ((rc = R.dots(p, q, nullptr, nullptr, 0, &pq))) return rc;
-    const double alpha = rho / pq;
-    hipLaunchKernelGGL(k_csr_cg_upd, dim3(G), dim3(256), 0, st, n, x, r, (const double*)p,
-                       (const double*)q, alpha);
-    if ((rc = R.nrm(r, &resid))) return rc;
-    resid /= normb;
-    R.push(resid);
-    if (resid <= *tol) {
-      *tol = resid;
-      *max_iter = i;
-      return CPGPU_OK;
-    }
-    rho_1 = rho;
+  if (*max_iter < 1) {
+    *tol = resid;
+    return CPGPU_MAXITER;
   }
-  *tol = resid;
+  if (!A->hst && hipHostMalloc(&A->hst, sizeof(CState), hipHostMallocDefault) != hipSuccess) {
+    A->hst = nullptr;
+    return set_error(A->ctx, CPGPU_ENOMEM, "pinned CSR solver state");
+  }
+  const int cap = R.hist ? R.hist_cap : 0;
+  if (cap > 0 && A->dhist.alloc(cap)) return set_error(A->ctx, CPGPU_ENOMEM, "CSR history");
+  // the state sits after the partials (2 kCsrG) and the totals (2)
+  CState* dst = reinterpret_cast<CState*>(A->part.p + 2 * kCsrG + 8);
+  CState* hs = static_cast<CState*>(A->hst);
+  CState init{};
+  init.normb = normb;
+  init.tol = *tol;
+  init.resid = resid;
+  init.max_iter = *max_iter;
+  init.hist_cap = cap;
+  *hs = init;
+  CPGPU_HIP(A->ctx, hipMemcpyAsync(dst, hs, sizeof(CState), hipMemcpyHostToDevice, st));
+  const double* dd = precond ? (const double*)A->dinv.p : nullptr;
+  double* part = A->part.p;
+  double* dh = cap > 0 ? A->dhist.p : nullptr;
+  hipLaunchKernelGGL(k_csr_pz, dim3(G), dim3(256), 0, st, n, (const double*)r, dd, z, part);
+  hipLaunchKernelGGL(k_csr_rho0, dim3(1), dim3(256), 0, st, (const double*)part, G, dst);
+  constexpr int kBatch = 16;  // even: the direction buffers alternate
+  const int skip = R.norm_ref && n > 1 ? 1 : 0;
+  for (;;) {
+    for (int k = 0; k < kBatch; ++k) {
+      const double* pold = P[k & 1];
+      double* pnew = P[(k + 1) & 1];
+      hipLaunchKernelGGL(k_csr_dirq, dim3(G), dim3(256), 0, st, n, (const int*)A->rp.p,
+                         (const int*)A->ci.p, (const double*)A->val.p, (const double*)z, pold,
+                         pnew, q, (const CState*)dst, part);
+      hipLaunchKernelGGL(k_csr_alpha, dim3(1), dim3(256), 0, st, (const double*)part, G, dst);
+      hipLaunchKernelGGL(k_csr_updz, dim3(G), dim3(256), 0, st, n, x, r, (const double*)pnew,
+                         (const double*)q, dd, z, (const CState*)dst, skip, part);
+      hipLaunchKernelGGL(k_csr_test, dim3(1), dim3(256), 0, st, (const double*)part, G, dst,
+                         dh);
+    }
+    CPGPU_HIP(A->ctx, hipGetLastError());
+    CPGPU_HIP(A->ctx, hipMemcpyAsync(hs, dst, sizeof(CState), hipMemcpyDeviceToHost, st));
+    CPGPU_HIP(A->ctx, hipStreamSynchronize(st));
+    if (hs->done) break;
+  }
+  const CState f = *hs;
+  if (cap > 0) {
+    const int nh = std::min(f.it, cap);
+    CPGPU_HIP(A->ctx, hipMemcpyAsync(R.hist, dh, sizeof(double) * nh, hipMemcpyDeviceToHost, st));
+    CPGPU_HIP(A->ctx, hipStreamSynchronize(st));
+  }
+  R.hl = f.it;
+  *tol = f.resid;
+  if (f.done == 1) {
+    *max_iter = f.it;
+    return CPGPU_OK;
+  }
   return CPGPU_MAXITER;
 }
 
 // Dia_BiCGSTAB (solver_matrix.hpp:599-672): strict '<' tests, the half-step
-// exit leaves max_iter unchanged, codes 2 (rho = 0) and 3 (omega = 0)
+// exit leaves max_iter unchanged, codes 2 (rho = 0) and 3 (omega = 0); the
+// initial residual test on the host, then the device-scalar iterations
+struct CsrBicgOps {
+  CsrRun* R;
+  double* x;
+  double *r, *rt, *s, *t;
+  double* P[2];
+  double* V[2];
+  const BState* state;
+  double* part;
+  int G, skip;
+  void rho0() {
+    hipLaunchKernelGGL(k_csr_dots<1>, dim3(G), dim3(256), 0, R->st, (const double*)rt,
+                       (const double*)r, nullptr, nullptr, R->A->n, 0, part);
+  }
+  void dir(int prev, int cur) {
+    cpgpu_csr* A = R->A;
+    hipLaunchKernelGGL(k_csr_bicg_dir, dim3(G), dim3(256), 0, R->st, A->n, (const int*)A->rp.p,
+                       (const int*)A->ci.p, (const double*)A->val.p, (const double*)A->dinv.p,
+                       (const double*)r, (const double*)rt, (const double*)P[prev],
+                       (const double*)V[prev], P[cur], V[cur], state, part);
+  }
+  void st(int cur) {
+    cpgpu_csr* A = R->A;
+    hipLaunchKernelGGL(k_csr_bicg_st, dim3(G), dim3(256), 0, R->st, A->n, (const int*)A->rp.p,
+                       (const int*)A->ci.p, (const double*)A->val.p, (const double*)A->dinv.p,
+                       (const double*)r, (const double*)V[cur], s, t, state, skip, part);
+  }
+  void xr(int cur) {
+    cpgpu_csr* A = R->A;
+    hipLaunchKernelGGL(k_bicg_xr, dim3(G), dim3(256), 0, R->st, (size_t)A->n,
+                       (const double*)A->dinv.p, x, r, (const double*)rt,
+                       (const double*)P[cur], (const double*)s, (const double*)t, state,
+                       skip, part);
+  }
+};
+
 int csr_bicgstab(CsrRun& R, double* x, const double* b, int* max_iter, double* tol) {
   cpgpu_csr* A = R.A;
-  const int n = A->n, G = R.G;
+  const int n = A->n;
   hipStream_t st = R.st;
-  double *r = A->vec[0].p, *rt = A->vec[1].p, *p = A->vec[2].p, *ph = A->vec[3].p;
-  double *v = A->vec[4].p, *s = A->vec[5].p, *sh = A->vec[6].p, *t = A->vec[7].p;
+  double *r = A->vec[0].p, *rt = A->vec[1].p;
   int rc;
   if ((rc = R.diapre())) return rc;
-  CPGPU_HIP(A->ctx, hipMemsetAsync(p, 0, sizeof(double) * n, st));
-  CPGPU_HIP(A->ctx, hipMemsetAsync(v, 0, sizeof(double) * n, st));
   double normb = 0, resid = 0;
   if ((rc = R.nrm(b, &normb))) return rc;
   R.resid(x, b, r);
@@ -356,57 +559,21 @@ int csr_bicgstab(CsrRun& R, double* x, const double* b, int* max_iter, double* t
     R.push(resid);
     return CPGPU_OK;
   }
-  double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0, omega = 0;
-  for (int i = 1; i <= *max_iter; ++i) {
-    if ((rc = R.dots(rt, r, nullptr, nullptr, 0, &rho_1))) return rc;
-    if (rho_1 == 0) {
-      if ((rc = R.nrm(r, &resid))) return rc;
-      *tol = resid / normb;
-      return CPGPU_BREAKDOWN_RHO;
-    }
-    if (i > 1) beta = (rho_1 / rho_2) * (alpha / omega);
-    hipLaunchKernelGGL(k_csr_bicg_p, dim3(G), dim3(256), 0, st, n, (const double*)r, p,
-                       (const double*)v, (const double*)A->dinv.p, ph, beta, omega,
-                       i == 1 ? 1 : 0);
-    R.mult(ph, v);
-    double rtv = 0;
-    if ((rc = R.dots(rt, v, nullptr, nullptr, 0, &rtv))) return rc;
-    alpha = rho_1 / rtv;
-    hipLaunchKernelGGL(k_csr_bicg_s, dim3(G), dim3(256), 0, st, n, (const double*)r,
-                       (const double*)v, s, alpha);
-    double ns = 0;
-    if ((rc = R.nrm(s, &ns))) return rc;
-    if ((resid = ns / normb) < *tol) {
-      hipLaunchKernelGGL(k_csr_axpy, dim3(G), dim3(256), 0, st, n, x, (const double*)ph,
-                         alpha);
-      *tol = resid;
-      R.push(resid);
-      return CPGPU_OK;  // max_iter unchanged, as the reference
-    }
-    hipLaunchKernelGGL(k_csr_precond, dim3(G), dim3(256), 0, st, n, (const double*)s,
-                       (const double*)A->dinv.p, sh);
-    R.mult(sh, t);
-    double ts_tt[2];
-    if ((rc = R.dots(t, s, t, t, 0, ts_tt))) return rc;
-    omega = ts_tt[0] / ts_tt[1];
-    hipLaunchKernelGGL(k_csr_bicg_x, dim3(G), dim3(256), 0, st, n, x, r, (const double*)ph,
-                       (const double*)sh, (const double*)s, (const double*)t, alpha, omega);
-    rho_2 = rho_1;
-    if ((rc = R.nrm(r, &resid))) return rc;
-    resid /= normb;
-    R.push(resid);
-    if (resid < *tol) {
-      *tol = resid;
-      *max_iter = i;
-      return CPGPU_OK;
-    }
-    if (omega == 0) {
-      *tol = resid;
-      return CPGPU_BREAKDOWN_OMEGA;
-    }
+  if (*max_iter < 1) {
+    *tol = resid;
+    return CPGPU_MAXITER;
   }
-  *tol = resid;
-  return CPGPU_MAXITER;
+  const int cap = R.hist ? R.hist_cap : 0;
+  if (cap > 0 && A->dhist.alloc(cap)) return set_error(A->ctx, CPGPU_ENOMEM, "CSR history");
+  BState* dst = reinterpret_cast<BState*>(A->part.p + 3 * kCsrG + 8);
+  CsrBicgOps ops{&R, x, r, rt, A->vec[6].p, A->vec[7].p, {A->vec[2].p, A->vec[4].p},
+                 {A->vec[3].p, A->vec[5].p}, dst, A->part.p, R.G,
+                 R.norm_ref && n > 1 ? 1 : 0};
+  int hl = 0;
+  const int code = bicg_run(A->ctx, st, ops, dst, normb, resid, max_iter, tol,
+                            cap > 0 ? A->dhist.p : nullptr, cap, R.hist, &hl);
+  R.hl = hl;
+  return code;
 }
 
 // Solver_Jacobi_ (solver_matrix.hpp:842-909): on convergence x keeps the
@@ -474,7 +641,7 @@ int cpgpu_csr_create(cpgpu_ctx* ctx, int n, const int* row_ptr, const int* col_i
   A->nnz = nnz;
   bool bad = A->rp.alloc(n + 1) || A->ci.alloc(std::max(nnz, 1)) ||
              A->val.alloc(std::max(nnz, 1)) || A->dinv.alloc(n) ||
-             A->part.alloc(2 * kCsrG + 2) || A->flag.alloc(1);
+             A->part.alloc(3 * kCsrG + 24) || A->flag.alloc(1);
   for (auto& v : A->vec) bad = bad || v.alloc(n);
   if (bad) {
     delete A;

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <vector>
 
+#include "bicg.hpp"
 #include "poisson.hpp"
 
 namespace cpgpu {
@@ -53,28 +54,6 @@ __global__ __launch_bounds__(256) void k_total_n(const double* part, int G,
   }
 }
 
-// p = r (first) or p = r + beta (p - omega v); ph = p * (1 / a_ii)
-__global__ __launch_bounds__(256) void k_bicg_p(Sten<double> s, const double* __restrict__ r,
-                                                double* __restrict__ p,
-                                                const double* __restrict__ v,
-                                                double* __restrict__ ph, double beta,
-                                                double omega, int first, Tiling tl) {
-  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
-    int i0, j0;
-    tile_origin(t, tl.ntj, i0, j0);
-    const int j = j0 + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < TI / BY; ++k) {
-      const int i = i0 + threadIdx.y + BY * k;
-      if (i < s.nx && j < s.ny) {
-        const size_t idx = (size_t)i * s.ny + j;
-        const double pv = first ? r[idx] : r[idx] + beta * (p[idx] - omega * v[idx]);
-        p[idx] = pv;
-        ph[idx] = pv * (1. / diag_at(s, i, j));
-      }
-    }
-  }
-}
 
 // y = A x
 __global__ __launch_bounds__(256) void k_apply64(Sten<double> s, const double* __restrict__ x,
@@ -92,29 +71,57 @@ __global__ __launch_bounds__(256) void k_apply64(Sten<double> s, const double* _
   }
 }
 
-// s = r - alpha v
-__global__ __launch_bounds__(256) void k_bicg_s(const double* __restrict__ r,
-                                                const double* __restrict__ v,
-                                                double* __restrict__ sv, double alpha,
-                                                size_t n) {
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
-       k += (size_t)gridDim.x * blockDim.x)
-    sv[k] = r[k] - alpha * v[k];
+
+
+
+
+// Dia_BiCGSTAB's operator kernels on the structured operator (the scalar
+// kernels, the update kernel and the batch loop are shared with the CSR
+// matrix).  A row of A x from values formed per cell by f (the reference's
+// CSR term order, apply_global's arithmetic).
+template <typename F>
+__device__ __forceinline__ double apply_fn(const Sten<double>& s, int i, int j, F f) {
+  const int ny = s.ny;
+  const size_t k = (size_t)i * ny + j;
+  const double c = f(k);
+  const double n = i > 0 ? f(k - ny) : 0.0;
+  const double w = j > 0 ? f(k - 1) : 0.0;
+  const double e = j < ny - 1 ? f(k + 1) : 0.0;
+  const double so = i < s.nx - 1 ? f(k + ny) : 0.0;
+  return row_apply(s, i, j, diag_at(s, i, j), c, n, so, w, e, s.KX[i], s.KX[i + 1], s.KY[j],
+                   s.KY[j + 1]);
 }
 
-// x += alpha ph (the half-step exit)
-__global__ __launch_bounds__(256) void k_axpy64(double* __restrict__ x,
-                                                const double* __restrict__ y,
-                                                double a, size_t n) {
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
-       k += (size_t)gridDim.x * blockDim.x)
-    x[k] += a * y[k];
+// dinv = 1 / a_ii (the factor of the reference's ph = p * (1 / a_ii))
+__global__ __launch_bounds__(256) void k_dinv64(Sten<double> s, double* __restrict__ dinv,
+                                                Tiling tl) {
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny) dinv[(size_t)i * s.ny + j] = 1. / diag_at(s, i, j);
+    }
+  }
 }
 
-// sh = s * (1 / a_ii)
-__global__ __launch_bounds__(256) void k_scale_diag(Sten<double> s,
-                                                    const double* __restrict__ in,
-                                                    double* __restrict__ out, Tiling tl) {
+// p = r (first) or r + beta (p_old - omega v_old), v = A (p D^-1) with the
+// neighbours' p formed in place; partials of rt.v
+__global__ __launch_bounds__(256) void k_sbicg_dir(
+    Sten<double> s, const double* __restrict__ d, const double* __restrict__ r,
+    const double* __restrict__ rt, const double* __restrict__ pold,
+    const double* __restrict__ vold, double* __restrict__ pnew, double* __restrict__ vnew,
+    const BState* st, double* part, Tiling tl) {
+  __shared__ double lds[4];
+  if (st->stop) return;
+  const bool first = st->it == 0;
+  const double beta = st->beta, omega = st->omega;
+  auto pat = [&](size_t c) {
+    return first ? r[c] : r[c] + beta * (pold[c] - omega * vold[c]);
+  };
+  double v[1] = {0.0};
   for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
     int i0, j0;
     tile_origin(t, tl.ntj, i0, j0);
@@ -124,25 +131,48 @@ __global__ __launch_bounds__(256) void k_scale_diag(Sten<double> s,
       const int i = i0 + threadIdx.y + BY * k;
       if (i < s.nx && j < s.ny) {
         const size_t idx = (size_t)i * s.ny + j;
-        out[idx] = in[idx] * (1. / diag_at(s, i, j));
+        const double vi = apply_fn(s, i, j, [&](size_t c) { return pat(c) * d[c]; });
+        pnew[idx] = pat(idx);
+        vnew[idx] = vi;
+        v[0] += rt[idx] * vi;
       }
     }
   }
+  store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
 }
 
-// x += alpha ph + omega sh; r = s - omega t
-__global__ __launch_bounds__(256) void k_bicg_x(double* __restrict__ x,
-                                                double* __restrict__ r,
-                                                const double* __restrict__ ph,
-                                                const double* __restrict__ sh,
-                                                const double* __restrict__ sv,
-                                                const double* __restrict__ tv,
-                                                double alpha, double omega, size_t n) {
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
-       k += (size_t)gridDim.x * blockDim.x) {
-    x[k] += alpha * ph[k] + omega * sh[k];
-    r[k] = sv[k] - omega * tv[k];
+// s = r - alpha v, t = A (s D^-1) with the neighbours' s formed in place;
+// partials of s.s (norm_ref: without the last unknown), t.s, t.t
+__global__ __launch_bounds__(256) void k_sbicg_st(
+    Sten<double> s, const double* __restrict__ d, const double* __restrict__ r,
+    const double* __restrict__ vv, double* __restrict__ sv, double* __restrict__ tv,
+    const BState* st, int norm_ref, double* part, Tiling tl) {
+  __shared__ double lds[12];
+  if (st->stop) return;
+  const double alpha = st->alpha;
+  const size_t nlast = (size_t)s.nx * s.ny - 1;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    const int j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int i = i0 + threadIdx.y + BY * k;
+      if (i < s.nx && j < s.ny) {
+        const size_t idx = (size_t)i * s.ny + j;
+        const double ti =
+            apply_fn(s, i, j, [&](size_t c) { return (r[c] - alpha * vv[c]) * d[c]; });
+        const double si = r[idx] - alpha * vv[idx];
+        sv[idx] = si;
+        tv[idx] = ti;
+        if (!(norm_ref && idx == nlast)) v[0] += si * si;
+        v[1] += ti * si;
+        v[2] += ti * ti;
+      }
+    }
   }
+  store_partials<double, 3>(v, part, gridDim.x, blockIdx.x, lds);
 }
 
 // r = b - A x
@@ -276,6 +306,42 @@ int dots(cpgpu_poisson* P, const double* x, const double* y, const double* u,
 
 }  // namespace
 
+namespace {
+struct StenBicgOps {
+  hipStream_t s;
+  Sten<double> sten;
+  Tiling tl;
+  size_t N;
+  double *d, *x, *r, *rt, *sv, *tv;
+  double* P[2];
+  double* V[2];
+  const BState* state;
+  double* part;
+  int G, norm_ref;
+  void rho0() {
+    hipLaunchKernelGGL(k_dots<1>, dim3(G), dim3(256), 0, s, (const double*)rt, (const double*)r,
+                       nullptr, nullptr, N, 0, part);
+  }
+  void dir(int prev, int cur) {
+    hipLaunchKernelGGL(k_sbicg_dir, dim3(G), dim3(TJ, BY), 0, s, sten, (const double*)d,
+                       (const double*)r, (const double*)rt, (const double*)P[prev],
+                       (const double*)V[prev], P[cur], V[cur], state, part, tl);
+  }
+  void st(int cur) {
+    hipLaunchKernelGGL(k_sbicg_st, dim3(G), dim3(TJ, BY), 0, s, sten, (const double*)d,
+                       (const double*)r, (const double*)V[cur], sv, tv, state, norm_ref, part,
+                       tl);
+  }
+  void xr(int cur) {
+    hipLaunchKernelGGL(k_bicg_xr, dim3(G), dim3(256), 0, s, N, (const double*)d, x, r,
+                       (const double*)rt, (const double*)P[cur], (const double*)sv,
+                       (const double*)tv, state, norm_ref, part);
+  }
+};
+}  // namespace
+
+// Dia_BiCGSTAB (solver_matrix.hpp:599-672): the initial residual test on
+// the host, then the device-scalar iterations of bicg.hpp
 int run_bicgstab(cpgpu_poisson* P, int* max_iter, double* tol, int norm_ref,
                  double* hist, int hist_cap, int* hist_len) {
   cpgpu_ctx* ctx = P->ctx;
@@ -285,93 +351,41 @@ int run_bicgstab(cpgpu_poisson* P, int* max_iter, double* tol, int norm_ref,
   const Sten<double> s = P->sten64();
   for (auto& v : P->bv)
     if (v.alloc(N)) return set_error(ctx, CPGPU_ENOMEM, "BiCGSTAB vectors");
+  if (P->bst.alloc(1)) return set_error(ctx, CPGPU_ENOMEM, "BiCGSTAB state");
+  if (hist && hist_cap > 0 && P->dhist.alloc(hist_cap))
+    return set_error(ctx, CPGPU_ENOMEM, "hist alloc");
   double* x = P->x.p;
   double* r = P->r.p;
-  double *rt = P->bv[0].p, *p = P->bv[1].p, *ph = P->bv[2].p, *v = P->bv[3].p;
-  double *sv = P->bv[4].p, *sh = P->bv[5].p, *tv = P->bv[6].p;
+  double* rt = P->bv[0].p;
+  double* dinv = P->bv[7].p;
   const dim3 g2(tl.G), b2(TJ, BY);
-  auto nrm = [&](const double* a, double* out) {
-    int rc = dots(P, a, a, nullptr, nullptr, norm_ref, out);
-    *out = std::sqrt(*out);
-    return rc;
-  };
-  int hl = 0;
-  auto push = [&](double val) {
-    if (hist && hl < hist_cap) hist[hl] = val;
-    ++hl;
-  };
   double normb = 0, resid = 0;
   int rc;
-  if ((rc = nrm(P->b.p, &normb))) return rc;
+  if ((rc = dots(P, P->b.p, P->b.p, nullptr, nullptr, norm_ref, &normb))) return rc;
+  normb = std::sqrt(normb);
   hipLaunchKernelGGL(k_resid64, g2, b2, 0, st, s, x, P->b.p, r, tl);
+  hipLaunchKernelGGL(k_dinv64, g2, b2, 0, st, s, dinv, tl);
   CPGPU_HIP(ctx, hipMemcpyAsync(rt, r, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
   if (normb == 0.0) normb = 1;
-  if ((rc = nrm(r, &resid))) return rc;
-  resid /= normb;
+  if ((rc = dots(P, r, r, nullptr, nullptr, norm_ref, &resid))) return rc;
+  resid = std::sqrt(resid) / normb;
   *hist_len = 0;
   if (resid <= *tol) {
     *tol = resid;
     *max_iter = 0;
-    push(resid);
-    *hist_len = hl;
+    if (hist && hist_cap > 0) hist[0] = resid;
+    *hist_len = 1;
     return CPGPU_OK;
   }
-  double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0, omega = 0;
-  for (int i = 1; i <= *max_iter; ++i) {
-    if ((rc = dots(P, rt, r, nullptr, nullptr, 0, &rho_1))) return rc;
-    if (rho_1 == 0) {
-      if ((rc = nrm(r, &resid))) return rc;
-      *tol = resid / normb;
-      *hist_len = hl;
-      return CPGPU_BREAKDOWN_RHO;
-    }
-    if (i > 1) beta = (rho_1 / rho_2) * (alpha / omega);
-    hipLaunchKernelGGL(k_bicg_p, g2, b2, 0, st, s, (const double*)r, p,
-                       (const double*)v, ph, beta, omega, i == 1 ? 1 : 0, tl);
-    hipLaunchKernelGGL(k_apply64, g2, b2, 0, st, s, (const double*)ph, v, tl);
-    double rtv = 0;
-    if ((rc = dots(P, rt, v, nullptr, nullptr, 0, &rtv))) return rc;
-    alpha = rho_1 / rtv;
-    hipLaunchKernelGGL(k_bicg_s, dim3(kG1), dim3(256), 0, st, (const double*)r,
-                       (const double*)v, sv, alpha, N);
-    double ns = 0;
-    if ((rc = nrm(sv, &ns))) return rc;
-    if ((resid = ns / normb) < *tol) {
-      hipLaunchKernelGGL(k_axpy64, dim3(kG1), dim3(256), 0, st, x, (const double*)ph,
-                         alpha, N);
-      CPGPU_HIP(ctx, hipStreamSynchronize(st));
-      *tol = resid;
-      push(resid);
-      *hist_len = hl;
-      return CPGPU_OK;  // max_iter unchanged, as the reference
-    }
-    hipLaunchKernelGGL(k_scale_diag, g2, b2, 0, st, s, (const double*)sv, sh, tl);
-    hipLaunchKernelGGL(k_apply64, g2, b2, 0, st, s, (const double*)sh, tv, tl);
-    double ts_tt[2];
-    if ((rc = dots(P, tv, sv, tv, tv, 0, ts_tt))) return rc;
-    omega = ts_tt[0] / ts_tt[1];
-    hipLaunchKernelGGL(k_bicg_x, dim3(kG1), dim3(256), 0, st, x, r, (const double*)ph,
-                       (const double*)sh, (const double*)sv, (const double*)tv, alpha,
-                       omega, N);
-    rho_2 = rho_1;
-    if ((rc = nrm(r, &resid))) return rc;
-    resid /= normb;
-    push(resid);
-    if (resid < *tol) {
-      *tol = resid;
-      *max_iter = i;
-      *hist_len = hl;
-      return CPGPU_OK;
-    }
-    if (omega == 0) {
-      *tol = resid;
-      *hist_len = hl;
-      return CPGPU_BREAKDOWN_OMEGA;
-    }
+  if (*max_iter < 1) {
+    *tol = resid;
+    return CPGPU_MAXITER;
   }
-  *tol = resid;
-  *hist_len = hl;
-  return CPGPU_MAXITER;
+  StenBicgOps ops{st, s, tl, N, dinv, x, r, rt, P->bv[5].p, P->bv[6].p,
+                  {P->bv[1].p, P->bv[2].p}, {P->bv[3].p, P->bv[4].p}, P->bst.p,
+                  P->part64.p, (int)tl.G, norm_ref};
+  return bicg_run(ctx, st, ops, P->bst.p, normb, resid, max_iter, tol,
+                  hist && hist_cap > 0 ? P->dhist.p : nullptr, hist_cap, hist, hist_len);
 }
 
 int run_jacobi(cpgpu_poisson* P, int* max_iter, double* tol, int norm_ref,

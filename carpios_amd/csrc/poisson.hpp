@@ -61,6 +61,15 @@ struct JState {
   int iter, done;
 };
 
+// Dia_BiCGSTAB state (device): the scalars of solver_matrix.hpp:599-672,
+// the iterations done, the history entries written (the full iterations and
+// a half-step exit), and the exit: stop once the run has ended, half when it
+// ended at the half-step test, code the status it returns.
+struct BState {
+  double normb, tol, resid, rho_1, rho_2, alpha, omega, beta;
+  int it, hl, max_iter, hist_cap, stop, half, code, pad;
+};
+
 // Flexible-CG state (device, mirrored to pinned host memory).
 struct FState {
   double normb;   // ||b|| of the stopping test
@@ -198,8 +207,9 @@ struct cpgpu_poisson {
   double pc_omega = 0.8;
   cpgpu::DevBuf<double> pf[2];
   // Dia_BiCGSTAB (rt, p, ph, v, s, sh, t) and Solver_Jacobi_ vectors
-  cpgpu::DevBuf<double> bv[7];
+  cpgpu::DevBuf<double> bv[8];
   cpgpu::DevBuf<cpgpu::JState> jst;
+  cpgpu::DevBuf<cpgpu::BState> bst;
   cpgpu::DevBuf<cpgpu::FState> fst;
   cpgpu::FState* h_fst = nullptr;  // pinned
   // error norms: the exact field and the interior cell sizes on the device

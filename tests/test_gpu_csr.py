@@ -316,3 +316,61 @@ def test_zero_diagonal_and_bad_input_are_errors(ctx):
     A.close()
     with pytest.raises(ca.CpgpuError):
         ca.CSR(ctx, rp, np.array([0, 1, 0, 2], dtype=np.int32), val)  # column 2 of 2
+
+
+@pytest.mark.parametrize("method", ["CG", "JPCG"])
+def test_device_scalar_cg_stops_inside_a_batch(ctx, method):
+    # CG and Jacobi-PCG keep rho, alpha and the residual on the device and
+    # launch iterations in batches of 16, reading the state once per batch;
+    # iterations after the stop return at entry.  A run capped at k
+    # iterations therefore reports k, its history is the first k entries of
+    # the uncapped run's bit for bit, and its iterate is the one the
+    # uncapped run held after k steps (the oracle's to rounding)
+    # (the C1 operator with a random right-hand side: the golden one takes
+    # plain CG to its tolerance in a single step)
+    g = golden()
+    H, _, _, _, _ = case(g, "c1")
+    b = np.random.default_rng(21).standard_normal(H.n)
+    A = ca.CSR(ctx, H.row_ptr, H.col, H.val)
+    x0 = np.zeros(H.n)
+    code, x, it, t, full = A.solve(method, x0, b, max_iter=20000, tol=1e-10)
+    assert code == 0 and len(full) == it > 40
+    for k in (1, 15, 16, 17, 33):
+        ck, xk, ik, tk, hk = A.solve(method, x0, b, max_iter=k, tol=1e-10)
+        print(f"{method} capped at {k}: code {ck} it {ik} hist {len(hk)} tol {tk:.3e}", flush=True)
+        assert ck == ca.MAXITER and ik == k and len(hk) == k
+        assert np.array_equal(hk, full[:k]) and tk == full[k - 1]
+        _, xo, ho = reassociated_cg(H, b, None, k, 1e-10, method == "JPCG", 0)
+        assert rel_l2(xk, xo) <= 1e-10
+        assert np.allclose(hk, ho, rtol=1e-8, atol=0)
+    # a history shorter than the run: the first hist_cap entries, the count
+    # still the run's
+    ck, xk, ik, tk, hk = A.solve(method, x0, b, max_iter=20000, tol=1e-10, hist_cap=5)
+    assert ck == 0 and ik == it and np.array_equal(hk, full[:5]) and np.array_equal(xk, x)
+    # the initial test: a converged start takes no iteration
+    ck, xk, ik, tk, hk = A.solve(method, x, b, max_iter=100, tol=1e-6)
+    assert ck == 0 and ik == 0 and len(hk) == 0 and np.array_equal(xk, x)
+    A.close()
+
+
+def test_device_scalar_bicgstab_stops_inside_a_batch(ctx):
+    # Dia_BiCGSTAB on the CSR matrix keeps its scalars on the device as CG
+    # does: capped runs report the cap and the uncapped run's history prefix
+    g = golden()
+    H, _, _, _, _ = case(g, "c1")
+    b = np.random.default_rng(22).standard_normal(H.n)
+    A = ca.CSR(ctx, H.row_ptr, H.col, H.val)
+    x0 = np.zeros(H.n)
+    code, x, it, t, full = A.solve("Dia_BiCGSTAB", x0, b, max_iter=20000, tol=1e-12)
+    assert code == 0 and len(full) > 40
+    for k in (1, 15, 16, 17, 33):
+        ck, xk, ik, tk, hk = A.solve("Dia_BiCGSTAB", x0, b, max_iter=k, tol=1e-12)
+        print(f"BiCGSTAB capped at {k}: code {ck} it {ik} hist {len(hk)} tol {tk:.3e}",
+              flush=True)
+        assert ck == ca.MAXITER and ik == k and len(hk) == k
+        assert np.array_equal(hk, full[:k]) and tk == full[k - 1]
+        if k <= 17:
+            _, xo, ho = reassociated_bicgstab(H, b, None, k, 1e-12, 0)
+            assert rel_l2(xk, xo) <= 1e-9
+            assert np.allclose(hk, ho, rtol=1e-7, atol=0)
+    A.close()

@@ -51,8 +51,10 @@ def test_helmholtz_matches_oracle(ctx, nx, ny, alpha, beta, method):
         # 1e-15 perturbations of b (exit code, history length, residual
         # prefix, solution spread).  The half-step exit leaves max_iter at
         # its input (solver_matrix.hpp:642-647); when the device takes the
-        # same kind of exit as the unperturbed oracle run, the 1e-10 field
-        # bound holds as for the other solvers.
+        # same kind of exit as the unperturbed oracle run, its solution is
+        # as close to that run's as the perturbed oracle runs are to it
+        # (band[3]; 4.6e-10 on this grid, whose ~120 steps amplify
+        # rounding), or within 1e-10 as for the other solvers.
         from test_gpu_krylov import check_against_band, oracle_band
         band = oracle_band("Dia_BiCGSTAB", M, M.b, max_iter=20000, tol=1e-11)
         xg = O.field_to_x(g, P.get_phi())
@@ -61,7 +63,7 @@ def test_helmholtz_matches_oracle(ctx, nx, ny, alpha, beta, method):
         print(f"  oracle band lengths {band[1]}..{band[2]}, spread {band[3]:.2e}; "
               f"half-step exit oracle {half_o}, gpu {half_g}", flush=True)
         if half_o == half_g:
-            assert rel(xg, band[0][1]) < 1e-10
+            assert rel(xg, band[0][1]) <= max(1e-10, band[3])
         else:
             # the other exit must itself occur among the oracle's perturbed
             # runs

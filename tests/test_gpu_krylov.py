@@ -173,6 +173,28 @@ def test_bicgstab_max_iter_and_converged_start(ctx):
     P.close()
 
 
+def test_bicgstab_stops_inside_a_batch(ctx):
+    # the scalars stay on the device and the host launches batches of 16
+    # iterations, reading the state once per batch; iterations after the
+    # stop return at entry.  A run capped at k iterations reports k, and its
+    # history is the first k entries of the uncapped run's bit for bit
+    og, A, src, desc = problem(64, 64, [1, 2, 1, 1], seed=2)
+    full = gpu_solve(ctx, og, src, desc, "Dia_BiCGSTAB", max_iter=5000, tol=1e-12,
+                     hist_cap=5000)
+    assert full[0] == 0 and len(full[4]) > 40
+    for k in (1, 15, 16, 17, 33):
+        g = gpu_solve(ctx, og, src, desc, "Dia_BiCGSTAB", max_iter=k, tol=1e-12,
+                      hist_cap=5000)
+        print(f"BiCGSTAB capped at {k}: code {g[0]} max_iter {g[2]} hist {len(g[4])} "
+              f"tol {g[3]:.3e}", flush=True)
+        assert g[0] == ca.MAXITER and g[2] == k and len(g[4]) == k
+        assert np.array_equal(g[4], full[4][:k]) and g[3] == full[4][k - 1]
+        if k <= 17:
+            o = O.solve("Dia_BiCGSTAB", A, A.b, max_iter=k, tol=1e-12)
+            assert o[0] == 1 and rel_l2(g[1], o[1]) <= 1e-9
+            assert np.allclose(g[4], o[4], rtol=1e-7, atol=0)
+
+
 JCASES = [(37, 53, [1, 2, 2, 1]), (20, 30, [2, 1, 2, 1]), (2, 3, [2, 1, 2, 1]),
           (1, 1, [1, 1, 1, 1])]
 
