@@ -379,11 +379,13 @@ def csr_probe(ctx, ca, no_cpu, m=2048, iters=200):
     the device) on the m^2 five-point Dirichlet Laplacian as an explicit CSR
     matrix (sorted columns, 5 m^2 entries): Jacobi-PCG iterations timed
     without convergence, against the algorithmic bytes of one iteration
-    (the product: 12 B per entry + row pointers + x and y once; the vector
-    steps: x, r, p, q, d read and x, r, z written, 64 B per row; the
-    direction: z and the old direction read, the new direction and the
-    product written, 32 B per row) and the oracle's CPU iterations on the
-    same matrix, on the host threads of the CPU baseline."""
+    (the product: 12 B per entry + row pointers; the direction formed with
+    the product: z and the old direction read, the new direction and the
+    product written, 32 B per row; the update: x, r, p, q, d read and x,
+    r, z written, 64 B per row -- the kernels here move 16 B per row more,
+    a separate direction pass that keeps the product's gather to one
+    vector) and the oracle's CPU iterations on the same matrix, on the host
+    threads of the CPU baseline."""
     import scipy.sparse as sp
     n = m * m
     T = sp.diags([-np.ones(m - 1), 4.0 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
@@ -394,17 +396,27 @@ def csr_probe(ctx, ca, no_cpu, m=2048, iters=200):
     nnz = int(rp[-1])
     b = np.random.default_rng(5).standard_normal(n)
     M = ca.CSR(ctx, rp, col, val)
-    M.solve("JPCG", np.zeros(n), b, max_iter=5, tol=1e-300, hist_cap=8)  # warm
-    walls = {}
-    for k in (iters, 2 * iters):
-        t0 = time.perf_counter()
-        M.solve("JPCG", np.zeros(n), b, max_iter=k, tol=1e-300, hist_cap=k + 2)
-        walls[k] = time.perf_counter() - t0
+    def per_iteration(method):
+        M.solve(method, np.zeros(n), b, max_iter=5, tol=1e-300, hist_cap=8)  # warm
+        walls = {}
+        for k in (iters, 2 * iters):
+            t0 = time.perf_counter()
+            M.solve(method, np.zeros(n), b, max_iter=k, tol=1e-300, hist_cap=k + 2)
+            walls[k] = time.perf_counter() - t0
+        # the difference of two run lengths: the iterations alone, without
+        # the copies of x and b and the initial test
+        return (walls[2 * iters] - walls[iters]) / iters, walls[iters]
+
+    t_it, wall = per_iteration("JPCG")
+    t_bi, _ = per_iteration("Dia_BiCGSTAB")
     M.close()
-    # the difference of two run lengths: the iterations alone, without the
-    # copies of x and b and the initial test
-    t_it = (walls[2 * iters] - walls[iters]) / iters
-    wall = walls[iters]
+    # Dia_BiCGSTAB: two products (12 B per entry + row pointers each), the
+    # direction with its product: r, rt, p, v, D^-1 read and p, v written
+    # (56 B per row); s with its product: r, v, D^-1 read and s, t written
+    # (40 B); the update: D^-1, x, rt, p, s, t read and x, r written (64 B).
+    # The kernels here move 40 B per row more (ph and sh written by
+    # elementwise passes and gathered by the products).
+    alg_bi = 2 * (12 * nnz + 4 * (n + 1)) + 160 * n
     alg = 12 * nnz + 4 * (n + 1) + 32 * n + 64 * n
     out = {"matrix": f"{m}^2 five-point Laplacian, CSR, {n} rows, {nnz} entries",
            "method": "JPCG (IC_CG loop with DiaPre), scalars on the device, state read once "
@@ -416,7 +428,10 @@ def csr_probe(ctx, ca, no_cpu, m=2048, iters=200):
            "hbm_frac": round(alg / t_it / 1e9 / HBM_PEAK_GBS, 4),
            "note": "ms_per_iteration from the difference of the two runs' wall times; the "
                    "other figure divides the shorter run's whole call (copies of x and b, the "
-                   "initial residual test) by its iterations"}
+                   "initial residual test) by its iterations",
+           "dia_bicgstab": {"ms_per_iteration": round(t_bi * 1e3, 4),
+                            "algorithmic_bytes_per_iteration": alg_bi,
+                            "hbm_frac": round(alg_bi / t_bi / 1e9 / HBM_PEAK_GBS, 4)}}
     if not no_cpu:
         import ctypes as C
         import oracle as O
@@ -440,6 +455,39 @@ def csr_probe(ctx, ca, no_cpu, m=2048, iters=200):
                                "kind": "port", "sample": f"{k} oracle JPCG iterations on the same CSR"}
         out["speedup_vs_cpu"] = round(t_cpu / t_it, 1)
     log(f"bench: CSR {out}")
+    return out
+
+
+def bicgstab_probe(ctx, ca, m=2048, iters=200):
+    """Dia_BiCGSTAB on the structured operator (matrix-free, the m^2
+    Dirichlet Laplacian with a random source): time per iteration from the
+    difference of two run lengths, against the algorithmic bytes of one
+    iteration (the direction kernel's r, rt, p, v, D^-1 read and p, v
+    written: 56 B per cell; the s/t kernel's r, v, D^-1 read and s, t
+    written: 40 B; the update's D^-1, x, rt, p, s, t read and x, r written:
+    64 B)."""
+    n = m * m
+    desc = ca.uniform_grid_desc(m, m, 1, [1, 1, 1, 1], [0.0, 0.0, 0.0, 0.0], 1.0)
+    P = ca.Poisson(ctx, desc)
+    src = np.zeros((m + 2, m + 2))
+    src[1:-1, 1:-1] = np.random.default_rng(6).standard_normal((m, m))
+    P.set_source(src)
+    P.fill_phi(0.0)
+    P.solve("Dia_BiCGSTAB", max_iter=5, tol=1e-300)  # warm
+    walls = {}
+    for k in (iters, 2 * iters):
+        P.fill_phi(0.0)
+        t0 = time.perf_counter()
+        code, it, _, _ = P.solve("Dia_BiCGSTAB", max_iter=k, tol=1e-300, hist_cap=k + 2)
+        walls[k] = time.perf_counter() - t0
+    P.close()
+    t_it = (walls[2 * iters] - walls[iters]) / iters
+    alg = 160 * n
+    out = {"grid": f"{m}^2 Dirichlet Poisson, matrix-free", "iterations": [iters, 2 * iters],
+           "ms_per_iteration": round(t_it * 1e3, 4), "algorithmic_bytes_per_iteration": alg,
+           "hbm_frac": round(alg / t_it / 1e9 / HBM_PEAK_GBS, 4),
+           "method": "Dia_BiCGSTAB, scalars on the device, state read once per 16 iterations"}
+    log(f"bench: BiCGSTAB {out}")
     return out
 
 
@@ -543,7 +591,8 @@ def main():
     ap.add_argument("--no-kim", action="store_true",
                     help="skip the NS_kim_ step probe")
     ap.add_argument("--no-csr", action="store_true",
-                    help="skip the general-CSR Jacobi-PCG probe")
+                    help="skip the general-CSR Jacobi-PCG / Dia_BiCGSTAB and the structured "
+                         "Dia_BiCGSTAB probes")
     ap.add_argument("--no-hbm", action="store_true",
                     help="skip the 4096^2 Krylov-kernel HBM probe")
     ap.add_argument("--probe", choices=("times", "solve", "unet", "apply"), default="",
@@ -554,7 +603,8 @@ def main():
     ap.add_argument("--variant", type=int, default=-1,
                     help="direction-kernel variant of --probe (default: autotune)")
     ap.add_argument("--only", default="",
-                    help="comma-separated legs to run alone (ns, blob, kim, csr, batch)")
+                    help="comma-separated legs to run alone (ns, blob, kim, csr, bicgstab, "
+                         "batch)")
     ap.add_argument("--ns-steps", type=int, default=100,
                     help="timed steps of the 512^2 NS_explicit_ cavity leg")
     ap.add_argument("--no-ns", action="store_true",
@@ -587,6 +637,7 @@ def main():
                 "blob": lambda: blob_probe(ctx, ca, a.no_cpu),
                 "kim": lambda: kim_probe(ctx, ca, a.no_cpu),
                 "csr": lambda: csr_probe(ctx, ca, a.no_cpu),
+                "bicgstab": lambda: bicgstab_probe(ctx, ca),
                 "batch": lambda: batched(ctx, ca, a.tol, a.no_cpu, not a.no_batch_learned)}
         out = {}
         for name in a.only.split(","):
@@ -832,6 +883,7 @@ def main():
     if not a.no_batch:
         batch = batched(ctx, ca, tol, a.no_cpu, not a.no_batch_learned)
     csr = None if a.no_csr else csr_probe(ctx, ca, a.no_cpu)
+    bicg = None if a.no_csr else bicgstab_probe(ctx, ca)
     kim = None if a.no_kim else kim_probe(ctx, ca, a.no_cpu)
     nsx = None if a.no_ns else ns_explicit_probe(ctx, ca, a.no_cpu, steps=a.ns_steps)
     blob = None if a.no_blob else blob_probe(ctx, ca, a.no_cpu)
@@ -873,6 +925,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_256": batch,
         "csr_jpcg": csr,
+        "dia_bicgstab_2048": bicg,
         "ns_kim_512": kim,
         "ns_explicit_512": nsx,
         "blob_4096": blob,

@@ -11,21 +11,22 @@
 //   x += alpha ph + omega sh;  r = s - omega t;  rho_2 = rho_1
 //   resid = ||r|| / ||b|| < tol: exit;  omega == 0: code 3
 //
-// runs as three operator kernels and three one-block scalar kernels:
-//   dir:   p and v = A ph (ph formed at each column from r, the previous p
-//          and v and D^-1, the stored p's bits), partials of rt.v
+// runs as three operator steps and three one-block scalar kernels:
+//   dir:   p and v = A ph, partials of rt.v
 //   alpha: alpha
-//   st:    s and t = A sh (sh formed at each column from r, v and D^-1),
-//          partials of s.s (the reference nrm2: without the last unknown
-//          when norm_ref), t.s and t.t
+//   st:    s and t = A sh, partials of s.s (the reference nrm2: without the
+//          last unknown when norm_ref), t.s and t.t
 //   omega: the half-step test, else omega
 //   xr:    x and r (the half-step exit: x += alpha ph only), partials of
 //          r.r and of rt.r, the next iteration's rho_1
 //   test:  the stopping tests, rho_2, the next rho_1 and beta
 // Every kernel returns at entry once the state says the run has stopped, so
 // the host launches batches of iterations and reads the state once per
-// batch.  p and v alternate between two buffers (the direction kernel
-// reads the previous ones at the neighbours while it writes the new ones).
+// batch.  The structured operator forms ph (sh) on an LDS tile with its
+// halo inside the product kernel, from the previous p and v, which
+// therefore alternate between two buffers (a tile's halo reads the
+// neighbouring tiles' previous p and v while they write the new ones); the
+// CSR matrix forms ph (sh) in an elementwise pass before its product.
 #pragma once
 
 #include <cmath>

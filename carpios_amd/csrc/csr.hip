@@ -176,59 +176,48 @@ __global__ __launch_bounds__(256) void k_csr_total(const double* part, int G, do
   }
 }
 
-// z = r (no preconditioner) or r * d
-
-// p = z (first) or z + beta p
-
-// x += alpha p; r -= alpha q
-
-// p = r (first) or r + beta (p - omega v); ph = p * d
-
-// s = r - alpha v
-
-// x += a y
-
-// x += alpha ph + omega sh; r = s - omega t
-
 // ---------------------------------------------------------------------
 // CG / Jacobi-PCG with its scalars on the device.  An iteration of
 // solver_matrix.hpp:264-374 (z = M r; rho = r.z; p = z + (rho/rho_1) p;
 // q = A p; alpha = rho/p.q; x += alpha p; r -= alpha q; resid = nrm(r)/|b|)
-// runs as four launches that read and write the state below; iterations
-// after the stop return at entry, so the host launches batches and reads
-// the state once per batch.  z and r.z of the next iteration are formed in
-// the update kernel from the updated r, the values the next iteration's
-// first two steps compute.
+// runs as five launches that read and write the state below (direction,
+// product with p.q, alpha, update with the preconditioner, stopping test);
+// iterations after the stop return at entry, so the host launches batches
+// and reads the state once per batch.  z and r.z of the next iteration are
+// formed in the update kernel from the updated r, the values the next
+// iteration's first two steps compute.  The direction is a pass of its own:
+// forming p at every column of the product would gather z and p_old per
+// matrix entry instead of p alone.
 struct CState {
   double normb, tol, resid, rho, rho_1, alpha;
   int it, max_iter, done, hist_cap;
 };
 static_assert(sizeof(CState) <= 8 * sizeof(double), "the CSR state slot holds 8 doubles");
 
-// p_new = z (first) or z + beta p_old, q = A p_new with p_new at the
-// column entries recomputed from z and p_old (the stored p's bits), and
-// partials of p.q
-__global__ __launch_bounds__(256) void k_csr_dirq(int n, const int* __restrict__ rp,
+// p = z (first) or z + beta p
+__global__ __launch_bounds__(256) void k_csr_cg_p(int n, const double* __restrict__ z,
+                                                  double* __restrict__ p, const CState* st) {
+  if (st->done) return;
+  const bool first = st->it == 0;
+  const double beta = first ? 0.0 : st->rho / st->rho_1;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    p[k] = first ? z[k] : z[k] + beta * p[k];
+}
+
+// q = A p; partials of p.q
+__global__ __launch_bounds__(256) void k_csr_cg_q(int n, const int* __restrict__ rp,
                                                   const int* __restrict__ ci,
                                                   const double* __restrict__ val,
-                                                  const double* __restrict__ z,
-                                                  const double* __restrict__ pold,
-                                                  double* __restrict__ pnew,
+                                                  const double* __restrict__ p,
                                                   double* __restrict__ q, const CState* st,
                                                   double* part) {
   __shared__ double lds[4];
   if (st->done) return;
-  const bool first = st->it == 0;
-  const double beta = first ? 0.0 : st->rho / st->rho_1;
   double v[1] = {0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double acc = row_sum(rp[i], rp[i + 1], ci, val, [&](int c) {
-      return first ? z[c] : z[c] + beta * pold[c];
-    });
-    const double pi = first ? z[i] : z[i] + beta * pold[i];
-    pnew[i] = pi;
-    q[i] = acc;
-    v[0] += pi * acc;
+    const double qi = row_sum(rp[i], rp[i + 1], ci, val, [&](int c) { return p[c]; });
+    q[i] = qi;
+    v[0] += p[i] * qi;
   }
   store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
 }
@@ -308,48 +297,75 @@ __global__ __launch_bounds__(256) void k_csr_rho0(const double* part, int G, CSt
   if (threadIdx.x == 0) st->rho = t[0];
 }
 
-// Dia_BiCGSTAB's operator kernels on the CSR matrix (the scalar kernels and
-// the batch loop are shared with the structured operator).
-// p = r (first) or r + beta (p_old - omega v_old), v = A (p D^-1) with the
-// columns' p formed in place; partials of rt.v
-__global__ __launch_bounds__(256) void k_csr_bicg_dir(
-    int n, const int* __restrict__ rp, const int* __restrict__ ci,
-    const double* __restrict__ val, const double* __restrict__ d, const double* __restrict__ r,
-    const double* __restrict__ rt, const double* __restrict__ pold,
-    const double* __restrict__ vold, double* __restrict__ pnew, double* __restrict__ vnew,
-    const BState* st, double* part) {
-  __shared__ double lds[4];
+// Dia_BiCGSTAB's operator kernels on the CSR matrix (the scalar kernels,
+// the update kernel and the batch loop are shared with the structured
+// operator).
+// p = r (first) or r + beta (p - omega v); ph = p D^-1
+__global__ __launch_bounds__(256) void k_csr_bicg_p(int n, const double* __restrict__ r,
+                                                    double* __restrict__ p,
+                                                    const double* __restrict__ v,
+                                                    const double* __restrict__ d,
+                                                    double* __restrict__ ph, const BState* st) {
   if (st->stop) return;
   const bool first = st->it == 0;
   const double beta = st->beta, omega = st->omega;
-  auto pat = [&](int c) { return first ? r[c] : r[c] + beta * (pold[c] - omega * vold[c]); };
-  double v[1] = {0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double vi = row_sum(rp[i], rp[i + 1], ci, val, [&](int c) { return pat(c) * d[c]; });
-    pnew[i] = pat(i);
-    vnew[i] = vi;
-    v[0] += rt[i] * vi;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const double pv = first ? r[k] : r[k] + beta * (p[k] - omega * v[k]);
+    p[k] = pv;
+    ph[k] = pv * d[k];
   }
-  store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
 }
 
-// s = r - alpha v, t = A (s D^-1) with the columns' s formed in place;
-// partials of s.s (without the last unknown when skip_last), t.s, t.t
-__global__ __launch_bounds__(256) void k_csr_bicg_st(
-    int n, const int* __restrict__ rp, const int* __restrict__ ci,
-    const double* __restrict__ val, const double* __restrict__ d, const double* __restrict__ r,
-    const double* __restrict__ vv, double* __restrict__ sv, double* __restrict__ tv,
-    const BState* st, int skip_last, double* part) {
-  __shared__ double lds[12];
+// v = A ph; partials of rt.v
+__global__ __launch_bounds__(256) void k_csr_bicg_v(int n, const int* __restrict__ rp,
+                                                    const int* __restrict__ ci,
+                                                    const double* __restrict__ val,
+                                                    const double* __restrict__ ph,
+                                                    const double* __restrict__ rt,
+                                                    double* __restrict__ v, const BState* st,
+                                                    double* part) {
+  __shared__ double lds[4];
+  if (st->stop) return;
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double vi = row_sum(rp[i], rp[i + 1], ci, val, [&](int c) { return ph[c]; });
+    v[i] = vi;
+    acc[0] += rt[i] * vi;
+  }
+  store_partials<double, 1>(acc, part, gridDim.x, blockIdx.x, lds);
+}
+
+// s = r - alpha v; sh = s D^-1
+__global__ __launch_bounds__(256) void k_csr_bicg_s(int n, const double* __restrict__ r,
+                                                    const double* __restrict__ v,
+                                                    const double* __restrict__ d,
+                                                    double* __restrict__ sv,
+                                                    double* __restrict__ sh, const BState* st) {
   if (st->stop) return;
   const double alpha = st->alpha;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const double si = r[k] - alpha * v[k];
+    sv[k] = si;
+    sh[k] = si * d[k];
+  }
+}
+
+// t = A sh; partials of s.s (without the last unknown when skip_last), t.s,
+// t.t
+__global__ __launch_bounds__(256) void k_csr_bicg_t(int n, const int* __restrict__ rp,
+                                                    const int* __restrict__ ci,
+                                                    const double* __restrict__ val,
+                                                    const double* __restrict__ sh,
+                                                    const double* __restrict__ sv,
+                                                    double* __restrict__ tv, const BState* st,
+                                                    int skip_last, double* part) {
+  __shared__ double lds[12];
+  if (st->stop) return;
   double v[3] = {0.0, 0.0, 0.0};
   const int m = skip_last ? n - 1 : n;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double ti =
-        row_sum(rp[i], rp[i + 1], ci, val, [&](int c) { return (r[c] - alpha * vv[c]) * d[c]; });
-    const double si = r[i] - alpha * vv[i];
-    sv[i] = si;
+    const double ti = row_sum(rp[i], rp[i + 1], ci, val, [&](int c) { return sh[c]; });
+    const double si = sv[i];
     tv[i] = ti;
     if (i < m) v[0] += si * si;
     v[1] += ti * si;
@@ -427,8 +443,7 @@ int csr_cg(CsrRun& R, double* x, const double* b, int* max_iter, double* tol, bo
   cpgpu_csr* A = R.A;
   const int n = A->n, G = R.G;
   hipStream_t st = R.st;
-  double *r = A->vec[0].p, *z = A->vec[1].p, *q = A->vec[3].p;
-  double* P[2] = {A->vec[2].p, A->vec[4].p};
+  double *r = A->vec[0].p, *z = A->vec[1].p, *p = A->vec[2].p, *q = A->vec[3].p;
   int rc;
   if (precond && (rc = R.diapre())) return rc;
   double normb = 0, resid = 0;
@@ -467,17 +482,17 @@ int csr_cg(CsrRun& R, double* x, const double* b, int* max_iter, double* tol, bo
   double* dh = cap > 0 ? A->dhist.p : nullptr;
   hipLaunchKernelGGL(k_csr_pz, dim3(G), dim3(256), 0, st, n, (const double*)r, dd, z, part);
   hipLaunchKernelGGL(k_csr_rho0, dim3(1), dim3(256), 0, st, (const double*)part, G, dst);
-  constexpr int kBatch = 16;  // even: the direction buffers alternate
+  constexpr int kBatch = 16;
   const int skip = R.norm_ref && n > 1 ? 1 : 0;
   for (;;) {
     for (int k = 0; k < kBatch; ++k) {
-      const double* pold = P[k & 1];
-      double* pnew = P[(k + 1) & 1];
-      hipLaunchKernelGGL(k_csr_dirq, dim3(G), dim3(256), 0, st, n, (const int*)A->rp.p,
-                         (const int*)A->ci.p, (const double*)A->val.p, (const double*)z, pold,
-                         pnew, q, (const CState*)dst, part);
+      hipLaunchKernelGGL(k_csr_cg_p, dim3(G), dim3(256), 0, st, n, (const double*)z, p,
+                         (const CState*)dst);
+      hipLaunchKernelGGL(k_csr_cg_q, dim3(G), dim3(256), 0, st, n, (const int*)A->rp.p,
+                         (const int*)A->ci.p, (const double*)A->val.p, (const double*)p, q,
+                         (const CState*)dst, part);
       hipLaunchKernelGGL(k_csr_alpha, dim3(1), dim3(256), 0, st, (const double*)part, G, dst);
-      hipLaunchKernelGGL(k_csr_updz, dim3(G), dim3(256), 0, st, n, x, r, (const double*)pnew,
+      hipLaunchKernelGGL(k_csr_updz, dim3(G), dim3(256), 0, st, n, x, r, (const double*)p,
                          (const double*)q, dd, z, (const CState*)dst, skip, part);
       hipLaunchKernelGGL(k_csr_test, dim3(1), dim3(256), 0, st, (const double*)part, G, dst,
                          dh);
@@ -508,9 +523,7 @@ int csr_cg(CsrRun& R, double* x, const double* b, int* max_iter, double* tol, bo
 struct CsrBicgOps {
   CsrRun* R;
   double* x;
-  double *r, *rt, *s, *t;
-  double* P[2];
-  double* V[2];
+  double *r, *rt, *p, *v, *s, *t, *ph, *sh;
   const BState* state;
   double* part;
   int G, skip;
@@ -518,25 +531,27 @@ struct CsrBicgOps {
     hipLaunchKernelGGL(k_csr_dots<1>, dim3(G), dim3(256), 0, R->st, (const double*)rt,
                        (const double*)r, nullptr, nullptr, R->A->n, 0, part);
   }
-  void dir(int prev, int cur) {
+  void dir(int, int) {
     cpgpu_csr* A = R->A;
-    hipLaunchKernelGGL(k_csr_bicg_dir, dim3(G), dim3(256), 0, R->st, A->n, (const int*)A->rp.p,
-                       (const int*)A->ci.p, (const double*)A->val.p, (const double*)A->dinv.p,
-                       (const double*)r, (const double*)rt, (const double*)P[prev],
-                       (const double*)V[prev], P[cur], V[cur], state, part);
+    hipLaunchKernelGGL(k_csr_bicg_p, dim3(G), dim3(256), 0, R->st, A->n, (const double*)r, p,
+                       (const double*)v, (const double*)A->dinv.p, ph, state);
+    hipLaunchKernelGGL(k_csr_bicg_v, dim3(G), dim3(256), 0, R->st, A->n, (const int*)A->rp.p,
+                       (const int*)A->ci.p, (const double*)A->val.p, (const double*)ph,
+                       (const double*)rt, v, state, part);
   }
-  void st(int cur) {
+  void st(int) {
     cpgpu_csr* A = R->A;
-    hipLaunchKernelGGL(k_csr_bicg_st, dim3(G), dim3(256), 0, R->st, A->n, (const int*)A->rp.p,
-                       (const int*)A->ci.p, (const double*)A->val.p, (const double*)A->dinv.p,
-                       (const double*)r, (const double*)V[cur], s, t, state, skip, part);
+    hipLaunchKernelGGL(k_csr_bicg_s, dim3(G), dim3(256), 0, R->st, A->n, (const double*)r,
+                       (const double*)v, (const double*)A->dinv.p, s, sh, state);
+    hipLaunchKernelGGL(k_csr_bicg_t, dim3(G), dim3(256), 0, R->st, A->n, (const int*)A->rp.p,
+                       (const int*)A->ci.p, (const double*)A->val.p, (const double*)sh,
+                       (const double*)s, t, state, skip, part);
   }
-  void xr(int cur) {
+  void xr(int) {
     cpgpu_csr* A = R->A;
     hipLaunchKernelGGL(k_bicg_xr, dim3(G), dim3(256), 0, R->st, (size_t)A->n,
-                       (const double*)A->dinv.p, x, r, (const double*)rt,
-                       (const double*)P[cur], (const double*)s, (const double*)t, state,
-                       skip, part);
+                       (const double*)A->dinv.p, x, r, (const double*)rt, (const double*)p,
+                       (const double*)s, (const double*)t, state, skip, part);
   }
 };
 
@@ -566,9 +581,8 @@ int csr_bicgstab(CsrRun& R, double* x, const double* b, int* max_iter, double* t
   const int cap = R.hist ? R.hist_cap : 0;
   if (cap > 0 && A->dhist.alloc(cap)) return set_error(A->ctx, CPGPU_ENOMEM, "CSR history");
   BState* dst = reinterpret_cast<BState*>(A->part.p + 3 * kCsrG + 8);
-  CsrBicgOps ops{&R, x, r, rt, A->vec[6].p, A->vec[7].p, {A->vec[2].p, A->vec[4].p},
-                 {A->vec[3].p, A->vec[5].p}, dst, A->part.p, R.G,
-                 R.norm_ref && n > 1 ? 1 : 0};
+  CsrBicgOps ops{&R, x, r, rt, A->vec[2].p, A->vec[3].p, A->vec[4].p, A->vec[5].p,
+                 A->vec[6].p, A->vec[7].p, dst, A->part.p, R.G, R.norm_ref && n > 1 ? 1 : 0};
   int hl = 0;
   const int code = bicg_run(A->ctx, st, ops, dst, normb, resid, max_iter, tol,
                             cap > 0 ? A->dhist.p : nullptr, cap, R.hist, &hl);

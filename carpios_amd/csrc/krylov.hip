@@ -77,20 +77,10 @@ __global__ __launch_bounds__(256) void k_apply64(Sten<double> s, const double* _
 
 // Dia_BiCGSTAB's operator kernels on the structured operator (the scalar
 // kernels, the update kernel and the batch loop are shared with the CSR
-// matrix).  A row of A x from values formed per cell by f (the reference's
-// CSR term order, apply_global's arithmetic).
-template <typename F>
-__device__ __forceinline__ double apply_fn(const Sten<double>& s, int i, int j, F f) {
-  const int ny = s.ny;
-  const size_t k = (size_t)i * ny + j;
-  const double c = f(k);
-  const double n = i > 0 ? f(k - ny) : 0.0;
-  const double w = j > 0 ? f(k - 1) : 0.0;
-  const double e = j < ny - 1 ? f(k + 1) : 0.0;
-  const double so = i < s.nx - 1 ? f(k + ny) : 0.0;
-  return row_apply(s, i, j, diag_at(s, i, j), c, n, so, w, e, s.KX[i], s.KX[i + 1], s.KY[j],
-                   s.KY[j + 1]);
-}
+// matrix).  Both stage a (TI+2) x (TJ+2) tile with its halo in LDS, as
+// k_cg_dir does: every global load of the tile issued first from clamped
+// valid addresses, ph (or sh) formed once per staged cell, then the rows of
+// A x from LDS in the reference's CSR term order (row_apply).
 
 // dinv = 1 / a_ii (the factor of the reference's ph = p * (1 / a_ii))
 __global__ __launch_bounds__(256) void k_dinv64(Sten<double> s, double* __restrict__ dinv,
@@ -107,69 +97,152 @@ __global__ __launch_bounds__(256) void k_dinv64(Sten<double> s, double* __restri
   }
 }
 
-// p = r (first) or r + beta (p_old - omega v_old), v = A (p D^-1) with the
-// neighbours' p formed in place; partials of rt.v
+// staged element k of this thread: its tile coordinates and clamped global index
+__device__ __forceinline__ size_t stage_index(const Sten<double>& s, int i0, int j0, int k) {
+  const int tid = threadIdx.x + TJ * threadIdx.y;
+  const int e = min(tid + TJ * BY * k, kStageE - 1);
+  const int ii = e / (TJ + 2), jj = e - ii * (TJ + 2);
+  const int gi = min(max(i0 - 1 + ii, 0), s.nx - 1), gj = min(max(j0 - 1 + jj, 0), s.ny - 1);
+  return (size_t)gi * s.ny + gj;
+}
+
+// the face coefficients of the tile's rows and columns into LDS
+__device__ __forceinline__ void stage_coeffs(const Sten<double>& s, int i0, int j0,
+                                             double ckx, double cky, TileLds<double>& L) {
+  const int tid = threadIdx.x + TJ * threadIdx.y;
+  if (tid < TI + 3) L.skx[tid] = ckx;
+  if (tid < TJ + 3) L.sky[tid] = cky;
+}
+
+// row (i, j) of A x from the staged tile
+__device__ __forceinline__ double tile_row(const Sten<double>& s, const TileLds<double>& L,
+                                           int li, int lj, int i, int j) {
+  const double kx0 = L.skx[li], kx1 = L.skx[li + 1], ky0 = L.sky[lj], ky1 = L.sky[lj + 1];
+  const double dg = diag_from(s, i, j, kx0, kx1, ky0, ky1);
+  return row_apply(s, i, j, dg, L.tile[li][lj], L.tile[li - 1][lj], L.tile[li + 1][lj],
+                   L.tile[li][lj - 1], L.tile[li][lj + 1], kx0, kx1, ky0, ky1);
+}
+
+// p = r (first) or r + beta (p_old - omega v_old), v = A (p D^-1);
+// partials of rt.v
 __global__ __launch_bounds__(256) void k_sbicg_dir(
     Sten<double> s, const double* __restrict__ d, const double* __restrict__ r,
     const double* __restrict__ rt, const double* __restrict__ pold,
     const double* __restrict__ vold, double* __restrict__ pnew, double* __restrict__ vnew,
     const BState* st, double* part, Tiling tl) {
   __shared__ double lds[4];
+  __shared__ TileLds<double> L;
   if (st->stop) return;
   const bool first = st->it == 0;
   const double beta = st->beta, omega = st->omega;
-  auto pat = [&](size_t c) {
-    return first ? r[c] : r[c] + beta * (pold[c] - omega * vold[c]);
-  };
+  const int nx = s.nx, ny = s.ny;
+  const int tid = threadIdx.x + TJ * threadIdx.y;
   double v[1] = {0.0};
   for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
     int i0, j0;
     tile_origin(t, tl.ntj, i0, j0);
-    const int j = j0 + threadIdx.x;
+    double br[kStageK], bd[kStageK], bp[kStageK], bv[kStageK];
+#pragma unroll
+    for (int k = 0; k < kStageK; ++k) {
+      const size_t m = stage_index(s, i0, j0, k);
+      br[k] = r[m];
+      bd[k] = d[m];
+      bp[k] = first ? 0.0 : pold[m];
+      bv[k] = first ? 0.0 : vold[m];
+    }
+    const double ckx = s.KX[min(max(i0 - 1 + tid, 0), nx)];
+    const double cky = s.KY[min(max(j0 - 1 + tid, 0), ny)];
+    __syncthreads();  // the previous tile's reads are done
+    stage_coeffs(s, i0, j0, ckx, cky, L);
+#pragma unroll
+    for (int k = 0; k < kStageK; ++k) {
+      const int e = tid + TJ * BY * k;
+      if (e >= kStageE) continue;
+      const int ii = e / (TJ + 2), jj = e - ii * (TJ + 2);
+      const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
+      double ph = 0.0;
+      if (gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
+        const double pv = first ? br[k] : br[k] + beta * (bp[k] - omega * bv[k]);
+        ph = pv * bd[k];
+        if (ii >= 1 && ii <= TI && jj >= 1 && jj <= TJ) pnew[(size_t)gi * ny + gj] = pv;
+      }
+      L.tile[ii][jj] = ph;
+    }
+    __syncthreads();
+    const int lj = threadIdx.x + 1, j = j0 + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < TI / BY; ++k) {
-      const int i = i0 + threadIdx.y + BY * k;
-      if (i < s.nx && j < s.ny) {
-        const size_t idx = (size_t)i * s.ny + j;
-        const double vi = apply_fn(s, i, j, [&](size_t c) { return pat(c) * d[c]; });
-        pnew[idx] = pat(idx);
-        vnew[idx] = vi;
-        v[0] += rt[idx] * vi;
-      }
+      const int li = threadIdx.y + BY * k + 1, i = i0 + li - 1;
+      if (i >= nx || j >= ny) continue;
+      const double a = tile_row(s, L, li, lj, i, j);
+      const size_t idx = (size_t)i * ny + j;
+      vnew[idx] = a;
+      v[0] += rt[idx] * a;
     }
   }
   store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
 }
 
-// s = r - alpha v, t = A (s D^-1) with the neighbours' s formed in place;
-// partials of s.s (norm_ref: without the last unknown), t.s, t.t
+// s = r - alpha v, t = A (s D^-1); partials of s.s (norm_ref: without the
+// last unknown), t.s, t.t
 __global__ __launch_bounds__(256) void k_sbicg_st(
     Sten<double> s, const double* __restrict__ d, const double* __restrict__ r,
     const double* __restrict__ vv, double* __restrict__ sv, double* __restrict__ tv,
     const BState* st, int norm_ref, double* part, Tiling tl) {
   __shared__ double lds[12];
+  __shared__ TileLds<double> L;
+  __shared__ double S[TI][TJ];
   if (st->stop) return;
   const double alpha = st->alpha;
-  const size_t nlast = (size_t)s.nx * s.ny - 1;
+  const int nx = s.nx, ny = s.ny;
+  const int tid = threadIdx.x + TJ * threadIdx.y;
+  const size_t nlast = (size_t)nx * ny - 1;
   double v[3] = {0.0, 0.0, 0.0};
   for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
     int i0, j0;
     tile_origin(t, tl.ntj, i0, j0);
-    const int j = j0 + threadIdx.x;
+    double br[kStageK], bd[kStageK], bv[kStageK];
+#pragma unroll
+    for (int k = 0; k < kStageK; ++k) {
+      const size_t m = stage_index(s, i0, j0, k);
+      br[k] = r[m];
+      bd[k] = d[m];
+      bv[k] = vv[m];
+    }
+    const double ckx = s.KX[min(max(i0 - 1 + tid, 0), nx)];
+    const double cky = s.KY[min(max(j0 - 1 + tid, 0), ny)];
+    __syncthreads();
+    stage_coeffs(s, i0, j0, ckx, cky, L);
+#pragma unroll
+    for (int k = 0; k < kStageK; ++k) {
+      const int e = tid + TJ * BY * k;
+      if (e >= kStageE) continue;
+      const int ii = e / (TJ + 2), jj = e - ii * (TJ + 2);
+      const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
+      double sh = 0.0;
+      if (gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
+        const double si = br[k] - alpha * bv[k];
+        sh = si * bd[k];
+        if (ii >= 1 && ii <= TI && jj >= 1 && jj <= TJ) {
+          sv[(size_t)gi * ny + gj] = si;
+          S[ii - 1][jj - 1] = si;
+        }
+      }
+      L.tile[ii][jj] = sh;
+    }
+    __syncthreads();
+    const int lj = threadIdx.x + 1, j = j0 + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < TI / BY; ++k) {
-      const int i = i0 + threadIdx.y + BY * k;
-      if (i < s.nx && j < s.ny) {
-        const size_t idx = (size_t)i * s.ny + j;
-        const double ti =
-            apply_fn(s, i, j, [&](size_t c) { return (r[c] - alpha * vv[c]) * d[c]; });
-        const double si = r[idx] - alpha * vv[idx];
-        sv[idx] = si;
-        tv[idx] = ti;
-        if (!(norm_ref && idx == nlast)) v[0] += si * si;
-        v[1] += ti * si;
-        v[2] += ti * ti;
-      }
+      const int li = threadIdx.y + BY * k + 1, i = i0 + li - 1;
+      if (i >= nx || j >= ny) continue;
+      const double ti = tile_row(s, L, li, lj, i, j);
+      const double si = S[li - 1][lj - 1];
+      const size_t idx = (size_t)i * ny + j;
+      tv[idx] = ti;
+      if (!(norm_ref && idx == nlast)) v[0] += si * si;
+      v[1] += ti * si;
+      v[2] += ti * ti;
     }
   }
   store_partials<double, 3>(v, part, gridDim.x, blockIdx.x, lds);
