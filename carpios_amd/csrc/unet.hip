@@ -287,6 +287,16 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
   constexpr int SP = 32;                     // window pixel pitch (BF16 values)
   constexpr int NS = XP * XP * 2;            // 16-byte skip chunks of the window
   constexpr int NSP = (NS + 255) / 256;
+  // window element (row, column, channel): a pixel's four 16-byte channel
+  // chunks are stored in the order chunk ^ ((column >> 1) & 3).  A 128-bit
+  // LDS read serves 16 consecutive lanes per cycle; at the plain 64-byte
+  // pitch the 16 pixels of a fragment read meet 2-way bank conflicts, with
+  // this order none (measured with SQ_LDS_BANK_CONFLICT on gfx950; the key
+  // (column >> 2) & 3 leaves them 2-way).  The key depends on the column
+  // only: a column's address serves every row at a constant offset.
+  auto swz = [](int row, int col, int ch) {
+    return (row * XP + col) * SP + (((ch >> 3) ^ ((col >> 1) & 3)) << 3) + (ch & 7);
+  };
   __shared__ __attribute__((aligned(16))) bf16 sc[XP * XP * SP];
   __shared__ __attribute__((aligned(16))) bf16 sd[TP * TP * 16];
   __shared__ __attribute__((aligned(16))) bf16 su[NC * 32];
@@ -358,7 +368,7 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
         const int r = pix / XP, c = pix - r * XP;
         const int gi = ti0 - 2 + r, gj = tj0 - 2 + c;
         const bool in = gi >= 0 && gi < H && gj >= 0 && gj < W;
-        *reinterpret_cast<uint4*>(sc + pix * SP + 16 + ch * 8) = in ? pre[k] : uint4{0u, 0u, 0u, 0u};
+        *reinterpret_cast<uint4*>(sc + swz(r, c, 16 + ch * 8)) = in ? pre[k] : uint4{0u, 0u, 0u, 0u};
       }
     }
     if (tv < NC * 4) {
@@ -392,7 +402,7 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
         const bool ok = fr >= 0 && fr < XP && fc >= 0 && fc < XP;
         const uint32_t h0 = (uint32_t)f2bf(a[0] + bur[0]) | ((uint32_t)f2bf(a[1] + bur[1]) << 16);
         const uint32_t h1 = (uint32_t)f2bf(a[2] + bur[2]) | ((uint32_t)f2bf(a[3] + bur[3]) << 16);
-        bf16* dst = ok ? sc + (fr * XP + fc) * SP + cq : junk + 4 * lane;
+        bf16* dst = ok ? sc + swz(fr, fc, cq) : junk + 4 * lane;
         *reinterpret_cast<uint2*>(dst) = uint2{cin ? h0 : 0u, cin ? h1 : 0u};
       }
     }
@@ -427,7 +437,7 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
       auto frag = [&](int tap, int a) {
         const int dy = tap / 3 - 1, dx = tap % 3 - 1;
         const int R = a < 5 ? min(R0 + a, TP - 1) : Rs, c = a < 5 ? px : cs;
-        return F::load(sc + ((R + 1 + dy) * XP + c + 1 + dx) * SP + kq);
+        return F::load(sc + swz(R + 1 + dy, c + 1 + dx, kq));
       };
       F::T wt = F::load(swm + px * 32 + kq), af[6];
 #pragma unroll
