@@ -348,6 +348,18 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
   // weight fragments fit in registers and are loaded before the staging
   constexpr bool kPre = KCH == 32 && NB <= 2;
   __shared__ __attribute__((aligned(16))) bf16 sx[TP * TP * KCH];
+  // tile element (pixel, channel): with 32 channels (four 16-byte chunks
+  // per pixel) the chunks are stored in the order chunk ^ ((column >> 1) & 3);
+  // a 128-bit LDS read serves 16 consecutive lanes per cycle, and 16
+  // consecutive pixels at the plain 64-byte pitch meet 2-way bank
+  // conflicts, in this order none (measured on gfx950).  The 64-channel
+  // tiles (one block per CU at their level, latency-bound) measured slower
+  // with the key's arithmetic than with the conflicts, and keep the plain
+  // order.
+  auto sxo = [](int pix, int col, int ch) {
+    const int key = CPP == 4 ? (col >> 1) & 3 : 0;
+    return pix * KCH + (((ch >> 3) ^ key) << 3) + (ch & 7);
+  };
   const int tj0 = blockIdx.x * 16, ti0 = blockIdx.y * 16;
   const int H = x.H, W = x.W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -380,8 +392,8 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
     const int e = threadIdx.x + 256 * k;
     if (e < NE) {
       const int pix = e / CPP, ch = e - pix * CPP;
-      *reinterpret_cast<uint4*>(sx + pix * KCH + ch * 8) =
-          pok[k] ? pre[k] : uint4{0u, 0u, 0u, 0u};
+      const int c = pix - (pix / TP) * TP;
+      *reinterpret_cast<uint4*>(sx + sxo(pix, c, ch * 8)) = pok[k] ? pre[k] : uint4{0u, 0u, 0u, 0u};
     }
   }
   __syncthreads();
@@ -398,7 +410,8 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         const int r = wave * 4 + a;
-        const typename F::T af = F::load(sx + ((r + 1 + dy) * TP + px + 1 + dx) * KCH + kq);
+        const typename F::T af =
+            F::load(sx + sxo((r + 1 + dy) * TP + px + 1 + dx, px + 1 + dx, kq));
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(wpre[tap][b], af, acc[a][b]);
       }
@@ -418,7 +431,7 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
       for (int a = 0; a < 4; ++a) {
         const int r = wave * 4 + a;
         const typename F::T af =
-            F::load(sx + ((r + 1 + dy) * TP + px + 1 + dx) * KCH + c0 + kq);
+            F::load(sx + sxo((r + 1 + dy) * TP + px + 1 + dx, px + 1 + dx, c0 + kq));
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(bf[b], af, acc[a][b]);
       }
