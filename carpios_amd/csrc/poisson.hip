@@ -344,17 +344,26 @@ struct CGVecs {
 // tile kernel, variants 1..4 the column march with H = 4, 8, 16, 32.
 // Variants 5 and 6 are the aligned march with H = 8, 16, variants 7..10 the
 // streaming aligned march over chunks of 32 and 64 rows with the XCD-aware
-// unit order, 64 and 128 rows in plain order; 5..10 are paired with the
-// vectorised update kernel when ny is even.
-constexpr int kDirVariants = 11;
+// unit order, 64 and 128 rows in plain order; variants 11..14 the block-wide
+// streaming march over chunks of 16, 32 and 64 rows in XCD order and 32
+// rows in plain order, 15 and 16 over 32 rows (XCD, plain order) with
+// non-temporal stores there and in the update kernel (16-byte vectors: the
+// streaming march of variant 9 when ny is not a multiple of 16 / s); 5..16
+// are paired with the vectorised update kernel when ny is even.
+constexpr int kDirVariants = 17;
 constexpr int kRollRows = 4;  // rows per group of the streaming march
 
 template <typename T, bool PC>
 int dir_blocks(int variant, int nx, int ny, const Tiling& tl) {
   if (variant == 0) return tl.G;
-  static const int Hs[kDirVariants] = {0, 4, 8, 16, 32, 8, 16, 32, 64, 64, 128};
+  static const int Hs[kDirVariants] = {0, 4, 8, 16, 32, 8, 16, 32, 64, 64, 128,
+                                       16, 32, 64, 32, 32, 32};
+  constexpr int V = 16 / (int)sizeof(T);
+  if (variant >= 11 && ny % V != 0) variant = 9;
   int nunits;
-  if (variant >= 7)
+  if (variant >= 11)
+    nunits = make_bmarch<T>(nx, ny, Hs[variant], variant != 14 && variant != 16).items;
+  else if (variant >= 7)
     nunits = make_roll(nx, ny, Hs[variant]).nunits;
   else
     nunits = (variant >= 5 ? make_amarch(nx, ny, Hs[variant]) : make_march(nx, ny, Hs[variant])).nunits;
@@ -386,6 +395,27 @@ void launch_dir(int variant, hipStream_t st, const Sten<T>& s, const T* r,
   CPGPU_LAUNCH(ev, (k_cg_dir_roll<T, PC, kRollRows>), dim3(G), block, 0, st, s, r, \
                pold, pnew, q, kst, partB, partA, hist, hist_cap,                    \
                make_roll(s.nx, s.ny, HC, X), Gprev, partR64, Gref)
+#define CPGPU_DIR_BMARCH(HC, X, NT)                                                       \
+  CPGPU_LAUNCH(ev, (k_cg_dir_bmarch<T, PC, kRollRows, NT>), dim3(G), block, 0, st, s, r, \
+               pold, pnew, q, kst, partB, partA, hist, hist_cap,                      \
+               make_bmarch<T>(s.nx, s.ny, HC, X), Gprev, partR64, Gref)
+  if (variant >= 11) {
+    // 16-byte vectors need ny % V == 0 and aligned vectors; otherwise the
+    // streaming march of variant 9 (any block count G works for it)
+    constexpr int V = 16 / (int)sizeof(T);
+    const uintptr_t al = (uintptr_t)r | (uintptr_t)pold | (uintptr_t)pnew | (uintptr_t)q;
+    if (s.ny % V != 0 || (al & 15)) variant = 9;
+  }
+  switch (variant) {
+    case 11: CPGPU_DIR_BMARCH(16, 1, false); return;
+    case 12: CPGPU_DIR_BMARCH(32, 1, false); return;
+    case 13: CPGPU_DIR_BMARCH(64, 1, false); return;
+    case 14: CPGPU_DIR_BMARCH(32, 0, false); return;
+    case 15: CPGPU_DIR_BMARCH(32, 1, true); return;
+    case 16: CPGPU_DIR_BMARCH(32, 0, true); return;
+    default: break;
+  }
+#undef CPGPU_DIR_BMARCH
   switch (variant) {
     case 1: CPGPU_DIR_MARCH(4); break;
     case 2: CPGPU_DIR_MARCH(8); break;
@@ -412,8 +442,12 @@ void launch_upd(int variant, hipStream_t st, const Sten<T>& s, T* x, const T* p,
   if (variant >= 5 && s.ny % 2 == 0) {
     const int ntj = (s.ny + 127) / 128;
     const int ntiles = ntj * ((s.nx + TI - 1) / TI);
-    CPGPU_LAUNCH(ev, (k_cg_upd2<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st, s, x, p, r, q,
-                 kst, partA, partB, ntiles, ntj, norm_ref, Gprev);
+    if (variant >= 15)
+      CPGPU_LAUNCH(ev, (k_cg_upd2<T, PC, true>), dim3(tl.G), dim3(TJ, BY), 0, st, s, x, p, r,
+                   q, kst, partA, partB, ntiles, ntj, norm_ref, Gprev);
+    else
+      CPGPU_LAUNCH(ev, (k_cg_upd2<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st, s, x, p, r, q,
+                   kst, partA, partB, ntiles, ntj, norm_ref, Gprev);
   } else {
     CPGPU_LAUNCH(ev, (k_cg_upd<T, PC>), dim3(tl.G), dim3(TJ, BY), 0, st, s, x, p, r, q,
                  kst, partA, partB, tl, norm_ref, Gprev);

@@ -926,6 +926,202 @@ __global__ __launch_bounds__(256) void k_cg_dir_roll(
   store_partials<T, 1>(v, partA, gridDim.x, blockIdx.x, lds);
 }
 
+// Block-wide streaming march.  The four wavefronts of a block own W = 256*V
+// aligned consecutive columns [c0, c0 + W) of a chunk of hc rows, V = 16 / s
+// columns per lane, so that every access of r, p_old, p and q is one 16-byte
+// vector per lane, and march down it in groups of R rows as k_cg_dir_roll
+// does (the next group's loads in flight while the current group is
+// formed and applied).  A row's horizontal neighbours come from the lane
+// itself, from the neighbouring lanes (shuffles) and, across the three wave
+// boundaries inside the block, through LDS; only the block's two outer halo
+// columns are read from memory, so the re-read columns are 2 per W instead
+// of 2 per 64, and the re-read rows 2 per chunk.  Needs ny % V == 0 and
+// 16-byte aligned vectors.
+struct BMarch {
+  int nstrip, nchunk, hc, items, xcd;
+};
+
+template <typename T>
+inline BMarch make_bmarch(int nx, int ny, int hc, int xcd) {
+  constexpr int W = 256 * (16 / (int)sizeof(T));
+  BMarch m;
+  m.nstrip = (ny + W - 1) / W;
+  m.hc = hc;
+  m.nchunk = (nx + hc - 1) / hc;
+  m.items = m.nstrip * m.nchunk;
+  m.xcd = xcd;
+  return m;
+}
+
+template <typename T, int R>
+struct BMarchBuf {
+  typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+  vec_t r[R], p[R];
+  T hr[R], hp[R];  // the block's halo column (left on thread 0, right elsewhere)
+  T kx[R + 2];     // KX of rows pbase-1 .. pbase+R
+};
+
+template <typename T, bool PC, int R, bool NT = false>
+__global__ __launch_bounds__(256) void k_cg_dir_bmarch(
+    Sten<T> s, const T* __restrict__ r, const T* __restrict__ pold,
+    T* __restrict__ pnew, T* __restrict__ q, KState* st,
+    const T* __restrict__ partB, T* __restrict__ partA, double* hist,
+    int hist_cap, BMarch mb, int Gprev, const double* __restrict__ partR64,
+    int Gref) {
+  constexpr int V = 16 / (int)sizeof(T);
+  constexpr int W = 256 * V;
+  typedef typename BMarchBuf<T, R>::vec_t vec_t;
+  if (st->done) return;
+  __shared__ T lds[16];
+  // first (xl) and last (xr) p of each wave's columns for the centre rows
+  // of a group, double-buffered by group parity
+  __shared__ T xl[2][R][4], xr[2][R][4];
+  const int nx = s.nx, ny = s.ny;
+  const int lane = threadIdx.x, wv = threadIdx.y;  // blockDim = (64, 4)
+  const int t = lane + 64 * wv;
+  const int G = gridDim.x;
+  const int lb = ((G & 7) || !mb.xcd)
+                     ? (int)blockIdx.x
+                     : (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
+
+  // every load of a group from clamped, valid addresses (masked when used)
+  auto load = [&](BMarchBuf<T, R>& b, int c0, int pbase, int iend) {
+    const int j = min(c0 + V * t, ny - V);
+    const int jh = t == 0 ? max(c0 - 1, 0) : min(c0 + W, ny - 1);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const size_t row = (size_t)min(max(pbase + k, 0), iend - 1) * ny;
+      b.r[k] = *reinterpret_cast<const vec_t*>(r + row + j);
+      b.p[k] = *reinterpret_cast<const vec_t*>(pold + row + j);
+      b.hr[k] = r[row + jh];
+      b.hp[k] = pold[row + jh];
+    }
+#pragma unroll
+    for (int k = 0; k < R + 2; ++k) b.kx[k] = s.KX[min(max(pbase - 1 + k, 0), nx)];
+  };
+
+  BMarchBuf<T, R> bufA, bufB;
+  int u = lb;
+  if (u < mb.items) {
+    const int chunk = u / mb.nstrip, strip = u - chunk * mb.nstrip;
+    load(bufA, strip * W, chunk * mb.hc - 1, min(chunk * mb.hc + mb.hc + 1, nx));
+  }
+  T beta;
+  bool first;
+  if (!dir_prologue<T>(st, partB, Gprev, partR64, Gref, hist, hist_cap, lds,
+                       &beta, &first))
+    return;
+
+  T acc_pq = T(0);
+  for (; u < mb.items; u += G) {  // block-uniform trip count
+    const int chunk = u / mb.nstrip, strip = u - chunk * mb.nstrip;
+    const int r0 = chunk * mb.hc, c0 = strip * W;
+    const int j = c0 + V * t;  // the lane's first column
+    const bool jin = j < ny;   // all V columns or none (ny % V == 0)
+    const int jc = min(j, ny - V);
+    T ky[V + 1];
+#pragma unroll
+    for (int v = 0; v <= V; ++v) ky[v] = s.KY[jc + v];
+    const int jh = t == 0 ? c0 - 1 : c0 + W;
+    const bool hin = (t == 0 && c0 > 0) || (t == 255 && c0 + W < ny);
+    const int jhc = min(max(jh, 0), ny - 1);
+    const T hy0 = s.KY[jhc], hy1 = s.KY[jhc + 1];
+    const int iend = min(r0 + mb.hc + 1, nx);  // p rows r0-1 .. iend-1
+    const int oend = min(r0 + mb.hc, nx);      // output rows r0 .. oend-1
+    if (u != lb) load(bufA, c0, r0 - 1, iend);
+    const int ngroups = ((mb.hc + 2 + 2 * R - 1) / (2 * R)) * 2;
+    // window: w[0], w[1] carry p rows gR-2, gR-1, w[2 .. R+1] the group;
+    // dw the matching diagonals; hw[m] the halo p of window row m+1
+    vec_t w[R + 2], dw[R + 2];
+    T hw[R + 1];
+#pragma unroll
+    for (int v = 0; v < V; ++v) w[0][v] = w[1][v] = dw[0][v] = dw[1][v] = T(0);
+    hw[0] = T(0);
+    auto step = [&](BMarchBuf<T, R>& cur, BMarchBuf<T, R>& nxt, int g) {
+      const int pbase = r0 - 1 + g * R;  // p row of cur slot 0
+      load(nxt, c0, pbase + R, iend);    // past the last group: cache hits
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int i = pbase + k;
+        const bool iok = i >= 0 && i < iend;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const T d = diag_from(s, i, jc + v, cur.kx[k + 1], cur.kx[k + 2], ky[v], ky[v + 1]);
+          const T z = PC ? cur.r[k][v] * (T(1) / d) : cur.r[k][v];
+          const T pv = first ? z : z + beta * cur.p[k][v];
+          w[k + 2][v] = iok && jin ? pv : T(0);
+          dw[k + 2][v] = d;
+        }
+        const T dh = diag_from(s, i, jhc, cur.kx[k + 1], cur.kx[k + 2], hy0, hy1);
+        const T zh = PC ? cur.hr[k] * (T(1) / dh) : cur.hr[k];
+        const T hv = first ? zh : zh + beta * cur.hp[k];
+        hw[k + 1] = iok && hin ? hv : T(0);
+      }
+      const int par = g & 1;
+      if (lane == 0) {
+#pragma unroll
+        for (int m = 0; m < R; ++m) xl[par][m][wv] = w[m + 1][0];
+      }
+      if (lane == 63) {
+#pragma unroll
+        for (int m = 0; m < R; ++m) xr[par][m][wv] = w[m + 1][V - 1];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const int i = pbase - 1 + m;  // output row, centre p = w[m + 1]
+        const vec_t pc = w[m + 1];
+        T sl = __shfl_up(pc[V - 1], 1, 64);
+        T sr = __shfl_down(pc[0], 1, 64);
+        if (lane == 0) sl = wv > 0 ? xr[par][m][wv - 1] : hw[m];
+        if (lane == 63) sr = wv < 3 ? xl[par][m][wv + 1] : hw[m];
+        if (jin && i >= r0 && i < oend) {
+          vec_t qv;
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            const int jv = j + v;
+            const T left = v == 0 ? sl : pc[v - 1];
+            const T right = v == V - 1 ? sr : pc[v + 1];
+            T acc;
+            if constexpr (sizeof(T) == 8) {
+              acc = T(0);
+              if (i > 0) acc = acc + w[m][v] * (-cur.kx[m]);
+              if (jv > 0) acc = acc + left * (-ky[v]);
+              acc = acc + pc[v] * dw[m + 1][v];
+              if (jv < ny - 1) acc = acc + right * (-ky[v + 1]);
+              if (i < nx - 1) acc = acc + w[m + 2][v] * (-cur.kx[m + 1]);
+            } else {
+              acc = row_apply_diff(s, i, jv, pc[v], w[m][v], w[m + 2][v], left, right,
+                                   cur.kx[m], cur.kx[m + 1], ky[v], ky[v + 1]);
+            }
+            qv[v] = acc;
+            acc_pq += pc[v] * acc;
+          }
+          const size_t k = (size_t)i * ny + j;
+          if constexpr (NT) {
+            __builtin_nontemporal_store(pc, reinterpret_cast<vec_t*>(pnew + k));
+            __builtin_nontemporal_store(qv, reinterpret_cast<vec_t*>(q + k));
+          } else {
+            *reinterpret_cast<vec_t*>(pnew + k) = pc;
+            *reinterpret_cast<vec_t*>(q + k) = qv;
+          }
+        }
+      }
+      w[0] = w[R];
+      w[1] = w[R + 1];
+      dw[0] = dw[R];
+      dw[1] = dw[R + 1];
+      hw[0] = hw[R];
+    };
+    for (int g = 0; g < ngroups; g += 2) {
+      step(bufA, bufB, g);
+      step(bufB, bufA, g + 1);
+    }
+  }
+  T v1[1] = {acc_pq};
+  store_partials<T, 1>(v1, partA, gridDim.x, blockIdx.x, lds);
+}
+
 // Update kernel with two consecutive cells per lane (16-byte FP64 / 8-byte
 // FP32 accesses) over TI x 128 tiles; same block count and partial slots as
 // k_cg_upd.  Needs ny even (rows start at vector boundaries).
@@ -940,7 +1136,7 @@ struct Vec2<float> {
   typedef float2 V;
 };
 
-template <typename T, bool PC>
+template <typename T, bool PC, bool NT = false>
 __global__ __launch_bounds__(256) void k_cg_upd2(
     Sten<T> s, T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r,
     const T* __restrict__ q, KState* st, const T* __restrict__ partA,
@@ -985,8 +1181,14 @@ __global__ __launch_bounds__(256) void k_cg_upd2(
       xo.y = xo.y + alpha * pv[k].y;
       ro.x = ro.x - alpha * qv[k].x;
       ro.y = ro.y - alpha * qv[k].y;
-      *reinterpret_cast<V*>(x + idx) = xo;
-      *reinterpret_cast<V*>(r + idx) = ro;
+      if constexpr (NT) {
+        typedef T nv_t __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(nv_t{xo.x, xo.y}, reinterpret_cast<nv_t*>(x + idx));
+        __builtin_nontemporal_store(nv_t{ro.x, ro.y}, reinterpret_cast<nv_t*>(r + idx));
+      } else {
+        *reinterpret_cast<V*>(x + idx) = xo;
+        *reinterpret_cast<V*>(r + idx) = ro;
+      }
       const T z0 = PC ? ro.x * (T(1) / diag_from(s, i, j, kx0[k], kx1[k], ky0, ky1)) : ro.x;
       const T z1 = PC ? ro.y * (T(1) / diag_from(s, i, j + 1, kx0[k], kx1[k], ky1, ky2)) : ro.y;
       v[SLOT_RZ] += ro.x * z0;

@@ -168,8 +168,14 @@ int cpgpu_poisson_set_mixed(cpgpu_poisson *P, double inner_tol, int max_outer);
  * aligned column march (64 aligned columns per wavefront) with 8, 16 rows,
  * 7..10 the streaming aligned march (loads of the next 4 rows in flight
  * while the current ones are computed) over chunks of 32, 64 rows with the
- * XCD-aware unit order and 64, 128 rows in plain order; 5..10 are paired
- * with the two-cells-per-lane update kernel when ny is even. */
+ * XCD-aware unit order and 64, 128 rows in plain order; 11..14 the
+ * block-wide streaming march (a block's four wavefronts march together over
+ * 256 16-byte vectors of a row, wave-boundary neighbours through LDS) over
+ * chunks of 16, 32, 64 rows in XCD order and 32 rows in plain order, 15..16
+ * the same over 32 rows (XCD, plain order) with non-temporal stores of p, q
+ * and of the update's x, r (the streaming march of 9 when ny is not a
+ * multiple of 2 (FP64) / 4 (FP32)); 5..16 are paired with the
+ * two-cells-per-lane update kernel when ny is even. */
 int cpgpu_poisson_set_variant(cpgpu_poisson *P, int v64, int v32);
 /* Autotuner: times reps launches of every variant on this problem (FP64 and
  * FP32), keeps the fastest and reports it with its time per launch (us).

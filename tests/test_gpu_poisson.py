@@ -201,7 +201,7 @@ def test_one_shot_solve_matches_problem_object(ctx):
 def test_direction_kernel_variants_bitwise_equal(ctx, nx, ny):
     og, A, src, desc = both(nx, ny, 1, [1, 2, 1, 2], [0.3, 0, 0, -0.2], seed=3)
     sols = []
-    for variant in range(11):
+    for variant in range(17):
         P = ca.Poisson(ctx, desc)
         P.set_variant(variant, variant)
         P.set_source(src)
@@ -216,6 +216,42 @@ def test_direction_kernel_variants_bitwise_equal(ctx, nx, ny):
         assert rel_l2(O.field_to_x(og, phi), x0) <= 1e-12
 
 
+@pytest.mark.parametrize("nx,ny", [(300, 128), (130, 1030), (70, 2048)])
+def test_block_march_variants_in_fp32_and_fp64(ctx, nx, ny):
+    # the block-wide march (variants 11..16, 16-byte vectors: 2 FP64 or 4
+    # FP32 columns per lane, 512 / 1024 columns per block; 15 and 16 with
+    # non-temporal stores in it and in the update kernel) against the
+    # streaming march of variant 9, in the FP64 iteration and in the FP32
+    # iterations of the mixed solve; ny = 1030 takes the 16-byte path in
+    # FP64 and the fallback in FP32, ny = 2048 spans several blocks
+    og, A, src, desc = both(nx, ny, 1, [1, 2, 1, 2], [0.3, 0, 0, -0.2], seed=5)
+    out = {}
+    for variant in (9, 11, 12, 13, 14, 15, 16):
+        P = ca.Poisson(ctx, desc)
+        P.set_variant(variant, variant)
+        P.set_source(src)
+        code, it, tol, _ = P.solve("JPCG", tol=1e-11)
+        assert code == 0
+        x64 = O.field_to_x(og, P.get_phi())
+        P.fill_phi(0.0)
+        code, itm, tolm, _ = P.solve("MIXED", tol=1e-10, norm=ca.NORM_TRUE)
+        assert code == 0 and tolm <= 1e-10
+        out[variant] = (it, x64, itm, O.field_to_x(og, P.get_phi()))
+        P.close()
+    it9, x9, itm9, xm9 = out[9]
+    for variant in (11, 12, 13, 14, 15, 16):
+        it, x64, itm, xm = out[variant]
+        assert abs(it - it9) <= 1
+        assert rel_l2(x64, x9) <= 1e-12
+        # the FP32 recurrences differ in the order of their dot products
+        # only, and their stop is checked every 16 iterations; across the
+        # eleven older variants the mixed solve of the 300 x 128 problem
+        # takes 1008 or 1152 iterations (a restart after a residual
+        # replacement, or none), so counts are held to 20%
+        assert abs(itm - itm9) <= max(32, itm9 // 5)
+        assert rel_l2(xm, x9) <= 1e-6 and rel_l2(xm9, x9) <= 1e-6
+
+
 def test_autotune_keeps_phi_and_picks_a_variant(ctx):
     og, A, src, desc = c1(256)
     P = ca.Poisson(ctx, desc)
@@ -224,7 +260,7 @@ def test_autotune_keeps_phi_and_picks_a_variant(ctx):
     og.interior(phi0)[:] = 0.125
     P.set_phi(phi0)
     res = P.autotune(reps=3)
-    assert 0 <= res["fp64"][0] <= 10 and 0 <= res["fp32"][0] <= 10
+    assert 0 <= res["fp64"][0] <= 16 and 0 <= res["fp32"][0] <= 16
     assert np.array_equal(og.interior(P.get_phi()), og.interior(phi0))
     P.close()
 
