@@ -217,6 +217,27 @@ def kim_probe(ctx, ca, no_cpu, n=512, steps=10):
     return out
 
 
+def ns_kernel_times(ctx, ns, N):
+    """One more step of `ns` with the NS kernels' own dispatch timestamps
+    against their algorithmic bytes per cell: the fused predictor reads u, v,
+    u f, v f and writes u*, v*, u*f, v*f, div and the projection's source
+    (80 B); the correction reads phi, u*f, v*f, u*, v* and writes u*f, v*f,
+    u*, v*, p, phi (88 B)."""
+    fam = {4: ("k_ns_predict", 80), 6: ("k_grad_correct", 88)}
+    for f in fam:
+        ctx.timer_enable(f)
+    ns.step(1)
+    kern = {}
+    for f, (name, bpc) in fam.items():
+        k, ms = ctx.timer_read(f)
+        ctx.timer_enable(f, False)
+        us = ms * 1e3 / max(k, 1)
+        gbs = bpc * N / (us * 1e-6) / 1e9
+        kern[name] = {"us": round(us, 3), "bytes_per_cell": bpc, "GBs": round(gbs, 1),
+                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return kern
+
+
 def ns_explicit_probe(ctx, ca, no_cpu, n=512, steps=100, warm=2, graphs=True):
     """The plain method on the GPU: the n^2 lid-driven cavity (Re 1000,
     rho = Re, mu = 1, van Leer TVD, CFL 0.5, dt0 = 0.5 h, diffusion number
@@ -245,18 +266,7 @@ def ns_explicit_probe(ctx, ca, no_cpu, n=512, steps=100, warm=2, graphs=True):
     ctx.sync()
     wall = time.perf_counter() - t0
     N = n * n
-    fam = {4: ("k_veo_star", 48), 5: ("k_c2f_div", 48), 6: ("k_grad_correct", 88)}
-    for f in fam:
-        ctx.timer_enable(f)
-    ns.step(1)
-    kern = {}
-    for f, (name, bpc) in fam.items():
-        k, ms = ctx.timer_read(f)
-        ctx.timer_enable(f, False)
-        us = ms * 1e3 / max(k, 1)
-        gbs = bpc * N / (us * 1e-6) / 1e9
-        kern[name] = {"us": round(us, 3), "bytes_per_cell": bpc, "GBs": round(gbs, 1),
-                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    kern = ns_kernel_times(ctx, ns, N)
     t, dt = ns.time()
     it_sum = int(np.sum(its))
     out = {"workload": f"{n}x{n} lid-driven cavity Re=1000, NS_explicit_ (van Leer, CFL 0.5, "
@@ -268,6 +278,15 @@ def ns_explicit_probe(ctx, ca, no_cpu, n=512, steps=100, warm=2, graphs=True):
            "jpcg_algorithmic_GBs": round(it_sum * 10 * N * 8 / wall / 1e9, 1),
            "autotune": tune, "kernels": kern,
            "note": "kernel figures: L2/MALL-resident at this size (16 fields of 2.1 MB)"}
+    # the same kernels where the fields far exceed the caches: one step of
+    # the 4096^2 cavity with the projection cut at 16 iterations (the NS
+    # kernels' work does not depend on the projection's iteration count)
+    m = 4096
+    big = ca.NS(ctx, m, m, 2, (-0.5, 0.5), (-0.5, 0.5),
+                **dict(kw, dt=0.5 / m, max_iter=16))
+    big.step(1)
+    out["kernels_4096"] = ns_kernel_times(ctx, big, m * m)
+    big.close()
     if not no_cpu:
         import oracle as O
         g = O.StructGrid(n, n, 2, -0.5, 0.5, -0.5, 0.5)

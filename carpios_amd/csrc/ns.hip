@@ -4,13 +4,13 @@
 // lib/structure/s_ns_explicit.hpp:118-155) followed by Time_::advance
 // (s_time.hpp:61-64), with every field resident in HBM:
 //   k_maxmag + k_dt ........ NS_::restrict_time (s_ns.hpp:146-169)
-//   k_ghost (u, v) ......... Equation_::apply_bc (s_equation.hpp:340-378)
-//   k_veo_star ............. TVD advection + explicit diffusion for both
-//                            components (s_ns_explicit.hpp:173-229,
-//                            s_operation.hpp:549-620, :664-688)
-//   k_ghost (u*, v*)
-//   k_c2f_div .............. InterpolateC2F + Div (s_operation.hpp:100-120,
-//                            :166-187) in one pass
+//   k_ns_predict ........... in one pass over LDS tiles: Equation_::apply_bc
+//                            of u, v (s_equation.hpp:340-378), TVD advection
+//                            + explicit diffusion for both components
+//                            (s_ns_explicit.hpp:173-229, s_operation.hpp:
+//                            549-620, :664-688), apply_bc of u*, v*,
+//                            InterpolateC2F + Div (s_operation.hpp:100-120,
+//                            :166-187)
 //   Poisson solve .......... the projection, all-Neumann, warm-started from
 //                            the previous phi (s_poisson.hpp:320-420)
 //   k_grad_correct ......... Grad, p = phi/dt, u*f -= grad phi,
@@ -130,15 +130,23 @@ __global__ __launch_bounds__(256) void k_dt(const double* partials, int G,
   }
 }
 
-// Ghost fill of one field (s_equation.hpp:340-378).  One thread per ghost
-// cell; edge ghosts first (z = 0), corners second (z = 1).
-__device__ __forceinline__ void ghost_one(const Axes& a, double* f,
-                                          const BCDev& bc, int i, int j) {
-  int ax, seg, ori, oi, oj;
-  if (i < 0) { ax = 0; seg = 0; ori = -1; oi = 0; oj = j; }
-  else if (i >= a.nx) { ax = 0; seg = 1; ori = 1; oi = a.nx - 1; oj = j; }
-  else if (j < 0) { ax = 1; seg = 2; ori = -1; oi = i; oj = 0; }
-  else { ax = 1; seg = 3; ori = 1; oi = i; oj = a.ny - 1; }
+// Ghost fill of one field (s_equation.hpp:340-378).  A ghost cell (i, j)
+// takes its boundary segment and its source cell (oi, oj) from the first
+// axis on which it lies outside the domain, so a corner's source is an edge
+// ghost; its value is the line through the segment's value at the face and
+// the source value (Interp_::Linear, arithmetic.hpp:428-431) for a
+// Dirichlet segment, source - value * distance for a Neumann one.
+__device__ __forceinline__ void ghost_src(const Axes& a, int i, int j, int& seg, int& oi,
+                                          int& oj) {
+  if (i < 0) { seg = 0; oi = 0; oj = j; }
+  else if (i >= a.nx) { seg = 1; oi = a.nx - 1; oj = j; }
+  else if (j < 0) { seg = 2; oi = i; oj = 0; }
+  else { seg = 3; oi = i; oj = a.ny - 1; }
+}
+
+__device__ __forceinline__ double ghost_formula(const Axes& a, const BCDev& bc, int i, int j,
+                                                int seg, int oi, int oj, double fo) {
+  const int ax = seg < 2 ? 0 : 1, ori = (seg & 1) ? 1 : -1;
   const int line = (ax == 0 ? j : i) + a.gl;
   const int off = seg == 0 ? 0 : seg == 1 ? a.NY : seg == 2 ? 2 * a.NY : 2 * a.NY + a.NX;
   const double val = bc.val ? bc.val[off + line] : 0.0;
@@ -146,14 +154,40 @@ __device__ __forceinline__ void ghost_one(const Axes& a, double* f,
   const double xo = cax(a, ax, ax == 0 ? oi : oj);
   const double so = sax(a, ax, ax == 0 ? oi : oj);
   const double xf = ori > 0 ? xo + so * 0.5 : xo - so * 0.5;
-  const double fo = f[fidx(a, oi, oj)];
   if (bc.type[seg] == CPGPU_BC_DIRICHLET) {
     const double u = (xg - xf) / (xo - xf);
-    f[fidx(a, i, j)] = (val * (1.0 - u)) + (fo * u);
-  } else {
-    const double dl = fabs(xg - xo);
-    f[fidx(a, i, j)] = fo - val * dl;
+    return (val * (1.0 - u)) + (fo * u);
   }
+  const double dl = fabs(xg - xo);
+  return fo - val * dl;
+}
+
+// One thread per ghost cell; edge ghosts first (k_ghost_edges), corners
+// second (k_ghost_corners), which read the edge ghosts.
+__device__ __forceinline__ void ghost_one(const Axes& a, double* f,
+                                          const BCDev& bc, int i, int j) {
+  int seg, oi, oj;
+  ghost_src(a, i, j, seg, oi, oj);
+  f[fidx(a, i, j)] = ghost_formula(a, bc, i, j, seg, oi, oj, f[fidx(a, oi, oj)]);
+}
+
+// The value ghost_one leaves in ghost cell (i, j) (edge or corner) of a
+// field whose interior values f(i, j) gives: an edge ghost from its source
+// cell, a corner from the edge ghost that is its source.
+template <class F>
+__device__ __forceinline__ double ghost_of(const Axes& a, const BCDev& bc, int i, int j,
+                                           const F& f) {
+  int seg, oi, oj;
+  ghost_src(a, i, j, seg, oi, oj);
+  double fo;
+  if (oi >= 0 && oi < a.nx && oj >= 0 && oj < a.ny) {
+    fo = f(oi, oj);
+  } else {
+    int s2, oi2, oj2;
+    ghost_src(a, oi, oj, s2, oi2, oj2);
+    fo = ghost_formula(a, bc, oi, oj, s2, oi2, oj2, f(oi2, oj2));
+  }
+  return ghost_formula(a, bc, i, j, seg, oi, oj, fo);
 }
 
 // edge ghosts of two fields (blockIdx.y selects the field)
@@ -212,20 +246,18 @@ struct TileAt {
   }
 };
 
-// VdotNabla_TVD for one cell and component (s_operation.hpp:549-596)
-template <int LIM, class PHI>
-__device__ __forceinline__ double tvd_cell(const Axes& a,
-                                           const double* __restrict__ uf,
-                                           const double* __restrict__ vf,
-                                           const PHI& phi, int i, int j) {
+// VdotNabla_TVD for one cell and component (s_operation.hpp:549-596); the
+// face velocities through accessors (device fields or LDS tiles)
+template <int LIM, class FU, class FV, class PHI>
+__device__ __forceinline__ double tvd_cell_at(const Axes& a, const FU& uf, const FV& vf,
+                                              const PHI& phi, int i, int j) {
   double vdd[2];
 #pragma unroll
   for (int d = 0; d < 2; ++d) {
-    const double* vface = d == 0 ? uf : vf;
     const int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1;
     const int c = d == 0 ? i : j;
-    const double vfp = vface[fidx(a, i, j)];
-    const double vfm = vface[fidx(a, i - ei, j - ej)];
+    const double vfp = d == 0 ? uf(i, j) : vf(i, j);
+    const double vfm = d == 0 ? uf(i - ei, j - ej) : vf(i - ei, j - ej);
     const double veo = (vfp + vfm) * 0.5;
     int Cm, Um, Dm, Cp, Up, Dp;
     if (veo >= 0.0) {
@@ -257,17 +289,25 @@ __device__ __forceinline__ double tvd_cell(const Axes& a,
   return res;
 }
 
+template <int LIM, class PHI>
+__device__ __forceinline__ double tvd_cell(const Axes& a,
+                                           const double* __restrict__ uf,
+                                           const double* __restrict__ vf,
+                                           const PHI& phi, int i, int j) {
+  return tvd_cell_at<LIM>(a, FieldAt{uf, a}, FieldAt{vf, a}, phi, i, j);
+}
+
 // NablaMuNabla with uniform mu (s_operation.hpp:664-688)
-__device__ __forceinline__ double diffusion_cell(const Axes& a, double mu,
-                                                 const double* __restrict__ phi,
-                                                 int i, int j) {
+template <class PHI>
+__device__ __forceinline__ double diffusion_cell_at(const Axes& a, double mu, const PHI& phi,
+                                                    int i, int j) {
   double td[2];
 #pragma unroll
   for (int d = 0; d < 2; ++d) {
     const int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1, c = d == 0 ? i : j;
-    const double f = phi[fidx(a, i, j)];
-    const double fp = phi[fidx(a, i + ei, j + ej)];
-    const double fm = phi[fidx(a, i - ei, j - ej)];
+    const double f = phi(i, j);
+    const double fp = phi(i + ei, j + ej);
+    const double fm = phi(i - ei, j - ej);
     const double dm = mu * (f - fm) / (cax(a, d, c) - cax(a, d, c - 1));
     const double dp = mu * (fp - f) / (cax(a, d, c + 1) - cax(a, d, c));
     td[d] = (dp - dm) / sax(a, d, c);
@@ -278,33 +318,10 @@ __device__ __forceinline__ double diffusion_cell(const Axes& a, double mu,
   return res;
 }
 
-// u* for both components (s_ns_explicit.hpp:196-211)
-template <int LIM>
-__global__ __launch_bounds__(256) void k_veo_star(Axes a, const double* __restrict__ u,
-                                                  const double* __restrict__ v,
-                                                  const double* __restrict__ uf,
-                                                  const double* __restrict__ vf,
-                                                  double* __restrict__ us,
-                                                  double* __restrict__ vs,
-                                                  const NSDev* st, double rho,
-                                                  double mu) {
-  const double dt = st->dt;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= a.nx || j >= a.ny) return;
-  const size_t f = fidx(a, i, j);
-  {
-    const double adv = tvd_cell<LIM>(a, uf, vf, FieldAt{u, a}, i, j);
-    const double dif = diffusion_cell(a, mu, u, i, j);
-    const double src = 0.0;
-    us[f] = (-adv + (dif + src) / rho) * dt + u[f];
-  }
-  {
-    const double adv = tvd_cell<LIM>(a, uf, vf, FieldAt{v, a}, i, j);
-    const double dif = diffusion_cell(a, mu, v, i, j);
-    const double src = 0.0;
-    vs[f] = (-adv + (dif + src) / rho) * dt + v[f];
-  }
+__device__ __forceinline__ double diffusion_cell(const Axes& a, double mu,
+                                                 const double* __restrict__ phi,
+                                                 int i, int j) {
+  return diffusion_cell_at(a, mu, FieldAt{phi, a}, i, j);
 }
 
 // face value of InterpolateC2F between cells k and k+1 along d
@@ -340,6 +357,302 @@ __global__ __launch_bounds__(256) void k_c2f_div(Axes a, const double* __restric
   dv += (fyp - fym) / sax(a, 1, j);
   div[fidx(a, i, j)] = dv;
   div_int[(size_t)i * a.ny + j] = dv;
+}
+
+// ---------------------------------------------------------------------
+// The explicit predictor in one pass over LDS tiles: apply_bc(u, v), u*, v*
+// (TVD advection + diffusion), apply_bc(u*, v*), InterpolateC2F and Div
+// (s_ns_explicit.hpp:173-229, s_equation.hpp:340-378, s_operation.hpp:
+// 100-120, :166-187).  A block owns a PI x PJ tile of cells.  It stages u and
+// v with a three-cell halo and the face velocities with a two-cell halo,
+// fills the ghost cells of the staged window from the staged interior (the
+// values apply_bc writes), forms u*, v* on the tile and a one-cell ring,
+// fills the ring's ghost cells of u*, v*, and then forms the tile's faces
+// and divergence.  It writes u*, v*, u*f, v*f, div, the projection's
+// interior-only source and the ghost cells of u, v, u*, v* that border its
+// tile (corners included).  Every value is computed with the operands and
+// operation order of the ghost kernels (k_ghost_*), of the cell functions
+// tvd_cell / diffusion_cell, and of k_c2f_div, so the results are the bits a
+// layer-by-layer evaluation gives.
+struct WinAt {
+  const double* t;  // cell (i, j) at t[(i - ib) * pitch + (j - jb)]
+  int ib, jb, pitch;
+  __device__ __forceinline__ double operator()(int i, int j) const {
+    return t[(i - ib) * pitch + (j - jb)];
+  }
+};
+
+// Per-position geometry of one axis over a tile's window, in LDS: for cell
+// k the size s[k]; for face k (between cells k and k+1) the sum
+// ss[k] = s[k] + s[k+1], the TVD ratios rc[k] = ss[k] / s[k] (upwind cell
+// below the face) and rd[k] = ss[k] / s[k+1] (above), the centre spacing
+// dc[k] = c[k+1] - c[k] and the C2F weight sum hh[k] = s[k]/2 + s[k+1]/2.
+// Each is formed once per tile from the same operands as the reference's
+// expressions (addition commutes bit for bit), so it is the value those
+// compute per cell; the TVD ratio R then costs no division per face.
+template <int N>
+struct GeoAxis {
+  double s[N], ss[N], rc[N], rd[N], dc[N], hh[N];
+};
+
+template <int N>
+__device__ __forceinline__ void geo_fill(GeoAxis<N>& g, const double* __restrict__ c,
+                                         const double* __restrict__ sz, int kb, int n, int gl,
+                                         int tid) {
+  for (int t = tid; t < N; t += 256) {
+    const int k = kb + t;
+    const bool ok0 = k >= -gl && k < n + gl, ok1 = k + 1 >= -gl && k + 1 < n + gl;
+    const double s0 = ok0 ? sz[k + gl] : 1.0, s1 = ok1 ? sz[k + 1 + gl] : 1.0;
+    const double c0 = ok0 ? c[k + gl] : 0.0, c1 = ok1 ? c[k + 1 + gl] : 1.0;
+    const double sum = s0 + s1;
+    g.s[t] = s0;
+    g.ss[t] = sum;
+    g.rc[t] = sum / s0;
+    g.rd[t] = sum / s1;
+    g.dc[t] = c1 - c0;
+    g.hh[t] = s0 * 0.5 + s1 * 0.5;
+  }
+}
+
+// tvd_cell_at's arithmetic with the geometry from LDS (GeoAxis of the x and
+// y windows starting at cells xb, yb); the per-cell upwind choice and every
+// rounding are the reference's (s_operation.hpp:549-596, limiters :501-546).
+template <int LIM, int NX_, int NY_, class FU, class FV, class PHI>
+__device__ __forceinline__ double tvd_cell_geo(const GeoAxis<NX_>& gx, const GeoAxis<NY_>& gy,
+                                               int xb, int yb, const FU& uf, const FV& vf,
+                                               const PHI& phi, int i, int j) {
+  double vdd[2];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1;
+    const int c = d == 0 ? i - xb : j - yb;  // position in the axis window
+    const double vfp = d == 0 ? uf(i, j) : vf(i, j);
+    const double vfm = d == 0 ? uf(i - ei, j - ej) : vf(i - ei, j - ej);
+    const double veo = (vfp + vfm) * 0.5;
+#define VAL(o) phi(i + ei * (o), j + ej * (o))
+#define GEO(f, k) (d == 0 ? gx.f[k] : gy.f[k])
+    double fp, fm;
+    if (veo >= 0.0) {
+      // face p: C = c, D = c+1, U = c-1; face m: C = c-1, D = c, U = c-2
+      double vC = VAL(0), vD = VAL(1), vU = VAL(-1);
+      double R = GEO(rc, c);
+      double r = (vC - vU) * GEO(ss, c) / (vD - vC + kSmall) / GEO(ss, c - 1);
+      fp = vC + limiter<LIM>(r, R) / R * (vD - vC);
+      vC = VAL(-1); vD = VAL(0); vU = VAL(-2);
+      R = GEO(rc, c - 1);
+      r = (vC - vU) * GEO(ss, c - 1) / (vD - vC + kSmall) / GEO(ss, c - 2);
+      fm = vC + limiter<LIM>(r, R) / R * (vD - vC);
+    } else {
+      // face p: C = c+1, D = c, U = c+2; face m: C = c, D = c-1, U = c+1
+      double vC = VAL(1), vD = VAL(0), vU = VAL(2);
+      double R = GEO(rd, c);
+      double r = (vC - vU) * GEO(ss, c) / (vD - vC + kSmall) / GEO(ss, c + 1);
+      fp = vC + limiter<LIM>(r, R) / R * (vD - vC);
+      vC = VAL(0); vD = VAL(-1); vU = VAL(1);
+      R = GEO(rd, c - 1);
+      r = (vC - vU) * GEO(ss, c - 1) / (vD - vC + kSmall) / GEO(ss, c);
+      fm = vC + limiter<LIM>(r, R) / R * (vD - vC);
+    }
+    vdd[d] = (vfp * fp - vfm * fm) / GEO(s, c);
+#undef VAL
+#undef GEO
+  }
+  double res = 0.0;
+  res += vdd[0];
+  res += vdd[1];
+  return res;
+}
+
+// diffusion_cell_at's arithmetic with the geometry from LDS
+template <int NX_, int NY_, class PHI>
+__device__ __forceinline__ double diffusion_cell_geo(const GeoAxis<NX_>& gx,
+                                                     const GeoAxis<NY_>& gy, int xb, int yb,
+                                                     double mu, const PHI& phi, int i, int j) {
+  double td[2];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const int ei = d == 0 ? 1 : 0, ej = d == 0 ? 0 : 1;
+    const int c = d == 0 ? i - xb : j - yb;
+    const double f = phi(i, j);
+    const double fp = phi(i + ei, j + ej);
+    const double fm = phi(i - ei, j - ej);
+    const double dm = mu * (f - fm) / (d == 0 ? gx.dc[c - 1] : gy.dc[c - 1]);
+    const double dp = mu * (fp - f) / (d == 0 ? gx.dc[c] : gy.dc[c]);
+    td[d] = (dp - dm) / (d == 0 ? gx.s[c] : gy.s[c]);
+  }
+  double res = 0.0;
+  res += td[0];
+  res += td[1];
+  return res;
+}
+
+// c2f's arithmetic (face between cells k and k+1 of a window) from LDS
+template <int N>
+__device__ __forceinline__ double c2f_geo(const GeoAxis<N>& g, int k, double ck, double ckp) {
+  const double hs = g.s[k] * 0.5;
+  const double hsp = g.s[k + 1] * 0.5;
+  return (ckp * hs + ck * hsp) / g.hh[k];
+}
+
+// The tile of k_ns_predict: PI x PJ cells, u and v staged with a three-cell
+// halo, the faces from two cells below the tile, u*, v* on a one-cell ring.
+template <int PI, int PJ>
+struct PredTile {
+  static constexpr int UWI = PI + 6, UWJ = PJ + 6;  // u, v: tile +- 3
+  static constexpr int FWI = PI + 3, FWJ = PJ + 3;  // faces: from tile - 2
+  static constexpr int RI = PI + 2, RJ = PJ + 2;    // ring: tile +- 1
+  static constexpr int RPER = (RI * RJ + 255) / 256;
+  static constexpr int GX = PI + 6, GY = PJ + 6;    // axis windows: tile +- 3
+};
+
+template <int LIM, int PI, int PJ>
+__global__ __launch_bounds__(256) void k_ns_predict(
+    Axes a, double* __restrict__ u, double* __restrict__ v, const double* __restrict__ uf,
+    const double* __restrict__ vf, double* __restrict__ us, double* __restrict__ vs,
+    double* __restrict__ usf, double* __restrict__ vsf, double* __restrict__ div,
+    double* __restrict__ div_int, BCDev bu, BCDev bv, const NSDev* st, double rho,
+    double mu, int ntj) {
+  typedef PredTile<PI, PJ> TL;
+  __shared__ double su[TL::UWI * TL::UWJ], sv[TL::UWI * TL::UWJ];
+  __shared__ double sf[2][TL::FWI * TL::FWJ];  // u f, v f; later u*, v* on the ring
+  __shared__ GeoAxis<TL::GX> gx;
+  __shared__ GeoAxis<TL::GY> gy;
+  const int tid = threadIdx.x + 64 * threadIdx.y;
+  const int ti = blockIdx.x / ntj, tj = blockIdx.x - ti * ntj;
+  const int i0 = ti * PI, j0 = tj * PJ;
+  const int nx = a.nx, ny = a.ny, gl = a.gl;
+  const int xb = i0 - 3, yb = j0 - 3;  // first cell of the axis windows
+  const double dt = st->dt;
+  auto inbuf = [&](int i, int j) { return i >= -gl && i < nx + gl && j >= -gl && j < ny + gl; };
+  auto inner = [&](int i, int j) { return i >= 0 && i < nx && j >= 0 && j < ny; };
+  // ghost cells bordering the tile: rows [ib, ie) x columns [jb, je) minus
+  // the interior
+  const int ib = ti == 0 ? -gl : i0, ie = i0 + PI >= nx ? nx + gl : i0 + PI;
+  const int jb = tj == 0 ? -gl : j0, je = j0 + PJ >= ny ? ny + gl : j0 + PJ;
+  const bool edge_tile = ti == 0 || tj == 0 || i0 + PI >= nx || j0 + PJ >= ny;
+  // the ring's stencils read rows i0-3 .. i0+PI+2 and columns alike
+  const bool win_edge = i0 < 3 || j0 < 3 || i0 + PI + 2 >= nx || j0 + PJ + 2 >= ny;
+
+  for (int e = tid; e < TL::UWI * TL::UWJ; e += 256) {
+    const int r = e / TL::UWJ, c = e - r * TL::UWJ;
+    const int i = i0 - 3 + r, j = j0 - 3 + c;
+    const bool ok = inbuf(i, j);
+    const size_t f = ok ? fidx(a, i, j) : 0;
+    su[e] = ok ? u[f] : 0.0;
+    sv[e] = ok ? v[f] : 0.0;
+  }
+  for (int e = tid; e < TL::FWI * TL::FWJ; e += 256) {
+    const int r = e / TL::FWJ, c = e - r * TL::FWJ;
+    const int i = i0 - 2 + r, j = j0 - 2 + c;
+    const bool ok = inbuf(i, j);
+    const size_t f = ok ? fidx(a, i, j) : 0;
+    sf[0][e] = ok ? uf[f] : 0.0;
+    sf[1][e] = ok ? vf[f] : 0.0;
+  }
+  geo_fill(gx, a.cx, a.sx, xb, nx, gl, tid);
+  geo_fill(gy, a.cy, a.sy, yb, ny, gl, tid);
+  __syncthreads();
+  const WinAt U{su, i0 - 3, j0 - 3, TL::UWJ}, V{sv, i0 - 3, j0 - 3, TL::UWJ};
+  // apply_bc(u, v): the window's ghost cells (reads interior cells only,
+  // writes ghost cells only) and the tile's own ghost cells in memory
+  if (win_edge) {
+    for (int e = tid; e < TL::UWI * TL::UWJ; e += 256) {
+      const int r = e / TL::UWJ, c = e - r * TL::UWJ;
+      const int i = i0 - 3 + r, j = j0 - 3 + c;
+      if (inbuf(i, j) && !inner(i, j)) {
+        su[e] = ghost_of(a, bu, i, j, U);
+        sv[e] = ghost_of(a, bv, i, j, V);
+      }
+    }
+  }
+  if (edge_tile) {
+    const int w = je - jb;
+    for (int e = tid; e < (ie - ib) * w; e += 256) {
+      const int i = ib + e / w, j = jb + e % w;
+      if (!inner(i, j)) {
+        u[fidx(a, i, j)] = ghost_of(a, bu, i, j, U);
+        v[fidx(a, i, j)] = ghost_of(a, bv, i, j, V);
+      }
+    }
+  }
+  __syncthreads();
+  // u*, v* on the ring's interior cells (s_ns_explicit.hpp:196-211)
+  const WinAt UF{sf[0], i0 - 2, j0 - 2, TL::FWJ}, VF{sf[1], i0 - 2, j0 - 2, TL::FWJ};
+  double ru[TL::RPER], rv[TL::RPER];
+#pragma unroll
+  for (int k = 0; k < TL::RPER; ++k) {
+    const int e = tid + 256 * k;
+    const int r = e / TL::RJ, c = e - r * TL::RJ;
+    const int i = i0 - 1 + r, j = j0 - 1 + c;
+    ru[k] = rv[k] = 0.0;
+    if (e < TL::RI * TL::RJ && inner(i, j)) {
+      const double src = 0.0;
+      double adv = tvd_cell_geo<LIM>(gx, gy, xb, yb, UF, VF, U, i, j);
+      double dif = diffusion_cell_geo(gx, gy, xb, yb, mu, U, i, j);
+      ru[k] = (-adv + (dif + src) / rho) * dt + U(i, j);
+      adv = tvd_cell_geo<LIM>(gx, gy, xb, yb, UF, VF, V, i, j);
+      dif = diffusion_cell_geo(gx, gy, xb, yb, mu, V, i, j);
+      rv[k] = (-adv + (dif + src) / rho) * dt + V(i, j);
+    }
+  }
+  __syncthreads();  // every read of the face window done
+  double* rus = sf[0];
+  double* rvs = sf[1];
+#pragma unroll
+  for (int k = 0; k < TL::RPER; ++k) {
+    const int e = tid + 256 * k;
+    if (e < TL::RI * TL::RJ) {
+      rus[e] = ru[k];
+      rvs[e] = rv[k];
+    }
+  }
+  __syncthreads();
+  const WinAt US{rus, i0 - 1, j0 - 1, TL::RJ}, VS{rvs, i0 - 1, j0 - 1, TL::RJ};
+  if (edge_tile) {
+    // apply_bc(u*, v*): the ring's edge ghosts (the faces read them) and the
+    // tile's own ghost cells in memory
+    for (int e = tid; e < TL::RI * TL::RJ; e += 256) {
+      const int r = e / TL::RJ, c = e - r * TL::RJ;
+      const int i = i0 - 1 + r, j = j0 - 1 + c;
+      const bool iin = i >= 0 && i < nx, jin = j >= 0 && j < ny;
+      if (inbuf(i, j) && iin != jin) {
+        rus[e] = ghost_of(a, bu, i, j, US);
+        rvs[e] = ghost_of(a, bv, i, j, VS);
+      }
+    }
+    const int w = je - jb;
+    for (int e = tid; e < (ie - ib) * w; e += 256) {
+      const int i = ib + e / w, j = jb + e % w;
+      if (!inner(i, j)) {
+        us[fidx(a, i, j)] = ghost_of(a, bu, i, j, US);
+        vs[fidx(a, i, j)] = ghost_of(a, bv, i, j, VS);
+      }
+    }
+  }
+  __syncthreads();
+  // faces and divergence of the tile (k_c2f_div's arithmetic)
+  for (int e = tid; e < PI * PJ; e += 256) {
+    const int i = i0 + e / PJ, j = j0 + e % PJ;
+    if (i >= nx || j >= ny) continue;
+    const int cx = i - xb, cy = j - yb;
+    const double uc = US(i, j), vc = VS(i, j);
+    const double fxp = c2f_geo(gx, cx, uc, US(i + 1, j));
+    const double fxm = c2f_geo(gx, cx - 1, US(i - 1, j), uc);
+    const double fyp = c2f_geo(gy, cy, vc, VS(i, j + 1));
+    const double fym = c2f_geo(gy, cy - 1, VS(i, j - 1), vc);
+    const size_t f = fidx(a, i, j);
+    us[f] = uc;
+    vs[f] = vc;
+    usf[f] = fxp;
+    vsf[f] = fyp;
+    if (i == 0) usf[fidx(a, -1, j)] = fxm;
+    if (j == 0) vsf[fidx(a, i, -1)] = fym;
+    double dv = 0.0;
+    dv += (fxp - fxm) / gx.s[cx];
+    dv += (fyp - fym) / gy.s[cy];
+    div[f] = dv;
+    div_int[(size_t)i * ny + j] = dv;
+  }
 }
 
 // Grad of phi (Neumann value 0 ghosts equal the boundary cell), p = phi/dt,
@@ -695,15 +1008,27 @@ int ns_upload(cpgpu_ctx* ctx, void* dst, const void* src, size_t bytes) {
   return CPGPU_OK;
 }
 
-template <int LIM>
-void launch_veo(cpgpu_ns* S, dim3 g, dim3 b) {
+template <int LIM, int PI, int PJ>
+void launch_predict_t(cpgpu_ns* S, const BCDev& bu, const BCDev& bv) {
+  const int ntj = (S->ny + PJ - 1) / PJ, nti = (S->nx + PI - 1) / PI;
   const EvPair ev = timer_events(S->ctx, 4);
-  CPGPU_LAUNCH(ev, k_veo_star<LIM>, g, b, 0, S->ctx->stream, S->axes(),
-               S->fld(F_U), S->fld(F_V), S->fld(F_UF), S->fld(F_VF),
-               S->fld(F_US), S->fld(F_VS), S->st.p, S->rho, S->mu);
+  CPGPU_LAUNCH(ev, (k_ns_predict<LIM, PI, PJ>), dim3(nti * ntj), dim3(64, 4), 0,
+               S->ctx->stream, S->axes(), S->fld(F_U), S->fld(F_V), S->fld(F_UF),
+               S->fld(F_VF), S->fld(F_US), S->fld(F_VS), S->fld(F_USF), S->fld(F_VSF),
+               S->fld(F_DIV), S->div_int.p, bu, bv, S->st.p, S->rho, S->mu, ntj);
 }
 
-int ns_project(cpgpu_ns* S, bool kim);
+// 8 x 64 tiles from 1024^2 cells on, 8 x 32 below (more blocks); measured
+// at 4096^2 against 16 x 64 and 8 x 32 tiles: 1.09, 1.20, 1.15 ms
+template <int LIM>
+void launch_predict(cpgpu_ns* S, const BCDev& bu, const BCDev& bv) {
+  if ((size_t)S->nx * S->ny >= (size_t)1024 * 1024)
+    launch_predict_t<LIM, 8, 64>(S, bu, bv);
+  else
+    launch_predict_t<LIM, 8, 32>(S, bu, bv);
+}
+
+int ns_project(cpgpu_ns* S, bool kim, bool faces_done = false);
 
 int ns_one_step(cpgpu_ns* S) {
   cpgpu_ctx* ctx = S->ctx;
@@ -719,40 +1044,32 @@ int ns_one_step(cpgpu_ns* S) {
                      S->fld(F_V), S->part.p);
   hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, st, S->part.p, Gm, S->st.p,
                      S->cfl, S->min_size, S->dnum, S->ddenom, S->dcoef);
-  // u*, v*
-  const dim3 gedge(64, 2), gcorner(1, 2);
-  hipLaunchKernelGGL(k_ghost_edges, gedge, dim3(256), 0, st, a, S->fld(F_U), bu,
-                     S->fld(F_V), bv);
-  hipLaunchKernelGGL(k_ghost_corners, gcorner, dim3(256), 0, st, a, S->fld(F_U),
-                     bu, S->fld(F_V), bv);
-  const dim3 b2(64, 4), g2((S->ny + 63) / 64, (S->nx + 3) / 4);
+  // apply_bc(u, v), u*, v*, apply_bc(u*, v*), faces and divergence
   switch (S->limiter) {
-    case 0: launch_veo<0>(S, g2, b2); break;
-    case 1: launch_veo<1>(S, g2, b2); break;
-    case 2: launch_veo<2>(S, g2, b2); break;
-    case 3: launch_veo<3>(S, g2, b2); break;
-    case 4: launch_veo<4>(S, g2, b2); break;
-    default: launch_veo<5>(S, g2, b2); break;
+    case 0: launch_predict<0>(S, bu, bv); break;
+    case 1: launch_predict<1>(S, bu, bv); break;
+    case 2: launch_predict<2>(S, bu, bv); break;
+    case 3: launch_predict<3>(S, bu, bv); break;
+    case 4: launch_predict<4>(S, bu, bv); break;
+    default: launch_predict<5>(S, bu, bv); break;
   }
-  hipLaunchKernelGGL(k_ghost_edges, gedge, dim3(256), 0, st, a, S->fld(F_US),
-                     bu, S->fld(F_VS), bv);
-  hipLaunchKernelGGL(k_ghost_corners, gcorner, dim3(256), 0, st, a,
-                     S->fld(F_US), bu, S->fld(F_VS), bv);
   CPGPU_HIP(ctx, hipGetLastError());
-  return ns_project(S, false);
+  return ns_project(S, false, true);
 }
 
-// C2F + Div of u*, the projection solve, grad phi, p, the corrections and
-// the time advance; u <- u*, uf <- u*f.
-int ns_project(cpgpu_ns* S, bool kim) {
+// C2F + Div of u* (unless the predictor formed them), the projection solve,
+// grad phi, p, the corrections and the time advance; u <- u*, uf <- u*f.
+int ns_project(cpgpu_ns* S, bool kim, bool faces_done) {
   cpgpu_ctx* ctx = S->ctx;
   hipStream_t st = ctx->stream;
   const Axes a = S->axes();
   const dim3 b2(64, 4), g2((S->ny + 63) / 64, (S->nx + 3) / 4);
-  const EvPair ev5 = timer_events(ctx, 5);
-  CPGPU_LAUNCH(ev5, k_c2f_div, g2, b2, 0, st, a, S->fld(F_US), S->fld(F_VS),
-               S->fld(F_USF), S->fld(F_VSF), S->fld(F_DIV), S->div_int.p);
-  CPGPU_HIP(ctx, hipGetLastError());
+  if (!faces_done) {
+    const EvPair ev5 = timer_events(ctx, 5);
+    CPGPU_LAUNCH(ev5, k_c2f_div, g2, b2, 0, st, a, S->fld(F_US), S->fld(F_VS),
+                 S->fld(F_USF), S->fld(F_VSF), S->fld(F_DIV), S->div_int.p);
+    CPGPU_HIP(ctx, hipGetLastError());
+  }
   // projection
   int rc = poisson_rhs_from_device(S->P, S->div_int.p, 0.0);
   if (rc) return rc;

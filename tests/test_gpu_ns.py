@@ -52,7 +52,8 @@ def compare(g, ora, gpu, tol=1e-12):
 
 @pytest.mark.parametrize("nx,ny,lim", [(48, 48, "VanLeer"), (37, 53, "superbee"),
                                        (64, 32, "QUICK"), (40, 40, "upwind2"),
-                                       (44, 36, "WAHYD"), (30, 50, "CUI")])
+                                       (44, 36, "WAHYD"), (30, 50, "CUI"),
+                                       (33, 130, "VanLeer"), (17, 67, "superbee")])
 def test_cavity_steps_match_oracle(ctx, nx, ny, lim):
     g, ora, gpu = pair(ctx, nx, ny, lim=lim)
     steps = 12
@@ -68,15 +69,21 @@ def test_cavity_steps_match_oracle(ctx, nx, ny, lim):
     print("rel L2:", {k: f"{v:.1e}" for k, v in errs.items()})
     for k, v in errs.items():
         assert v <= 1e-12, (k, v)
+    # dt = CFL h / max|u| inherits the fields' agreement (the projections'
+    # dot products are summed in another order, so u agrees to ~1e-14, and
+    # dt may differ from the oracle's in its last bits)
     t, dt = gpu.time()
-    assert dt == ora.dt and abs(t - ora.st.t) <= 1e-15 * steps
+    assert abs(dt - ora.dt) <= 1e-12 * ora.dt and abs(t - ora.st.t) <= 1e-12 * ora.st.t
     gpu.close()
 
 
-def test_first_step_bitwise_until_projection(ctx):
+@pytest.mark.parametrize("nx,ny", [(33, 29), (33, 130), (70, 65), (16, 64), (19, 200)])
+def test_first_step_bitwise_until_projection(ctx, nx, ny):
     # before the Krylov solve everything is elementwise in the reference
-    # order: u*, v* and the divergence agree bit for bit
-    g, ora, gpu = pair(ctx, 33, 29)
+    # order: the divergence agrees bit for bit (the fused predictor's 16 x 64
+    # tiles: ny = 130 and 200 span several tile columns, nx = 33 has a tile
+    # row whose stencils reach the ghost rows without owning them)
+    g, ora, gpu = pair(ctx, nx, ny)
     ora.step()
     gpu.step(1)
     for name in ("div",):
