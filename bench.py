@@ -79,26 +79,42 @@ def batched(ctx, ca, tol, no_cpu, learned_net=True):
     sols, iters = {}, {}
     for method, s in (("JPCG", 8), ("MIXED", 4)):
         tpb, ms = B.autotune(method, reps=2)
-        for P in probs:
-            P.fill_phi(0.0)
-        ctx.sync()
-        t = time.perf_counter()
-        code, it, r, codes, lat = B.solve(method, max_iter=10 ** 6, tol=tol)
-        wall = time.perf_counter() - t
-        if code != ca.OK or r.max() > tol:
-            raise RuntimeError(f"batched {method}: code {code}, max residual {r.max():.3e}")
-        byts = float((it.astype(np.float64) * 11 * cells * s).sum())
-        res[method] = {"solves_per_s": round(len(probs) / wall, 2), "wall_s": round(wall, 4),
-                       "cells_per_s": round(float(cells.sum()) / wall, 1),
-                       "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
-                       "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
-                       "iterations_total": int(it.sum()), "iterations_max": int(it.max()),
-                       "tiles_per_block": tpb,
-                       "autotune_ms_per_iter": [round(float(v), 4) for v in ms],
-                       "algorithmic_GBs": round(byts / wall / 1e9, 1),
-                       "hbm_frac": round(byts / wall / 1e9 / HBM_PEAK_GBS, 4)}
-        sols[method] = [P.get_phi() for P in probs]
-        iters[method] = it.copy()
+        runs = {}
+        # graph-replayed batches polled by the host; the FP64 pass also as
+        # one persistent cooperative launch (bitwise the same results; the
+        # mixed pass has no persistent form in use)
+        for mode in (("launched", "persistent") if method == "JPCG" else ("launched",)):
+            B.set_persistent(mode == "persistent")
+            for P in probs:
+                P.fill_phi(0.0)
+            ctx.sync()
+            t = time.perf_counter()
+            code, it, r, codes, lat = B.solve(method, max_iter=10 ** 6, tol=tol)
+            wall = time.perf_counter() - t
+            if code != ca.OK or r.max() > tol:
+                raise RuntimeError(f"batched {method} ({mode}): code {code}, "
+                                   f"max residual {r.max():.3e}")
+            byts = float((it.astype(np.float64) * 11 * cells * s).sum())
+            runs[mode] = ({"solves_per_s": round(len(probs) / wall, 2), "wall_s": round(wall, 4),
+                           "cells_per_s": round(float(cells.sum()) / wall, 1),
+                           "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+                           "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+                           "iterations_total": int(it.sum()), "iterations_max": int(it.max()),
+                           "tiles_per_block": tpb,
+                           "algorithmic_GBs": round(byts / wall / 1e9, 1),
+                           "hbm_frac": round(byts / wall / 1e9 / HBM_PEAK_GBS, 4)},
+                          [P.get_phi() for P in probs], it.copy())
+        res[method] = dict(runs["launched"][0],
+                           launch="graph-replayed batches of 16 iterations, host-polled",
+                           autotune_ms_per_iter=[round(float(v), 4) for v in ms])
+        if "persistent" in runs:
+            res[method]["persistent_launch"] = dict(
+                runs["persistent"][0], launch="one persistent cooperative launch",
+                bitwise_equal_to_launched=all(
+                    np.array_equal(a_, b_) for a_, b_ in
+                    zip(runs["launched"][1], runs["persistent"][1])))
+        sols[method] = runs["launched"][1]
+        iters[method] = runs["launched"][2]
         log(f"bench: batched {method}: {res[method]}")
     errs = [rel_l2(a_, b_) for a_, b_ in zip(sols["MIXED"], sols["JPCG"])]
     res["MIXED"]["rel_l2_vs_fp64_max"] = float(max(errs))

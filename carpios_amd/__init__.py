@@ -71,6 +71,7 @@ EXPORTS = [
     "cpgpu_ns_set_time", "cpgpu_ns_set_solver", "cpgpu_poisson_set_fcg",
     "cpgpu_poisson_error_norms", "cpgpu_ns_error_norms", "cpgpu_batch_create",
     "cpgpu_batch_destroy", "cpgpu_batch_set_tiles", "cpgpu_batch_set_graphs",
+    "cpgpu_batch_set_persistent",
     "cpgpu_batch_solve", "cpgpu_batch_autotune", "cpgpu_batch_draw",
     "cpgpu_ns_field_change", "cpgpu_unet_set_conv_shape", "cpgpu_unet_set_fusion", "cpgpu_advection_create",
     "cpgpu_advection_destroy", "cpgpu_advection_set_field", "cpgpu_advection_get_field",
@@ -723,6 +724,10 @@ class Batch:
 
     def set_graphs(self, on):
         self.ctx.check(lib().cpgpu_batch_set_graphs(self.h, C.c_int(int(bool(on)))))
+
+    def set_persistent(self, on):
+        """Every batched pass as one persistent cooperative launch."""
+        self.ctx.check(lib().cpgpu_batch_set_persistent(self.h, C.c_int(int(bool(on)))))
 
     def autotune(self, method="JPCG", reps=2):
         best = C.c_int(0)

@@ -70,6 +70,8 @@ struct cpgpu_batch {
   double clock_khz = 100000.0;
   // instantiated batch graphs: (mode, Bcur) -> exec
   std::vector<std::pair<std::pair<int, int>, hipGraphExec_t>> graphs;
+  bool persistent = false;  // one cooperative launch per batched pass
+  DevBuf<unsigned> gbar;    // its grid barrier (generation, abort, per-block flags)
   ~cpgpu_batch() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
     if (h_st) (void)hipHostFree(h_st);
@@ -143,13 +145,14 @@ __global__ void k_b_t0(unsigned long long* ts, int n) {
 }
 
 // r = b - A x and the partials of r.z, r.r, b.b (true norms)
+// The kernels' bodies are device functions of the logical block index lb
+// (the launch index of a plain launch, the loop index of the persistent
+// solve), so both run the same code on the same block map.
 template <typename T>
-__global__ __launch_bounds__(256) void k_b_init(const BProb* __restrict__ probs,
-                                                const int* __restrict__ bmap,
-                                                T* __restrict__ partB, int Btot) {
+__device__ __forceinline__ void b_init(const BProb* __restrict__ probs, T* __restrict__ partB,
+                                       int Btot, int pr, int rank) {
   __shared__ T lds[16];
-  const BProb& P = probs[bmap[blockIdx.x]];
-  const int rank = blockIdx.x - P.blk0;
+  const BProb& P = probs[pr];
   const Sten<T> s = bsten<T>(P);
   const BVecs<T> v = bvecs<T>(P);
   T acc[3] = {T(0), T(0), T(0)};
@@ -174,24 +177,26 @@ __global__ __launch_bounds__(256) void k_b_init(const BProb* __restrict__ probs,
   store_partials<T, 3>(acc, partB + P.pofs, Btot, rank, lds);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_b_init(const BProb* __restrict__ probs,
+                                                const int* __restrict__ bmap,
+                                                T* __restrict__ partB, int Btot) {
+  const int p = bmap[blockIdx.x];
+  b_init<T>(probs, partB, Btot, p, (int)blockIdx.x - probs[p].blk0);
+}
+
 // Direction kernel of the live problem's iteration it = st->it_dir:
 // prologue (stopping test of iteration it-1, beta), then p = z + beta p_old
 // over an LDS tile with halo, q = A p, partial p.q.  The body is the
 // single-problem direction kernel's.
 template <typename T>
-__global__ __launch_bounds__(256) void k_b_dir(const BProb* __restrict__ probs,
-                                               const int* __restrict__ bmap,
-                                               KState* __restrict__ sts,
-                                               const T* __restrict__ partB,
-                                               T* __restrict__ partA,
-                                               const double* __restrict__ partR64,
-                                               int Btot, int flip,
-                                               unsigned long long* __restrict__ ts) {
-  const int p = bmap[blockIdx.x];
+__device__ __forceinline__ void b_dir(const BProb* __restrict__ probs, KState* __restrict__ sts,
+                                      const T* __restrict__ partB, T* __restrict__ partA,
+                                      const double* __restrict__ partR64, int Btot, int flip,
+                                      unsigned long long* __restrict__ ts, int p, int rank) {
   KState* st = sts + p;
   if (st->done) return;
   const BProb& P = probs[p];
-  const int rank = blockIdx.x - P.blk0;
   __shared__ T lds[16];
   __shared__ double lds64[4];
   __shared__ TileLds<T> L;
@@ -258,19 +263,28 @@ __global__ __launch_bounds__(256) void k_b_dir(const BProb* __restrict__ probs,
   store_partials<T, 1>(acc, partA + P.pofs, Btot, rank, lds);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_b_dir(const BProb* __restrict__ probs,
+                                               const int* __restrict__ bmap,
+                                               KState* __restrict__ sts,
+                                               const T* __restrict__ partB,
+                                               T* __restrict__ partA,
+                                               const double* __restrict__ partR64,
+                                               int Btot, int flip,
+                                               unsigned long long* __restrict__ ts) {
+  const int p = bmap[blockIdx.x];
+  b_dir<T>(probs, sts, partB, partA, partR64, Btot, flip, ts, p, (int)blockIdx.x - probs[p].blk0);
+}
+
 // Update kernel of iteration it = st->it_upd: alpha = rho / p.q; x += alpha p;
 // r -= alpha q; partials of r.z, r.r.
 template <typename T>
-__global__ __launch_bounds__(256) void k_b_upd(const BProb* __restrict__ probs,
-                                               const int* __restrict__ bmap,
-                                               KState* __restrict__ sts,
-                                               const T* __restrict__ partA,
-                                               T* __restrict__ partB, int Btot, int flip) {
-  const int p = bmap[blockIdx.x];
+__device__ __forceinline__ void b_upd(const BProb* __restrict__ probs, KState* __restrict__ sts,
+                                      const T* __restrict__ partA, T* __restrict__ partB,
+                                      int Btot, int flip, int p, int rank) {
   KState* st = sts + p;
   if (st->done) return;
   const BProb& P = probs[p];
-  const int rank = blockIdx.x - P.blk0;
   __shared__ T lds[16];
   const int it = st->it_upd;
   T pq[1];
@@ -318,14 +332,22 @@ __global__ __launch_bounds__(256) void k_b_upd(const BProb* __restrict__ probs,
   if (rank == 0 && threadIdx.x == 0 && threadIdx.y == 0) st->it_dir = it + 1;
 }
 
-// x64 += x32; x32 = 0 on live problems (every problem when all != 0)
-__global__ __launch_bounds__(256) void k_b_merge(const BProb* __restrict__ probs,
-                                                 const int* __restrict__ bmap,
-                                                 const KState* __restrict__ sts, int all) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_b_upd(const BProb* __restrict__ probs,
+                                               const int* __restrict__ bmap,
+                                               KState* __restrict__ sts,
+                                               const T* __restrict__ partA,
+                                               T* __restrict__ partB, int Btot, int flip) {
   const int p = bmap[blockIdx.x];
+  b_upd<T>(probs, sts, partA, partB, Btot, flip, p, (int)blockIdx.x - probs[p].blk0);
+}
+
+// x64 += x32; x32 = 0 on live problems (every problem when all != 0)
+__device__ __forceinline__ void b_merge(const BProb* __restrict__ probs,
+                                        const KState* __restrict__ sts, int all, int p,
+                                        int rank) {
   if (!all && sts[p].done) return;
   const BProb& P = probs[p];
-  const int rank = blockIdx.x - P.blk0;
   for (int t = rank; t < P.ntiles; t += P.nblk) {
     int i0, j0;
     tile_origin(t, P.ntj, i0, j0);
@@ -341,17 +363,22 @@ __global__ __launch_bounds__(256) void k_b_merge(const BProb* __restrict__ probs
   }
 }
 
+__global__ __launch_bounds__(256) void k_b_merge(const BProb* __restrict__ probs,
+                                                 const int* __restrict__ bmap,
+                                                 const KState* __restrict__ sts, int all) {
+  const int p = bmap[blockIdx.x];
+  b_merge(probs, sts, all, p, (int)blockIdx.x - probs[p].blk0);
+}
+
 // r32 = (float)(b - A x64) with the FP64 operator on live problems;
 // partials of the FP32 r.z, r.r and of the FP64 r.r.
-__global__ __launch_bounds__(256) void k_b_refresh(const BProb* __restrict__ probs,
-                                                   const int* __restrict__ bmap,
-                                                   const KState* __restrict__ sts,
-                                                   float* __restrict__ partB,
-                                                   double* __restrict__ partR64, int Btot) {
-  const int p = bmap[blockIdx.x];
+__device__ __forceinline__ void b_refresh(const BProb* __restrict__ probs,
+                                          const KState* __restrict__ sts,
+                                          float* __restrict__ partB,
+                                          double* __restrict__ partR64, int Btot, int p,
+                                          int rank) {
   if (sts[p].done) return;
   const BProb& P = probs[p];
-  const int rank = blockIdx.x - P.blk0;
   __shared__ float lds[16];
   __shared__ double lds64[4];
   const Sten<double> s64 = bsten<double>(P);
@@ -396,13 +423,20 @@ __global__ __launch_bounds__(256) void k_b_refresh(const BProb* __restrict__ pro
   store_partials<double, 1>(w, partR64 + P.pofs, Btot, rank, lds64);
 }
 
+__global__ __launch_bounds__(256) void k_b_refresh(const BProb* __restrict__ probs,
+                                                   const int* __restrict__ bmap,
+                                                   const KState* __restrict__ sts,
+                                                   float* __restrict__ partB,
+                                                   double* __restrict__ partR64, int Btot) {
+  const int p = bmap[blockIdx.x];
+  b_refresh(probs, sts, partB, partR64, Btot, p, (int)blockIdx.x - probs[p].blk0);
+}
+
 // partials of b.b (FP64) for the mixed mode's ||b||
-__global__ __launch_bounds__(256) void k_b_bb(const BProb* __restrict__ probs,
-                                              const int* __restrict__ bmap,
-                                              double* __restrict__ partR64, int Btot) {
+__device__ __forceinline__ void b_bb(const BProb* __restrict__ probs,
+                                     double* __restrict__ partR64, int Btot, int pr, int rank) {
   __shared__ double lds64[4];
-  const BProb& P = probs[bmap[blockIdx.x]];
-  const int rank = blockIdx.x - P.blk0;
+  const BProb& P = probs[pr];
   double w[1] = {0.0};
   for (int t = rank; t < P.ntiles; t += P.nblk) {
     int i0, j0;
@@ -419,25 +453,37 @@ __global__ __launch_bounds__(256) void k_b_bb(const BProb* __restrict__ probs,
   store_partials<double, 1>(w, partR64 + P.pofs, Btot, rank, lds64);
 }
 
-// st[p].normb = ||b|| (1 when b = 0); one block per problem
-__global__ __launch_bounds__(256) void k_b_setnorm(const BProb* __restrict__ probs,
-                                                   const double* __restrict__ partR64,
-                                                   int Btot, KState* __restrict__ sts) {
+__global__ __launch_bounds__(256) void k_b_bb(const BProb* __restrict__ probs,
+                                              const int* __restrict__ bmap,
+                                              double* __restrict__ partR64, int Btot) {
+  const int p = bmap[blockIdx.x];
+  b_bb(probs, partR64, Btot, p, (int)blockIdx.x - probs[p].blk0);
+}
+
+// st[p].normb = ||b|| (1 when b = 0) of problem p
+__device__ __forceinline__ void b_setnorm(const BProb* __restrict__ probs,
+                                          const double* __restrict__ partR64, int Btot,
+                                          KState* __restrict__ sts, int p) {
   __shared__ double lds64[4];
-  const BProb& P = probs[blockIdx.x];
+  const BProb& P = probs[p];
   double bb[1];
   reduce_prob<double, 1>(partR64 + P.pofs, Btot, P.nblk, bb, lds64);
   if (threadIdx.x == 0 && threadIdx.y == 0) {
     const double nb = std::sqrt(bb[0]);
-    sts[blockIdx.x].normb = nb == 0.0 ? 1.0 : nb;
+    sts[p].normb = nb == 0.0 ? 1.0 : nb;
   }
 }
 
+// one block per problem
+__global__ __launch_bounds__(256) void k_b_setnorm(const BProb* __restrict__ probs,
+                                                   const double* __restrict__ partR64,
+                                                   int Btot, KState* __restrict__ sts) {
+  b_setnorm(probs, partR64, Btot, sts, blockIdx.x);
+}
+
 // x32 = 0 on every problem
-__global__ __launch_bounds__(256) void k_b_zero32(const BProb* __restrict__ probs,
-                                                  const int* __restrict__ bmap) {
-  const BProb& P = probs[bmap[blockIdx.x]];
-  const int rank = blockIdx.x - P.blk0;
+__device__ __forceinline__ void b_zero32(const BProb* __restrict__ probs, int pr, int rank) {
+  const BProb& P = probs[pr];
   for (int t = rank; t < P.ntiles; t += P.nblk) {
     int i0, j0;
     tile_origin(t, P.ntj, i0, j0);
@@ -450,7 +496,202 @@ __global__ __launch_bounds__(256) void k_b_zero32(const BProb* __restrict__ prob
   }
 }
 
+__global__ __launch_bounds__(256) void k_b_zero32(const BProb* __restrict__ probs,
+                                                  const int* __restrict__ bmap) {
+  const int p = bmap[blockIdx.x];
+  b_zero32(probs, p, (int)blockIdx.x - probs[p].blk0);
+}
+
 const dim3 kBlock(TJ, BY);
+
+// ---------------------------------------------------------------------
+// Persistent batched solve: a whole batched pass (set-up, every iteration,
+// the mixed mode's merges and replacements, the final merge) in one
+// cooperative launch of G co-resident blocks.  The phases of the plain
+// launches become loops of the same device functions over the logical
+// blocks of the same block map (logical block lb on physical block
+// lb mod G), separated by a grid barrier; partial slots, tile assignment and
+// arithmetic are those of the plain launches, so the results are the same
+// bits.  After every K iterations (and the mixed mode's replacement) the
+// blocks read the problems' stop flags, written before the last barrier, and
+// all leave together once every member has stopped.
+
+// Grid barrier over the G co-resident blocks of a cooperative launch, with
+// one arrival flag per block (a gather by block 0, no contended atomics:
+// device-scope read-modify-writes on one counter serialise, each a memory
+// round trip).  Barrier number k (counted from 1) is complete when every
+// flag holds k: thread 0 of every block stores k into its flag with a
+// release (its block's writes, ordered before it by the block barrier,
+// reach device scope); block 0's threads wait for all the flags, then its
+// thread 0 stores k into the generation word with a release; the other
+// blocks wait for the generation and acquire.  Flags and generation only
+// grow, so nothing is ever reset.  Waits poll relaxed loads with sleeps,
+// every 32nd one an acquire load: a relaxed device-scope load may be
+// served by the XCD's own L2, which the other XCDs' writes reach only
+// through an acquire's invalidation, and invalidating on every poll slows
+// the blocks still at work.  A wait longer than `ticks` of the constant wall
+// clock raises the abort flag, which every waiting thread polls too, so no
+// block can wait for ever: all of them leave and the kernel ends (the host
+// then reports an error).
+constexpr int kBarStride = 32;                 // unsigned per slot (128 bytes)
+constexpr int kBarGen = 0, kBarAbort = kBarStride;
+constexpr int kBarFlags = 2 * kBarStride;      // flag of block b at kBarFlags + b * kBarStride
+inline size_t bar_words(int G) { return (size_t)kBarFlags + (size_t)G * kBarStride; }
+
+// waits until *w != `until_not` (returns true) or the wait is aborted
+__device__ __forceinline__ bool bar_wait(unsigned* w, unsigned until_not, unsigned* abort_w,
+                                         unsigned long long t0, unsigned long long ticks) {
+  for (unsigned poll = 1;; ++poll) {
+    const unsigned v = (poll & 31u) == 0u
+                           ? __hip_atomic_load(w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                           : __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v != until_not) return true;
+    if (__hip_atomic_load(abort_w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+        wall_clock64() - t0 > ticks) {
+      __hip_atomic_store(abort_w, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+__device__ bool grid_sync(unsigned* gb, unsigned G, unsigned& gen,
+                          unsigned long long ticks) {
+  __shared__ int ok_s;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const unsigned k = gen + 1;
+  __syncthreads();
+  if (tid == 0) {
+    ok_s = 1;
+    __hip_atomic_store(&gb[kBarFlags + blockIdx.x * kBarStride], k, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const unsigned long long t0 = wall_clock64();
+  if (blockIdx.x == 0) {
+    bool ok = true;
+    for (unsigned b = tid; b < G && ok; b += kThreads)
+      ok = bar_wait(&gb[kBarFlags + b * kBarStride], k - 1u, &gb[kBarAbort], t0, ticks);
+    if (!ok) ok_s = 0;
+    __syncthreads();
+    if (tid == 0 && ok_s) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __hip_atomic_store(&gb[kBarGen], k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  } else if (tid == 0) {
+    if (!bar_wait(&gb[kBarGen], k - 1u, &gb[kBarAbort], t0, ticks)) ok_s = 0;
+  }
+  if (tid == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  gen = k;
+  __syncthreads();
+  return ok_s != 0;
+}
+
+// The live problems' logical blocks: problem p (stop flag 0) holds its
+// nblk_p consecutive indices from pre[p] (pre has n + 1 entries, in LDS),
+// and the count is returned.  Every block builds the same table from the
+// same stop flags (read after a grid barrier), so a pass covers only the
+// live problems, as the host's compaction of the launched batches does; the
+// ranks and partial slots are the fixed ones of the layout.
+__device__ int live_table(const BProb* __restrict__ probs, const KState* __restrict__ sts,
+                          int n, int* pre) {
+  __shared__ int scan[kThreads];
+  __shared__ int carry;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += kThreads) {
+    const int p = base + tid;
+    const int v = p < n && sts[p].done == 0 ? probs[p].nblk : 0;
+    scan[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < kThreads; off <<= 1) {  // inclusive scan
+      const int add = tid >= off ? scan[tid - off] : 0;
+      __syncthreads();
+      scan[tid] += add;
+      __syncthreads();
+    }
+    if (p < n) pre[p] = carry + scan[tid] - v;
+    __syncthreads();
+    if (tid == 0) carry += scan[kThreads - 1];
+    __syncthreads();
+  }
+  if (tid == 0) pre[n] = carry;
+  __syncthreads();
+  return pre[n];
+}
+
+// the problem of live logical block lb: pre[p] <= lb < pre[p + 1]
+__device__ __forceinline__ int live_problem(const int* pre, int n, int lb) {
+  int lo = 0, hi = n;  // pre[lo] <= lb < pre[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= lb) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int MODE>  // 0 FP64 Jacobi-PCG, 1 FP32 iterations with FP64 replacement
+__global__ __launch_bounds__(256) void k_b_persist(
+    const BProb* __restrict__ probs, KState* __restrict__ sts, int n,
+    double* __restrict__ partB64, double* __restrict__ partA64, float* __restrict__ partB32,
+    float* __restrict__ partA32, double* __restrict__ partR64, int Btot,
+    unsigned long long* __restrict__ ts, unsigned* gb, int K, long long rounds,
+    unsigned long long ticks) {
+  extern __shared__ int pre[];  // n + 1 live-table entries
+  const unsigned G = gridDim.x;
+  unsigned gen = 0;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  if (blockIdx.x == 0 && tid == 0) ts[n] = wall_clock64();
+  int Blive = live_table(probs, sts, n, pre);
+#define FOR_LB                                                  \
+  for (int lb = blockIdx.x, p = 0, rank = 0; lb < Blive &&      \
+                                             (p = live_problem(pre, n, lb), rank = lb - pre[p], true); \
+       lb += G)
+#define SYNC \
+  if (!grid_sync(gb, G, gen, ticks)) return
+  if constexpr (MODE == 0) {
+    FOR_LB b_init<double>(probs, partB64, Btot, p, rank);
+  } else {
+    FOR_LB {
+      b_zero32(probs, p, rank);
+      b_bb(probs, partR64, Btot, p, rank);
+    }
+    SYNC;
+    for (int q = blockIdx.x; q < n; q += G) b_setnorm(probs, partR64, Btot, sts, q);
+    SYNC;
+    FOR_LB b_refresh(probs, sts, partB32, partR64, Btot, p, rank);
+  }
+  SYNC;
+  for (long long round = 0; round < rounds; ++round) {
+    for (int k = 0; k < K; ++k) {
+      const int flip = k & 1;
+      if constexpr (MODE == 0) {
+        FOR_LB b_dir<double>(probs, sts, partB64, partA64, partR64, Btot, flip, ts, p, rank);
+        SYNC;
+        FOR_LB b_upd<double>(probs, sts, partA64, partB64, Btot, flip, p, rank);
+      } else {
+        FOR_LB b_dir<float>(probs, sts, partB32, partA32, partR64, Btot, flip, ts, p, rank);
+        SYNC;
+        FOR_LB b_upd<float>(probs, sts, partA32, partB32, Btot, flip, p, rank);
+      }
+      SYNC;
+    }
+    if constexpr (MODE == 1) {
+      FOR_LB b_merge(probs, sts, 0, p, rank);
+      SYNC;
+      FOR_LB b_refresh(probs, sts, partB32, partR64, Btot, p, rank);
+      SYNC;
+    }
+    // the stop flags were written before the last barrier: every block
+    // builds the same table and takes the same decision
+    Blive = live_table(probs, sts, n, pre);
+    if (Blive == 0) break;
+  }
+#undef FOR_LB
+#undef SYNC
+}
 
 // Block layout for `tpb` tiles per block: fixed partial slots per problem
 // and, for the launch, the live problems' blocks (all when live is empty).
@@ -725,6 +966,80 @@ static int run_batched(cpgpu_batch* B, int mode, int max_iter, double tol,
   return CPGPU_OK;
 }
 
+// The batched pass of run_batched as one persistent cooperative launch
+// (k_b_persist); same states, layout and results.
+static int run_persistent(cpgpu_batch* B, int mode, int max_iter, double tol,
+                          const std::vector<char>& member) {
+  cpgpu_ctx* ctx = B->ctx;
+  hipStream_t st = ctx->stream;
+  const int n = B->n;
+  for (int p = 0; p < n; ++p) {
+    KState& k = B->h_st[p];
+    std::memset(&k, 0, sizeof(k));
+    k.tol = tol;
+    k.max_iter = max_iter;
+    k.it_dir = 1;
+    k.mode = mode;
+    k.refresh_every = B->K;
+    k.normb = 1.0;
+    if (!member[p]) k.done = 3;
+  }
+  CPGPU_HIP(ctx, hipMemcpyAsync(B->st.p, B->h_st, sizeof(KState) * n, hipMemcpyHostToDevice, st));
+  std::vector<int> bmap;
+  build_layout(B, &member, bmap);
+  int rc = upload_layout(B, bmap);
+  if (rc) return rc;
+  if (mode != 0)
+    return set_error(ctx, CPGPU_EUNSUPPORTED, "persistent batch: FP64 Jacobi-PCG only");
+  if (B->Bcur > 0) {
+    int coop = 0, ncu = 0, per_cu = 0;
+    CPGPU_HIP(ctx, hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ctx->device));
+    if (!coop) return set_error(ctx, CPGPU_EUNSUPPORTED, "device without cooperative launches");
+    CPGPU_HIP(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const void* kern = mode == 0 ? (const void*)k_b_persist<0> : (const void*)k_b_persist<1>;
+    const size_t dyn = sizeof(int) * ((size_t)n + 1);  // live table
+    if (dyn > 32768) return set_error(ctx, CPGPU_EINVAL, "persistent batch: at most 8191 problems");
+    CPGPU_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, dyn));
+    if (per_cu < 1) return set_error(ctx, CPGPU_EHIP, "persistent batch kernel does not fit a CU");
+    // every block co-resident (the barrier needs it), no more than the work
+    const int G = std::max(1, std::min(B->Bcur, per_cu * ncu));
+    if (B->gbar.alloc(bar_words(G))) return set_error(ctx, CPGPU_ENOMEM, "grid barrier");
+    CPGPU_HIP(ctx, hipMemsetAsync(B->gbar.p, 0, bar_words(G) * sizeof(unsigned), st));
+    const BProb* probs = B->probs.p;
+    int nn = n, Btot = B->Btot, K = B->K;
+    KState* sts = B->st.p;
+    double *pB64 = B->partB64.p, *pA64 = B->partA64.p, *pR64 = B->partR64.p;
+    float *pB32 = B->partB32.p, *pA32 = B->partA32.p;
+    unsigned long long* ts = B->tstamp.p;
+    unsigned* gb = B->gbar.p;
+    long long rounds = ((long long)max_iter + 2) / B->K + 2;
+    // a barrier wait longer than 20 s aborts the launch
+    unsigned long long ticks = (unsigned long long)(B->clock_khz * 1e3 * 20.0);
+    void* args[] = {(void*)&probs, (void*)&sts,  (void*)&nn,   (void*)&pB64, (void*)&pA64,
+                    (void*)&pB32,  (void*)&pA32, (void*)&pR64, (void*)&Btot, (void*)&ts,
+                    (void*)&gb,    (void*)&K,    (void*)&rounds, (void*)&ticks};
+    CPGPU_HIP(ctx, hipLaunchCooperativeKernel(kern, dim3(G), kBlock, args, (unsigned)dyn, st));
+    unsigned abort_flag = 0;
+    CPGPU_HIP(ctx, hipMemcpyAsync(&abort_flag, B->gbar.p + kBarAbort, sizeof(unsigned),
+                                  hipMemcpyDeviceToHost, st));
+    CPGPU_HIP(ctx, hipStreamSynchronize(st));
+    if (abort_flag) return set_error(ctx, CPGPU_EHIP, "persistent batch: grid barrier timed out");
+  } else {
+    hipLaunchKernelGGL(k_b_t0, dim3(1), dim3(64), 0, st, B->tstamp.p, n);
+  }
+  CPGPU_HIP(ctx, hipMemcpyAsync(B->h_st, B->st.p, sizeof(KState) * n, hipMemcpyDeviceToHost, st));
+  CPGPU_HIP(ctx, hipMemcpyAsync(B->h_ts, B->tstamp.p, sizeof(unsigned long long) * (n + 1),
+                                hipMemcpyDeviceToHost, st));
+  CPGPU_HIP(ctx, hipStreamSynchronize(st));
+  return CPGPU_OK;
+}
+
+int cpgpu_batch_set_persistent(cpgpu_batch* B, int on) {
+  if (!B) return CPGPU_EINVAL;
+  B->persistent = on != 0;
+  return CPGPU_OK;
+}
+
 int cpgpu_batch_solve(cpgpu_batch* B, int method, int max_iter, double tol, int* iters,
                       double* resid, int* codes, double* latency_ms) {
   if (!B || max_iter < 0 || !(tol >= 0.0)) return CPGPU_EINVAL;
@@ -744,7 +1059,12 @@ int cpgpu_batch_solve(cpgpu_batch* B, int method, int max_iter, double tol, int*
   // the host time at the pass's start
   auto batched = [&](const std::vector<char>& member, int mode) -> int {
     const double off = since();
-    int rc = run_batched(B, mode, max_iter, tol, member);
+    // the persistent launch serves the FP64 pass; the mixed-precision pass
+    // stays on launched batches (its persistent form, k_b_persist<1>, hit a
+    // grid barrier time-out on the 256-problem set whose cause is not
+    // established, so it is not run)
+    int rc = B->persistent && mode == 0 ? run_persistent(B, mode, max_iter, tol, member)
+                                        : run_batched(B, mode, max_iter, tol, member);
     if (rc) return rc;
     for (int p = 0; p < n; ++p) {
       if (!member[p]) continue;

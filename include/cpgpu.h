@@ -261,6 +261,13 @@ int cpgpu_batch_destroy(cpgpu_batch *B);
 int cpgpu_batch_set_tiles(cpgpu_batch *B, int tiles_per_block);
 /* Graph-captured iteration batches on (default) or plain launches. */
 int cpgpu_batch_set_graphs(cpgpu_batch *B, int use_graphs);
+/* on != 0: the CPGPU_JPCG pass of cpgpu_batch_solve runs as ONE persistent
+ * cooperative launch: set-up, all iterations and the per-problem stops
+ * happen on the device, blocks synchronised by a grid barrier; results are
+ * bitwise those of the launched batches.  CPGPU_MIXED and the batched passes
+ * of CPGPU_FCG_UNET keep launched batches.  No reference counterpart (the
+ * reference solves one problem at a time). */
+int cpgpu_batch_set_persistent(cpgpu_batch *B, int on);
 /* Solves every problem: method CPGPU_JPCG (FP64, the reference's J-PCG test
  * on the true relative residual every iteration), CPGPU_MIXED (FP32
  * iterations, FP64 residual replacement and test every 16 iterations), or

@@ -29,7 +29,7 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-@pytest.mark.parametrize("graphs", [True, False])
+@pytest.mark.parametrize("graphs", [True, False, "persistent"])
 def test_batch_is_bitwise_the_single_problem_solve(ctx, graphs):
     single = make(ctx)
     ref = []
@@ -39,7 +39,10 @@ def test_batch_is_bitwise_the_single_problem_solve(ctx, graphs):
         ref.append((code, it, tol, P.get_phi()))
     probs = make(ctx)
     B = ca.Batch(ctx, probs)
-    B.set_graphs(graphs)
+    if graphs == "persistent":
+        B.set_persistent(True)  # the whole solve in one cooperative launch
+    else:
+        B.set_graphs(graphs)
     code, it, res, codes, lat = B.solve("JPCG", max_iter=20000, tol=1e-10)
     assert code == 0 and (codes == 0).all()
     for k, P in enumerate(probs):
@@ -177,3 +180,38 @@ def test_batch_learned_members_reach_tolerance(ctx):
     net.close()
     for P in ref + probs:
         P.close()
+
+
+MIXED_PERSISTENT_OFF = ("the persistent form of the mixed-precision pass is switched off: "
+                        "one of its grid barrier waits timed out on the 256-problem set "
+                        "and the cause is not established")
+
+
+@pytest.mark.parametrize("method,tpb", [
+    ("JPCG", 2), ("JPCG", 8),
+    pytest.param("MIXED", 1, marks=pytest.mark.skip(reason=MIXED_PERSISTENT_OFF)),
+    pytest.param("MIXED", 4, marks=pytest.mark.skip(reason=MIXED_PERSISTENT_OFF))])
+def test_persistent_launch_is_bitwise_the_launched_batches(ctx, method, tpb):
+    # one cooperative launch (set-up, every iteration, replacements, stops on
+    # the device) against the graph-replayed batches: same block map, same
+    # partial slots, so the same bits, iteration counts and stop codes; one
+    # member stops at max_iter
+    out = []
+    for persistent in (False, True):
+        probs = make(ctx, SHAPES, seed=11)
+        B = ca.Batch(ctx, probs)
+        B.set_tiles(tpb)
+        B.set_persistent(persistent)
+        code, it, res, codes, lat = B.solve(method, max_iter=700, tol=1e-9)
+        out.append((code, it.copy(), res.copy(), codes.copy(), [P.get_phi() for P in probs], lat))
+        B.close()
+        for P in probs:
+            P.close()
+    (c0, it0, r0, k0, phi0, _), (c1, it1, r1, k1, phi1, lat1) = out
+    print(method, tpb, "iterations", list(it1), "codes", list(k1))
+    assert c0 == c1 and np.array_equal(it0, it1) and np.array_equal(k0, k1)
+    assert np.array_equal(r0, r1)
+    for a, b in zip(phi0, phi1):
+        assert np.array_equal(a, b)
+    assert (k1 == ca.MAXITER).any() and (k1 == ca.OK).any()  # both stop kinds exercised
+    assert (lat1 > 0).all()
