@@ -609,6 +609,10 @@ def main():
                     help="flexible-CG sample rounds after the first fit")
     ap.add_argument("--first-fit", type=int, default=2000)
     ap.add_argument("--round-fit", type=int, default=1500)
+    ap.add_argument("--nu", type=int, default=2,
+                    help="weighted-Jacobi pre- and post-sweeps of the learned cycle")
+    ap.add_argument("--omega", type=float, default=0.8,
+                    help="weight of the learned cycle's Jacobi sweeps")
     ap.add_argument("--round-steps", type=int, default=67,
                     help="cavity steps per flexible-CG sample round (6 samples each: with "
                          "the plain steps' 160, 2170 training samples by default)")
@@ -697,7 +701,7 @@ def main():
 
     # ---- training (untimed) ----
     net = ca.UNet(ctx, seed=1)
-    tr = learned.CavityTrainer(ns, net, nu=2, omega=0.8, tol=tol)
+    tr = learned.CavityTrainer(ns, net, nu=a.nu, omega=a.omega, tol=tol)
     t0 = time.perf_counter()
     if a.load_net:
         # a network and cavity state saved by an earlier --save-net run (a

@@ -9,6 +9,7 @@
 // A breakdown (p.q <= 0) or stagnation hands the current iterate to FP64
 // Jacobi-PCG.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -54,8 +55,16 @@ __global__ __launch_bounds__(256) void k_fcg_init(Sten<double> s,
   store_partials<double, 2>(v, part, gridDim.x, blockIdx.x, lds);
 }
 
+// the host's copy of the state (values first, the sequence number last)
+__device__ __forceinline__ void post_mail(FMail* mail, double resid, int breakdown, int seq) {
+  if (!mail) return;
+  mail->resid = resid;
+  mail->breakdown = breakdown;
+  __hip_atomic_store(&mail->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ __launch_bounds__(256) void k_fcg_start(const double* part, int G,
-                                                   FState* st) {
+                                                   FState* st, FMail* mail) {
   __shared__ double lds[8];
   double t[2];
   reduce_partials<double, 2>(part, G, t, lds);
@@ -66,6 +75,7 @@ __global__ __launch_bounds__(256) void k_fcg_start(const double* part, int G,
     st->pq = 0.0;
     st->iter = 0;
     st->breakdown = 0;
+    post_mail(mail, st->resid, 0, 0);
   }
 }
 
@@ -180,13 +190,14 @@ __global__ __launch_bounds__(256) void k_fcg_upd(const double* __restrict__ p,
 }
 
 __global__ __launch_bounds__(256) void k_fcg_check(const double* part, int G,
-                                                   FState* st) {
+                                                   FState* st, FMail* mail) {
   __shared__ double lds[4];
   double t[1];
   reduce_partials<double, 1>(part, G, t, lds);
   if (threadIdx.x == 0) {
     st->resid = std::sqrt(t[0]) / st->normb;
     st->iter = st->iter + 1;
+    post_mail(mail, st->resid, st->breakdown, st->iter);
   }
 }
 
@@ -373,6 +384,17 @@ int fcg_alloc(cpgpu_poisson* P) {
     P->h_fst = nullptr;
     return set_error(P->ctx, CPGPU_ENOMEM, "pinned flexible-CG state");
   }
+  if (!P->h_mail) {
+    // without a mapped mailbox the state is copied back (read_state)
+    if (hipHostMalloc((void**)&P->h_mail, sizeof(FMail),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&P->d_mail, P->h_mail, 0) != hipSuccess) {
+      if (P->h_mail) (void)hipHostFree(P->h_mail);
+      P->h_mail = nullptr;
+      P->d_mail = nullptr;
+      (void)hipGetLastError();
+    }
+  }
   return CPGPU_OK;
 }
 
@@ -461,7 +483,8 @@ int launch_fcg_iter(cpgpu_poisson* P, bool first, int parity, bool singular,
                      singular ? 1 : 0, P->partB64.p, tl);
   hipLaunchKernelGGL(k_fcg_upd, dim3(G1), dim3(256), 0, st, pnew, P->q.p, P->x.p, P->r.p,
                      P->r32.p, P->partB64.p, (int)tl.G, P->fst.p, P->partR64.p, N, norm_ref);
-  hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(256), 0, st, P->partR64.p, G1, P->fst.p);
+  hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(256), 0, st, P->partR64.p, G1, P->fst.p,
+                     P->d_mail);
   CPGPU_HIP(P->ctx, hipGetLastError());
   return CPGPU_OK;
 }
@@ -495,8 +518,27 @@ hipGraphExec_t fcg_graph(cpgpu_poisson* P, int parity, bool singular, int norm_r
   return exec;
 }
 
-int read_state(cpgpu_poisson* P, FState* out) {
+// The state after update number `seq` (the start: 0).  With the mailbox the
+// host polls the device's own write (a few microseconds after the check
+// kernel instead of a copy and a stream synchronisation); a mailbox that
+// shows nothing within 5 s of polling (or none at all) falls back to the
+// copy, after which the stream is idle and the state final.
+int read_state(cpgpu_poisson* P, FState* out, int seq) {
   hipStream_t st = P->ctx->stream;
+  if (P->h_mail) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spin = 0;; ++spin) {
+      if (__atomic_load_n(&P->h_mail->seq, __ATOMIC_ACQUIRE) == seq) {
+        out->resid = P->h_mail->resid;
+        out->breakdown = P->h_mail->breakdown;
+        out->iter = seq;
+        return CPGPU_OK;
+      }
+      if ((spin & 1023u) == 1023u &&
+          std::chrono::steady_clock::now() - t0 > std::chrono::seconds(5))
+        break;
+    }
+  }
   CPGPU_HIP(P->ctx, hipMemcpyAsync(P->h_fst, P->fst.p, sizeof(FState),
                                    hipMemcpyDeviceToHost, st));
   CPGPU_HIP(P->ctx, hipStreamSynchronize(st));
@@ -542,11 +584,12 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
   hipLaunchKernelGGL(k_fcg_init, dim3(tl.G), dim3(TJ, BY), 0, st, P->sten64(),
                      P->x.p, P->b.p, P->r.p, P->r32.p, P->partB64.p, tl,
                      norm_ref);
+  if (P->h_mail) __atomic_store_n(&P->h_mail->seq, -1, __ATOMIC_RELEASE);
   hipLaunchKernelGGL(k_fcg_start, dim3(1), dim3(256), 0, st, P->partB64.p,
-                     (int)tl.G, P->fst.p);
+                     (int)tl.G, P->fst.p, P->d_mail);
   CPGPU_HIP(ctx, hipGetLastError());
   FState fs;
-  if ((rc = read_state(P, &fs))) return rc;
+  if ((rc = read_state(P, &fs, 0))) return rc;
   double res = fs.resid;
   std::vector<double> trace;
   trace.push_back(res);
@@ -573,7 +616,7 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
       return rc;
     }
     ++k;
-    if ((rc = read_state(P, &fs))) return rc;
+    if ((rc = read_state(P, &fs, k))) return rc;
     res = fs.resid;
     trace.push_back(res);
     if (fs.breakdown || !std::isfinite(res)) {

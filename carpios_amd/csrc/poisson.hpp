@@ -79,6 +79,16 @@ struct FState {
   int breakdown;  // p.q <= 0 or not finite
 };
 
+// The part of FState the host needs after every flexible-CG iteration,
+// written by the device into host memory (mapped, coherent) with the update
+// count as a sequence number stored last, so the host polls it instead of a
+// copy and a stream synchronisation.
+struct FMail {
+  double resid;
+  int breakdown;
+  int seq;  // FState::iter of the values above; stored last, with a release
+};
+
 // SplitMix64-seeded xoshiro256** (Blackman & Vigna); doubles from the top
 // 53 bits; normals by Box-Muller.  The generator of every seeded input
 // (blob sources, batches of random problems).
@@ -212,6 +222,8 @@ struct cpgpu_poisson {
   cpgpu::DevBuf<cpgpu::BState> bst;
   cpgpu::DevBuf<cpgpu::FState> fst;
   cpgpu::FState* h_fst = nullptr;  // pinned
+  cpgpu::FMail* h_mail = nullptr;  // host-mapped, coherent (null: copy instead)
+  cpgpu::FMail* d_mail = nullptr;  // its device address
   // error norms: the exact field and the interior cell sizes on the device
   cpgpu::DevBuf<double> err_ref, dsxi, dsyi;
 
@@ -243,6 +255,7 @@ struct cpgpu_poisson {
     if (h_kst) (void)hipHostFree(h_kst);
     if (h_rst) (void)hipHostFree(h_rst);
     if (h_fst) (void)hipHostFree(h_fst);
+    if (h_mail) (void)hipHostFree(h_mail);
   }
 };
 
