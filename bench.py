@@ -714,7 +714,7 @@ def main():
         tr.log = [("loaded", a.load_net)]
     else:
         tr.run(plain=a.train_plain, first_fit=a.first_fit, rounds=a.rounds,
-               round_steps=a.round_steps, round_fit=a.round_fit)
+               round_steps=a.round_steps, round_fit=a.round_fit, progress=log)
     ctx.sync()
     t_train = time.perf_counter() - t0
     log(f"bench: trained in {t_train:.1f} s on {tr.slot} samples: {tr.log}")

@@ -151,14 +151,22 @@ class CavityTrainer:
         return losses
 
     def run(self, plain=100, first_fit=2000, rounds=5, round_steps=40,
-            round_fit=1500):
+            round_fit=1500, progress=None):
         """The schedule used by the benchmark: plain steps, a first fit, then
-        `rounds` of flexible-CG sample collection and refits."""
+        `rounds` of flexible-CG sample collection and refits.  `progress`, if
+        given, receives a line after every phase."""
+        def done():
+            if progress:
+                progress(f"training: {self.log[-1]}")
         self.plain_steps(plain)
+        done()
         self.fit(first_fit, 1e-3, 0)
+        done()
         for r in range(rounds):
             self.fcg_steps(round_steps)
+            done()
             self.fit(round_fit, 5e-4, r + 1)
+            done()
         return self.log
 
 
