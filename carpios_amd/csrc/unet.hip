@@ -247,8 +247,8 @@ __global__ __launch_bounds__(256, 5) void k_enc0_apply(const float* __restrict__
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
-            o[u] = (uint32_t)f2bf(s[2 * u] > 0.f ? s[2 * u] : 0.f) |
-                   ((uint32_t)f2bf(s[2 * u + 1] > 0.f ? s[2 * u + 1] : 0.f) << 16);
+            o[u] = bf16x2_bits(s[2 * u] > 0.f ? s[2 * u] : 0.f,
+                               s[2 * u + 1] > 0.f ? s[2 * u + 1] : 0.f);
         }
         *reinterpret_cast<uint4*>(sa + p * 16 + 8 * h) = uint4{o[0], o[1], o[2], o[3]};
       }
@@ -302,7 +302,6 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
   __shared__ __attribute__((aligned(16))) bf16 su[NC * 32];
   __shared__ __attribute__((aligned(16))) bf16 swl[9 * 16 * 16];  // last conv [tap][co][ci]
   __shared__ __attribute__((aligned(16))) bf16 swm[9 * 16 * 32];  // middle conv [tap][co][ci]
-  __shared__ __attribute__((aligned(16))) bf16 junk[4 * 64];      // discarded up-conv stores
   typedef Frag<32> F;
   const int H = skip.H, W = skip.W, Hc = d1.H, Wc = d1.W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -385,8 +384,8 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
     fetch(t + tw.per < tw.end ? t + tw.per : t);
     // B: up-convolution into channels 0..15 of the window.  Fine pixel
     // (4 I + wave, 4 J + j) of coarse pixel (I, J) sits at window row
-    // 4 pi + wave - 2, column 4 pj + j - 2; stores outside the window go to
-    // this lane's junk slot.  Lanes past the 36 coarse pixels repeat the last.
+    // 4 pi + wave - 2, column 4 pj + j - 2; stores outside the window are
+    // masked off.  Lanes past the 36 coarse pixels repeat the last.
 #pragma unroll
     for (int g = 0; g < 3; ++g) {
       const int cpl = min(g * 16 + px, NC - 1);
@@ -395,15 +394,21 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
       const int I = ti0 / 4 - 1 + pi, J = tj0 / 4 - 1 + pj;
       const bool cin = I >= 0 && I < Hc && J >= 0 && J < Wc;
       const int fr = 4 * pi + wave - 2;
+      const bool frok = fr >= 0 && fr < XP;
+      // swz(fr, fc, cq) for the window columns fc = 4 pj - 2 + j: the
+      // swizzle key (fc >> 1) & 3 is 2 pj - 1 for j = 0, 1 and 2 pj for
+      // j = 2, 3 (columns before the window are not stored)
+      const int rb = (fr * XP + 4 * pj - 2) * SP + (cq & 7);
+      const int k01 = ((cq >> 3) ^ ((2 * pj - 1) & 3)) << 3, k23 = ((cq >> 3) ^ ((2 * pj) & 3)) << 3;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const f32x4_t a = F::mfma(wuf[j], bfr, f32x4_t{0.f, 0.f, 0.f, 0.f});
         const int fc = 4 * pj + j - 2;
-        const bool ok = fr >= 0 && fr < XP && fc >= 0 && fc < XP;
-        const uint32_t h0 = (uint32_t)f2bf(a[0] + bur[0]) | ((uint32_t)f2bf(a[1] + bur[1]) << 16);
-        const uint32_t h1 = (uint32_t)f2bf(a[2] + bur[2]) | ((uint32_t)f2bf(a[3] + bur[3]) << 16);
-        bf16* dst = ok ? sc + swz(fr, fc, cq) : junk + 4 * lane;
-        *reinterpret_cast<uint2*>(dst) = uint2{cin ? h0 : 0u, cin ? h1 : 0u};
+        const uint32_t h0 = bf16x2_bits(a[0] + bur[0], a[1] + bur[1]);
+        const uint32_t h1 = bf16x2_bits(a[2] + bur[2], a[3] + bur[3]);
+        if (frok && fc >= 0 && fc < XP)
+          *reinterpret_cast<uint2*>(sc + rb + j * SP + (j < 2 ? k01 : k23)) =
+              uint2{cin ? h0 : 0u, cin ? h1 : 0u};
       }
     }
     __syncthreads();
@@ -418,8 +423,8 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
         const float s = a[u] + bmr[u];
         q[u] = s > 0.f ? s : 0.f;
       }
-      const uint32_t h0 = (uint32_t)f2bf(q[0]) | ((uint32_t)f2bf(q[1]) << 16);
-      const uint32_t h1 = (uint32_t)f2bf(q[2]) | ((uint32_t)f2bf(q[3]) << 16);
+      const uint32_t h0 = bf16x2_bits(q[0], q[1]);
+      const uint32_t h1 = bf16x2_bits(q[2], q[3]);
       *reinterpret_cast<uint2*>(sd + (R * TP + c) * 16 + cq) = uint2{in ? h0 : 0u, in ? h1 : 0u};
     };
     {
@@ -434,10 +439,24 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
       // software-pipelined: the next tap's weight and six fragments are
       // read from LDS before this tap's six MFMAs, so the reads' latency
       // overlaps them (read just before use, every MFMA waited for its read)
+      // fragment addresses split into a row part per slot and a column part
+      // per slot kind (the five row slots share column px) and dx, so that a
+      // read costs one addition and the tap's row step is the read's
+      // immediate offset (swz(row, col, kq) = row XP SP + colpart(col))
+      auto colpart = [&](int col) {
+        return col * SP + (((kq >> 3) ^ ((col >> 1) & 3)) << 3);
+      };
+      int rowp[6], colp[2][3];
+#pragma unroll
+      for (int a = 0; a < 6; ++a) rowp[a] = ((a < 5 ? min(R0 + a, TP - 1) : Rs) + 1) * XP * SP;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        colp[0][d] = colpart(px + d);
+        colp[1][d] = colpart(cs + d);
+      }
       auto frag = [&](int tap, int a) {
         const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-        const int R = a < 5 ? min(R0 + a, TP - 1) : Rs, c = a < 5 ? px : cs;
-        return F::load(sc + swz(R + 1 + dy, c + 1 + dx, kq));
+        return F::load(sc + (rowp[a] + colp[a < 5 ? 0 : 1][dx + 1]) + dy * XP * SP);
       };
       F::T wt = F::load(swm + px * 32 + kq), af[6];
 #pragma unroll

@@ -41,6 +41,18 @@ __device__ __host__ __forceinline__ float bf2f(bf16 b) {
 // bit-level formula compiled to a branch per value for its NaN case
 __device__ __forceinline__ bf16 f2bf(float f) { return __builtin_bit_cast(bf16, (__bf16)f); }
 
+// two values rounded as f2bf, packed lo | hi << 16: one v_cvt_pk_bf16_f32
+// (two scalar conversions side by side are not always combined)
+__device__ __forceinline__ uint32_t bf16x2_bits(float lo, float hi) {
+  typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
+  typedef float f2_t __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2_t{lo, hi}), bf2_t));
+}
+// the FP32 value of half `hi` (0 low, 1 high) of a packed pair
+__device__ __forceinline__ float bf16x2_half(uint32_t bits, int hi) {
+  return __uint_as_float(hi ? (bits & 0xffff0000u) : (bits << 16));
+}
+
 // A tensor view: element (m, c) at p[m*cs + co + c]
 struct View {
   bf16* p;
@@ -176,7 +188,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(View x, View y,
           v0 = v0 > 0.f ? v0 : 0.f;
           v1 = v1 > 0.f ? v1 : 0.f;
         }
-        h[t >> 1] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+        h[t >> 1] = bf16x2_bits(v0, v1);
       }
       *reinterpret_cast<uint2*>(y.p + (size_t)m * y.cs + y.co + co) = uint2{h[0], h[1]};
     }
@@ -252,7 +264,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd_ks(View x, View y, const bf16
       v0 = v0 > 0.f ? v0 : 0.f;
       v1 = v1 > 0.f ? v1 : 0.f;
     }
-    h[t >> 1] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+    h[t >> 1] = bf16x2_bits(v0, v1);
   }
   *reinterpret_cast<uint2*>(y.p + (size_t)m * y.cs + y.co + co) = uint2{h[0], h[1]};
 }
@@ -454,7 +466,7 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
             v0 = v0 > 0.f ? v0 : 0.f;
             v1 = v1 > 0.f ? v1 : 0.f;
           }
-          h[t >> 1] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+          h[t >> 1] = bf16x2_bits(v0, v1);
         }
         *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) = uint2{h[0], h[1]};
       } else {
@@ -540,16 +552,15 @@ __global__ __launch_bounds__(256) void k_conv3_tile16(View x, View yb, GView yf,
     for (int b = 0; b < NB; ++b) {
       const int n = b * 16 + cq;
       if (FWD) {
-        bf16 q[4];
+        float v[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          float v = acc[a][b][t] + bias[n + t];
-          if (relu) v = v > 0.f ? v : 0.f;
-          q[t] = f2bf(v);
-          ps[b][t] += bf2f(q[t]);
+          v[t] = acc[a][b][t] + bias[n + t];
+          if (relu) v[t] = v[t] > 0.f ? v[t] : 0.f;
         }
-        const uint2 h = uint2{(uint32_t)q[0] | ((uint32_t)q[1] << 16),
-                              (uint32_t)q[2] | ((uint32_t)q[3] << 16)};
+        const uint2 h = uint2{bf16x2_bits(v[0], v[1]), bf16x2_bits(v[2], v[3])};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ps[b][t] += bf16x2_half(t < 2 ? h.x : h.y, t & 1);
         *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) = h;
       } else {
         *reinterpret_cast<float4*>(yf.p + m * yf.cs + yf.co + n) =
@@ -701,21 +712,25 @@ __device__ __forceinline__ void conv16_epilogue(const f32x4_t (&acc)[4][NB],
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int n = b * 16 + cq;
-      bf16 q[4];
+      float v[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        float v = acc[a][b][u] + ep.bias[b][u];
-        if (relu) v = v > 0.f ? v : 0.f;
-        q[u] = f2bf(v);
-        if (POOL) ps[b][u] += bf2f(q[u]);
+        v[u] = acc[a][b][u] + ep.bias[b][u];
+        if (relu) v[u] = v[u] > 0.f ? v[u] : 0.f;
       }
-      if (STORE)
-        *reinterpret_cast<uint2*>(yb.p + (size_t)m * yb.cs + yb.co + n) =
-            uint2{(uint32_t)q[0] | ((uint32_t)q[1] << 16), (uint32_t)q[2] | ((uint32_t)q[3] << 16)};
+      const uint32_t h0 = bf16x2_bits(v[0], v[1]), h1 = bf16x2_bits(v[2], v[3]);
+      float q[4];  // the rounded values
+#pragma unroll
+      for (int u = 0; u < 4; ++u) q[u] = bf16x2_half(u < 2 ? h0 : h1, u & 1);
+      if (POOL) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ps[b][u] += q[u];
+      }
+      if (STORE) *reinterpret_cast<uint2*>(yb.p + (size_t)m * yb.cs + yb.co + n) = uint2{h0, h1};
       if (OUT) {
         float part = 0.f;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) part += ep.wo[u] * bf2f(q[u]);
+        for (int u = 0; u < 4; ++u) part += ep.wo[u] * q[u];
         part += __shfl_xor(part, 16, 64);
         part += __shfl_xor(part, 32, 64);
         if (lane < 16) yout[m] = (ep.wo16 + part) * ep.ysc;
@@ -898,8 +913,7 @@ __global__ __launch_bounds__(256) void k_up_fwd(View x, View y,
       uint32_t h[2];
 #pragma unroll
       for (int t = 0; t < 4; t += 2)
-        h[t >> 1] = (uint32_t)f2bf(acc[a][b][t] + bias[co + t]) |
-                    ((uint32_t)f2bf(acc[a][b][t + 1] + bias[co + t + 1]) << 16);
+        h[t >> 1] = bf16x2_bits(acc[a][b][t] + bias[co + t], acc[a][b][t + 1] + bias[co + t + 1]);
       *reinterpret_cast<uint2*>(y.p + f * y.cs + y.co + co) = uint2{h[0], h[1]};
     }
   }
