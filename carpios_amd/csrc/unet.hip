@@ -15,6 +15,7 @@
 // Stride-4 pooling gives the 1024^2 grid a 16^2 bottleneck, so one
 // application couples the whole domain.  Grid sides must be multiples of 64.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -356,6 +357,9 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
     asm volatile("" : "+v"(tv));
     const int px = tv & 15, kq = 8 * ((tv >> 4) & 3), cq = 4 * ((tv >> 4) & 3);
     const int lane = tv & 63;
+    // tiles whose window (and coarse window) lies inside the grid, most of
+    // them, skip the per-pixel bounds tests
+    const bool inner = ti0 > 0 && tj0 > 0 && ti0 + 16 < H && tj0 + 16 < W;
     // A: stage the skip window and the coarse pixels.  No barrier is needed
     // before it: the previous tile's last reads of sc and su precede its
     // barrier before the final convolution.
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
         const int pix = e >> 1, ch = e & 1;
         const int r = pix / XP, c = pix - r * XP;
         const int gi = ti0 - 2 + r, gj = tj0 - 2 + c;
-        const bool in = gi >= 0 && gi < H && gj >= 0 && gj < W;
+        const bool in = inner || (gi >= 0 && gi < H && gj >= 0 && gj < W);
         *reinterpret_cast<uint4*>(sc + swz(r, c, 16 + ch * 8)) = in ? pre[k] : uint4{0u, 0u, 0u, 0u};
       }
     }
@@ -392,7 +396,7 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
       const int pi = cpl / 6, pj = cpl - 6 * pi;
       const F::T bfr = F::load(su + cpl * 32 + kq);
       const int I = ti0 / 4 - 1 + pi, J = tj0 / 4 - 1 + pj;
-      const bool cin = I >= 0 && I < Hc && J >= 0 && J < Wc;
+      const bool cin = inner || (I >= 0 && I < Hc && J >= 0 && J < Wc);
       const int fr = 4 * pi + wave - 2;
       const bool frok = fr >= 0 && fr < XP;
       // swz(fr, fc, cq) for the window columns fc = 4 pj - 2 + j: the
@@ -414,9 +418,10 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
     __syncthreads();
     // C: the 32 -> 16 convolution on the 18 x 18 halo into sd (zero outside
     // the grid)
-    auto mid_store = [&](const f32x4_t& a, int R, int c) {
+    auto mid_store = [&](const f32x4_t& a, int R, int c, auto check) {
       const int gi = ti0 - 1 + R, gj = tj0 - 1 + c;
-      const bool in = gi >= 0 && gi < H && gj >= 0 && gj < W;
+      bool in = true;
+      if constexpr (decltype(check)::value) in = gi >= 0 && gi < H && gj >= 0 && gj < W;
       float q[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -477,9 +482,15 @@ __global__ __launch_bounds__(256, 3) void k_dec0_apply(View d1, View skip,
 #pragma unroll
         for (int a = 0; a < 6; ++a) af[a] = an[a];
       }
+      if (inner) {
 #pragma unroll
-      for (int a = 0; a < 5; ++a) mid_store(acc[a], min(R0 + a, TP - 1), px);
-      mid_store(acc[5], Rs, cs);
+        for (int a = 0; a < 5; ++a) mid_store(acc[a], min(R0 + a, TP - 1), px, std::false_type{});
+        mid_store(acc[5], Rs, cs, std::false_type{});
+      } else {
+#pragma unroll
+        for (int a = 0; a < 5; ++a) mid_store(acc[a], min(R0 + a, TP - 1), px, std::true_type{});
+        mid_store(acc[5], Rs, cs, std::true_type{});
+      }
     }
     __syncthreads();
     // D: the 16 -> 16 convolution and the output layer
