@@ -539,9 +539,35 @@ def kernel_probe(ca, mode, size, precision=32, reps=50, method="MIXED", tol=1e-8
       unet:  `reps` forward passes of the U-Net (seed 1) on a size^2 input;
       apply: `reps` applications of the same network as a preconditioner
              applies it (the fused level-0 kernels), as warm starts of a
-             size^2 Poisson problem."""
+             size^2 Poisson problem;
+      fcg:   `reps` flexible-CG solves of 20 iterations each with that
+             network in the 2 + 2 sweep cycle, on the same problem (the
+             wall time per iteration: cycle, network and Krylov kernels)."""
     n = size
     ctx = ca.Context(0)
+    if mode == "fcg":
+        net = ca.UNet(ctx, seed=1)
+        types = [ca.BC_DIRICHLET] * 3 + [ca.BC_NEUMANN]
+        P = ca.Poisson(ctx, ca.uniform_grid_desc(n, n, 1, types))
+        P.set_source_blobs(seed=0, nblobs=64, wmin=0.01, wmax=0.1)
+        P.set_unet(net, ca.UNET_PRECOND)
+        P.set_fcg(2, 2, 0.8)
+        its, wall = 0, 0.0
+        for r in range(reps + 1):
+            P.fill_phi(0.0)
+            ctx.sync()
+            t = time.perf_counter()
+            code, it, res, _ = P.solve("FCG_UNET", max_iter=20, tol=1e-300, norm=ca.NORM_TRUE)
+            ctx.sync()
+            if r:  # the first solve allocates and warms up
+                wall += time.perf_counter() - t
+                its += it
+        P.set_unet(None, 0)
+        P.close()
+        net.close()
+        ctx.close()
+        return {"size": n, "mode": "fcg", "reps": reps, "iterations": its,
+                "us_per_iteration": round(wall * 1e6 / max(its, 1), 2)}
     if mode == "apply":
         net = ca.UNet(ctx, seed=1)
         types = [ca.BC_DIRICHLET] * 3 + [ca.BC_NEUMANN]
@@ -634,7 +660,7 @@ def main():
                          "Dia_BiCGSTAB probes")
     ap.add_argument("--no-hbm", action="store_true",
                     help="skip the 4096^2 Krylov-kernel HBM probe")
-    ap.add_argument("--probe", choices=("times", "solve", "unet", "apply"), default="",
+    ap.add_argument("--probe", choices=("times", "solve", "unet", "apply", "fcg"), default="",
                     help="kernels alone on the --size^2 blob problem, for profilers")
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--reps", type=int, default=50)

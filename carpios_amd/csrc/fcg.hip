@@ -241,13 +241,19 @@ __global__ __launch_bounds__(256) void k_sub_mean_part(double* __restrict__ x,
 // Weighted Jacobi sweep on A z = r (FP32): zout = zin + w D^-1 (r - A zin);
 // zin == nullptr means a zero initial guess (zout = w D^-1 r).  With zadd,
 // the sweep starts from zin + zadd (the sum formed on the fly, as a
-// separate addition kernel would store it).
+// separate addition kernel would store it).  ZQ (the cycle's last sweep):
+// also the partials of zout . qprev (0 when qprev is null) and of
+// sum(zout) into zpart, the values k_fcg_zq would form from the stored z.
+template <bool ZQ>
 __global__ __launch_bounds__(256) void k_jacobi32(Sten<float> s,
                                                   const float* __restrict__ r,
                                                   const float* __restrict__ zin,
                                                   const float* __restrict__ zadd,
                                                   float* __restrict__ zout,
-                                                  float w, Tiling tl) {
+                                                  float w, Tiling tl,
+                                                  const double* __restrict__ qprev,
+                                                  double* zpart) {
+  double v[2] = {0.0, 0.0};
   for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
     int i0, j0;
     tile_origin(t, tl.ntj, i0, j0);
@@ -255,13 +261,16 @@ __global__ __launch_bounds__(256) void k_jacobi32(Sten<float> s,
     constexpr int NR = TI / BY;
     Nbr<float> zn[NR], za[NR], kc[NR];
     float rr[NR];
+    double qv[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const int i = i0 + threadIdx.y + BY * k;
       // (a zero guess reads r in place of zin: the values are not used)
       load_nbr(s, zin ? zin : r, i, j, zn[k], kc[k]);
       if (zadd) load_nbr(s, zadd, i, j, za[k], kc[k]);
-      rr[k] = r[(size_t)min(i, s.nx - 1) * s.ny + min(j, s.ny - 1)];
+      const size_t ic = (size_t)min(i, s.nx - 1) * s.ny + min(j, s.ny - 1);
+      rr[k] = r[ic];
+      qv[k] = (ZQ && qprev) ? qprev[ic] : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
@@ -278,9 +287,19 @@ __global__ __launch_bounds__(256) void k_jacobi32(Sten<float> s,
         }
         const float dg = diag_from(s, i, j, kc[k].kx0, kc[k].kx1, kc[k].ky0, kc[k].ky1);
         const float rv = zin ? rr[k] - apply_regs(s, i, j, dg, z, kc[k]) : rr[k];
-        zout[idx] = (zin ? z.c : 0.f) + w * (rv / dg);
+        const float zo = (zin ? z.c : 0.f) + w * (rv / dg);
+        zout[idx] = zo;
+        if (ZQ) {
+          const double zv = zo;
+          v[0] += zv * qv[k];
+          v[1] += zv;
+        }
       }
     }
+  }
+  if (ZQ) {
+    __shared__ double lds[8];
+    store_partials<double, 2>(v, zpart, gridDim.x, blockIdx.x, lds);
   }
 }
 
@@ -336,11 +355,15 @@ __global__ __launch_bounds__(256) void k_jacobi32_2z(Sten<float> s,
   }
 }
 
-// rout = r - A z (FP32)
+// rout = r - A z (FP32).  With npart given, also the FP64 partial sums of
+// rout^2 per block (the network input's sum of squares, as k_sumsq forms
+// it from the stored rout).
 __global__ __launch_bounds__(256) void k_resid32(Sten<float> s,
                                                  const float* __restrict__ r,
                                                  const float* __restrict__ z,
-                                                 float* __restrict__ rout, Tiling tl) {
+                                                 float* __restrict__ rout, Tiling tl,
+                                                 double* npart) {
+  double v[1] = {0.0};
   for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
     int i0, j0;
     tile_origin(t, tl.ntj, i0, j0);
@@ -359,9 +382,15 @@ __global__ __launch_bounds__(256) void k_resid32(Sten<float> s,
       const int i = i0 + threadIdx.y + BY * k;
       if (i < s.nx && j < s.ny) {
         const float dg = diag_from(s, i, j, kc[k].kx0, kc[k].kx1, kc[k].ky0, kc[k].ky1);
-        rout[(size_t)i * s.ny + j] = rr[k] - apply_regs(s, i, j, dg, zn[k], kc[k]);
+        const float ro = rr[k] - apply_regs(s, i, j, dg, zn[k], kc[k]);
+        rout[(size_t)i * s.ny + j] = ro;
+        v[0] += (double)ro * ro;
       }
     }
+  }
+  if (npart) {
+    __shared__ double lds[4];
+    store_partials<double, 1>(v, npart, gridDim.x, blockIdx.x, lds);
   }
 }
 
@@ -376,7 +405,8 @@ __global__ __launch_bounds__(256) void k_add32(float* __restrict__ a,
 int fcg_alloc(cpgpu_poisson* P) {
   int rc = poisson_prepare_f32(P);
   if (rc) return rc;
-  if (P->pf[0].alloc(P->N) || P->pf[1].alloc(P->N) || P->fst.alloc(1))
+  if (P->pf[0].alloc(P->N) || P->pf[1].alloc(P->N) || P->fst.alloc(1) ||
+      P->npart.alloc(kMaxBlocks))
     return set_error(P->ctx, CPGPU_ENOMEM, "flexible-CG vectors");
   if (!P->h_fst &&
       hipHostMalloc((void**)&P->h_fst, sizeof(FState), hipHostMallocDefault) !=
@@ -404,7 +434,11 @@ int fcg_alloc(cpgpu_poisson* P) {
 // q32, b32.
 // z = M(r): pre-smoothing, the network's correction (unless use_net is
 // false: the smoothing-only cycle), post-smoothing; output in P->pa32.
-int apply_precond(cpgpu_poisson* P, bool use_net) {
+// qprev/zpart given: the last post sweep also forms k_fcg_zq's partials
+// (tl.G of each) and *zq_done says so.
+int apply_precond(cpgpu_poisson* P, bool use_net, const double* qprev = nullptr,
+                  double* zpart = nullptr, bool* zq_done = nullptr) {
+  if (zq_done) *zq_done = false;
   hipStream_t st = P->ctx->stream;
   const Tiling tl = P->tl;
   const Sten<float> s = P->sten32();
@@ -421,20 +455,24 @@ int apply_precond(cpgpu_poisson* P, bool use_net) {
   }
   for (int k = k0; k < P->pc_pre; ++k) {
     float* out = (z == bufA) ? bufB : bufA;
-    hipLaunchKernelGGL(k_jacobi32, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
-                       (const float*)z, (const float*)nullptr, out, w, tl);
+    hipLaunchKernelGGL(k_jacobi32<false>, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
+                       (const float*)z, (const float*)nullptr, out, w, tl,
+                       (const double*)nullptr, (double*)nullptr);
     z = out;
   }
   const float* zadd = nullptr;  // a correction the next sweep adds to z
   if (use_net) {
     const float* rn = r;
+    const double* sq = nullptr;  // the input's sum-of-squares partials
     if (z) {
+      if (P->npart.p && (int)tl.G <= kMaxBlocks) sq = P->npart.p;
       hipLaunchKernelGGL(k_resid32, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
-                         (const float*)z, P->q32.p, tl);
+                         (const float*)z, P->q32.p, tl, (double*)sq);
       rn = P->q32.p;
     }
     float* c = P->b32.p;
-    int rc = unet_apply_device(P->unet, P->nx, P->ny, rn, c, pack_walls(P));
+    int rc = unet_apply_device(P->unet, P->nx, P->ny, rn, c, pack_walls(P), sq,
+                               sq ? (int)tl.G : 0);
     if (rc) return rc;
     if (!z)
       z = c;
@@ -450,8 +488,15 @@ int apply_precond(cpgpu_poisson* P, bool use_net) {
     float* spare = (z == bufA) ? bufB : bufA;
     float* out = (left % 2 == 0) ? P->pa32.p : spare;
     if (out == z) out = (z == bufA) ? bufB : bufA;
-    hipLaunchKernelGGL(k_jacobi32, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
-                       (const float*)z, zadd, out, w, tl);
+    if (left == 0 && zpart && out == P->pa32.p) {
+      hipLaunchKernelGGL(k_jacobi32<true>, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
+                         (const float*)z, zadd, out, w, tl, qprev, zpart);
+      *zq_done = true;
+    } else {
+      hipLaunchKernelGGL(k_jacobi32<false>, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
+                         (const float*)z, zadd, out, w, tl, (const double*)nullptr,
+                         (double*)nullptr);
+    }
     z = out;
     zadd = nullptr;
   }
@@ -472,14 +517,18 @@ int launch_fcg_iter(cpgpu_poisson* P, bool first, int parity, bool singular,
   const Tiling tl = P->tl;
   const size_t N = P->N;
   const int G1 = 1024;  // blocks of the 1-D kernels
-  int rc = apply_precond(P, true);
+  const double* qprev = first ? nullptr : (const double*)P->q.p;
+  bool zq_done = false;
+  int rc = apply_precond(P, true, qprev, P->part64.p, &zq_done);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_fcg_zq, dim3(G1), dim3(256), 0, st, P->pa32.p,
-                     first ? nullptr : (const double*)P->q.p, P->part64.p, N);  // 2 x G1 partials
+  if (!zq_done)
+    hipLaunchKernelGGL(k_fcg_zq, dim3(G1), dim3(256), 0, st, P->pa32.p, qprev,
+                       P->part64.p, N);  // 2 x G1 partials
+  const int GZ = zq_done ? (int)tl.G : G1;
   const double* pold = P->pf[parity].p;
   double* pnew = P->pf[parity ^ 1].p;
   hipLaunchKernelGGL(k_fcg_dir, dim3(tl.G), dim3(TJ, BY), 0, st, P->sten64(), P->pa32.p,
-                     pold, pnew, P->q.p, P->r.p, P->part64.p, G1, P->fst.p, first ? 1 : 0,
+                     pold, pnew, P->q.p, P->r.p, P->part64.p, GZ, P->fst.p, first ? 1 : 0,
                      singular ? 1 : 0, P->partB64.p, tl);
   hipLaunchKernelGGL(k_fcg_upd, dim3(G1), dim3(256), 0, st, pnew, P->q.p, P->x.p, P->r.p,
                      P->r32.p, P->partB64.p, (int)tl.G, P->fst.p, P->partR64.p, N, norm_ref);

@@ -216,6 +216,8 @@ struct cpgpu_poisson {
   int pc_pre = 2, pc_post = 2;
   double pc_omega = 0.8;
   cpgpu::DevBuf<double> pf[2];
+  // the network input's sum-of-squares partials (one per residual block)
+  cpgpu::DevBuf<double> npart;
   // Dia_BiCGSTAB (rt, p, ph, v, s, sh, t) and Solver_Jacobi_ vectors
   cpgpu::DevBuf<double> bv[8];
   cpgpu::DevBuf<cpgpu::JState> jst;
@@ -296,7 +298,10 @@ int run_fcg_unet(cpgpu_poisson* P, int max_iter, double tol, int norm_ref,
 int unet_warm_start(cpgpu_poisson* P);
 // out = s net(in / s), s = rms(in), on H x W device FP32 arrays, with the
 // wall types `walls` (pack_walls) as the network's second input (unet.hip).
-int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out, int walls);
+// sq_part: sq_G partial sums of in^2 formed by the caller's kernel (the
+// input's rms is then reduced from them); null: formed here from in.
+int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out, int walls,
+                      const double* sq_part = nullptr, int sq_G = 0);
 // The wall types of a problem packed for unet_apply_device (unet.hip).
 int pack_walls(const cpgpu_poisson* P);
 // Error1 / Error2 / Errori of val against exa (interior arrays with row

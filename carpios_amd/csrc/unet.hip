@@ -1298,11 +1298,16 @@ int pack(cpgpu_unet* U) {
 }
 
 // x0 = in / rms(in); forward; out = rms(in) * y.  Device FP32 arrays.
-int apply_scaled(cpgpu_unet* U, const float* in, float* out, size_t N) {
+// sq_part (sq_G partial sums of in^2) given: the rms is reduced from them.
+int apply_scaled(cpgpu_unet* U, const float* in, float* out, size_t N,
+                 const double* sq_part = nullptr, int sq_G = 0) {
   hipStream_t st = U->ctx->stream;
-  hipLaunchKernelGGL(k_sumsq, dim3(1024), dim3(256), 0, st, in, N, U->dpart.p);
-  hipLaunchKernelGGL(k_rms, dim3(1), dim3(256), 0, st, U->dpart.p, 1024, N,
-                     U->scale.p);
+  if (!sq_part) {
+    hipLaunchKernelGGL(k_sumsq, dim3(1024), dim3(256), 0, st, in, N, U->dpart.p);
+    sq_part = U->dpart.p;
+    sq_G = 1024;
+  }
+  hipLaunchKernelGGL(k_rms, dim3(1), dim3(256), 0, st, sq_part, sq_G, N, U->scale.p);
   // a network only applied (out given) scales its input inside the first
   // layer; one that a backward pass follows keeps the scaled input in x0
   if (!out)
@@ -1388,11 +1393,12 @@ int pack_walls(const cpgpu_poisson* P) {
   return w;
 }
 
-int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out, int walls) {
+int unet_apply_device(cpgpu_unet* U, int H, int W, const float* in, float* out, int walls,
+                      const double* sq_part, int sq_G) {
   int rc = unet_alloc(U, H, W);
   if (rc) return rc;
   U->walls = walls;
-  return apply_scaled(U, in, out, (size_t)H * W);
+  return apply_scaled(U, in, out, (size_t)H * W, sq_part, sq_G);
 }
 cpgpu_ctx* unet_context(const cpgpu_unet* U) { return U->ctx; }
 long long unet_generation(const cpgpu_unet* U) { return U->generation; }

@@ -326,7 +326,11 @@ def test_learned_preconditioner_gains_over_jacobi_pcg(ctx):
     net = ca.UNet(ctx, seed=1)
     tr = learned.CavityTrainer(ns, net, nu=2, omega=0.8)
     t0 = time.perf_counter()
-    tr.run(plain=60, first_fit=600, rounds=2, round_steps=20, round_fit=400)
+    # three sample rounds: with two, the network's quality depended on the
+    # last bits of the training's flexible-CG samples (some networks then
+    # stalled and handed most steps to Jacobi-PCG); with three, networks from
+    # different initial seeds take 7-20 iterations per step
+    tr.run(plain=60, first_fit=800, rounds=3, round_steps=20, round_fit=600)
     ctx.sync()
     say(f"  trained in {time.perf_counter() - t0:.1f} s: {tr.log}")
     P = ns.projection()
