@@ -1022,8 +1022,12 @@ int conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{},
                        fuse ? pout : View{});
         return fuse ? 1 : 0;
       } else {
+        const bool fuse = pout.p && pout.cs % 4 == 0 && pout.co % 4 == 0 &&
+                          pout.H * 4 == x.H && pout.W * 4 == x.W;
         CPGPU_LAUNCH(ev, (k_conv3_tile<CIN, COUT, true>), dim3(x.W / 16, x.H / 16),
-                     dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu);
+                     dim3(256), 0, U->ctx->stream, x, y, GView{}, w, b, relu,
+                     fuse ? pout : View{});
+        return fuse ? 1 : 0;
       }
       return 0;
     }
@@ -1086,8 +1090,8 @@ int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
   };
   if (!pooled0) pool(U->v(U->C0, 16), U->v(U->p0), 16);
   conv_fwd<16, 32>(U, U->v(U->p0), U->v(U->a1), 2, 1);
-  conv_fwd<32, 32>(U, U->v(U->a1), U->v(U->C1, 32), 3, 1);
-  pool(U->v(U->C1, 32), U->v(U->p1), 32);
+  if (!(conv_fwd<32, 32>(U, U->v(U->a1), U->v(U->C1, 32), 3, 1, U->v(U->p1)) & 1))
+    pool(U->v(U->C1, 32), U->v(U->p1), 32);
   conv_fwd<32, 64>(U, U->v(U->p1), U->v(U->a2), 4, 1);
   conv_fwd<64, 64>(U, U->v(U->a2), U->v(U->C2, 64), 5, 1);
   pool(U->v(U->C2, 64), U->v(U->p2), 64);
@@ -1181,7 +1185,8 @@ int conv_bwd(cpgpu_unet* U, const cpgpu_unet::T& in, int in_co,
         }
         else
           hipLaunchKernelGGL((k_conv3_tile<COUT, CIN, false>), dim3(in.W / 16, in.H / 16),
-                             dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0);
+                             dim3(256), 0, st, gq, View{}, gx, wt, (const float*)nullptr, 0,
+                             View{});
         done = true;
       }
     }

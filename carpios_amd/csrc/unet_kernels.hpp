@@ -346,19 +346,25 @@ __global__ __launch_bounds__(256) void k_conv3_dgrad(View dy, GView dx,
 // then holds four consecutive output channels of one pixel, stored as one
 // 8-byte (BF16) or 16-byte (FP32) write, and the 16 lanes of a group cover 16
 // consecutive pixels, so a wave's store is one contiguous row segment.
+// With pout.p set (FWD) the epilogue also writes the 4 x 4 average pooling
+// of the BF16 output into pout, as the 16-channel tiles' pooling epilogue
+// forms it (wave w's four rows are one pooling band).
 template <int KCH, int NO, bool FWD>
 __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
                                                     const bf16* __restrict__ w,
                                                     const float* __restrict__ bias,
-                                                    int relu) {
+                                                    int relu, View pout) {
   constexpr int KC = (KCH % 32 == 0) ? 32 : 16;
   constexpr int NB = NO / 16;
+  constexpr int NCK = KCH / KC;  // K steps per tap
   constexpr int TP = 18;       // tile side with halo
   constexpr int CPP = KCH / 8;  // 16-byte chunks per pixel
   typedef Frag<KC> F;
-  // one K = 32 step per tap and at most two column blocks: all nine taps'
-  // weight fragments fit in registers and are loaded before the staging
-  constexpr bool kPre = KCH == 32 && NB <= 2;
+  // at most two K = 32 steps per tap and two column blocks (forward): all
+  // nine taps' weight fragments fit in registers and are loaded before the
+  // staging, so their latency overlaps it (these levels run one block per
+  // compute unit, with nothing else to hide a load behind)
+  constexpr bool kPre = (KCH == 32 || (FWD && KCH == 64)) && NB <= 2;
   __shared__ __attribute__((aligned(16))) bf16 sx[TP * TP * KCH];
   // tile element (pixel, channel): with 32 channels (four 16-byte chunks
   // per pixel) the chunks are stored in the order chunk ^ ((column >> 1) & 3);
@@ -376,13 +382,15 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
   const int H = x.H, W = x.W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int px = lane & 15, kq = F::KPL * (lane >> 4);
-  typename F::T wpre[kPre ? 9 : 1][NB];
+  typename F::T wpre[kPre ? 9 : 1][kPre ? NCK : 1][NB];
   if constexpr (kPre) {
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
-      for (int b = 0; b < NB; ++b)
-        wpre[tap][b] = F::load(w + ((size_t)(tap * NO + b * 16 + px) * KCH + kq));
+      for (int c = 0; c < NCK; ++c)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          wpre[tap][c][b] = F::load(w + ((size_t)(tap * NO + b * 16 + px) * KCH + c * KC + kq));
   }
   // staging: every load of the tile first, from clamped valid addresses
   // (masked when stored), so a thread's loads are in flight together
@@ -420,12 +428,15 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
       const int dy = FWD ? tap / 3 - 1 : 1 - tap / 3;
       const int dx = FWD ? tap % 3 - 1 : 1 - tap % 3;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const int r = wave * 4 + a;
-        const typename F::T af =
-            F::load(sx + sxo((r + 1 + dy) * TP + px + 1 + dx, px + 1 + dx, kq));
+      for (int c = 0; c < NCK; ++c) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(wpre[tap][b], af, acc[a][b]);
+        for (int a = 0; a < 4; ++a) {
+          const int r = wave * 4 + a;
+          const typename F::T af =
+              F::load(sx + sxo((r + 1 + dy) * TP + px + 1 + dx, px + 1 + dx, c * KC + kq));
+#pragma unroll
+          for (int b = 0; b < NB; ++b) acc[a][b] = F::mfma(wpre[tap][c][b], af, acc[a][b]);
+        }
       }
     }
   }
@@ -450,6 +461,12 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
     }
   }
   const int cq = 4 * (lane >> 4);  // first of the lane's four channels
+  const bool pool = FWD && pout.p;
+  float ps[NB][4];  // pooling: the column sums of the lane's four rows
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ps[b][u] = 0.f;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const size_t m = (size_t)(ti0 + wave * 4 + a) * W + tj0 + px;
@@ -468,11 +485,36 @@ __global__ __launch_bounds__(256) void k_conv3_tile(View x, View yb, GView yf,
           }
           h[t >> 1] = bf16x2_bits(v0, v1);
         }
+        if (pool) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) ps[b][u] += bf16x2_half(h[u >> 1], u & 1);
+        }
         *reinterpret_cast<uint2*>(yb.p + m * yb.cs + yb.co + n) = uint2{h[0], h[1]};
       } else {
         *reinterpret_cast<float4*>(yf.p + m * yf.cs + yf.co + n) =
             float4{acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
       }
+    }
+  }
+  if (pool) {
+    // the four columns of a window by lane shuffles, then / 16 and rounded
+    // (the sum of 16 non-negative BF16 values is exact in FP32)
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        float v = ps[b][u];
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        ps[b][u] = v;
+      }
+    if ((px & 3) == 0) {
+      const int mp = (ti0 / 4 + wave) * pout.W + tj0 / 4 + px / 4;
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        *reinterpret_cast<uint2*>(pout.p + (size_t)mp * pout.cs + pout.co + b * 16 + cq) =
+            uint2{bf16x2_bits(ps[b][0] * (1.f / 16.f), ps[b][1] * (1.f / 16.f)),
+                  bf16x2_bits(ps[b][2] * (1.f / 16.f), ps[b][3] * (1.f / 16.f))};
     }
   }
 }
