@@ -982,8 +982,30 @@ int conv_fwd(cpgpu_unet* U, View x, View y, int l, int relu, View pout = View{},
   const EvPair ev = (CIN == 16 && COUT == 16 && M >= kSmallM)
                         ? timer_events(U->ctx, out_fusable ? 3 : 2) : EvPair();
   if (M < kSmallM) {
-    hipLaunchKernelGGL((k_conv3_fwd_ks<CIN, COUT>), dim3((M + 15) / 16, COUT / 16),
-                       dim3(256), 0, U->ctx->stream, x, y, w, b, relu);
+    // 4 x 4 pixel groups where the sides allow (the same products per
+    // pixel), which lets the epilogue pool
+    if (x.H % 4 == 0 && x.W % 4 == 0) {
+      const bool fuse = pout.p && pout.cs % 4 == 0 && pout.co % 4 == 0 &&
+                        pout.H * 4 == x.H && pout.W * 4 == x.W;
+      const View po = fuse ? pout : View{};
+      // from 64 x 64 pixels and 64 input channels on, a block takes two
+      // pixel groups x two channel groups (its weight fragments then serve
+      // two pixel groups, its activations two channel groups): 128 -> 64 at
+      // 64^2 9.9 -> 6.9 us, 64 -> 64 5.8 -> 4.8 us; smaller levels and
+      // inputs measured fastest with one of each
+      if constexpr (CIN >= 64 && COUT % 32 == 0) {
+        if (M >= 4096 && (M / 16) % 2 == 0) {
+          hipLaunchKernelGGL((k_conv3_fwd_ks<CIN, COUT, true, 2, 2>), dim3(M / 32, COUT / 32),
+                             dim3(256), 0, U->ctx->stream, x, y, w, b, relu, po);
+          return fuse ? 1 : 0;
+        }
+      }
+      hipLaunchKernelGGL((k_conv3_fwd_ks<CIN, COUT, true>), dim3(M / 16, COUT / 16),
+                         dim3(256), 0, U->ctx->stream, x, y, w, b, relu, po);
+      return fuse ? 1 : 0;
+    }
+    hipLaunchKernelGGL((k_conv3_fwd_ks<CIN, COUT, false>), dim3((M + 15) / 16, COUT / 16),
+                       dim3(256), 0, U->ctx->stream, x, y, w, b, relu, View{});
     return 0;
   }
   if constexpr (CIN <= 64) {
@@ -1042,7 +1064,12 @@ void up_fwd(cpgpu_unet* U, View x, View y, int l) {
   const int M = x.H * x.W;
   const bf16* w = U->wbf.p + U->wbf_off[l];
   const float* b = U->params.p + U->boff[l];
-  if (M < kSmallM)
+  // (64 x 64 coarse pixels: two pixel and two column groups per wave,
+  // 6.3 -> 4.9 us for 64 -> 32; at 16 x 16 one of each is fastest)
+  if (M < kSmallM && M >= 4096)
+    hipLaunchKernelGGL((k_up_fwd<CIN, COUT, 2, 2>), (tile_grid<2, 2>(M, 16 * COUT)),
+                       dim3(256), 0, U->ctx->stream, x, y, w, b);
+  else if (M < kSmallM)
     hipLaunchKernelGGL((k_up_fwd<CIN, COUT, 1, 1>), (tile_grid<1, 1>(M, 16 * COUT)),
                        dim3(256), 0, U->ctx->stream, x, y, w, b);
   else
@@ -1093,8 +1120,8 @@ int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
   if (!(conv_fwd<32, 32>(U, U->v(U->a1), U->v(U->C1, 32), 3, 1, U->v(U->p1)) & 1))
     pool(U->v(U->C1, 32), U->v(U->p1), 32);
   conv_fwd<32, 64>(U, U->v(U->p1), U->v(U->a2), 4, 1);
-  conv_fwd<64, 64>(U, U->v(U->a2), U->v(U->C2, 64), 5, 1);
-  pool(U->v(U->C2, 64), U->v(U->p2), 64);
+  if (!(conv_fwd<64, 64>(U, U->v(U->a2), U->v(U->C2, 64), 5, 1, U->v(U->p2)) & 1))
+    pool(U->v(U->C2, 64), U->v(U->p2), 64);
   conv_fwd<64, 128>(U, U->v(U->p2), U->v(U->a3), 6, 1);
   conv_fwd<128, 128>(U, U->v(U->a3), U->v(U->b3), 7, 1);
   up_fwd<128, 64>(U, U->v(U->b3), U->v(U->C2, 0), 8);

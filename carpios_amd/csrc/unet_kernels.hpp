@@ -197,76 +197,136 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(View x, View y,
 
 // conv 3x3 forward for small levels (a few thousand pixels or fewer), the
 // nine taps split over the block's four waves: wave w sums taps
-// [kT0[w], kT0[w + 1]) = {0,1,2}, {3,4}, {5,6}, {7,8} for the block's 16
-// pixels x 16 output channels, every load of its taps issued before its
-// MFMAs (one memory round trip per wave, where a wave walking all nine taps
-// makes nine), and wave 0 adds the four partial sums in wave order from LDS
-// and applies the epilogue.  grid: x = 16-pixel groups, y = 16-channel groups.
-template <int CIN, int COUT>
+// [kT0[w], kT0[w + 1]) = {0,1,2}, {3,4}, {5,6}, {7,8} for the block's
+// 16 MP pixels x 16 NB output channels, every load of its taps issued before
+// its MFMAs (one memory round trip per wave, where a wave walking all nine
+// taps makes nine), and wave 0 adds the four partial sums in wave order from
+// LDS and applies the epilogue.  grid: x = groups of MP 16-pixel groups,
+// y = groups of NB 16-channel groups; a block's weight fragments serve its
+// MP pixel groups and its activation fragments its NB channel groups.
+// BLK4 (sides multiples of 4): a 16-pixel group is a 4 x 4 pixel block (lane
+// l % 16 at row (l % 16) / 4, column l % 4 of block g), and with pout.p set
+// the epilogue also writes the block's average (the 4 x 4 pooling of the
+// BF16 output, summed by lane shuffles: the sum of 16 non-negative BF16
+// values is exact in FP32, as in k_pool_fwd).
+template <int CIN, int COUT, bool BLK4 = false, int MP = 1, int NB = 1>
 __global__ __launch_bounds__(256) void k_conv3_fwd_ks(View x, View y, const bf16* __restrict__ w,
-                                                      const float* __restrict__ bias, int relu) {
+                                                      const float* __restrict__ bias, int relu,
+                                                      View pout) {
   constexpr int KC = (CIN % 32 == 0) ? 32 : 16;
   constexpr int NC = CIN / KC;
   typedef Frag<KC> F;
-  __shared__ float red[3][256];
+  __shared__ float red[3][MP * NB * 256];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int M = x.H * x.W;
-  const int m = blockIdx.x * 16 + (lane & 15);
-  const int n0 = blockIdx.y * 16;
-  const int pi = m < M ? m / x.W : -1000000, pj = m < M ? m - (m / x.W) * x.W : 0;
+  int m[MP], pi[MP], pj[MP];
+#pragma unroll
+  for (int a = 0; a < MP; ++a) {
+    const int g = blockIdx.x * MP + a;
+    if constexpr (BLK4) {
+      const int nbw = x.W >> 2, bi = g / nbw, bj = g - bi * nbw;
+      pi[a] = 4 * bi + ((lane & 15) >> 2);
+      pj[a] = 4 * bj + (lane & 3);
+      m[a] = pi[a] < x.H ? pi[a] * x.W + pj[a] : M;
+    } else {
+      m[a] = g * 16 + (lane & 15);
+      pi[a] = m[a] < M ? m[a] / x.W : -1000000;
+      pj[a] = m[a] < M ? m[a] - (m[a] / x.W) * x.W : 0;
+    }
+  }
+  const int n0 = blockIdx.y * 16 * NB;
   const int kq = F::KPL * (lane >> 4);
   const int t0 = wave == 0 ? 0 : 1 + 2 * wave, nt = wave == 0 ? 3 : 2;
-  typename F::T bf[3][NC], af[3][NC];
-  bool ok[3];
+  typename F::T bf[3][NC][NB], af[3][NC][MP];
+  bool ok[3][MP];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int tap = min(t0 + k, 8);
-    const int si = pi + tap / 3 - 1, sj = pj + tap % 3 - 1;
-    ok[k] = k < nt && si >= 0 && si < x.H && sj >= 0 && sj < x.W;
-    const int ci = min(max(si, 0), x.H - 1), cj = min(max(sj, 0), x.W - 1);
-    const size_t off = (size_t)(ci * x.W + cj) * x.cs + x.co + kq;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      bf[k][c] = F::load(w + ((size_t)(tap * COUT + n0 + (lane & 15)) * CIN + c * KC + kq));
-      af[k][c] = F::load(x.p + off + c * KC);
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        bf[k][c][b] =
+            F::load(w + ((size_t)(tap * COUT + n0 + b * 16 + (lane & 15)) * CIN + c * KC + kq));
+#pragma unroll
+    for (int a = 0; a < MP; ++a) {
+      const int si = pi[a] + tap / 3 - 1, sj = pj[a] + tap % 3 - 1;
+      ok[k][a] = k < nt && si >= 0 && si < x.H && sj >= 0 && sj < x.W;
+      const int ci = min(max(si, 0), x.H - 1), cj = min(max(sj, 0), x.W - 1);
+      const size_t off = (size_t)(ci * x.W + cj) * x.cs + x.co + kq;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) af[k][c][a] = F::load(x.p + off + c * KC);
     }
   }
-  f32x4_t acc = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  f32x4_t acc[MP][NB];
+#pragma unroll
+  for (int a = 0; a < MP; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     if (k < nt) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) acc = F::mfma(bf[k][c], ok[k] ? af[k][c] : F::zero(), acc);
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int a = 0; a < MP; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+            acc[a][b] = F::mfma(bf[k][c][b], ok[k][a] ? af[k][c][a] : F::zero(), acc[a][b]);
     }
   }
   const int e = (4 * (lane >> 4)) * 16 + (lane & 15);  // + 16 t: D[4 (lane / 16) + t][lane % 16]
   if (wave > 0) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) red[wave - 1][e + 16 * t] = acc[t];
+    for (int a = 0; a < MP; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) red[wave - 1][(a * NB + b) * 256 + e + 16 * t] = acc[a][b][t];
   }
   __syncthreads();
-  if (wave > 0 || m >= M) return;
+  if (wave > 0) return;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    float v = acc[t];
-    v += red[0][e + 16 * t];
-    v += red[1][e + 16 * t];
-    v += red[2][e + 16 * t];
-    acc[t] = v;
-  }
-  const int co = n0 + 4 * (lane >> 4);
-  uint32_t h[2];
+  for (int a = 0; a < MP; ++a) {
 #pragma unroll
-  for (int t = 0; t < 4; t += 2) {
-    float v0 = acc[t] + bias[co + t];
-    float v1 = acc[t + 1] + bias[co + t + 1];
-    if (relu) {
-      v0 = v0 > 0.f ? v0 : 0.f;
-      v1 = v1 > 0.f ? v1 : 0.f;
+    for (int b = 0; b < NB; ++b) {
+      const int co = n0 + b * 16 + 4 * (lane >> 4);
+      uint32_t h[2];
+#pragma unroll
+      for (int t = 0; t < 4; t += 2) {
+        float v[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int ix = (a * NB + b) * 256 + e + 16 * (t + u);
+          float s = acc[a][b][t + u];
+          s += red[0][ix];
+          s += red[1][ix];
+          s += red[2][ix];
+          s += bias[co + t + u];
+          v[u] = relu ? (s > 0.f ? s : 0.f) : s;
+        }
+        h[t >> 1] = bf16x2_bits(v[0], v[1]);
+      }
+      if (m[a] < M)
+        *reinterpret_cast<uint2*>(y.p + (size_t)m[a] * y.cs + y.co + co) = uint2{h[0], h[1]};
+      if (BLK4 && pout.p) {
+        float ps[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          float v = bf16x2_half(h[u >> 1], u & 1);
+          v += __shfl_xor(v, 1, 64);
+          v += __shfl_xor(v, 2, 64);
+          v += __shfl_xor(v, 4, 64);
+          v += __shfl_xor(v, 8, 64);
+          ps[u] = v * (1.f / 16.f);
+        }
+        if ((lane & 15) == 0 && m[a] < M)
+          *reinterpret_cast<uint2*>(pout.p + (size_t)(blockIdx.x * MP + a) * pout.cs + pout.co +
+                                    co) = uint2{bf16x2_bits(ps[0], ps[1]),
+                                                bf16x2_bits(ps[2], ps[3])};
+      }
     }
-    h[t >> 1] = bf16x2_bits(v0, v1);
   }
-  *reinterpret_cast<uint2*>(y.p + (size_t)m * y.cs + y.co + co) = uint2{h[0], h[1]};
 }
 
 // conv 3x3 input gradient: dx[m][ci] = sum_tap sum_co g[m - off][co]
