@@ -355,6 +355,118 @@ __global__ __launch_bounds__(256) void k_jacobi32_2z(Sten<float> s,
   }
 }
 
+// Two weighted Jacobi sweeps from a zero guess and the residual they leave,
+// in one pass over an LDS tile: z1 = w D^-1 r on the tile and a halo of two,
+// z2 = z1 + w D^-1 (r - A z1) on the tile and a halo of one (stored as zout
+// on the tile), rout = r - A z2 on the tile, and the FP64 partial sums of
+// rout^2 per block into npart.  Per value the arithmetic of k_jacobi32_2z
+// followed by k_resid32 (values outside the grid are never used).
+constexpr int kZR = 2;  // halo of the r and z1 tiles
+__global__ __launch_bounds__(256) void k_jacobi32_2zr(Sten<float> s,
+                                                      const float* __restrict__ r,
+                                                      float* __restrict__ zout,
+                                                      float* __restrict__ rout, float w,
+                                                      Tiling tl, double* npart) {
+  constexpr int RH = TI + 2 * kZR, RW = TJ + 2 * kZR;  // r and z1: 20 x 68
+  constexpr int ZH = TI + 2, ZW = TJ + 2;              // z2: 18 x 66
+  __shared__ float rt[RH][RW], z1t[RH][RW], z2t[ZH][ZW];
+  __shared__ float kxs[RH + 1], kys[RW + 1];
+  const int nx = s.nx, ny = s.ny;
+  const int tid = threadIdx.x + threadIdx.y * TJ;
+  double v[1] = {0.0};
+  for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    int i0, j0;
+    tile_origin(t, tl.ntj, i0, j0);
+    // r and the face coefficients, every load first (clamped addresses)
+    constexpr int NE = RH * RW, NPT = (NE + 255) / 256;
+    float rv[NPT];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int e = min(tid + 256 * k, NE - 1);
+      const int a = e / RW, b = e - a * RW;
+      const int gi = min(max(i0 - kZR + a, 0), nx - 1), gj = min(max(j0 - kZR + b, 0), ny - 1);
+      rv[k] = r[(size_t)gi * ny + gj];
+    }
+    float kv = 0.f;
+    if (tid < RH + 1)
+      kv = s.KX[min(max(i0 - kZR + tid, 0), nx)];
+    else if (tid < RH + 1 + RW + 1)
+      kv = s.KY[min(max(j0 - kZR + tid - (RH + 1), 0), ny)];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int e = tid + 256 * k;
+      if (e < NE) {
+        const int a = e / RW, b = e - a * RW;
+        rt[a][b] = rv[k];
+      }
+    }
+    if (tid < RH + 1)
+      kxs[tid] = kv;
+    else if (tid < RH + 1 + RW + 1)
+      kys[tid - (RH + 1)] = kv;
+    __syncthreads();
+    // z1 on the tile and a halo of two
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int e = tid + 256 * k;
+      if (e < NE) {
+        const int a = e / RW, b = e - a * RW;
+        const int gi = i0 - kZR + a, gj = j0 - kZR + b;
+        float z1 = 0.f;
+        if (gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
+          const float dg = diag_from(s, gi, gj, kxs[a], kxs[a + 1], kys[b], kys[b + 1]);
+          z1 = 0.f + w * (rt[a][b] / dg);
+        }
+        z1t[a][b] = z1;
+      }
+    }
+    __syncthreads();
+    // z2 on the tile and a halo of one; the tile's values stored
+    constexpr int NZ = ZH * ZW, NZT = (NZ + 255) / 256;
+#pragma unroll
+    for (int k = 0; k < NZT; ++k) {
+      const int e = tid + 256 * k;
+      if (e < NZ) {
+        const int a = e / ZW, b = e - a * ZW;  // z2t[a][b] = z1t[a + 1][b + 1]'s cell
+        const int gi = i0 - 1 + a, gj = j0 - 1 + b;
+        float z2 = 0.f;
+        if (gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
+          const int ra = a + 1, rb = b + 1;
+          const float kx0 = kxs[ra], kx1 = kxs[ra + 1], ky0 = kys[rb], ky1 = kys[rb + 1];
+          const float dg = diag_from(s, gi, gj, kx0, kx1, ky0, ky1);
+          const float c = z1t[ra][rb];
+          const float res = rt[ra][rb] - row_apply(s, gi, gj, dg, c, z1t[ra - 1][rb],
+                                                   z1t[ra + 1][rb], z1t[ra][rb - 1],
+                                                   z1t[ra][rb + 1], kx0, kx1, ky0, ky1);
+          z2 = c + w * (res / dg);
+          if (a >= 1 && a <= TI && b >= 1 && b <= TJ) zout[(size_t)gi * ny + gj] = z2;
+        }
+        z2t[a][b] = z2;
+      }
+    }
+    __syncthreads();
+    // rout = r - A z2 on the tile
+    const int lj = threadIdx.x + 1, j = j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TI / BY; ++k) {
+      const int li = threadIdx.y + BY * k + 1, i = i0 + li - 1;
+      if (i < nx && j < ny) {
+        const int ra = li + 1, rb = lj + 1;
+        const float kx0 = kxs[ra], kx1 = kxs[ra + 1], ky0 = kys[rb], ky1 = kys[rb + 1];
+        const float dg = diag_from(s, i, j, kx0, kx1, ky0, ky1);
+        const float ro = rt[ra][rb] - row_apply(s, i, j, dg, z2t[li][lj], z2t[li - 1][lj],
+                                                z2t[li + 1][lj], z2t[li][lj - 1],
+                                                z2t[li][lj + 1], kx0, kx1, ky0, ky1);
+        rout[(size_t)i * ny + j] = ro;
+        v[0] += (double)ro * ro;
+      }
+    }
+    __syncthreads();  // the tiles are reused by the block's next tile
+  }
+  __shared__ double lds[4];
+  store_partials<double, 1>(v, npart, gridDim.x, blockIdx.x, lds);
+}
+
 // rout = r - A z (FP32).  With npart given, also the FP64 partial sums of
 // rout^2 per block (the network input's sum of squares, as k_sumsq forms
 // it from the stored rout).
@@ -448,7 +560,15 @@ int apply_precond(cpgpu_poisson* P, bool use_net, const double* qprev = nullptr,
   float* bufB = P->pb32.p;
   float* z = nullptr;  // null: z = 0
   int k0 = 0;
-  if (P->pc_pre >= 2) {  // the first two sweeps from zero in one pass
+  // exactly two pre-sweeps before the network: both and the residual the
+  // network takes in one pass (its sum-of-squares partials with it)
+  const bool zr = use_net && P->pc_pre == 2 && P->npart.p && (int)tl.G <= kMaxBlocks;
+  if (zr) {
+    hipLaunchKernelGGL(k_jacobi32_2zr, dim3(tl.G), dim3(TJ, BY), 0, st, s, r, bufA,
+                       P->q32.p, w, tl, P->npart.p);
+    z = bufA;
+    k0 = 2;
+  } else if (P->pc_pre >= 2) {  // the first two sweeps from zero in one pass
     hipLaunchKernelGGL(k_jacobi32_2z, dim3(tl.G), dim3(TJ, BY), 0, st, s, r, bufA, w, tl);
     z = bufA;
     k0 = 2;
@@ -464,7 +584,10 @@ int apply_precond(cpgpu_poisson* P, bool use_net, const double* qprev = nullptr,
   if (use_net) {
     const float* rn = r;
     const double* sq = nullptr;  // the input's sum-of-squares partials
-    if (z) {
+    if (zr) {
+      sq = P->npart.p;
+      rn = P->q32.p;
+    } else if (z) {
       if (P->npart.p && (int)tl.G <= kMaxBlocks) sq = P->npart.p;
       hipLaunchKernelGGL(k_resid32, dim3(tl.G), dim3(TJ, BY), 0, st, s, r,
                          (const float*)z, P->q32.p, tl, (double*)sq);
