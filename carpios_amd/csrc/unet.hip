@@ -147,9 +147,12 @@ __global__ __launch_bounds__(256) void k_first_fwd(const float* __restrict__ x,
 // skipped: fma(w, 0, s) = s up to the sign of a zero, which the ReLU and
 // every later product and sum leave without effect.  Item (pixel, channel
 // half h) per thread, h wave-uniform so that the weights are scalar loads.
+// With sq_part given, the input's rms is formed here from the sq_G partial
+// sums of x^2 (as k_rms forms it, in every block; block 0 stores it in
+// *scale for the decoder), otherwise read from *scale.
 __global__ __launch_bounds__(256, 5) void k_enc0_apply(const float* __restrict__ x,
-                                                      const float* __restrict__ scale, int H,
-                                                      int W, int walls,
+                                                      float* scale, const double* sq_part,
+                                                      int sq_G, int H, int W, int walls,
                                                       const float* __restrict__ w0,
                                                       const float* __restrict__ b0,
                                                       const bf16* __restrict__ w1,
@@ -165,7 +168,6 @@ __global__ __launch_bounds__(256, 5) void k_enc0_apply(const float* __restrict__
   conv16_weights<16, 1>(w1, lane, wf);
   Conv16Epi<1> ep;
   ep.load(b1, nullptr, nullptr, lane);
-  const float sc = 1.f / *scale;
   const int h = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 7);
   const float* w0h = w0 + 16 * h;  // this half's weights: w0h[q * 32 + 2 u + {0, 1}]
   const float* b0h = b0 + 8 * h;
@@ -184,6 +186,18 @@ __global__ __launch_bounds__(256, 5) void k_enc0_apply(const float* __restrict__
   };
   const TileWalk tw(ntiles);
   if (tw.first < tw.end) fetch(tw.first);
+  float scv;
+  if (sq_part) {  // (the first tile's loads are in flight meanwhile)
+    __shared__ double lds[4];
+    double tot[1];
+    reduce_partials<double, 1>(sq_part, sq_G, tot, lds);
+    const double rms = std::sqrt(tot[0] / ((double)H * W));
+    scv = rms > 0.0 ? (float)rms : 1.f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *scale = scv;
+  } else {
+    scv = *scale;
+  }
+  const float sc = 1.f / scv;
   for (int t = tw.first; t < tw.end; t += tw.per) {
     const int ty = t / ntx;
     const int tj0 = (t - ty * ntx) * 16, ti0 = ty * 16;
@@ -1082,8 +1096,10 @@ void up_fwd(cpgpu_unet* U, View x, View y, int l) {
 // then not formed.
 // raw (applying only): the unscaled input, scaled by 1 / U->scale inside the
 // first layer instead of through U->x0.
+// sq_part (applying only): sq_G partial sums of raw^2, from which the scale
+// is formed (inside the fused encoder where it runs, else by k_rms).
 int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
-            const float* raw = nullptr) {
+            const float* raw = nullptr, const double* sq_part = nullptr, int sq_G = 0) {
   hipStream_t st = U->ctx->stream;
   const int M = U->H * U->W;
   // an applied network (output and unscaled input given) runs level 0 in
@@ -1099,11 +1115,15 @@ int forward(cpgpu_unet* U, float* final_out = nullptr, bool* out_done = nullptr,
   if (fused0) {
     const EvPair ev = timer_events(U->ctx, 8);
     CPGPU_LAUNCH(ev, k_enc0_apply, dim3(std::min(ntiles, U->ncu * 5)), dim3(256), 0,
-                 st, raw, U->scale.p, U->H, U->W, U->walls, U->params.p + U->woff[0],
+                 st, raw, U->scale.p, sq_part, sq_G, U->H, U->W, U->walls,
+                 U->params.p + U->woff[0],
                  U->params.p + U->boff[0], U->wbf.p + U->wbf_off[1], U->params.p + U->boff[1],
                  U->v(U->a0), U->v(U->p0), ntx, ntiles);
     pooled0 = true;
   } else {
+    if (sq_part)
+      hipLaunchKernelGGL(k_rms, dim3(1), dim3(256), 0, st, sq_part, sq_G, (size_t)M,
+                         U->scale.p);
     hipLaunchKernelGGL(k_first_fwd, dim3((M + 255) / 256), dim3(256), 0, st,
                        raw ? raw : (const float*)U->x0.p, U->H, U->W, U->walls, U->v(U->a0),
                        U->params.p + U->woff[0], U->params.p + U->boff[0],
@@ -1339,14 +1359,19 @@ int apply_scaled(cpgpu_unet* U, const float* in, float* out, size_t N,
     sq_part = U->dpart.p;
     sq_G = 1024;
   }
-  hipLaunchKernelGGL(k_rms, dim3(1), dim3(256), 0, st, sq_part, sq_G, N, U->scale.p);
+  // an applied network forms its scale in its first kernel; one that a
+  // backward pass follows needs it here
+  if (!out) {
+    hipLaunchKernelGGL(k_rms, dim3(1), dim3(256), 0, st, sq_part, sq_G, N, U->scale.p);
+    sq_part = nullptr;
+  }
   // a network only applied (out given) scales its input inside the first
   // layer; one that a backward pass follows keeps the scaled input in x0
   if (!out)
     hipLaunchKernelGGL(k_scale_copy, dim3(1024), dim3(256), 0, st, in, U->x0.p,
                        U->scale.p, 1, 1.f, N);
   bool done = false;
-  int rc = forward(U, out, &done, out ? in : nullptr);
+  int rc = forward(U, out, &done, out ? in : nullptr, sq_part, sq_G);
   if (rc) return rc;
   if (out && !done)
     hipLaunchKernelGGL(k_scale_copy, dim3(1024), dim3(256), 0, st, U->y.p, out,
