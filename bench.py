@@ -634,7 +634,7 @@ def main():
     ap.add_argument("--rounds", type=int, default=5,
                     help="flexible-CG sample rounds after the first fit")
     ap.add_argument("--first-fit", type=int, default=2000)
-    ap.add_argument("--round-fit", type=int, default=1500)
+    ap.add_argument("--round-fit", type=int, default=2500)
     ap.add_argument("--nu", type=int, default=2,
                     help="weighted-Jacobi pre- and post-sweeps of the learned cycle")
     ap.add_argument("--omega", type=float, default=0.8,
