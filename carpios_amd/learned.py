@@ -150,22 +150,58 @@ class CavityTrainer:
                          time.perf_counter() - t))
         return losses
 
+    def validate(self):
+        """Flexible-CG iterations of the next cavity step's solve to the
+        trainer's tolerance with the current network (the cavity state is
+        restored afterwards); a stalled solve counts as its total with the
+        Jacobi-PCG finish."""
+        P, ns = self.P, self.ns
+        state = ns.save()
+        P.set_unet(self.net, UNET_PRECOND)
+        ns.set_solver("FCG_UNET", max_iter=10 ** 7, tol=self.tol, norm=NORM_TRUE)
+        its, _ = ns.step(1)
+        P.set_unet(None, 0)
+        ns.restore(state)
+        return int(its[0])
+
     def run(self, plain=100, first_fit=2000, rounds=5, round_steps=40,
-            round_fit=1500, progress=None):
+            round_fit=1500, progress=None, select=True):
         """The schedule used by the benchmark: plain steps, a first fit, then
-        `rounds` of flexible-CG sample collection and refits.  `progress`, if
-        given, receives a line after every phase."""
+        `rounds` of flexible-CG sample collection and refits.  With `select`,
+        every fit is scored by validate() and the parameters that scored
+        best (the later of equals) are kept at the end: the refits' outcome
+        depends on the last bits of their samples, and the last fit is not
+        always the best.  `progress`, if given, receives a line after every
+        phase."""
         def done():
             if progress:
                 progress(f"training: {self.log[-1]}")
+        best = None
+
+        def score():
+            nonlocal best
+            if not select:
+                return
+            t = time.perf_counter()
+            it = self.validate()
+            self.log.append(("validate", it, time.perf_counter() - t))
+            done()
+            if best is None or it <= best[0]:
+                best = (it, self.net.get_params())
         self.plain_steps(plain)
         done()
         self.fit(first_fit, 1e-3, 0)
         done()
+        score()
         for r in range(rounds):
             self.fcg_steps(round_steps)
             done()
             self.fit(round_fit, 5e-4, r + 1)
+            done()
+            score()
+        if best is not None:
+            self.net.set_params(best[1])
+            self.log.append(("selected", best[0]))
             done()
         return self.log
 
