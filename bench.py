@@ -631,7 +631,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--train-plain", type=int, default=100,
                     help="plain-solver cavity steps whose errors seed training")
-    ap.add_argument("--rounds", type=int, default=5,
+    ap.add_argument("--rounds", type=int, default=6,
                     help="flexible-CG sample rounds after the first fit")
     ap.add_argument("--first-fit", type=int, default=2000)
     ap.add_argument("--round-fit", type=int, default=2500)
@@ -641,7 +641,7 @@ def main():
                     help="weight of the learned cycle's Jacobi sweeps")
     ap.add_argument("--round-steps", type=int, default=67,
                     help="cavity steps per flexible-CG sample round (6 samples each: with "
-                         "the plain steps' 160, 2170 training samples by default)")
+                         "the plain steps' 160, 2572 training samples by default)")
     ap.add_argument("--save-net", default="",
                     help="save the trained network and the cavity state (npz)")
     ap.add_argument("--load-net", default="",
