@@ -1087,8 +1087,11 @@ int cpgpu_batch_solve(cpgpu_batch* B, int method, int max_iter, double tol, int*
   }
   // CPGPU_FCG_UNET: members with a U-Net attached as preconditioner run
   // flexible CG one after another (smallest first, one shape after another:
-  // the network's activations are laid out per shape); the others, and the
-  // learned members whose flexible CG stalls, the batched mixed iteration
+  // the network's activations are laid out per shape); then one batched
+  // mixed pass takes the others together with the learned members whose
+  // flexible CG stalled (from their iterates).  One pass: a pass's wall time
+  // follows its slowest member, and two passes in turn (the others first)
+  // would add the stalled members' thousands of iterations after it.
   std::vector<char> member(n, 1);
   std::vector<int> learned;
   for (int p = 0; p < n; ++p)
@@ -1096,14 +1099,11 @@ int cpgpu_batch_solve(cpgpu_batch* B, int method, int max_iter, double tol, int*
       member[p] = 0;
       learned.push_back(p);
     }
-  int rc = batched(member, 1);
-  if (rc) return rc;
+  int rc = 0;
   std::sort(learned.begin(), learned.end(), [&](int a, int b) {
     const cpgpu_poisson *A = B->P[a], *Q = B->P[b];
     return A->N != Q->N ? A->N < Q->N : (A->nx != Q->nx ? A->nx < Q->nx : a < b);
   });
-  std::vector<char> rest(n, 0);
-  bool any_rest = false;
   // flexible-CG iterations a learned member gets before the batched pass
   // takes over from its iterate (an application of the network costs tens
   // of batched iterations, so a slowly converging member is cheaper there)
@@ -1122,12 +1122,9 @@ int cpgpu_batch_solve(cpgpu_batch* B, int method, int max_iter, double tol, int*
     if (resid) resid[p] = t;
     if (codes) codes[p] = code;
     if (latency_ms) latency_ms[p] = since();
-    if (code != CPGPU_OK) {
-      rest[p] = 1;
-      any_rest = true;
-    }
+    if (code != CPGPU_OK) member[p] = 1;  // the batched pass goes on from its iterate
   }
-  if (any_rest && (rc = batched(rest, 1))) return rc;  // from the flexible-CG iterates
+  if ((rc = batched(member, 1))) return rc;
   return all_conv ? CPGPU_OK : CPGPU_MAXITER;
 }
 
