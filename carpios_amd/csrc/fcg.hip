@@ -178,13 +178,34 @@ __global__ __launch_bounds__(256) void k_fcg_upd(const double* __restrict__ p,
     if (!ok) st->breakdown = 1;
   }
   double v[1] = {0.0};
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+  // pairs of elements per thread (16-byte loads and stores); for n odd the
+  // last element is block 0's thread 0's
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const size_t n2 = n / 2;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n2;
        k += (size_t)gridDim.x * blockDim.x) {
-    x[k] = x[k] + alpha * p[k];
-    const double rv = r[k] - alpha * q[k];
-    r[k] = rv;
-    r32[k] = (float)rv;
-    if (!(norm_ref && k == n - 1)) v[0] += rv * rv;
+    const d2 pv = reinterpret_cast<const d2*>(p)[k], qv = reinterpret_cast<const d2*>(q)[k];
+    d2 xv = reinterpret_cast<const d2*>(x)[k], rv = reinterpret_cast<const d2*>(r)[k];
+    xv.x = xv.x + alpha * pv.x;
+    xv.y = xv.y + alpha * pv.y;
+    rv.x = rv.x - alpha * qv.x;
+    rv.y = rv.y - alpha * qv.y;
+    reinterpret_cast<d2*>(x)[k] = xv;
+    reinterpret_cast<d2*>(r)[k] = rv;
+    reinterpret_cast<f2*>(r32)[k] = f2{(float)rv.x, (float)rv.y};
+    v[0] += rv.x * rv.x;
+    if (!(norm_ref && 2 * k + 1 == n - 1)) v[0] += rv.y * rv.y;
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const size_t k = n - 1;
+    {
+      x[k] = x[k] + alpha * p[k];
+      const double rv = r[k] - alpha * q[k];
+      r[k] = rv;
+      r32[k] = (float)rv;
+      if (!norm_ref) v[0] += rv * rv;
+    }
   }
   store_partials<double, 1>(v, part, gridDim.x, blockIdx.x, lds);
 }
